@@ -1,0 +1,137 @@
+/*
+ * pictor_hip.h -- C ABI of the MI355X (gfx950) particle step of PICTOR.
+ *
+ * The reference calls zero-argument Fortran subroutines from MAIN.F90 in a
+ * fixed order and keeps all state in module globals (src/vars.F90,
+ * src/var_gpu.F90).  This ABI replaces those calls one for one; the state
+ * lives in an opaque context instead of module globals.  All pointers are
+ * host pointers in the reference layout:
+ *
+ *   fields    real(8) F(mx,my,mz), column-major, F(1,1,1) at offset 0,
+ *             mx = nx+5, my = ny+5, mz = nz+5 (mz = 1 in 2-D),
+ *             physical cells 3..m-3 (src/comm/subdomains/subdomains.F90:42-67)
+ *   particles SoA q,x,y,z,u,v,w real(8) + flv integer(4), 1-based in Fortran,
+ *             holes marked flv = 0 (src/vars.F90:43-45, 125-127)
+ *
+ * Every call returns 0 on success or a PICTOR_ERR_* code; the message of the
+ * last failure is pictor_last_error().  One context = one HIP stream on the
+ * current device; calls on one context must come from one host thread.
+ */
+#ifndef PICTOR_HIP_H
+#define PICTOR_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PICTOR_OK 0
+#define PICTOR_ERR_ARG 1         /* invalid argument or configuration        */
+#define PICTOR_ERR_HIP 2         /* a HIP runtime call failed                */
+#define PICTOR_ERR_BOUNDS 3      /* a particle left the allowed window       */
+#define PICTOR_ERR_CAPACITY 12   /* particle storage exhausted; the reference's
+                                    Abort(12), src/mem/particles_gpu.F90:21 */
+
+#define PICTOR_FORM_FP64 0       /* reference SoA in FP64                    */
+#define PICTOR_FORM_FP32_SPLIT 1 /* int32 cell + FP32 offset, FP32 push/field,
+                                    fixed-point (int64) current              */
+
+/* Run configuration: the compile-time parameters of parameters.F90:5-87 and
+ * src/vars.F90:10-18 as a runtime struct.  grid_dx = grid_dy = grid_dz = 1,
+ * EMspeedBoost = 1, curr_filter = 0, periodic faces on every side. */
+typedef struct pictor_config {
+    int32_t ndim;          /* 2 (-DtwoD) or 3                                */
+    int32_t nx, ny, nz;    /* physical cells; nz ignored in 2-D              */
+    int32_t epc;           /* particles per cell per species (parameters.F90:17) */
+    int32_t form;          /* PICTOR_FORM_*                                  */
+    int32_t tile;          /* tile edge in cells (0 = choose); must divide nx, ny (nz) */
+    int32_t reserved;
+    double c;              /* parameters.F90:20 */
+    double c_ompe;         /* parameters.F90:19 */
+    double g0;             /* parameters.F90:16 */
+    double mi_me;          /* parameters.F90:18 */
+    double capacity_factor;/* particle storage per tile / particles at layout time (0 = 1.25) */
+} pictor_config;
+
+typedef struct pictor_ctx pictor_ctx;
+
+/* ---- life cycle: Initialisations, src/MAIN.F90:338-395; InitScale,
+ *      src/init/initialise.F90:44-53; InitAll_gpu, src/init/initialise_gpu.F90:9-101 */
+int pictor_init(const pictor_config *cfg, pictor_ctx **out);
+int pictor_finalize(pictor_ctx *ctx);
+const char *pictor_last_error(void);
+/* Derived scalars: {qi, qe, masse, massi, ompe, fldc, xlen, ylen, zlen, mx, my, mz,
+ * tile edge, number of tiles, 0, 0}. */
+int pictor_get_scalars(pictor_ctx *ctx, double out[16]);
+
+/* ---- host <-> device copies in the reference layout (the CUDA Fortran array
+ *      assignments of src/mem/particles_gpu.F90:28-72, src/maxwell/fields_gpu.F90:19-56) */
+int pictor_upload_fields(pictor_ctx *ctx, const double *ex, const double *ey, const double *ez,
+                         const double *bx, const double *by, const double *bz);
+int pictor_download_fields(pictor_ctx *ctx, double *ex, double *ey, double *ez,
+                           double *bx, double *by, double *bz,
+                           double *jx, double *jy, double *jz);
+/* Holes (flv = 0) are dropped.  The FP32 split form needs one charge per flavour. */
+int pictor_upload_particles(pictor_ctx *ctx, int64_t n, const double *q, const double *x,
+                            const double *y, const double *z, const double *u, const double *v,
+                            const double *w, const int32_t *flv);
+int pictor_particle_count(pictor_ctx *ctx, int64_t *n);
+/* Output in tile order; tile (optional, may be NULL) receives each particle's tile id. */
+int pictor_download_particles(pictor_ctx *ctx, int64_t cap, double *q, double *x, double *y,
+                              double *z, double *u, double *v, double *w, int32_t *flv,
+                              int32_t *tile, int64_t *n_out);
+
+/* ---- the step, one call per MAIN.F90 phase ---- */
+/* UpdateBfieldHalf + SendRecvFlds(B): em_update.F90:41-77, comm_fld.F90:75-91 */
+int pictor_update_b_half(pictor_ctx *ctx);
+/* MoveDepositPrtl: movdep.F90:12-258 (GPU: movdep_gpu.F90:14-49).  Also bins
+ * every particle into its destination tile for the next step. */
+int pictor_move_deposit(pictor_ctx *ctx);
+/* ExchangePrtl + AppendParticles on one periodic rank: comm_prtl.F90:11-65,
+ * loadprtlout.F90:13-181.  The wrap itself happens inside pictor_move_deposit;
+ * this call commits the new tile bins and re-lays out storage on overflow. */
+int pictor_exchange_particles(pictor_ctx *ctx);
+/* UpdateEfield + SendRecvFlds(E): em_update.F90:9-39 */
+int pictor_update_e(pictor_ctx *ctx);
+/* UpdateCurrentAllFaces: comm_fld.F90:15-72 (ghost current added into owners) */
+int pictor_current_halo_add(pictor_ctx *ctx);
+/* AddCurrent + SendRecvFlds(E): em_update.F90:79-102 */
+int pictor_add_current(pictor_ctx *ctx);
+/* SendRecvFlds on E (which = 0) or B (which = 1): comm_fld.F90:75-130 */
+int pictor_field_halo(pictor_ctx *ctx, int32_t which);
+/* ResetCurrent: current.F90:13-17 */
+int pictor_reset_current(pictor_ctx *ctx);
+/* ReorderPrtlArr: mem_prtl.F90:166-301 -- counting sort into tiles */
+int pictor_sort_tiles(pictor_ctx *ctx);
+/* nsteps whole time steps of MAIN.F90:42-284 (J reset at the step start). */
+int pictor_step(pictor_ctx *ctx, int32_t nsteps);
+int pictor_synchronize(pictor_ctx *ctx);
+
+/* ---- diagnostics: CalcEnergy/CalcDivE, savedata_routines.F90:265-355 (FP64).
+ * out[0] B energy, out[1] E energy, out[2..3] kinetic energy of flavours 1, 2,
+ * out[4..6] sum m*u, m*v, m*w, out[7] max|div E - rho|, out[8] max|rho|.   */
+int pictor_diagnostics(pictor_ctx *ctx, double out[9]);
+
+/* ---- kernel timers (HIP events on the context stream) ----
+ * id 0 = push/deposit kernel, 1 = field kernels, 2 = whole step.
+ * Returns the summed milliseconds and the number of launches since reset. */
+int pictor_timer_reset(pictor_ctx *ctx);
+int pictor_timer_get(pictor_ctx *ctx, int32_t id, double *total_ms, int64_t *count);
+/* Bytes of device memory owned by the context. */
+int pictor_memory_bytes(pictor_ctx *ctx, int64_t *bytes);
+
+/* ---- synthetic input (setup_weibel.F90:17-26 restated for the benchmark
+ * configurations): epc/2 uniformly random positions per cell per InitPrtl call,
+ * two calls with opposite drifts, both flavours at every position
+ * (init_prtl.F90:138-221), Gaussian thermal gamma*beta of spread uth per
+ * component, drift added with AddDriftVel (prob_dist.F90:223-254).
+ * Arrays must hold 2*epc*nx*ny(*nz) entries; counter-based RNG (seed).      */
+int pictor_setup_weibel(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, int32_t epc,
+                        double drift, double uth, uint64_t seed, int64_t cap,
+                        double *q, double *x, double *y, double *z, double *u, double *v,
+                        double *w, int32_t *flv, int64_t *n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

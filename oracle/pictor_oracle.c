@@ -1,0 +1,990 @@
+/*
+ * pictor_oracle.c -- CPU FP64 restatement of PICTOR's particle step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path (see the header).
+ * Build with -O2 -ffp-contract=off so that no multiply-add is fused and the
+ * rounding sequence is the one the cited Fortran expressions define.
+ *
+ * Reference files followed (paths relative to the PICTOR tree):
+ *   src/maxwell/em_update.F90:9-102            Yee E, B half step, E -= J
+ *   src/movdep/movdep.F90:12-258               Boris push + zigzag deposit
+ *   src/movdep/movdep_routines.F90:6-311       VecE/VecB gather, VecJ reduce
+ *   src/vars.F90:313-347                       weight tables
+ *   src/comm/comm_fld.F90:15-307               halo copy / current add
+ *   src/comm/subdomains/subdomains.F90:52-67,592-740  index ranges, shifts
+ *   src/comm/loadprtlout.F90:13-181            outlier scan + removal
+ *   src/comm/comm_prtl.F90:11-65, src/mem/mem_prtl.F90:135-160  append
+ *   src/mem/mem_prtl.F90:166-301               counting sort by cell
+ *   src/init/initialise.F90:44-53,84-85        charge/mass scaling
+ *   src/io/savedata_routines.F90:265-355       energy, div E
+ */
+#include "pictor_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define VBS 32  /* VecBlockSize, src/vars.F90:36 */
+
+struct orc_ws {
+    int nthreads;
+    int nv;            /* 4 (2-D) / 8 (3-D) corner values per cell */
+    int nj;            /* 8 (2-D) / 12 (3-D) current entries per cell */
+    int64_t ncell;     /* mx*my*mz */
+    double *vec[6];    /* VecEx..VecBz (nv, ncell) */
+    double *vecj;      /* VecJ (nj, ncell, nthreads) */
+};
+
+/* ---- weight tables, src/vars.F90:313-347 ---- */
+static const double wtx1_2[4] = {1, 0, 1, 0}, wtx2_2[4] = {-1, 1, -1, 1};
+static const double wty1_2[4] = {1, 1, 0, 0}, wty2_2[4] = {-1, -1, 1, 1};
+static const double wtFx_2[8] = {1, 1, 0, 0, 0, 0, 0, 0};
+static const double wtFy_2[8] = {0, 0, 1, 1, 0, 0, 0, 0};
+static const double wtFz_2[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+static const double Jwtx1_2[8] = {1, 1, 1, 0, 1, 0, 1, 0};
+static const double Jwtx2_2[8] = {0, 0, -1, 1, -1, 1, -1, 1};
+static const double Jwty1_2[8] = {1, 0, 1, 1, 1, 1, 0, 0};
+static const double Jwty2_2[8] = {-1, 1, 0, 0, -1, -1, 1, 1};
+static const double Jwtz1_2[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+static const double Jwtz2_2[8] = {-1, -1, -1, -1, 0, 0, 0, 0};
+
+static const double wtx1_3[8] = {1, 0, 1, 0, 1, 0, 1, 0}, wtx2_3[8] = {-1, 1, -1, 1, -1, 1, -1, 1};
+static const double wty1_3[8] = {1, 1, 0, 0, 1, 1, 0, 0}, wty2_3[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+static const double wtz1_3[8] = {1, 1, 1, 1, 0, 0, 0, 0}, wtz2_3[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+static const double wtFx_3[12] = {1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0};
+static const double wtFy_3[12] = {0, 0, 0, 0, 1, 1, 1, 1, 0, 0, 0, 0};
+static const double wtFz_3[12] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1};
+static const double Jwtx1_3[12] = {1, 1, 1, 1, 1, 0, 1, 0, 1, 0, 1, 0};
+static const double Jwtx2_3[12] = {0, 0, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1};
+static const double Jwty1_3[12] = {1, 0, 1, 0, 1, 1, 1, 1, 1, 1, 0, 0};
+static const double Jwty2_3[12] = {-1, 1, -1, 1, 0, 0, 0, 0, -1, -1, 1, 1};
+static const double Jwtz1_3[12] = {1, 1, 0, 0, 1, 1, 0, 0, 1, 1, 1, 1};
+static const double Jwtz2_3[12] = {-1, -1, 1, 1, -1, -1, 1, 1, 0, 0, 0, 0};
+
+/* 1-based (i,j,k) -> offset in a column-major (mx,my,mz) array */
+#define IX(p, i, j, k) ((int64_t)((i) - 1) + (int64_t)(p)->mx * ((int64_t)((j) - 1) + (int64_t)(p)->my * (int64_t)((k) - 1)))
+
+int orc_params_init(orc_params *p, int ndim, int nx, int ny, int nz, double c,
+                    double c_ompe, double g0, double mi_me, int epc)
+{
+    if (ndim != 2 && ndim != 3) return ORC_ERR_ARG;
+    if (nx < 6 || ny < 6 || (ndim == 3 && nz < 6) || epc < 1) return ORC_ERR_ARG;
+    memset(p, 0, sizeof(*p));
+    p->ndim = ndim;
+    p->nx = nx; p->ny = ny; p->nz = (ndim == 2) ? 1 : nz;
+    p->mx = nx + 5; p->my = ny + 5; p->mz = (ndim == 2) ? 1 : nz + 5;
+    p->epc = epc;
+    p->c = c; p->c_ompe = c_ompe; p->g0 = g0; p->mi_me = mi_me;
+    p->me = 1.0;
+    p->mi = mi_me;
+    /* initialise.F90:44-53; cell_volume = 1, grid_dx = grid_dy = grid_dz = 1 */
+    p->ompe = c / c_ompe;
+    p->qi = (g0 * (p->ompe * p->ompe)) / (((double)epc / 1.0) * (1.0 + p->me / p->mi));
+    p->qe = -p->qi;
+    p->qme = -1.0;
+    p->qmi = 1.0 / p->mi;
+    p->masse = fabs(p->qe) * p->me;
+    p->massi = p->qi * p->mi;
+    /* vars.F90:220-223 (EMspeedBoost = 1) */
+    p->fldc = c * 1.0;
+    p->fld_halfc = c * 0.5 * 1.0;
+    p->cinv = 1.0 / c;
+    p->sqc = c * c;
+    p->inv_dx = 1.0; p->inv_dy = 1.0; p->inv_dz = 1.0;
+    /* subdomains.F90:52-67 */
+    p->xmin = 3.0; p->xmax = p->mx - 2;
+    p->ymin = 3.0; p->ymax = p->my - 2;
+    if (ndim == 2) { p->zmin = 1.0; p->zmax = 2.0; }
+    else { p->zmin = 3.0; p->zmax = p->mz - 2; }
+    p->xlen = p->xmax - p->xmin;
+    p->ylen = p->ymax - p->ymin;
+    p->zlen = p->zmax - p->zmin;
+    p->nflvr = 2;
+    p->flvrqm[0] = p->qmi;   /* initialise.F90:84 */
+    p->flvrqm[1] = p->qme;   /* initialise.F90:85 */
+    p->reorder_period = 20;
+    return ORC_OK;
+}
+
+orc_ws *orc_ws_create(const orc_params *p, int nthreads)
+{
+    orc_ws *w = (orc_ws *)calloc(1, sizeof(orc_ws));
+    if (!w) return NULL;
+    if (nthreads < 1) nthreads = 1;
+#ifdef _OPENMP
+    omp_set_num_threads(nthreads);
+#else
+    nthreads = 1;
+#endif
+    w->nthreads = nthreads;
+    w->nv = (p->ndim == 2) ? 4 : 8;
+    w->nj = (p->ndim == 2) ? 8 : 12;
+    w->ncell = (int64_t)p->mx * p->my * p->mz;
+    for (int c = 0; c < 6; c++) {
+        w->vec[c] = (double *)calloc((size_t)(w->nv * w->ncell), sizeof(double));
+        if (!w->vec[c]) { orc_ws_destroy(w); return NULL; }
+    }
+    w->vecj = (double *)calloc((size_t)(w->nj * w->ncell * nthreads), sizeof(double));
+    if (!w->vecj) { orc_ws_destroy(w); return NULL; }
+    return w;
+}
+
+void orc_ws_destroy(orc_ws *w)
+{
+    if (!w) return;
+    for (int c = 0; c < 6; c++) free(w->vec[c]);
+    free(w->vecj);
+    free(w);
+}
+
+int orc_ws_threads(const orc_ws *w) { return w->nthreads; }
+const double *orc_ws_vec(const orc_ws *w, int which) { return (which >= 0 && which < 6) ? w->vec[which] : NULL; }
+int orc_ws_vec_len(const orc_ws *w) { return w->nv; }
+
+/* ------------------------------------------------------------------ */
+/* Maxwell solver                                                      */
+/* ------------------------------------------------------------------ */
+
+/* em_update.F90:9-39 */
+void orc_update_e(const orc_params *p, orc_fields *f)
+{
+    const double fc = p->fldc;
+    const int mx = p->mx, my = p->my, mz = p->mz;
+    double *Ex = f->Ex, *Ey = f->Ey, *Ez = f->Ez;
+    const double *Bx = f->Bx, *By = f->By, *Bz = f->Bz;
+    if (p->ndim == 2) {
+        #pragma omp parallel for schedule(static)
+        for (int j = 2; j <= my - 1; j++)
+            for (int i = 2; i <= mx - 1; i++) {
+                int64_t o = IX(p, i, j, 1);
+                Ex[o] = Ex[o] + fc * p->inv_dy * (Bz[o] - Bz[IX(p, i, j - 1, 1)]);
+                Ey[o] = Ey[o] - fc * p->inv_dx * (Bz[o] - Bz[IX(p, i - 1, j, 1)]);
+                Ez[o] = Ez[o] + fc * p->inv_dx * (By[o] - By[IX(p, i - 1, j, 1)])
+                              - fc * p->inv_dy * (Bx[o] - Bx[IX(p, i, j - 1, 1)]);
+            }
+    } else {
+        #pragma omp parallel for schedule(static)
+        for (int k = 2; k <= mz - 1; k++)
+            for (int j = 2; j <= my - 1; j++)
+                for (int i = 2; i <= mx - 1; i++) {
+                    int64_t o = IX(p, i, j, k);
+                    Ex[o] = Ex[o] + fc * p->inv_dy * (Bz[o] - Bz[IX(p, i, j - 1, k)])
+                                  - fc * p->inv_dz * (By[o] - By[IX(p, i, j, k - 1)]);
+                    Ey[o] = Ey[o] + fc * p->inv_dz * (Bx[o] - Bx[IX(p, i, j, k - 1)])
+                                  - fc * p->inv_dx * (Bz[o] - Bz[IX(p, i - 1, j, k)]);
+                    Ez[o] = Ez[o] + fc * p->inv_dx * (By[o] - By[IX(p, i - 1, j, k)])
+                                  - fc * p->inv_dy * (Bx[o] - Bx[IX(p, i, j - 1, k)]);
+                }
+    }
+}
+
+/* em_update.F90:41-77 */
+void orc_update_b_half(const orc_params *p, orc_fields *f)
+{
+    const double hc = p->fld_halfc;
+    const int mx = p->mx, my = p->my, mz = p->mz;
+    const double *Ex = f->Ex, *Ey = f->Ey, *Ez = f->Ez;
+    double *Bx = f->Bx, *By = f->By, *Bz = f->Bz;
+    if (p->ndim == 2) {
+        #pragma omp parallel for schedule(static)
+        for (int j = 2; j <= my - 1; j++)
+            for (int i = 2; i <= mx - 1; i++) {
+                int64_t o = IX(p, i, j, 1);
+                Bx[o] = Bx[o] - hc * p->inv_dy * (Ez[IX(p, i, j + 1, 1)] - Ez[o]);
+                By[o] = By[o] + hc * p->inv_dx * (Ez[IX(p, i + 1, j, 1)] - Ez[o]);
+                Bz[o] = Bz[o] - hc * p->inv_dx * (Ey[IX(p, i + 1, j, 1)] - Ey[o])
+                              + hc * p->inv_dy * (Ex[IX(p, i, j + 1, 1)] - Ex[o]);
+            }
+    } else {
+        #pragma omp parallel for schedule(static)
+        for (int k = 2; k <= mz - 1; k++)
+            for (int j = 2; j <= my - 1; j++)
+                for (int i = 2; i <= mx - 1; i++) {
+                    int64_t o = IX(p, i, j, k);
+                    Bx[o] = Bx[o] - hc * p->inv_dy * (Ez[IX(p, i, j + 1, k)] - Ez[o])
+                                  + hc * p->inv_dz * (Ey[IX(p, i, j, k + 1)] - Ey[o]);
+                    By[o] = By[o] - hc * p->inv_dz * (Ex[IX(p, i, j, k + 1)] - Ex[o])
+                                  + hc * p->inv_dx * (Ez[IX(p, i + 1, j, k)] - Ez[o]);
+                    Bz[o] = Bz[o] - hc * p->inv_dx * (Ey[IX(p, i + 1, j, k)] - Ey[o])
+                                  + hc * p->inv_dy * (Ex[IX(p, i, j + 1, k)] - Ex[o]);
+                }
+    }
+}
+
+/* em_update.F90:79-102 */
+void orc_add_current(const orc_params *p, orc_fields *f)
+{
+    const int mx = p->mx, my = p->my, mz = p->mz;
+    int k0 = 1, k1 = 1;
+    if (p->ndim == 3) { k0 = 3; k1 = mz - 3; }
+    for (int k = k0; k <= k1; k++)
+        #pragma omp parallel for schedule(static)
+        for (int j = 3; j <= my - 3; j++)
+            for (int i = 3; i <= mx - 3; i++) {
+                int64_t o = IX(p, i, j, k);
+                f->Ex[o] = f->Ex[o] - f->Jx[o] * p->inv_dy * p->inv_dz;
+                f->Ey[o] = f->Ey[o] - f->Jy[o] * p->inv_dx * p->inv_dz;
+                f->Ez[o] = f->Ez[o] - f->Jz[o] * p->inv_dy * p->inv_dx;
+            }
+}
+
+/* current.F90:13-17 */
+void orc_reset_current(const orc_params *p, orc_fields *f)
+{
+    size_t n = (size_t)p->mx * p->my * p->mz;
+    memset(f->Jx, 0, n * sizeof(double));
+    memset(f->Jy, 0, n * sizeof(double));
+    memset(f->Jz, 0, n * sizeof(double));
+}
+
+/* ------------------------------------------------------------------ */
+/* Periodic halos on one rank (indepLBaxis = 0, vars.F90:241).          */
+/* Ranges from subdomains.F90:592-680: send face 1 = 3..5, send face 2 =  */
+/* m-4..m-3, recv face 1 = 1..2, recv face 2 = m-2..m.  Faces across the  */
+/* independent axis x span the full cross-section; y and z faces span    */
+/* x = 3..mx-3 only (SetCommFldDomain step 2 with extended edges).        */
+/* ------------------------------------------------------------------ */
+
+typedef struct { int lo[3], hi[3]; } box3;
+
+static void copy_box(const orc_params *p, double *F, box3 dst, int axis, int shift)
+{
+    for (int k = dst.lo[2]; k <= dst.hi[2]; k++)
+        for (int j = dst.lo[1]; j <= dst.hi[1]; j++)
+            for (int i = dst.lo[0]; i <= dst.hi[0]; i++) {
+                int s[3] = {i, j, k};
+                s[axis] += shift;
+                F[IX(p, i, j, k)] = F[IX(p, s[0], s[1], s[2])];
+            }
+}
+
+static void add_box(const orc_params *p, double *F, box3 dst, int axis, int shift)
+{
+    for (int k = dst.lo[2]; k <= dst.hi[2]; k++)
+        for (int j = dst.lo[1]; j <= dst.hi[1]; j++)
+            for (int i = dst.lo[0]; i <= dst.hi[0]; i++) {
+                int s[3] = {i, j, k};
+                s[axis] += shift;
+                F[IX(p, i, j, k)] = F[IX(p, i, j, k)] + F[IX(p, s[0], s[1], s[2])];
+            }
+}
+
+/* cross-section box of the faces normal to `axis` */
+static box3 face_box(const orc_params *p, int axis, int lo, int hi)
+{
+    box3 b;
+    int m[3] = {p->mx, p->my, p->mz};
+    for (int a = 0; a < 3; a++) { b.lo[a] = 1; b.hi[a] = m[a]; }
+    if (axis != 0) { b.lo[0] = 3; b.hi[0] = p->mx - 3; }
+    b.lo[axis] = lo; b.hi[axis] = hi;
+    return b;
+}
+
+/* SendRecvFldsYZ/ZX/XY for one axis: ghosts 1..2 <- m-4..m-3, m-2..m <- 3..5 */
+static void halo_copy_axis(const orc_params *p, double *F, int axis)
+{
+    int m = (axis == 0) ? p->mx : (axis == 1) ? p->my : p->mz;
+    copy_box(p, F, face_box(p, axis, 1, 2), axis, m - 5);
+    copy_box(p, F, face_box(p, axis, m - 2, m), axis, -(m - 5));
+}
+
+/* SendRecvCurrentYZ/ZX/XY for one axis: 3..5 += m-2..m, then m-4..m-3 += 1..2 */
+static void halo_add_axis(const orc_params *p, double *F, int axis)
+{
+    int m = (axis == 0) ? p->mx : (axis == 1) ? p->my : p->mz;
+    add_box(p, F, face_box(p, axis, 3, 5), axis, m - 5);
+    add_box(p, F, face_box(p, axis, m - 4, m - 3), axis, -(m - 5));
+}
+
+/* comm_fld.F90:75-91 with indepLBaxis = 0: ZX, (XY), then YZ */
+void orc_field_halo(const orc_params *p, double *fx, double *fy, double *fz)
+{
+    double *F[3] = {fx, fy, fz};
+    for (int c = 0; c < 3; c++) halo_copy_axis(p, F[c], 1);
+    if (p->ndim == 3)
+        for (int c = 0; c < 3; c++) halo_copy_axis(p, F[c], 2);
+    for (int c = 0; c < 3; c++) halo_copy_axis(p, F[c], 0);
+}
+
+/* comm_fld.F90:15-33 with indepLBaxis = 0: YZ, ZX, (XY) */
+void orc_current_halo_add(const orc_params *p, double *jx, double *jy, double *jz)
+{
+    double *F[3] = {jx, jy, jz};
+    for (int c = 0; c < 3; c++) halo_add_axis(p, F[c], 0);
+    for (int c = 0; c < 3; c++) halo_add_axis(p, F[c], 1);
+    if (p->ndim == 3)
+        for (int c = 0; c < 3; c++) halo_add_axis(p, F[c], 2);
+}
+
+/* ------------------------------------------------------------------ */
+/* Gather of node-averaged corner values, movdep_routines.F90:6-223     */
+/* ------------------------------------------------------------------ */
+
+void orc_gather_vec_e(const orc_params *p, orc_ws *w, const orc_fields *f)
+{
+    const int mx = p->mx, my = p->my;
+    const int nv = w->nv;
+    int k0 = 1, k1 = 1;
+    if (p->ndim == 3) { k0 = 2; k1 = p->mz - 1; }
+    const double *Fx = f->Ex, *Fy = f->Ey, *Fz = f->Ez;
+    double *Vx = w->vec[0], *Vy = w->vec[1], *Vz = w->vec[2];
+#define F3(A, i, j, k) A[IX(p, i, j, k)]
+    for (int k = k0; k <= k1; k++)
+        #pragma omp parallel for schedule(static)
+        for (int j = 2; j <= my - 1; j++)
+            for (int i = 2; i <= mx - 1; i++) {
+                int64_t s = (int64_t)(k - 1) * my * mx + (int64_t)(j - 1) * mx + i;  /* 1-based */
+                double *v = Vx + nv * (s - 1);
+                v[0] = (F3(Fx, i - 1, j, k) + F3(Fx, i, j, k)) * 0.5;
+                v[1] = (F3(Fx, i, j, k) + F3(Fx, i + 1, j, k)) * 0.5;
+                v[2] = (F3(Fx, i - 1, j + 1, k) + F3(Fx, i, j + 1, k)) * 0.5;
+                v[3] = (F3(Fx, i, j + 1, k) + F3(Fx, i + 1, j + 1, k)) * 0.5;
+                if (p->ndim == 3) {
+                    v[4] = (F3(Fx, i - 1, j, k + 1) + F3(Fx, i, j, k + 1)) * 0.5;
+                    v[5] = (F3(Fx, i, j, k + 1) + F3(Fx, i + 1, j, k + 1)) * 0.5;
+                    v[6] = (F3(Fx, i - 1, j + 1, k + 1) + F3(Fx, i, j + 1, k + 1)) * 0.5;
+                    v[7] = (F3(Fx, i, j + 1, k + 1) + F3(Fx, i + 1, j + 1, k + 1)) * 0.5;
+                }
+                v = Vy + nv * (s - 1);
+                v[0] = (F3(Fy, i, j - 1, k) + F3(Fy, i, j, k)) * 0.5;
+                v[1] = (F3(Fy, i + 1, j - 1, k) + F3(Fy, i + 1, j, k)) * 0.5;
+                v[2] = (F3(Fy, i, j, k) + F3(Fy, i, j + 1, k)) * 0.5;
+                v[3] = (F3(Fy, i + 1, j, k) + F3(Fy, i + 1, j + 1, k)) * 0.5;
+                if (p->ndim == 3) {
+                    v[4] = (F3(Fy, i, j - 1, k + 1) + F3(Fy, i, j, k + 1)) * 0.5;
+                    v[5] = (F3(Fy, i + 1, j - 1, k + 1) + F3(Fy, i + 1, j, k + 1)) * 0.5;
+                    v[6] = (F3(Fy, i, j, k + 1) + F3(Fy, i, j + 1, k + 1)) * 0.5;
+                    v[7] = (F3(Fy, i + 1, j, k + 1) + F3(Fy, i + 1, j + 1, k + 1)) * 0.5;
+                }
+                v = Vz + nv * (s - 1);
+                if (p->ndim == 3) {
+                    v[0] = (F3(Fz, i, j, k - 1) + F3(Fz, i, j, k)) * 0.5;
+                    v[1] = (F3(Fz, i + 1, j, k - 1) + F3(Fz, i + 1, j, k)) * 0.5;
+                    v[2] = (F3(Fz, i, j + 1, k - 1) + F3(Fz, i, j + 1, k)) * 0.5;
+                    v[3] = (F3(Fz, i + 1, j + 1, k - 1) + F3(Fz, i + 1, j + 1, k)) * 0.5;
+                    v[4] = (F3(Fz, i, j, k) + F3(Fz, i, j, k + 1)) * 0.5;
+                    v[5] = (F3(Fz, i + 1, j, k) + F3(Fz, i + 1, j, k + 1)) * 0.5;
+                    v[6] = (F3(Fz, i, j + 1, k) + F3(Fz, i, j + 1, k + 1)) * 0.5;
+                    v[7] = (F3(Fz, i + 1, j + 1, k) + F3(Fz, i + 1, j + 1, k + 1)) * 0.5;
+                } else {
+                    v[0] = F3(Fz, i, j, k);
+                    v[1] = F3(Fz, i + 1, j, k);
+                    v[2] = F3(Fz, i, j + 1, k);
+                    v[3] = F3(Fz, i + 1, j + 1, k);
+                }
+            }
+}
+
+void orc_gather_vec_b(const orc_params *p, orc_ws *w, const orc_fields *f)
+{
+    const int mx = p->mx, my = p->my;
+    const int nv = w->nv;
+    int k0 = 1, k1 = 1;
+    if (p->ndim == 3) { k0 = 2; k1 = p->mz - 1; }
+    const double *Bx = f->Bx, *By = f->By, *Bz = f->Bz;
+    const double ex = p->Bx_ext0, ey = p->By_ext0, ez = p->Bz_ext0;
+    double *Vx = w->vec[3], *Vy = w->vec[4], *Vz = w->vec[5];
+    for (int k = k0; k <= k1; k++)
+        #pragma omp parallel for schedule(static)
+        for (int j = 2; j <= my - 1; j++)
+            for (int i = 2; i <= mx - 1; i++) {
+                int64_t s = (int64_t)(k - 1) * my * mx + (int64_t)(j - 1) * mx + i;
+                double *v = Vx + nv * (s - 1);
+                if (p->ndim == 3) {
+                    v[0] = (F3(Bx, i, j - 1, k - 1) + F3(Bx, i, j, k - 1) + F3(Bx, i, j - 1, k) + F3(Bx, i, j, k)) * 0.25 + ex;
+                    v[1] = (F3(Bx, i + 1, j - 1, k - 1) + F3(Bx, i + 1, j, k - 1) + F3(Bx, i + 1, j - 1, k) + F3(Bx, i + 1, j, k)) * 0.25 + ex;
+                    v[2] = (F3(Bx, i, j, k - 1) + F3(Bx, i, j + 1, k - 1) + F3(Bx, i, j, k) + F3(Bx, i, j + 1, k)) * 0.25 + ex;
+                    v[3] = (F3(Bx, i + 1, j, k - 1) + F3(Bx, i + 1, j + 1, k - 1) + F3(Bx, i + 1, j, k) + F3(Bx, i + 1, j + 1, k)) * 0.25 + ex;
+                    v[4] = (F3(Bx, i, j - 1, k) + F3(Bx, i, j, k) + F3(Bx, i, j - 1, k + 1) + F3(Bx, i, j, k + 1)) * 0.25 + ex;
+                    v[5] = (F3(Bx, i + 1, j - 1, k) + F3(Bx, i + 1, j, k) + F3(Bx, i + 1, j - 1, k + 1) + F3(Bx, i + 1, j, k + 1)) * 0.25 + ex;
+                    v[6] = (F3(Bx, i, j, k) + F3(Bx, i, j + 1, k) + F3(Bx, i, j, k + 1) + F3(Bx, i, j + 1, k + 1)) * 0.25 + ex;
+                    v[7] = (F3(Bx, i + 1, j, k) + F3(Bx, i + 1, j + 1, k) + F3(Bx, i + 1, j, k + 1) + F3(Bx, i + 1, j + 1, k + 1)) * 0.25 + ex;
+                } else {
+                    v[0] = (F3(Bx, i, j - 1, k) + F3(Bx, i, j, k)) * 0.5 + ex;
+                    v[1] = (F3(Bx, i + 1, j - 1, k) + F3(Bx, i + 1, j, k)) * 0.5 + ex;
+                    v[2] = (F3(Bx, i, j, k) + F3(Bx, i, j + 1, k)) * 0.5 + ex;
+                    v[3] = (F3(Bx, i + 1, j, k) + F3(Bx, i + 1, j + 1, k)) * 0.5 + ex;
+                }
+                v = Vy + nv * (s - 1);
+                if (p->ndim == 3) {
+                    v[0] = (F3(By, i - 1, j, k - 1) + F3(By, i - 1, j, k) + F3(By, i, j, k - 1) + F3(By, i, j, k)) * 0.25 + ey;
+                    v[1] = (F3(By, i, j, k - 1) + F3(By, i, j, k) + F3(By, i + 1, j, k - 1) + F3(By, i + 1, j, k)) * 0.25 + ey;
+                    v[2] = (F3(By, i - 1, j + 1, k - 1) + F3(By, i - 1, j + 1, k) + F3(By, i, j + 1, k - 1) + F3(By, i, j + 1, k)) * 0.25 + ey;
+                    v[3] = (F3(By, i, j + 1, k - 1) + F3(By, i, j + 1, k) + F3(By, i + 1, j + 1, k - 1) + F3(By, i + 1, j + 1, k)) * 0.25 + ey;
+                    v[4] = (F3(By, i - 1, j, k) + F3(By, i - 1, j, k + 1) + F3(By, i, j, k) + F3(By, i, j, k + 1)) * 0.25 + ey;
+                    v[5] = (F3(By, i, j, k) + F3(By, i, j, k + 1) + F3(By, i + 1, j, k) + F3(By, i + 1, j, k + 1)) * 0.25 + ey;
+                    v[6] = (F3(By, i - 1, j + 1, k) + F3(By, i - 1, j + 1, k + 1) + F3(By, i, j + 1, k) + F3(By, i, j + 1, k + 1)) * 0.25 + ey;
+                    v[7] = (F3(By, i, j + 1, k) + F3(By, i, j + 1, k + 1) + F3(By, i + 1, j + 1, k) + F3(By, i + 1, j + 1, k + 1)) * 0.25 + ey;
+                } else {
+                    v[0] = (F3(By, i - 1, j, k) + F3(By, i, j, k)) * 0.5 + ey;
+                    v[1] = (F3(By, i, j, k) + F3(By, i + 1, j, k)) * 0.5 + ey;
+                    v[2] = (F3(By, i - 1, j + 1, k) + F3(By, i, j + 1, k)) * 0.5 + ey;
+                    v[3] = (F3(By, i, j + 1, k) + F3(By, i + 1, j + 1, k)) * 0.5 + ey;
+                }
+                v = Vz + nv * (s - 1);
+                v[0] = (F3(Bz, i - 1, j - 1, k) + F3(Bz, i - 1, j, k) + F3(Bz, i, j - 1, k) + F3(Bz, i, j, k)) * 0.25 + ez;
+                v[1] = (F3(Bz, i, j - 1, k) + F3(Bz, i, j, k) + F3(Bz, i + 1, j - 1, k) + F3(Bz, i + 1, j, k)) * 0.25 + ez;
+                v[2] = (F3(Bz, i - 1, j, k) + F3(Bz, i - 1, j + 1, k) + F3(Bz, i, j, k) + F3(Bz, i, j + 1, k)) * 0.25 + ez;
+                v[3] = (F3(Bz, i, j, k) + F3(Bz, i, j + 1, k) + F3(Bz, i + 1, j, k) + F3(Bz, i + 1, j + 1, k)) * 0.25 + ez;
+                if (p->ndim == 3) {
+                    v[4] = (F3(Bz, i - 1, j - 1, k + 1) + F3(Bz, i - 1, j, k + 1) + F3(Bz, i, j - 1, k + 1) + F3(Bz, i, j, k + 1)) * 0.25 + ez;
+                    v[5] = (F3(Bz, i, j - 1, k + 1) + F3(Bz, i, j, k + 1) + F3(Bz, i + 1, j - 1, k + 1) + F3(Bz, i + 1, j, k + 1)) * 0.25 + ez;
+                    v[6] = (F3(Bz, i - 1, j, k + 1) + F3(Bz, i - 1, j + 1, k + 1) + F3(Bz, i, j, k + 1) + F3(Bz, i, j + 1, k + 1)) * 0.25 + ez;
+                    v[7] = (F3(Bz, i, j, k + 1) + F3(Bz, i, j + 1, k + 1) + F3(Bz, i + 1, j, k + 1) + F3(Bz, i + 1, j + 1, k + 1)) * 0.25 + ez;
+                }
+            }
+#undef F3
+}
+
+/* ------------------------------------------------------------------ */
+/* Mover: movdep.F90:12-258                                            */
+/* ------------------------------------------------------------------ */
+
+static inline double dmin(double a, double b) { return (b < a) ? b : a; }
+static inline double dmax(double a, double b) { return (b > a) ? b : a; }
+static inline int imin2(int a, int b) { return a < b ? a : b; }
+static inline int imax2(int a, int b) { return a > b ? a : b; }
+
+/* ReduceCurrentMatrix, movdep_routines.F90:243-311.  The reference threads
+ * the scatter over j1 (2-D) / k1 (3-D) while iteration j1 also writes row
+ * j1+1, which races at chunk borders; the threaded build here runs the same
+ * scatter as two passes over even and odd rows so that it is race free. */
+static void reduce_current(const orc_params *p, orc_ws *w, orc_fields *f)
+{
+    const int nj = w->nj, nt = w->nthreads;
+    const int64_t ncell = w->ncell;
+    double *V = w->vecj;
+    if (nt > 1) {
+        #pragma omp parallel for schedule(static)
+        for (int64_t c = 0; c < ncell; c++)
+            for (int t = 1; t < nt; t++)
+                for (int n = 0; n < nj; n++)
+                    V[n + nj * c] += V[n + nj * c + (int64_t)nj * ncell * t];
+    }
+    const int mx = p->mx, my = p->my, mz = p->mz;
+    if (p->ndim == 2) {
+        for (int par = 0; par < 2; par++) {
+            #pragma omp parallel for schedule(static)
+            for (int j1 = 1 + par; j1 <= my - 1; j1 += 2)
+                for (int i1 = 1; i1 <= mx - 1; i1++) {
+                    const double *v = V + nj * ((int64_t)(j1 - 1) * mx + i1 - 1);
+                    f->Jx[IX(p, i1, j1, 1)] += v[0];
+                    f->Jx[IX(p, i1, j1 + 1, 1)] += v[1];
+                    f->Jy[IX(p, i1, j1, 1)] += v[2];
+                    f->Jy[IX(p, i1 + 1, j1, 1)] += v[3];
+                    f->Jz[IX(p, i1, j1, 1)] += v[4];
+                    f->Jz[IX(p, i1 + 1, j1, 1)] += v[5];
+                    f->Jz[IX(p, i1, j1 + 1, 1)] += v[6];
+                    f->Jz[IX(p, i1 + 1, j1 + 1, 1)] += v[7];
+                }
+        }
+    } else {
+        for (int par = 0; par < 2; par++) {
+            #pragma omp parallel for schedule(static)
+            for (int k1 = 1 + par; k1 <= mz - 1; k1 += 2)
+                for (int j1 = 1; j1 <= my - 1; j1++)
+                    for (int i1 = 1; i1 <= mx - 1; i1++) {
+                        const double *v = V + nj * ((int64_t)(k1 - 1) * my * mx + (int64_t)(j1 - 1) * mx + i1 - 1);
+                        f->Jx[IX(p, i1, j1, k1)] += v[0];
+                        f->Jx[IX(p, i1, j1 + 1, k1)] += v[1];
+                        f->Jx[IX(p, i1, j1, k1 + 1)] += v[2];
+                        f->Jx[IX(p, i1, j1 + 1, k1 + 1)] += v[3];
+                        f->Jy[IX(p, i1, j1, k1)] += v[4];
+                        f->Jy[IX(p, i1 + 1, j1, k1)] += v[5];
+                        f->Jy[IX(p, i1, j1, k1 + 1)] += v[6];
+                        f->Jy[IX(p, i1 + 1, j1, k1 + 1)] += v[7];
+                        f->Jz[IX(p, i1, j1, k1)] += v[8];
+                        f->Jz[IX(p, i1 + 1, j1, k1)] += v[9];
+                        f->Jz[IX(p, i1, j1 + 1, k1)] += v[10];
+                        f->Jz[IX(p, i1 + 1, j1 + 1, k1)] += v[11];
+                    }
+        }
+    }
+}
+
+void orc_move_deposit(const orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s)
+{
+    const int two = (p->ndim == 2);
+    const int nv = w->nv, nj = w->nj;
+    const int mx = p->mx, my = p->my;
+    const int64_t used = s->used, ncell = w->ncell;
+    const double c = p->c, sqc = p->sqc, cinv = p->cinv, qi = p->qi;
+    const double *wtx1 = two ? wtx1_2 : wtx1_3, *wtx2 = two ? wtx2_2 : wtx2_3;
+    const double *wty1 = two ? wty1_2 : wty1_3, *wty2 = two ? wty2_2 : wty2_3;
+    const double *wtFx = two ? wtFx_2 : wtFx_3, *wtFy = two ? wtFy_2 : wtFy_3, *wtFz = two ? wtFz_2 : wtFz_3;
+    const double *Jwtx1 = two ? Jwtx1_2 : Jwtx1_3, *Jwtx2 = two ? Jwtx2_2 : Jwtx2_3;
+    const double *Jwty1 = two ? Jwty1_2 : Jwty1_3, *Jwty2 = two ? Jwty2_2 : Jwty2_3;
+    const double *Jwtz1 = two ? Jwtz1_2 : Jwtz1_3, *Jwtz2 = two ? Jwtz2_2 : Jwtz2_3;
+
+    orc_gather_vec_e(p, w, f);
+    orc_gather_vec_b(p, w, f);
+    memset(w->vecj, 0, sizeof(double) * (size_t)(nj * ncell * w->nthreads));   /* VecJ = 0 */
+
+    const double *VEx = w->vec[0], *VEy = w->vec[1], *VEz = w->vec[2];
+    const double *VBx = w->vec[3], *VBy = w->vec[4], *VBz = w->vec[5];
+
+    #pragma omp parallel for schedule(static)
+    for (int64_t off = 0; off < used; off += VBS) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *VJ = w->vecj + (int64_t)nj * ncell * tid;
+        const int nb = (int)((used - off < VBS) ? (used - off) : VBS);
+        double pEx[VBS], pEy[VBS], pEz[VBS], pBx[VBS], pBy[VBS], pBz[VBS];
+        double u0[VBS], v0[VBS], w0[VBS], u1[VBS], v1[VBS], w1[VBS], g[VBS], ff[VBS];
+        double x0[VBS], y0[VBS], z0[VBS], qm[VBS];
+        int ip0[VBS], jp0[VBS], kp0[VBS];
+
+        for (int n = 0; n < nb; n++) {
+            x0[n] = s->x[off + n]; y0[n] = s->y[off + n]; z0[n] = s->z[off + n];
+        }
+        for (int n = 0; n < nb; n++)
+            qm[n] = (s->flv[off + n] == 0) ? 0.0 : p->flvrqm[s->flv[off + n] - 1] * 0.5;
+        for (int n = 0; n < nb; n++) {
+            pEx[n] = 0.0; pEy[n] = 0.0; pEz[n] = 0.0;
+            pBx[n] = 0.0; pBy[n] = 0.0; pBz[n] = 0.0;
+        }
+        for (int n = 0; n < nb; n++) {
+            double wt[8];
+            ip0[n] = (int)x0[n];        /* Fortran real -> integer truncation */
+            jp0[n] = (int)y0[n];
+            kp0[n] = (int)z0[n];
+            double dx = x0[n] - ip0[n];
+            double dy = y0[n] - jp0[n];
+            int64_t sidx;
+            if (two) {
+                for (int nn = 0; nn < 4; nn++)
+                    wt[nn] = (wtx1[nn] + wtx2[nn] * dx) * (wty1[nn] + wty2[nn] * dy);
+                sidx = (int64_t)(jp0[n] - 1) * mx + ip0[n];
+            } else {
+                double dz = z0[n] - kp0[n];
+                for (int nn = 0; nn < 8; nn++)
+                    wt[nn] = (wtx1[nn] + wtx2[nn] * dx) * (wty1[nn] + wty2[nn] * dy) * (wtz1_3[nn] + wtz2_3[nn] * dz);
+                sidx = (int64_t)(kp0[n] - 1) * mx * my + (int64_t)(jp0[n] - 1) * mx + ip0[n];
+            }
+            const int64_t b = nv * (sidx - 1);
+            for (int nn = 0; nn < nv; nn++) {
+                pEx[n] = pEx[n] + wt[nn] * VEx[b + nn];
+                pEy[n] = pEy[n] + wt[nn] * VEy[b + nn];
+                pEz[n] = pEz[n] + wt[nn] * VEz[b + nn];
+                pBx[n] = pBx[n] + wt[nn] * VBx[b + nn];
+                pBy[n] = pBy[n] + wt[nn] * VBy[b + nn];
+                pBz[n] = pBz[n] + wt[nn] * VBz[b + nn];
+            }
+        }
+        for (int n = 0; n < nb; n++) {
+            pEx[n] = pEx[n] * qm[n]; pEy[n] = pEy[n] * qm[n]; pEz[n] = pEz[n] * qm[n];
+            pBx[n] = pBx[n] * qm[n]; pBy[n] = pBy[n] * qm[n]; pBz[n] = pBz[n] * qm[n];
+        }
+        /* Boris pusher, movdep.F90:143-173 */
+        for (int n = 0; n < nb; n++) {
+            const int64_t m = off + n;
+            u0[n] = c * s->u[m] + pEx[n];
+            v0[n] = c * s->v[m] + pEy[n];
+            w0[n] = c * s->w[m] + pEz[n];
+            g[n] = 1.0 / sqrt(sqc + u0[n] * u0[n] + v0[n] * v0[n] + w0[n] * w0[n]);
+            pBx[n] = g[n] * pBx[n];
+            pBy[n] = g[n] * pBy[n];
+            pBz[n] = g[n] * pBz[n];
+            ff[n] = 2.0 / (1.0 + pBx[n] * pBx[n] + pBy[n] * pBy[n] + pBz[n] * pBz[n]);
+            u1[n] = (u0[n] + v0[n] * pBz[n] - w0[n] * pBy[n]) * ff[n];
+            v1[n] = (v0[n] + w0[n] * pBx[n] - u0[n] * pBz[n]) * ff[n];
+            w1[n] = (w0[n] + u0[n] * pBy[n] - v0[n] * pBx[n]) * ff[n];
+            u0[n] = u0[n] + v1[n] * pBz[n] - w1[n] * pBy[n] + pEx[n];
+            v0[n] = v0[n] + w1[n] * pBx[n] - u1[n] * pBz[n] + pEy[n];
+            w0[n] = w0[n] + u1[n] * pBy[n] - v1[n] * pBx[n] + pEz[n];
+            s->u[m] = u0[n] * cinv;
+            s->v[m] = v0[n] * cinv;
+            s->w[m] = w0[n] * cinv;
+            g[n] = sqc / sqrt(sqc + u0[n] * u0[n] + v0[n] * v0[n] + w0[n] * w0[n]);
+            s->x[m] = s->x[m] + s->u[m] * g[n] * p->inv_dx;
+            s->y[m] = s->y[m] + s->v[m] * g[n] * p->inv_dy;
+            s->z[m] = s->z[m] + s->w[m] * g[n] * p->inv_dz;
+        }
+        /* zigzag deposit, movdep.F90:177-237 */
+        for (int n = 0; n < nb; n++) {
+            const int64_t m = off + n;
+            const int ip = (int)s->x[m], jp = (int)s->y[m], kp = (int)s->z[m];
+            const double xr = dmin((double)(imin2(ip0[n], ip) + 1), dmax((double)imax2(ip0[n], ip), 0.5 * (x0[n] + s->x[m])));
+            const double yr = dmin((double)(imin2(jp0[n], jp) + 1), dmax((double)imax2(jp0[n], jp), 0.5 * (y0[n] + s->y[m])));
+            const double zr = dmin((double)(imin2(kp0[n], kp) + 1), dmax((double)imax2(kp0[n], kp), 0.5 * (z0[n] + s->z[m])));
+            const double qthis = s->q[m] * qi;
+            double Fx1 = qthis * (xr - x0[n]);
+            double Fy1 = qthis * (yr - y0[n]);
+            double Fz1 = qthis * (zr - z0[n]);
+            double Wx1 = 0.5 * (x0[n] + xr) - ip0[n];
+            double Wy1 = 0.5 * (y0[n] + yr) - jp0[n];
+            double Wz1;
+            int64_t sidx;
+            if (two) {
+                Wz1 = 0.0;
+                sidx = (int64_t)(jp0[n] - 1) * mx + ip0[n];
+            } else {
+                Wz1 = 0.5 * (z0[n] + zr) - kp0[n];
+                sidx = (int64_t)(kp0[n] - 1) * mx * my + (int64_t)(jp0[n] - 1) * mx + ip0[n];
+            }
+            double *vj = VJ + nj * (sidx - 1);
+            for (int nn = 0; nn < nj; nn++)
+                vj[nn] = vj[nn] + (wtFx[nn] * Fx1 + wtFy[nn] * Fy1 + wtFz[nn] * Fz1) * (Jwtx1[nn] + Jwtx2[nn] * Wx1)
+                                  * (Jwty1[nn] + Jwty2[nn] * Wy1) * (Jwtz1[nn] + Jwtz2[nn] * Wz1);
+
+            double Fx2 = qthis * (s->x[m] - xr);
+            double Fy2 = qthis * (s->y[m] - yr);
+            double Fz2 = qthis * (s->z[m] - zr);
+            double Wx2 = 0.5 * (s->x[m] + xr) - ip;
+            double Wy2 = 0.5 * (s->y[m] + yr) - jp;
+            double Wz2;
+            if (two) {
+                Wz2 = 0.0;
+                sidx = (int64_t)(jp - 1) * mx + ip;
+            } else {
+                Wz2 = 0.5 * (s->z[m] + zr) - kp;
+                sidx = (int64_t)(kp - 1) * mx * my + (int64_t)(jp - 1) * mx + ip;
+            }
+            vj = VJ + nj * (sidx - 1);
+            for (int nn = 0; nn < nj; nn++)
+                vj[nn] = vj[nn] + (wtFx[nn] * Fx2 + wtFy[nn] * Fy2 + wtFz[nn] * Fz2) * (Jwtx1[nn] + Jwtx2[nn] * Wx2)
+                                  * (Jwty1[nn] + Jwty2[nn] * Wy2) * (Jwtz1[nn] + Jwtz2[nn] * Wz2);
+        }
+        /* 2-D periodic wrap in z, movdep.F90:241-249 */
+        if (two)
+            for (int n = 0; n < nb; n++) {
+                const int64_t m = off + n;
+                if (s->z[m] > p->zmax) s->z[m] = s->z[m] - p->zlen;
+                else if (s->z[m] < p->zmin) s->z[m] = p->zlen + s->z[m];
+            }
+    }
+    reduce_current(p, w, f);
+}
+
+/* ------------------------------------------------------------------ */
+/* Particle exchange on one periodic rank                              */
+/* ------------------------------------------------------------------ */
+
+/* RemovePrtl, loadprtlout.F90:164-181 */
+static void remove_prtl(const orc_params *p, orc_particles *s, int64_t i)
+{
+    s->q[i] = 0; s->flv[i] = 0;
+    s->u[i] = 0; s->v[i] = 0; s->w[i] = 0;
+    s->x[i] = 3.8; s->y[i] = 3.8;
+    s->z[i] = (p->ndim == 2) ? 1.5 : 3.8;
+}
+
+typedef struct { int64_t n, cap; double *d; } obuf;   /* 7 doubles + flv per record */
+
+static int obuf_push(obuf *b, const orc_particles *s, int64_t i, double xs, double ys, double zs)
+{
+    if (b->n == b->cap) {
+        int64_t nc = b->cap ? 2 * b->cap : 1024;
+        double *nd = (double *)realloc(b->d, sizeof(double) * 8 * (size_t)nc);
+        if (!nd) return ORC_ERR_CAPACITY;
+        b->d = nd; b->cap = nc;
+    }
+    double *r = b->d + 8 * b->n;
+    /* CopyTransferPrtl, loadprtlout.F90:128-142: position minus the send shift */
+    r[0] = s->q[i]; r[1] = s->x[i] - xs; r[2] = s->y[i] - ys; r[3] = s->z[i] - zs;
+    r[4] = s->u[i]; r[5] = s->v[i]; r[6] = s->w[i]; r[7] = (double)s->flv[i];
+    b->n++;
+    return ORC_OK;
+}
+
+/* ExchangePrtl + AppendParticles for nSubDomains = 1 (every face is its own
+ * neighbour).  Send shifts: face 2 xlen, face 4 ylen, face 6 zlen
+ * (subdomains.F90:720-740); the receiving face adds its own shift
+ * (mem_prtl.F90:135-160).  Receive face n gets what the opposite send face
+ * sent (tag pairing subdomains.F90:595-610), and arrivals are appended in
+ * receive-face order 1..6. */
+int orc_exchange_particles(const orc_params *p, orc_particles *s)
+{
+    obuf out[6];
+    memset(out, 0, sizeof(out));
+    const double xmin = p->xmin, xmax = p->xmax, ymin = p->ymin, ymax = p->ymax, zmin = p->zmin, zmax = p->zmax;
+    const double sendx[6] = {0, p->xlen, 0, 0, 0, 0};
+    const double sendy[6] = {0, 0, 0, p->ylen, 0, 0};
+    const double sendz[6] = {0, 0, 0, 0, 0, p->zlen};
+    const int three = (p->ndim == 3);
+    int rc = ORC_OK;
+    /* LoadPrtlOutliers, loadprtlout.F90:22-97 */
+    for (int64_t i = 0; i < s->used && rc == ORC_OK; i++) {
+        if (s->flv[i] == 0) continue;
+        double x = s->x[i], y = s->y[i], z = s->z[i];
+        if (x < xmin) {
+            if ((y <= (ymax + xmin - x)) && (y >= (ymin - xmin + x))) {
+                if (!three || ((z <= (zmax + xmin - x)) && (z >= (zmin - xmin + x)))) {
+                    rc |= obuf_push(&out[0], s, i, sendx[0], sendy[0], sendz[0]);
+                    remove_prtl(p, s, i);
+                }
+            }
+        } else if (x > xmax) {
+            if ((y <= (ymax - xmax + x)) && (y >= (ymin + xmax - x))) {
+                if (!three || ((z <= (zmax - xmax + x)) && (z >= (zmin + xmax - x)))) {
+                    rc |= obuf_push(&out[1], s, i, sendx[1], sendy[1], sendz[1]);
+                    remove_prtl(p, s, i);
+                }
+            }
+        }
+        x = s->x[i]; y = s->y[i]; z = s->z[i];
+        if (y < ymin) {
+            if ((y < (ymin - xmin + x)) && (y < (ymin + xmax - x))) {
+                if (!three || ((z >= (zmin - ymin + y)) && (z <= (zmax + ymin - y)))) {
+                    rc |= obuf_push(&out[2], s, i, sendx[2], sendy[2], sendz[2]);
+                    remove_prtl(p, s, i);
+                }
+            }
+        } else if (y > ymax) {
+            if ((y > (ymax + xmin - x)) && (y > (ymax - xmax + x))) {
+                if (!three || ((z >= (zmin + ymax - y)) && (z <= (zmax - ymax + y)))) {
+                    rc |= obuf_push(&out[3], s, i, sendx[3], sendy[3], sendz[3]);
+                    remove_prtl(p, s, i);
+                }
+            }
+        }
+        if (three) {
+            x = s->x[i]; y = s->y[i]; z = s->z[i];
+            if (z < zmin) {
+                if ((z < (zmin - xmin + x)) && (z < (zmin + xmax - x))) {
+                    if ((z < (zmin - ymin + y)) && (z < (zmin + ymax - y))) {
+                        rc |= obuf_push(&out[4], s, i, sendx[4], sendy[4], sendz[4]);
+                        remove_prtl(p, s, i);
+                    }
+                }
+            } else if (z > zmax) {
+                if ((z > (zmax + xmin - x)) && (z > (zmax - xmax + x))) {
+                    if ((z > (zmax + ymin - y)) && (z > (zmax - ymax + y))) {
+                        rc |= obuf_push(&out[5], s, i, sendx[5], sendy[5], sendz[5]);
+                        remove_prtl(p, s, i);
+                    }
+                }
+            }
+        }
+    }
+    /* append: receive face n <- send face n^1 (0-based pairs (0,1),(2,3),(4,5)) */
+    const double recvx[6] = {0, p->xlen, 0, 0, 0, 0};
+    const double recvy[6] = {0, 0, 0, p->ylen, 0, 0};
+    const double recvz[6] = {0, 0, 0, 0, 0, p->zlen};
+    int64_t total = 0;
+    for (int n = 0; n < 6; n++) total += out[n].n;
+    if (rc == ORC_OK && s->used + total > s->cap) rc = ORC_ERR_CAPACITY;
+    if (rc == ORC_OK) {
+        int nfaces = three ? 6 : 4;
+        for (int n = 0; n < nfaces; n++) {
+            obuf *b = &out[n ^ 1];
+            for (int64_t r = 0; r < b->n; r++) {
+                const double *d = b->d + 8 * r;
+                int64_t m = s->used + r;
+                s->q[m] = d[0];
+                s->x[m] = d[1] + recvx[n];
+                s->y[m] = d[2] + recvy[n];
+                s->z[m] = d[3] + recvz[n];
+                s->u[m] = d[4]; s->v[m] = d[5]; s->w[m] = d[6];
+                s->flv[m] = (int32_t)d[7];
+            }
+            s->used += b->n;
+        }
+    }
+    for (int n = 0; n < 6; n++) free(out[n].d);
+    return rc;
+}
+
+/* ReorderPrtlArr, mem_prtl.F90:166-301 (single thread: stable by index) */
+int orc_reorder(const orc_params *p, orc_particles *s)
+{
+    const int mx = p->mx, my = p->my, mz = p->mz;
+    const int64_t ncell = (int64_t)mx * my * mz;
+    int64_t *pc = (int64_t *)calloc((size_t)ncell, sizeof(int64_t));
+    if (!pc) return ORC_ERR_CAPACITY;
+    int64_t np = 0;
+    for (int64_t n = 0; n < s->used; n++) {
+        if (s->flv[n] == 0) continue;
+        int i = (int)floor(s->x[n]), j = (int)floor(s->y[n]);
+        int k = (p->ndim == 2) ? 1 : (int)floor(s->z[n]);
+        if (i < 1 || i > mx || j < 1 || j > my || k < 1 || k > mz) { free(pc); return ORC_ERR_ARG; }
+        pc[IX(p, i, j, k)]++;
+        np++;
+    }
+    int64_t offset = 0;
+    for (int64_t c = 0; c < ncell; c++) { int64_t t = pc[c]; pc[c] = offset; offset += t; }
+    double *buf = (double *)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
+    int32_t *ibuf = (int32_t *)malloc(sizeof(int32_t) * (size_t)(np > 0 ? np : 1));
+    int64_t *dst = (int64_t *)malloc(sizeof(int64_t) * (size_t)(s->used > 0 ? s->used : 1));
+    if (!buf || !ibuf || !dst) { free(pc); free(buf); free(ibuf); free(dst); return ORC_ERR_CAPACITY; }
+    for (int64_t n = 0; n < s->used; n++) {
+        if (s->flv[n] == 0) { dst[n] = -1; continue; }
+        int i = (int)floor(s->x[n]), j = (int)floor(s->y[n]);
+        int k = (p->ndim == 2) ? 1 : (int)floor(s->z[n]);
+        dst[n] = pc[IX(p, i, j, k)]++;
+    }
+    double *arrs[7] = {s->q, s->x, s->y, s->z, s->u, s->v, s->w};
+    for (int a = 0; a < 7; a++) {
+        for (int64_t n = 0; n < s->used; n++) if (dst[n] >= 0) buf[dst[n]] = arrs[a][n];
+        memcpy(arrs[a], buf, sizeof(double) * (size_t)np);
+    }
+    for (int64_t n = 0; n < s->used; n++) if (dst[n] >= 0) ibuf[dst[n]] = s->flv[n];
+    memcpy(s->flv, ibuf, sizeof(int32_t) * (size_t)np);
+    for (int64_t n = np; n < s->used; n++) { s->q[n] = 0; s->flv[n] = 0; }
+    s->used = np;
+    free(pc); free(buf); free(ibuf); free(dst);
+    return ORC_OK;
+}
+
+/* One time step in the MAIN.F90:42-284 order.  The reference zeroes J at
+ * the end of a step (MAIN.F90:248); here the same reset runs at the start
+ * of the next step so J stays readable after the step (nothing reads J in
+ * between). */
+int orc_step(const orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, int t)
+{
+    orc_reset_current(p, f);
+    orc_update_b_half(p, f);
+    orc_field_halo(p, f->Bx, f->By, f->Bz);
+    orc_move_deposit(p, w, f, s);
+    int rc = orc_exchange_particles(p, s);
+    if (rc != ORC_OK) return rc;
+    orc_update_b_half(p, f);
+    orc_field_halo(p, f->Bx, f->By, f->Bz);
+    orc_update_e(p, f);
+    orc_field_halo(p, f->Ex, f->Ey, f->Ez);
+    orc_current_halo_add(p, f->Jx, f->Jy, f->Jz);
+    orc_add_current(p, f);
+    orc_field_halo(p, f->Ex, f->Ey, f->Ez);
+    if (p->reorder_period > 0 && t % p->reorder_period == 0) rc = orc_reorder(p, s);
+    return rc;
+}
+
+int orc_run(const orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, int t0, int nsteps)
+{
+    for (int t = t0; t < t0 + nsteps; t++) {
+        int rc = orc_step(p, w, f, s, t);
+        if (rc != ORC_OK) return rc;
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Diagnostics                                                         */
+/* ------------------------------------------------------------------ */
+
+/* CalcEnergy, savedata_routines.F90:265-327: out = {B energy, E energy,
+ * particle energy flavour 1, flavour 2}; wtp = 1 (unit weights). */
+void orc_energy(const orc_params *p, const orc_fields *f, const orc_particles *s, double out[4])
+{
+    double be = 0, ee = 0;
+    int k0 = 1, k1 = 1;
+    if (p->ndim == 3) { k0 = 3; k1 = p->mz - 3; }
+    for (int k = k0; k <= k1; k++)
+        for (int j = 3; j <= p->my - 3; j++)
+            for (int i = 3; i <= p->mx - 3; i++) {
+                int64_t o = IX(p, i, j, k);
+                be = be + (f->Bx[o] * f->Bx[o] + f->By[o] * f->By[o] + f->Bz[o] * f->Bz[o]);
+                ee = ee + (f->Ex[o] * f->Ex[o] + f->Ey[o] * f->Ey[o] + f->Ez[o] * f->Ez[o]);
+            }
+    out[0] = be * 0.5;
+    out[1] = ee * 0.5;
+    out[2] = 0; out[3] = 0;
+    const double mec2 = p->masse * p->c * p->c;
+    for (int64_t n = 0; n < s->used; n++) {
+        int fl = s->flv[n];
+        if (fl < 1 || fl > 2) continue;
+        double gamma = sqrt(1.0 + s->u[n] * s->u[n] + s->v[n] * s->v[n] + s->w[n] * s->w[n]);
+        if (s->q[n] > 0)
+            out[1 + fl] = out[1 + fl] + (gamma - 1.0) * 1.0 * (s->q[n] * p->qi / p->flvrqm[fl - 1]) * p->c * p->c;
+        if (s->q[n] < 0)
+            out[1 + fl] = out[1 + fl] + (gamma - 1.0) * 1.0 * mec2;
+    }
+}
+
+/* CalcDivE, savedata_routines.F90:329-355 */
+void orc_div_e(const orc_params *p, const orc_fields *f, double *div)
+{
+    memset(div, 0, sizeof(double) * (size_t)p->mx * p->my * p->mz);
+    int k0 = 1, k1 = 1;
+    if (p->ndim == 3) { k0 = 2; k1 = p->mz - 2; }
+    for (int k = k0; k <= k1; k++)
+        for (int j = 2; j <= p->my - 2; j++)
+            for (int i = 2; i <= p->mx - 2; i++) {
+                int64_t o = IX(p, i, j, k);
+                if (p->ndim == 2)
+                    div[o] = f->Ex[o] - f->Ex[IX(p, i - 1, j, k)] + f->Ey[o] - f->Ey[IX(p, i, j - 1, k)];
+                else
+                    div[o] = f->Ex[o] - f->Ex[IX(p, i - 1, j, k)] + f->Ey[o] - f->Ey[IX(p, i, j - 1, k)]
+                           + f->Ez[o] - f->Ez[IX(p, i, j, k - 1)];
+            }
+}
+
+static inline int fold(int n, int lo, int hi, int len)
+{
+    while (n < lo) n += len;
+    while (n > hi) n -= len;
+    return n;
+}
+
+/* CIC node charge rho(i,j,k) = sum q*qi * S_x * S_y (* S_z) at the nodes of
+ * the cell holding the particle, folded onto physical nodes 3..m-3.  With
+ * charge-conserving deposition div E - rho is invariant in time. */
+void orc_rho(const orc_params *p, const orc_particles *s, double *rho)
+{
+    memset(rho, 0, sizeof(double) * (size_t)p->mx * p->my * p->mz);
+    for (int64_t n = 0; n < s->used; n++) {
+        if (s->flv[n] == 0) continue;
+        const double qt = s->q[n] * p->qi;
+        const int i = (int)floor(s->x[n]), j = (int)floor(s->y[n]);
+        const double dx = s->x[n] - i, dy = s->y[n] - j;
+        const double sx[2] = {1.0 - dx, dx}, sy[2] = {1.0 - dy, dy};
+        if (p->ndim == 2) {
+            for (int b = 0; b < 2; b++)
+                for (int a = 0; a < 2; a++) {
+                    int ii = fold(i + a, 3, p->mx - 3, p->nx), jj = fold(j + b, 3, p->my - 3, p->ny);
+                    rho[IX(p, ii, jj, 1)] += qt * sx[a] * sy[b];
+                }
+        } else {
+            const int k = (int)floor(s->z[n]);
+            const double dz = s->z[n] - k;
+            const double sz[2] = {1.0 - dz, dz};
+            for (int cc = 0; cc < 2; cc++)
+                for (int b = 0; b < 2; b++)
+                    for (int a = 0; a < 2; a++) {
+                        int ii = fold(i + a, 3, p->mx - 3, p->nx), jj = fold(j + b, 3, p->my - 3, p->ny);
+                        int kk = fold(k + cc, 3, p->mz - 3, p->nz);
+                        rho[IX(p, ii, jj, kk)] += qt * sx[a] * sy[b] * sz[cc];
+                    }
+        }
+    }
+}
+
+void orc_gauss(const orc_params *p, const orc_fields *f, const orc_particles *s, double out[2])
+{
+    size_t n = (size_t)p->mx * p->my * p->mz;
+    double *div = (double *)malloc(sizeof(double) * n);
+    double *rho = (double *)malloc(sizeof(double) * n);
+    out[0] = out[1] = -1.0;
+    if (!div || !rho) { free(div); free(rho); return; }
+    orc_div_e(p, f, div);
+    orc_rho(p, s, rho);
+    double rmax = 0, amax = 0;
+    int k0 = 1, k1 = 1;
+    if (p->ndim == 3) { k0 = 3; k1 = p->mz - 3; }
+    for (int k = k0; k <= k1; k++)
+        for (int j = 3; j <= p->my - 3; j++)
+            for (int i = 3; i <= p->mx - 3; i++) {
+                int64_t o = IX(p, i, j, k);
+                double r = fabs(div[o] - rho[o]);
+                if (r > rmax) rmax = r;
+                if (fabs(rho[o]) > amax) amax = fabs(rho[o]);
+            }
+    out[0] = rmax; out[1] = amax;
+    free(div); free(rho);
+}
+
+/* sum of m * gamma*beta per component (m = massi for flavour 1, masse for 2) */
+void orc_momentum(const orc_params *p, const orc_particles *s, double out[3])
+{
+    out[0] = out[1] = out[2] = 0;
+    for (int64_t n = 0; n < s->used; n++) {
+        int fl = s->flv[n];
+        if (fl < 1 || fl > 2) continue;
+        double m = (fl == 1) ? p->massi : p->masse;
+        out[0] += m * s->u[n]; out[1] += m * s->v[n]; out[2] += m * s->w[n];
+    }
+}

@@ -1,0 +1,98 @@
+/*
+ * pictor_oracle.h -- CPU FP64 restatement of PICTOR's particle step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against (tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg).  It is never linked into, loaded by or called from the
+ * product library (pictor_amd/).
+ *
+ * Every routine follows the expression order of the cited Fortran so that a
+ * build with -ffp-contract=off reproduces the reference CPU path's rounding
+ * (single thread).  Arrays use the reference layout: fields are (mx,my,mz)
+ * column-major with 1-based indices (element (1,1,1) at offset 0); particles
+ * are SoA q,x,y,z,u,v,w (double) + flv (int32), holes marked flv=0.
+ */
+#ifndef PICTOR_ORACLE_H
+#define PICTOR_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_params {
+    int32_t ndim;            /* 2 or 3 */
+    int32_t nx, ny, nz;      /* physical cells (nz ignored in 2-D) */
+    int32_t mx, my, mz;      /* n + 5 (mz = 1 in 2-D), subdomains.F90:42-50 */
+    int32_t epc;             /* particles per cell per species */
+    double c, c_ompe, g0, mi_me;
+    double me, mi, ompe, qi, qe, qmi, qme, masse, massi;  /* initialise.F90:44-53 */
+    double fldc, fld_halfc, cinv, sqc;                    /* vars.F90:220-223 */
+    double inv_dx, inv_dy, inv_dz;                        /* vars.F90:15-17 */
+    double xmin, xmax, ymin, ymax, zmin, zmax, xlen, ylen, zlen; /* subdomains.F90:52-67 */
+    double Bx_ext0, By_ext0, Bz_ext0;
+    int32_t nflvr;
+    int32_t reorder_period;  /* parameters.F90:71 */
+    double flvrqm[8];        /* initialise.F90:84-85 */
+} orc_params;
+
+typedef struct orc_fields {
+    double *Ex, *Ey, *Ez, *Bx, *By, *Bz, *Jx, *Jy, *Jz;
+} orc_fields;
+
+typedef struct orc_particles {
+    int64_t cap;    /* allocated length of every array */
+    int64_t used;   /* used_prtl_arr_size (vars.F90:125-127) */
+    double *q, *x, *y, *z, *u, *v, *w;
+    int32_t *flv;
+} orc_particles;
+
+typedef struct orc_ws orc_ws;   /* VecE/VecB/VecJ work arrays (mem_fld.F90:6-32) */
+
+/* status codes */
+#define ORC_OK 0
+#define ORC_ERR_CAPACITY 12     /* same meaning as the reference's Abort(12) */
+#define ORC_ERR_ARG 1
+
+int  orc_params_init(orc_params *p, int ndim, int nx, int ny, int nz, double c,
+                     double c_ompe, double g0, double mi_me, int epc);
+orc_ws *orc_ws_create(const orc_params *p, int nthreads);
+void orc_ws_destroy(orc_ws *w);
+int  orc_ws_threads(const orc_ws *w);
+
+/* Maxwell solver, em_update.F90 */
+void orc_update_b_half(const orc_params *p, orc_fields *f);
+void orc_update_e(const orc_params *p, orc_fields *f);
+void orc_add_current(const orc_params *p, orc_fields *f);
+void orc_reset_current(const orc_params *p, orc_fields *f);
+/* Periodic single-rank halos, comm_fld.F90 */
+void orc_field_halo(const orc_params *p, double *fx, double *fy, double *fz);
+void orc_current_halo_add(const orc_params *p, double *jx, double *jy, double *jz);
+/* Particle mover, movdep.F90 + movdep_routines.F90 */
+void orc_gather_vec_e(const orc_params *p, orc_ws *w, const orc_fields *f);
+void orc_gather_vec_b(const orc_params *p, orc_ws *w, const orc_fields *f);
+void orc_move_deposit(const orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s);
+/* The VecE/VecB corner arrays of the last gather (for tests). */
+const double *orc_ws_vec(const orc_ws *w, int which);   /* 0..5 = Ex,Ey,Ez,Bx,By,Bz */
+int  orc_ws_vec_len(const orc_ws *w);                   /* 4 (2-D) or 8 (3-D) */
+/* Particle exchange (wrap + append), loadprtlout.F90 + comm_prtl.F90 */
+int  orc_exchange_particles(const orc_params *p, orc_particles *s);
+/* Counting sort by cell, mem_prtl.F90:166-301 */
+int  orc_reorder(const orc_params *p, orc_particles *s);
+/* One MAIN.F90 time step (J reset moved to the step start). */
+int  orc_step(const orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, int t);
+int  orc_run(const orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, int t0, int nsteps);
+
+/* Diagnostics, savedata_routines.F90:265-355 */
+void orc_energy(const orc_params *p, const orc_fields *f, const orc_particles *s, double out[4]);
+void orc_div_e(const orc_params *p, const orc_fields *f, double *div);
+/* CIC charge density at the nodes, periodically folded onto physical nodes. */
+void orc_rho(const orc_params *p, const orc_particles *s, double *rho);
+/* max |div E - rho| and max |rho| over physical nodes. */
+void orc_gauss(const orc_params *p, const orc_fields *f, const orc_particles *s, double out[2]);
+void orc_momentum(const orc_params *p, const orc_particles *s, double out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,232 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, where it is the checker (or the timed CPU
+baseline), never the product path.
+
+Arrays follow the reference layout: a field is a numpy array of shape
+(mx, my, mz) in Fortran order, so F(i,j,k) of the Fortran code is
+F[i-1, j-1, k-1] here; particles are 1-D arrays q,x,y,z,u,v,w (float64)
+and flv (int32) of length `cap`, the first `used` entries live.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "liboracle.so")
+
+c_dp = ctypes.POINTER(ctypes.c_double)
+c_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = (
+        [(n, ctypes.c_int32) for n in ("ndim", "nx", "ny", "nz", "mx", "my", "mz", "epc")]
+        + [(n, ctypes.c_double) for n in (
+            "c", "c_ompe", "g0", "mi_me", "me", "mi", "ompe", "qi", "qe", "qmi", "qme", "masse", "massi",
+            "fldc", "fld_halfc", "cinv", "sqc", "inv_dx", "inv_dy", "inv_dz",
+            "xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "xlen", "ylen", "zlen",
+            "Bx_ext0", "By_ext0", "Bz_ext0")]
+        + [("nflvr", ctypes.c_int32), ("reorder_period", ctypes.c_int32), ("flvrqm", ctypes.c_double * 8)]
+    )
+
+
+class OrcFields(ctypes.Structure):
+    _fields_ = [(n, c_dp) for n in ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz")]
+
+
+class OrcParticles(ctypes.Structure):
+    _fields_ = ([("cap", ctypes.c_int64), ("used", ctypes.c_int64)]
+                + [(n, c_dp) for n in ("q", "x", "y", "z", "u", "v", "w")]
+                + [("flv", c_ip)])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIBPATH):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = ctypes.CDLL(_LIBPATH)
+        P = ctypes.POINTER(OrcParams)
+        F = ctypes.POINTER(OrcFields)
+        S = ctypes.POINTER(OrcParticles)
+        W = ctypes.c_void_p
+        L.orc_params_init.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_ws_create.argtypes = [P, ctypes.c_int]
+        L.orc_ws_create.restype = W
+        L.orc_ws_destroy.argtypes = [W]
+        for name in ("orc_update_b_half", "orc_update_e", "orc_add_current", "orc_reset_current"):
+            getattr(L, name).argtypes = [P, F]
+        L.orc_field_halo.argtypes = [P, c_dp, c_dp, c_dp]
+        L.orc_current_halo_add.argtypes = [P, c_dp, c_dp, c_dp]
+        L.orc_gather_vec_e.argtypes = [P, W, F]
+        L.orc_gather_vec_b.argtypes = [P, W, F]
+        L.orc_move_deposit.argtypes = [P, W, F, S]
+        L.orc_ws_vec.argtypes = [W, ctypes.c_int]
+        L.orc_ws_vec.restype = c_dp
+        L.orc_ws_vec_len.argtypes = [W]
+        L.orc_exchange_particles.argtypes = [P, S]
+        L.orc_reorder.argtypes = [P, S]
+        L.orc_step.argtypes = [P, W, F, S, ctypes.c_int]
+        L.orc_run.argtypes = [P, W, F, S, ctypes.c_int, ctypes.c_int]
+        L.orc_energy.argtypes = [P, F, S, c_dp]
+        L.orc_div_e.argtypes = [P, F, c_dp]
+        L.orc_rho.argtypes = [P, S, c_dp]
+        L.orc_gauss.argtypes = [P, F, S, c_dp]
+        L.orc_momentum.argtypes = [P, S, c_dp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t=c_dp):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    """FP64 reference CPU path on one periodic rank (restated PICTOR)."""
+
+    FIELD_NAMES = ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz")
+
+    def __init__(self, ndim=2, nx=16, ny=16, nz=1, c=0.45, c_ompe=10.0, g0=1.0, mi_me=1.0, epc=16,
+                 nthreads=1, cap=1024, reorder_period=20):
+        L = lib()
+        self.p = OrcParams()
+        rc = L.orc_params_init(ctypes.byref(self.p), ndim, nx, ny, nz, c, c_ompe, g0, mi_me, epc)
+        if rc:
+            raise ValueError("invalid oracle configuration")
+        self.p.reorder_period = reorder_period
+        self.shape = (self.p.mx, self.p.my, self.p.mz)
+        self.F = {n: np.zeros(self.shape, order="F") for n in self.FIELD_NAMES}
+        self.f = OrcFields(*[_ptr(self.F[n]) for n in self.FIELD_NAMES])
+        self._alloc_particles(cap)
+        self.ws = L.orc_ws_create(ctypes.byref(self.p), nthreads)
+        if not self.ws:
+            raise MemoryError("oracle workspace allocation failed")
+
+    def __del__(self):
+        try:
+            if getattr(self, "ws", None):
+                lib().orc_ws_destroy(self.ws)
+                self.ws = None
+        except Exception:
+            pass
+
+    # ---- particles ----
+    def _alloc_particles(self, cap):
+        self.P = {n: np.zeros(cap) for n in ("q", "x", "y", "z", "u", "v", "w")}
+        self.P["flv"] = np.zeros(cap, dtype=np.int32)
+        self.s = OrcParticles(cap, 0, *[_ptr(self.P[n]) for n in ("q", "x", "y", "z", "u", "v", "w")],
+                              _ptr(self.P["flv"], c_ip))
+
+    def load_particles(self, q, x, y, z, u, v, w, flv, cap=None):
+        n = len(x)
+        cap = max(cap or 0, int(n * 1.25) + 1024)
+        self._alloc_particles(cap)
+        for name, arr in zip(("q", "x", "y", "z", "u", "v", "w"), (q, x, y, z, u, v, w)):
+            self.P[name][:n] = arr
+        self.P["flv"][:n] = flv
+        self.s.used = n
+
+    def particles(self, live_only=True):
+        n = self.s.used
+        out = {k: v[:n].copy() for k, v in self.P.items()}
+        if live_only:
+            keep = out["flv"] != 0
+            out = {k: v[keep] for k, v in out.items()}
+        return out
+
+    @property
+    def used(self):
+        return self.s.used
+
+    # ---- fields ----
+    def set_fields(self, **arrays):
+        for k, v in arrays.items():
+            self.F[k][...] = v
+
+    # ---- phases of MAIN.F90 ----
+    def update_b_half(self):
+        lib().orc_update_b_half(ctypes.byref(self.p), ctypes.byref(self.f))
+
+    def update_e(self):
+        lib().orc_update_e(ctypes.byref(self.p), ctypes.byref(self.f))
+
+    def add_current(self):
+        lib().orc_add_current(ctypes.byref(self.p), ctypes.byref(self.f))
+
+    def reset_current(self):
+        lib().orc_reset_current(ctypes.byref(self.p), ctypes.byref(self.f))
+
+    def field_halo(self, which):
+        names = ("Ex", "Ey", "Ez") if which in ("E", 0) else ("Bx", "By", "Bz")
+        lib().orc_field_halo(ctypes.byref(self.p), *[_ptr(self.F[n]) for n in names])
+
+    def current_halo_add(self):
+        lib().orc_current_halo_add(ctypes.byref(self.p), *[_ptr(self.F[n]) for n in ("Jx", "Jy", "Jz")])
+
+    def move_deposit(self):
+        lib().orc_move_deposit(ctypes.byref(self.p), self.ws, ctypes.byref(self.f), ctypes.byref(self.s))
+
+    def gather_vec(self):
+        L = lib()
+        L.orc_gather_vec_e(ctypes.byref(self.p), self.ws, ctypes.byref(self.f))
+        L.orc_gather_vec_b(ctypes.byref(self.p), self.ws, ctypes.byref(self.f))
+        nv = L.orc_ws_vec_len(self.ws)
+        ncell = self.p.mx * self.p.my * self.p.mz
+        out = []
+        for c in range(6):
+            ptr = L.orc_ws_vec(self.ws, c)
+            out.append(np.ctypeslib.as_array(ptr, shape=(ncell * nv,)).reshape(ncell, nv).copy())
+        return out
+
+    def exchange(self):
+        rc = lib().orc_exchange_particles(ctypes.byref(self.p), ctypes.byref(self.s))
+        if rc:
+            raise RuntimeError(f"oracle exchange failed ({rc})")
+
+    def reorder(self):
+        rc = lib().orc_reorder(ctypes.byref(self.p), ctypes.byref(self.s))
+        if rc:
+            raise RuntimeError(f"oracle reorder failed ({rc})")
+
+    def step(self, t):
+        rc = lib().orc_step(ctypes.byref(self.p), self.ws, ctypes.byref(self.f), ctypes.byref(self.s), t)
+        if rc:
+            raise RuntimeError(f"oracle step failed ({rc})")
+
+    def run(self, t0, nsteps):
+        rc = lib().orc_run(ctypes.byref(self.p), self.ws, ctypes.byref(self.f), ctypes.byref(self.s), t0, nsteps)
+        if rc:
+            raise RuntimeError(f"oracle run failed ({rc})")
+
+    # ---- diagnostics ----
+    def energy(self):
+        out = np.zeros(4)
+        lib().orc_energy(ctypes.byref(self.p), ctypes.byref(self.f), ctypes.byref(self.s), _ptr(out))
+        return out
+
+    def gauss(self):
+        out = np.zeros(2)
+        lib().orc_gauss(ctypes.byref(self.p), ctypes.byref(self.f), ctypes.byref(self.s), _ptr(out))
+        return out
+
+    def momentum(self):
+        out = np.zeros(3)
+        lib().orc_momentum(ctypes.byref(self.p), ctypes.byref(self.s), _ptr(out))
+        return out
+
+    def rho(self):
+        out = np.zeros(self.shape, order="F")
+        lib().orc_rho(ctypes.byref(self.p), ctypes.byref(self.s), _ptr(out))
+        return out
+
+    def div_e(self):
+        out = np.zeros(self.shape, order="F")
+        lib().orc_div_e(ctypes.byref(self.p), ctypes.byref(self.f), _ptr(out))
+        return out

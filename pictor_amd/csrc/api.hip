@@ -1,0 +1,959 @@
+// api.hip -- context, storage layout and the extern "C" entry points.
+//
+// Host-side control of the particle step.  The reference's MAIN loop calls
+// one subroutine per phase (src/MAIN.F90:42-284); each pictor_* phase call
+// below enqueues the kernels of that phase on the context's stream.  The
+// fused pictor_step runs a whole time step with the same kernels.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pictor_hip.h"
+#include "pictor_common.h"
+
+using namespace pictor;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPTRY(expr)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail(PICTOR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+inline void dfree(void *p) { (void)hipFree(p); }
+
+template <typename T>
+hipError_t dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void **)p, n * sizeof(T));
+}
+
+}  // namespace
+
+struct pictor_ctx {
+    pictor_config cfg{};
+    GridP g{};
+    PhysP ph{};
+    TileP tp{};
+    int form = 0;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+
+    // fields: double (FP64 form) or float (split form); J double or int64 fixed point
+    void *E[3] = {nullptr, nullptr, nullptr};
+    void *B[3] = {nullptr, nullptr, nullptr};
+    void *J[3] = {nullptr, nullptr, nullptr};
+
+    // tile bins
+    std::vector<long long> h_base;
+    std::vector<int> h_cap;
+    long long *d_base = nullptr;
+    int *d_cap = nullptr;
+    int *d_cnt[2] = {nullptr, nullptr};
+    long long layout_cap = 0;   // particle slots the layout needs per buffer
+
+    // particle buffers: [0],[1] ping-pong, spill, staging for host transfers
+    PartF64 pf[2]{};
+    PartSplit ps[2]{};
+    long long buf_cap[2] = {0, 0};
+    int cur = 0;
+    bool pending_swap = false;
+    PartF64 spill_f{};
+    PartSplit spill_s{};
+    int spill_cap = 0;
+    int *d_spill_count = nullptr;
+    int *d_err = nullptr;
+    int *h_flags = nullptr;   // pinned: {spill count, error bits}
+
+    // diagnostics scratch
+    double *d_partial = nullptr;
+    int partial_len = 0;
+    double *d_rho = nullptr;
+
+    // timers: 0 push, 1 field kernels, 2 whole step
+    hipEvent_t ev[8] = {};
+    double t_ms[3] = {0, 0, 0};
+    long long t_n[3] = {0, 0, 0};
+
+    long long np = 0;
+};
+
+namespace {
+
+template <typename R>
+void field_ptrs(pictor_ctx *c, R *E[3], R *B[3])
+{
+    for (int i = 0; i < 3; i++) { E[i] = (R *)c->E[i]; B[i] = (R *)c->B[i]; }
+}
+
+void free_f64(PartF64 &p)
+{
+    dfree(p.q); dfree(p.x); dfree(p.y); dfree(p.z);
+    dfree(p.u); dfree(p.v); dfree(p.w); dfree(p.flv);
+    p = PartF64{};
+}
+
+void free_split(PartSplit &p)
+{
+    dfree(p.cell); dfree(p.ox); dfree(p.oy); dfree(p.oz);
+    dfree(p.u); dfree(p.v); dfree(p.w); dfree(p.sp);
+    p = PartSplit{};
+}
+
+hipError_t alloc_f64(PartF64 &p, long long n)
+{
+    hipError_t e;
+    if ((e = dalloc(&p.q, n)) != hipSuccess) return e;
+    if ((e = dalloc(&p.x, n)) != hipSuccess) return e;
+    if ((e = dalloc(&p.y, n)) != hipSuccess) return e;
+    if ((e = dalloc(&p.z, n)) != hipSuccess) return e;
+    if ((e = dalloc(&p.u, n)) != hipSuccess) return e;
+    if ((e = dalloc(&p.v, n)) != hipSuccess) return e;
+    if ((e = dalloc(&p.w, n)) != hipSuccess) return e;
+    return dalloc(&p.flv, n);
+}
+
+hipError_t alloc_split(PartSplit &p, long long n, bool with_sp)
+{
+    hipError_t e;
+    if ((e = dalloc(&p.cell, n)) != hipSuccess) return e;
+    if ((e = dalloc(&p.ox, n)) != hipSuccess) return e;
+    if ((e = dalloc(&p.oy, n)) != hipSuccess) return e;
+    if ((e = dalloc(&p.oz, n)) != hipSuccess) return e;
+    if ((e = dalloc(&p.u, n)) != hipSuccess) return e;
+    if ((e = dalloc(&p.v, n)) != hipSuccess) return e;
+    if ((e = dalloc(&p.w, n)) != hipSuccess) return e;
+    if (with_sp) return dalloc(&p.sp, n);
+    return hipSuccess;
+}
+
+int ensure_buffer(pictor_ctx *c, int which, long long need)
+{
+    if (c->buf_cap[which] >= need) return PICTOR_OK;
+    long long n = (long long)(need * 1.1) + 1024;
+    if (c->form == PICTOR_FORM_FP64) {
+        free_f64(c->pf[which]);
+        HIPTRY(alloc_f64(c->pf[which], n));
+    } else {
+        free_split(c->ps[which]);
+        HIPTRY(alloc_split(c->ps[which], n, false));
+    }
+    c->buf_cap[which] = n;
+    return PICTOR_OK;
+}
+
+int ensure_spill(pictor_ctx *c, int need)
+{
+    if (c->spill_cap >= need) return PICTOR_OK;
+    if (c->form == PICTOR_FORM_FP64) {
+        free_f64(c->spill_f);
+        HIPTRY(alloc_f64(c->spill_f, need));
+    } else {
+        free_split(c->spill_s);
+        HIPTRY(alloc_split(c->spill_s, need, true));
+    }
+    c->spill_cap = need;
+    return PICTOR_OK;
+}
+
+// Bin capacities from the particle count of every (tile, species) group.
+// A tile can lose or gain at most the particles within c cells of a face per
+// step; the side bins are sized for that flux with a margin, the centre bin
+// for the whole group times capacity_factor.
+void compute_layout(pictor_ctx *c, const std::vector<long long> &counts)
+{
+    const TileP &tp = c->tp;
+    const double cf = c->cfg.capacity_factor > 0 ? c->cfg.capacity_factor : 1.25;
+    const double frac = std::min(1.0, c->ph.c / tp.T);
+    c->h_base.assign((size_t)tp.ntiles * tp.NB, 0);
+    c->h_cap.assign((size_t)tp.ntiles * tp.NB, 0);
+    long long off = 0;
+    for (int t = 0; t < tp.ntiles; t++)
+        for (int s = 0; s < tp.nsp; s++) {
+            const long long n = counts[(size_t)t * tp.nsp + s];
+            for (int d = 0; d < tp.ND; d++) {
+                int nz = 0, r = d;
+                for (int a = 0; a < c->g.ndim; a++) { nz += (r % 3 != 1); r /= 3; }
+                double cap;
+                if (nz == 0) cap = std::ceil(n * cf) + 64;
+                else cap = std::ceil(n * std::pow(frac, nz) * 1.6 * std::pow(2.0, nz - 1)) + (nz == 1 ? 32 : 16);
+                long long ci = ((long long)cap + 15) / 16 * 16;
+                const size_t b = (size_t)t * tp.NB + s * tp.ND + d;
+                c->h_base[b] = off;
+                c->h_cap[b] = (int)std::min<long long>(ci, 0x7fffffff);
+                off += ci;
+            }
+        }
+    c->layout_cap = off;
+}
+
+int upload_layout(pictor_ctx *c)
+{
+    const size_t nb = (size_t)c->tp.ntiles * c->tp.NB;
+    HIPTRY(hipMemcpyAsync(c->d_base, c->h_base.data(), nb * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemcpyAsync(c->d_cap, c->h_cap.data(), nb * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    return PICTOR_OK;
+}
+
+// centre-bin counts from per-group totals
+int upload_center_counts(pictor_ctx *c, int which, const std::vector<long long> &counts)
+{
+    const TileP &tp = c->tp;
+    std::vector<int> cnt((size_t)tp.ntiles * tp.NB, 0);
+    const int centre = (tp.ND - 1) / 2;
+    for (int t = 0; t < tp.ntiles; t++)
+        for (int s = 0; s < tp.nsp; s++)
+            cnt[(size_t)t * tp.NB + s * tp.ND + centre] = (int)counts[(size_t)t * tp.nsp + s];
+    HIPTRY(hipMemcpyAsync(c->d_cnt[which], cnt.data(), cnt.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));   // cnt is a local host buffer
+    return PICTOR_OK;
+}
+
+std::vector<long long> group_offsets(const pictor_ctx *c, bool dense, const std::vector<long long> &counts)
+{
+    // dense: consecutive runs (download); otherwise the centre-bin bases of the layout
+    const TileP &tp = c->tp;
+    const int centre = (tp.ND - 1) / 2;
+    const size_t ng = (size_t)tp.ntiles * tp.nsp;
+    std::vector<long long> off(ng + 1, 0);
+    long long acc = 0;
+    for (size_t gi = 0; gi < ng; gi++) {
+        if (dense) { off[gi] = acc; acc += counts[gi]; }
+        else off[gi] = c->h_base[(gi / tp.nsp) * tp.NB + (gi % tp.nsp) * tp.ND + centre];
+    }
+    off[ng] = dense ? acc : c->layout_cap;
+    return off;
+}
+
+// per-group particle counts of buffer `which` (host copy of the bin counts)
+int group_counts(pictor_ctx *c, int which, std::vector<long long> &counts)
+{
+    const TileP &tp = c->tp;
+    std::vector<int> cnt((size_t)tp.ntiles * tp.NB);
+    HIPTRY(hipMemcpyAsync(cnt.data(), c->d_cnt[which], cnt.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    counts.assign((size_t)tp.ntiles * tp.nsp, 0);
+    for (int t = 0; t < tp.ntiles; t++)
+        for (int s = 0; s < tp.nsp; s++) {
+            long long a = 0;
+            for (int d = 0; d < tp.ND; d++) a += cnt[(size_t)t * tp.NB + s * tp.ND + d];
+            counts[(size_t)t * tp.nsp + s] = a;
+        }
+    return PICTOR_OK;
+}
+
+// Re-layout: buffer `cur` bins + `nspill` spilled particles -> new layout in the other buffer.
+int relayout(pictor_ctx *c, int nspill)
+{
+    const TileP &tp = c->tp;
+    const size_t ng = (size_t)tp.ntiles * tp.nsp;
+    std::vector<long long> counts;
+    int rc = group_counts(c, c->cur, counts);
+    if (rc) return rc;
+    int *d_gcount = nullptr, *d_grp = nullptr, *d_fill = nullptr;
+    long long *d_off = nullptr;
+    HIPTRY(dalloc(&d_gcount, ng));
+    HIPTRY(dalloc(&d_fill, ng));
+    HIPTRY(dalloc(&d_off, ng + 1));
+    HIPTRY(dalloc(&d_grp, std::max(nspill, 1)));
+    HIPTRY(hipMemsetAsync(d_gcount, 0, ng * sizeof(int), c->stream));
+    if (nspill > 0) {
+        if (c->form == PICTOR_FORM_FP64) launch_spill_tiles_f64(c->g, tp, c->spill_f, nspill, d_grp, d_gcount, c->stream);
+        else launch_spill_tiles_split(c->g, tp, c->spill_s, nspill, d_grp, d_gcount, c->stream);
+    }
+    std::vector<int> sc(ng);
+    HIPTRY(hipMemcpyAsync(sc.data(), d_gcount, ng * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    std::vector<long long> total(ng);
+    long long np = 0;
+    for (size_t gi = 0; gi < ng; gi++) { total[gi] = counts[gi] + sc[gi]; np += total[gi]; }
+    // old layout stays on the device for the compaction source
+    std::vector<long long> old_base = c->h_base;
+    compute_layout(c, total);
+    const int dst = 1 - c->cur;
+    rc = ensure_buffer(c, dst, c->layout_cap);
+    if (rc) return rc;
+    std::vector<long long> off = group_offsets(c, false, total);
+    HIPTRY(hipMemcpyAsync(d_off, off.data(), (ng + 1) * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    // fill starts after the compacted bins of each group
+    std::vector<int> fill(ng);
+    for (size_t gi = 0; gi < ng; gi++) fill[gi] = (int)counts[gi];
+    HIPTRY(hipMemcpyAsync(d_fill, fill.data(), ng * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (c->form == PICTOR_FORM_FP64) {
+        launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], d_off, c->pf[dst], c->stream);
+        launch_spill_scatter_f64(tp, c->spill_f, nspill, d_grp, d_off, d_fill, c->pf[dst], c->stream);
+    } else {
+        launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], d_off, c->ps[dst], c->stream);
+        launch_spill_scatter_split(tp, c->spill_s, nspill, d_grp, d_off, d_fill, c->ps[dst], c->stream);
+    }
+    HIPTRY(hipGetLastError());
+    rc = upload_layout(c);
+    if (rc) return rc;
+    rc = upload_center_counts(c, dst, total);
+    if (rc) return rc;
+    HIPTRY(hipMemsetAsync(c->d_spill_count, 0, sizeof(int), c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    dfree(d_gcount); dfree(d_fill); dfree(d_off); dfree(d_grp);
+    c->cur = dst;
+    // the other buffer must hold the new layout too
+    rc = ensure_buffer(c, 1 - dst, c->layout_cap);
+    if (rc) return rc;
+    rc = ensure_spill(c, (int)std::min<long long>(std::max<long long>(65536, np / 64), 1 << 30));
+    if (rc) return rc;
+    c->np = np;
+    return PICTOR_OK;
+}
+
+// spill / error check after a push: sync, and re-layout if particles spilled
+int commit_push(pictor_ctx *c)
+{
+    HIPTRY(hipMemcpyAsync(c->h_flags, c->d_spill_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipMemcpyAsync(c->h_flags + 1, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    c->cur = 1 - c->cur;
+    c->pending_swap = false;
+    const int nspill = c->h_flags[0], err = c->h_flags[1];
+    if (err & kErrBounds) return fail(PICTOR_ERR_BOUNDS, "a particle was outside its tile window or moved more than one tile");
+    if ((err & kErrCapacity) || nspill > c->spill_cap)
+        return fail(PICTOR_ERR_CAPACITY, "particle spill buffer exhausted");
+    if (nspill > 0) return relayout(c, nspill);
+    return PICTOR_OK;
+}
+
+int check_ctx(pictor_ctx *c)
+{
+    if (!c) return fail(PICTOR_ERR_ARG, "null context");
+    HIPTRY(hipSetDevice(c->dev));
+    return PICTOR_OK;
+}
+
+int do_b_half(pictor_ctx *c)
+{
+    if (c->form == PICTOR_FORM_FP64) {
+        double *E[3], *B[3];
+        field_ptrs(c, E, B);
+        launch_update_b_half<double>(c->g, c->ph, E, B, c->stream);
+        launch_ghost_copy<double>(c->g, B, c->stream);
+    } else {
+        float *E[3], *B[3];
+        field_ptrs(c, E, B);
+        launch_update_b_half<float>(c->g, c->ph, E, B, c->stream);
+        launch_ghost_copy<float>(c->g, B, c->stream);
+    }
+    HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
+int do_push(pictor_ctx *c, bool timed)
+{
+    if (c->pending_swap) return fail(PICTOR_ERR_ARG, "pictor_move_deposit called twice without pictor_exchange_particles");
+    HIPTRY(hipMemsetAsync(c->d_spill_count, 0, sizeof(int), c->stream));
+    const int a = c->cur, b = 1 - c->cur;
+    BinsP bins{c->d_base, c->d_cap, c->d_cnt[a], c->d_cnt[b]};
+    SpillP sp{c->d_spill_count, c->spill_cap, c->d_err};
+    if (timed) HIPTRY(hipEventRecord(c->ev[0], c->stream));
+    if (c->g.ndim != 2) return fail(PICTOR_ERR_ARG, "3-D push not available");
+    if (c->form == PICTOR_FORM_FP64) {
+        double *E[3], *B[3], *J[3];
+        field_ptrs(c, E, B);
+        for (int i = 0; i < 3; i++) J[i] = (double *)c->J[i];
+        launch_push2d_f64(c->tp.T, c->g, c->ph, c->tp, E, B, J, c->pf[a], c->pf[b], bins, c->spill_f, sp, c->stream);
+    } else {
+        float *E[3], *B[3];
+        long long *J[3];
+        field_ptrs(c, E, B);
+        for (int i = 0; i < 3; i++) J[i] = (long long *)c->J[i];
+        launch_push2d_split(c->tp.T, c->g, c->ph, c->tp, E, B, J, c->ps[a], c->ps[b], bins, c->spill_s, sp, c->stream);
+    }
+    HIPTRY(hipGetLastError());
+    if (timed) HIPTRY(hipEventRecord(c->ev[1], c->stream));
+    c->pending_swap = true;
+    return PICTOR_OK;
+}
+
+int do_reset_current(pictor_ctx *c)
+{
+    for (int i = 0; i < 3; i++) {
+        if (c->form == PICTOR_FORM_FP64) launch_fill_zero<double>((double *)c->J[i], c->g.ncell, c->stream);
+        else launch_fill_zero<long long>((long long *)c->J[i], c->g.ncell, c->stream);
+    }
+    HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
+int do_fold(pictor_ctx *c)
+{
+    if (c->form == PICTOR_FORM_FP64) {
+        double *J[3];
+        for (int i = 0; i < 3; i++) J[i] = (double *)c->J[i];
+        launch_current_fold<double>(c->g, J, c->stream);
+    } else {
+        long long *J[3];
+        for (int i = 0; i < 3; i++) J[i] = (long long *)c->J[i];
+        launch_current_fold<long long>(c->g, J, c->stream);
+    }
+    HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
+double jscale(const pictor_ctx *c) { return c->ph.qi * kFixInv; }
+
+int do_e_phase(pictor_ctx *c, bool curl, bool add)
+{
+    if (c->form == PICTOR_FORM_FP64) {
+        double *E[3], *B[3], *J[3];
+        field_ptrs(c, E, B);
+        for (int i = 0; i < 3; i++) J[i] = (double *)c->J[i];
+        if (curl && add) launch_update_e_add_current<double, double>(c->g, c->ph, E, B, J, 1.0, c->stream);
+        else if (curl) launch_update_e<double>(c->g, c->ph, E, B, c->stream);
+        else launch_add_current<double, double>(c->g, c->ph, E, J, 1.0, c->stream);
+        launch_ghost_copy<double>(c->g, E, c->stream);
+    } else {
+        float *E[3], *B[3];
+        long long *J[3];
+        field_ptrs(c, E, B);
+        for (int i = 0; i < 3; i++) J[i] = (long long *)c->J[i];
+        if (curl && add) launch_update_e_add_current<float, long long>(c->g, c->ph, E, B, J, jscale(c), c->stream);
+        else if (curl) launch_update_e<float>(c->g, c->ph, E, B, c->stream);
+        else launch_add_current<float, long long>(c->g, c->ph, E, J, jscale(c), c->stream);
+        launch_ghost_copy<float>(c->g, E, c->stream);
+    }
+    HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
+int choose_tile(const pictor_config *cfg)
+{
+    const int cands[3] = {32, 16, 8};
+    auto fits = [&](int T) {
+        if (cfg->nx % T || cfg->ny % T) return false;
+        if (cfg->ndim == 3 && cfg->nz % T) return false;
+        return true;
+    };
+    int best = 0;
+    for (int T : cands) {
+        if (!fits(T)) continue;
+        long long nt = (long long)(cfg->nx / T) * (cfg->ny / T) * (cfg->ndim == 3 ? cfg->nz / T : 1);
+        if (nt >= 1024) return T;
+        best = T;   // smallest fitting tile so far
+    }
+    return best;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pictor_last_error(void) { return g_err.c_str(); }
+
+int pictor_init(const pictor_config *cfg, pictor_ctx **out)
+{
+    if (!cfg || !out) return fail(PICTOR_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->ndim != 2) return fail(PICTOR_ERR_ARG, "only ndim = 2 is available");
+    if (cfg->nx < 8 || cfg->ny < 8) return fail(PICTOR_ERR_ARG, "nx, ny must be >= 8");
+    if (cfg->form != PICTOR_FORM_FP64 && cfg->form != PICTOR_FORM_FP32_SPLIT) return fail(PICTOR_ERR_ARG, "unknown form");
+    if (!(cfg->c > 0.0 && cfg->c < 1.0)) return fail(PICTOR_ERR_ARG, "Courant number c must lie in (0,1)");
+    if (cfg->epc < 1 || cfg->c_ompe <= 0 || cfg->mi_me <= 0) return fail(PICTOR_ERR_ARG, "invalid epc, c_ompe or mi_me");
+    int T = cfg->tile ? cfg->tile : choose_tile(cfg);
+    if (T != 8 && T != 16 && T != 32) return fail(PICTOR_ERR_ARG, "tile must be 8, 16 or 32 and divide nx, ny");
+    if (cfg->nx % T || cfg->ny % T) return fail(PICTOR_ERR_ARG, "tile must divide nx and ny");
+
+    pictor_ctx *c = new pictor_ctx();
+    c->cfg = *cfg;
+    c->cfg.tile = T;
+    c->form = cfg->form;
+    if (hipGetDevice(&c->dev) != hipSuccess) { delete c; return fail(PICTOR_ERR_HIP, "no HIP device"); }
+
+    GridP &g = c->g;
+    g.ndim = cfg->ndim;
+    g.nx = cfg->nx; g.ny = cfg->ny; g.nz = (cfg->ndim == 2) ? 1 : cfg->nz;
+    g.mx = g.nx + 5; g.my = g.ny + 5; g.mz = (cfg->ndim == 2) ? 1 : g.nz + 5;
+    g.ncell = (long long)g.mx * g.my * g.mz;
+
+    // InitScale, initialise.F90:44-53; vars.F90:220-223; subdomains.F90:52-67
+    PhysP &ph = c->ph;
+    const double me = 1.0, mi = cfg->mi_me;
+    const double ompe = cfg->c / cfg->c_ompe;
+    ph.c = cfg->c;
+    ph.sqc = cfg->c * cfg->c;
+    ph.cinv = 1.0 / cfg->c;
+    ph.qi = (cfg->g0 * (ompe * ompe)) / (((double)cfg->epc / 1.0) * (1.0 + me / mi));
+    ph.fldc = cfg->c * 1.0;
+    ph.fld_halfc = cfg->c * 0.5 * 1.0;
+    ph.inv_dx = ph.inv_dy = ph.inv_dz = 1.0;
+    ph.xmin = 3.0; ph.xmax = g.mx - 2;
+    ph.ymin = 3.0; ph.ymax = g.my - 2;
+    if (g.ndim == 2) { ph.zmin = 1.0; ph.zmax = 2.0; }
+    else { ph.zmin = 3.0; ph.zmax = g.mz - 2; }
+    ph.xlen = ph.xmax - ph.xmin; ph.ylen = ph.ymax - ph.ymin; ph.zlen = ph.zmax - ph.zmin;
+    ph.bext[0] = ph.bext[1] = ph.bext[2] = 0.0;
+    ph.nflvr = 2;
+    ph.flvrqm[0] = 1.0 / mi;   // qmi
+    ph.flvrqm[1] = -1.0;       // qme
+    ph.flvq[0] = 1.0;          // FlvrCharge, initialise.F90:86-87 (unit weight)
+    ph.flvq[1] = -1.0;
+    ph.mass[0] = ph.qi * mi;                 // massi
+    ph.mass[1] = std::fabs(-ph.qi) * me;     // masse
+
+    TileP &tp = c->tp;
+    tp.T = T;
+    tp.ntx = g.nx / T; tp.nty = g.ny / T; tp.ntz = (g.ndim == 3) ? g.nz / T : 1;
+    tp.ntiles = tp.ntx * tp.nty * tp.ntz;
+    tp.ND = (g.ndim == 2) ? 9 : 27;
+    tp.nsp = (c->form == PICTOR_FORM_FP64) ? 1 : ph.nflvr;
+    tp.NB = tp.nsp * tp.ND;
+    tp.xcd_remap = (tp.ntiles % 8 == 0) ? 1 : 0;
+
+    auto cleanup_fail = [&](int rc) { pictor_finalize(c); return rc; };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup_fail(fail(PICTOR_ERR_HIP, "hipStreamCreate failed"));
+    const size_t rs = (c->form == PICTOR_FORM_FP64) ? sizeof(double) : sizeof(float);
+    for (int i = 0; i < 3; i++) {
+        if (hipMalloc(&c->E[i], g.ncell * rs) != hipSuccess || hipMalloc(&c->B[i], g.ncell * rs) != hipSuccess ||
+            hipMalloc(&c->J[i], g.ncell * 8) != hipSuccess)
+            return cleanup_fail(fail(PICTOR_ERR_HIP, "field allocation failed"));
+        hipMemsetAsync(c->E[i], 0, g.ncell * rs, c->stream);
+        hipMemsetAsync(c->B[i], 0, g.ncell * rs, c->stream);
+        hipMemsetAsync(c->J[i], 0, g.ncell * 8, c->stream);
+    }
+    const size_t nb = (size_t)tp.ntiles * tp.NB;
+    if (dalloc(&c->d_base, nb) || dalloc(&c->d_cap, nb) || dalloc(&c->d_cnt[0], nb) || dalloc(&c->d_cnt[1], nb) ||
+        dalloc(&c->d_spill_count, 1) || dalloc(&c->d_err, 1))
+        return cleanup_fail(fail(PICTOR_ERR_HIP, "bin allocation failed"));
+    hipMemsetAsync(c->d_cnt[0], 0, nb * sizeof(int), c->stream);
+    hipMemsetAsync(c->d_cnt[1], 0, nb * sizeof(int), c->stream);
+    hipMemsetAsync(c->d_spill_count, 0, sizeof(int), c->stream);
+    hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream);
+    if (hipHostMalloc((void **)&c->h_flags, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+        return cleanup_fail(fail(PICTOR_ERR_HIP, "pinned allocation failed"));
+    for (int i = 0; i < 8; i++)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) return cleanup_fail(fail(PICTOR_ERR_HIP, "event creation failed"));
+    // empty layout so that a step without particles is valid
+    std::vector<long long> zero((size_t)tp.ntiles * tp.nsp, 0);
+    compute_layout(c, zero);
+    int rc = upload_layout(c);
+    if (!rc) rc = ensure_buffer(c, 0, c->layout_cap);
+    if (!rc) rc = ensure_buffer(c, 1, c->layout_cap);
+    if (!rc) rc = ensure_spill(c, 65536);
+    c->partial_len = std::max(1024, tp.ntiles) * 5;
+    if (!rc && (dalloc(&c->d_partial, c->partial_len) != hipSuccess || dalloc(&c->d_rho, g.ncell) != hipSuccess))
+        rc = fail(PICTOR_ERR_HIP, "diagnostic allocation failed");
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(PICTOR_ERR_HIP, "init synchronize failed");
+    if (rc) return cleanup_fail(rc);
+    *out = c;
+    return PICTOR_OK;
+}
+
+int pictor_finalize(pictor_ctx *c)
+{
+    if (!c) return PICTOR_OK;
+    hipSetDevice(c->dev);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 3; i++) { dfree(c->E[i]); dfree(c->B[i]); dfree(c->J[i]); }
+    dfree(c->d_base); dfree(c->d_cap); dfree(c->d_cnt[0]); dfree(c->d_cnt[1]);
+    dfree(c->d_spill_count); dfree(c->d_err);
+    for (int i = 0; i < 2; i++) { free_f64(c->pf[i]); free_split(c->ps[i]); }
+    free_f64(c->spill_f); free_split(c->spill_s);
+    dfree(c->d_partial); dfree(c->d_rho);
+    if (c->h_flags) hipHostFree(c->h_flags);
+    for (int i = 0; i < 8; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return PICTOR_OK;
+}
+
+int pictor_get_scalars(pictor_ctx *c, double out[16])
+{
+    if (!c || !out) return fail(PICTOR_ERR_ARG, "null argument");
+    out[0] = c->ph.qi; out[1] = -c->ph.qi; out[2] = c->ph.mass[1]; out[3] = c->ph.mass[0];
+    out[4] = c->cfg.c / c->cfg.c_ompe; out[5] = c->ph.fldc;
+    out[6] = c->ph.xlen; out[7] = c->ph.ylen; out[8] = c->ph.zlen;
+    out[9] = c->g.mx; out[10] = c->g.my; out[11] = c->g.mz;
+    out[12] = c->tp.T; out[13] = c->tp.ntiles; out[14] = 0; out[15] = 0;
+    return PICTOR_OK;
+}
+
+int pictor_upload_fields(pictor_ctx *c, const double *ex, const double *ey, const double *ez, const double *bx,
+                         const double *by, const double *bz)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    const double *src[6] = {ex, ey, ez, bx, by, bz};
+    void *dst[6] = {c->E[0], c->E[1], c->E[2], c->B[0], c->B[1], c->B[2]};
+    const size_t n = (size_t)c->g.ncell;
+    double *tmp = nullptr;
+    if (c->form != PICTOR_FORM_FP64) HIPTRY(dalloc(&tmp, n));
+    for (int i = 0; i < 6; i++) {
+        if (!src[i]) return fail(PICTOR_ERR_ARG, "null field pointer");
+        if (c->form == PICTOR_FORM_FP64) {
+            HIPTRY(hipMemcpyAsync(dst[i], src[i], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        } else {
+            HIPTRY(hipMemcpyAsync(tmp, src[i], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            launch_convert_fields_f32(tmp, (float *)dst[i], (long long)n, c->stream);
+        }
+    }
+    HIPTRY(hipStreamSynchronize(c->stream));
+    if (tmp) dfree(tmp);
+    return PICTOR_OK;
+}
+
+int pictor_download_fields(pictor_ctx *c, double *ex, double *ey, double *ez, double *bx, double *by, double *bz,
+                           double *jx, double *jy, double *jz)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    double *dst[9] = {ex, ey, ez, bx, by, bz, jx, jy, jz};
+    void *src[9] = {c->E[0], c->E[1], c->E[2], c->B[0], c->B[1], c->B[2], c->J[0], c->J[1], c->J[2]};
+    const size_t n = (size_t)c->g.ncell;
+    double *tmp = nullptr;
+    if (c->form != PICTOR_FORM_FP64) HIPTRY(dalloc(&tmp, n));
+    for (int i = 0; i < 9; i++) {
+        if (!dst[i]) continue;
+        if (c->form == PICTOR_FORM_FP64) {
+            HIPTRY(hipMemcpyAsync(dst[i], src[i], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        } else {
+            if (i < 6) launch_convert_fields_f64((const float *)src[i], tmp, (long long)n, c->stream);
+            else launch_convert_jfix((const long long *)src[i], tmp, (long long)n, jscale(c), c->stream);
+            HIPTRY(hipMemcpyAsync(dst[i], tmp, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPTRY(hipStreamSynchronize(c->stream));
+        }
+    }
+    HIPTRY(hipStreamSynchronize(c->stream));
+    if (tmp) dfree(tmp);
+    return PICTOR_OK;
+}
+
+int pictor_upload_particles(pictor_ctx *c, int64_t n, const double *q, const double *x, const double *y,
+                            const double *z, const double *u, const double *v, const double *w, const int32_t *flv)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (n < 0) return fail(PICTOR_ERR_ARG, "negative particle count");
+    if (n > 0 && (!q || !x || !y || !z || !u || !v || !w || !flv)) return fail(PICTOR_ERR_ARG, "null particle pointer");
+    if (n >= 0x7fffffffLL) return fail(PICTOR_ERR_ARG, "more than 2^31-1 particles per device");
+    c->pending_swap = false;
+    const TileP &tp = c->tp;
+    const size_t ng = (size_t)tp.ntiles * tp.nsp;
+    PartF64 raw{};
+    int *d_tile = nullptr, *d_count = nullptr, *d_fill = nullptr;
+    long long *d_off = nullptr;
+    HIPTRY(alloc_f64(raw, n));
+    HIPTRY(dalloc(&d_tile, n));
+    HIPTRY(dalloc(&d_count, ng));
+    HIPTRY(dalloc(&d_fill, ng));
+    HIPTRY(dalloc(&d_off, ng + 1));
+    const double *hs[7] = {q, x, y, z, u, v, w};
+    double *ds[7] = {raw.q, raw.x, raw.y, raw.z, raw.u, raw.v, raw.w};
+    for (int i = 0; i < 7 && n > 0; i++)
+        HIPTRY(hipMemcpyAsync(ds[i], hs[i], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (n > 0) HIPTRY(hipMemcpyAsync(raw.flv, flv, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemsetAsync(d_count, 0, ng * sizeof(int), c->stream));
+    HIPTRY(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+    launch_raw_tiles_f64(c->g, c->ph, tp, raw, n, c->form == PICTOR_FORM_FP32_SPLIT ? 1 : 0, d_tile, d_count,
+                         c->d_err, c->stream);
+    HIPTRY(hipGetLastError());
+    std::vector<int> cnt(ng);
+    HIPTRY(hipMemcpyAsync(cnt.data(), d_count, ng * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipMemcpyAsync(c->h_flags + 1, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    if (c->h_flags[1]) {
+        hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream);
+        free_f64(raw); dfree(d_tile); dfree(d_count); dfree(d_fill); dfree(d_off);
+        return fail(PICTOR_ERR_ARG, "particle outside the domain (x,y in [2, m-1)), invalid flavour, or (split form) a charge differing from its flavour's");
+    }
+    std::vector<long long> counts(ng);
+    long long np = 0;
+    for (size_t i = 0; i < ng; i++) { counts[i] = cnt[i]; np += cnt[i]; }
+    compute_layout(c, counts);
+    rc = ensure_buffer(c, 0, c->layout_cap);
+    if (!rc) rc = ensure_buffer(c, 1, c->layout_cap);
+    if (!rc) rc = ensure_spill(c, (int)std::min<long long>(std::max<long long>(65536, np / 64), 1 << 30));
+    if (rc) return rc;
+    std::vector<long long> off = group_offsets(c, false, counts);
+    HIPTRY(hipMemcpyAsync(d_off, off.data(), (ng + 1) * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemsetAsync(d_fill, 0, ng * sizeof(int), c->stream));
+    c->cur = 0;
+    if (c->form == PICTOR_FORM_FP64)
+        launch_raw_scatter_f64(c->g, tp, raw, n, d_tile, d_off, d_fill, c->pf[0], c->stream);
+    else
+        launch_raw_scatter_split(c->g, c->ph, tp, raw, n, d_tile, d_off, d_fill, c->ps[0], c->stream);
+    HIPTRY(hipGetLastError());
+    rc = upload_layout(c);
+    if (!rc) rc = upload_center_counts(c, 0, counts);
+    if (rc) return rc;
+    HIPTRY(hipStreamSynchronize(c->stream));
+    free_f64(raw);
+    dfree(d_tile); dfree(d_count); dfree(d_fill); dfree(d_off);
+    c->np = np;
+    return PICTOR_OK;
+}
+
+int pictor_particle_count(pictor_ctx *c, int64_t *n)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::vector<long long> counts;
+    rc = group_counts(c, c->cur, counts);
+    if (rc) return rc;
+    long long a = 0;
+    for (long long v : counts) a += v;
+    *n = a;
+    return PICTOR_OK;
+}
+
+int pictor_download_particles(pictor_ctx *c, int64_t cap, double *q, double *x, double *y, double *z, double *u,
+                              double *v, double *w, int32_t *flv, int32_t *tile, int64_t *n_out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->pending_swap) return fail(PICTOR_ERR_ARG, "download between move_deposit and exchange_particles");
+    const TileP &tp = c->tp;
+    const size_t ng = (size_t)tp.ntiles * tp.nsp;
+    std::vector<long long> counts;
+    rc = group_counts(c, c->cur, counts);
+    if (rc) return rc;
+    std::vector<long long> off = group_offsets(c, true, counts);
+    const long long n = off[ng];
+    if (n_out) *n_out = n;
+    if (n > cap) return fail(PICTOR_ERR_ARG, "output arrays too small");
+    if (n == 0) return PICTOR_OK;
+    long long *d_off = nullptr;
+    int *d_tile = nullptr;
+    PartF64 st{};
+    PartSplit ss{};
+    HIPTRY(dalloc(&d_off, ng + 1));
+    HIPTRY(alloc_f64(st, n));
+    HIPTRY(hipMemcpyAsync(d_off, off.data(), (ng + 1) * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    if (c->form == PICTOR_FORM_FP64) {
+        launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], d_off, st, c->stream);
+    } else {
+        HIPTRY(alloc_split(ss, n, true));
+        launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], d_off, ss, c->stream);
+        launch_split_to_f64(c->g, c->ph, ss, n, st, c->stream);
+    }
+    if (tile) {
+        HIPTRY(dalloc(&d_tile, n));
+        launch_tile_ids(tp, d_off, n, d_tile, c->stream);
+    }
+    HIPTRY(hipGetLastError());
+    double *hd[7] = {q, x, y, z, u, v, w};
+    double *dd[7] = {st.q, st.x, st.y, st.z, st.u, st.v, st.w};
+    for (int i = 0; i < 7; i++)
+        if (hd[i]) HIPTRY(hipMemcpyAsync(hd[i], dd[i], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (flv) HIPTRY(hipMemcpyAsync(flv, st.flv, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (tile) HIPTRY(hipMemcpyAsync(tile, d_tile, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    dfree(d_off); dfree(d_tile);
+    free_f64(st); free_split(ss);
+    return PICTOR_OK;
+}
+
+int pictor_update_b_half(pictor_ctx *c)
+{
+    int rc = check_ctx(c);
+    return rc ? rc : do_b_half(c);
+}
+
+int pictor_move_deposit(pictor_ctx *c)
+{
+    int rc = check_ctx(c);
+    return rc ? rc : do_push(c, false);
+}
+
+int pictor_exchange_particles(pictor_ctx *c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->pending_swap) return fail(PICTOR_ERR_ARG, "pictor_exchange_particles without a preceding pictor_move_deposit");
+    return commit_push(c);
+}
+
+int pictor_update_e(pictor_ctx *c)
+{
+    int rc = check_ctx(c);
+    return rc ? rc : do_e_phase(c, true, false);
+}
+
+int pictor_current_halo_add(pictor_ctx *c)
+{
+    int rc = check_ctx(c);
+    return rc ? rc : do_fold(c);
+}
+
+int pictor_add_current(pictor_ctx *c)
+{
+    int rc = check_ctx(c);
+    return rc ? rc : do_e_phase(c, false, true);
+}
+
+int pictor_field_halo(pictor_ctx *c, int32_t which)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (which != 0 && which != 1) return fail(PICTOR_ERR_ARG, "which must be 0 (E) or 1 (B)");
+    void **F = which == 0 ? c->E : c->B;
+    if (c->form == PICTOR_FORM_FP64) {
+        double *P[3] = {(double *)F[0], (double *)F[1], (double *)F[2]};
+        launch_ghost_copy<double>(c->g, P, c->stream);
+    } else {
+        float *P[3] = {(float *)F[0], (float *)F[1], (float *)F[2]};
+        launch_ghost_copy<float>(c->g, P, c->stream);
+    }
+    HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
+int pictor_reset_current(pictor_ctx *c)
+{
+    int rc = check_ctx(c);
+    return rc ? rc : do_reset_current(c);
+}
+
+int pictor_sort_tiles(pictor_ctx *c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->pending_swap) return fail(PICTOR_ERR_ARG, "sort between move_deposit and exchange_particles");
+    return relayout(c, 0);
+}
+
+int pictor_step(pictor_ctx *c, int32_t nsteps)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->pending_swap) return fail(PICTOR_ERR_ARG, "pictor_step inside a split step");
+    for (int s = 0; s < nsteps; s++) {
+        HIPTRY(hipEventRecord(c->ev[4], c->stream));
+        if ((rc = do_reset_current(c))) return rc;
+        HIPTRY(hipEventRecord(c->ev[2], c->stream));
+        if ((rc = do_b_half(c))) return rc;
+        if ((rc = do_push(c, true))) return rc;
+        if ((rc = do_b_half(c))) return rc;
+        if ((rc = do_fold(c))) return rc;
+        if ((rc = do_e_phase(c, true, true))) return rc;
+        HIPTRY(hipEventRecord(c->ev[3], c->stream));
+        HIPTRY(hipEventRecord(c->ev[5], c->stream));
+        rc = commit_push(c);
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) { c->t_ms[0] += ms; c->t_n[0]++; }
+        float f1 = 0, f2 = 0;
+        if (hipEventElapsedTime(&f1, c->ev[2], c->ev[3]) == hipSuccess) { c->t_ms[1] += f1 - ms; c->t_n[1]++; }
+        if (hipEventElapsedTime(&f2, c->ev[4], c->ev[5]) == hipSuccess) { c->t_ms[2] += f2; c->t_n[2]++; }
+        (void)f1;
+        if (rc) return rc;
+    }
+    return PICTOR_OK;
+}
+
+int pictor_synchronize(pictor_ctx *c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    HIPTRY(hipStreamSynchronize(c->stream));
+    return PICTOR_OK;
+}
+
+int pictor_diagnostics(pictor_ctx *c, double out[9])
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->pending_swap) return fail(PICTOR_ERR_ARG, "diagnostics between move_deposit and exchange_particles");
+    const int nblk = 1024;
+    std::vector<double> part((size_t)std::max(nblk * 2, c->tp.ntiles * 5));
+    // field energies
+    if (c->form == PICTOR_FORM_FP64) {
+        double *E[3], *B[3];
+        field_ptrs(c, E, B);
+        launch_field_energy<double>(c->g, E, B, c->d_partial, nblk, c->stream);
+    } else {
+        float *E[3], *B[3];
+        field_ptrs(c, E, B);
+        launch_field_energy<float>(c->g, E, B, c->d_partial, nblk, c->stream);
+    }
+    HIPTRY(hipMemcpyAsync(part.data(), c->d_partial, nblk * 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    double be = 0, ee = 0;
+    for (int b = 0; b < nblk; b++) { be += part[2 * b]; ee += part[2 * b + 1]; }
+    out[0] = be * 0.5;
+    out[1] = ee * 0.5;
+    // particle energies, momentum, charge density
+    launch_fill_zero<double>(c->d_rho, c->g.ncell, c->stream);
+    if (c->form == PICTOR_FORM_FP64)
+        launch_particle_diag_f64(c->g, c->ph, c->tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], c->d_partial, c->d_rho,
+                                 c->stream);
+    else
+        launch_particle_diag_split(c->g, c->ph, c->tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], c->d_partial,
+                                   c->d_rho, c->stream);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(part.data(), c->d_partial, (size_t)c->tp.ntiles * 5 * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int t = 0; t < c->tp.ntiles; t++)
+        for (int a = 0; a < 5; a++) acc[a] += part[(size_t)t * 5 + a];
+    for (int a = 0; a < 5; a++) out[2 + a] = acc[a];
+    // Gauss residual
+    if (c->form == PICTOR_FORM_FP64) {
+        double *E[3], *B[3];
+        field_ptrs(c, E, B);
+        launch_gauss<double>(c->g, E, c->d_rho, c->d_partial, nblk, c->stream);
+    } else {
+        float *E[3], *B[3];
+        field_ptrs(c, E, B);
+        launch_gauss<float>(c->g, E, c->d_rho, c->d_partial, nblk, c->stream);
+    }
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(part.data(), c->d_partial, nblk * 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    double rm = 0, am = 0;
+    for (int b = 0; b < nblk; b++) { rm = std::max(rm, part[2 * b]); am = std::max(am, part[2 * b + 1]); }
+    out[7] = rm;
+    out[8] = am;
+    return PICTOR_OK;
+}
+
+int pictor_timer_reset(pictor_ctx *c)
+{
+    if (!c) return fail(PICTOR_ERR_ARG, "null context");
+    for (int i = 0; i < 3; i++) { c->t_ms[i] = 0; c->t_n[i] = 0; }
+    return PICTOR_OK;
+}
+
+int pictor_timer_get(pictor_ctx *c, int32_t id, double *total_ms, int64_t *count)
+{
+    if (!c || id < 0 || id > 2) return fail(PICTOR_ERR_ARG, "bad timer id");
+    if (total_ms) *total_ms = c->t_ms[id];
+    if (count) *count = c->t_n[id];
+    return PICTOR_OK;
+}
+
+int pictor_memory_bytes(pictor_ctx *c, int64_t *bytes)
+{
+    if (!c || !bytes) return fail(PICTOR_ERR_ARG, "null argument");
+    const long long rs = (c->form == PICTOR_FORM_FP64) ? 8 : 4;
+    long long b = c->g.ncell * (6 * rs + 3 * 8 + 8);
+    const long long per = (c->form == PICTOR_FORM_FP64) ? 60 : 28;
+    b += (c->buf_cap[0] + c->buf_cap[1]) * per + (long long)c->spill_cap * (per + 4);
+    b += (long long)c->tp.ntiles * c->tp.NB * (8 + 4 + 8);
+    *bytes = b;
+    return PICTOR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,366 @@
+// kernels_fields.hip -- Yee FDTD, periodic halos and field reductions (gfx950).
+//
+// All kernels are HBM-streaming stencils: one thread per cell, 64-wide rows
+// along x (the contiguous index of the reference layout) so that each wave
+// reads whole 512-byte (FP64) or 256-byte (FP32) row segments.
+//
+// Expression order follows src/maxwell/em_update.F90:9-102 exactly; with
+// -ffp-contract=off the FP64 results are bitwise those of the reference
+// CPU loops.  The update kernels write physical cells 3..m-3 only and the
+// ghost layers are then copied from their periodic images, which is what the
+// reference's loop over 2..m-1 followed by SendRecvFlds leaves in memory.
+#include "pictor_common.h"
+
+namespace pictor {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+inline dim3 phys_grid(const GridP &g)
+{
+    int nz = (g.ndim == 3) ? g.nz : 1;
+    return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, nz);
+}
+
+// em_update.F90:41-77 (B half step), physical cells only
+template <typename R>
+__global__ __launch_bounds__(BX *BY) void k_update_b_half(GridP g, R hc, R idx_, R idy_, R idz_, const R *__restrict__ Ex,
+                                                          const R *__restrict__ Ey, const R *__restrict__ Ez,
+                                                          R *__restrict__ Bx, R *__restrict__ By, R *__restrict__ Bz)
+{
+    int i = 3 + blockIdx.x * BX + threadIdx.x;
+    int j = 3 + blockIdx.y * BY + threadIdx.y;
+    int k = (g.ndim == 3) ? 3 + (int)blockIdx.z : 1;
+    if (i > g.mx - 3 || j > g.my - 3) return;
+    long long o = fidx(g, i, j, k);
+    long long oi = o + 1, oj = o + g.mx, ok = o + (long long)g.mx * g.my;
+    if (g.ndim == 2) {
+        Bx[o] = Bx[o] - hc * idy_ * (Ez[oj] - Ez[o]);
+        By[o] = By[o] + hc * idx_ * (Ez[oi] - Ez[o]);
+        Bz[o] = Bz[o] - hc * idx_ * (Ey[oi] - Ey[o]) + hc * idy_ * (Ex[oj] - Ex[o]);
+    } else {
+        Bx[o] = Bx[o] - hc * idy_ * (Ez[oj] - Ez[o]) + hc * idz_ * (Ey[ok] - Ey[o]);
+        By[o] = By[o] - hc * idz_ * (Ex[ok] - Ex[o]) + hc * idx_ * (Ez[oi] - Ez[o]);
+        Bz[o] = Bz[o] - hc * idx_ * (Ey[oi] - Ey[o]) + hc * idy_ * (Ex[oj] - Ex[o]);
+    }
+}
+
+template <typename R, typename JT>
+__device__ inline R jvalue(const JT *J, long long o, double scale);
+template <>
+__device__ inline double jvalue<double, double>(const double *J, long long o, double)
+{
+    return J[o];
+}
+template <>
+__device__ inline float jvalue<float, double>(const double *J, long long o, double)
+{
+    return (float)J[o];
+}
+template <>
+__device__ inline float jvalue<float, long long>(const long long *J, long long o, double scale)
+{
+    return (float)((double)J[o] * scale);
+}
+
+// em_update.F90:9-39 (E full step); with ADD, AddCurrent (em_update.F90:79-102)
+// follows in the same thread: (E + c curl B) - J, the reference's two roundings.
+template <typename R, typename JT, bool CURL, bool ADD>
+__global__ __launch_bounds__(BX *BY) void k_update_e(GridP g, R fc, R idx_, R idy_, R idz_, const R *__restrict__ Bx,
+                                                     const R *__restrict__ By, const R *__restrict__ Bz,
+                                                     R *__restrict__ Ex, R *__restrict__ Ey, R *__restrict__ Ez,
+                                                     const JT *__restrict__ Jx, const JT *__restrict__ Jy,
+                                                     const JT *__restrict__ Jz, double jscale)
+{
+    int i = 3 + blockIdx.x * BX + threadIdx.x;
+    int j = 3 + blockIdx.y * BY + threadIdx.y;
+    int k = (g.ndim == 3) ? 3 + (int)blockIdx.z : 1;
+    if (i > g.mx - 3 || j > g.my - 3) return;
+    long long o = fidx(g, i, j, k);
+    long long oi = o - 1, oj = o - g.mx, ok = o - (long long)g.mx * g.my;
+    R ex = Ex[o], ey = Ey[o], ez = Ez[o];
+    if (CURL) {
+        if (g.ndim == 2) {
+            ex = ex + fc * idy_ * (Bz[o] - Bz[oj]);
+            ey = ey - fc * idx_ * (Bz[o] - Bz[oi]);
+            ez = ez + fc * idx_ * (By[o] - By[oi]) - fc * idy_ * (Bx[o] - Bx[oj]);
+        } else {
+            ex = ex + fc * idy_ * (Bz[o] - Bz[oj]) - fc * idz_ * (By[o] - By[ok]);
+            ey = ey + fc * idz_ * (Bx[o] - Bx[ok]) - fc * idx_ * (Bz[o] - Bz[oi]);
+            ez = ez + fc * idx_ * (By[o] - By[oi]) - fc * idy_ * (Bx[o] - Bx[oj]);
+        }
+    }
+    if (ADD) {
+        bool inner = (g.ndim == 2) || (k >= 3 && k <= g.mz - 3);
+        if (inner) {
+            ex = ex - jvalue<R, JT>(Jx, o, jscale) * idy_ * idz_;
+            ey = ey - jvalue<R, JT>(Jy, o, jscale) * idx_ * idz_;
+            ez = ez - jvalue<R, JT>(Jz, o, jscale) * idy_ * idx_;
+        }
+    }
+    Ex[o] = ex; Ey[o] = ey; Ez[o] = ez;
+}
+
+// Ghost layers 1..2 and m-2..m take the value of their periodic image
+// (comm_fld.F90:75-130 on one rank; corners end as the doubly shifted image).
+template <typename R>
+__global__ __launch_bounds__(256) void k_ghost_copy(GridP g, R *__restrict__ F0, R *__restrict__ F1, R *__restrict__ F2)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.ncell) return;
+    int i = (int)(t % g.mx) + 1;
+    long long r = t / g.mx;
+    int j = (int)(r % g.my) + 1;
+    int k = (int)(r / g.my) + 1;
+    int si = i, sj = j, sk = k;
+    if (i <= 2) si = i + g.nx; else if (i >= g.mx - 2) si = i - g.nx;
+    if (j <= 2) sj = j + g.ny; else if (j >= g.my - 2) sj = j - g.ny;
+    if (g.ndim == 3) { if (k <= 2) sk = k + g.nz; else if (k >= g.mz - 2) sk = k - g.nz; }
+    if (si == i && sj == j && sk == k) return;
+    long long s = fidx(g, si, sj, sk);
+    F0[t] = F0[s];
+    F1[t] = F1[s];
+    F2[t] = F2[s];
+}
+
+// UpdateCurrentAllFaces for one axis: 3..5 += m-2..m and m-4..m-3 += 1..2.
+// Axis 0 spans the full cross-section, axes 1 and 2 the x range 3..mx-3.
+template <typename JT>
+__global__ __launch_bounds__(256) void k_current_fold(GridP g, int axis, JT *__restrict__ J0, JT *__restrict__ J1,
+                                                      JT *__restrict__ J2)
+{
+    // target layers: 5 planes normal to `axis`
+    int m[3] = {g.mx, g.my, g.mz};
+    int n[3] = {g.nx, g.ny, g.nz};
+    int ext[3] = {g.mx, g.my, g.mz};
+    int lo0 = 1;
+    if (axis != 0) { ext[0] = g.mx - 5; lo0 = 3; }
+    ext[axis] = 5;
+    long long total = (long long)ext[0] * ext[1] * ext[2];
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int c[3];
+    c[0] = (int)(t % ext[0]);
+    long long r = t / ext[0];
+    c[1] = (int)(r % ext[1]);
+    c[2] = (int)(r / ext[1]);
+    int p[3] = {c[0] + 1, c[1] + 1, c[2] + 1};
+    if (axis != 0) p[0] = c[0] + lo0;
+    int layer = c[axis];
+    int shift;
+    if (layer < 3) { p[axis] = 3 + layer; shift = n[axis]; }
+    else { p[axis] = m[axis] - 4 + (layer - 3); shift = -n[axis]; }
+    int q[3] = {p[0], p[1], p[2]};
+    q[axis] += shift;
+    long long o = fidx(g, p[0], p[1], p[2]);
+    long long s = fidx(g, q[0], q[1], q[2]);
+    J0[o] = J0[o] + J0[s];
+    J1[o] = J1[o] + J1[s];
+    J2[o] = J2[o] + J2[s];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill_zero(T *p, long long n)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) p[t] = T(0);
+}
+
+// CalcFldEnergy, savedata_routines.F90:310-327: per-block FP64 partial sums
+template <typename R>
+__global__ __launch_bounds__(256) void k_field_energy(GridP g, const R *Ex, const R *Ey, const R *Ez, const R *Bx,
+                                                      const R *By, const R *Bz, double *partial)
+{
+    __shared__ double sb[256], se[256];
+    long long nphys = (long long)g.nx * g.ny * ((g.ndim == 3) ? g.nz : 1);
+    double be = 0, ee = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nphys;
+         t += (long long)gridDim.x * blockDim.x) {
+        int i = (int)(t % g.nx) + 3;
+        long long r = t / g.nx;
+        int j = (int)(r % g.ny) + 3;
+        int k = (g.ndim == 3) ? (int)(r / g.ny) + 3 : 1;
+        long long o = fidx(g, i, j, k);
+        double bx = Bx[o], by = By[o], bz = Bz[o], ex = Ex[o], ey = Ey[o], ez = Ez[o];
+        be += (bx * bx + by * by + bz * bz);
+        ee += (ex * ex + ey * ey + ez * ez);
+    }
+    sb[threadIdx.x] = be; se[threadIdx.x] = ee;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) { sb[threadIdx.x] += sb[threadIdx.x + s]; se[threadIdx.x] += se[threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = sb[0]; partial[2 * blockIdx.x + 1] = se[0]; }
+}
+
+// max |div E - rho| and max |rho| over physical nodes (CalcDivE stencil,
+// savedata_routines.F90:329-355)
+template <typename R>
+__global__ __launch_bounds__(256) void k_gauss(GridP g, const R *Ex, const R *Ey, const R *Ez, const double *rho,
+                                               double *partial)
+{
+    __shared__ double sr[256], sa[256];
+    long long nphys = (long long)g.nx * g.ny * ((g.ndim == 3) ? g.nz : 1);
+    double rm = 0, am = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nphys;
+         t += (long long)gridDim.x * blockDim.x) {
+        int i = (int)(t % g.nx) + 3;
+        long long r = t / g.nx;
+        int j = (int)(r % g.ny) + 3;
+        int k = (g.ndim == 3) ? (int)(r / g.ny) + 3 : 1;
+        long long o = fidx(g, i, j, k);
+        double d = (double)Ex[o] - (double)Ex[o - 1] + (double)Ey[o] - (double)Ey[o - g.mx];
+        if (g.ndim == 3) d = d + (double)Ez[o] - (double)Ez[o - (long long)g.mx * g.my];
+        double res = fabs(d - rho[o]);
+        rm = fmax(rm, res);
+        am = fmax(am, fabs(rho[o]));
+    }
+    sr[threadIdx.x] = rm; sa[threadIdx.x] = am;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + s]);
+            sa[threadIdx.x] = fmax(sa[threadIdx.x], sa[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = sr[0]; partial[2 * blockIdx.x + 1] = sa[0]; }
+}
+
+__global__ void k_cvt_f32(const double *s, float *d, long long n)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) d[t] = (float)s[t];
+}
+__global__ void k_cvt_f64(const float *s, double *d, long long n)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) d[t] = (double)s[t];
+}
+__global__ void k_cvt_jfix(const long long *s, double *d, long long n, double scale)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) d[t] = (double)s[t] * scale;
+}
+
+inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+template <typename R>
+void launch_update_b_half(const GridP &g, const PhysP &ph, R *const E[3], R *const B[3], hipStream_t s)
+{
+    hipLaunchKernelGGL(k_update_b_half<R>, phys_grid(g), dim3(BX, BY), 0, s, g, (R)ph.fld_halfc, (R)ph.inv_dx,
+                       (R)ph.inv_dy, (R)ph.inv_dz, E[0], E[1], E[2], B[0], B[1], B[2]);
+}
+
+template <typename R>
+void launch_update_e(const GridP &g, const PhysP &ph, R *const E[3], R *const B[3], hipStream_t s)
+{
+    hipLaunchKernelGGL((k_update_e<R, double, true, false>), phys_grid(g), dim3(BX, BY), 0, s, g, (R)ph.fldc,
+                       (R)ph.inv_dx, (R)ph.inv_dy, (R)ph.inv_dz, B[0], B[1], B[2], E[0], E[1], E[2],
+                       (const double *)nullptr, (const double *)nullptr, (const double *)nullptr, 0.0);
+}
+
+template <typename R, typename JT>
+void launch_update_e_add_current(const GridP &g, const PhysP &ph, R *const E[3], R *const B[3], JT *const J[3],
+                                 double jscale, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_update_e<R, JT, true, true>), phys_grid(g), dim3(BX, BY), 0, s, g, (R)ph.fldc,
+                       (R)ph.inv_dx, (R)ph.inv_dy, (R)ph.inv_dz, B[0], B[1], B[2], E[0], E[1], E[2], J[0], J[1],
+                       J[2], jscale);
+}
+
+template <typename R, typename JT>
+void launch_add_current(const GridP &g, const PhysP &ph, R *const E[3], JT *const J[3], double jscale,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL((k_update_e<R, JT, false, true>), phys_grid(g), dim3(BX, BY), 0, s, g, (R)ph.fldc,
+                       (R)ph.inv_dx, (R)ph.inv_dy, (R)ph.inv_dz, (const R *)nullptr, (const R *)nullptr,
+                       (const R *)nullptr, E[0], E[1], E[2], J[0], J[1], J[2], jscale);
+}
+
+template <typename R>
+void launch_ghost_copy(const GridP &g, R *const F[3], hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ghost_copy<R>, dim3(nblocks(g.ncell, 256)), dim3(256), 0, s, g, F[0], F[1], F[2]);
+}
+
+template <typename JT>
+void launch_current_fold(const GridP &g, JT *const J[3], hipStream_t s)
+{
+    int naxes = (g.ndim == 3) ? 3 : 2;
+    for (int a = 0; a < naxes; a++) {
+        long long ext[3] = {g.mx, g.my, g.mz};
+        if (a != 0) ext[0] = g.mx - 5;
+        ext[a] = 5;
+        long long total = ext[0] * ext[1] * ext[2];
+        hipLaunchKernelGGL(k_current_fold<JT>, dim3(nblocks(total, 256)), dim3(256), 0, s, g, a, J[0], J[1], J[2]);
+    }
+}
+
+template <typename T>
+void launch_fill_zero(T *p, long long n, hipStream_t s)
+{
+    unsigned nb = nblocks(n, 256);
+    if (nb > 65536u) nb = 65536u;
+    if (nb == 0) return;
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3(nb), dim3(256), 0, s, p, n);
+}
+
+template <typename R>
+void launch_field_energy(const GridP &g, R *const E[3], R *const B[3], double *partial, int nblk, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_field_energy<R>, dim3(nblk), dim3(256), 0, s, g, E[0], E[1], E[2], B[0], B[1], B[2],
+                       partial);
+}
+
+template <typename R>
+void launch_gauss(const GridP &g, R *const E[3], const double *rho, double *partial, int nblk, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gauss<R>, dim3(nblk), dim3(256), 0, s, g, E[0], E[1], E[2], rho, partial);
+}
+
+void launch_convert_fields_f32(const double *src, float *dst, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cvt_f32, dim3(nblocks(n, 256)), dim3(256), 0, s, src, dst, n);
+}
+void launch_convert_fields_f64(const float *src, double *dst, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cvt_f64, dim3(nblocks(n, 256)), dim3(256), 0, s, src, dst, n);
+}
+void launch_convert_jfix(const long long *src, double *dst, long long n, double scale, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cvt_jfix, dim3(nblocks(n, 256)), dim3(256), 0, s, src, dst, n, scale);
+}
+
+template void launch_update_b_half<double>(const GridP &, const PhysP &, double *const[3], double *const[3], hipStream_t);
+template void launch_update_b_half<float>(const GridP &, const PhysP &, float *const[3], float *const[3], hipStream_t);
+template void launch_update_e<double>(const GridP &, const PhysP &, double *const[3], double *const[3], hipStream_t);
+template void launch_update_e<float>(const GridP &, const PhysP &, float *const[3], float *const[3], hipStream_t);
+template void launch_update_e_add_current<double, double>(const GridP &, const PhysP &, double *const[3],
+                                                          double *const[3], double *const[3], double, hipStream_t);
+template void launch_update_e_add_current<float, long long>(const GridP &, const PhysP &, float *const[3],
+                                                            float *const[3], long long *const[3], double,
+                                                            hipStream_t);
+template void launch_add_current<double, double>(const GridP &, const PhysP &, double *const[3], double *const[3],
+                                                 double, hipStream_t);
+template void launch_add_current<float, long long>(const GridP &, const PhysP &, float *const[3],
+                                                   long long *const[3], double, hipStream_t);
+template void launch_ghost_copy<double>(const GridP &, double *const[3], hipStream_t);
+template void launch_ghost_copy<float>(const GridP &, float *const[3], hipStream_t);
+template void launch_current_fold<double>(const GridP &, double *const[3], hipStream_t);
+template void launch_current_fold<long long>(const GridP &, long long *const[3], hipStream_t);
+template void launch_fill_zero<double>(double *, long long, hipStream_t);
+template void launch_fill_zero<float>(float *, long long, hipStream_t);
+template void launch_fill_zero<long long>(long long *, long long, hipStream_t);
+template void launch_fill_zero<int>(int *, long long, hipStream_t);
+template void launch_field_energy<double>(const GridP &, double *const[3], double *const[3], double *, int, hipStream_t);
+template void launch_field_energy<float>(const GridP &, float *const[3], float *const[3], double *, int, hipStream_t);
+template void launch_gauss<double>(const GridP &, double *const[3], const double *, double *, int, hipStream_t);
+template void launch_gauss<float>(const GridP &, float *const[3], const double *, double *, int, hipStream_t);
+
+}  // namespace pictor

@@ -1,0 +1,512 @@
+// kernels_push.hip -- tile-local Boris push + zigzag deposit + binning (gfx950).
+//
+// One workgroup owns one tile of T x T cells.  It
+//   1. stages the node-centred E and B of a (T+5)^2 window into LDS; these are
+//      exactly the reference's VecE/VecB corner values (movdep_routines.F90:
+//      6-223: corner nn of cell (i,j) is node (i + mod(nn-1,2), j + (nn-1)/2)),
+//   2. streams its particles from HBM once (SoA, coalesced inside every bin),
+//      gathers from LDS, pushes (movdep.F90:143-173), deposits the two zigzag
+//      half-segments (movdep.F90:177-237) into an LDS current window,
+//   3. applies the periodic exchange of the reference (loadprtlout.F90:22-97)
+//      and writes each particle into the bin of its destination tile, slots
+//      being reserved per wavefront with one LDS atomic per destination,
+//   4. flushes the current window to the global J arrays (ghost layers
+//      included; the halo fold adds them into their owners afterwards).
+//
+// FP64 form: positions are the reference's local cell coordinates, the push
+// follows the reference expression order (bitwise with -ffp-contract=off) and
+// the current window accumulates with LDS FP64 atomics.
+// Split form: int32 cell + FP32 offsets, FP32 push, and the current window is
+// int64 fixed point (2^40 units per charge*cell) so that accumulation is exact
+// and the result does not depend on particle order: it is deterministic.
+#include "pictor_common.h"
+
+namespace pictor {
+namespace {
+
+__device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ inline double dmax(double a, double b) { return (b > a) ? b : a; }
+
+__device__ inline unsigned long long lanemask_lt()
+{
+    unsigned lane = __lane_id();
+    return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+// Per-wave reservation of consecutive slots in the LDS counter of each
+// destination; every lane of the wave must call it (d = -1 for idle lanes).
+__device__ inline int reserve_slot(int d, int *sCnt)
+{
+    const bool act = d >= 0;
+    unsigned long long pending = __ballot(act);
+    int slot = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int ld = __shfl(d, leader);
+        const unsigned long long m = __ballot(act && d == ld);
+        int b = 0;
+        if ((int)__lane_id() == leader) b = atomicAdd(&sCnt[ld], __popcll(m));
+        b = __shfl(b, leader);
+        if (act && d == ld) slot = b + __popcll(m & lanemask_lt());
+        pending &= ~m;
+    }
+    return slot;
+}
+
+__device__ inline int wrap_tile(int t, int n)
+{
+    return (t < 0) ? t + n : (t >= n ? t - n : t);
+}
+
+__device__ inline int norm_dir(int dd, int n)
+{
+    if (dd > 1) dd -= n;
+    else if (dd < -1) dd += n;
+    return dd;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 form, 2-D
+// ---------------------------------------------------------------------------
+template <int T, int BS>
+__global__ __launch_bounds__(BS) void k_push2d_f64(GridP g, PhysP ph, TileP tp, const double *__restrict__ Ex,
+                                                   const double *__restrict__ Ey, const double *__restrict__ Ez,
+                                                   const double *__restrict__ Bx, const double *__restrict__ By,
+                                                   const double *__restrict__ Bz, double *__restrict__ Jx,
+                                                   double *__restrict__ Jy, double *__restrict__ Jz, PartF64 A,
+                                                   PartF64 Bo, BinsP bins, PartF64 spill, SpillP sp)
+{
+    constexpr int W = T + 5;
+    constexpr int WW = W * W;
+    __shared__ double sN[6][WW];
+    __shared__ double sJ[3][WW];
+    __shared__ int sStart[10];
+    __shared__ long long sSrc[9];
+    __shared__ long long sDstBase[9];
+    __shared__ int sDstCap[9], sDstBin[9], sCnt[9];
+
+    const int tile = tile_remap(tp, blockIdx.x);
+    const int tx = tile % tp.ntx, ty = tile / tp.ntx;
+    const int iw = 1 + tx * T, jw = 1 + ty * T;   // window origin = first cell - 2 (reference index)
+    const int tid = threadIdx.x;
+    const int mx = g.mx;
+
+    if (tid < 9) {
+        const int b = tile * tp.NB + tid;
+        sSrc[tid] = bins.base[b];
+        const int ddx = tid % 3 - 1, ddy = tid / 3 - 1;
+        const int dt = wrap_tile(ty + ddy, tp.nty) * tp.ntx + wrap_tile(tx + ddx, tp.ntx);
+        const int db = dt * tp.NB + tid;
+        sDstBin[tid] = db;
+        sDstBase[tid] = bins.base[db];
+        sDstCap[tid] = bins.cap[db];
+        sCnt[tid] = 0;
+    }
+    if (tid == 0) {
+        int acc = 0;
+        for (int b = 0; b < 9; b++) { sStart[b] = acc; acc += bins.cnt_in[tile * tp.NB + b]; }
+        sStart[9] = acc;
+    }
+    const double bx0 = ph.bext[0], by0 = ph.bext[1], bz0 = ph.bext[2];
+    for (int e = tid; e < WW; e += BS) {
+        const int ii = e % W, jj = e / W;
+        double nex = 0, ney = 0, nez = 0, nbx = 0, nby = 0, nbz = 0;
+        if (ii >= 1 && jj >= 1) {
+            const long long o = (long long)(iw + ii - 1) + (long long)mx * (jw + jj - 1);
+            nex = (Ex[o - 1] + Ex[o]) * 0.5;
+            ney = (Ey[o - mx] + Ey[o]) * 0.5;
+            nez = Ez[o];
+            nbx = (Bx[o - mx] + Bx[o]) * 0.5 + bx0;
+            nby = (By[o - 1] + By[o]) * 0.5 + by0;
+            nbz = (Bz[o - 1 - mx] + Bz[o - 1] + Bz[o - mx] + Bz[o]) * 0.25 + bz0;
+        }
+        sN[0][e] = nex; sN[1][e] = ney; sN[2][e] = nez;
+        sN[3][e] = nbx; sN[4][e] = nby; sN[5][e] = nbz;
+        sJ[0][e] = 0.0; sJ[1][e] = 0.0; sJ[2][e] = 0.0;
+    }
+    __syncthreads();
+
+    const double c = ph.c, sqc = ph.sqc, cinv = ph.cinv, qi = ph.qi;
+    const int total = sStart[9];
+    for (int base = 0; base < total; base += BS) {
+        const int k = base + tid;
+        const bool active = k < total;
+        double x0 = 0, y0 = 0, z0 = 0, un = 0, vn = 0, wn = 0, q = 0;
+        double xn = 0, yn = 0, zn = 0;
+        int flv = 0, d = -1;
+        if (active) {
+            int b = 0;
+            while (k >= sStart[b + 1]) b++;
+            const long long src = sSrc[b] + (k - sStart[b]);
+            x0 = A.x[src]; y0 = A.y[src]; z0 = A.z[src];
+            un = A.u[src]; vn = A.v[src]; wn = A.w[src];
+            q = A.q[src]; flv = A.flv[src];
+            const int ip0 = (int)x0, jp0 = (int)y0, kp0 = (int)z0;
+            const int ii0 = ip0 - iw, jj0 = jp0 - jw;
+            xn = x0; yn = y0; zn = z0;
+            if (ii0 < 1 || ii0 > W - 3 || jj0 < 1 || jj0 > W - 3 || flv < 1 || flv > ph.nflvr) {
+                atomicOr(sp.err, (int)kErrBounds);
+                d = 4;
+            } else {
+                // gather, movdep.F90:88-139
+                const double dx = x0 - ip0, dy = y0 - jp0;
+                const double w1 = (1.0 - dx) * (1.0 - dy), w2 = dx * (1.0 - dy);
+                const double w3 = (1.0 - dx) * dy, w4 = dx * dy;
+                const int e0 = jj0 * W + ii0;
+                double pf[6];
+#pragma unroll
+                for (int cc = 0; cc < 6; cc++) {
+                    double a = 0.0;
+                    a = a + w1 * sN[cc][e0];
+                    a = a + w2 * sN[cc][e0 + 1];
+                    a = a + w3 * sN[cc][e0 + W];
+                    a = a + w4 * sN[cc][e0 + W + 1];
+                    pf[cc] = a;
+                }
+                const double qm = ph.flvrqm[flv - 1] * 0.5;
+                double pEx = pf[0] * qm, pEy = pf[1] * qm, pEz = pf[2] * qm;
+                double pBx = pf[3] * qm, pBy = pf[4] * qm, pBz = pf[5] * qm;
+                // Boris, movdep.F90:143-173
+                double u0 = c * un + pEx;
+                double v0 = c * vn + pEy;
+                double w0 = c * wn + pEz;
+                double gg = 1.0 / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+                pBx = gg * pBx; pBy = gg * pBy; pBz = gg * pBz;
+                const double f = 2.0 / (1.0 + pBx * pBx + pBy * pBy + pBz * pBz);
+                const double u1 = (u0 + v0 * pBz - w0 * pBy) * f;
+                const double v1 = (v0 + w0 * pBx - u0 * pBz) * f;
+                const double w1b = (w0 + u0 * pBy - v0 * pBx) * f;
+                u0 = u0 + v1 * pBz - w1b * pBy + pEx;
+                v0 = v0 + w1b * pBx - u1 * pBz + pEy;
+                w0 = w0 + u1 * pBy - v1 * pBx + pEz;
+                un = u0 * cinv; vn = v0 * cinv; wn = w0 * cinv;
+                gg = sqc / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+                xn = x0 + un * gg * ph.inv_dx;
+                yn = y0 + vn * gg * ph.inv_dy;
+                zn = z0 + wn * gg * ph.inv_dz;
+                // zigzag deposit, movdep.F90:177-237
+                const int ip = (int)xn, jp = (int)yn, kp = (int)zn;
+                const double xr = dmin((double)(min(ip0, ip) + 1), dmax((double)max(ip0, ip), 0.5 * (x0 + xn)));
+                const double yr = dmin((double)(min(jp0, jp) + 1), dmax((double)max(jp0, jp), 0.5 * (y0 + yn)));
+                const double zr = dmin((double)(min(kp0, kp) + 1), dmax((double)max(kp0, kp), 0.5 * (z0 + zn)));
+                const double qthis = q * qi;
+                {
+                    const double Fx = qthis * (xr - x0), Fy = qthis * (yr - y0), Fz = qthis * (zr - z0);
+                    const double Wx = 0.5 * (x0 + xr) - ip0, Wy = 0.5 * (y0 + yr) - jp0;
+                    const int e = e0;
+                    atomicAdd(&sJ[0][e], Fx * (1.0 - Wy));
+                    atomicAdd(&sJ[0][e + W], Fx * Wy);
+                    atomicAdd(&sJ[1][e], Fy * (1.0 - Wx));
+                    atomicAdd(&sJ[1][e + 1], Fy * Wx);
+                    atomicAdd(&sJ[2][e], (Fz * (1.0 - Wx)) * (1.0 - Wy));
+                    atomicAdd(&sJ[2][e + 1], (Fz * Wx) * (1.0 - Wy));
+                    atomicAdd(&sJ[2][e + W], (Fz * (1.0 - Wx)) * Wy);
+                    atomicAdd(&sJ[2][e + W + 1], (Fz * Wx) * Wy);
+                }
+                {
+                    const double Fx = qthis * (xn - xr), Fy = qthis * (yn - yr), Fz = qthis * (zn - zr);
+                    const double Wx = 0.5 * (xn + xr) - ip, Wy = 0.5 * (yn + yr) - jp;
+                    const int e = (jp - jw) * W + (ip - iw);
+                    atomicAdd(&sJ[0][e], Fx * (1.0 - Wy));
+                    atomicAdd(&sJ[0][e + W], Fx * Wy);
+                    atomicAdd(&sJ[1][e], Fy * (1.0 - Wx));
+                    atomicAdd(&sJ[1][e + 1], Fy * Wx);
+                    atomicAdd(&sJ[2][e], (Fz * (1.0 - Wx)) * (1.0 - Wy));
+                    atomicAdd(&sJ[2][e + 1], (Fz * Wx) * (1.0 - Wy));
+                    atomicAdd(&sJ[2][e + W], (Fz * (1.0 - Wx)) * Wy);
+                    atomicAdd(&sJ[2][e + W + 1], (Fz * Wx) * Wy);
+                }
+                // 2-D z wrap, movdep.F90:241-249
+                if (zn > ph.zmax) zn = zn - ph.zlen;
+                else if (zn < ph.zmin) zn = ph.zlen + zn;
+                // periodic exchange, loadprtlout.F90:31-76 + shifts subdomains.F90:720-740
+                bool sent = false;
+                if (xn < ph.xmin) {
+                    if ((yn <= (ph.ymax + ph.xmin - xn)) && (yn >= (ph.ymin - ph.xmin + xn))) { xn = xn + ph.xlen; sent = true; }
+                } else if (xn > ph.xmax) {
+                    if ((yn <= (ph.ymax - ph.xmax + xn)) && (yn >= (ph.ymin + ph.xmax - xn))) { xn = xn - ph.xlen; sent = true; }
+                }
+                if (!sent) {
+                    if (yn < ph.ymin) {
+                        if ((yn < (ph.ymin - ph.xmin + xn)) && (yn < (ph.ymin + ph.xmax - xn))) yn = yn + ph.ylen;
+                    } else if (yn > ph.ymax) {
+                        if ((yn > (ph.ymax + ph.xmin - xn)) && (yn > (ph.ymax - ph.xmax + xn))) yn = yn - ph.ylen;
+                    }
+                }
+                int cx = (int)xn - 3, cy = (int)yn - 3;
+                cx = cx < 0 ? 0 : (cx > g.nx - 1 ? g.nx - 1 : cx);
+                cy = cy < 0 ? 0 : (cy > g.ny - 1 ? g.ny - 1 : cy);
+                const int ddx = norm_dir(cx / T - tx, tp.ntx), ddy = norm_dir(cy / T - ty, tp.nty);
+                if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1) {
+                    atomicOr(sp.err, (int)kErrBounds);
+                    d = 4;
+                } else {
+                    d = (ddx + 1) + 3 * (ddy + 1);
+                }
+            }
+        }
+        const int slot = reserve_slot(d, sCnt);
+        if (active) {
+            if (slot < sDstCap[d]) {
+                const long long o = sDstBase[d] + slot;
+                Bo.x[o] = xn; Bo.y[o] = yn; Bo.z[o] = zn;
+                Bo.u[o] = un; Bo.v[o] = vn; Bo.w[o] = wn;
+                Bo.q[o] = q; Bo.flv[o] = flv;
+            } else {
+                const int s = atomicAdd(sp.count, 1);
+                if (s < sp.cap) {
+                    spill.x[s] = xn; spill.y[s] = yn; spill.z[s] = zn;
+                    spill.u[s] = un; spill.v[s] = vn; spill.w[s] = wn;
+                    spill.q[s] = q; spill.flv[s] = flv;
+                } else {
+                    atomicOr(sp.err, (int)kErrCapacity);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < WW; e += BS) {
+        const long long o = (long long)(iw + e % W - 1) + (long long)mx * (jw + e / W - 1);
+        const double a = sJ[0][e], b = sJ[1][e], cz = sJ[2][e];
+        if (a != 0.0) unsafeAtomicAdd(&Jx[o], a);
+        if (b != 0.0) unsafeAtomicAdd(&Jy[o], b);
+        if (cz != 0.0) unsafeAtomicAdd(&Jz[o], cz);
+    }
+    if (tid < 9) bins.cnt_out[sDstBin[tid]] = min(sCnt[tid], sDstCap[tid]);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 split form, 2-D
+// ---------------------------------------------------------------------------
+__device__ inline unsigned long long fixp(float v)
+{
+    return (unsigned long long)__float2ll_rn(v * 1099511627776.0f);   // v * 2^40, exact scaling
+}
+
+template <int T, int BS>
+__global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp, const float *__restrict__ Ex,
+                                                     const float *__restrict__ Ey, const float *__restrict__ Ez,
+                                                     const float *__restrict__ Bx, const float *__restrict__ By,
+                                                     const float *__restrict__ Bz, long long *__restrict__ Jx,
+                                                     long long *__restrict__ Jy, long long *__restrict__ Jz,
+                                                     PartSplit A, PartSplit Bo, BinsP bins, PartSplit spill,
+                                                     SpillP sp)
+{
+    constexpr int W = T + 5;
+    constexpr int WW = W * W;
+    __shared__ float sN[6][WW];
+    __shared__ unsigned long long sJ[3][WW];
+    __shared__ int sStart[10];
+    __shared__ long long sSrc[9];
+    __shared__ long long sDstBase[9];
+    __shared__ int sDstCap[9], sDstBin[9], sCnt[9];
+
+    const int tile = tile_remap(tp, blockIdx.x);
+    const int tx = tile % tp.ntx, ty = tile / tp.ntx;
+    const int iw = 1 + tx * T, jw = 1 + ty * T;
+    const int cx0 = tx * T - 2, cy0 = ty * T - 2;   // window origin as a physical cell index
+    const int tid = threadIdx.x;
+    const int mx = g.mx, nx = g.nx, ny = g.ny;
+
+    const float bx0 = (float)ph.bext[0], by0 = (float)ph.bext[1], bz0 = (float)ph.bext[2];
+    for (int e = tid; e < WW; e += BS) {
+        const int ii = e % W, jj = e / W;
+        float nex = 0, ney = 0, nez = 0, nbx = 0, nby = 0, nbz = 0;
+        if (ii >= 1 && jj >= 1) {
+            const long long o = (long long)(iw + ii - 1) + (long long)mx * (jw + jj - 1);
+            nex = (Ex[o - 1] + Ex[o]) * 0.5f;
+            ney = (Ey[o - mx] + Ey[o]) * 0.5f;
+            nez = Ez[o];
+            nbx = (Bx[o - mx] + Bx[o]) * 0.5f + bx0;
+            nby = (By[o - 1] + By[o]) * 0.5f + by0;
+            nbz = (Bz[o - 1 - mx] + Bz[o - 1] + Bz[o - mx] + Bz[o]) * 0.25f + bz0;
+        }
+        sN[0][e] = nex; sN[1][e] = ney; sN[2][e] = nez;
+        sN[3][e] = nbx; sN[4][e] = nby; sN[5][e] = nbz;
+        sJ[0][e] = 0ull; sJ[1][e] = 0ull; sJ[2][e] = 0ull;
+    }
+
+    const float c = (float)ph.c, sqc = (float)ph.sqc, cinv = (float)ph.cinv;
+    const float idx_ = (float)ph.inv_dx, idy_ = (float)ph.inv_dy, idz_ = (float)ph.inv_dz;
+
+    for (int s = 0; s < tp.nsp; s++) {
+        if (tid < 9) {
+            const int b = tile * tp.NB + s * 9 + tid;
+            sSrc[tid] = bins.base[b];
+            const int ddx = tid % 3 - 1, ddy = tid / 3 - 1;
+            const int dt = wrap_tile(ty + ddy, tp.nty) * tp.ntx + wrap_tile(tx + ddx, tp.ntx);
+            const int db = dt * tp.NB + s * 9 + tid;
+            sDstBin[tid] = db;
+            sDstBase[tid] = bins.base[db];
+            sDstCap[tid] = bins.cap[db];
+            sCnt[tid] = 0;
+        }
+        if (tid == 0) {
+            int acc = 0;
+            for (int b = 0; b < 9; b++) { sStart[b] = acc; acc += bins.cnt_in[tile * tp.NB + s * 9 + b]; }
+            sStart[9] = acc;
+        }
+        __syncthreads();
+        const float qm = (float)(ph.flvrqm[s] * 0.5);
+        const float qs = (float)ph.flvq[s];
+        const int total = sStart[9];
+        for (int base = 0; base < total; base += BS) {
+            const int k = base + tid;
+            const bool active = k < total;
+            int d = -1, ncell = 0;
+            float ox = 0, oy = 0, oz = 0, un = 0, vn = 0, wn = 0;
+            if (active) {
+                int b = 0;
+                while (k >= sStart[b + 1]) b++;
+                const long long src = sSrc[b] + (k - sStart[b]);
+                const int cell = A.cell[src];
+                const float x0 = A.ox[src], y0 = A.oy[src], z0 = A.oz[src];
+                un = A.u[src]; vn = A.v[src]; wn = A.w[src];
+                const int ci = cell % nx, cj = cell / nx;
+                const int ii0 = ci - cx0, jj0 = cj - cy0;
+                ox = x0; oy = y0; oz = z0; ncell = cell;
+                if (ii0 < 2 || ii0 > T + 1 || jj0 < 2 || jj0 > T + 1) {
+                    atomicOr(sp.err, (int)kErrBounds);
+                    d = 4;
+                } else {
+                    const float w1 = (1.0f - x0) * (1.0f - y0), w2 = x0 * (1.0f - y0);
+                    const float w3 = (1.0f - x0) * y0, w4 = x0 * y0;
+                    const int e0 = jj0 * W + ii0;
+                    float pf[6];
+#pragma unroll
+                    for (int cc = 0; cc < 6; cc++) {
+                        float a = w1 * sN[cc][e0];
+                        a = a + w2 * sN[cc][e0 + 1];
+                        a = a + w3 * sN[cc][e0 + W];
+                        a = a + w4 * sN[cc][e0 + W + 1];
+                        pf[cc] = a * qm;
+                    }
+                    float u0 = c * un + pf[0];
+                    float v0 = c * vn + pf[1];
+                    float w0 = c * wn + pf[2];
+                    float gg = 1.0f / sqrtf(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+                    const float pBx = gg * pf[3], pBy = gg * pf[4], pBz = gg * pf[5];
+                    const float f = 2.0f / (1.0f + pBx * pBx + pBy * pBy + pBz * pBz);
+                    const float u1 = (u0 + v0 * pBz - w0 * pBy) * f;
+                    const float v1 = (v0 + w0 * pBx - u0 * pBz) * f;
+                    const float w1b = (w0 + u0 * pBy - v0 * pBx) * f;
+                    u0 = u0 + v1 * pBz - w1b * pBy + pf[0];
+                    v0 = v0 + w1b * pBx - u1 * pBz + pf[1];
+                    w0 = w0 + u1 * pBy - v1 * pBx + pf[2];
+                    un = u0 * cinv; vn = v0 * cinv; wn = w0 * cinv;
+                    gg = sqc / sqrtf(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+                    const float x1 = x0 + un * gg * idx_;
+                    const float y1 = y0 + vn * gg * idy_;
+                    const float z1 = z0 + wn * gg * idz_;
+                    int di = (int)floorf(x1), dj = (int)floorf(y1);
+                    const int dk = (int)floorf(z1);
+                    const float xr = fminf((float)(min(0, di) + 1), fmaxf((float)max(0, di), 0.5f * (x0 + x1)));
+                    const float yr = fminf((float)(min(0, dj) + 1), fmaxf((float)max(0, dj), 0.5f * (y0 + y1)));
+                    const float zr = fminf((float)(min(0, dk) + 1), fmaxf((float)max(0, dk), 0.5f * (z0 + z1)));
+                    {
+                        const float Fx = qs * (xr - x0), Fy = qs * (yr - y0), Fz = qs * (zr - z0);
+                        const float Wx = 0.5f * (x0 + xr), Wy = 0.5f * (y0 + yr);
+                        const int e = e0;
+                        atomicAdd(&sJ[0][e], fixp(Fx * (1.0f - Wy)));
+                        atomicAdd(&sJ[0][e + W], fixp(Fx * Wy));
+                        atomicAdd(&sJ[1][e], fixp(Fy * (1.0f - Wx)));
+                        atomicAdd(&sJ[1][e + 1], fixp(Fy * Wx));
+                        atomicAdd(&sJ[2][e], fixp((Fz * (1.0f - Wx)) * (1.0f - Wy)));
+                        atomicAdd(&sJ[2][e + 1], fixp((Fz * Wx) * (1.0f - Wy)));
+                        atomicAdd(&sJ[2][e + W], fixp((Fz * (1.0f - Wx)) * Wy));
+                        atomicAdd(&sJ[2][e + W + 1], fixp((Fz * Wx) * Wy));
+                    }
+                    {
+                        const float Fx = qs * (x1 - xr), Fy = qs * (y1 - yr), Fz = qs * (z1 - zr);
+                        const float Wx = 0.5f * (x1 + xr) - (float)di, Wy = 0.5f * (y1 + yr) - (float)dj;
+                        const int e = e0 + dj * W + di;
+                        atomicAdd(&sJ[0][e], fixp(Fx * (1.0f - Wy)));
+                        atomicAdd(&sJ[0][e + W], fixp(Fx * Wy));
+                        atomicAdd(&sJ[1][e], fixp(Fy * (1.0f - Wx)));
+                        atomicAdd(&sJ[1][e + 1], fixp(Fy * Wx));
+                        atomicAdd(&sJ[2][e], fixp((Fz * (1.0f - Wx)) * (1.0f - Wy)));
+                        atomicAdd(&sJ[2][e + 1], fixp((Fz * Wx) * (1.0f - Wy)));
+                        atomicAdd(&sJ[2][e + W], fixp((Fz * (1.0f - Wx)) * Wy));
+                        atomicAdd(&sJ[2][e + W + 1], fixp((Fz * Wx) * Wy));
+                    }
+                    ox = x1 - (float)di;
+                    if (ox >= 1.0f) { ox = 0.0f; di += 1; }
+                    oy = y1 - (float)dj;
+                    if (oy >= 1.0f) { oy = 0.0f; dj += 1; }
+                    oz = z1 - (float)dk;
+                    if (oz >= 1.0f) oz = 0.0f;
+                    int nci = ci + di, ncj = cj + dj;
+                    nci = nci < 0 ? nci + nx : (nci >= nx ? nci - nx : nci);
+                    ncj = ncj < 0 ? ncj + ny : (ncj >= ny ? ncj - ny : ncj);
+                    ncell = nci + nx * ncj;
+                    const int ddx = norm_dir(nci / T - tx, tp.ntx), ddy = norm_dir(ncj / T - ty, tp.nty);
+                    d = (ddx + 1) + 3 * (ddy + 1);
+                }
+            }
+            const int slot = reserve_slot(d, sCnt);
+            if (active) {
+                if (slot < sDstCap[d]) {
+                    const long long o = sDstBase[d] + slot;
+                    Bo.cell[o] = ncell; Bo.ox[o] = ox; Bo.oy[o] = oy; Bo.oz[o] = oz;
+                    Bo.u[o] = un; Bo.v[o] = vn; Bo.w[o] = wn;
+                } else {
+                    const int si = atomicAdd(sp.count, 1);
+                    if (si < sp.cap) {
+                        spill.cell[si] = ncell; spill.ox[si] = ox; spill.oy[si] = oy; spill.oz[si] = oz;
+                        spill.u[si] = un; spill.v[si] = vn; spill.w[si] = wn; spill.sp[si] = s;
+                    } else {
+                        atomicOr(sp.err, (int)kErrCapacity);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < 9) bins.cnt_out[sDstBin[tid]] = min(sCnt[tid], sDstCap[tid]);
+        __syncthreads();
+    }
+    for (int e = tid; e < WW; e += BS) {
+        const long long o = (long long)(iw + e % W - 1) + (long long)mx * (jw + e / W - 1);
+        const unsigned long long a = sJ[0][e], b = sJ[1][e], cz = sJ[2][e];
+        if (a) atomicAdd((unsigned long long *)&Jx[o], a);
+        if (b) atomicAdd((unsigned long long *)&Jy[o], b);
+        if (cz) atomicAdd((unsigned long long *)&Jz[o], cz);
+    }
+}
+
+}  // namespace
+
+void launch_push2d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, double *const E[3],
+                       double *const B[3], double *const J[3], PartF64 A, PartF64 Bo, BinsP bins, PartF64 spill,
+                       SpillP sp, hipStream_t s)
+{
+    dim3 grid(tp.ntiles);
+#define PUSH_F64(TT, BSZ)                                                                                          \
+    hipLaunchKernelGGL((k_push2d_f64<TT, BSZ>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], B[0], B[1], \
+                       B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp)
+    switch (T) {
+    case 8: PUSH_F64(8, 256); break;
+    case 16: PUSH_F64(16, 256); break;
+    case 32: PUSH_F64(32, 512); break;
+    default: break;
+    }
+#undef PUSH_F64
+}
+
+void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp, float *const E[3],
+                         float *const B[3], long long *const J[3], PartSplit A, PartSplit Bo, BinsP bins,
+                         PartSplit spill, SpillP sp, hipStream_t s)
+{
+    dim3 grid(tp.ntiles);
+#define PUSH_SPLIT(TT, BSZ)                                                                                         \
+    hipLaunchKernelGGL((k_push2d_split<TT, BSZ>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], B[0], B[1], \
+                       B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp)
+    switch (T) {
+    case 8: PUSH_SPLIT(8, 256); break;
+    case 16: PUSH_SPLIT(16, 256); break;
+    case 32: PUSH_SPLIT(32, 512); break;
+    default: break;
+    }
+#undef PUSH_SPLIT
+}
+
+}  // namespace pictor

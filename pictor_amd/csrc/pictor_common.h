@@ -1,0 +1,152 @@
+// pictor_common.h -- shared types and launchers of the gfx950 particle step.
+//
+// Reference layout on the device: every field is one (mx,my,mz) column-major
+// array whose element (1,1,1) sits at offset 0 (src/vars.F90:23-24), so the
+// host's Fortran arrays are copied without reshaping.  Particles live in
+// tile bins: bin (tile, species, d) holds the particles of `tile` that
+// arrived from direction d = (ddx+1) + 3*(ddy+1) [+ 9*(ddz+1)] in the last
+// step, d = centre for the ones that stayed.  Every bin has exactly one
+// writer (the source tile's workgroup), so binning needs no global atomics.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pictor {
+
+constexpr int kFixShift = 40;                 // fixed-point current: 2^40 units per (charge x cell)
+constexpr double kFixScale = 1099511627776.0; // 2^40
+constexpr double kFixInv = 1.0 / 1099511627776.0;
+
+enum ErrBits : int { kErrBounds = 1, kErrCapacity = 2 };
+
+struct GridP {
+    int ndim, nx, ny, nz, mx, my, mz;
+    long long ncell;   // mx*my*mz
+};
+
+struct PhysP {
+    double c, sqc, cinv, qi, fldc, fld_halfc;
+    double inv_dx, inv_dy, inv_dz;
+    double xmin, xmax, ymin, ymax, zmin, zmax, xlen, ylen, zlen;
+    double bext[3];
+    double flvrqm[4];  // q/m per flavour (initialise.F90:84-85)
+    double flvq[4];    // charge of one particle per flavour (split form)
+    double mass[4];    // massi, masse (initialise.F90:49-50)
+    int nflvr;
+};
+
+struct TileP {
+    int T, ntx, nty, ntz, ntiles;
+    int ND;    // 9 (2-D) or 27 (3-D) arrival directions
+    int nsp;   // species bins per tile (1 in the FP64 form, nflvr in the split form)
+    int NB;    // nsp*ND bins per tile
+    int xcd_remap;  // 1 when ntiles % 8 == 0: give each XCD a contiguous block of tiles
+};
+
+// FP64 form: the reference SoA (src/vars.F90:43-45)
+struct PartF64 {
+    double *q, *x, *y, *z, *u, *v, *w;
+    int *flv;
+};
+
+// FP32 split form: linear physical cell index + FP32 in-cell offsets.
+struct PartSplit {
+    int *cell;
+    float *ox, *oy, *oz, *u, *v, *w;
+    int *sp;   // species id, used only by the spill / raw buffers
+};
+
+__host__ __device__ inline long long fidx(const GridP &g, int i, int j, int k)
+{
+    return (long long)(i - 1) + (long long)g.mx * ((long long)(j - 1) + (long long)g.my * (long long)(k - 1));
+}
+
+__device__ inline int tile_remap(const TileP &tp, int bid)
+{
+    if (!tp.xcd_remap) return bid;
+    int per = tp.ntiles >> 3;
+    return (bid & 7) * per + (bid >> 3);
+}
+
+// ---------------- launchers (kernels_fields.hip) ----------------
+template <typename R>
+void launch_update_b_half(const GridP &g, const PhysP &ph, R *const E[3], R *const B[3], hipStream_t s);
+template <typename R>
+void launch_update_e(const GridP &g, const PhysP &ph, R *const E[3], R *const B[3], hipStream_t s);
+// E update fused with AddCurrent; J is double (FP64 form) or fixed point (split form)
+template <typename R, typename JT>
+void launch_update_e_add_current(const GridP &g, const PhysP &ph, R *const E[3], R *const B[3],
+                                 JT *const J[3], double jscale, hipStream_t s);
+template <typename R, typename JT>
+void launch_add_current(const GridP &g, const PhysP &ph, R *const E[3], JT *const J[3], double jscale,
+                        hipStream_t s);
+template <typename R>
+void launch_ghost_copy(const GridP &g, R *const F[3], hipStream_t s);
+template <typename JT>
+void launch_current_fold(const GridP &g, JT *const J[3], hipStream_t s);
+template <typename T>
+void launch_fill_zero(T *p, long long n, hipStream_t s);
+template <typename R>
+void launch_field_energy(const GridP &g, R *const E[3], R *const B[3], double *partial, int nblk, hipStream_t s);
+template <typename R>
+void launch_gauss(const GridP &g, R *const E[3], const double *rho, double *partial, int nblk, hipStream_t s);
+void launch_convert_fields_f32(const double *src, float *dst, long long n, hipStream_t s);
+void launch_convert_fields_f64(const float *src, double *dst, long long n, hipStream_t s);
+void launch_convert_jfix(const long long *src, double *dst, long long n, double scale, hipStream_t s);
+
+// ---------------- launchers (kernels_push.hip) ----------------
+struct BinsP {
+    const long long *base;  // per bin
+    const int *cap;         // per bin
+    const int *cnt_in;      // per bin, buffer read this step
+    int *cnt_out;           // per bin, buffer written this step
+};
+
+struct SpillP {
+    int *count;
+    int cap;
+    int *err;
+};
+
+void launch_push2d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, double *const E[3],
+                       double *const B[3], double *const J[3], PartF64 A, PartF64 Bo, BinsP bins,
+                       PartF64 spill, SpillP sp, hipStream_t s);
+void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp, float *const E[3],
+                         float *const B[3], long long *const J[3], PartSplit A, PartSplit Bo, BinsP bins,
+                         PartSplit spill, SpillP sp, hipStream_t s);
+
+// ---------------- launchers (kernels_layout.hip) ----------------
+// raw FP64 particles (upload): tile id per particle + per (tile, species) counts
+void launch_raw_tiles_f64(const GridP &g, const PhysP &ph, const TileP &tp, PartF64 raw, long long n,
+                          int split, int *tile_out, int *counts, int *err, hipStream_t s);
+void launch_raw_scatter_f64(const GridP &g, const TileP &tp, PartF64 raw, long long n, const int *tile_of,
+                            const long long *base, int *fill, PartF64 out, hipStream_t s);
+void launch_raw_scatter_split(const GridP &g, const PhysP &ph, const TileP &tp, PartF64 raw, long long n,
+                              const int *tile_of, const long long *base, int *fill, PartSplit out,
+                              hipStream_t s);
+// spill particles: tile per spill particle + counts
+void launch_spill_tiles_f64(const GridP &g, const TileP &tp, PartF64 spill, int n, int *tile_out, int *counts,
+                            hipStream_t s);
+void launch_spill_tiles_split(const GridP &g, const TileP &tp, PartSplit spill, int n, int *tile_out,
+                              int *counts, hipStream_t s);
+// compaction of every (tile, species) group of bins into one dense run
+void launch_compact_f64(const TileP &tp, PartF64 src, const long long *base, const int *cnt,
+                        const long long *dst_off, PartF64 dst, hipStream_t s);
+void launch_compact_split(const TileP &tp, PartSplit src, const long long *base, const int *cnt,
+                          const long long *dst_off, PartSplit dst, hipStream_t s);
+void launch_spill_scatter_f64(const TileP &tp, PartF64 spill, int n, const int *tile_of,
+                              const long long *dst_off, int *fill, PartF64 dst, hipStream_t s);
+void launch_spill_scatter_split(const TileP &tp, PartSplit spill, int n, const int *tile_of,
+                                const long long *dst_off, int *fill, PartSplit dst, hipStream_t s);
+// split -> FP64 reference coordinates, in place into an FP64 staging buffer
+void launch_split_to_f64(const GridP &g, const PhysP &ph, PartSplit src, long long n, PartF64 dst,
+                         hipStream_t s);
+void launch_tile_ids(const TileP &tp, const long long *dst_off, long long n, int *tile_out, hipStream_t s);
+// diagnostics over the particles of every bin
+void launch_particle_diag_f64(const GridP &g, const PhysP &ph, const TileP &tp, PartF64 p, const long long *base,
+                              const int *cnt, double *partial, double *rho, hipStream_t s);
+void launch_particle_diag_split(const GridP &g, const PhysP &ph, const TileP &tp, PartSplit p,
+                                const long long *base, const int *cnt, double *partial, double *rho,
+                                hipStream_t s);
+
+}  // namespace pictor

@@ -1,0 +1,188 @@
+"""Python host of the gfx950 particle step, mirroring PICTOR's MAIN loop.
+
+The reference drives one time step by calling zero-argument subroutines in
+a fixed order (src/MAIN.F90:42-284).  `Simulation` exposes the same phases
+under the same names, each one a call through the C ABI of
+libpictor_hip.so; `Simulation.step()` is the fused form of one whole step.
+Arrays are numpy arrays in the reference layout: fields (mx, my, mz) in
+Fortran order, particles 1-D SoA.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FORM_FP32_SPLIT, FORM_FP64, PictorConfig, check, lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def _p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+DIAG_KEYS = ("B_energy", "E_energy", "prtl_energy_1", "prtl_energy_2", "px", "py", "pz", "gauss_residual", "rho_max")
+
+
+class Simulation:
+    """One periodic domain on one GPU (parameters.F90 as runtime settings).
+
+    form: "fp64" (reference SoA in FP64) or "fp32_split" (int32 cell +
+    FP32 offset, FP32 push/fields, fixed-point current).
+    """
+
+    def __init__(self, nx, ny, nz=1, ndim=2, epc=16, c=0.45, c_ompe=10.0, g0=1.0, mi_me=1.0,
+                 form="fp64", tile=0, capacity_factor=0.0):
+        self.cfg = PictorConfig(ndim, nx, ny, nz, epc, FORM_FP64 if form == "fp64" else FORM_FP32_SPLIT,
+                                tile, 0, c, c_ompe, g0, mi_me, capacity_factor)
+        if form not in ("fp64", "fp32_split"):
+            raise ValueError("form must be 'fp64' or 'fp32_split'")
+        self.form = form
+        self._ctx = ctypes.c_void_p()
+        check(lib().pictor_init(ctypes.byref(self.cfg), ctypes.byref(self._ctx)))
+        sc = np.zeros(16)
+        check(lib().pictor_get_scalars(self._ctx, _p(sc)))
+        self.qi = sc[0]
+        self.shape = (int(sc[9]), int(sc[10]), int(sc[11]))
+        self.tile = int(sc[12])
+        self.ntiles = int(sc[13])
+        self.t = 0
+
+    def close(self):
+        if self._ctx:
+            lib().pictor_finalize(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- state transfer (reference layout) ----
+    def upload_fields(self, Ex, Ey, Ez, Bx, By, Bz):
+        arrs = [np.asfortranarray(a, dtype=np.float64) for a in (Ex, Ey, Ez, Bx, By, Bz)]
+        for a in arrs:
+            if a.shape != self.shape:
+                raise ValueError(f"field shape {a.shape} != {self.shape}")
+        check(lib().pictor_upload_fields(self._ctx, *[_p(a) for a in arrs]))
+
+    def download_fields(self):
+        out = {n: np.zeros(self.shape, order="F") for n in ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz")}
+        check(lib().pictor_download_fields(self._ctx, *[_p(out[n]) for n in ("Ex", "Ey", "Ez", "Bx", "By", "Bz",
+                                                                              "Jx", "Jy", "Jz")]))
+        return out
+
+    def upload_particles(self, q, x, y, z, u, v, w, flv):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (q, x, y, z, u, v, w)]
+        f = np.ascontiguousarray(flv, dtype=np.int32)
+        n = len(f)
+        check(lib().pictor_upload_particles(self._ctx, n, *[_p(a) for a in arrs], _p(f, _ip)))
+
+    def particle_count(self):
+        n = ctypes.c_int64()
+        check(lib().pictor_particle_count(self._ctx, ctypes.byref(n)))
+        return n.value
+
+    def download_particles(self, with_tile=False):
+        n = self.particle_count()
+        out = {k: np.zeros(n) for k in ("q", "x", "y", "z", "u", "v", "w")}
+        out["flv"] = np.zeros(n, dtype=np.int32)
+        tile = np.zeros(n, dtype=np.int32) if with_tile else None
+        nout = ctypes.c_int64()
+        check(lib().pictor_download_particles(self._ctx, n, *[_p(out[k]) for k in ("q", "x", "y", "z", "u", "v", "w")],
+                                              _p(out["flv"], _ip), _p(tile, _ip), ctypes.byref(nout)))
+        if with_tile:
+            out["tile"] = tile
+        return out
+
+    # ---- the phases of MAIN.F90, same names as the reference ----
+    def UpdateBfieldHalf(self):          # em_update.F90:41-77 + SendRecvFlds(B)
+        check(lib().pictor_update_b_half(self._ctx))
+
+    def MoveDepositPrtl(self):           # movdep.F90:12-258
+        check(lib().pictor_move_deposit(self._ctx))
+
+    def ExchangePrtl(self):              # comm_prtl.F90:36-65 + AppendParticles
+        check(lib().pictor_exchange_particles(self._ctx))
+
+    def UpdateEfield(self):              # em_update.F90:9-39 + SendRecvFlds(E)
+        check(lib().pictor_update_e(self._ctx))
+
+    def UpdateCurrentAllFaces(self):     # comm_fld.F90:15-33
+        check(lib().pictor_current_halo_add(self._ctx))
+
+    def AddCurrent(self):                # em_update.F90:79-102 + SendRecvFlds(E)
+        check(lib().pictor_add_current(self._ctx))
+
+    def SendRecvFlds(self, which):       # comm_fld.F90:75-91; which "E" or "B"
+        check(lib().pictor_field_halo(self._ctx, 0 if which == "E" else 1))
+
+    def ResetCurrent(self):              # current.F90:13-17
+        check(lib().pictor_reset_current(self._ctx))
+
+    def ReorderPrtl(self):               # mem_prtl.F90:162-301
+        check(lib().pictor_sort_tiles(self._ctx))
+
+    def step_phases(self):
+        """One time step as the sequence of separate phase calls (MAIN.F90 order)."""
+        self.ResetCurrent()
+        self.UpdateBfieldHalf()
+        self.MoveDepositPrtl()
+        self.ExchangePrtl()
+        self.UpdateBfieldHalf()
+        self.UpdateEfield()
+        self.UpdateCurrentAllFaces()
+        self.AddCurrent()
+        self.t += 1
+
+    def step(self, n=1):
+        """n fused time steps."""
+        check(lib().pictor_step(self._ctx, n))
+        self.t += n
+
+    def synchronize(self):
+        check(lib().pictor_synchronize(self._ctx))
+
+    def diagnostics(self):
+        out = np.zeros(9)
+        check(lib().pictor_diagnostics(self._ctx, _p(out)))
+        return dict(zip(DIAG_KEYS, out.tolist()))
+
+    def timer_reset(self):
+        check(lib().pictor_timer_reset(self._ctx))
+
+    def timer(self, which):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(lib().pictor_timer_get(self._ctx, {"push": 0, "fields": 1, "step": 2}[which], ctypes.byref(ms),
+                                     ctypes.byref(n)))
+        return ms.value, n.value
+
+    def memory_bytes(self):
+        b = ctypes.c_int64()
+        check(lib().pictor_memory_bytes(self._ctx, ctypes.byref(b)))
+        return b.value
+
+
+def setup_weibel(nx, ny, nz=1, ndim=2, epc=16, drift=0.5, uth=0.01, seed=12345):
+    """Seeded counter-streaming pair plasma in the reference SoA (host arrays)."""
+    ncell = nx * ny * (nz if ndim == 3 else 1)
+    n = 2 * epc * ncell
+    out = {k: np.zeros(n) for k in ("q", "x", "y", "z", "u", "v", "w")}
+    out["flv"] = np.zeros(n, dtype=np.int32)
+    nout = ctypes.c_int64()
+    check(lib().pictor_setup_weibel(ndim, nx, ny, nz, epc, drift, uth, seed, n,
+                                    *[_p(out[k]) for k in ("q", "x", "y", "z", "u", "v", "w")],
+                                    _p(out["flv"], _ip), ctypes.byref(nout)))
+    return out
+
+
+__all__ = ["Simulation", "setup_weibel", "DIAG_KEYS", "FORM_FP64", "FORM_FP32_SPLIT", "_lib"]

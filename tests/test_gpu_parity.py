@@ -1,0 +1,188 @@
+"""HIP path vs the FP64 CPU oracle, through the C ABI (needs an MI355X).
+
+Bars: FP64 form bit-exact for fields after Yee/halo sweeps and for particle
+positions, momenta, cell indices and tile assignment after one push from
+identical state; current and multi-step fields within a stated tolerance
+(the current is summed in a different order on the GPU); Gauss's law at
+rounding level.  The FP32 split form is checked against the FP64 oracle with
+the tolerances written in each test, and for bitwise run-to-run determinism.
+"""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Oracle
+from pictor_amd import PictorError, Simulation, setup_weibel
+from tests._util import FIELDS, assert_same_multiset, rel_err, say, smooth_fields, sort_particles, tiles_of
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(nx, ny, epc, form="fp64", tile=8, seed=3, drift=0.5, uth=0.01, fields=True, c=0.45):
+    say(f"nx={nx} ny={ny} epc={epc} form={form} tile={tile} seed={seed}")
+    P = setup_weibel(nx, ny, epc=epc, drift=drift, uth=uth, seed=seed)
+    o = Oracle(2, nx, ny, epc=epc, c=c, cap=2 * len(P["x"]) + 4096)
+    o.load_particles(**P)
+    sim = Simulation(nx, ny, epc=epc, c=c, form=form, tile=tile)
+    if fields:
+        F = smooth_fields(o, seed=seed)
+        sim.upload_fields(*[F[n] for n in FIELDS])
+    sim.upload_particles(**P)
+    return o, sim, P
+
+
+def test_yee_and_halo_bitwise():
+    nx, ny = 40, 24
+    o = Oracle(2, nx, ny, epc=2)
+    F = smooth_fields(o, amp_e=1.0, amp_b=1.0, seed=11, modes=6)
+    rng = np.random.default_rng(5)
+    for n in FIELDS:   # add small-scale noise so that every stencil term matters
+        F[n][3:-2, 3:-2, :] += 1e-3 * rng.standard_normal(F[n][3:-2, 3:-2, :].shape)
+    o.set_fields(**F)
+    o.field_halo("E"); o.field_halo("B")
+    F = {n: o.F[n].copy(order="F") for n in FIELDS}
+    with Simulation(nx, ny, epc=2, form="fp64", tile=8) as sim:
+        sim.upload_fields(*[F[n] for n in FIELDS])
+        for _ in range(7):
+            sim.UpdateBfieldHalf()
+            o.update_b_half(); o.field_halo("B")
+            sim.UpdateEfield()
+            o.update_e(); o.field_halo("E")
+        G = sim.download_fields()
+    for n in FIELDS:
+        assert np.array_equal(G[n], o.F[n]), n
+
+
+def test_push_one_step_bitwise_fp64():
+    nx, ny = 32, 24
+    o, sim, P = _pair(nx, ny, epc=8, seed=7)
+    with sim:
+        sim.ResetCurrent(); sim.MoveDepositPrtl(); sim.ExchangePrtl()
+        o.reset_current(); o.move_deposit(); o.exchange()
+        G = sim.download_particles(with_tile=True)
+        assert_same_multiset(G, o.particles())
+        # particle -> tile assignment is a function of the cell index
+        assert np.array_equal(G["tile"], tiles_of(G, nx, ny, 8))
+        J = sim.download_fields()
+        for n in ("Jx", "Jy", "Jz"):
+            assert rel_err(J[n], o.F[n]) < 1e-13, n
+        sim.UpdateCurrentAllFaces(); o.current_halo_add()
+        J = sim.download_fields()
+        for n in ("Jx", "Jy", "Jz"):
+            assert rel_err(J[n], o.F[n]) < 1e-13, n
+
+
+def test_fused_step_equals_phases():
+    nx, ny = 32, 32
+    P = setup_weibel(nx, ny, epc=4, seed=2)
+    o = Oracle(2, nx, ny, epc=4)
+    F = smooth_fields(o, seed=2)
+    res = []
+    for fused in (False, True):
+        say(f"fused={fused}")
+        with Simulation(nx, ny, epc=4, form="fp64", tile=8) as sim:
+            sim.upload_fields(*[F[n] for n in FIELDS])
+            sim.upload_particles(**P)
+            for _ in range(3):
+                if fused:
+                    sim.step(1)
+                else:
+                    sim.step_phases()
+            res.append((sim.download_fields(), sim.download_particles()))
+    # the FP64 form sums the current with FP64 atomics, so two runs agree to rounding only
+    for n in FIELDS:
+        assert rel_err(res[0][0][n], res[1][0][n]) < 1e-12, n
+    a, b = sort_particles(res[0][1]), sort_particles(res[1][1])
+    for c in ("x", "y", "z", "u", "v", "w"):
+        assert np.max(np.abs(a[c] - b[c])) < 1e-12, c
+    assert np.array_equal(a["flv"], b["flv"])
+
+
+def test_steps_fp64_vs_oracle():
+    nx, ny = 32, 32
+    o, sim, P = _pair(nx, ny, epc=16, seed=4, fields=False)
+    nsteps = 40
+    with sim:
+        for t in range(1, nsteps + 1):
+            o.step(t)
+        sim.step(nsteps)
+        G = sim.download_fields()
+        d = sim.diagnostics()
+        eo = o.energy()
+        go = o.gauss()
+        for n in FIELDS:
+            assert rel_err(G[n], o.F[n]) < 1e-9, n
+        assert abs(d["B_energy"] - eo[0]) <= 1e-9 * eo[0]
+        assert abs(d["E_energy"] - eo[1]) <= 1e-9 * eo[1]
+        assert abs(d["prtl_energy_1"] - eo[2]) <= 1e-12 * eo[2]
+        assert abs(d["prtl_energy_2"] - eo[3]) <= 1e-12 * eo[3]
+        # Gauss's law at rounding level (relative to the largest node charge)
+        assert d["gauss_residual"] <= 1e-12 * d["rho_max"], d
+        assert go[0] <= 1e-12 * go[1]
+        assert sim.particle_count() == len(P["x"])
+
+
+def test_split_vs_fp64_oracle():
+    nx, ny = 32, 32
+    o, sim, P = _pair(nx, ny, epc=16, form="fp32_split", seed=4, fields=False)
+    nsteps = 40
+    with sim:
+        for t in range(1, nsteps + 1):
+            o.step(t)
+        sim.step(nsteps)
+        d = sim.diagnostics()
+        eo = o.energy()
+        say("split vs fp64:", d, eo)
+        # FP32 particle/field arithmetic: field energy within 1e-3 relative after 40 steps,
+        # particle kinetic energy within 1e-5, Gauss residual within 1e-6 of max |rho|
+        assert abs(d["B_energy"] - eo[0]) <= 1e-3 * eo[0]
+        assert abs(d["E_energy"] - eo[1]) <= 1e-3 * eo[1]
+        assert abs(d["prtl_energy_1"] - eo[2]) <= 1e-5 * eo[2]
+        assert d["gauss_residual"] <= 1e-6 * d["rho_max"], d
+        assert sim.particle_count() == len(P["x"])
+
+
+def test_split_deterministic():
+    nx, ny = 64, 32
+    P = setup_weibel(nx, ny, epc=8, seed=9)
+    res = []
+    for rep in range(2):
+        say(f"rep={rep}")
+        with Simulation(nx, ny, epc=8, form="fp32_split", tile=8) as sim:
+            sim.upload_particles(**P)
+            sim.step(25)
+            res.append((sim.download_fields(), sim.diagnostics()))
+    for n in ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz"):
+        assert np.array_equal(res[0][0][n], res[1][0][n]), n
+    for k in ("B_energy", "E_energy"):
+        assert res[0][1][k] == res[1][1][k]
+
+
+def test_spill_relayout_keeps_particles():
+    # every particle starts in one 8x8 tile and streams into empty tiles: the
+    # bins of the empty tiles overflow, particles spill and the layout is rebuilt
+    nx, ny = 32, 32
+    n = 4000
+    rng = np.random.default_rng(1)
+    P = {"q": np.where(np.arange(n) % 2 == 0, 1.0, -1.0), "x": 3 + rng.uniform(0, 8, n), "y": 3 + rng.uniform(0, 8, n),
+         "z": 1 + rng.uniform(0, 1, n), "u": np.full(n, 0.8), "v": np.full(n, 0.6), "w": np.zeros(n),
+         "flv": np.where(np.arange(n) % 2 == 0, 1, 2).astype(np.int32)}
+    o = Oracle(2, nx, ny, epc=4, cap=3 * n)
+    o.load_particles(**P)
+    with Simulation(nx, ny, epc=4, form="fp64", tile=8) as sim:
+        sim.upload_particles(**P)
+        for t in range(1, 31):
+            say(f"step {t}")
+            sim.step(1)
+            o.step(t)
+            assert sim.particle_count() == n
+        G = sim.download_particles()
+        assert np.allclose(np.sort(G["x"]), np.sort(o.particles()["x"]), rtol=0, atol=1e-9)
+
+
+def test_upload_rejects_out_of_domain():
+    with Simulation(16, 16, epc=2, form="fp64", tile=8) as sim:
+        P = setup_weibel(16, 16, epc=2, seed=1)
+        P["x"][5] = 0.5
+        with pytest.raises(PictorError) as e:
+            sim.upload_particles(**P)
+        assert e.value.code == 1
