@@ -37,10 +37,10 @@ BYTES_PER_PARTICLE = {"fp32_split": 2 * (4 + 3 * 4 + 3 * 4), "fp64": 7 * 8 + 4 +
 GRID_BYTES = {"fp32_split": 6 * 4 + 3 * 4, "fp64": 6 * 8 + 3 * 8}
 
 
-def run_gpu(form, nx, ny, epc, P, steps, warmup):
+def run_gpu(form, nx, ny, epc, P, steps, warmup, calibrate=False, tile=0):
     from pictor_amd import Simulation
 
-    sim = Simulation(nx, ny, epc=epc, c=0.45, c_ompe=10.0, form=form)
+    sim = Simulation(nx, ny, epc=epc, c=0.45, c_ompe=10.0, form=form, tile=tile)
     t0 = time.time()
     sim.upload_particles(**P)
     sim.synchronize()
@@ -58,6 +58,11 @@ def run_gpu(form, nx, ny, epc, P, steps, warmup):
     step_ms, _ = sim.timer("step")
     diag = sim.diagnostics()
     mem = sim.memory_bytes()
+    if calibrate:
+        # plain streaming kernels of known size for scaling HBM counters:
+        # FP32 -> FP64 field conversion (read 4 B, write 8 B per cell) and
+        # current scaling (read 8 B, write 8 B per cell), 9 arrays of (mx*my)
+        sim.download_fields()
     tile = sim.tile
     sim.close()
     avg_push = push_ms / max(npush, 1) / 1e3
@@ -102,6 +107,8 @@ def main():
     ap.add_argument("--forms", default="fp32_split,fp64")
     ap.add_argument("--cpu-steps", type=int, default=-1, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--calibrate", action="store_true", help="also run plain copy kernels (counter scaling)")
+    ap.add_argument("--tile", type=int, default=0, help="tile edge in cells (0 = library default)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -120,13 +127,13 @@ def main():
     forms = [f for f in args.forms.split(",") if f]
     res = {}
     for form in forms:
-        res[form] = run_gpu(form, nx, ny, epc, P, args.steps, args.warmup)
+        res[form] = run_gpu(form, nx, ny, epc, P, args.steps, args.warmup, args.calibrate, args.tile)
         r = res[form]
         print(f"# {form}: {r['value']:.4e} particle-updates/s, {r['ms_per_step']:.3f} ms/step, "
               f"push {r['push_ms']:.3f} ms ({r['roofline']['frac'] * 100:.1f}% HBM)", file=sys.stderr, flush=True)
 
     cpu = None
-    cpu_steps = args.cpu_steps if args.cpu_steps >= 0 else (2 if nx >= 2048 else 20)
+    cpu_steps = args.cpu_steps if args.cpu_steps >= 0 else (5 if nx >= 2048 else 20)
     if cpu_steps > 0:
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
         v, dt = run_cpu(nx, ny, epc, P, cpu_steps, threads)

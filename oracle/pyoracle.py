@@ -164,7 +164,7 @@ class Oracle:
         lib().orc_reset_current(ctypes.byref(self.p), ctypes.byref(self.f))
 
     def field_halo(self, which):
-        names = ("Ex", "Ey", "Ez") if which in ("E", 0) else ("Bx", "By", "Bz")
+        names = {"E": ("Ex", "Ey", "Ez"), 0: ("Ex", "Ey", "Ez"), "J": ("Jx", "Jy", "Jz")}.get(which, ("Bx", "By", "Bz"))
         lib().orc_field_halo(ctypes.byref(self.p), *[_ptr(self.F[n]) for n in names])
 
     def current_halo_add(self):

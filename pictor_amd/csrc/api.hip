@@ -266,9 +266,10 @@ int relayout(pictor_ctx *c, int nspill)
     std::vector<long long> counts;
     int rc = group_counts(c, c->cur, counts);
     if (rc) return rc;
-    int *d_gcount = nullptr, *d_grp = nullptr, *d_fill = nullptr;
+    int *d_gcount = nullptr, *d_grp = nullptr, *d_fill = nullptr, *d_gtotal = nullptr;
     long long *d_off = nullptr;
     HIPTRY(dalloc(&d_gcount, ng));
+    HIPTRY(dalloc(&d_gtotal, ng));
     HIPTRY(dalloc(&d_fill, ng));
     HIPTRY(dalloc(&d_off, ng + 1));
     HIPTRY(dalloc(&d_grp, std::max(nspill, 1)));
@@ -292,15 +293,16 @@ int relayout(pictor_ctx *c, int nspill)
     std::vector<long long> off = group_offsets(c, false, total);
     HIPTRY(hipMemcpyAsync(d_off, off.data(), (ng + 1) * sizeof(long long), hipMemcpyHostToDevice, c->stream));
     // fill starts after the compacted bins of each group
-    std::vector<int> fill(ng);
-    for (size_t gi = 0; gi < ng; gi++) fill[gi] = (int)counts[gi];
+    std::vector<int> fill(ng), gtot(ng);
+    for (size_t gi = 0; gi < ng; gi++) { fill[gi] = (int)counts[gi]; gtot[gi] = (int)total[gi]; }
     HIPTRY(hipMemcpyAsync(d_fill, fill.data(), ng * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemcpyAsync(d_gtotal, gtot.data(), ng * sizeof(int), hipMemcpyHostToDevice, c->stream));
     if (c->form == PICTOR_FORM_FP64) {
-        launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], d_off, c->pf[dst], c->stream);
-        launch_spill_scatter_f64(tp, c->spill_f, nspill, d_grp, d_off, d_fill, c->pf[dst], c->stream);
+        launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], d_off, d_gtotal, c->pf[dst], c->stream);
+        launch_spill_scatter_f64(tp, c->spill_f, nspill, d_grp, d_off, d_fill, d_gtotal, c->pf[dst], c->stream);
     } else {
-        launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], d_off, c->ps[dst], c->stream);
-        launch_spill_scatter_split(tp, c->spill_s, nspill, d_grp, d_off, d_fill, c->ps[dst], c->stream);
+        launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], d_off, d_gtotal, c->ps[dst], c->stream);
+        launch_spill_scatter_split(tp, c->spill_s, nspill, d_grp, d_off, d_fill, d_gtotal, c->ps[dst], c->stream);
     }
     HIPTRY(hipGetLastError());
     rc = upload_layout(c);
@@ -309,7 +311,7 @@ int relayout(pictor_ctx *c, int nspill)
     if (rc) return rc;
     HIPTRY(hipMemsetAsync(c->d_spill_count, 0, sizeof(int), c->stream));
     HIPTRY(hipStreamSynchronize(c->stream));
-    dfree(d_gcount); dfree(d_fill); dfree(d_off); dfree(d_grp);
+    dfree(d_gcount); dfree(d_fill); dfree(d_off); dfree(d_grp); dfree(d_gtotal);
     c->cur = dst;
     // the other buffer must hold the new layout too
     rc = ensure_buffer(c, 1 - dst, c->layout_cap);
@@ -330,6 +332,8 @@ int commit_push(pictor_ctx *c)
     c->pending_swap = false;
     const int nspill = c->h_flags[0], err = c->h_flags[1];
     if (err & kErrBounds) return fail(PICTOR_ERR_BOUNDS, "a particle was outside its tile window or moved more than one tile");
+    if (err & kErrPrecision)
+        return fail(PICTOR_ERR_CAPACITY, "current at a node exceeded the exact fixed-point range (|J| >= 4096 q cell)");
     if ((err & kErrCapacity) || nspill > c->spill_cap)
         return fail(PICTOR_ERR_CAPACITY, "particle spill buffer exhausted");
     if (nspill > 0) return relayout(c, nspill);
@@ -376,9 +380,9 @@ int do_push(pictor_ctx *c, bool timed)
         launch_push2d_f64(c->tp.T, c->g, c->ph, c->tp, E, B, J, c->pf[a], c->pf[b], bins, c->spill_f, sp, c->stream);
     } else {
         float *E[3], *B[3];
-        long long *J[3];
+        double *J[3];
         field_ptrs(c, E, B);
-        for (int i = 0; i < 3; i++) J[i] = (long long *)c->J[i];
+        for (int i = 0; i < 3; i++) J[i] = (double *)c->J[i];
         launch_push2d_split(c->tp.T, c->g, c->ph, c->tp, E, B, J, c->ps[a], c->ps[b], bins, c->spill_s, sp, c->stream);
     }
     HIPTRY(hipGetLastError());
@@ -389,25 +393,16 @@ int do_push(pictor_ctx *c, bool timed)
 
 int do_reset_current(pictor_ctx *c)
 {
-    for (int i = 0; i < 3; i++) {
-        if (c->form == PICTOR_FORM_FP64) launch_fill_zero<double>((double *)c->J[i], c->g.ncell, c->stream);
-        else launch_fill_zero<long long>((long long *)c->J[i], c->g.ncell, c->stream);
-    }
+    for (int i = 0; i < 3; i++) launch_fill_zero<double>((double *)c->J[i], c->g.ncell, c->stream);
     HIPTRY(hipGetLastError());
     return PICTOR_OK;
 }
 
 int do_fold(pictor_ctx *c)
 {
-    if (c->form == PICTOR_FORM_FP64) {
-        double *J[3];
-        for (int i = 0; i < 3; i++) J[i] = (double *)c->J[i];
-        launch_current_fold<double>(c->g, J, c->stream);
-    } else {
-        long long *J[3];
-        for (int i = 0; i < 3; i++) J[i] = (long long *)c->J[i];
-        launch_current_fold<long long>(c->g, J, c->stream);
-    }
+    double *J[3];
+    for (int i = 0; i < 3; i++) J[i] = (double *)c->J[i];
+    launch_current_fold<double>(c->g, J, c->stream);
     HIPTRY(hipGetLastError());
     return PICTOR_OK;
 }
@@ -426,12 +421,12 @@ int do_e_phase(pictor_ctx *c, bool curl, bool add)
         launch_ghost_copy<double>(c->g, E, c->stream);
     } else {
         float *E[3], *B[3];
-        long long *J[3];
+        double *J[3];
         field_ptrs(c, E, B);
-        for (int i = 0; i < 3; i++) J[i] = (long long *)c->J[i];
-        if (curl && add) launch_update_e_add_current<float, long long>(c->g, c->ph, E, B, J, jscale(c), c->stream);
+        for (int i = 0; i < 3; i++) J[i] = (double *)c->J[i];
+        if (curl && add) launch_update_e_add_current<float, double>(c->g, c->ph, E, B, J, jscale(c), c->stream);
         else if (curl) launch_update_e<float>(c->g, c->ph, E, B, c->stream);
-        else launch_add_current<float, long long>(c->g, c->ph, E, J, jscale(c), c->stream);
+        else launch_add_current<float, double>(c->g, c->ph, E, J, jscale(c), c->stream);
         launch_ghost_copy<float>(c->g, E, c->stream);
     }
     HIPTRY(hipGetLastError());
@@ -468,6 +463,7 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     *out = nullptr;
     if (cfg->ndim != 2) return fail(PICTOR_ERR_ARG, "only ndim = 2 is available");
     if (cfg->nx < 8 || cfg->ny < 8) return fail(PICTOR_ERR_ARG, "nx, ny must be >= 8");
+    if (cfg->ndim == 2 && (cfg->nx > 32768 || cfg->ny > 32768)) return fail(PICTOR_ERR_ARG, "2-D nx, ny must be <= 32768");
     if (cfg->form != PICTOR_FORM_FP64 && cfg->form != PICTOR_FORM_FP32_SPLIT) return fail(PICTOR_ERR_ARG, "unknown form");
     if (!(cfg->c > 0.0 && cfg->c < 1.0)) return fail(PICTOR_ERR_ARG, "Courant number c must lie in (0,1)");
     if (cfg->epc < 1 || cfg->c_ompe <= 0 || cfg->mi_me <= 0) return fail(PICTOR_ERR_ARG, "invalid epc, c_ompe or mi_me");
@@ -630,7 +626,7 @@ int pictor_download_fields(pictor_ctx *c, double *ex, double *ey, double *ez, do
             HIPTRY(hipMemcpyAsync(dst[i], src[i], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         } else {
             if (i < 6) launch_convert_fields_f64((const float *)src[i], tmp, (long long)n, c->stream);
-            else launch_convert_jfix((const long long *)src[i], tmp, (long long)n, jscale(c), c->stream);
+            else launch_convert_jfix((const double *)src[i], tmp, (long long)n, jscale(c), c->stream);
             HIPTRY(hipMemcpyAsync(dst[i], tmp, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
             HIPTRY(hipStreamSynchronize(c->stream));
         }
@@ -691,9 +687,9 @@ int pictor_upload_particles(pictor_ctx *c, int64_t n, const double *q, const dou
     HIPTRY(hipMemsetAsync(d_fill, 0, ng * sizeof(int), c->stream));
     c->cur = 0;
     if (c->form == PICTOR_FORM_FP64)
-        launch_raw_scatter_f64(c->g, tp, raw, n, d_tile, d_off, d_fill, c->pf[0], c->stream);
+        launch_raw_scatter_f64(c->g, tp, raw, n, d_tile, d_off, d_fill, d_count, c->pf[0], c->stream);
     else
-        launch_raw_scatter_split(c->g, c->ph, tp, raw, n, d_tile, d_off, d_fill, c->ps[0], c->stream);
+        launch_raw_scatter_split(c->g, c->ph, tp, raw, n, d_tile, d_off, d_fill, d_count, c->ps[0], c->stream);
     HIPTRY(hipGetLastError());
     rc = upload_layout(c);
     if (!rc) rc = upload_center_counts(c, 0, counts);
@@ -742,10 +738,10 @@ int pictor_download_particles(pictor_ctx *c, int64_t cap, double *q, double *x, 
     HIPTRY(alloc_f64(st, n));
     HIPTRY(hipMemcpyAsync(d_off, off.data(), (ng + 1) * sizeof(long long), hipMemcpyHostToDevice, c->stream));
     if (c->form == PICTOR_FORM_FP64) {
-        launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], d_off, st, c->stream);
+        launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], d_off, nullptr, st, c->stream);
     } else {
         HIPTRY(alloc_split(ss, n, true));
-        launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], d_off, ss, c->stream);
+        launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], d_off, nullptr, ss, c->stream);
         launch_split_to_f64(c->g, c->ph, ss, n, st, c->stream);
     }
     if (tile) {

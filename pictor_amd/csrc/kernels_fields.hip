@@ -53,15 +53,11 @@ __device__ inline double jvalue<double, double>(const double *J, long long o, do
 {
     return J[o];
 }
+// split form: J holds exact sums of contributions quantized to 2^-40 (charge x cell)
 template <>
-__device__ inline float jvalue<float, double>(const double *J, long long o, double)
+__device__ inline float jvalue<float, double>(const double *J, long long o, double scale)
 {
-    return (float)J[o];
-}
-template <>
-__device__ inline float jvalue<float, long long>(const long long *J, long long o, double scale)
-{
-    return (float)((double)J[o] * scale);
+    return (float)(J[o] * scale);
 }
 
 // em_update.F90:9-39 (E full step); with ADD, AddCurrent (em_update.F90:79-102)
@@ -240,7 +236,7 @@ __global__ void k_cvt_f64(const float *s, double *d, long long n)
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) d[t] = (double)s[t];
 }
-__global__ void k_cvt_jfix(const long long *s, double *d, long long n, double scale)
+__global__ void k_cvt_jfix(const double *s, double *d, long long n, double scale)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) d[t] = (double)s[t] * scale;
@@ -332,7 +328,7 @@ void launch_convert_fields_f64(const float *src, double *dst, long long n, hipSt
 {
     hipLaunchKernelGGL(k_cvt_f64, dim3(nblocks(n, 256)), dim3(256), 0, s, src, dst, n);
 }
-void launch_convert_jfix(const long long *src, double *dst, long long n, double scale, hipStream_t s)
+void launch_convert_jfix(const double *src, double *dst, long long n, double scale, hipStream_t s)
 {
     hipLaunchKernelGGL(k_cvt_jfix, dim3(nblocks(n, 256)), dim3(256), 0, s, src, dst, n, scale);
 }
@@ -343,17 +339,15 @@ template void launch_update_e<double>(const GridP &, const PhysP &, double *cons
 template void launch_update_e<float>(const GridP &, const PhysP &, float *const[3], float *const[3], hipStream_t);
 template void launch_update_e_add_current<double, double>(const GridP &, const PhysP &, double *const[3],
                                                           double *const[3], double *const[3], double, hipStream_t);
-template void launch_update_e_add_current<float, long long>(const GridP &, const PhysP &, float *const[3],
-                                                            float *const[3], long long *const[3], double,
-                                                            hipStream_t);
+template void launch_update_e_add_current<float, double>(const GridP &, const PhysP &, float *const[3],
+                                                         float *const[3], double *const[3], double, hipStream_t);
 template void launch_add_current<double, double>(const GridP &, const PhysP &, double *const[3], double *const[3],
                                                  double, hipStream_t);
-template void launch_add_current<float, long long>(const GridP &, const PhysP &, float *const[3],
-                                                   long long *const[3], double, hipStream_t);
+template void launch_add_current<float, double>(const GridP &, const PhysP &, float *const[3], double *const[3],
+                                                double, hipStream_t);
 template void launch_ghost_copy<double>(const GridP &, double *const[3], hipStream_t);
 template void launch_ghost_copy<float>(const GridP &, float *const[3], hipStream_t);
 template void launch_current_fold<double>(const GridP &, double *const[3], hipStream_t);
-template void launch_current_fold<long long>(const GridP &, long long *const[3], hipStream_t);
 template void launch_fill_zero<double>(double *, long long, hipStream_t);
 template void launch_fill_zero<float>(float *, long long, hipStream_t);
 template void launch_fill_zero<long long>(long long *, long long, hipStream_t);

@@ -11,6 +11,15 @@
 namespace pictor {
 namespace {
 
+// Bijective stride permutation of [0, n): 1000003 is prime and larger than any
+// group, so it is coprime with n.  Neighbouring slots (same cell) land far apart.
+__device__ inline long long perm_slot(long long s, const int *gtotal, int grp)
+{
+    if (!gtotal) return s;
+    const long long n = gtotal[grp];
+    return n > 1 ? (s * 1000003LL) % n : s;
+}
+
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 __device__ inline int wrapi(int v, int n)
@@ -77,20 +86,20 @@ __global__ void k_raw_tiles(GridP g, PhysP ph, TileP tp, PartF64 raw, long long 
 }
 
 __global__ void k_raw_scatter_f64(TileP tp, PartF64 raw, long long n, const int *tile_of, const long long *cbase,
-                                  int *fill, PartF64 out)
+                                  int *fill, const int *gtotal, PartF64 out)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int t = tile_of[i];
     if (t < 0) return;
     const int slot = atomicAdd(&fill[t], 1);
-    const long long o = cbase[t] + slot;
+    const long long o = cbase[t] + perm_slot(slot, gtotal, t);
     out.q[o] = raw.q[i]; out.x[o] = raw.x[i]; out.y[o] = raw.y[i]; out.z[o] = raw.z[i];
     out.u[o] = raw.u[i]; out.v[o] = raw.v[i]; out.w[o] = raw.w[i]; out.flv[o] = raw.flv[i];
 }
 
 __global__ void k_raw_scatter_split(GridP g, TileP tp, PartF64 raw, long long n, const int *tile_of,
-                                    const long long *cbase, int *fill, PartSplit out)
+                                    const long long *cbase, int *fill, const int *gtotal, PartSplit out)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -105,8 +114,8 @@ __global__ void k_raw_scatter_split(GridP g, TileP tp, PartF64 raw, long long n,
     else { const double f = floor(raw.z[i]); oz = (float)(raw.z[i] - f); if (oz >= 1.0f) oz = 0.0f; }
     const int grp = t;
     const int slot = atomicAdd(&fill[grp], 1);
-    const long long o = cbase[grp] + slot;
-    out.cell[o] = cx + g.nx * (cy + g.ny * cz);
+    const long long o = cbase[grp] + perm_slot(slot, gtotal, grp);
+    out.cell[o] = cell_pack(g.ndim, cx, cy, cz);
     out.ox[o] = ox; out.oy[o] = oy; out.oz[o] = oz;
     out.u[o] = (float)raw.u[i]; out.v[o] = (float)raw.v[i]; out.w[o] = (float)raw.w[i];
     (void)s;
@@ -125,9 +134,8 @@ __global__ void k_spill_tiles_split(GridP g, TileP tp, PartSplit sp, int n, int 
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int cell = sp.cell[i];
-    const int cx = cell % g.nx, r = cell / g.nx;
-    const int cy = r % g.ny, cz = r / g.ny;
+    int cx, cy, cz;
+    cell_unpack(g.ndim, sp.cell[i], cx, cy, cz);
     const int t = (cz / tp.T * tp.nty + cy / tp.T) * tp.ntx + cx / tp.T;
     const int grp = t * tp.nsp + sp.sp[i];
     tile_out[i] = grp;
@@ -136,7 +144,8 @@ __global__ void k_spill_tiles_split(GridP g, TileP tp, PartSplit sp, int n, int 
 
 // one workgroup per (tile, species) group: its ND bins -> one dense run at dst_off[group]
 template <typename CopyF>
-__device__ inline void compact_group(const TileP &tp, const long long *base, const int *cnt, long long dst0, CopyF copy)
+__device__ inline void compact_group(const TileP &tp, const long long *base, const int *cnt, long long dst0,
+                                     const int *gtotal, CopyF copy)
 {
     __shared__ int sStart[28];
     __shared__ long long sBase[27];
@@ -153,24 +162,24 @@ __device__ inline void compact_group(const TileP &tp, const long long *base, con
     for (int k = threadIdx.x; k < total; k += blockDim.x) {
         int d = 0;
         while (k >= sStart[d + 1]) d++;
-        copy(sBase[d] + (k - sStart[d]), dst0 + k);
+        copy(sBase[d] + (k - sStart[d]), dst0 + perm_slot(k, gtotal, grp));
     }
 }
 
 __global__ void k_compact_f64(TileP tp, PartF64 src, const long long *base, const int *cnt, const long long *dst_off,
-                              PartF64 dst)
+                              const int *gtotal, PartF64 dst)
 {
-    compact_group(tp, base, cnt, dst_off[blockIdx.x], [&](long long a, long long b) {
+    compact_group(tp, base, cnt, dst_off[blockIdx.x], gtotal, [&](long long a, long long b) {
         dst.q[b] = src.q[a]; dst.x[b] = src.x[a]; dst.y[b] = src.y[a]; dst.z[b] = src.z[a];
         dst.u[b] = src.u[a]; dst.v[b] = src.v[a]; dst.w[b] = src.w[a]; dst.flv[b] = src.flv[a];
     });
 }
 
 __global__ void k_compact_split(TileP tp, PartSplit src, const long long *base, const int *cnt,
-                                const long long *dst_off, PartSplit dst)
+                                const long long *dst_off, const int *gtotal, PartSplit dst)
 {
     const int s = blockIdx.x % tp.nsp;
-    compact_group(tp, base, cnt, dst_off[blockIdx.x], [&](long long a, long long b) {
+    compact_group(tp, base, cnt, dst_off[blockIdx.x], gtotal, [&](long long a, long long b) {
         dst.cell[b] = src.cell[a]; dst.ox[b] = src.ox[a]; dst.oy[b] = src.oy[a]; dst.oz[b] = src.oz[a];
         dst.u[b] = src.u[a]; dst.v[b] = src.v[a]; dst.w[b] = src.w[a];
         if (dst.sp) dst.sp[b] = s;
@@ -178,23 +187,23 @@ __global__ void k_compact_split(TileP tp, PartSplit src, const long long *base, 
 }
 
 __global__ void k_spill_scatter_f64(PartF64 sp, int n, const int *grp_of, const long long *dst_off, int *fill,
-                                    PartF64 dst)
+                                    const int *gtotal, PartF64 dst)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int grp = grp_of[i];
-    const long long o = dst_off[grp] + atomicAdd(&fill[grp], 1);
+    const long long o = dst_off[grp] + perm_slot(atomicAdd(&fill[grp], 1), gtotal, grp);
     dst.q[o] = sp.q[i]; dst.x[o] = sp.x[i]; dst.y[o] = sp.y[i]; dst.z[o] = sp.z[i];
     dst.u[o] = sp.u[i]; dst.v[o] = sp.v[i]; dst.w[o] = sp.w[i]; dst.flv[o] = sp.flv[i];
 }
 
 __global__ void k_spill_scatter_split(PartSplit sp, int n, const int *grp_of, const long long *dst_off, int *fill,
-                                      PartSplit dst)
+                                      const int *gtotal, PartSplit dst)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int grp = grp_of[i];
-    const long long o = dst_off[grp] + atomicAdd(&fill[grp], 1);
+    const long long o = dst_off[grp] + perm_slot(atomicAdd(&fill[grp], 1), gtotal, grp);
     dst.cell[o] = sp.cell[i]; dst.ox[o] = sp.ox[i]; dst.oy[o] = sp.oy[i]; dst.oz[o] = sp.oz[i];
     dst.u[o] = sp.u[i]; dst.v[o] = sp.v[i]; dst.w[o] = sp.w[i];
 }
@@ -204,9 +213,8 @@ __global__ void k_split_to_f64(GridP g, PhysP ph, PartSplit src, long long n, Pa
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int cell = src.cell[i];
-    const int cx = cell % g.nx, r = cell / g.nx;
-    const int cy = r % g.ny, cz = r / g.ny;
+    int cx, cy, cz;
+    cell_unpack(g.ndim, src.cell[i], cx, cy, cz);
     const int s = src.sp[i];
     dst.x[i] = (double)(3 + cx) + (double)src.ox[i];
     dst.y[i] = (double)(3 + cy) + (double)src.oy[i];
@@ -308,9 +316,8 @@ __global__ __launch_bounds__(256) void k_particle_diag_split(GridP g, PhysP ph, 
     particle_diag_tile(g, ph, tp, base, cnt, partial, rho,
                        [&](long long o, int s, double &x, double &y, double &z, double &u, double &v, double &w,
                            double &q, int &flv) {
-                           const int cell = p.cell[o];
-                           const int cx = cell % g.nx, r = cell / g.nx;
-                           const int cy = r % g.ny, cz = r / g.ny;
+                           int cx, cy, cz;
+                           cell_unpack(g.ndim, p.cell[o], cx, cy, cz);
                            x = (double)(3 + cx) + (double)p.ox[o];
                            y = (double)(3 + cy) + (double)p.oy[o];
                            z = (g.ndim == 3) ? (double)(3 + cz) + (double)p.oz[o] : 1.0 + (double)p.oz[o];
@@ -332,18 +339,20 @@ void launch_raw_tiles_f64(const GridP &g, const PhysP &ph, const TileP &tp, Part
 }
 
 void launch_raw_scatter_f64(const GridP &, const TileP &tp, PartF64 raw, long long n, const int *tile_of,
-                            const long long *base, int *fill, PartF64 out, hipStream_t s)
+                            const long long *base, int *fill, const int *gtotal, PartF64 out, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_raw_scatter_f64, dim3(nblk(n, 256)), dim3(256), 0, s, tp, raw, n, tile_of, base, fill, out);
+    hipLaunchKernelGGL(k_raw_scatter_f64, dim3(nblk(n, 256)), dim3(256), 0, s, tp, raw, n, tile_of, base, fill, gtotal,
+                       out);
 }
 
 void launch_raw_scatter_split(const GridP &g, const PhysP &, const TileP &tp, PartF64 raw, long long n,
-                              const int *tile_of, const long long *base, int *fill, PartSplit out, hipStream_t s)
+                              const int *tile_of, const long long *base, int *fill, const int *gtotal, PartSplit out,
+                              hipStream_t s)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_raw_scatter_split, dim3(nblk(n, 256)), dim3(256), 0, s, g, tp, raw, n, tile_of, base, fill,
-                       out);
+                       gtotal, out);
 }
 
 void launch_spill_tiles_f64(const GridP &g, const TileP &tp, PartF64 spill, int n, int *tile_out, int *counts,
@@ -361,31 +370,33 @@ void launch_spill_tiles_split(const GridP &g, const TileP &tp, PartSplit spill, 
 }
 
 void launch_compact_f64(const TileP &tp, PartF64 src, const long long *base, const int *cnt, const long long *dst_off,
-                        PartF64 dst, hipStream_t s)
+                        const int *gtotal, PartF64 dst, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_compact_f64, dim3(tp.ntiles * tp.nsp), dim3(256), 0, s, tp, src, base, cnt, dst_off, dst);
+    hipLaunchKernelGGL(k_compact_f64, dim3(tp.ntiles * tp.nsp), dim3(256), 0, s, tp, src, base, cnt, dst_off, gtotal,
+                       dst);
 }
 
 void launch_compact_split(const TileP &tp, PartSplit src, const long long *base, const int *cnt,
-                          const long long *dst_off, PartSplit dst, hipStream_t s)
+                          const long long *dst_off, const int *gtotal, PartSplit dst, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_compact_split, dim3(tp.ntiles * tp.nsp), dim3(256), 0, s, tp, src, base, cnt, dst_off, dst);
+    hipLaunchKernelGGL(k_compact_split, dim3(tp.ntiles * tp.nsp), dim3(256), 0, s, tp, src, base, cnt, dst_off, gtotal,
+                       dst);
 }
 
 void launch_spill_scatter_f64(const TileP &, PartF64 spill, int n, const int *tile_of, const long long *dst_off,
-                              int *fill, PartF64 dst, hipStream_t s)
+                              int *fill, const int *gtotal, PartF64 dst, hipStream_t s)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_spill_scatter_f64, dim3(nblk(n, 256)), dim3(256), 0, s, spill, n, tile_of, dst_off, fill,
-                       dst);
+                       gtotal, dst);
 }
 
 void launch_spill_scatter_split(const TileP &, PartSplit spill, int n, const int *tile_of, const long long *dst_off,
-                                int *fill, PartSplit dst, hipStream_t s)
+                                int *fill, const int *gtotal, PartSplit dst, hipStream_t s)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_spill_scatter_split, dim3(nblk(n, 256)), dim3(256), 0, s, spill, n, tile_of, dst_off, fill,
-                       dst);
+                       gtotal, dst);
 }
 
 void launch_split_to_f64(const GridP &g, const PhysP &ph, PartSplit src, long long n, PartF64 dst, hipStream_t s)

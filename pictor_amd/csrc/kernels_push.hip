@@ -16,9 +16,12 @@
 // FP64 form: positions are the reference's local cell coordinates, the push
 // follows the reference expression order (bitwise with -ffp-contract=off) and
 // the current window accumulates with LDS FP64 atomics.
-// Split form: int32 cell + FP32 offsets, FP32 push, and the current window is
-// int64 fixed point (2^40 units per charge*cell) so that accumulation is exact
-// and the result does not depend on particle order: it is deterministic.
+// Split form: int32 cell + FP32 offsets, FP32 push; every current contribution
+// is quantised to a multiple of 2^-36 (charge x cell) and accumulated in FP64,
+// where all such sums are exact, so the current does not depend on the order
+// of the atomics: the split form is deterministic.  Both species of a tile go
+// through one pass, and each thread prefetches its next particle while it
+// pushes the current one.
 #include "pictor_common.h"
 
 namespace pictor {
@@ -278,28 +281,58 @@ __global__ __launch_bounds__(BS) void k_push2d_f64(GridP g, PhysP ph, TileP tp, 
 // ---------------------------------------------------------------------------
 // FP32 split form, 2-D
 // ---------------------------------------------------------------------------
-__device__ inline unsigned long long fixp(float v)
+// FP32 -> exact integer count of 2^-36 units (the argument is already scaled by 2^36)
+__device__ inline double quant(float v) { return (double)__builtin_rintf(v); }
+
+// byte-offset accessors: a uniform base plus a 32-bit lane offset lets the
+// compiler use SGPR-base addressing (one VALU op per access)
+template <typename T>
+__device__ inline T ldo(const T *base, unsigned off)
 {
-    return (unsigned long long)__float2ll_rn(v * 1099511627776.0f);   // v * 2^40, exact scaling
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <typename T>
+__device__ inline void sto(T *base, unsigned off, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
+struct SplitRec {
+    int cell, s;
+    float ox, oy, oz, u, v, w;
+};
+
+// virtual bin v of a tile: the centre bins of all species first (most
+// particles), then the other arrival directions
+__device__ inline void vbin(int v, int nsp, int &s, int &d)
+{
+    if (v < nsp) { s = v; d = 4; return; }
+    const int r = v - nsp;
+    s = r >> 3;
+    const int q = r & 7;
+    d = q < 4 ? q : q + 1;
 }
 
 template <int T, int BS>
 __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp, const float *__restrict__ Ex,
                                                      const float *__restrict__ Ey, const float *__restrict__ Ez,
                                                      const float *__restrict__ Bx, const float *__restrict__ By,
-                                                     const float *__restrict__ Bz, long long *__restrict__ Jx,
-                                                     long long *__restrict__ Jy, long long *__restrict__ Jz,
-                                                     PartSplit A, PartSplit Bo, BinsP bins, PartSplit spill,
-                                                     SpillP sp)
+                                                     const float *__restrict__ Bz, double *__restrict__ Jx,
+                                                     double *__restrict__ Jy, double *__restrict__ Jz, PartSplit A,
+                                                     PartSplit Bo, BinsP bins, PartSplit spill, SpillP sp)
 {
+    // FP32 form: fused multiply-add allowed (the FP64 form keeps it off for
+    // bitwise parity with the reference order)
+#pragma clang fp contract(fast)
     constexpr int W = T + 5;
     constexpr int WW = W * W;
-    __shared__ float sN[6][WW];
-    __shared__ unsigned long long sJ[3][WW];
-    __shared__ int sStart[10];
-    __shared__ long long sSrc[9];
-    __shared__ long long sDstBase[9];
-    __shared__ int sDstCap[9], sDstBin[9], sCnt[9];
+    constexpr int MB = 18;   // species (<= 2) x 9 directions
+    __shared__ float4 sN[2 * WW];   // node e: [2e] = (Ex,Ey,Ez,0), [2e+1] = (Bx,By,Bz,0), node-centred
+    __shared__ double sJ[3][WW];
+    __shared__ int sStart[MB + 1], sVs[MB];
+    __shared__ unsigned sSrc[MB], sDstBase[MB];
+    __shared__ int sDstCap[MB], sDstBin[MB], sCnt[MB];
+    __shared__ float sQm[4], sQs[4];
 
     const int tile = tile_remap(tp, blockIdx.x);
     const int tx = tile % tp.ntx, ty = tile / tp.ntx;
@@ -307,170 +340,193 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
     const int cx0 = tx * T - 2, cy0 = ty * T - 2;   // window origin as a physical cell index
     const int tid = threadIdx.x;
     const int mx = g.mx, nx = g.nx, ny = g.ny;
+    const int nsp = tp.nsp, NV = nsp * 9;
 
+    if (tid < NV) {
+        int s, d;
+        vbin(tid, nsp, s, d);
+        sSrc[tid] = (unsigned)bins.base[tile * tp.NB + s * 9 + d];
+        sVs[tid] = s;
+        // destination slot tid = (species, direction) = (tid / 9, tid % 9)
+        const int ds = tid / 9, dd = tid % 9;
+        const int dt = wrap_tile(ty + dd / 3 - 1, tp.nty) * tp.ntx + wrap_tile(tx + dd % 3 - 1, tp.ntx);
+        const int db = dt * tp.NB + ds * 9 + dd;
+        sDstBin[tid] = db;
+        sDstBase[tid] = (unsigned)bins.base[db];
+        sDstCap[tid] = bins.cap[db];
+        sCnt[tid] = 0;
+    }
+    if (tid == 0) {
+        int acc = 0;
+        for (int v = 0; v < NV; v++) {
+            int s, d;
+            vbin(v, nsp, s, d);
+            sStart[v] = acc;
+            acc += bins.cnt_in[tile * tp.NB + s * 9 + d];
+        }
+        sStart[NV] = acc;
+        // q/m / 2 and the charge pre-scaled by 2^36 (exact) for the fixed-point current
+        for (int s = 0; s < nsp; s++) { sQm[s] = (float)(ph.flvrqm[s] * 0.5); sQs[s] = (float)(ph.flvq[s] * kFixScale); }
+    }
     const float bx0 = (float)ph.bext[0], by0 = (float)ph.bext[1], bz0 = (float)ph.bext[2];
     for (int e = tid; e < WW; e += BS) {
         const int ii = e % W, jj = e / W;
-        float nex = 0, ney = 0, nez = 0, nbx = 0, nby = 0, nbz = 0;
+        float4 ne = make_float4(0.f, 0.f, 0.f, 0.f), nb = make_float4(0.f, 0.f, 0.f, 0.f);
         if (ii >= 1 && jj >= 1) {
             const long long o = (long long)(iw + ii - 1) + (long long)mx * (jw + jj - 1);
-            nex = (Ex[o - 1] + Ex[o]) * 0.5f;
-            ney = (Ey[o - mx] + Ey[o]) * 0.5f;
-            nez = Ez[o];
-            nbx = (Bx[o - mx] + Bx[o]) * 0.5f + bx0;
-            nby = (By[o - 1] + By[o]) * 0.5f + by0;
-            nbz = (Bz[o - 1 - mx] + Bz[o - 1] + Bz[o - mx] + Bz[o]) * 0.25f + bz0;
+            ne.x = (Ex[o - 1] + Ex[o]) * 0.5f;
+            ne.y = (Ey[o - mx] + Ey[o]) * 0.5f;
+            ne.z = Ez[o];
+            nb.x = (Bx[o - mx] + Bx[o]) * 0.5f + bx0;
+            nb.y = (By[o - 1] + By[o]) * 0.5f + by0;
+            nb.z = (Bz[o - 1 - mx] + Bz[o - 1] + Bz[o - mx] + Bz[o]) * 0.25f + bz0;
         }
-        sN[0][e] = nex; sN[1][e] = ney; sN[2][e] = nez;
-        sN[3][e] = nbx; sN[4][e] = nby; sN[5][e] = nbz;
-        sJ[0][e] = 0ull; sJ[1][e] = 0ull; sJ[2][e] = 0ull;
+        sN[2 * e] = ne;
+        sN[2 * e + 1] = nb;
+        sJ[0][e] = 0.0; sJ[1][e] = 0.0; sJ[2][e] = 0.0;
     }
+    __syncthreads();
 
     const float c = (float)ph.c, sqc = (float)ph.sqc, cinv = (float)ph.cinv;
     const float idx_ = (float)ph.inv_dx, idy_ = (float)ph.inv_dy, idz_ = (float)ph.inv_dz;
+    const int total = sStart[NV];
 
-    for (int s = 0; s < tp.nsp; s++) {
-        if (tid < 9) {
-            const int b = tile * tp.NB + s * 9 + tid;
-            sSrc[tid] = bins.base[b];
-            const int ddx = tid % 3 - 1, ddy = tid / 3 - 1;
-            const int dt = wrap_tile(ty + ddy, tp.nty) * tp.ntx + wrap_tile(tx + ddx, tp.ntx);
-            const int db = dt * tp.NB + s * 9 + tid;
-            sDstBin[tid] = db;
-            sDstBase[tid] = bins.base[db];
-            sDstCap[tid] = bins.cap[db];
-            sCnt[tid] = 0;
-        }
-        if (tid == 0) {
-            int acc = 0;
-            for (int b = 0; b < 9; b++) { sStart[b] = acc; acc += bins.cnt_in[tile * tp.NB + s * 9 + b]; }
-            sStart[9] = acc;
-        }
-        __syncthreads();
-        const float qm = (float)(ph.flvrqm[s] * 0.5);
-        const float qs = (float)ph.flvq[s];
-        const int total = sStart[9];
-        for (int base = 0; base < total; base += BS) {
-            const int k = base + tid;
-            const bool active = k < total;
-            int d = -1, ncell = 0;
-            float ox = 0, oy = 0, oz = 0, un = 0, vn = 0, wn = 0;
-            if (active) {
-                int b = 0;
-                while (k >= sStart[b + 1]) b++;
-                const long long src = sSrc[b] + (k - sStart[b]);
-                const int cell = A.cell[src];
-                const float x0 = A.ox[src], y0 = A.oy[src], z0 = A.oz[src];
-                un = A.u[src]; vn = A.v[src]; wn = A.w[src];
-                const int ci = cell % nx, cj = cell / nx;
-                const int ii0 = ci - cx0, jj0 = cj - cy0;
-                ox = x0; oy = y0; oz = z0; ncell = cell;
-                if (ii0 < 2 || ii0 > T + 1 || jj0 < 2 || jj0 > T + 1) {
-                    atomicOr(sp.err, (int)kErrBounds);
-                    d = 4;
-                } else {
-                    const float w1 = (1.0f - x0) * (1.0f - y0), w2 = x0 * (1.0f - y0);
-                    const float w3 = (1.0f - x0) * y0, w4 = x0 * y0;
-                    const int e0 = jj0 * W + ii0;
-                    float pf[6];
-#pragma unroll
-                    for (int cc = 0; cc < 6; cc++) {
-                        float a = w1 * sN[cc][e0];
-                        a = a + w2 * sN[cc][e0 + 1];
-                        a = a + w3 * sN[cc][e0 + W];
-                        a = a + w4 * sN[cc][e0 + W + 1];
-                        pf[cc] = a * qm;
-                    }
-                    float u0 = c * un + pf[0];
-                    float v0 = c * vn + pf[1];
-                    float w0 = c * wn + pf[2];
-                    float gg = 1.0f / sqrtf(sqc + u0 * u0 + v0 * v0 + w0 * w0);
-                    const float pBx = gg * pf[3], pBy = gg * pf[4], pBz = gg * pf[5];
-                    const float f = 2.0f / (1.0f + pBx * pBx + pBy * pBy + pBz * pBz);
-                    const float u1 = (u0 + v0 * pBz - w0 * pBy) * f;
-                    const float v1 = (v0 + w0 * pBx - u0 * pBz) * f;
-                    const float w1b = (w0 + u0 * pBy - v0 * pBx) * f;
-                    u0 = u0 + v1 * pBz - w1b * pBy + pf[0];
-                    v0 = v0 + w1b * pBx - u1 * pBz + pf[1];
-                    w0 = w0 + u1 * pBy - v1 * pBx + pf[2];
-                    un = u0 * cinv; vn = v0 * cinv; wn = w0 * cinv;
-                    gg = sqc / sqrtf(sqc + u0 * u0 + v0 * v0 + w0 * w0);
-                    const float x1 = x0 + un * gg * idx_;
-                    const float y1 = y0 + vn * gg * idy_;
-                    const float z1 = z0 + wn * gg * idz_;
-                    int di = (int)floorf(x1), dj = (int)floorf(y1);
-                    const int dk = (int)floorf(z1);
-                    const float xr = fminf((float)(min(0, di) + 1), fmaxf((float)max(0, di), 0.5f * (x0 + x1)));
-                    const float yr = fminf((float)(min(0, dj) + 1), fmaxf((float)max(0, dj), 0.5f * (y0 + y1)));
-                    const float zr = fminf((float)(min(0, dk) + 1), fmaxf((float)max(0, dk), 0.5f * (z0 + z1)));
-                    {
-                        const float Fx = qs * (xr - x0), Fy = qs * (yr - y0), Fz = qs * (zr - z0);
-                        const float Wx = 0.5f * (x0 + xr), Wy = 0.5f * (y0 + yr);
-                        const int e = e0;
-                        atomicAdd(&sJ[0][e], fixp(Fx * (1.0f - Wy)));
-                        atomicAdd(&sJ[0][e + W], fixp(Fx * Wy));
-                        atomicAdd(&sJ[1][e], fixp(Fy * (1.0f - Wx)));
-                        atomicAdd(&sJ[1][e + 1], fixp(Fy * Wx));
-                        atomicAdd(&sJ[2][e], fixp((Fz * (1.0f - Wx)) * (1.0f - Wy)));
-                        atomicAdd(&sJ[2][e + 1], fixp((Fz * Wx) * (1.0f - Wy)));
-                        atomicAdd(&sJ[2][e + W], fixp((Fz * (1.0f - Wx)) * Wy));
-                        atomicAdd(&sJ[2][e + W + 1], fixp((Fz * Wx) * Wy));
-                    }
-                    {
-                        const float Fx = qs * (x1 - xr), Fy = qs * (y1 - yr), Fz = qs * (z1 - zr);
-                        const float Wx = 0.5f * (x1 + xr) - (float)di, Wy = 0.5f * (y1 + yr) - (float)dj;
-                        const int e = e0 + dj * W + di;
-                        atomicAdd(&sJ[0][e], fixp(Fx * (1.0f - Wy)));
-                        atomicAdd(&sJ[0][e + W], fixp(Fx * Wy));
-                        atomicAdd(&sJ[1][e], fixp(Fy * (1.0f - Wx)));
-                        atomicAdd(&sJ[1][e + 1], fixp(Fy * Wx));
-                        atomicAdd(&sJ[2][e], fixp((Fz * (1.0f - Wx)) * (1.0f - Wy)));
-                        atomicAdd(&sJ[2][e + 1], fixp((Fz * Wx) * (1.0f - Wy)));
-                        atomicAdd(&sJ[2][e + W], fixp((Fz * (1.0f - Wx)) * Wy));
-                        atomicAdd(&sJ[2][e + W + 1], fixp((Fz * Wx) * Wy));
-                    }
-                    ox = x1 - (float)di;
-                    if (ox >= 1.0f) { ox = 0.0f; di += 1; }
-                    oy = y1 - (float)dj;
-                    if (oy >= 1.0f) { oy = 0.0f; dj += 1; }
-                    oz = z1 - (float)dk;
-                    if (oz >= 1.0f) oz = 0.0f;
-                    int nci = ci + di, ncj = cj + dj;
-                    nci = nci < 0 ? nci + nx : (nci >= nx ? nci - nx : nci);
-                    ncj = ncj < 0 ? ncj + ny : (ncj >= ny ? ncj - ny : ncj);
-                    ncell = nci + nx * ncj;
-                    const int ddx = norm_dir(nci / T - tx, tp.ntx), ddy = norm_dir(ncj / T - ty, tp.nty);
-                    d = (ddx + 1) + 3 * (ddy + 1);
+    auto load = [&](int k, SplitRec &r) {
+        int v = 0;
+        while (k >= sStart[v + 1]) v++;
+        const unsigned off = (sSrc[v] + (unsigned)(k - sStart[v])) * 4u;
+        r.cell = ldo(A.cell, off);
+        r.ox = ldo(A.ox, off); r.oy = ldo(A.oy, off); r.oz = ldo(A.oz, off);
+        r.u = ldo(A.u, off); r.v = ldo(A.v, off); r.w = ldo(A.w, off);
+        r.s = sVs[v];
+    };
+
+    SplitRec nxt{};
+    if (tid < total) load(tid, nxt);
+    for (int base = 0; base < total; base += BS) {
+        const int k = base + tid;
+        const bool active = k < total;
+        const SplitRec cur = nxt;
+        if (k + BS < total) load(k + BS, nxt);   // prefetch the next particle of this thread
+        int d = -1, ncell = cur.cell;
+        float ox = cur.ox, oy = cur.oy, oz = cur.oz, un = cur.u, vn = cur.v, wn = cur.w;
+        if (active) {
+            const int ci = cur.cell & 0xFFFF, cj = cur.cell >> 16;
+            const int ii0 = ci - cx0, jj0 = cj - cy0;
+            if (ii0 < 2 || ii0 > T + 1 || jj0 < 2 || jj0 > T + 1) {
+                atomicOr(sp.err, (int)kErrBounds);
+                d = cur.s * 9 + 4;
+            } else {
+                const float x0 = cur.ox, y0 = cur.oy, z0 = cur.oz;
+                const float qm = sQm[cur.s], qs = sQs[cur.s];
+                const float w1 = (1.0f - x0) * (1.0f - y0), w2 = x0 * (1.0f - y0);
+                const float w3 = (1.0f - x0) * y0, w4 = x0 * y0;
+                const int e0 = jj0 * W + ii0;
+                const float4 a0 = sN[2 * e0], b0 = sN[2 * e0 + 1];
+                const float4 a1 = sN[2 * e0 + 2], b1 = sN[2 * e0 + 3];
+                const float4 a2 = sN[2 * (e0 + W)], b2 = sN[2 * (e0 + W) + 1];
+                const float4 a3 = sN[2 * (e0 + W) + 2], b3 = sN[2 * (e0 + W) + 3];
+                const float pEx = (w1 * a0.x + w2 * a1.x + w3 * a2.x + w4 * a3.x) * qm;
+                const float pEy = (w1 * a0.y + w2 * a1.y + w3 * a2.y + w4 * a3.y) * qm;
+                const float pEz = (w1 * a0.z + w2 * a1.z + w3 * a2.z + w4 * a3.z) * qm;
+                const float hBx = (w1 * b0.x + w2 * b1.x + w3 * b2.x + w4 * b3.x) * qm;
+                const float hBy = (w1 * b0.y + w2 * b1.y + w3 * b2.y + w4 * b3.y) * qm;
+                const float hBz = (w1 * b0.z + w2 * b1.z + w3 * b2.z + w4 * b3.z) * qm;
+                // Boris, movdep.F90:143-173 (hardware rsq/rcp: about 1 ulp, FP32 form)
+                float u0 = c * un + pEx;
+                float v0 = c * vn + pEy;
+                float w0 = c * wn + pEz;
+                float gg = __builtin_amdgcn_rsqf(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+                const float pBx = gg * hBx, pBy = gg * hBy, pBz = gg * hBz;
+                const float f = 2.0f * __builtin_amdgcn_rcpf(1.0f + pBx * pBx + pBy * pBy + pBz * pBz);
+                const float u1 = (u0 + v0 * pBz - w0 * pBy) * f;
+                const float v1 = (v0 + w0 * pBx - u0 * pBz) * f;
+                const float w1b = (w0 + u0 * pBy - v0 * pBx) * f;
+                u0 = u0 + v1 * pBz - w1b * pBy + pEx;
+                v0 = v0 + w1b * pBx - u1 * pBz + pEy;
+                w0 = w0 + u1 * pBy - v1 * pBx + pEz;
+                un = u0 * cinv; vn = v0 * cinv; wn = w0 * cinv;
+                gg = sqc * __builtin_amdgcn_rsqf(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+                const float x1 = x0 + un * gg * idx_;
+                const float y1 = y0 + vn * gg * idy_;
+                const float z1 = z0 + wn * gg * idz_;
+                int di = (int)floorf(x1), dj = (int)floorf(y1);
+                const int dk = (int)floorf(z1);
+                const float xr = fminf((float)(min(0, di) + 1), fmaxf((float)max(0, di), 0.5f * (x0 + x1)));
+                const float yr = fminf((float)(min(0, dj) + 1), fmaxf((float)max(0, dj), 0.5f * (y0 + y1)));
+                const float zr = fminf((float)(min(0, dk) + 1), fmaxf((float)max(0, dk), 0.5f * (z0 + z1)));
+                {
+                    const float Fx = qs * (xr - x0), Fy = qs * (yr - y0), Fz = qs * (zr - z0);
+                    const float Wx = 0.5f * (x0 + xr), Wy = 0.5f * (y0 + yr);
+                    const float zl = Fz * (1.0f - Wx), zh = Fz * Wx;
+                    double *jp = &sJ[0][e0];
+                    atomicAdd(jp, quant(Fx * (1.0f - Wy)));
+                    atomicAdd(jp + W, quant(Fx * Wy));
+                    atomicAdd(jp + WW, quant(Fy * (1.0f - Wx)));
+                    atomicAdd(jp + WW + 1, quant(Fy * Wx));
+                    atomicAdd(jp + 2 * WW, quant(zl * (1.0f - Wy)));
+                    atomicAdd(jp + 2 * WW + 1, quant(zh * (1.0f - Wy)));
+                    atomicAdd(jp + 2 * WW + W, quant(zl * Wy));
+                    atomicAdd(jp + 2 * WW + W + 1, quant(zh * Wy));
                 }
+                {
+                    const float Fx = qs * (x1 - xr), Fy = qs * (y1 - yr), Fz = qs * (z1 - zr);
+                    const float Wx = 0.5f * (x1 + xr) - (float)di, Wy = 0.5f * (y1 + yr) - (float)dj;
+                    const float zl = Fz * (1.0f - Wx), zh = Fz * Wx;
+                    double *jp = &sJ[0][e0 + dj * W + di];
+                    atomicAdd(jp, quant(Fx * (1.0f - Wy)));
+                    atomicAdd(jp + W, quant(Fx * Wy));
+                    atomicAdd(jp + WW, quant(Fy * (1.0f - Wx)));
+                    atomicAdd(jp + WW + 1, quant(Fy * Wx));
+                    atomicAdd(jp + 2 * WW, quant(zl * (1.0f - Wy)));
+                    atomicAdd(jp + 2 * WW + 1, quant(zh * (1.0f - Wy)));
+                    atomicAdd(jp + 2 * WW + W, quant(zl * Wy));
+                    atomicAdd(jp + 2 * WW + W + 1, quant(zh * Wy));
+                }
+                ox = x1 - (float)di;
+                if (ox >= 1.0f) { ox = 0.0f; di += 1; }
+                oy = y1 - (float)dj;
+                if (oy >= 1.0f) { oy = 0.0f; dj += 1; }
+                oz = z1 - (float)dk;
+                if (oz >= 1.0f) oz = 0.0f;
+                int nci = ci + di, ncj = cj + dj;
+                nci = nci < 0 ? nci + nx : (nci >= nx ? nci - nx : nci);
+                ncj = ncj < 0 ? ncj + ny : (ncj >= ny ? ncj - ny : ncj);
+                ncell = nci | (ncj << 16);
+                const int ddx = norm_dir(nci / T - tx, tp.ntx), ddy = norm_dir(ncj / T - ty, tp.nty);
+                d = cur.s * 9 + (ddx + 1) + 3 * (ddy + 1);
             }
-            const int slot = reserve_slot(d, sCnt);
-            if (active) {
-                if (slot < sDstCap[d]) {
-                    const long long o = sDstBase[d] + slot;
-                    Bo.cell[o] = ncell; Bo.ox[o] = ox; Bo.oy[o] = oy; Bo.oz[o] = oz;
-                    Bo.u[o] = un; Bo.v[o] = vn; Bo.w[o] = wn;
+        }
+        const int slot = reserve_slot(d, sCnt);
+        if (active) {
+            if (slot < sDstCap[d]) {
+                const unsigned off = (sDstBase[d] + (unsigned)slot) * 4u;
+                sto(Bo.cell, off, ncell); sto(Bo.ox, off, ox); sto(Bo.oy, off, oy); sto(Bo.oz, off, oz);
+                sto(Bo.u, off, un); sto(Bo.v, off, vn); sto(Bo.w, off, wn);
+            } else {
+                const int si = atomicAdd(sp.count, 1);
+                if (si < sp.cap) {
+                    spill.cell[si] = ncell; spill.ox[si] = ox; spill.oy[si] = oy; spill.oz[si] = oz;
+                    spill.u[si] = un; spill.v[si] = vn; spill.w[si] = wn; spill.sp[si] = cur.s;
                 } else {
-                    const int si = atomicAdd(sp.count, 1);
-                    if (si < sp.cap) {
-                        spill.cell[si] = ncell; spill.ox[si] = ox; spill.oy[si] = oy; spill.oz[si] = oz;
-                        spill.u[si] = un; spill.v[si] = vn; spill.w[si] = wn; spill.sp[si] = s;
-                    } else {
-                        atomicOr(sp.err, (int)kErrCapacity);
-                    }
+                    atomicOr(sp.err, (int)kErrCapacity);
                 }
             }
         }
-        __syncthreads();
-        if (tid < 9) bins.cnt_out[sDstBin[tid]] = min(sCnt[tid], sDstCap[tid]);
-        __syncthreads();
     }
+    __syncthreads();
+    if (tid < NV) bins.cnt_out[sDstBin[tid]] = min(sCnt[tid], sDstCap[tid]);
+    constexpr double lim = kFixLimit * kFixScale;
+    bool big = false;
     for (int e = tid; e < WW; e += BS) {
         const long long o = (long long)(iw + e % W - 1) + (long long)mx * (jw + e / W - 1);
-        const unsigned long long a = sJ[0][e], b = sJ[1][e], cz = sJ[2][e];
-        if (a) atomicAdd((unsigned long long *)&Jx[o], a);
-        if (b) atomicAdd((unsigned long long *)&Jy[o], b);
-        if (cz) atomicAdd((unsigned long long *)&Jz[o], cz);
+        const double a = sJ[0][e], b = sJ[1][e], cz = sJ[2][e];
+        big = big || fabs(a) >= lim || fabs(b) >= lim || fabs(cz) >= lim;
+        if (a != 0.0) unsafeAtomicAdd(&Jx[o], a);
+        if (b != 0.0) unsafeAtomicAdd(&Jy[o], b);
+        if (cz != 0.0) unsafeAtomicAdd(&Jz[o], cz);
     }
+    if (big) atomicOr(sp.err, (int)kErrPrecision);
 }
 
 }  // namespace
@@ -493,7 +549,7 @@ void launch_push2d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, 
 }
 
 void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp, float *const E[3],
-                         float *const B[3], long long *const J[3], PartSplit A, PartSplit Bo, BinsP bins,
+                         float *const B[3], double *const J[3], PartSplit A, PartSplit Bo, BinsP bins,
                          PartSplit spill, SpillP sp, hipStream_t s)
 {
     dim3 grid(tp.ntiles);

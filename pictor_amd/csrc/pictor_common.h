@@ -13,11 +13,18 @@
 
 namespace pictor {
 
-constexpr int kFixShift = 40;                 // fixed-point current: 2^40 units per (charge x cell)
-constexpr double kFixScale = 1099511627776.0; // 2^40
-constexpr double kFixInv = 1.0 / 1099511627776.0;
+// Split-form current: every contribution is rounded to a multiple of 2^-36
+// (charge x cell) and summed in FP64 as an integer count of 2^-36 units.  All
+// partial sums stay integers below 2^53, so every addition is exact: the sum
+// does not depend on the order of the atomics and is deterministic.  A node
+// gets contributions from at most 4 tile windows and their ghost images; each
+// tile's partial sum is checked against kFixLimit at the flush, which keeps the
+// total below 8 * 2^14 = 2^17 = 2^53 * 2^-36.
+constexpr double kFixScale = 68719476736.0;          // 2^36
+constexpr double kFixInv = 1.0 / 68719476736.0;
+constexpr double kFixLimit = 16384.0;                // per-tile |node sum| bound (charge x cell)
 
-enum ErrBits : int { kErrBounds = 1, kErrCapacity = 2 };
+enum ErrBits : int { kErrBounds = 1, kErrCapacity = 2, kErrPrecision = 4 };
 
 struct GridP {
     int ndim, nx, ny, nz, mx, my, mz;
@@ -49,7 +56,19 @@ struct PartF64 {
     int *flv;
 };
 
-// FP32 split form: linear physical cell index + FP32 in-cell offsets.
+// FP32 split form: packed physical cell coordinates + FP32 in-cell offsets.
+// Cell word: 2-D  i | j << 16 (nx, ny <= 32768); 3-D  i | j << 11 | k << 22
+// (nx, ny <= 2048, nz <= 512), so that no integer division is needed.
+__host__ __device__ inline int cell_pack(int ndim, int i, int j, int k)
+{
+    return ndim == 2 ? (i | (j << 16)) : (i | (j << 11) | (k << 22));
+}
+__host__ __device__ inline void cell_unpack(int ndim, int c, int &i, int &j, int &k)
+{
+    if (ndim == 2) { i = c & 0xFFFF; j = (c >> 16) & 0xFFFF; k = 0; }
+    else { i = c & 0x7FF; j = (c >> 11) & 0x7FF; k = (c >> 22) & 0x1FF; }
+}
+
 struct PartSplit {
     int *cell;
     float *ox, *oy, *oz, *u, *v, *w;
@@ -92,7 +111,7 @@ template <typename R>
 void launch_gauss(const GridP &g, R *const E[3], const double *rho, double *partial, int nblk, hipStream_t s);
 void launch_convert_fields_f32(const double *src, float *dst, long long n, hipStream_t s);
 void launch_convert_fields_f64(const float *src, double *dst, long long n, hipStream_t s);
-void launch_convert_jfix(const long long *src, double *dst, long long n, double scale, hipStream_t s);
+void launch_convert_jfix(const double *src, double *dst, long long n, double scale, hipStream_t s);
 
 // ---------------- launchers (kernels_push.hip) ----------------
 struct BinsP {
@@ -112,32 +131,36 @@ void launch_push2d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, 
                        double *const B[3], double *const J[3], PartF64 A, PartF64 Bo, BinsP bins,
                        PartF64 spill, SpillP sp, hipStream_t s);
 void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp, float *const E[3],
-                         float *const B[3], long long *const J[3], PartSplit A, PartSplit Bo, BinsP bins,
+                         float *const B[3], double *const J[3], PartSplit A, PartSplit Bo, BinsP bins,
                          PartSplit spill, SpillP sp, hipStream_t s);
 
 // ---------------- launchers (kernels_layout.hip) ----------------
 // raw FP64 particles (upload): tile id per particle + per (tile, species) counts
 void launch_raw_tiles_f64(const GridP &g, const PhysP &ph, const TileP &tp, PartF64 raw, long long n,
                           int split, int *tile_out, int *counts, int *err, hipStream_t s);
+// gtotal (per-group particle count): slots are permuted within each group so that the
+// particles of one cell do not share a wavefront (same-address LDS atomics serialise)
 void launch_raw_scatter_f64(const GridP &g, const TileP &tp, PartF64 raw, long long n, const int *tile_of,
-                            const long long *base, int *fill, PartF64 out, hipStream_t s);
+                            const long long *base, int *fill, const int *gtotal, PartF64 out, hipStream_t s);
 void launch_raw_scatter_split(const GridP &g, const PhysP &ph, const TileP &tp, PartF64 raw, long long n,
-                              const int *tile_of, const long long *base, int *fill, PartSplit out,
-                              hipStream_t s);
+                              const int *tile_of, const long long *base, int *fill, const int *gtotal,
+                              PartSplit out, hipStream_t s);
 // spill particles: tile per spill particle + counts
 void launch_spill_tiles_f64(const GridP &g, const TileP &tp, PartF64 spill, int n, int *tile_out, int *counts,
                             hipStream_t s);
 void launch_spill_tiles_split(const GridP &g, const TileP &tp, PartSplit spill, int n, int *tile_out,
                               int *counts, hipStream_t s);
 // compaction of every (tile, species) group of bins into one dense run
+// gtotal == nullptr: dense copy in bin order (download); otherwise permuted as above
 void launch_compact_f64(const TileP &tp, PartF64 src, const long long *base, const int *cnt,
-                        const long long *dst_off, PartF64 dst, hipStream_t s);
+                        const long long *dst_off, const int *gtotal, PartF64 dst, hipStream_t s);
 void launch_compact_split(const TileP &tp, PartSplit src, const long long *base, const int *cnt,
-                          const long long *dst_off, PartSplit dst, hipStream_t s);
+                          const long long *dst_off, const int *gtotal, PartSplit dst, hipStream_t s);
 void launch_spill_scatter_f64(const TileP &tp, PartF64 spill, int n, const int *tile_of,
-                              const long long *dst_off, int *fill, PartF64 dst, hipStream_t s);
+                              const long long *dst_off, int *fill, const int *gtotal, PartF64 dst, hipStream_t s);
 void launch_spill_scatter_split(const TileP &tp, PartSplit spill, int n, const int *tile_of,
-                                const long long *dst_off, int *fill, PartSplit dst, hipStream_t s);
+                                const long long *dst_off, int *fill, const int *gtotal, PartSplit dst,
+                                hipStream_t s);
 // split -> FP64 reference coordinates, in place into an FP64 staging buffer
 void launch_split_to_f64(const GridP &g, const PhysP &ph, PartSplit src, long long n, PartF64 dst,
                          hipStream_t s);
