@@ -37,6 +37,21 @@ BYTES_PER_PARTICLE = {"fp32_split": 2 * (4 + 3 * 4 + 3 * 4), "fp64": 7 * 8 + 4 +
 GRID_BYTES = {"fp32_split": 6 * 4 + 3 * 4, "fp64": 6 * 8 + 3 * 8}
 
 
+def measured_traffic(form, workload, tile):
+    # HBM bytes per launch of the push kernel from the committed rocprofv3 counter runs
+    # (FETCH_SIZE x 2.0 + WRITE_SIZE, see the file named by "source"), or None
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get("k_push2d_split" if form == "fp32_split" else "k_push2d_f64")
+    if e and e.get("workload") == workload and e.get("tile") == tile:
+        return e["bytes_per_launch"]
+    return None
+
+
 def run_gpu(form, nx, ny, epc, P, steps, warmup, calibrate=False, tile=0):
     from pictor_amd import Simulation
 
@@ -129,6 +144,7 @@ def main():
     for form in forms:
         res[form] = run_gpu(form, nx, ny, epc, P, args.steps, args.warmup, args.calibrate, args.tile)
         r = res[form]
+        r["roofline"]["traffic"] = measured_traffic(form, args.config, r["tile"])
         print(f"# {form}: {r['value']:.4e} particle-updates/s, {r['ms_per_step']:.3f} ms/step, "
               f"push {r['push_ms']:.3f} ms ({r['roofline']['frac'] * 100:.1f}% HBM)", file=sys.stderr, flush=True)
 

@@ -327,7 +327,9 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
     constexpr int W = T + 5;
     constexpr int WW = W * W;
     constexpr int MB = 18;   // species (<= 2) x 9 directions
-    __shared__ float4 sN[2 * WW];   // node e: [2e] = (Ex,Ey,Ez,0), [2e+1] = (Bx,By,Bz,0), node-centred
+    // cell-corner field layout: cell lc holds 6 float4 = (component c at corners 1..4),
+    // the reference's VecE/VecB(nn, cell) (movdep_routines.F90:6-223) for the tile's cells
+    __shared__ float4 sC[6 * T * T];
     __shared__ double sJ[3][WW];
     __shared__ int sStart[MB + 1], sVs[MB];
     __shared__ unsigned sSrc[MB], sDstBase[MB];
@@ -369,22 +371,24 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
         for (int s = 0; s < nsp; s++) { sQm[s] = (float)(ph.flvrqm[s] * 0.5); sQs[s] = (float)(ph.flvq[s] * kFixScale); }
     }
     const float bx0 = (float)ph.bext[0], by0 = (float)ph.bext[1], bz0 = (float)ph.bext[2];
-    for (int e = tid; e < WW; e += BS) {
-        const int ii = e % W, jj = e / W;
-        float4 ne = make_float4(0.f, 0.f, 0.f, 0.f), nb = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (ii >= 1 && jj >= 1) {
-            const long long o = (long long)(iw + ii - 1) + (long long)mx * (jw + jj - 1);
-            ne.x = (Ex[o - 1] + Ex[o]) * 0.5f;
-            ne.y = (Ey[o - mx] + Ey[o]) * 0.5f;
-            ne.z = Ez[o];
-            nb.x = (Bx[o - mx] + Bx[o]) * 0.5f + bx0;
-            nb.y = (By[o - 1] + By[o]) * 0.5f + by0;
-            nb.z = (Bz[o - 1 - mx] + Bz[o - 1] + Bz[o - mx] + Bz[o]) * 0.25f + bz0;
+    for (int lc = tid; lc < T * T; lc += BS) {
+        // reference index of the cell: i = 3 + tx*T + li; its corner nodes (i..i+1, j..j+1)
+        const long long o = (long long)(iw + 2 + lc % T - 1) + (long long)mx * (jw + 2 + lc / T - 1);
+        float v[6][4];
+#pragma unroll
+        for (int cn = 0; cn < 4; cn++) {
+            const long long p = o + (cn & 1) + (cn >> 1) * (long long)mx;
+            v[0][cn] = (Ex[p - 1] + Ex[p]) * 0.5f;
+            v[1][cn] = (Ey[p - mx] + Ey[p]) * 0.5f;
+            v[2][cn] = Ez[p];
+            v[3][cn] = (Bx[p - mx] + Bx[p]) * 0.5f + bx0;
+            v[4][cn] = (By[p - 1] + By[p]) * 0.5f + by0;
+            v[5][cn] = (Bz[p - 1 - mx] + Bz[p - 1] + Bz[p - mx] + Bz[p]) * 0.25f + bz0;
         }
-        sN[2 * e] = ne;
-        sN[2 * e + 1] = nb;
-        sJ[0][e] = 0.0; sJ[1][e] = 0.0; sJ[2][e] = 0.0;
+#pragma unroll
+        for (int cc = 0; cc < 6; cc++) sC[6 * lc + cc] = make_float4(v[cc][0], v[cc][1], v[cc][2], v[cc][3]);
     }
+    for (int e = tid; e < WW; e += BS) { sJ[0][e] = 0.0; sJ[1][e] = 0.0; sJ[2][e] = 0.0; }
     __syncthreads();
 
     const float c = (float)ph.c, sqc = (float)ph.sqc, cinv = (float)ph.cinv;
@@ -422,16 +426,15 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                 const float w1 = (1.0f - x0) * (1.0f - y0), w2 = x0 * (1.0f - y0);
                 const float w3 = (1.0f - x0) * y0, w4 = x0 * y0;
                 const int e0 = jj0 * W + ii0;
-                const float4 a0 = sN[2 * e0], b0 = sN[2 * e0 + 1];
-                const float4 a1 = sN[2 * e0 + 2], b1 = sN[2 * e0 + 3];
-                const float4 a2 = sN[2 * (e0 + W)], b2 = sN[2 * (e0 + W) + 1];
-                const float4 a3 = sN[2 * (e0 + W) + 2], b3 = sN[2 * (e0 + W) + 3];
-                const float pEx = (w1 * a0.x + w2 * a1.x + w3 * a2.x + w4 * a3.x) * qm;
-                const float pEy = (w1 * a0.y + w2 * a1.y + w3 * a2.y + w4 * a3.y) * qm;
-                const float pEz = (w1 * a0.z + w2 * a1.z + w3 * a2.z + w4 * a3.z) * qm;
-                const float hBx = (w1 * b0.x + w2 * b1.x + w3 * b2.x + w4 * b3.x) * qm;
-                const float hBy = (w1 * b0.y + w2 * b1.y + w3 * b2.y + w4 * b3.y) * qm;
-                const float hBz = (w1 * b0.z + w2 * b1.z + w3 * b2.z + w4 * b3.z) * qm;
+                const int lc = (jj0 - 2) * T + (ii0 - 2);
+                const float4 gEx = sC[6 * lc], gEy = sC[6 * lc + 1], gEz = sC[6 * lc + 2];
+                const float4 gBx = sC[6 * lc + 3], gBy = sC[6 * lc + 4], gBz = sC[6 * lc + 5];
+                const float pEx = (w1 * gEx.x + w2 * gEx.y + w3 * gEx.z + w4 * gEx.w) * qm;
+                const float pEy = (w1 * gEy.x + w2 * gEy.y + w3 * gEy.z + w4 * gEy.w) * qm;
+                const float pEz = (w1 * gEz.x + w2 * gEz.y + w3 * gEz.z + w4 * gEz.w) * qm;
+                const float hBx = (w1 * gBx.x + w2 * gBx.y + w3 * gBx.z + w4 * gBx.w) * qm;
+                const float hBy = (w1 * gBy.x + w2 * gBy.y + w3 * gBy.z + w4 * gBy.w) * qm;
+                const float hBz = (w1 * gBz.x + w2 * gBz.y + w3 * gBz.z + w4 * gBz.w) * qm;
                 // Boris, movdep.F90:143-173 (hardware rsq/rcp: about 1 ulp, FP32 form)
                 float u0 = c * un + pEx;
                 float v0 = c * vn + pEy;
@@ -455,33 +458,55 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                 const float xr = fminf((float)(min(0, di) + 1), fmaxf((float)max(0, di), 0.5f * (x0 + x1)));
                 const float yr = fminf((float)(min(0, dj) + 1), fmaxf((float)max(0, dj), 0.5f * (y0 + y1)));
                 const float zr = fminf((float)(min(0, dk) + 1), fmaxf((float)max(0, dk), 0.5f * (z0 + z1)));
+                // the two zigzag half-segments: 8 node values each, as exact
+                // integer counts of 2^-36 units
+                double a[8], b[8];
                 {
                     const float Fx = qs * (xr - x0), Fy = qs * (yr - y0), Fz = qs * (zr - z0);
                     const float Wx = 0.5f * (x0 + xr), Wy = 0.5f * (y0 + yr);
                     const float zl = Fz * (1.0f - Wx), zh = Fz * Wx;
-                    double *jp = &sJ[0][e0];
-                    atomicAdd(jp, quant(Fx * (1.0f - Wy)));
-                    atomicAdd(jp + W, quant(Fx * Wy));
-                    atomicAdd(jp + WW, quant(Fy * (1.0f - Wx)));
-                    atomicAdd(jp + WW + 1, quant(Fy * Wx));
-                    atomicAdd(jp + 2 * WW, quant(zl * (1.0f - Wy)));
-                    atomicAdd(jp + 2 * WW + 1, quant(zh * (1.0f - Wy)));
-                    atomicAdd(jp + 2 * WW + W, quant(zl * Wy));
-                    atomicAdd(jp + 2 * WW + W + 1, quant(zh * Wy));
+                    a[0] = quant(Fx * (1.0f - Wy)); a[1] = quant(Fx * Wy);
+                    a[2] = quant(Fy * (1.0f - Wx)); a[3] = quant(Fy * Wx);
+                    a[4] = quant(zl * (1.0f - Wy)); a[5] = quant(zh * (1.0f - Wy));
+                    a[6] = quant(zl * Wy); a[7] = quant(zh * Wy);
                 }
                 {
                     const float Fx = qs * (x1 - xr), Fy = qs * (y1 - yr), Fz = qs * (z1 - zr);
                     const float Wx = 0.5f * (x1 + xr) - (float)di, Wy = 0.5f * (y1 + yr) - (float)dj;
                     const float zl = Fz * (1.0f - Wx), zh = Fz * Wx;
+                    b[0] = quant(Fx * (1.0f - Wy)); b[1] = quant(Fx * Wy);
+                    b[2] = quant(Fy * (1.0f - Wx)); b[3] = quant(Fy * Wx);
+                    b[4] = quant(zl * (1.0f - Wy)); b[5] = quant(zh * (1.0f - Wy));
+                    b[6] = quant(zl * Wy); b[7] = quant(zh * Wy);
+                }
+                // a particle that stays in its cell deposits both halves on the same
+                // 8 nodes: add them first (exact) and halve the LDS atomics
+                const bool cross = (di != 0) || (dj != 0);
+                if (!cross) {
+#pragma unroll
+                    for (int q = 0; q < 8; q++) a[q] += b[q];
+                }
+                {
+                    double *jp = &sJ[0][e0];
+                    atomicAdd(jp, a[0]);
+                    atomicAdd(jp + W, a[1]);
+                    atomicAdd(jp + WW, a[2]);
+                    atomicAdd(jp + WW + 1, a[3]);
+                    atomicAdd(jp + 2 * WW, a[4]);
+                    atomicAdd(jp + 2 * WW + 1, a[5]);
+                    atomicAdd(jp + 2 * WW + W, a[6]);
+                    atomicAdd(jp + 2 * WW + W + 1, a[7]);
+                }
+                if (cross) {
                     double *jp = &sJ[0][e0 + dj * W + di];
-                    atomicAdd(jp, quant(Fx * (1.0f - Wy)));
-                    atomicAdd(jp + W, quant(Fx * Wy));
-                    atomicAdd(jp + WW, quant(Fy * (1.0f - Wx)));
-                    atomicAdd(jp + WW + 1, quant(Fy * Wx));
-                    atomicAdd(jp + 2 * WW, quant(zl * (1.0f - Wy)));
-                    atomicAdd(jp + 2 * WW + 1, quant(zh * (1.0f - Wy)));
-                    atomicAdd(jp + 2 * WW + W, quant(zl * Wy));
-                    atomicAdd(jp + 2 * WW + W + 1, quant(zh * Wy));
+                    atomicAdd(jp, b[0]);
+                    atomicAdd(jp + W, b[1]);
+                    atomicAdd(jp + WW, b[2]);
+                    atomicAdd(jp + WW + 1, b[3]);
+                    atomicAdd(jp + 2 * WW, b[4]);
+                    atomicAdd(jp + 2 * WW + 1, b[5]);
+                    atomicAdd(jp + 2 * WW + W, b[6]);
+                    atomicAdd(jp + 2 * WW + W + 1, b[7]);
                 }
                 ox = x1 - (float)di;
                 if (ox >= 1.0f) { ox = 0.0f; di += 1; }
@@ -559,7 +584,7 @@ void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp
     switch (T) {
     case 8: PUSH_SPLIT(8, 256); break;
     case 16: PUSH_SPLIT(16, 256); break;
-    case 32: PUSH_SPLIT(32, 512); break;
+    case 32: PUSH_SPLIT(32, 1024); break;
     default: break;
     }
 #undef PUSH_SPLIT
