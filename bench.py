@@ -10,8 +10,9 @@ value = particles * steps / wall time of the timed steps (FP32 split form).
 The FP64 form runs the same workload and is reported beside it.
 roofline: the push/deposit kernel, timed with HIP events on its own stream;
 algorithmic bytes per launch = particles * bytes per particle + cells * grid
-bytes.  cpu_baseline: the FP64 C oracle (restated reference
-CPU path, OpenMP) on a bounded number of steps of the same workload.
+bytes (BYTES_PER_PARTICLE, GRID_BYTES below).  cpu_baseline: the FP64 C
+oracle (restated reference CPU path, OpenMP) on a bounded number of steps of
+the same workload.
 """
 import argparse
 import json

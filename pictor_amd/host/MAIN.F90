@@ -1,0 +1,147 @@
+! MAIN.F90 -- the reference's time loop (src/MAIN.F90:42-284) driving the gfx950 particle step.
+!
+! Each phase subroutine below keeps the name and position of the call it
+! replaces in the reference loop and forwards to the C ABI.  Phases the
+! periodic one-GPU run does not need (BCs, output, load balancing, filters)
+! are left out.  Usage:  pictor_main [name=value ...]  with names from
+! parameters.F90 (nx, ny, epc, tfinish, form, fused, seed, c, c_ompe,
+! energy_save_period, prtl_reorder_period).
+program MAIN
+   use iso_c_binding
+   use parameters
+   use vars
+   use setup
+   use pictor_cabi
+   implicit none
+   integer(c_int64_t) :: c0, c1, crate
+   real(c_double) :: secs
+
+   call ReadArgs
+   call Initialisations
+   call SaveEnergy(0)
+   call system_clock(c0, crate)
+   do t = 1, tfinish
+      if (fused == 1) then
+         call check(pictor_step(gpu, 1_c_int32_t), 'pictor_step')
+      else
+         call UpdateBfieldHalf          ! MAIN.F90:51/54, SendRecvFlds(B) :58
+         call MoveDepositPrtl           ! MAIN.F90:65/68
+         call ExchangePrtl              ! MAIN.F90:88/92 (+ AppendParticles :147)
+         call UpdateBfieldHalf          ! MAIN.F90:103/114
+         call UpdateEfield              ! MAIN.F90:108/115
+         call UpdateCurrentAllFaces     ! MAIN.F90:124/129
+         call AddCurrent                ! MAIN.F90:171/174, SendRecvFlds(E) :184/190
+         call ResetCurrent              ! MAIN.F90:248/251
+         call ReorderPrtl               ! MAIN.F90:261/264
+      end if
+      if (energy_save_period > 0) then
+         if (mod(t, energy_save_period) == 0) call SaveEnergy(t)
+      end if
+   end do
+   call check(pictor_synchronize(gpu), 'pictor_synchronize')
+   call system_clock(c1)
+   secs = real(c1 - c0, c_double)/real(crate, c_double)
+   write (*, '(a,i0,a,es12.4,a,es12.4)') 'steps ', tfinish, '  seconds ', secs, &
+      '  particle-updates/s ', real(used_prtl_arr_size, c_double)*tfinish/max(secs, 1d-30)
+   call check(pictor_finalize(gpu), 'pictor_finalize')
+   write (*, '(a)') 'Simulation Finished!'
+
+contains
+
+   subroutine check(rc, where)
+      integer(c_int), intent(in) :: rc
+      character(len=*), intent(in) :: where
+      call pictor_check(rc, where)
+   end subroutine check
+
+   subroutine ReadArgs
+      integer :: i, k, ios
+      character(len=256) :: arg, key, val
+      do i = 1, command_argument_count()
+         call get_command_argument(i, arg)
+         k = index(arg, '=')
+         if (k == 0) cycle
+         key = arg(1:k - 1)
+         val = arg(k + 1:)
+         select case (trim(key))
+         case ('nx'); read (val, *, iostat=ios) nx
+         case ('ny'); read (val, *, iostat=ios) ny
+         case ('epc'); read (val, *, iostat=ios) epc
+         case ('tfinish'); read (val, *, iostat=ios) tfinish
+         case ('form'); read (val, *, iostat=ios) form
+         case ('fused'); read (val, *, iostat=ios) fused
+         case ('seed'); read (val, *, iostat=ios) seed
+         case ('c'); read (val, *, iostat=ios) c
+         case ('c_ompe'); read (val, *, iostat=ios) c_ompe
+         case ('energy_save_period'); read (val, *, iostat=ios) energy_save_period
+         case ('prtl_reorder_period'); read (val, *, iostat=ios) prtl_reorder_period
+         case default
+            write (*, '(a,a)') 'unknown parameter ', trim(key)
+            error stop 2
+         end select
+      end do
+   end subroutine ReadArgs
+
+   ! Initialisations, MAIN.F90:338-395: InitScale and the allocations happen in
+   ! pictor_init; InitSetup loads the particles; InitAll_gpu is the upload.
+   subroutine Initialisations
+      type(pictor_config) :: cfg
+      mx = nx + 5
+      my = ny + 5
+      mz = 1
+      allocate (Ex(mx, my, mz), Ey(mx, my, mz), Ez(mx, my, mz))
+      allocate (Bx(mx, my, mz), By(mx, my, mz), Bz(mx, my, mz))
+      allocate (Jx(mx, my, mz), Jy(mx, my, mz), Jz(mx, my, mz))
+      Ex = 0; Ey = 0; Ez = 0; Bx = 0; By = 0; Bz = 0; Jx = 0; Jy = 0; Jz = 0
+      call InitSetup
+      cfg = pictor_config(2, nx, ny, 1, epc, form, 0, 0, c, c_ompe, g0, mi_me, 0.0d0)
+      call check(pictor_init(cfg, gpu), 'pictor_init')
+      call check(pictor_upload_fields(gpu, Ex, Ey, Ez, Bx, By, Bz), 'pictor_upload_fields')
+      call check(pictor_upload_particles(gpu, used_prtl_arr_size, qp, xp, yp, zp, up, vp, wp, flvp), &
+                 'pictor_upload_particles')
+      write (*, '(a,i0,a,i0,a,i0,a,i0)') 'nx ', nx, ' ny ', ny, ' particles ', used_prtl_arr_size, ' form ', form
+   end subroutine Initialisations
+
+   subroutine UpdateBfieldHalf
+      call check(pictor_update_b_half(gpu), 'UpdateBfieldHalf')
+   end subroutine UpdateBfieldHalf
+
+   subroutine MoveDepositPrtl
+      call check(pictor_move_deposit(gpu), 'MoveDepositPrtl')
+   end subroutine MoveDepositPrtl
+
+   subroutine ExchangePrtl
+      call check(pictor_exchange_particles(gpu), 'ExchangePrtl')
+   end subroutine ExchangePrtl
+
+   subroutine UpdateEfield
+      call check(pictor_update_e(gpu), 'UpdateEfield')
+   end subroutine UpdateEfield
+
+   subroutine UpdateCurrentAllFaces
+      call check(pictor_current_halo_add(gpu), 'UpdateCurrentAllFaces')
+   end subroutine UpdateCurrentAllFaces
+
+   subroutine AddCurrent
+      call check(pictor_add_current(gpu), 'AddCurrent')
+   end subroutine AddCurrent
+
+   subroutine ResetCurrent
+      call check(pictor_reset_current(gpu), 'ResetCurrent')
+   end subroutine ResetCurrent
+
+   subroutine ReorderPrtl
+      if (prtl_reorder_period > 0) then
+         if (modulo(t, prtl_reorder_period) == 0) call check(pictor_sort_tiles(gpu), 'ReorderPrtl')
+      end if
+   end subroutine ReorderPrtl
+
+   ! CalcEnergy (savedata_routines.F90:265-327) on the device, FP64 reductions
+   subroutine SaveEnergy(step)
+      integer, intent(in) :: step
+      real(c_double) :: d(9)
+      call check(pictor_diagnostics(gpu, d), 'pictor_diagnostics')
+      write (*, '(a,i8,5es26.17)') 'energy ', step, d(1), d(2), d(3), d(4), d(8)
+   end subroutine SaveEnergy
+
+end program MAIN

@@ -1,0 +1,29 @@
+! parameters.F90 -- run parameters of the Fortran host (parameters.F90:5-87 of the reference).
+!
+! The reference fixes these at compile time; the same names are kept here.
+! The defaults are the 2-D Weibel configuration of the benchmark (256 x 256
+! cells, 16 particles per cell per species, Courant 0.45, 10 cells per skin
+! depth, e-/e+ pair plasma).  Every value can be overridden on the command
+! line as name=value (see MAIN.F90).
+module parameters
+   implicit none
+   integer, parameter :: psn = kind(1.0d0)          ! computation precision of the host arrays
+   integer :: nx = 256                               ! no. of grid cells along x
+   integer :: ny = 256                               ! no. of grid cells along y
+   integer :: nz = 1                                 ! 2-D
+   integer :: tfinish = 100                          ! number of time steps
+   real(psn) :: g0 = 1.0_psn                         ! Lorentz factor scale of the skin depth
+   integer :: epc = 16                               ! particles per cell per species
+   real(psn) :: mi_me = 1.0_psn                      ! pair plasma
+   real(psn) :: c_ompe = 10.0_psn                    ! cells per skin depth
+   real(psn) :: c = 0.45_psn                         ! Courant number
+   integer, parameter :: curr_filter = 0
+   integer :: prtl_reorder_period = 20               ! counting sort into tiles every 20 steps
+   integer :: energy_save_period = 10                ! print CalcEnergy every this many steps
+   ! run-time choices of the GPU path
+   integer :: form = 1                               ! 0 = FP64 reference SoA, 1 = FP32 split
+   integer :: fused = 0                              ! 1 = pictor_step per step, 0 = one call per phase
+   real(psn) :: drift = 0.5_psn                      ! beam drift (units of c)
+   real(psn) :: uth = 0.01_psn                       ! thermal spread of gamma*beta
+   integer :: seed = 12345
+end module parameters
