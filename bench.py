@@ -2,7 +2,9 @@
 
 Workload (default): 2-D Weibel, 2048 x 2048 cells, 32 particles per cell per
 species, 2 species (268,435,456 particles), Courant 0.45, c/omega_p = 10
-cells, drift +-0.5c, thermal spread 0.01c.  A step is one whole time step
+cells, drift +-0.5c, thermal spread 0.01c.  --config weibel3d_256: 3-D
+Weibel, 256^3 cells, 8 particles per cell per species (268,435,456
+particles), drift +-0.3c, thermal spread 0.02c.  A step is one whole time step
 (B half step, push + deposit + tile binning, B half step, current halo fold,
 E update + AddCurrent, halos) with all inputs resident in HBM.
 
@@ -26,19 +28,21 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12  # B/s, MI355X
 
 CONFIGS = {
-    # name: (nx, ny, epc per species)
-    "weibel2d_2048": (2048, 2048, 32),
-    "weibel2d_512": (512, 512, 16),
-    "weibel2d_256": (256, 256, 16),
+    # name: (ndim, nx, ny, nz, epc per species, drift, thermal spread)
+    "weibel2d_2048": (2, 2048, 2048, 1, 32, 0.5, 0.01),
+    "weibel2d_512": (2, 512, 512, 1, 16, 0.5, 0.01),
+    "weibel2d_256": (2, 256, 256, 1, 16, 0.5, 0.01),
+    "weibel3d_256": (3, 256, 256, 256, 8, 0.3, 0.02),
+    "weibel3d_128": (3, 128, 128, 128, 8, 0.3, 0.02),
 }
 
 # algorithmic bytes per particle update of the push kernel (read + write)
 BYTES_PER_PARTICLE = {"fp32_split": 2 * (4 + 3 * 4 + 3 * 4), "fp64": 7 * 8 + 4 + 6 * 8}
-# grid bytes per cell of the push kernel: read E,B (6 values), write J (3 values)
-GRID_BYTES = {"fp32_split": 6 * 4 + 3 * 4, "fp64": 6 * 8 + 3 * 8}
+# grid bytes per cell of the push kernel: read E,B (6 values), write J (3 FP64 values)
+GRID_BYTES = {"fp32_split": 6 * 4 + 3 * 8, "fp64": 6 * 8 + 3 * 8}
 
 
-def measured_traffic(form, workload, tile):
+def measured_traffic(form, workload, tile, ndim):
     # HBM bytes per launch of the push kernel from the committed rocprofv3 counter runs
     # (FETCH_SIZE x 2.0 + WRITE_SIZE, see the file named by "source"), or None
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -47,16 +51,16 @@ def measured_traffic(form, workload, tile):
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    e = t.get("k_push2d_split" if form == "fp32_split" else "k_push2d_f64")
+    e = t.get(f"k_push{ndim}d_" + ("split" if form == "fp32_split" else "f64"))
     if e and e.get("workload") == workload and e.get("tile") == tile:
         return e["bytes_per_launch"]
     return None
 
 
-def run_gpu(form, nx, ny, epc, P, steps, warmup, calibrate=False, tile=0):
+def run_gpu(form, ndim, nx, ny, nz, epc, P, steps, warmup, calibrate=False, tile=0):
     from pictor_amd import Simulation
 
-    sim = Simulation(nx, ny, epc=epc, c=0.45, c_ompe=10.0, form=form, tile=tile)
+    sim = Simulation(nx, ny, nz, ndim=ndim, epc=epc, c=0.45, c_ompe=10.0, form=form, tile=tile)
     t0 = time.time()
     sim.upload_particles(**P)
     sim.synchronize()
@@ -82,7 +86,7 @@ def run_gpu(form, nx, ny, epc, P, steps, warmup, calibrate=False, tile=0):
     tile = sim.tile
     sim.close()
     avg_push = push_ms / max(npush, 1) / 1e3
-    ncell = nx * ny
+    ncell = nx * ny * (nz if ndim == 3 else 1)
     bytes_launch = np_ * BYTES_PER_PARTICLE[form] + ncell * GRID_BYTES[form]
     achieved = bytes_launch / avg_push
     return {
@@ -102,10 +106,10 @@ def run_gpu(form, nx, ny, epc, P, steps, warmup, calibrate=False, tile=0):
     }
 
 
-def run_cpu(nx, ny, epc, P, steps, threads):
+def run_cpu(ndim, nx, ny, nz, epc, P, steps, threads):
     from oracle.pyoracle import Oracle
 
-    o = Oracle(2, nx, ny, epc=epc, c=0.45, c_ompe=10.0, nthreads=threads, cap=int(len(P["x"]) * 1.05) + 4096)
+    o = Oracle(ndim, nx, ny, nz, epc=epc, c=0.45, c_ompe=10.0, nthreads=threads, cap=int(len(P["x"]) * 1.05) + 4096)
     o.load_particles(**P)
     o.step(1)  # warm-up step (first touch of the work arrays)
     t0 = time.perf_counter()
@@ -135,25 +139,26 @@ def main():
     import numpy as np  # noqa: F401
     from pictor_amd import setup_weibel
 
-    nx, ny, epc = CONFIGS[args.config]
+    ndim, nx, ny, nz, epc, drift, uth = CONFIGS[args.config]
     t0 = time.time()
-    P = setup_weibel(nx, ny, epc=epc, drift=0.5, uth=0.01, seed=12345)
+    P = setup_weibel(nx, ny, nz, ndim=ndim, epc=epc, drift=drift, uth=uth, seed=12345)
     print(f"# setup {len(P['x'])} particles in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
     forms = [f for f in args.forms.split(",") if f]
     res = {}
     for form in forms:
-        res[form] = run_gpu(form, nx, ny, epc, P, args.steps, args.warmup, args.calibrate, args.tile)
+        res[form] = run_gpu(form, ndim, nx, ny, nz, epc, P, args.steps, args.warmup, args.calibrate, args.tile)
         r = res[form]
-        r["roofline"]["traffic"] = measured_traffic(form, args.config, r["tile"])
+        r["roofline"]["traffic"] = measured_traffic(form, args.config, r["tile"], ndim)
         print(f"# {form}: {r['value']:.4e} particle-updates/s, {r['ms_per_step']:.3f} ms/step, "
               f"push {r['push_ms']:.3f} ms ({r['roofline']['frac'] * 100:.1f}% HBM)", file=sys.stderr, flush=True)
 
     cpu = None
-    cpu_steps = args.cpu_steps if args.cpu_steps >= 0 else (5 if nx >= 2048 else 20)
+    big = len(P["x"]) > 100_000_000
+    cpu_steps = args.cpu_steps if args.cpu_steps >= 0 else (2 if ndim == 3 and big else (5 if big else 20))
     if cpu_steps > 0:
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-        v, dt = run_cpu(nx, ny, epc, P, cpu_steps, threads)
+        v, dt = run_cpu(ndim, nx, ny, nz, epc, P, cpu_steps, threads)
         cpu = {"value": v, "unit": "particle-updates/s", "cores": threads, "kind": "port",
                "sample": f"{cpu_steps} full steps of the same workload ({len(P['x'])} particles, FP64 oracle, "
                          f"OpenMP) after 1 warm-up step, {dt:.1f}s"}
@@ -174,8 +179,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if head == "fp32_split" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: 2-D Weibel {nx}x{ny}, {epc} ppc/species, 2 species, "
-                               f"{r['particles']} particles, c=0.45, c/wp=10 cells",
+        "config": {"workload": f"{args.config}: {ndim}-D Weibel "
+                               + ("x".join(str(v) for v in ((nx, ny, nz) if ndim == 3 else (nx, ny))))
+                               + f", {epc} ppc/species, 2 species, {r['particles']} particles, drift +-{drift}c, "
+                               f"thermal {uth}c, c=0.45, c/wp=10 cells",
                    "form": head, "tile": r["tile"], "particles": r["particles"]},
         "roofline": r["roofline"],
         "cpu_baseline": cpu,

@@ -146,7 +146,11 @@ hipError_t alloc_split(PartSplit &p, long long n, bool with_sp)
 int ensure_buffer(pictor_ctx *c, int which, long long need)
 {
     if (c->buf_cap[which] >= need) return PICTOR_OK;
+    // the split-form kernels address a buffer with 32-bit byte offsets of 4-byte elements
+    if (c->form != PICTOR_FORM_FP64 && need >= (1LL << 30))
+        return fail(PICTOR_ERR_CAPACITY, "FP32 split form: more than 2^30 particle slots per buffer");
     long long n = (long long)(need * 1.1) + 1024;
+    if (c->form != PICTOR_FORM_FP64) n = std::min(n, (1LL << 30) - 1);
     if (c->form == PICTOR_FORM_FP64) {
         free_f64(c->pf[which]);
         HIPTRY(alloc_f64(c->pf[which], n));
@@ -372,18 +376,24 @@ int do_push(pictor_ctx *c, bool timed)
     BinsP bins{c->d_base, c->d_cap, c->d_cnt[a], c->d_cnt[b]};
     SpillP sp{c->d_spill_count, c->spill_cap, c->d_err};
     if (timed) HIPTRY(hipEventRecord(c->ev[0], c->stream));
-    if (c->g.ndim != 2) return fail(PICTOR_ERR_ARG, "3-D push not available");
+    const bool three = (c->g.ndim == 3);
     if (c->form == PICTOR_FORM_FP64) {
         double *E[3], *B[3], *J[3];
         field_ptrs(c, E, B);
         for (int i = 0; i < 3; i++) J[i] = (double *)c->J[i];
-        launch_push2d_f64(c->tp.T, c->g, c->ph, c->tp, E, B, J, c->pf[a], c->pf[b], bins, c->spill_f, sp, c->stream);
+        if (three)
+            launch_push3d_f64(c->tp.T, c->g, c->ph, c->tp, E, B, J, c->pf[a], c->pf[b], bins, c->spill_f, sp, c->stream);
+        else
+            launch_push2d_f64(c->tp.T, c->g, c->ph, c->tp, E, B, J, c->pf[a], c->pf[b], bins, c->spill_f, sp, c->stream);
     } else {
         float *E[3], *B[3];
         double *J[3];
         field_ptrs(c, E, B);
         for (int i = 0; i < 3; i++) J[i] = (double *)c->J[i];
-        launch_push2d_split(c->tp.T, c->g, c->ph, c->tp, E, B, J, c->ps[a], c->ps[b], bins, c->spill_s, sp, c->stream);
+        if (three)
+            launch_push3d_split(c->tp.T, c->g, c->ph, c->tp, E, B, J, c->ps[a], c->ps[b], bins, c->spill_s, sp, c->stream);
+        else
+            launch_push2d_split(c->tp.T, c->g, c->ph, c->tp, E, B, J, c->ps[a], c->ps[b], bins, c->spill_s, sp, c->stream);
     }
     HIPTRY(hipGetLastError());
     if (timed) HIPTRY(hipEventRecord(c->ev[1], c->stream));
@@ -435,14 +445,18 @@ int do_e_phase(pictor_ctx *c, bool curl, bool add)
 
 int choose_tile(const pictor_config *cfg)
 {
-    const int cands[3] = {32, 16, 8};
+    // 3-D windows are (T+3)^3 / (T+5)^3 nodes: 8 is the largest tile whose LDS fits
+    const int cands2[3] = {32, 16, 8}, cands3[2] = {8, 4};
+    const int *cands = cfg->ndim == 3 ? cands3 : cands2;
+    const int ncand = cfg->ndim == 3 ? 2 : 3;
     auto fits = [&](int T) {
         if (cfg->nx % T || cfg->ny % T) return false;
         if (cfg->ndim == 3 && cfg->nz % T) return false;
         return true;
     };
     int best = 0;
-    for (int T : cands) {
+    for (int q = 0; q < ncand; q++) {
+        const int T = cands[q];
         if (!fits(T)) continue;
         long long nt = (long long)(cfg->nx / T) * (cfg->ny / T) * (cfg->ndim == 3 ? cfg->nz / T : 1);
         if (nt >= 1024) return T;
@@ -461,15 +475,18 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
 {
     if (!cfg || !out) return fail(PICTOR_ERR_ARG, "null argument");
     *out = nullptr;
-    if (cfg->ndim != 2) return fail(PICTOR_ERR_ARG, "only ndim = 2 is available");
-    if (cfg->nx < 8 || cfg->ny < 8) return fail(PICTOR_ERR_ARG, "nx, ny must be >= 8");
+    if (cfg->ndim != 2 && cfg->ndim != 3) return fail(PICTOR_ERR_ARG, "ndim must be 2 or 3");
+    if (cfg->nx < 8 || cfg->ny < 8 || (cfg->ndim == 3 && cfg->nz < 8)) return fail(PICTOR_ERR_ARG, "nx, ny (, nz) must be >= 8");
     if (cfg->ndim == 2 && (cfg->nx > 32768 || cfg->ny > 32768)) return fail(PICTOR_ERR_ARG, "2-D nx, ny must be <= 32768");
+    if (cfg->ndim == 3 && (cfg->nx > 2048 || cfg->ny > 2048 || cfg->nz > 512))
+        return fail(PICTOR_ERR_ARG, "3-D nx, ny must be <= 2048 and nz <= 512");
     if (cfg->form != PICTOR_FORM_FP64 && cfg->form != PICTOR_FORM_FP32_SPLIT) return fail(PICTOR_ERR_ARG, "unknown form");
     if (!(cfg->c > 0.0 && cfg->c < 1.0)) return fail(PICTOR_ERR_ARG, "Courant number c must lie in (0,1)");
     if (cfg->epc < 1 || cfg->c_ompe <= 0 || cfg->mi_me <= 0) return fail(PICTOR_ERR_ARG, "invalid epc, c_ompe or mi_me");
     int T = cfg->tile ? cfg->tile : choose_tile(cfg);
-    if (T != 8 && T != 16 && T != 32) return fail(PICTOR_ERR_ARG, "tile must be 8, 16 or 32 and divide nx, ny");
-    if (cfg->nx % T || cfg->ny % T) return fail(PICTOR_ERR_ARG, "tile must divide nx and ny");
+    if (cfg->ndim == 2 && T != 8 && T != 16 && T != 32) return fail(PICTOR_ERR_ARG, "2-D tile must be 8, 16 or 32 and divide nx, ny");
+    if (cfg->ndim == 3 && T != 4 && T != 8) return fail(PICTOR_ERR_ARG, "3-D tile must be 4 or 8 and divide nx, ny, nz");
+    if (cfg->nx % T || cfg->ny % T || (cfg->ndim == 3 && cfg->nz % T)) return fail(PICTOR_ERR_ARG, "tile must divide the grid");
 
     pictor_ctx *c = new pictor_ctx();
     c->cfg = *cfg;

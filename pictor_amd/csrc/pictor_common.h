@@ -134,6 +134,14 @@ void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp
                          float *const B[3], double *const J[3], PartSplit A, PartSplit Bo, BinsP bins,
                          PartSplit spill, SpillP sp, hipStream_t s);
 
+// kernels_push3d.hip: T = 4 or 8
+void launch_push3d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, double *const E[3],
+                       double *const B[3], double *const J[3], PartF64 A, PartF64 Bo, BinsP bins,
+                       PartF64 spill, SpillP sp, hipStream_t s);
+void launch_push3d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp, float *const E[3],
+                         float *const B[3], double *const J[3], PartSplit A, PartSplit Bo, BinsP bins,
+                         PartSplit spill, SpillP sp, hipStream_t s);
+
 // ---------------- launchers (kernels_layout.hip) ----------------
 // raw FP64 particles (upload): tile id per particle + per (tile, species) counts
 void launch_raw_tiles_f64(const GridP &g, const PhysP &ph, const TileP &tp, PartF64 raw, long long n,

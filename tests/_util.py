@@ -62,3 +62,10 @@ def tiles_of(P, nx, ny, T):
     cx = np.clip(np.floor(P["x"]).astype(np.int64) - 3, 0, nx - 1)
     cy = np.clip(np.floor(P["y"]).astype(np.int64) - 3, 0, ny - 1)
     return (cy // T) * (nx // T) + cx // T
+
+
+def tiles_of3(P, nx, ny, nz, T):
+    cx = np.clip(np.floor(P["x"]).astype(np.int64) - 3, 0, nx - 1)
+    cy = np.clip(np.floor(P["y"]).astype(np.int64) - 3, 0, ny - 1)
+    cz = np.clip(np.floor(P["z"]).astype(np.int64) - 3, 0, nz - 1)
+    return ((cz // T) * (ny // T) + cy // T) * (nx // T) + cx // T
