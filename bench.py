@@ -118,6 +118,186 @@ def run_cpu(ndim, nx, ny, nz, epc, P, steps, threads):
     return len(P["x"]) * steps / dt, dt
 
 
+def _series_gpu(form, ndim, nx, ny, nz, epc, P, steps, every):
+    from pictor_amd import Simulation
+
+    rows = []
+    with Simulation(nx, ny, nz, ndim=ndim, epc=epc, c=0.45, c_ompe=10.0, form=form) as sim:
+        sim.upload_particles(**P)
+        rows.append((0, sim.diagnostics()))
+        for t in range(every, steps + 1, every):
+            sim.step(every)
+            rows.append((t, sim.diagnostics()))
+    return rows
+
+
+def _series_oracle(ndim, nx, ny, nz, epc, P, steps, every, threads):
+    from oracle.pyoracle import Oracle
+
+    o = Oracle(ndim, nx, ny, nz, epc=epc, c=0.45, c_ompe=10.0, nthreads=threads,
+               cap=int(len(P["x"]) * 1.05) + 4096)
+    o.load_particles(**P)
+
+    def diag():
+        e = o.energy()
+        g = o.gauss()
+        return {"B_energy": e[0], "E_energy": e[1], "prtl_energy_1": e[2], "prtl_energy_2": e[3],
+                "gauss_residual": g[0], "rho_max": g[1]}
+
+    rows = [(0, diag())]
+    for t in range(1, steps + 1):
+        o.step(t)
+        if t % every == 0:
+            rows.append((t, diag()))
+    return rows
+
+
+def _growth_window(ref):
+    # linear phase of the instability: magnetic energy between 10x its smallest
+    # value and 1/10 of its largest (the reference series decides the window)
+    import numpy as np
+
+    t = np.array([r[0] for r in ref], dtype=float)
+    eb = np.array([r[1]["B_energy"] for r in ref])
+    lo, hi = np.log(eb[eb > 0].min()) + np.log(10.0), np.log(eb.max()) - np.log(10.0)
+    le = np.log(np.maximum(eb, 1e-300))
+    i_hi = int(np.argmax(le >= hi)) if np.any(le >= hi) else len(le) - 1
+    cand = np.nonzero((le >= lo) & (np.arange(len(le)) <= i_hi))[0]
+    if len(cand) < 3:
+        return None
+    return t[cand[0]], t[i_hi]
+
+
+def _growth_rate(rows, window, ompe):
+    import numpy as np
+
+    if window is None:
+        return None
+    t = np.array([r[0] for r in rows], dtype=float)
+    eb = np.array([r[1]["B_energy"] for r in rows])
+    m = (t >= window[0]) & (t <= window[1])
+    slope = np.polyfit(t[m], np.log(eb[m]), 1)[0]
+    return slope / 2.0 / ompe     # amplitude growth rate in units of omega_p
+
+
+def accuracy_report(args, ndim, nx, ny, nz, epc, P):
+    """FP32 split form vs the FP64 forms: field energy, growth rate and Gauss residual over a run."""
+    import numpy as np
+
+    steps, every = args.steps, args.every
+    ompe = 0.45 / 10.0
+    series = {}
+    for form in ("fp64", "fp32_split"):
+        t0 = time.time()
+        series[form] = _series_gpu(form, ndim, nx, ny, nz, epc, P, steps, every)
+        print(f"# {form}: {steps} steps in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    if args.cpu_steps != 0:
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        t0 = time.time()
+        series["oracle"] = _series_oracle(ndim, nx, ny, nz, epc, P, steps, every, threads)
+        print(f"# oracle: {steps} steps in {time.time() - t0:.1f}s on {threads} threads", file=sys.stderr, flush=True)
+    ref_name = "oracle" if "oracle" in series else "fp64"
+    ref = series[ref_name]
+    window = _growth_window(ref)
+
+    def fe(r):
+        return r[1]["B_energy"] + r[1]["E_energy"]
+
+    out = {"config": args.config, "steps": steps, "every": every, "reference": ref_name,
+           "growth_window_steps": window, "forms": {}}
+    for form, rows in series.items():
+        if form == ref_name:
+            continue
+        err = np.array([abs(fe(a) - fe(b)) / fe(b) for a, b in zip(rows, ref) if fe(b) > 0])
+        errb = np.array([abs(a[1]["B_energy"] - b[1]["B_energy"]) / b[1]["B_energy"]
+                         for a, b in zip(rows, ref) if b[1]["B_energy"] > 0])
+        lin = [i for i, r in enumerate(ref) if window and r[0] <= window[1]]
+        out["forms"][form] = {
+            "field_energy_rel_err_max": float(err.max()),
+            "field_energy_rel_err_final": float(err[-1]),
+            "field_energy_rel_err_max_linear_phase": float(err[lin].max()) if lin else None,
+            "B_energy_rel_err_max": float(errb.max()),
+            "growth_rate": _growth_rate(rows, window, ompe),
+            # max over the run of max|div E - rho|, relative to the run's max|rho| (at t = 0 rho is
+            # zero up to rounding: the two species start at the same positions)
+            "gauss_residual_rel_max": float(max(r[1]["gauss_residual"] for r in rows)
+                                            / max(max(r[1]["rho_max"] for r in rows), 1e-300)),
+        }
+    out["growth_rate_reference"] = _growth_rate(ref, window, ompe)
+    out["gauss_residual_rel_max_reference"] = float(max(r[1]["gauss_residual"] for r in ref)
+                                                    / max(max(r[1]["rho_max"] for r in ref), 1e-300))
+    out["series"] = {f: [(t, d["B_energy"], d["E_energy"], d["gauss_residual"], d["rho_max"]) for t, d in rows]
+                     for f, rows in series.items()}
+    return out
+
+
+def adjoint_report(args):
+    """C5: gradient of the final magnetic energy of the 2-D Weibel run with
+    respect to the drift speed and all initial momenta, FP32 split forward
+    with checkpoints every --segment steps; checked against central finite
+    differences of the FP64 engine (eps = 1e-4) and the FP64 adjoint."""
+    import numpy as np
+    import torch
+
+    from pictor_amd import Simulation, setup_weibel
+    from pictor_amd.adjoint import ParticleStepObjective, drift_boost, drift_signs
+
+    nx = ny = args.adjoint_n
+    epc, drift, uth, nsteps, seg = args.adjoint_epc, 0.5, 0.01, args.steps, args.segment
+    P0 = setup_weibel(nx, ny, epc=epc, drift=0.0, uth=uth, seed=12345)
+    sgn = torch.from_numpy(drift_signs(len(P0["x"]), epc))
+    th = [torch.from_numpy(P0[k]) for k in ("u", "v", "w")]
+    out = {"workload": f"adjoint: 2-D Weibel {nx}x{ny}, {epc} ppc/species, {len(P0['x'])} particles, "
+                       f"{nsteps} steps, checkpoints every {seg}", "forms": {}}
+    grads = {}
+    for form in ("fp32_split", "fp64"):
+        d = torch.tensor(drift, dtype=torch.float64, requires_grad=True)
+        u, v, w = drift_boost(th[0], th[1], th[2], sgn, d)
+        for t in (u, v, w):
+            t.retain_grad()
+        with Simulation(nx, ny, epc=epc, form=form) as sim:
+            L = ParticleStepObjective.apply(u, v, w, sim, P0, nsteps, seg)
+            L.backward()
+            st = sim.adjoint_stats()
+            # plain forward of the same steps with the production kernels
+            P = dict(P0)
+            P["u"], P["v"], P["w"] = u.detach().numpy(), v.detach().numpy(), w.detach().numpy()
+            sim.upload_particles(**P)
+            sim.synchronize()
+            t0 = time.perf_counter()
+            sim.step(nsteps)
+            sim.synchronize()
+            t_prod = (time.perf_counter() - t0) * 1e3
+            L_prod = sim.diagnostics()["B_energy"]
+        grads[form] = (float(d.grad), [t.grad.numpy().copy() for t in (u, v, w)], float(L))
+        out["forms"][form] = {
+            "objective": float(L), "objective_production_forward": L_prod, "d_drift": float(d.grad),
+            "forward_ms": st["forward_ms"], "recompute_ms": st["recompute_ms"], "reverse_ms": st["reverse_ms"],
+            "production_forward_ms": t_prod, "device_bytes": st["device_bytes"],
+            "backward_over_forward": (st["recompute_ms"] + st["reverse_ms"]) / st["forward_ms"],
+            "gradient_over_production_forward": (st["forward_ms"] + st["recompute_ms"] + st["reverse_ms"]) / t_prod,
+        }
+        print(f"# adjoint {form}: {out['forms'][form]}", file=sys.stderr, flush=True)
+    # central finite difference of the FP64 objective in the drift speed
+    eps = 1e-4
+    vals = []
+    with Simulation(nx, ny, epc=epc, form="fp64") as sim:
+        for dd in (drift + eps, drift - eps):
+            u, v, w = drift_boost(th[0], th[1], th[2], sgn, torch.tensor(dd, dtype=torch.float64))
+            vals.append(sim.adjoint_forward(P0["q"], P0["x"], P0["y"], P0["z"], u.numpy(), v.numpy(), w.numpy(),
+                                            P0["flv"], nsteps, seg))
+    fd = (vals[0] - vals[1]) / (2 * eps)
+    g64, g32 = grads["fp64"], grads["fp32_split"]
+    num = np.sqrt(sum(np.sum((g32[1][k] - g64[1][k]) ** 2) for k in range(3)))
+    den = np.sqrt(sum(np.sum(g64[1][k] ** 2) for k in range(3)))
+    out["fd_drift_fp64"] = fd
+    out["rel_err_fp64_adjoint_vs_fd"] = abs(g64[0] - fd) / abs(fd)
+    out["rel_err_split_adjoint_vs_fd"] = abs(g32[0] - fd) / abs(fd)
+    out["rel_err_split_vs_fp64_drift"] = abs(g32[0] - g64[0]) / abs(g64[0])
+    out["rel_l2_split_vs_fp64_momenta"] = float(num / den)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,12 +309,26 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--calibrate", action="store_true", help="also run plain copy kernels (counter scaling)")
     ap.add_argument("--tile", type=int, default=0, help="tile edge in cells (0 = library default)")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="instead of timing: run --steps steps of both forms (and the oracle unless "
+                         "--cpu-steps 0) and report field energy, growth rate and Gauss residual errors")
+    ap.add_argument("--every", type=int, default=10, help="diagnostic interval of --accuracy")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="instead of timing the step: the adjoint workload (gradients, FD check, cost ratios) "
+                         "over --steps steps with checkpoints every --segment")
+    ap.add_argument("--segment", type=int, default=20)
+    ap.add_argument("--adjoint-n", type=int, default=512)
+    ap.add_argument("--adjoint-epc", type=int, default=16)
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU slab decomposition is a design, not code)", file=sys.stderr)
         sys.exit(2)
+
+    if args.adjoint:
+        print(json.dumps(adjoint_report(args)), flush=True)
+        return
 
     import numpy as np  # noqa: F401
     from pictor_amd import setup_weibel
@@ -143,6 +337,10 @@ def main():
     t0 = time.time()
     P = setup_weibel(nx, ny, nz, ndim=ndim, epc=epc, drift=drift, uth=uth, seed=12345)
     print(f"# setup {len(P['x'])} particles in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    if args.accuracy:
+        print(json.dumps(accuracy_report(args, ndim, nx, ny, nz, epc, P)), flush=True)
+        return
 
     forms = [f for f in args.forms.split(",") if f]
     res = {}

@@ -131,6 +131,28 @@ int pictor_setup_weibel(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, int32_
                         double *q, double *x, double *y, double *z, double *u, double *v,
                         double *w, int32_t *flv, int64_t *n_out);
 
+/* ---- adjoint of the 2-D particle step (no counterpart in the reference:
+ * the hand-written adjoint of push + deposit + FDTD).  Objective
+ * L = 1/2 sum B^2 over the physical cells after nsteps steps, started from
+ * the context's current E and B and the given particles (caller's order,
+ * reference SoA, q = flavour charge).  The context's form selects the
+ * arithmetic: FP32 split (hardware rsq/rcp, fixed-point current) or FP64.
+ * pictor_adjoint_forward runs the steps keeping a checkpoint every
+ * `segment` steps and returns L; pictor_adjoint_backward recomputes each
+ * segment from its checkpoint and runs the reverse steps, writing
+ * scale * dL/du, dL/dv, dL/dw of the initial momenta (any may be NULL).
+ * The context's own particles and fields are not touched.
+ * pictor_adjoint_stats: out[0] forward ms, out[1] recompute ms, out[2]
+ * reverse-step ms, out[3] device bytes of the engine, out[4] nsteps,
+ * out[5] segment, out[6] checkpoints, out[7] L.                           */
+int pictor_adjoint_forward(pictor_ctx *ctx, int64_t n, const double *q, const double *x,
+                           const double *y, const double *z, const double *u, const double *v,
+                           const double *w, const int32_t *flv, int32_t nsteps, int32_t segment,
+                           double *objective);
+int pictor_adjoint_backward(pictor_ctx *ctx, double scale, double *du, double *dv, double *dw);
+int pictor_adjoint_stats(pictor_ctx *ctx, double out[8]);
+int pictor_adjoint_release(pictor_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,7 +27,8 @@ EXPORTS = (
     "pictor_current_halo_add", "pictor_add_current", "pictor_field_halo",
     "pictor_reset_current", "pictor_sort_tiles", "pictor_step", "pictor_synchronize",
     "pictor_diagnostics", "pictor_timer_reset", "pictor_timer_get", "pictor_memory_bytes",
-    "pictor_setup_weibel",
+    "pictor_setup_weibel", "pictor_adjoint_forward", "pictor_adjoint_backward", "pictor_adjoint_stats",
+    "pictor_adjoint_release",
 )
 
 
@@ -83,6 +84,10 @@ def lib():
     L.pictor_memory_bytes.argtypes = [c_vp, ctypes.POINTER(i64)]
     L.pictor_setup_weibel.argtypes = [i32, i32, i32, i32, i32, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                                       i64] + [c_dp] * 7 + [c_ip, ctypes.POINTER(i64)]
+    L.pictor_adjoint_forward.argtypes = [c_vp, i64] + [c_dp] * 7 + [c_ip, i32, i32, c_dp]
+    L.pictor_adjoint_backward.argtypes = [c_vp, ctypes.c_double, c_dp, c_dp, c_dp]
+    L.pictor_adjoint_stats.argtypes = [c_vp, c_dp]
+    L.pictor_adjoint_release.argtypes = [c_vp]
     for name in EXPORTS:
         if name != "pictor_last_error":
             getattr(L, name).restype = ctypes.c_int

@@ -13,86 +13,14 @@
 #include <vector>
 
 #include "../../include/pictor_hip.h"
+#include "ctx.h"
 #include "pictor_common.h"
 
 using namespace pictor;
 
-namespace {
-
+namespace pictor {
 thread_local std::string g_err;
-
-int fail(int code, const std::string &msg)
-{
-    g_err = msg;
-    return code;
 }
-
-#define HIPTRY(expr)                                                                                    \
-    do {                                                                                                \
-        hipError_t e_ = (expr);                                                                         \
-        if (e_ != hipSuccess)                                                                           \
-            return fail(PICTOR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
-    } while (0)
-
-inline void dfree(void *p) { (void)hipFree(p); }
-
-template <typename T>
-hipError_t dalloc(T **p, size_t n)
-{
-    *p = nullptr;
-    if (n == 0) n = 1;
-    return hipMalloc((void **)p, n * sizeof(T));
-}
-
-}  // namespace
-
-struct pictor_ctx {
-    pictor_config cfg{};
-    GridP g{};
-    PhysP ph{};
-    TileP tp{};
-    int form = 0;
-    int dev = 0;
-    hipStream_t stream = nullptr;
-
-    // fields: double (FP64 form) or float (split form); J double or int64 fixed point
-    void *E[3] = {nullptr, nullptr, nullptr};
-    void *B[3] = {nullptr, nullptr, nullptr};
-    void *J[3] = {nullptr, nullptr, nullptr};
-
-    // tile bins
-    std::vector<long long> h_base;
-    std::vector<int> h_cap;
-    long long *d_base = nullptr;
-    int *d_cap = nullptr;
-    int *d_cnt[2] = {nullptr, nullptr};
-    long long layout_cap = 0;   // particle slots the layout needs per buffer
-
-    // particle buffers: [0],[1] ping-pong, spill, staging for host transfers
-    PartF64 pf[2]{};
-    PartSplit ps[2]{};
-    long long buf_cap[2] = {0, 0};
-    int cur = 0;
-    bool pending_swap = false;
-    PartF64 spill_f{};
-    PartSplit spill_s{};
-    int spill_cap = 0;
-    int *d_spill_count = nullptr;
-    int *d_err = nullptr;
-    int *h_flags = nullptr;   // pinned: {spill count, error bits}
-
-    // diagnostics scratch
-    double *d_partial = nullptr;
-    int partial_len = 0;
-    double *d_rho = nullptr;
-
-    // timers: 0 push, 1 field kernels, 2 whole step
-    hipEvent_t ev[8] = {};
-    double t_ms[3] = {0, 0, 0};
-    long long t_n[3] = {0, 0, 0};
-
-    long long np = 0;
-};
 
 namespace {
 
@@ -579,6 +507,8 @@ int pictor_finalize(pictor_ctx *c)
     if (!c) return PICTOR_OK;
     hipSetDevice(c->dev);
     if (c->stream) hipStreamSynchronize(c->stream);
+    delete c->adj;
+    c->adj = nullptr;
     for (int i = 0; i < 3; i++) { dfree(c->E[i]); dfree(c->B[i]); dfree(c->J[i]); }
     dfree(c->d_base); dfree(c->d_cap); dfree(c->d_cnt[0]); dfree(c->d_cnt[1]);
     dfree(c->d_spill_count); dfree(c->d_err);

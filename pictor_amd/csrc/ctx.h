@@ -1,0 +1,99 @@
+// ctx.h -- the context behind the opaque pictor_ctx handle (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+
+#include "../../include/pictor_hip.h"
+#include "pictor_common.h"
+
+namespace pictor {
+
+// message of the last failed call on this thread (pictor_last_error)
+extern thread_local std::string g_err;
+
+inline int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPTRY(expr)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return ::pictor::fail(PICTOR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+inline void dfree(void *p) { (void)hipFree(p); }
+
+template <typename T>
+hipError_t dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void **)p, n * sizeof(T));
+}
+
+// adjoint engine owned by a context (adjoint_api.hip)
+struct AdjEngineBase {
+    virtual ~AdjEngineBase() {}
+};
+
+}  // namespace pictor
+
+struct pictor_ctx {
+    using GridP = pictor::GridP;
+    using PhysP = pictor::PhysP;
+    using TileP = pictor::TileP;
+    using PartF64 = pictor::PartF64;
+    using PartSplit = pictor::PartSplit;
+
+    pictor_config cfg{};
+    GridP g{};
+    PhysP ph{};
+    TileP tp{};
+    int form = 0;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+
+    // fields: double (FP64 form) or float (split form); J double or int64 fixed point
+    void *E[3] = {nullptr, nullptr, nullptr};
+    void *B[3] = {nullptr, nullptr, nullptr};
+    void *J[3] = {nullptr, nullptr, nullptr};
+
+    // tile bins
+    std::vector<long long> h_base;
+    std::vector<int> h_cap;
+    long long *d_base = nullptr;
+    int *d_cap = nullptr;
+    int *d_cnt[2] = {nullptr, nullptr};
+    long long layout_cap = 0;   // particle slots the layout needs per buffer
+
+    // particle buffers: [0],[1] ping-pong, spill, staging for host transfers
+    PartF64 pf[2]{};
+    PartSplit ps[2]{};
+    long long buf_cap[2] = {0, 0};
+    int cur = 0;
+    bool pending_swap = false;
+    PartF64 spill_f{};
+    PartSplit spill_s{};
+    int spill_cap = 0;
+    int *d_spill_count = nullptr;
+    int *d_err = nullptr;
+    int *h_flags = nullptr;   // pinned: {spill count, error bits}
+
+    // diagnostics scratch
+    double *d_partial = nullptr;
+    int partial_len = 0;
+    double *d_rho = nullptr;
+
+    // timers: 0 push, 1 field kernels, 2 whole step
+    hipEvent_t ev[8] = {};
+    double t_ms[3] = {0, 0, 0};
+    long long t_n[3] = {0, 0, 0};
+
+    long long np = 0;
+
+    pictor::AdjEngineBase *adj = nullptr;   // created by pictor_adjoint_forward
+};

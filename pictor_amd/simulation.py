@@ -171,6 +171,35 @@ class Simulation:
         check(lib().pictor_memory_bytes(self._ctx, ctypes.byref(b)))
         return b.value
 
+    # ---- adjoint of the 2-D step (pictor_adjoint_*) ----
+    def adjoint_forward(self, q, x, y, z, u, v, w, flv, nsteps, segment):
+        """Objective 1/2 sum B^2 after nsteps steps from these particles and the
+        current fields, keeping a checkpoint every `segment` steps."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (q, x, y, z, u, v, w)]
+        f = np.ascontiguousarray(flv, dtype=np.int32)
+        L = ctypes.c_double()
+        check(lib().pictor_adjoint_forward(self._ctx, len(arrs[1]), *[_p(a) for a in arrs], _p(f, _ip), nsteps,
+                                           segment, ctypes.byref(L)))
+        self._adj_n = len(arrs[1])
+        return L.value
+
+    def adjoint_backward(self, scale=1.0):
+        """scale * dL/d(u, v, w) of the initial momenta (numpy, caller's order)."""
+        n = self._adj_n
+        du, dv, dw = np.zeros(n), np.zeros(n), np.zeros(n)
+        check(lib().pictor_adjoint_backward(self._ctx, scale, _p(du), _p(dv), _p(dw)))
+        return du, dv, dw
+
+    def adjoint_stats(self):
+        out = np.zeros(8)
+        check(lib().pictor_adjoint_stats(self._ctx, _p(out)))
+        keys = ("forward_ms", "recompute_ms", "reverse_ms", "device_bytes", "nsteps", "segment", "checkpoints",
+                "objective")
+        return dict(zip(keys, out.tolist()))
+
+    def adjoint_release(self):
+        check(lib().pictor_adjoint_release(self._ctx))
+
 
 def setup_weibel(nx, ny, nz=1, ndim=2, epc=16, drift=0.5, uth=0.01, seed=12345):
     """Seeded counter-streaming pair plasma in the reference SoA (host arrays)."""

@@ -1,0 +1,141 @@
+"""Adjoint of the 2-D particle step (needs an MI355X).
+
+Bars:
+  * the engine's FP64 forward objective equals the FP64 oracle's after the
+    same steps (relative 1e-10), the FP32 engine's the production split
+    form's (relative 1e-5);
+  * FP64 adjoint gradients within 1e-3 (relative) of central finite
+    differences with eps = 1e-4: for the drift speed (through the torch
+    AddDriftVel boost), for a random direction in the space of all initial
+    momenta, and for single momentum components;
+  * the checkpoint segment length does not change the gradient (1e-10);
+  * FP32 split adjoint within 2e-2 (relative L2) of the FP64 gradient.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle.pyoracle import Oracle
+from pictor_amd import Simulation, setup_weibel
+from pictor_amd.adjoint import ParticleStepObjective, drift_boost, drift_signs
+from tests._util import say
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, EPC = 32, 32, 8
+
+
+def _load(drift, seed=21, uth=0.05):
+    P0 = setup_weibel(NX, NY, epc=EPC, drift=0.0, uth=uth, seed=seed)
+    sgn = torch.from_numpy(drift_signs(len(P0["x"]), EPC))
+    return P0, sgn
+
+
+def _objective(sim, P0, u, v, w, nsteps, segment):
+    return sim.adjoint_forward(P0["q"], P0["x"], P0["y"], P0["z"], u, v, w, P0["flv"], nsteps, segment)
+
+
+def _grad(form, drift, nsteps, segment, seed=21):
+    P0, sgn = _load(drift, seed)
+    d = torch.tensor(drift, dtype=torch.float64, requires_grad=True)
+    th = [torch.from_numpy(P0[k]) for k in ("u", "v", "w")]
+    u, v, w = drift_boost(th[0], th[1], th[2], sgn, d)
+    for t in (u, v, w):
+        t.retain_grad()
+    with Simulation(NX, NY, epc=EPC, form=form, tile=8) as sim:
+        L = ParticleStepObjective.apply(u, v, w, sim, P0, nsteps, segment)
+        L.backward()
+        st = sim.adjoint_stats()
+    say(f"{form} L={float(L):.6e} dL/ddrift={float(d.grad):.6e} stats={st}")
+    return float(L), float(d.grad), [t.grad.numpy().copy() for t in (u, v, w)], (u.detach().numpy(),
+                                                                                v.detach().numpy(),
+                                                                                w.detach().numpy()), P0
+
+
+def test_engine_forward_matches_oracle_fp64():
+    nsteps = 20
+    P = setup_weibel(NX, NY, epc=EPC, drift=0.5, uth=0.05, seed=5)
+    o = Oracle(2, NX, NY, epc=EPC, cap=2 * len(P["x"]))
+    o.load_particles(**P)
+    for t in range(1, nsteps + 1):
+        o.step(t)
+    with Simulation(NX, NY, epc=EPC, form="fp64", tile=8) as sim:
+        L = sim.adjoint_forward(P["q"], P["x"], P["y"], P["z"], P["u"], P["v"], P["w"], P["flv"], nsteps, 5)
+    Lo = o.energy()[0]
+    say("engine", L, "oracle", Lo)
+    assert abs(L - Lo) <= 1e-10 * Lo
+
+
+def test_engine_forward_matches_production_split():
+    nsteps = 20
+    P = setup_weibel(NX, NY, epc=EPC, drift=0.5, uth=0.05, seed=5)
+    with Simulation(NX, NY, epc=EPC, form="fp32_split", tile=8) as sim:
+        L = sim.adjoint_forward(P["q"], P["x"], P["y"], P["z"], P["u"], P["v"], P["w"], P["flv"], nsteps, 5)
+        sim.upload_particles(**P)
+        sim.step(nsteps)
+        Lp = sim.diagnostics()["B_energy"]
+    say("engine", L, "production", Lp, "bitwise" if L == Lp else "differs")
+    assert abs(L - Lp) <= 1e-5 * Lp
+
+
+def test_fd_drift_fp64():
+    drift, nsteps, seg, eps = 0.5, 40, 10, 1e-4
+    L, g, _, _, P0 = _grad("fp64", drift, nsteps, seg)
+    sgn = torch.from_numpy(drift_signs(len(P0["x"]), EPC))
+    th = [torch.from_numpy(P0[k]) for k in ("u", "v", "w")]
+    vals = []
+    with Simulation(NX, NY, epc=EPC, form="fp64", tile=8) as sim:
+        for dd in (drift + eps, drift - eps):
+            u, v, w = drift_boost(th[0], th[1], th[2], sgn, torch.tensor(dd, dtype=torch.float64))
+            vals.append(_objective(sim, P0, u.numpy(), v.numpy(), w.numpy(), nsteps, seg))
+    fd = (vals[0] - vals[1]) / (2 * eps)
+    say(f"drift: adjoint {g:.10e} fd {fd:.10e}")
+    assert abs(g - fd) <= 1e-3 * abs(fd)
+
+
+def test_fd_momenta_fp64():
+    drift, nsteps, seg, eps = 0.5, 40, 10, 1e-4
+    L, _, G, (u0, v0, w0), P0 = _grad("fp64", drift, nsteps, seg)
+    rng = np.random.default_rng(0)
+    r = [rng.standard_normal(len(u0)) for _ in range(3)]
+    nrm = np.sqrt(sum(float(a @ a) for a in r))
+    r = [a / nrm for a in r]
+    dirder = sum(float(G[k] @ r[k]) for k in range(3))
+    with Simulation(NX, NY, epc=EPC, form="fp64", tile=8) as sim:
+        Lp = _objective(sim, P0, u0 + eps * r[0], v0 + eps * r[1], w0 + eps * r[2], nsteps, seg)
+        Lm = _objective(sim, P0, u0 - eps * r[0], v0 - eps * r[1], w0 - eps * r[2], nsteps, seg)
+        fd = (Lp - Lm) / (2 * eps)
+        say(f"direction: adjoint {dirder:.10e} fd {fd:.10e}")
+        assert abs(dirder - fd) <= 1e-3 * abs(fd)
+        # single components: the largest gradient entries of u and w
+        for comp, base in ((0, u0), (2, w0)):
+            for i in np.argsort(-np.abs(G[comp]))[:3]:
+                pert = [u0.copy(), v0.copy(), w0.copy()]
+                pert[comp][i] += eps
+                Lp = _objective(sim, P0, *pert, nsteps, seg)
+                pert[comp][i] -= 2 * eps
+                Lm = _objective(sim, P0, *pert, nsteps, seg)
+                fd = (Lp - Lm) / (2 * eps)
+                say(f"component {comp} particle {i}: adjoint {G[comp][i]:.10e} fd {fd:.10e}")
+                assert abs(G[comp][i] - fd) <= 1e-3 * abs(fd)
+
+
+def test_segment_length_does_not_change_gradient():
+    a = _grad("fp64", 0.5, 30, 30)
+    b = _grad("fp64", 0.5, 30, 7)
+    assert abs(a[0] - b[0]) <= 1e-12 * a[0]
+    assert abs(a[1] - b[1]) <= 1e-10 * abs(a[1])
+    for k in range(3):
+        assert np.linalg.norm(a[2][k] - b[2][k]) <= 1e-10 * np.linalg.norm(a[2][k])
+
+
+def test_split_adjoint_close_to_fp64():
+    a = _grad("fp64", 0.5, 40, 10)
+    b = _grad("fp32_split", 0.5, 40, 10)
+    say(f"drift gradient fp64 {a[1]:.6e} split {b[1]:.6e}")
+    assert abs(a[0] - b[0]) <= 1e-3 * a[0]
+    assert abs(a[1] - b[1]) <= 2e-2 * abs(a[1])
+    num = np.sqrt(sum(np.sum((a[2][k] - b[2][k]) ** 2) for k in range(3)))
+    den = np.sqrt(sum(np.sum(a[2][k] ** 2) for k in range(3)))
+    say(f"momentum gradient relative L2 difference {num / den:.3e}")
+    assert num <= 2e-2 * den
