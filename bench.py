@@ -252,9 +252,9 @@ def adjoint_report(args):
     grads = {}
     for form in ("fp32_split", "fp64"):
         d = torch.tensor(drift, dtype=torch.float64, requires_grad=True)
-        u, v, w = drift_boost(th[0], th[1], th[2], sgn, d)
-        for t in (u, v, w):
-            t.retain_grad()
+        tl = [t.clone().requires_grad_(True) for t in th]
+        u, v, w = drift_boost(tl[0], tl[1], tl[2], sgn, d)
+        u.retain_grad()
         with Simulation(nx, ny, epc=epc, form=form) as sim:
             L = ParticleStepObjective.apply(u, v, w, sim, P0, nsteps, seg)
             L.backward()
@@ -278,21 +278,24 @@ def adjoint_report(args):
             "gradient_over_production_forward": (st["forward_ms"] + st["recompute_ms"] + st["reverse_ms"]) / t_prod,
         }
         print(f"# adjoint {form}: {out['forms'][form]}", file=sys.stderr, flush=True)
-    # central finite difference of the FP64 objective in the drift speed
-    eps = 1e-4
-    vals = []
-    with Simulation(nx, ny, epc=epc, form="fp64") as sim:
-        for dd in (drift + eps, drift - eps):
-            u, v, w = drift_boost(th[0], th[1], th[2], sgn, torch.tensor(dd, dtype=torch.float64))
-            vals.append(sim.adjoint_forward(P0["q"], P0["x"], P0["y"], P0["z"], u.numpy(), v.numpy(), w.numpy(),
-                                            P0["flv"], nsteps, seg))
-    fd = (vals[0] - vals[1]) / (2 * eps)
+    # central finite differences of the FP64 objective in the drift speed; the zigzag
+    # current jumps where a particle's crossing state changes, so a wide interval
+    # measures those jumps as well (see tests/test_gpu_adjoint.py)
     g64, g32 = grads["fp64"], grads["fp32_split"]
+    out["fd_drift_fp64"] = {}
+    with Simulation(nx, ny, epc=epc, form="fp64") as sim:
+        for eps in (1e-4, 1e-5, 1e-6, 1e-7, 1e-8):
+            vals = []
+            for dd in (drift + eps, drift - eps):
+                u, v, w = drift_boost(th[0], th[1], th[2], sgn, torch.tensor(dd, dtype=torch.float64))
+                vals.append(sim.adjoint_forward(P0["q"], P0["x"], P0["y"], P0["z"], u.numpy(), v.numpy(),
+                                                w.numpy(), P0["flv"], nsteps, seg))
+            fd = (vals[0] - vals[1]) / (2 * eps)
+            out["fd_drift_fp64"][f"{eps:g}"] = {"fd": fd, "rel_err_fp64_adjoint": abs(g64[0] - fd) / abs(fd),
+                                                "rel_err_split_adjoint": abs(g32[0] - fd) / abs(fd)}
+            print(f"# fd eps={eps:g}: {out['fd_drift_fp64'][f'{eps:g}']}", file=sys.stderr, flush=True)
     num = np.sqrt(sum(np.sum((g32[1][k] - g64[1][k]) ** 2) for k in range(3)))
     den = np.sqrt(sum(np.sum(g64[1][k] ** 2) for k in range(3)))
-    out["fd_drift_fp64"] = fd
-    out["rel_err_fp64_adjoint_vs_fd"] = abs(g64[0] - fd) / abs(fd)
-    out["rel_err_split_adjoint_vs_fd"] = abs(g32[0] - fd) / abs(fd)
     out["rel_err_split_vs_fp64_drift"] = abs(g32[0] - g64[0]) / abs(g64[0])
     out["rel_l2_split_vs_fp64_momenta"] = float(num / den)
     return out

@@ -36,6 +36,10 @@ extern "C" {
 #define PICTOR_FORM_FP32_SPLIT 1 /* int32 cell + FP32 offset, FP32 push/field,
                                     fixed-point (int64) current              */
 
+#define PICTOR_FDTD_YEE 0        /* src/maxwell/em_update.F90                */
+#define PICTOR_FDTD_LEHE 1       /* src/maxwell/em_update_lehe.F90:8-155:
+                                    extended B-update stencils, Yee E update */
+
 /* Run configuration: the compile-time parameters of parameters.F90:5-87 and
  * src/vars.F90:10-18 as a runtime struct.  grid_dx = grid_dy = grid_dz = 1,
  * EMspeedBoost = 1, curr_filter = 0, periodic faces on every side. */
@@ -45,7 +49,7 @@ typedef struct pictor_config {
     int32_t epc;           /* particles per cell per species (parameters.F90:17) */
     int32_t form;          /* PICTOR_FORM_*                                  */
     int32_t tile;          /* tile edge in cells (0 = choose); must divide nx, ny (nz) */
-    int32_t reserved;
+    int32_t fdtd;          /* PICTOR_FDTD_*: the B half step of em_update.F90 or em_update_lehe.F90 */
     double c;              /* parameters.F90:20 */
     double c_ompe;         /* parameters.F90:19 */
     double g0;             /* parameters.F90:16 */

@@ -181,8 +181,97 @@ void orc_update_e(const orc_params *p, orc_fields *f)
 }
 
 /* em_update.F90:41-77 */
+/* em_update_lehe.F90:8-25 with grid_dx = grid_dy = grid_dz = 1 */
+void orc_set_lehe(orc_params *p)
+{
+    const double pi = 3.14159265358979323846264;   /* vars.F90:224 */
+    double *L = p->lehe;
+    const double beta_xy = 0.125 * 1.0, beta_xz = 0.125 * 1.0, beta_yx = 0.125, beta_yz = 0.0;
+    const double beta_zx = 0.125, beta_zy = 0.0;
+    const double sn = sin(0.5 * pi * p->c);
+    const double delta_x = 0.25 * (1.0 - (p->cinv * p->cinv) * (sn * sn));
+    const double delta_y = 0.0, delta_z = 0.0;
+    if (p->ndim == 3) {
+        L[0] = 1.0 - 2.0 * beta_xy - 2.0 * beta_xz - 3.0 * delta_x;
+        L[1] = 1.0 - 2.0 * beta_yx - 2.0 * beta_yz - 3.0 * delta_y;
+        L[2] = 1.0 - 2.0 * beta_zx - 2.0 * beta_zy - 3.0 * delta_z;
+    } else {
+        L[0] = 1.0 - 2.0 * beta_xy - 3.0 * delta_x;
+        L[1] = 1.0 - 2.0 * beta_yx - 3.0 * delta_y;
+        L[2] = 1.0;
+    }
+    L[3] = beta_xy; L[4] = beta_xz; L[5] = beta_yx; L[6] = beta_yz; L[7] = beta_zx; L[8] = beta_zy;
+    L[9] = delta_x; L[10] = delta_y; L[11] = delta_z;
+    p->fdtd = 1;
+}
+
+#define F3(A, i, j, k) A[IX(p, i, j, k)]
+/* gradX/gradY/gradZ, em_update_lehe.F90:120-155 (term order kept) */
+static double lehe_gradx(const orc_params *p, const double *F, int i, int j, int k)
+{
+    const double *L = p->lehe, d = p->inv_dx;
+    double g = L[0] * d * (F3(F, i + 1, j, k) - F3(F, i, j, k))
+             + L[3] * d * (F3(F, i + 1, j + 1, k) - F3(F, i, j + 1, k))
+             + L[3] * d * (F3(F, i + 1, j - 1, k) - F3(F, i, j - 1, k));
+    if (p->ndim == 3)
+        g = g + L[4] * d * (F3(F, i + 1, j, k + 1) - F3(F, i, j, k + 1))
+              + L[4] * d * (F3(F, i + 1, j, k - 1) - F3(F, i, j, k - 1));
+    return g + L[9] * d * (F3(F, i + 2, j, k) - F3(F, i - 1, j, k));
+}
+static double lehe_grady(const orc_params *p, const double *F, int i, int j, int k)
+{
+    const double *L = p->lehe, d = p->inv_dy;
+    double g = L[1] * d * (F3(F, i, j + 1, k) - F3(F, i, j, k))
+             + L[5] * d * (F3(F, i + 1, j + 1, k) - F3(F, i + 1, j, k))
+             + L[5] * d * (F3(F, i - 1, j + 1, k) - F3(F, i - 1, j, k));
+    if (p->ndim == 3)
+        g = g + L[6] * d * (F3(F, i, j + 1, k + 1) - F3(F, i, j, k + 1))
+              + L[6] * d * (F3(F, i, j + 1, k - 1) - F3(F, i, j, k - 1));
+    return g + L[10] * d * (F3(F, i, j + 2, k) - F3(F, i, j - 1, k));
+}
+static double lehe_gradz(const orc_params *p, const double *F, int i, int j, int k)
+{
+    const double *L = p->lehe, d = p->inv_dz;
+    return L[2] * d * (F3(F, i, j, k + 1) - F3(F, i, j, k))
+         + L[7] * d * (F3(F, i + 1, j, k + 1) - F3(F, i + 1, j, k))
+         + L[7] * d * (F3(F, i - 1, j, k + 1) - F3(F, i - 1, j, k))
+         + L[8] * d * (F3(F, i, j + 1, k + 1) - F3(F, i, j + 1, k))
+         + L[8] * d * (F3(F, i, j - 1, k + 1) - F3(F, i, j - 1, k))
+         + L[11] * d * (F3(F, i, j, k + 2) - F3(F, i, j, k - 1));
+}
+#undef F3
+
+/* UpdateBfieldHalf, em_update_lehe.F90:88-118 (cells 3..m-3) */
+static void update_b_half_lehe(const orc_params *p, orc_fields *f)
+{
+    const double hc = p->fld_halfc;
+    const double *Ex = f->Ex, *Ey = f->Ey, *Ez = f->Ez;
+    double *Bx = f->Bx, *By = f->By, *Bz = f->Bz;
+    int k0 = 1, k1 = 1;
+    if (p->ndim == 3) { k0 = 3; k1 = p->mz - 3; }
+    for (int k = k0; k <= k1; k++)
+        #pragma omp parallel for schedule(static)
+        for (int j = 3; j <= p->my - 3; j++)
+            for (int i = 3; i <= p->mx - 3; i++) {
+                int64_t o = IX(p, i, j, k);
+                if (p->ndim == 3) {
+                    Bx[o] = Bx[o] - hc * lehe_grady(p, Ez, i, j, k) + hc * lehe_gradz(p, Ey, i, j, k);
+                    By[o] = By[o] - hc * lehe_gradz(p, Ex, i, j, k) + hc * lehe_gradx(p, Ez, i, j, k);
+                    Bz[o] = Bz[o] - hc * lehe_gradx(p, Ey, i, j, k) + hc * lehe_grady(p, Ex, i, j, k);
+                } else {
+                    Bx[o] = Bx[o] - hc * lehe_grady(p, Ez, i, j, k);
+                    By[o] = By[o] + hc * lehe_gradx(p, Ez, i, j, k);
+                    Bz[o] = Bz[o] - hc * lehe_gradx(p, Ey, i, j, k) + hc * lehe_grady(p, Ex, i, j, k);
+                }
+            }
+}
+
 void orc_update_b_half(const orc_params *p, orc_fields *f)
 {
+    if (p->fdtd == 1) {
+        update_b_half_lehe(p, f);
+        return;
+    }
     const double hc = p->fld_halfc;
     const int mx = p->mx, my = p->my, mz = p->mz;
     const double *Ex = f->Ex, *Ey = f->Ey, *Ez = f->Ez;

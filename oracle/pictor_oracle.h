@@ -34,6 +34,11 @@ typedef struct orc_params {
     int32_t nflvr;
     int32_t reorder_period;  /* parameters.F90:71 */
     double flvrqm[8];        /* initialise.F90:84-85 */
+    int32_t fdtd;            /* 0 = Yee (em_update.F90), 1 = Lehe (em_update_lehe.F90) */
+    int32_t pad_;
+    /* em_update_lehe.F90:8-25: alpha_x, alpha_y, alpha_z, beta_xy, beta_xz, beta_yx,
+     * beta_yz, beta_zx, beta_zy, delta_x, delta_y, delta_z */
+    double lehe[12];
 } orc_params;
 
 typedef struct orc_fields {
@@ -60,7 +65,9 @@ orc_ws *orc_ws_create(const orc_params *p, int nthreads);
 void orc_ws_destroy(orc_ws *w);
 int  orc_ws_threads(const orc_ws *w);
 
-/* Maxwell solver, em_update.F90 */
+/* Lehe coefficients (em_update_lehe.F90:8-25) and fdtd = 1 */
+void orc_set_lehe(orc_params *p);
+/* Maxwell solver, em_update.F90 (fdtd 0) / em_update_lehe.F90 B half step (fdtd 1) */
 void orc_update_b_half(const orc_params *p, orc_fields *f);
 void orc_update_e(const orc_params *p, orc_fields *f);
 void orc_add_current(const orc_params *p, orc_fields *f);

@@ -30,6 +30,7 @@ class OrcParams(ctypes.Structure):
             "xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "xlen", "ylen", "zlen",
             "Bx_ext0", "By_ext0", "Bz_ext0")]
         + [("nflvr", ctypes.c_int32), ("reorder_period", ctypes.c_int32), ("flvrqm", ctypes.c_double * 8)]
+        + [("fdtd", ctypes.c_int32), ("pad_", ctypes.c_int32), ("lehe", ctypes.c_double * 12)]
     )
 
 
@@ -58,6 +59,7 @@ def lib():
         W = ctypes.c_void_p
         L.orc_params_init.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_set_lehe.argtypes = [P]
         L.orc_ws_create.argtypes = [P, ctypes.c_int]
         L.orc_ws_create.restype = W
         L.orc_ws_destroy.argtypes = [W]
@@ -94,12 +96,16 @@ class Oracle:
     FIELD_NAMES = ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz")
 
     def __init__(self, ndim=2, nx=16, ny=16, nz=1, c=0.45, c_ompe=10.0, g0=1.0, mi_me=1.0, epc=16,
-                 nthreads=1, cap=1024, reorder_period=20):
+                 nthreads=1, cap=1024, reorder_period=20, fdtd="yee"):
         L = lib()
         self.p = OrcParams()
         rc = L.orc_params_init(ctypes.byref(self.p), ndim, nx, ny, nz, c, c_ompe, g0, mi_me, epc)
         if rc:
             raise ValueError("invalid oracle configuration")
+        if fdtd == "lehe":
+            L.orc_set_lehe(ctypes.byref(self.p))
+        elif fdtd != "yee":
+            raise ValueError("fdtd must be 'yee' or 'lehe'")
         self.p.reorder_period = reorder_period
         self.shape = (self.p.mx, self.p.my, self.p.mz)
         self.F = {n: np.zeros(self.shape, order="F") for n in self.FIELD_NAMES}

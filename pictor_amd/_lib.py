@@ -18,6 +18,9 @@ PICTOR_ERR_CAPACITY = 12
 FORM_FP64 = 0
 FORM_FP32_SPLIT = 1
 
+FDTD_YEE = 0
+FDTD_LEHE = 1
+
 # every entry point of include/pictor_hip.h
 EXPORTS = (
     "pictor_init", "pictor_finalize", "pictor_last_error", "pictor_get_scalars",
@@ -35,7 +38,7 @@ EXPORTS = (
 class PictorConfig(ctypes.Structure):
     _fields_ = [
         ("ndim", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
-        ("epc", ctypes.c_int32), ("form", ctypes.c_int32), ("tile", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("epc", ctypes.c_int32), ("form", ctypes.c_int32), ("tile", ctypes.c_int32), ("fdtd", ctypes.c_int32),
         ("c", ctypes.c_double), ("c_ompe", ctypes.c_double), ("g0", ctypes.c_double), ("mi_me", ctypes.c_double),
         ("capacity_factor", ctypes.c_double),
     ]

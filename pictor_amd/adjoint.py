@@ -63,9 +63,9 @@ def weibel_gradients(nx, ny, epc, drift, uth, seed, nsteps, segment, form="fp32_
     P = setup_weibel(nx, ny, epc=epc, drift=0.0, uth=uth, seed=seed)
     sgn = torch.from_numpy(drift_signs(len(P["x"]), epc)).to(device)
     d = torch.tensor(float(drift), dtype=torch.float64, device=device, requires_grad=True)
-    th = [torch.from_numpy(P[k]).to(device) for k in ("u", "v", "w")]
+    th = [torch.from_numpy(P[k]).to(device).requires_grad_(True) for k in ("u", "v", "w")]
     u, v, w = drift_boost(th[0], th[1], th[2], sgn, d)
-    u.retain_grad(); v.retain_grad(); w.retain_grad()
+    u.retain_grad()
     with Simulation(nx, ny, epc=epc, form=form) as sim:
         L = ParticleStepObjective.apply(u, v, w, sim, P, nsteps, segment)
         L.backward()

@@ -388,6 +388,7 @@ int pictor_adjoint_forward(pictor_ctx *c, int64_t n, const double *q, const doub
     int rc = check_ctx2(c);
     if (rc) return rc;
     if (c->g.ndim != 2) return fail(PICTOR_ERR_ARG, "the adjoint is available for ndim = 2");
+    if (c->ph.fdtd != PICTOR_FDTD_YEE) return fail(PICTOR_ERR_ARG, "the adjoint is available for the Yee solver");
     if (c->pending_swap) return fail(PICTOR_ERR_ARG, "adjoint inside a split step");
     if (n < 1 || nsteps < 1 || segment < 1) return fail(PICTOR_ERR_ARG, "n, nsteps and segment must be >= 1");
     if (!q || !x || !y || !z || !u || !v || !w || !flv) return fail(PICTOR_ERR_ARG, "null particle array");

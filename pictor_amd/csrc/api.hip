@@ -284,12 +284,14 @@ int do_b_half(pictor_ctx *c)
     if (c->form == PICTOR_FORM_FP64) {
         double *E[3], *B[3];
         field_ptrs(c, E, B);
-        launch_update_b_half<double>(c->g, c->ph, E, B, c->stream);
+        if (c->ph.fdtd == PICTOR_FDTD_LEHE) launch_update_b_half_lehe<double>(c->g, c->ph, c->ph.lehe, E, B, c->stream);
+        else launch_update_b_half<double>(c->g, c->ph, E, B, c->stream);
         launch_ghost_copy<double>(c->g, B, c->stream);
     } else {
         float *E[3], *B[3];
         field_ptrs(c, E, B);
-        launch_update_b_half<float>(c->g, c->ph, E, B, c->stream);
+        if (c->ph.fdtd == PICTOR_FDTD_LEHE) launch_update_b_half_lehe<float>(c->g, c->ph, c->ph.lehe, E, B, c->stream);
+        else launch_update_b_half<float>(c->g, c->ph, E, B, c->stream);
         launch_ghost_copy<float>(c->g, B, c->stream);
     }
     HIPTRY(hipGetLastError());
@@ -409,6 +411,7 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     if (cfg->ndim == 3 && (cfg->nx > 2048 || cfg->ny > 2048 || cfg->nz > 512))
         return fail(PICTOR_ERR_ARG, "3-D nx, ny must be <= 2048 and nz <= 512");
     if (cfg->form != PICTOR_FORM_FP64 && cfg->form != PICTOR_FORM_FP32_SPLIT) return fail(PICTOR_ERR_ARG, "unknown form");
+    if (cfg->fdtd != PICTOR_FDTD_YEE && cfg->fdtd != PICTOR_FDTD_LEHE) return fail(PICTOR_ERR_ARG, "unknown fdtd");
     if (!(cfg->c > 0.0 && cfg->c < 1.0)) return fail(PICTOR_ERR_ARG, "Courant number c must lie in (0,1)");
     if (cfg->epc < 1 || cfg->c_ompe <= 0 || cfg->mi_me <= 0) return fail(PICTOR_ERR_ARG, "invalid epc, c_ompe or mi_me");
     int T = cfg->tile ? cfg->tile : choose_tile(cfg);
@@ -452,6 +455,27 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     ph.flvq[1] = -1.0;
     ph.mass[0] = ph.qi * mi;                 // massi
     ph.mass[1] = std::fabs(-ph.qi) * me;     // masse
+    // em_update_lehe.F90:8-25 (unit cells): beta_xy = beta_xz = beta_yx = beta_zx = 1/8,
+    // beta_yz = beta_zy = 0, delta_x = (1 - sin^2(pi c / 2) / c^2) / 4, delta_y = delta_z = 0
+    ph.fdtd = cfg->fdtd;
+    {
+        const double pi = 3.14159265358979323846264;   // vars.F90:224
+        const double bxy = 0.125 * 1.0, bxz = 0.125 * 1.0, byx = 0.125, byz = 0.0, bzx = 0.125, bzy = 0.0;
+        const double sn = std::sin(0.5 * pi * cfg->c);
+        const double dx = 0.25 * (1.0 - (ph.cinv * ph.cinv) * (sn * sn)), dy = 0.0, dz = 0.0;
+        double *L = ph.lehe;
+        if (cfg->ndim == 3) {
+            L[0] = 1.0 - 2.0 * bxy - 2.0 * bxz - 3.0 * dx;
+            L[1] = 1.0 - 2.0 * byx - 2.0 * byz - 3.0 * dy;
+            L[2] = 1.0 - 2.0 * bzx - 2.0 * bzy - 3.0 * dz;
+        } else {
+            L[0] = 1.0 - 2.0 * bxy - 3.0 * dx;
+            L[1] = 1.0 - 2.0 * byx - 3.0 * dy;
+            L[2] = 1.0;
+        }
+        L[3] = bxy; L[4] = bxz; L[5] = byx; L[6] = byz; L[7] = bzx; L[8] = bzy;
+        L[9] = dx; L[10] = dy; L[11] = dz;
+    }
 
     TileP &tp = c->tp;
     tp.T = T;

@@ -46,6 +46,62 @@ __global__ __launch_bounds__(BX *BY) void k_update_b_half(GridP g, R hc, R idx_,
     }
 }
 
+// Lehe B half step, em_update_lehe.F90:88-155: extended x, y (and z)
+// gradients with the reference's coefficients and term order; cells 3..m-3.
+template <typename R>
+struct LeheC {
+    R ax, ay, az, bxy, bxz, byx, byz, bzx, bzy, dx, dy, dz;
+};
+
+template <typename R>
+__device__ inline R lgx(const GridP &g, const LeheC<R> &L, R d, const R *F, long long o)
+{
+    const long long sy = g.mx, sz = (long long)g.mx * g.my;
+    R v = L.ax * d * (F[o + 1] - F[o]) + L.bxy * d * (F[o + 1 + sy] - F[o + sy]) +
+          L.bxy * d * (F[o + 1 - sy] - F[o - sy]);
+    if (g.ndim == 3) v = v + L.bxz * d * (F[o + 1 + sz] - F[o + sz]) + L.bxz * d * (F[o + 1 - sz] - F[o - sz]);
+    return v + L.dx * d * (F[o + 2] - F[o - 1]);
+}
+template <typename R>
+__device__ inline R lgy(const GridP &g, const LeheC<R> &L, R d, const R *F, long long o)
+{
+    const long long sy = g.mx, sz = (long long)g.mx * g.my;
+    R v = L.ay * d * (F[o + sy] - F[o]) + L.byx * d * (F[o + 1 + sy] - F[o + 1]) +
+          L.byx * d * (F[o - 1 + sy] - F[o - 1]);
+    if (g.ndim == 3) v = v + L.byz * d * (F[o + sy + sz] - F[o + sz]) + L.byz * d * (F[o + sy - sz] - F[o - sz]);
+    return v + L.dy * d * (F[o + 2 * sy] - F[o - sy]);
+}
+template <typename R>
+__device__ inline R lgz(const GridP &g, const LeheC<R> &L, R d, const R *F, long long o)
+{
+    const long long sy = g.mx, sz = (long long)g.mx * g.my;
+    return L.az * d * (F[o + sz] - F[o]) + L.bzx * d * (F[o + 1 + sz] - F[o + 1]) +
+           L.bzx * d * (F[o - 1 + sz] - F[o - 1]) + L.bzy * d * (F[o + sy + sz] - F[o + sy]) +
+           L.bzy * d * (F[o - sy + sz] - F[o - sy]) + L.dz * d * (F[o + 2 * sz] - F[o - sz]);
+}
+
+template <typename R>
+__global__ __launch_bounds__(BX *BY) void k_update_b_half_lehe(GridP g, R hc, R idx_, R idy_, R idz_, LeheC<R> L,
+                                                               const R *__restrict__ Ex, const R *__restrict__ Ey,
+                                                               const R *__restrict__ Ez, R *__restrict__ Bx,
+                                                               R *__restrict__ By, R *__restrict__ Bz)
+{
+    int i = 3 + blockIdx.x * BX + threadIdx.x;
+    int j = 3 + blockIdx.y * BY + threadIdx.y;
+    int k = (g.ndim == 3) ? 3 + (int)blockIdx.z : 1;
+    if (i > g.mx - 3 || j > g.my - 3) return;
+    long long o = fidx(g, i, j, k);
+    if (g.ndim == 2) {
+        Bx[o] = Bx[o] - hc * lgy(g, L, idy_, Ez, o);
+        By[o] = By[o] + hc * lgx(g, L, idx_, Ez, o);
+        Bz[o] = Bz[o] - hc * lgx(g, L, idx_, Ey, o) + hc * lgy(g, L, idy_, Ex, o);
+    } else {
+        Bx[o] = Bx[o] - hc * lgy(g, L, idy_, Ez, o) + hc * lgz(g, L, idz_, Ey, o);
+        By[o] = By[o] - hc * lgz(g, L, idz_, Ex, o) + hc * lgx(g, L, idx_, Ez, o);
+        Bz[o] = Bz[o] - hc * lgx(g, L, idx_, Ey, o) + hc * lgy(g, L, idy_, Ex, o);
+    }
+}
+
 template <typename R, typename JT>
 __device__ inline R jvalue(const JT *J, long long o, double scale);
 template <>
@@ -53,7 +109,7 @@ __device__ inline double jvalue<double, double>(const double *J, long long o, do
 {
     return J[o];
 }
-// split form: J holds exact sums of contributions quantized to 2^-40 (charge x cell)
+// split form: J holds exact sums of contributions quantized to 2^-36 (charge x cell)
 template <>
 __device__ inline float jvalue<float, double>(const double *J, long long o, double scale)
 {
@@ -254,6 +310,16 @@ void launch_update_b_half(const GridP &g, const PhysP &ph, R *const E[3], R *con
 }
 
 template <typename R>
+void launch_update_b_half_lehe(const GridP &g, const PhysP &ph, const double lehe[12], R *const E[3], R *const B[3],
+                               hipStream_t s)
+{
+    LeheC<R> L{(R)lehe[0], (R)lehe[1], (R)lehe[2], (R)lehe[3], (R)lehe[4],  (R)lehe[5],
+               (R)lehe[6], (R)lehe[7], (R)lehe[8], (R)lehe[9], (R)lehe[10], (R)lehe[11]};
+    hipLaunchKernelGGL(k_update_b_half_lehe<R>, phys_grid(g), dim3(BX, BY), 0, s, g, (R)ph.fld_halfc, (R)ph.inv_dx,
+                       (R)ph.inv_dy, (R)ph.inv_dz, L, E[0], E[1], E[2], B[0], B[1], B[2]);
+}
+
+template <typename R>
 void launch_update_e(const GridP &g, const PhysP &ph, R *const E[3], R *const B[3], hipStream_t s)
 {
     hipLaunchKernelGGL((k_update_e<R, double, true, false>), phys_grid(g), dim3(BX, BY), 0, s, g, (R)ph.fldc,
@@ -335,6 +401,10 @@ void launch_convert_jfix(const double *src, double *dst, long long n, double sca
 
 template void launch_update_b_half<double>(const GridP &, const PhysP &, double *const[3], double *const[3], hipStream_t);
 template void launch_update_b_half<float>(const GridP &, const PhysP &, float *const[3], float *const[3], hipStream_t);
+template void launch_update_b_half_lehe<double>(const GridP &, const PhysP &, const double[12], double *const[3],
+                                                double *const[3], hipStream_t);
+template void launch_update_b_half_lehe<float>(const GridP &, const PhysP &, const double[12], float *const[3],
+                                               float *const[3], hipStream_t);
 template void launch_update_e<double>(const GridP &, const PhysP &, double *const[3], double *const[3], hipStream_t);
 template void launch_update_e<float>(const GridP &, const PhysP &, float *const[3], float *const[3], hipStream_t);
 template void launch_update_e_add_current<double, double>(const GridP &, const PhysP &, double *const[3],

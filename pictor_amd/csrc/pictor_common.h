@@ -40,6 +40,8 @@ struct PhysP {
     double flvq[4];    // charge of one particle per flavour (split form)
     double mass[4];    // massi, masse (initialise.F90:49-50)
     int nflvr;
+    int fdtd;          // 0 Yee, 1 Lehe
+    double lehe[12];   // Lehe coefficients (fdtd 1)
 };
 
 struct TileP {
@@ -90,6 +92,11 @@ __device__ inline int tile_remap(const TileP &tp, int bid)
 // ---------------- launchers (kernels_fields.hip) ----------------
 template <typename R>
 void launch_update_b_half(const GridP &g, const PhysP &ph, R *const E[3], R *const B[3], hipStream_t s);
+// Lehe B half step; lehe = alpha_x, alpha_y, alpha_z, beta_xy, beta_xz, beta_yx, beta_yz, beta_zx,
+// beta_zy, delta_x, delta_y, delta_z (em_update_lehe.F90:8-25)
+template <typename R>
+void launch_update_b_half_lehe(const GridP &g, const PhysP &ph, const double lehe[12], R *const E[3], R *const B[3],
+                               hipStream_t s);
 template <typename R>
 void launch_update_e(const GridP &g, const PhysP &ph, R *const E[3], R *const B[3], hipStream_t s);
 // E update fused with AddCurrent; J is double (FP64 form) or fixed point (split form)

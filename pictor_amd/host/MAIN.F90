@@ -4,7 +4,7 @@
 ! replaces in the reference loop and forwards to the C ABI.  Phases the
 ! periodic one-GPU run does not need (BCs, output, load balancing, filters)
 ! are left out.  Usage:  pictor_main [name=value ...]  with names from
-! parameters.F90 (nx, ny, epc, tfinish, form, fused, seed, c, c_ompe,
+! parameters.F90 (nx, ny, epc, tfinish, form, fused, fdtd, seed, c, c_ompe,
 ! energy_save_period, prtl_reorder_period).
 program MAIN
    use iso_c_binding
@@ -70,6 +70,7 @@ contains
          case ('tfinish'); read (val, *, iostat=ios) tfinish
          case ('form'); read (val, *, iostat=ios) form
          case ('fused'); read (val, *, iostat=ios) fused
+         case ('fdtd'); read (val, *, iostat=ios) fdtd
          case ('seed'); read (val, *, iostat=ios) seed
          case ('c'); read (val, *, iostat=ios) c
          case ('c_ompe'); read (val, *, iostat=ios) c_ompe
@@ -94,7 +95,7 @@ contains
       allocate (Jx(mx, my, mz), Jy(mx, my, mz), Jz(mx, my, mz))
       Ex = 0; Ey = 0; Ez = 0; Bx = 0; By = 0; Bz = 0; Jx = 0; Jy = 0; Jz = 0
       call InitSetup
-      cfg = pictor_config(2, nx, ny, 1, epc, form, 0, 0, c, c_ompe, g0, mi_me, 0.0d0)
+      cfg = pictor_config(2, nx, ny, 1, epc, form, 0, fdtd, c, c_ompe, g0, mi_me, 0.0d0)
       call check(pictor_init(cfg, gpu), 'pictor_init')
       call check(pictor_upload_fields(gpu, Ex, Ey, Ez, Bx, By, Bz), 'pictor_upload_fields')
       call check(pictor_upload_particles(gpu, used_prtl_arr_size, qp, xp, yp, zp, up, vp, wp, flvp), &

@@ -23,6 +23,7 @@ module parameters
    ! run-time choices of the GPU path
    integer :: form = 1                               ! 0 = FP64 reference SoA, 1 = FP32 split
    integer :: fused = 0                              ! 1 = pictor_step per step, 0 = one call per phase
+   integer :: fdtd = 0                               ! 0 = Yee (em_update.F90), 1 = Lehe (em_update_lehe.F90)
    real(psn) :: drift = 0.5_psn                      ! beam drift (units of c)
    real(psn) :: uth = 0.01_psn                       ! thermal spread of gamma*beta
    integer :: seed = 12345

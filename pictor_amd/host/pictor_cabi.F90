@@ -14,7 +14,7 @@ module pictor_cabi
    integer(c_int32_t), parameter :: PICTOR_FORM_FP32_SPLIT = 1
 
    type, bind(C) :: pictor_config
-      integer(c_int32_t) :: ndim, nx, ny, nz, epc, form, tile, reserved
+      integer(c_int32_t) :: ndim, nx, ny, nz, epc, form, tile, fdtd
       real(c_double) :: c, c_ompe, g0, mi_me, capacity_factor
    end type pictor_config
 

@@ -1,13 +1,22 @@
 """Adjoint of the 2-D particle step (needs an MI355X).
 
+The zigzag current (movdep.F90:177-237) is a discontinuous function of the
+new position where a particle's cell-crossing state changes: the relay point
+jumps from the midpoint to the crossed face.  The objective is therefore
+piecewise smooth with many small jumps, and a central difference over an
+interval that contains crossing-state changes measures those jumps, not the
+derivative (with eps = 1e-4 on the drift speed, hundreds of particle-steps
+change state).  The adjoint is the exact derivative of the smooth pieces, so
+the finite-difference checks use eps = 1e-7 in FP64, where no particle-step
+changes state; the eps = 1e-4 difference is reported by bench.py --adjoint.
+
 Bars:
   * the engine's FP64 forward objective equals the FP64 oracle's after the
     same steps (relative 1e-10), the FP32 engine's the production split
     form's (relative 1e-5);
-  * FP64 adjoint gradients within 1e-3 (relative) of central finite
-    differences with eps = 1e-4: for the drift speed (through the torch
-    AddDriftVel boost), for a random direction in the space of all initial
-    momenta, and for single momentum components;
+  * FP64 adjoint within 1e-3 (relative) of central differences with
+    eps = 1e-7: drift speed (through the torch AddDriftVel boost), a random
+    direction in the space of all initial momenta, single components;
   * the checkpoint segment length does not change the gradient (1e-10);
   * FP32 split adjoint within 2e-2 (relative L2) of the FP64 gradient.
 """
@@ -38,10 +47,9 @@ def _objective(sim, P0, u, v, w, nsteps, segment):
 def _grad(form, drift, nsteps, segment, seed=21):
     P0, sgn = _load(drift, seed)
     d = torch.tensor(drift, dtype=torch.float64, requires_grad=True)
-    th = [torch.from_numpy(P0[k]) for k in ("u", "v", "w")]
+    th = [torch.from_numpy(P0[k]).requires_grad_(True) for k in ("u", "v", "w")]
     u, v, w = drift_boost(th[0], th[1], th[2], sgn, d)
-    for t in (u, v, w):
-        t.retain_grad()
+    u.retain_grad()
     with Simulation(NX, NY, epc=EPC, form=form, tile=8) as sim:
         L = ParticleStepObjective.apply(u, v, w, sim, P0, nsteps, segment)
         L.backward()
@@ -79,7 +87,7 @@ def test_engine_forward_matches_production_split():
 
 
 def test_fd_drift_fp64():
-    drift, nsteps, seg, eps = 0.5, 40, 10, 1e-4
+    drift, nsteps, seg, eps = 0.5, 40, 10, 1e-7
     L, g, _, _, P0 = _grad("fp64", drift, nsteps, seg)
     sgn = torch.from_numpy(drift_signs(len(P0["x"]), EPC))
     th = [torch.from_numpy(P0[k]) for k in ("u", "v", "w")]
@@ -94,7 +102,7 @@ def test_fd_drift_fp64():
 
 
 def test_fd_momenta_fp64():
-    drift, nsteps, seg, eps = 0.5, 40, 10, 1e-4
+    drift, nsteps, seg, eps = 0.5, 40, 10, 1e-7
     L, _, G, (u0, v0, w0), P0 = _grad("fp64", drift, nsteps, seg)
     rng = np.random.default_rng(0)
     r = [rng.standard_normal(len(u0)) for _ in range(3)]

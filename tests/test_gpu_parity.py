@@ -186,3 +186,55 @@ def test_upload_rejects_out_of_domain():
         with pytest.raises(PictorError) as e:
             sim.upload_particles(**P)
         assert e.value.code == 1
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_lehe_fields_bitwise(ndim):
+    n = (24, 16, 8) if ndim == 3 else (32, 24, 1)
+    say(f"lehe ndim={ndim} n={n}")
+    o = Oracle(ndim, *n, epc=2, fdtd="lehe")
+    F = smooth_fields(o, amp_e=1.0, amp_b=1.0, seed=13, modes=6)
+    rng = np.random.default_rng(8)
+    sl = (slice(3, -2), slice(3, -2), slice(3, -2) if ndim == 3 else slice(None))
+    for k in FIELDS:
+        F[k][sl] += 1e-3 * rng.standard_normal(F[k][sl].shape)
+    o.set_fields(**F)
+    o.field_halo("E"); o.field_halo("B")
+    F = {k: o.F[k].copy(order="F") for k in FIELDS}
+    with Simulation(*n, ndim=ndim, epc=2, form="fp64", tile=8 if ndim == 2 else 4, fdtd="lehe") as sim:
+        sim.upload_fields(*[F[k] for k in FIELDS])
+        for _ in range(6):
+            sim.UpdateBfieldHalf()
+            o.update_b_half(); o.field_halo("B")
+            sim.UpdateEfield()
+            o.update_e(); o.field_halo("E")
+        G = sim.download_fields()
+    for k in FIELDS:
+        assert np.array_equal(G[k], o.F[k]), k
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32_split"])
+def test_lehe_steps_vs_oracle(form):
+    nx, ny = 32, 32
+    say(f"lehe steps form={form}")
+    P = setup_weibel(nx, ny, epc=16, drift=0.5, uth=0.01, seed=4)
+    o = Oracle(2, nx, ny, epc=16, cap=2 * len(P["x"]) + 4096, fdtd="lehe")
+    o.load_particles(**P)
+    nsteps = 40
+    with Simulation(nx, ny, epc=16, form=form, tile=8, fdtd="lehe") as sim:
+        sim.upload_particles(**P)
+        for t in range(1, nsteps + 1):
+            o.step(t)
+        sim.step(nsteps)
+        d = sim.diagnostics()
+        eo = o.energy()
+        if form == "fp64":
+            G = sim.download_fields()
+            for k in FIELDS:
+                assert rel_err(G[k], o.F[k]) < 1e-9, k
+            assert abs(d["B_energy"] - eo[0]) <= 1e-9 * eo[0]
+            assert d["gauss_residual"] <= 1e-12 * d["rho_max"], d
+        else:
+            assert abs(d["B_energy"] - eo[0]) <= 1e-3 * eo[0]
+            assert abs(d["E_energy"] - eo[1]) <= 1e-3 * eo[1]
+            assert d["gauss_residual"] <= 1e-6 * d["rho_max"], d
