@@ -250,12 +250,14 @@ def adjoint_report(args):
     out = {"workload": f"adjoint: 2-D Weibel {nx}x{ny}, {epc} ppc/species, {len(P0['x'])} particles, "
                        f"{nsteps} steps, checkpoints every {seg}", "forms": {}}
     grads = {}
-    for form in ("fp32_split", "fp64"):
+    for form in args.adjoint_forms.split(","):
         d = torch.tensor(drift, dtype=torch.float64, requires_grad=True)
         tl = [t.clone().requires_grad_(True) for t in th]
         u, v, w = drift_boost(tl[0], tl[1], tl[2], sgn, d)
-        u.retain_grad()
+        for t in (u, v, w):
+            t.retain_grad()
         with Simulation(nx, ny, epc=epc, form=form) as sim:
+            sim.adjoint_engine(args.adjoint_engine)
             L = ParticleStepObjective.apply(u, v, w, sim, P0, nsteps, seg)
             L.backward()
             st = sim.adjoint_stats()
@@ -278,6 +280,8 @@ def adjoint_report(args):
             "gradient_over_production_forward": (st["forward_ms"] + st["recompute_ms"] + st["reverse_ms"]) / t_prod,
         }
         print(f"# adjoint {form}: {out['forms'][form]}", file=sys.stderr, flush=True)
+    if "fp64" not in grads or "fp32_split" not in grads or not args.adjoint_fd:
+        return out
     # central finite differences of the FP64 objective in the drift speed; the zigzag
     # current jumps where a particle's crossing state changes, so a wide interval
     # measures those jumps as well (see tests/test_gpu_adjoint.py)
@@ -322,6 +326,9 @@ def main():
     ap.add_argument("--segment", type=int, default=20)
     ap.add_argument("--adjoint-n", type=int, default=512)
     ap.add_argument("--adjoint-epc", type=int, default=16)
+    ap.add_argument("--adjoint-engine", default="sorted", choices=("sorted", "unsorted"))
+    ap.add_argument("--adjoint-forms", default="fp32_split,fp64")
+    ap.add_argument("--adjoint-fd", type=int, default=1, help="finite-difference check of the drift gradient")
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -148,13 +148,21 @@ int pictor_setup_weibel(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, int32_
  * The context's own particles and fields are not touched.
  * pictor_adjoint_stats: out[0] forward ms, out[1] recompute ms, out[2]
  * reverse-step ms, out[3] device bytes of the engine, out[4] nsteps,
- * out[5] segment, out[6] checkpoints, out[7] L.                           */
+ * out[5] segment, out[6] checkpoints, out[7] L, out[8] engine (0 tile-sorted,
+ * 1 unsorted), out[9] forward passes repeated with larger tile bins. */
 int pictor_adjoint_forward(pictor_ctx *ctx, int64_t n, const double *q, const double *x,
                            const double *y, const double *z, const double *u, const double *v,
                            const double *w, const int32_t *flv, int32_t nsteps, int32_t segment,
                            double *objective);
 int pictor_adjoint_backward(pictor_ctx *ctx, double scale, double *du, double *dv, double *dw);
-int pictor_adjoint_stats(pictor_ctx *ctx, double out[8]);
+int pictor_adjoint_stats(pictor_ctx *ctx, double out[10]);
+/* Engine of the FP32 split form: 0 = tile-sorted (default; the particles in
+ * the tile bins of pictor_move_deposit, every forward step the production
+ * push, the reverse step over the same bins with LDS gather and LDS
+ * accumulation of the field adjoint; tile edge 8, 16 or 32), 1 = unsorted
+ * (one thread per particle, global atomics).  The FP64 form always uses the
+ * unsorted engine.  Takes effect at the next pictor_adjoint_forward. */
+int pictor_adjoint_engine(pictor_ctx *ctx, int32_t engine);
 int pictor_adjoint_release(pictor_ctx *ctx);
 
 #ifdef __cplusplus

@@ -32,10 +32,13 @@ def drift_signs(n, epc):
 
 
 def drift_boost(u_th, v_th, w_th, sign, drift):
+    """Boosted momenta (u, v, w); v and w are copies of v_th, w_th, so that a
+    gradient retained on them is the derivative at fixed u (u depends on
+    v_th and w_th through gamma_th)."""
     gth = torch.sqrt(1.0 + u_th * u_th + v_th * v_th + w_th * w_th)
     gd = 1.0 / torch.sqrt((1.0 - drift) * (1.0 + drift))
     u = gd * (u_th + sign * drift * gth)
-    return u, v_th, w_th
+    return u, v_th.clone(), w_th.clone()
 
 
 class ParticleStepObjective(torch.autograd.Function):
@@ -65,7 +68,8 @@ def weibel_gradients(nx, ny, epc, drift, uth, seed, nsteps, segment, form="fp32_
     d = torch.tensor(float(drift), dtype=torch.float64, device=device, requires_grad=True)
     th = [torch.from_numpy(P[k]).to(device).requires_grad_(True) for k in ("u", "v", "w")]
     u, v, w = drift_boost(th[0], th[1], th[2], sgn, d)
-    u.retain_grad()
+    for t in (u, v, w):
+        t.retain_grad()
     with Simulation(nx, ny, epc=epc, form=form) as sim:
         L = ParticleStepObjective.apply(u, v, w, sim, P, nsteps, segment)
         L.backward()

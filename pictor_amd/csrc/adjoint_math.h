@@ -88,9 +88,10 @@ __device__ inline void push_fwd(const AdjP &ap, int s, R x0, R y0, R z0, R un, R
     q.r2 = rsq_<R>(sqc + U2[0] * U2[0] + U2[1] * U2[1] + U2[2] * U2[2]);
     q.G = sqc * q.r2;
     const R xs[3] = {x0, y0, z0};
+    const R idc = (R)ap.inv_d;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        q.x1[k] = xs[k] + q.un1[k] * q.G;   // inv_dx = inv_dy = inv_dz = 1
+        q.x1[k] = xs[k] + q.un1[k] * q.G * idc;   // idc = 1: the production push's rounding of the advance
         q.d[k] = (int)floor_<R>(q.x1[k]);
         q.xr[k] = vmin<R>((R)(min(0, q.d[k]) + 1), vmax<R>((R)max(0, q.d[k]), (R)0.5 * (xs[k] + q.x1[k])));
         q.F1[k] = qs * (q.xr[k] - xs[k]);

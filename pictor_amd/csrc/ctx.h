@@ -96,4 +96,5 @@ struct pictor_ctx {
     long long np = 0;
 
     pictor::AdjEngineBase *adj = nullptr;   // created by pictor_adjoint_forward
+    int adj_engine = 0;                      // 0 tile-sorted (FP32 split form), 1 unsorted
 };

@@ -118,6 +118,7 @@ __global__ void k_raw_scatter_split(GridP g, TileP tp, PartF64 raw, long long n,
     out.cell[o] = cell_pack(g.ndim, cx, cy, cz);
     out.ox[o] = ox; out.oy[o] = oy; out.oz[o] = oz;
     out.u[o] = (float)raw.u[i]; out.v[o] = (float)raw.v[i]; out.w[o] = (float)raw.w[i];
+    if (out.id) out.id[o] = (int)i;
     (void)s;
 }
 

@@ -245,6 +245,7 @@ __global__ __launch_bounds__(BS) void k_push2d_f64(GridP g, PhysP ph, TileP tp, 
 struct SplitRec {
     int cell, s;
     float ox, oy, oz, u, v, w;
+    int slot, id;   // input slot and particle index (TRACK only)
 };
 
 // virtual bin v of a tile: the centre bins of all species first (most
@@ -258,13 +259,14 @@ __device__ inline void vbin(int v, int nsp, int &s, int &d)
     d = q < 4 ? q : q + 1;
 }
 
-template <int T, int BS>
+template <int T, int BS, bool TRACK = false>
 __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp, const float *__restrict__ Ex,
                                                      const float *__restrict__ Ey, const float *__restrict__ Ez,
                                                      const float *__restrict__ Bx, const float *__restrict__ By,
                                                      const float *__restrict__ Bz, double *__restrict__ Jx,
                                                      double *__restrict__ Jy, double *__restrict__ Jz, PartSplit A,
-                                                     PartSplit Bo, BinsP bins, PartSplit spill, SpillP sp)
+                                                     PartSplit Bo, BinsP bins, PartSplit spill, SpillP sp,
+                                                     int *__restrict__ dst)
 {
     // FP32 form: fused multiply-add allowed (the FP64 form keeps it off for
     // bitwise parity with the reference order)
@@ -343,11 +345,13 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
     auto load = [&](int k, SplitRec &r) {
         int v = 0;
         while (k >= sStart[v + 1]) v++;
-        const unsigned off = (sSrc[v] + (unsigned)(k - sStart[v])) * 4u;
+        const unsigned slot = sSrc[v] + (unsigned)(k - sStart[v]);
+        const unsigned off = slot * 4u;
         r.cell = ldo(A.cell, off);
         r.ox = ldo(A.ox, off); r.oy = ldo(A.oy, off); r.oz = ldo(A.oz, off);
         r.u = ldo(A.u, off); r.v = ldo(A.v, off); r.w = ldo(A.w, off);
         r.s = sVs[v];
+        if (TRACK) { r.slot = (int)slot; r.id = ldo(A.id, off); }
     };
 
     SplitRec nxt{};
@@ -473,6 +477,10 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                 const unsigned off = (sDstBase[d] + (unsigned)slot) * 4u;
                 sto(Bo.cell, off, ncell); sto(Bo.ox, off, ox); sto(Bo.oy, off, oy); sto(Bo.oz, off, oz);
                 sto(Bo.u, off, un); sto(Bo.v, off, vn); sto(Bo.w, off, wn);
+                if (TRACK) {
+                    sto(Bo.id, off, cur.id);
+                    if (dst) dst[cur.slot] = (int)(sDstBase[d] + (unsigned)slot);
+                }
             } else {
                 const int si = atomicAdd(sp.count, 1);
                 if (si < sp.cap) {
@@ -481,6 +489,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                 } else {
                     atomicOr(sp.err, (int)kErrCapacity);
                 }
+                if (TRACK && dst) dst[cur.slot] = -1;
             }
         }
     }
@@ -525,7 +534,24 @@ void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp
     dim3 grid(tp.ntiles);
 #define PUSH_SPLIT(TT, BSZ)                                                                                         \
     hipLaunchKernelGGL((k_push2d_split<TT, BSZ>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], B[0], B[1], \
-                       B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp)
+                       B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp, (int *)nullptr)
+    switch (T) {
+    case 8: PUSH_SPLIT(8, 256); break;
+    case 16: PUSH_SPLIT(16, 256); break;
+    case 32: PUSH_SPLIT(32, 1024); break;
+    default: break;
+    }
+#undef PUSH_SPLIT
+}
+
+void launch_push2d_split_track(int T, const GridP &g, const PhysP &ph, const TileP &tp, float *const E[3],
+                               float *const B[3], double *const J[3], PartSplit A, PartSplit Bo, BinsP bins,
+                               PartSplit spill, SpillP sp, int *dst, hipStream_t s)
+{
+    dim3 grid(tp.ntiles);
+#define PUSH_SPLIT(TT, BSZ)                                                                                         \
+    hipLaunchKernelGGL((k_push2d_split<TT, BSZ, true>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], B[0], B[1], \
+                       B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp, dst)
     switch (T) {
     case 8: PUSH_SPLIT(8, 256); break;
     case 16: PUSH_SPLIT(16, 256); break;

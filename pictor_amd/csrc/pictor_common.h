@@ -75,6 +75,7 @@ struct PartSplit {
     int *cell;
     float *ox, *oy, *oz, *u, *v, *w;
     int *sp;   // species id, used only by the spill / raw buffers
+    int *id;   // caller's index of the particle, carried only by the adjoint engine's buffers
 };
 
 __host__ __device__ inline long long fidx(const GridP &g, int i, int j, int k)
@@ -140,6 +141,11 @@ void launch_push2d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, 
 void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp, float *const E[3],
                          float *const B[3], double *const J[3], PartSplit A, PartSplit Bo, BinsP bins,
                          PartSplit spill, SpillP sp, hipStream_t s);
+// the same step carrying the particle indices (A.id -> Bo.id); dst (optional):
+// the output slot of every input slot (-1 for a particle sent to the spill buffer)
+void launch_push2d_split_track(int T, const GridP &g, const PhysP &ph, const TileP &tp, float *const E[3],
+                               float *const B[3], double *const J[3], PartSplit A, PartSplit Bo, BinsP bins,
+                               PartSplit spill, SpillP sp, int *dst, hipStream_t s);
 
 // kernels_push3d.hip: T = 4 or 8
 void launch_push3d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, double *const E[3],

@@ -194,14 +194,18 @@ class Simulation:
         return du, dv, dw
 
     def adjoint_stats(self):
-        out = np.zeros(8)
+        out = np.zeros(10)
         check(lib().pictor_adjoint_stats(self._ctx, _p(out)))
         keys = ("forward_ms", "recompute_ms", "reverse_ms", "device_bytes", "nsteps", "segment", "checkpoints",
-                "objective")
+                "objective", "engine", "relayouts")
         return dict(zip(keys, out.tolist()))
 
     def adjoint_release(self):
         check(lib().pictor_adjoint_release(self._ctx))
+
+    def adjoint_engine(self, engine):
+        """'sorted' (tile-sorted FP32 kernels, the default) or 'unsorted'."""
+        check(lib().pictor_adjoint_engine(self._ctx, {"sorted": 0, "unsorted": 1}[engine]))
 
 
 def setup_weibel(nx, ny, nz=1, ndim=2, epc=16, drift=0.5, uth=0.01, seed=12345):
