@@ -40,6 +40,14 @@ extern "C" {
 #define PICTOR_FDTD_LEHE 1       /* src/maxwell/em_update_lehe.F90:8-155:
                                     extended B-update stencils, Yee E update */
 
+#define PICTOR_GEOM_CARTESIAN 0  /* periodic x, y (, z)                      */
+#define PICTOR_GEOM_CYL 1        /* the -Dcyl build with the axis (inc_axis):
+                                    x = r (index i at r = i - 3.5), y = theta
+                                    (ny even, 2 pi periodic), z periodic; FP64
+                                    form; src/cyl/cyl_em_update.F90,
+                                    cyl_movdep.F90; conducting wall beyond
+                                    r = nx - 1/2                              */
+
 /* Run configuration: the compile-time parameters of parameters.F90:5-87 and
  * src/vars.F90:10-18 as a runtime struct.  grid_dx = grid_dy = grid_dz = 1,
  * EMspeedBoost = 1, curr_filter = 0, periodic faces on every side. */
@@ -50,6 +58,8 @@ typedef struct pictor_config {
     int32_t form;          /* PICTOR_FORM_*                                  */
     int32_t tile;          /* tile edge in cells (0 = choose); must divide nx, ny (nz) */
     int32_t fdtd;          /* PICTOR_FDTD_*: the B half step of em_update.F90 or em_update_lehe.F90 */
+    int32_t geometry;      /* PICTOR_GEOM_* (the reference's -Dcyl switch)   */
+    int32_t reserved;      /* 0 */
     double c;              /* parameters.F90:20 */
     double c_ompe;         /* parameters.F90:19 */
     double g0;             /* parameters.F90:16 */

@@ -1077,3 +1077,608 @@ void orc_momentum(const orc_params *p, const orc_particles *s, double out[3])
         out[0] += m * s->u[n]; out[1] += m * s->v[n]; out[2] += m * s->w[n];
     }
 }
+
+/* ================================================================== */
+/* Cylindrical geometry (the reference built with -Dcyl, inc_axis)     */
+/* x = r, y = theta, z = z; one rank that holds the axis.              */
+/* ================================================================== */
+
+/* InitParam_cyl, cyl_common.F90:19-24, for one rank with xborders(0) = 0;
+ * the Lehe-cyl coefficients of cyl_em_update_lehe.F90:6-17 with dtheta
+ * evaluated at run time (grid_dx = grid_dz = 1). */
+int orc_cyl_init(orc_cyl *cy, const orc_params *p)
+{
+    if (p->ny % 2 != 0) return ORC_ERR_ARG;   /* the axis maps j to j + ny/2 */
+    memset(cy, 0, sizeof(*cy));
+    const double pi = 3.14159265358979323846264;   /* vars.F90:224 */
+    cy->dtheta = 2.0 * pi / p->ny;
+    cy->ax_perm_area = 4.0 / p->ny;
+    cy->inv_dtheta = 1.0 / cy->dtheta;
+    cy->rshift = -0.5;
+    cy->rmin = 0.0 + cy->rshift;               /* xborders(procxind) + rshift */
+    cy->r_proc = 0.0 + cy->rshift - p->xmin;   /* cyl_movdep.F90:56 */
+    cy->second_half_new_cell = 1;
+    cy->substep_limit = 1000000;
+    const double bxy = 0.0, bxz = 0.125, byx = 0.0, byz = 0.125;
+    const double bzx = 0.125 * (1.0 / 1.0) * (1.0 / 1.0);
+    const double bzy = 0.125 * (1.0 / (1.0 * cy->dtheta)) * (1.0 / (1.0 * cy->dtheta));
+    const double dx = 0.0, dy = 0.0;
+    const double sn = sin(0.5 * pi * p->c);
+    const double dz = 0.25 * (1.0 - (p->cinv * p->cinv) * (sn * sn));
+    cy->lehe[0] = 1.0 - 2.0 * bxy - 2.0 * bxz - 3.0 * dx;
+    cy->lehe[1] = 1.0 - 2.0 * byx - 2.0 * byz - 3.0 * dy;
+    cy->lehe[2] = 1.0 - 2.0 * bzx - 2.0 * bzy - 3.0 * dz;
+    cy->lehe[3] = bxy; cy->lehe[4] = bxz; cy->lehe[5] = byx;
+    cy->lehe[6] = byz; cy->lehe[7] = bzx; cy->lehe[8] = bzy;
+    cy->lehe[9] = dx; cy->lehe[10] = dy; cy->lehe[11] = dz;
+    return ORC_OK;
+}
+
+/* the grid point across the axis: j + ny/2 folded into the physical range */
+static inline int cyl_jpp(const orc_params *p, int j)
+{
+    int jpp = j + p->ny / 2;
+    if (jpp > p->my - 3) jpp = jpp - p->ny;
+    return jpp;
+}
+
+/* UpdateEFldAxis, cyl_em_update.F90:185-225 */
+static void cyl_e_axis(const orc_params *p, orc_fields *f)
+{
+    const int k1 = (p->ndim == 2) ? 1 : p->mz;
+    for (int k = 1; k <= k1; k++)
+        for (int j = 1; j <= p->my; j++) {
+            const int jpp = cyl_jpp(p, j);
+            f->Ex[IX(p, 3, j, k)] = 0.5 * (f->Ex[IX(p, 4, j, k)] - f->Ex[IX(p, 4, jpp, k)]);
+            f->Ex[IX(p, 2, j, k)] = -f->Ex[IX(p, 4, jpp, k)];
+            f->Ey[IX(p, 3, j, k)] = -f->Ey[IX(p, 4, jpp, k)];
+            f->Ez[IX(p, 3, j, k)] = f->Ez[IX(p, 4, jpp, k)];
+        }
+}
+
+/* UpdateBFldAxis, cyl_em_update.F90:227-260 */
+static void cyl_b_axis(const orc_params *p, orc_fields *f)
+{
+    const int k1 = (p->ndim == 2) ? 1 : p->mz;
+    for (int k = 1; k <= k1; k++)
+        for (int j = 1; j <= p->my; j++) {
+            const int jpp = cyl_jpp(p, j);
+            f->By[IX(p, 3, j, k)] = 0.5 * (f->By[IX(p, 4, j, k)] - f->By[IX(p, 4, jpp, k)]);
+            f->By[IX(p, 2, j, k)] = -f->By[IX(p, 4, jpp, k)];
+            f->Bx[IX(p, 3, j, k)] = -f->Bx[IX(p, 4, jpp, k)];
+            f->Bz[IX(p, 2, j, k)] = f->Bz[IX(p, 4, jpp, k)];
+        }
+}
+
+/* UpdateEfield, cyl_em_update.F90:7-46 (axis rank: i from 4) */
+void orc_cyl_update_e(const orc_params *p, const orc_cyl *cy, orc_fields *f)
+{
+    const int three = (p->ndim == 3);
+    const int i1 = 4, k1 = three ? 3 : 1, k2 = three ? p->mz - 3 : 1;
+    const double fldc = p->fldc, dtheta = cy->dtheta, rmin = cy->rmin;
+    const double gdx = 1.0, gidx = p->inv_dx, gidz = p->inv_dz;
+    double *Ex = f->Ex, *Ey = f->Ey, *Ez = f->Ez;
+    const double *Bx = f->Bx, *By = f->By, *Bz = f->Bz;
+    for (int k = k1; k <= k2; k++)
+        for (int j = 3; j <= p->my - 3; j++)
+            for (int i = i1; i <= p->mx - 3; i++) {
+                const double r = ((i - 3.0) + rmin) * gdx;
+                const double rp_half = r + 0.5 * gdx;
+                const double rm_half = r - 0.5 * gdx;
+                const int64_t o = IX(p, i, j, k);
+                if (three) {
+                    Ex[o] = Ex[o] + fldc * (Bz[o] - Bz[IX(p, i, j - 1, k)]) / (rp_half * dtheta)
+                            - fldc * gidz * (By[o] - By[IX(p, i, j, k - 1)]);
+                    Ey[o] = Ey[o] + fldc * gidz * (Bx[o] - Bx[IX(p, i, j, k - 1)])
+                            - fldc * gidx * (Bz[o] - Bz[IX(p, i - 1, j, k)]);
+                    Ez[o] = Ez[o] + fldc * gidx * (rp_half * By[o] - rm_half * By[IX(p, i - 1, j, k)]) / r
+                            - fldc * (Bx[o] - Bx[IX(p, i, j - 1, k)]) / (r * dtheta);
+                } else {
+                    Ex[o] = Ex[o] + fldc * (Bz[o] - Bz[IX(p, i, j - 1, k)]) / (rp_half * dtheta);
+                    Ey[o] = Ey[o] - fldc * gidx * (Bz[o] - Bz[IX(p, i - 1, j, k)]);
+                    Ez[o] = Ez[o] + fldc * (gidx * (rp_half * By[o] - rm_half * By[IX(p, i - 1, j, k)])
+                                            - (Bx[o] - Bx[IX(p, i, j - 1, k)]) / dtheta) / r;
+                }
+            }
+    cyl_e_axis(p, f);
+}
+
+/* gradX, gradX1, gradY, gradZ of cyl_em_update_lehe.F90:255-289.  The
+ * reference file ends gradX, gradX1 and gradY with a dangling continuation;
+ * the terms it would add have zero coefficients there (beta_xy = delta_x =
+ * beta_yx = delta_y = 0), so the sums below are complete. */
+static double cyl_gradx(const orc_cyl *cy, const orc_params *p, const double *F, int i, int j, int k)
+{
+    const double *L = cy->lehe;
+    return L[0] * (F[IX(p, i + 1, j, k)] - F[IX(p, i, j, k)])
+           + L[4] * (F[IX(p, i + 1, j, k + 1)] - F[IX(p, i, j, k + 1)])
+           + L[4] * (F[IX(p, i + 1, j, k - 1)] - F[IX(p, i, j, k - 1)]);
+}
+
+static double cyl_gradx1(const orc_cyl *cy, const orc_params *p, const double *F, int i, int j, int k, double rp,
+                         double r)
+{
+    const double *L = cy->lehe;
+    return L[0] * (rp * F[IX(p, i + 1, j, k)] - r * F[IX(p, i, j, k)])
+           + L[4] * (rp * F[IX(p, i + 1, j, k + 1)] - r * F[IX(p, i, j, k + 1)])
+           + L[4] * (rp * F[IX(p, i + 1, j, k - 1)] - r * F[IX(p, i, j, k - 1)]);
+}
+
+static double cyl_grady(const orc_cyl *cy, const orc_params *p, const double *F, int i, int j, int k)
+{
+    const double *L = cy->lehe;
+    const double gidy = p->inv_dy;
+    return L[1] * gidy * (F[IX(p, i, j + 1, k)] - F[IX(p, i, j, k)])
+           + L[6] * gidy * (F[IX(p, i, j + 1, k + 1)] - F[IX(p, i, j, k + 1)])
+           + L[6] * gidy * (F[IX(p, i, j + 1, k - 1)] - F[IX(p, i, j, k - 1)]);
+}
+
+static double cyl_gradz(const orc_cyl *cy, const orc_params *p, const double *F, int i, int j, int k)
+{
+    const double *L = cy->lehe;
+    return L[2] * (F[IX(p, i, j, k + 1)] - F[IX(p, i, j, k)])
+           + L[7] * (F[IX(p, i + 1, j, k + 1)] - F[IX(p, i + 1, j, k)])
+           + L[7] * (F[IX(p, i - 1, j, k + 1)] - F[IX(p, i - 1, j, k)])
+           + L[8] * (F[IX(p, i, j + 1, k + 1)] - F[IX(p, i, j + 1, k)])
+           + L[8] * (F[IX(p, i, j - 1, k + 1)] - F[IX(p, i, j - 1, k)])
+           + L[11] * (F[IX(p, i, j, k + 2)] - F[IX(p, i, j, k - 1)]);
+}
+
+/* UpdateBzAxis, cyl_em_update.F90:262-280: Faraday's law over the disk r < 1/2 */
+static void cyl_bz_axis(const orc_params *p, const orc_cyl *cy, orc_fields *f)
+{
+    const int three = (p->ndim == 3);
+    const int kt = three ? p->mz - 1 : 1;
+    for (int k = 1; k <= kt; k++) {
+        double ax = 0.0;
+        for (int j = 3; j <= p->my - 3; j++) ax = ax - p->fld_halfc * cy->ax_perm_area * f->Ey[IX(p, 4, j, k)];
+        ax = f->Bz[IX(p, 3, 3, k)] + ax;
+        for (int j = 1; j <= p->my - 1; j++) f->Bz[IX(p, 3, j, k)] = ax;
+    }
+}
+
+/* UpdateBfieldHalf, cyl_em_update.F90:49-110 (fdtd = 0) and
+ * cyl_em_update_lehe.F90:62-118 (fdtd = 1, 3-D), axis rank */
+void orc_cyl_update_b_half(const orc_params *p, const orc_cyl *cy, orc_fields *f)
+{
+    const int three = (p->ndim == 3);
+    const int lehe = (p->fdtd == 1) && three;
+    const int i1 = 4, k1 = 1, k2 = three ? p->mz - 1 : 1;
+    const double h = p->fld_halfc, dtheta = cy->dtheta, rmin = cy->rmin;
+    const double gdx = 1.0, gidx = p->inv_dx, gidz = p->inv_dz;
+    double *Bx = f->Bx, *By = f->By, *Bz = f->Bz;
+    const double *Ex = f->Ex, *Ey = f->Ey, *Ez = f->Ez;
+    for (int k = k1; k <= k2; k++)
+        for (int j = 1; j <= p->my - 1; j++)
+            for (int i = i1; i <= p->mx - 1; i++) {
+                const double r = ((i - 3.0) + rmin) * gdx;
+                const double rp_half = r + 0.5 * gdx;
+                const double rp = r + gdx;
+                const int64_t o = IX(p, i, j, k);
+                if (lehe) {
+                    /* gradZ reaches j - 1, k - 1 and k + 2: the outermost ghost layers are
+                     * skipped (the reference reads outside its arrays there; the halo
+                     * exchange overwrites these layers) */
+                    if (j < 2 || k < 2 || k > p->mz - 2) continue;
+                    Bx[o] = Bx[o] - h * (cyl_grady(cy, p, Ez, i, j, k)) / (r * dtheta)
+                            + h * gidz * (cyl_gradz(cy, p, Ey, i, j, k));
+                    By[o] = By[o] - h * gidz * (cyl_gradz(cy, p, Ex, i, j, k)) + h * gidx * (cyl_gradx(cy, p, Ez, i, j, k));
+                    Bz[o] = Bz[o] - h * (gidx * cyl_gradx1(cy, p, Ey, i, j, k, rp, r) - cyl_grady(cy, p, Ex, i, j, k) / dtheta)
+                                        / rp_half;
+                } else if (three) {
+                    Bx[o] = Bx[o] - h * (Ez[IX(p, i, j + 1, k)] - Ez[o]) / (r * dtheta)
+                            + h * gidz * (Ey[IX(p, i, j, k + 1)] - Ey[o]);
+                    By[o] = By[o] - h * gidz * (Ex[IX(p, i, j, k + 1)] - Ex[o]) + h * gidx * (Ez[IX(p, i + 1, j, k)] - Ez[o]);
+                    Bz[o] = Bz[o] - h * (gidx * (rp * Ey[IX(p, i + 1, j, k)] - r * Ey[o])
+                                         - (Ex[IX(p, i, j + 1, k)] - Ex[o]) / dtheta) / rp_half;
+                } else {
+                    Bx[o] = Bx[o] - h * (Ez[IX(p, i, j + 1, k)] - Ez[o]) / (r * dtheta);
+                    By[o] = By[o] + h * gidx * (Ez[IX(p, i + 1, j, k)] - Ez[o]);
+                    Bz[o] = Bz[o] - h * (gidx * (rp * Ey[IX(p, i + 1, j, k)] - r * Ey[o])
+                                         - (Ex[IX(p, i, j + 1, k)] - Ex[o]) / dtheta) / rp_half;
+                }
+            }
+    cyl_bz_axis(p, cy, f);
+    cyl_b_axis(p, f);
+}
+
+/* AddCurrent, cyl_em_update.F90:114-140 (RZtwoD = .false.) */
+void orc_cyl_add_current(const orc_params *p, const orc_cyl *cy, orc_fields *f)
+{
+    const int three = (p->ndim == 3);
+    const int i1 = 4, k1 = three ? 3 : 1, k2 = three ? p->mz - 3 : 1;
+    const double dtheta = cy->dtheta, rmin = cy->rmin;
+    const double gdx = 1.0, gidx = p->inv_dx, gidz = p->inv_dz;
+    for (int k = k1; k <= k2; k++)
+        for (int j = 3; j <= p->my - 3; j++)
+            for (int i = i1; i <= p->mx - 3; i++) {
+                const double r = ((i - 3.0) + rmin) * gdx;
+                const double rp_half = r + 0.5 * gdx;
+                const int64_t o = IX(p, i, j, k);
+                f->Ex[o] = f->Ex[o] - f->Jx[o] * gidz / (rp_half * dtheta);
+                f->Ey[o] = f->Ey[o] - f->Jy[o] * gidx * gidz;
+                f->Ez[o] = f->Ez[o] - f->Jz[o] * gidx / (r * dtheta);
+            }
+    cyl_e_axis(p, f);
+}
+
+/* AxisCurrentBC, cyl_movdep.F90:451-465 */
+void orc_cyl_axis_current_bc(const orc_params *p, orc_fields *f)
+{
+    for (int k = 1; k <= p->mz; k++)
+        for (int j = 1; j <= p->my; j++) {
+            f->Jy[IX(p, 4, j, k)] = f->Jy[IX(p, 4, j, k)] - f->Jy[IX(p, 3, j, k)];
+            f->Jz[IX(p, 4, j, k)] = f->Jz[IX(p, 4, j, k)] - f->Jz[IX(p, 3, j, k)];
+        }
+    for (int k = 1; k <= p->mz; k++)
+        for (int j = 1; j <= p->my; j++) {
+            f->Jx[IX(p, 3, j, k)] = 0.0;
+            f->Jy[IX(p, 2, j, k)] = 0.0;
+            f->Jz[IX(p, 2, j, k)] = 0.0;
+        }
+}
+
+/* Field halos of the axis rank: theta (periodic) faces, then z faces (3-D),
+ * as comm_fld.F90:75-91 with the x faces left out (the axis routines fill
+ * the inner x ghosts); the outer x ghosts are held at zero, a conducting
+ * wall at r = nx - 1/2 where the tangential E sits. */
+void orc_cyl_field_halo(const orc_params *p, double *fx, double *fy, double *fz)
+{
+    double *F[3] = {fx, fy, fz};
+    for (int c = 0; c < 3; c++) halo_copy_axis(p, F[c], 1);
+    if (p->ndim == 3)
+        for (int c = 0; c < 3; c++) halo_copy_axis(p, F[c], 2);
+    for (int c = 0; c < 3; c++)
+        for (int k = 1; k <= p->mz; k++)
+            for (int j = 1; j <= p->my; j++)
+                for (int i = p->mx - 2; i <= p->mx; i++) F[c][IX(p, i, j, k)] = 0.0;
+}
+
+/* Current halo of the axis rank: theta faces, then z faces (3-D) */
+void orc_cyl_current_halo_add(const orc_params *p, double *jx, double *jy, double *jz)
+{
+    double *F[3] = {jx, jy, jz};
+    for (int c = 0; c < 3; c++) halo_add_axis(p, F[c], 1);
+    if (p->ndim == 3)
+        for (int c = 0; c < 3; c++) halo_add_axis(p, F[c], 2);
+}
+
+/* DepositCurrentPIC, deposit.F90:11-103 */
+static void cyl_deposit_pic(const orc_params *p, orc_fields *f, double x0, double y0, double z0, double x, double y,
+                            double z, double q)
+{
+    const double qthis = q * p->qi;
+    int i1 = (int)floor(x0), i2 = (int)floor(x);
+    int j1 = (int)floor(y0), j2 = (int)floor(y);
+    int k1 = (int)floor(z0), k2 = (int)floor(z);
+    const double xr = dmin((double)(imin2(i1, i2) + 1), dmax((double)imax2(i1, i2), 0.5 * (x0 + x)));
+    const double yr = dmin((double)(imin2(j1, j2) + 1), dmax((double)imax2(j1, j2), 0.5 * (y0 + y)));
+    const double zr = dmin((double)(imin2(k1, k2) + 1), dmax((double)imax2(k1, k2), 0.5 * (z0 + z)));
+    const double Fx1 = qthis * (xr - x0), Fy1 = qthis * (yr - y0), Fz1 = qthis * (zr - z0);
+    const double Wx1 = 0.5 * (x0 + xr) - i1, Wy1 = 0.5 * (y0 + yr) - j1;
+    const double Wx2 = 0.5 * (x + xr) - i2, Wy2 = 0.5 * (y + yr) - j2;
+    double Wz1, Wz2;
+    const int three = (p->ndim == 3);
+    if (!three) { k1 = 1; k2 = 1; Wz1 = 0.0; Wz2 = 0.0; }
+    else { Wz1 = 0.5 * (z0 + zr) - k1; Wz2 = 0.5 * (z + zr) - k2; }
+    const double Fx2 = qthis * (x - xr), Fy2 = qthis * (y - yr), Fz2 = qthis * (z - zr);
+    double *Jx = f->Jx, *Jy = f->Jy, *Jz = f->Jz;
+#define JA(A, i, j, k, v) A[IX(p, i, j, k)] = A[IX(p, i, j, k)] + (v)
+    JA(Jx, i1, j1, k1, Fx1 * (1.0 - Wy1) * (1.0 - Wz1));
+    JA(Jx, i1, j1 + 1, k1, Fx1 * Wy1 * (1.0 - Wz1));
+    if (three) {
+        JA(Jx, i1, j1, k1 + 1, Fx1 * (1.0 - Wy1) * Wz1);
+        JA(Jx, i1, j1 + 1, k1 + 1, Fx1 * Wy1 * Wz1);
+    }
+    JA(Jx, i2, j2, k2, Fx2 * (1.0 - Wy2) * (1.0 - Wz2));
+    JA(Jx, i2, j2 + 1, k2, Fx2 * Wy2 * (1.0 - Wz2));
+    if (three) {
+        JA(Jx, i2, j2, k2 + 1, Fx2 * (1.0 - Wy2) * Wz2);
+        JA(Jx, i2, j2 + 1, k2 + 1, Fx2 * Wy2 * Wz2);
+    }
+    JA(Jy, i1, j1, k1, Fy1 * (1.0 - Wx1) * (1.0 - Wz1));
+    JA(Jy, i1 + 1, j1, k1, Fy1 * Wx1 * (1.0 - Wz1));
+    if (three) {
+        JA(Jy, i1, j1, k1 + 1, Fy1 * (1.0 - Wx1) * Wz1);
+        JA(Jy, i1 + 1, j1, k1 + 1, Fy1 * Wx1 * Wz1);
+    }
+    JA(Jy, i2, j2, k2, Fy2 * (1.0 - Wx2) * (1.0 - Wz2));
+    JA(Jy, i2 + 1, j2, k2, Fy2 * Wx2 * (1.0 - Wz2));
+    if (three) {
+        JA(Jy, i2, j2, k2 + 1, Fy2 * (1.0 - Wx2) * Wz2);
+        JA(Jy, i2 + 1, j2, k2 + 1, Fy2 * Wx2 * Wz2);
+    }
+    JA(Jz, i1, j1, k1, Fz1 * (1.0 - Wx1) * (1.0 - Wy1));
+    JA(Jz, i1 + 1, j1, k1, Fz1 * Wx1 * (1.0 - Wy1));
+    JA(Jz, i1, j1 + 1, k1, Fz1 * (1.0 - Wx1) * Wy1);
+    JA(Jz, i1 + 1, j1 + 1, k1, Fz1 * Wx1 * Wy1);
+    JA(Jz, i2, j2, k2, Fz2 * (1.0 - Wx2) * (1.0 - Wy2));
+    JA(Jz, i2 + 1, j2, k2, Fz2 * Wx2 * (1.0 - Wy2));
+    JA(Jz, i2, j2 + 1, k2, Fz2 * (1.0 - Wx2) * Wy2);
+    JA(Jz, i2 + 1, j2 + 1, k2, Fz2 * Wx2 * Wy2);
+#undef JA
+}
+
+/* Interp_VecEM_GridPoints, interpolation.F90:434-490 */
+static void cyl_interp(const orc_params *p, const orc_ws *w, double x, double y, double z, double e[3], double b[3])
+{
+    const int two = (p->ndim == 2);
+    const int ip0 = (int)x, jp0 = (int)y;
+    const double dx = x - ip0, dy = y - jp0;
+    double wt[8];
+    int64_t sidx;
+    if (!two) {
+        const int kp0 = (int)z;
+        const double dz = z - kp0;
+        for (int nn = 0; nn < 8; nn++)
+            wt[nn] = (wtx1_3[nn] + wtx2_3[nn] * dx) * (wty1_3[nn] + wty2_3[nn] * dy) * (wtz1_3[nn] + wtz2_3[nn] * dz);
+        sidx = (int64_t)(kp0 - 1) * p->mx * p->my + (int64_t)(jp0 - 1) * p->mx + ip0;
+    } else {
+        for (int nn = 0; nn < 4; nn++) wt[nn] = (wtx1_2[nn] + wtx2_2[nn] * dx) * (wty1_2[nn] + wty2_2[nn] * dy);
+        sidx = (int64_t)(jp0 - 1) * p->mx + ip0;
+    }
+    const int nv = w->nv;
+    const int64_t bo = nv * (sidx - 1);
+    for (int c = 0; c < 3; c++) { e[c] = 0.0; b[c] = 0.0; }
+    for (int nn = 0; nn < nv; nn++) {
+        e[0] = e[0] + wt[nn] * w->vec[0][bo + nn];
+        e[1] = e[1] + wt[nn] * w->vec[1][bo + nn];
+        e[2] = e[2] + wt[nn] * w->vec[2][bo + nn];
+        b[0] = b[0] + wt[nn] * w->vec[3][bo + nn];
+        b[1] = b[1] + wt[nn] * w->vec[4][bo + nn];
+        b[2] = b[2] + wt[nn] * w->vec[5][bo + nn];
+    }
+}
+
+/* MovePrtlAxis, cyl_movdep.F90:381-446: one straight sub-step of length dt;
+ * returns 0 when the azimuthal move exceeds 0.45 cells (the caller halves dt) */
+static int cyl_move_axis(const orc_params *p, const orc_cyl *cy, const orc_ws *w, orc_particles *s, int64_t n,
+                         double x, double y, double z, double dt)
+{
+    double e[3], b[3];
+    cyl_interp(p, w, x, y, z, e, b);
+    const double qm = (s->flv[n] == 0) ? 0.0 : p->flvrqm[s->flv[n] - 1] * 0.5;
+    double pEx = e[0] * qm * dt, pEy = e[1] * qm * dt, pEz = e[2] * qm * dt;
+    double pBx = b[0] * qm * dt, pBy = b[1] * qm * dt, pBz = b[2] * qm * dt;
+    const double c = p->c, sqc = p->sqc, cinv = p->cinv;
+    double u0 = c * s->u[n] + pEx;
+    double v0 = c * s->v[n] + pEy;
+    double w0 = c * s->w[n] + pEz;
+    double g = 1.0 / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+    pBx = g * pBx; pBy = g * pBy; pBz = g * pBz;
+    const double f = 2.0 / (1.0 + pBx * pBx + pBy * pBy + pBz * pBz);
+    double u1 = (u0 + v0 * pBz - w0 * pBy) * f;
+    double v1 = (v0 + w0 * pBx - u0 * pBz) * f;
+    const double w1 = (w0 + u0 * pBy - v0 * pBx) * f;
+    u0 = u0 + v1 * pBz - w1 * pBy + pEx;
+    v0 = v0 + w1 * pBx - u1 * pBz + pEy;
+    w0 = w0 + u1 * pBy - v1 * pBx + pEz;
+    u1 = u0 * cinv;
+    v1 = v0 * cinv;
+    g = sqc / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+    const double x1 = (x - 3.5) * 1.0 + u1 * g * dt;
+    const double y1 = v1 * g * dt;
+    const double r1 = sqrt(x1 * x1 + y1 * y1);
+    double dy;
+    if (x1 > 0) dy = atan(y1 / x1) * cy->inv_dtheta;
+    else dy = 0.0;
+    if (fabs(dy) > 0.45) return 0;
+    if (x1 > 0) {
+        s->x[n] = r1 * p->inv_dx + 3.5;
+        s->u[n] = (x1 * u1 + y1 * v1) / r1;
+        s->v[n] = (-y1 * u1 + x1 * v1) / r1;
+    } else {
+        s->x[n] = x1 * p->inv_dx + 3.5;
+        s->u[n] = u1;
+        s->v[n] = v1;
+    }
+    s->y[n] = s->y[n] + dy;
+    s->w[n] = w0 * cinv;
+    s->z[n] = s->z[n] + s->w[n] * g * dt * p->inv_dz;
+    return 1;
+}
+
+/* PrtlAxisBC, cyl_movdep.F90:366-379 */
+static void cyl_axis_bc(const orc_params *p, orc_particles *s, int64_t n)
+{
+    if (s->x[n] < 3.5) {
+        s->x[n] = 3.5 + (3.5 - s->x[n]);
+        s->y[n] = s->y[n] + p->ny / 2.0;
+        s->u[n] = -s->u[n];
+        s->v[n] = -s->v[n];
+    }
+    if (s->y[n] > p->ymax) s->y[n] = s->y[n] - p->ylen;
+    if (s->y[n] < p->ymin) s->y[n] = s->y[n] + p->ylen;
+}
+
+/* SubStepMovDep, cyl_movdep.F90:315-349; returns the number of sub-steps or -1 past the limit */
+static int64_t cyl_substep(const orc_params *p, const orc_cyl *cy, const orc_ws *w, orc_fields *f, orc_particles *s,
+                           int64_t n, double x0, double y0, double z0, double u0, double v0, double w0)
+{
+    if (s->flv[n] == 0) return 0;
+    s->x[n] = x0; s->y[n] = y0; s->z[n] = z0;
+    s->u[n] = u0; s->v[n] = v0; s->w[n] = w0;
+    double t = 0.0, dt = 1.0;
+    int64_t tries = 0;
+    while (t < 1.0) {
+        if (++tries > cy->substep_limit) return -1;
+        double delt = dt;
+        if (t + dt > 1.0) delt = 1.0 - t;
+        const int res = cyl_move_axis(p, cy, w, s, n, x0, y0, z0, delt);
+        if (res == 1) {
+            t = t + delt;
+            cyl_deposit_pic(p, f, x0, y0, z0, s->x[n], s->y[n], s->z[n], s->q[n]);
+            cyl_axis_bc(p, s, n);
+            dt = dt * 1.1;
+            x0 = s->x[n]; y0 = s->y[n]; z0 = s->z[n];
+        } else {
+            dt = dt * 0.5;
+        }
+    }
+    return tries;
+}
+
+/* MoveDepositPrtl, cyl_movdep.F90:14-289, one rank holding the axis.
+ * second_half_new_cell = 1 deposits the second zigzag half-segment at the
+ * particle's new cell (charge-conserving); 0 keeps the reference's
+ * short_arr_ind of (ip0, jp0) for it (cyl_movdep.F90:249,252).
+ * Returns ORC_ERR_ARG if a particle reaches the outer two cells or a
+ * sub-step loop passes substep_limit. */
+int orc_cyl_move_deposit(const orc_params *p, orc_cyl *cy, orc_ws *w, orc_fields *f, orc_particles *s)
+{
+    const int two = (p->ndim == 2);
+    const int nj = w->nj;
+    const int mx = p->mx, my = p->my;
+    const int64_t used = s->used, ncell = w->ncell;
+    const double c = p->c, sqc = p->sqc, cinv = p->cinv, qi = p->qi;
+    const double *wtFx = two ? wtFx_2 : wtFx_3, *wtFy = two ? wtFy_2 : wtFy_3, *wtFz = two ? wtFz_2 : wtFz_3;
+    const double *Jwtx1 = two ? Jwtx1_2 : Jwtx1_3, *Jwtx2 = two ? Jwtx2_2 : Jwtx2_3;
+    const double *Jwty1 = two ? Jwty1_2 : Jwty1_3, *Jwty2 = two ? Jwty2_2 : Jwty2_3;
+    const double *Jwtz1 = two ? Jwtz1_2 : Jwtz1_3, *Jwtz2 = two ? Jwtz2_2 : Jwtz2_3;
+    const double r_proc = cy->r_proc;
+    int rc = ORC_OK;
+    cy->substeps = 0;
+
+    orc_gather_vec_e(p, w, f);
+    orc_gather_vec_b(p, w, f);
+    memset(w->vecj, 0, sizeof(double) * (size_t)(nj * ncell * w->nthreads));
+    double *VJ = w->vecj;   /* one thread: the sub-steps deposit into J directly */
+
+    for (int64_t off = 0; off < used; off += VBS) {
+        const int nb = (int)((used - off < VBS) ? (used - off) : VBS);
+        double x0[VBS], y0[VBS], z0[VBS], qthis[VBS], ui[VBS], vi[VBS], wi[VBS];
+        int ip0[VBS], jp0[VBS], kp0[VBS];
+        for (int n = 0; n < nb; n++) {
+            const int64_t m = off + n;
+            x0[n] = s->x[m]; y0[n] = s->y[m]; z0[n] = s->z[m];
+            qthis[n] = s->q[m] * qi;
+            ui[n] = s->u[m]; vi[n] = s->v[m]; wi[n] = s->w[m];
+        }
+        for (int n = 0; n < nb; n++) {
+            const int64_t m = off + n;
+            const double qm = (s->flv[m] == 0) ? 0.0 : p->flvrqm[s->flv[m] - 1] * 0.5;
+            double e[3], b[3];
+            ip0[n] = (int)x0[n];
+            jp0[n] = (int)y0[n];
+            kp0[n] = two ? 1 : (int)z0[n];
+            cyl_interp(p, w, x0[n], y0[n], z0[n], e, b);
+            double pEx = e[0] * qm, pEy = e[1] * qm, pEz = e[2] * qm;
+            double pBx = b[0] * qm, pBy = b[1] * qm, pBz = b[2] * qm;
+            double u0 = c * s->u[m] + pEx;
+            double v0 = c * s->v[m] + pEy;
+            double w0 = c * s->w[m] + pEz;
+            double g = 1.0 / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+            pBx = g * pBx; pBy = g * pBy; pBz = g * pBz;
+            const double ff = 2.0 / (1.0 + pBx * pBx + pBy * pBy + pBz * pBz);
+            double u1 = (u0 + v0 * pBz - w0 * pBy) * ff;
+            double v1 = (v0 + w0 * pBx - u0 * pBz) * ff;
+            const double w1 = (w0 + u0 * pBy - v0 * pBx) * ff;
+            u0 = u0 + v1 * pBz - w1 * pBy + pEx;
+            v0 = v0 + w1 * pBx - u1 * pBz + pEy;
+            w0 = w0 + u1 * pBy - v1 * pBx + pEz;
+            u1 = u0 * cinv;
+            v1 = v0 * cinv;
+            s->w[m] = w0 * cinv;
+            g = sqc / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+            const double x1 = (s->x[m] + r_proc) * 1.0 + u1 * g;
+            const double y1 = v1 * g;
+            double r1 = sqrt(x1 * x1 + y1 * y1);
+            s->x[m] = r1 * p->inv_dx - r_proc;
+            s->y[m] = s->y[m] + atan(y1 / x1) * cy->inv_dtheta;
+            s->z[m] = s->z[m] + s->w[m] * g * p->inv_dz;
+            r1 = 1.0 / r1;
+            s->u[m] = (x1 * u1 + y1 * v1) * r1;
+            s->v[m] = (-y1 * u1 + x1 * v1) * r1;
+        }
+        /* MovDepAxis, cyl_movdep.F90:300-312 */
+        for (int n = 0; n < nb && rc == ORC_OK; n++)
+            if (x0[n] < 4.5) {
+                const int64_t k = cyl_substep(p, cy, w, f, s, off + n, x0[n], y0[n], z0[n], ui[n], vi[n], wi[n]);
+                if (k < 0) rc = ORC_ERR_ARG;
+                else cy->substeps += k;
+                qthis[n] = 0.0;
+            }
+        for (int n = 0; n < nb && rc == ORC_OK; n++) {
+            const int64_t m = off + n;
+            if (s->x[m] >= mx - 3) { rc = ORC_ERR_ARG; break; }   /* the outer wall region */
+            const int ip = (int)s->x[m], jp = (int)s->y[m];
+            /* near the axis one step can cross several theta cells; the deposit
+             * window of the arrays ends at rows 1 and my - 1 */
+            if (jp < 1 || jp > my - 1) { rc = ORC_ERR_ARG; break; }
+            const int kp = two ? 1 : (int)s->z[m];
+            const double xr = dmin((double)(imin2(ip0[n], ip) + 1), dmax((double)imax2(ip0[n], ip), 0.5 * (x0[n] + s->x[m])));
+            const double yr = dmin((double)(imin2(jp0[n], jp) + 1), dmax((double)imax2(jp0[n], jp), 0.5 * (y0[n] + s->y[m])));
+            const double zr = dmin((double)(imin2(kp0[n], kp) + 1), dmax((double)imax2(kp0[n], kp), 0.5 * (z0[n] + s->z[m])));
+            const double Fx1 = qthis[n] * (xr - x0[n]);
+            const double Fy1 = qthis[n] * (yr - y0[n]);
+            const double Fz1 = qthis[n] * (zr - z0[n]);
+            const double Wx1 = 0.5 * (x0[n] + xr) - ip0[n];
+            const double Wy1 = 0.5 * (y0[n] + yr) - jp0[n];
+            const double Wz1 = two ? 0.0 : 0.5 * (z0[n] + zr) - kp0[n];
+            int64_t sidx = two ? (int64_t)(jp0[n] - 1) * mx + ip0[n]
+                               : (int64_t)(kp0[n] - 1) * mx * my + (int64_t)(jp0[n] - 1) * mx + ip0[n];
+            double *vj = VJ + nj * (sidx - 1);
+            for (int nn = 0; nn < nj; nn++)
+                vj[nn] = vj[nn] + (wtFx[nn] * Fx1 + wtFy[nn] * Fy1 + wtFz[nn] * Fz1) * (Jwtx1[nn] + Jwtx2[nn] * Wx1)
+                                  * (Jwty1[nn] + Jwty2[nn] * Wy1) * (Jwtz1[nn] + Jwtz2[nn] * Wz1);
+            const double Fx2 = qthis[n] * (s->x[m] - xr);
+            const double Fy2 = qthis[n] * (s->y[m] - yr);
+            const double Fz2 = qthis[n] * (s->z[m] - zr);
+            const double Wx2 = 0.5 * (s->x[m] + xr) - ip;
+            const double Wy2 = 0.5 * (s->y[m] + yr) - jp;
+            const double Wz2 = two ? 0.0 : 0.5 * (s->z[m] + zr) - kp;
+            if (cy->second_half_new_cell)
+                sidx = two ? (int64_t)(jp - 1) * mx + ip : (int64_t)(kp - 1) * mx * my + (int64_t)(jp - 1) * mx + ip;
+            vj = VJ + nj * (sidx - 1);
+            for (int nn = 0; nn < nj; nn++)
+                vj[nn] = vj[nn] + (wtFx[nn] * Fx2 + wtFy[nn] * Fy2 + wtFz[nn] * Fz2) * (Jwtx1[nn] + Jwtx2[nn] * Wx2)
+                                  * (Jwty1[nn] + Jwty2[nn] * Wy2) * (Jwtz1[nn] + Jwtz2[nn] * Wz2);
+        }
+        if (two)
+            for (int n = 0; n < nb; n++) {
+                const int64_t m = off + n;
+                if (s->z[m] > p->zmax) s->z[m] = s->z[m] - p->zlen;
+                else if (s->z[m] < p->zmin) s->z[m] = p->zlen + s->z[m];
+            }
+        if (rc != ORC_OK) break;
+    }
+    if (rc != ORC_OK) return rc;
+    reduce_current(p, w, f);
+    orc_cyl_axis_current_bc(p, f);
+    return ORC_OK;
+}
+
+/* ExchangePrtl on the axis rank: periodic wrap in theta and (3-D) z, in
+ * place (the reference removes and appends; the order of particles is not
+ * part of the result) */
+void orc_cyl_exchange_particles(const orc_params *p, orc_particles *s)
+{
+    for (int64_t n = 0; n < s->used; n++) {
+        if (s->flv[n] == 0) continue;
+        if (s->y[n] < p->ymin) s->y[n] = s->y[n] + p->ylen;
+        else if (s->y[n] > p->ymax) s->y[n] = s->y[n] - p->ylen;
+        if (p->ndim == 3) {
+            if (s->z[n] < p->zmin) s->z[n] = s->z[n] + p->zlen;
+            else if (s->z[n] > p->zmax) s->z[n] = s->z[n] - p->zlen;
+        }
+    }
+}
+
+/* One time step in the MAIN.F90:42-284 order, cylindrical build */
+int orc_cyl_step(const orc_params *p, orc_cyl *cy, orc_ws *w, orc_fields *f, orc_particles *s)
+{
+    orc_reset_current(p, f);
+    orc_cyl_update_b_half(p, cy, f);
+    orc_cyl_field_halo(p, f->Bx, f->By, f->Bz);
+    int rc = orc_cyl_move_deposit(p, cy, w, f, s);
+    if (rc != ORC_OK) return rc;
+    orc_cyl_exchange_particles(p, s);
+    orc_cyl_update_b_half(p, cy, f);
+    orc_cyl_field_halo(p, f->Bx, f->By, f->Bz);
+    orc_cyl_update_e(p, cy, f);
+    orc_cyl_field_halo(p, f->Ex, f->Ey, f->Ez);
+    orc_cyl_current_halo_add(p, f->Jx, f->Jy, f->Jz);
+    orc_cyl_add_current(p, cy, f);
+    orc_cyl_field_halo(p, f->Ex, f->Ey, f->Ez);
+    return ORC_OK;
+}

@@ -99,6 +99,30 @@ void orc_rho(const orc_params *p, const orc_particles *s, double *rho);
 void orc_gauss(const orc_params *p, const orc_fields *f, const orc_particles *s, double out[2]);
 void orc_momentum(const orc_params *p, const orc_particles *s, double out[3]);
 
+/* ---- cylindrical geometry (-Dcyl, inc_axis, one rank holding the axis):
+ * x = r, y = theta, z = z; r of grid index i is i - 3.5 (the axis at x = 3.5) */
+typedef struct orc_cyl {
+    double dtheta, inv_dtheta, ax_perm_area, rshift;  /* cyl_common.F90:19-24 */
+    double rmin;             /* xborders(0) + rshift */
+    double r_proc;           /* xborders(0) + rshift - xmin, cyl_movdep.F90:56 */
+    int32_t second_half_new_cell;  /* 1: second zigzag half at the new cell; 0: cyl_movdep.F90:249,252 indexing */
+    int32_t pad_;
+    int64_t substep_limit;   /* bound on the axis sub-step loop (unbounded in the reference) */
+    int64_t substeps;        /* sub-steps taken by the last orc_cyl_move_deposit */
+    double lehe[12];         /* cyl_em_update_lehe.F90:6-17, the order of orc_params.lehe */
+} orc_cyl;
+
+int  orc_cyl_init(orc_cyl *cy, const orc_params *p);
+void orc_cyl_update_e(const orc_params *p, const orc_cyl *cy, orc_fields *f);
+void orc_cyl_update_b_half(const orc_params *p, const orc_cyl *cy, orc_fields *f);
+void orc_cyl_add_current(const orc_params *p, const orc_cyl *cy, orc_fields *f);
+void orc_cyl_axis_current_bc(const orc_params *p, orc_fields *f);
+void orc_cyl_field_halo(const orc_params *p, double *fx, double *fy, double *fz);
+void orc_cyl_current_halo_add(const orc_params *p, double *jx, double *jy, double *jz);
+int  orc_cyl_move_deposit(const orc_params *p, orc_cyl *cy, orc_ws *w, orc_fields *f, orc_particles *s);
+void orc_cyl_exchange_particles(const orc_params *p, orc_particles *s);
+int  orc_cyl_step(const orc_params *p, orc_cyl *cy, orc_ws *w, orc_fields *f, orc_particles *s);
+
 #ifdef __cplusplus
 }
 #endif

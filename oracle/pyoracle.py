@@ -34,6 +34,12 @@ class OrcParams(ctypes.Structure):
     )
 
 
+class OrcCyl(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_double) for n in ("dtheta", "inv_dtheta", "ax_perm_area", "rshift", "rmin", "r_proc")]
+                + [("second_half_new_cell", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                   ("substep_limit", ctypes.c_int64), ("substeps", ctypes.c_int64), ("lehe", ctypes.c_double * 12)])
+
+
 class OrcFields(ctypes.Structure):
     _fields_ = [(n, c_dp) for n in ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz")]
 
@@ -82,6 +88,16 @@ def lib():
         L.orc_rho.argtypes = [P, S, c_dp]
         L.orc_gauss.argtypes = [P, F, S, c_dp]
         L.orc_momentum.argtypes = [P, S, c_dp]
+        C = ctypes.POINTER(OrcCyl)
+        L.orc_cyl_init.argtypes = [C, P]
+        for name in ("orc_cyl_update_e", "orc_cyl_update_b_half", "orc_cyl_add_current"):
+            getattr(L, name).argtypes = [P, C, F]
+        L.orc_cyl_axis_current_bc.argtypes = [P, F]
+        L.orc_cyl_field_halo.argtypes = [P, c_dp, c_dp, c_dp]
+        L.orc_cyl_current_halo_add.argtypes = [P, c_dp, c_dp, c_dp]
+        L.orc_cyl_move_deposit.argtypes = [P, C, W, F, S]
+        L.orc_cyl_exchange_particles.argtypes = [P, S]
+        L.orc_cyl_step.argtypes = [P, C, W, F, S]
         _lib = L
     return _lib
 
@@ -236,3 +252,59 @@ class Oracle:
         out = np.zeros(self.shape, order="F")
         lib().orc_div_e(ctypes.byref(self.p), ctypes.byref(self.f), _ptr(out))
         return out
+
+
+class CylOracle(Oracle):
+    """FP64 reference CPU path of the cylindrical build (-Dcyl, inc_axis) on the
+    rank holding the axis: x = r (grid index i at r = i - 3.5), y = theta
+    (ny cells over 2 pi), z = z.  The outer x ghosts are a conducting wall."""
+
+    def __init__(self, ndim=2, nr=16, ntheta=16, nz=1, c=0.45, c_ompe=10.0, g0=1.0, mi_me=1.0, epc=16,
+                 cap=1024, fdtd="yee", second_half_new_cell=True):
+        super().__init__(ndim, nr, ntheta, nz, c=c, c_ompe=c_ompe, g0=g0, mi_me=mi_me, epc=epc, nthreads=1,
+                         cap=cap, reorder_period=0)
+        self.cy = OrcCyl()
+        if lib().orc_cyl_init(ctypes.byref(self.cy), ctypes.byref(self.p)):
+            raise ValueError("the cylindrical grid needs an even number of theta cells")
+        self.p.fdtd = {"yee": 0, "lehe": 1}[fdtd]
+        self.cy.second_half_new_cell = 1 if second_half_new_cell else 0
+
+    def update_b_half(self):
+        lib().orc_cyl_update_b_half(ctypes.byref(self.p), ctypes.byref(self.cy), ctypes.byref(self.f))
+
+    def update_e(self):
+        lib().orc_cyl_update_e(ctypes.byref(self.p), ctypes.byref(self.cy), ctypes.byref(self.f))
+
+    def add_current(self):
+        lib().orc_cyl_add_current(ctypes.byref(self.p), ctypes.byref(self.cy), ctypes.byref(self.f))
+
+    def field_halo(self, which):
+        names = {"E": ("Ex", "Ey", "Ez"), 0: ("Ex", "Ey", "Ez")}.get(which, ("Bx", "By", "Bz"))
+        lib().orc_cyl_field_halo(ctypes.byref(self.p), *[_ptr(self.F[n]) for n in names])
+
+    def current_halo_add(self):
+        lib().orc_cyl_current_halo_add(ctypes.byref(self.p), *[_ptr(self.F[n]) for n in ("Jx", "Jy", "Jz")])
+
+    def move_deposit(self):
+        rc = lib().orc_cyl_move_deposit(ctypes.byref(self.p), ctypes.byref(self.cy), self.ws, ctypes.byref(self.f),
+                                        ctypes.byref(self.s))
+        if rc:
+            raise RuntimeError("cylindrical push failed: a particle reached the outer wall or the axis sub-steps diverged")
+        return self.cy.substeps
+
+    def exchange(self):
+        lib().orc_cyl_exchange_particles(ctypes.byref(self.p), ctypes.byref(self.s))
+
+    def step(self, t=0):
+        rc = lib().orc_cyl_step(ctypes.byref(self.p), ctypes.byref(self.cy), self.ws, ctypes.byref(self.f),
+                                ctypes.byref(self.s))
+        if rc:
+            raise RuntimeError(f"cylindrical oracle step failed ({rc})")
+
+    def run(self, t0, nsteps):
+        for t in range(t0, t0 + nsteps):
+            self.step(t)
+
+    def radius(self, i):
+        """r of grid index i (1-based, the reference's i)"""
+        return (i - 3.0) + self.cy.rmin

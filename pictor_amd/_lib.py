@@ -20,6 +20,8 @@ FORM_FP32_SPLIT = 1
 
 FDTD_YEE = 0
 FDTD_LEHE = 1
+GEOM_CARTESIAN = 0
+GEOM_CYL = 1
 
 # every entry point of include/pictor_hip.h
 EXPORTS = (
@@ -39,6 +41,7 @@ class PictorConfig(ctypes.Structure):
     _fields_ = [
         ("ndim", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
         ("epc", ctypes.c_int32), ("form", ctypes.c_int32), ("tile", ctypes.c_int32), ("fdtd", ctypes.c_int32),
+        ("geometry", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("c", ctypes.c_double), ("c_ompe", ctypes.c_double), ("g0", ctypes.c_double), ("mi_me", ctypes.c_double),
         ("capacity_factor", ctypes.c_double),
     ]

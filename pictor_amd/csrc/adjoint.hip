@@ -102,11 +102,26 @@ __global__ __launch_bounds__(256) void k_eng_push(AdjP ap, const typename V4<R>:
     corners<R>(nE, nB, o, mx, gc);
     PState<R> q;
     push_fwd<R>(ap, s, in.ox[p], in.oy[p], in.oz[p], in.u[p], in.v[p], in.w[p], gc, q);
-    double a[8], b[8];
-    seg_values<R>(q.F1, q.W1, a);
-    seg_values<R>(q.F2, q.W2, b);
     int di = q.d[0], dj = q.d[1];
-    const bool cross = (di != 0) || (dj != 0);
+    // stored offsets; a position rounded onto the upper face belongs to the next cell
+    R ox = q.x1[0] - (R)di;
+    if (ox >= (R)1) { ox = (R)0; di += 1; }
+    R oy = q.x1[1] - (R)dj;
+    if (oy >= (R)1) { oy = (R)0; dj += 1; }
+    R oz = q.x1[2] - (R)q.d[2];
+    if (oz >= (R)1) oz = (R)0;
+    double a[8], b[8];
+    int ei = di, ej = dj;   // cell of the second half-segment
+    if constexpr (std::is_same<R, float>::value) {
+        // the split form deposits as the production kernel: FP64 to the stored position
+        pushdev::zigzag2d_split(ap.qs[s], in.ox[p], in.oy[p], in.oz[p], ox, oy, q.x1[2], di, dj, a, b);
+    } else {
+        seg_values<R>(q.F1, q.W1, a);
+        seg_values<R>(q.F2, q.W2, b);
+        ei = q.d[0];
+        ej = q.d[1];
+    }
+    const bool cross = (ei != 0) || (ej != 0);
     if (!cross) {
 #pragma unroll
         for (int k = 0; k < 8; k++) a[k] += b[k];
@@ -116,18 +131,12 @@ __global__ __launch_bounds__(256) void k_eng_push(AdjP ap, const typename V4<R>:
     unsafeAtomicAdd(&Jz[o], a[4]); unsafeAtomicAdd(&Jz[o + 1], a[5]);
     unsafeAtomicAdd(&Jz[o + mx], a[6]); unsafeAtomicAdd(&Jz[o + mx + 1], a[7]);
     if (cross) {
-        const long long o2 = o + di + (long long)dj * mx;
+        const long long o2 = o + ei + (long long)ej * mx;
         unsafeAtomicAdd(&Jx[o2], b[0]); unsafeAtomicAdd(&Jx[o2 + mx], b[1]);
         unsafeAtomicAdd(&Jy[o2], b[2]); unsafeAtomicAdd(&Jy[o2 + 1], b[3]);
         unsafeAtomicAdd(&Jz[o2], b[4]); unsafeAtomicAdd(&Jz[o2 + 1], b[5]);
         unsafeAtomicAdd(&Jz[o2 + mx], b[6]); unsafeAtomicAdd(&Jz[o2 + mx + 1], b[7]);
     }
-    R ox = q.x1[0] - (R)di;
-    if (ox >= (R)1) { ox = (R)0; di += 1; }
-    R oy = q.x1[1] - (R)dj;
-    if (oy >= (R)1) { oy = (R)0; dj += 1; }
-    R oz = q.x1[2] - (R)q.d[2];
-    if (oz >= (R)1) oz = (R)0;
     int nci = ci + di, ncj = cj + dj;
     nci = nci < 0 ? nci + ap.nx : (nci >= ap.nx ? nci - ap.nx : nci);
     ncj = ncj < 0 ? ncj + ap.ny : (ncj >= ap.ny ? ncj - ap.ny : ncj);

@@ -412,12 +412,20 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
         return fail(PICTOR_ERR_ARG, "3-D nx, ny must be <= 2048 and nz <= 512");
     if (cfg->form != PICTOR_FORM_FP64 && cfg->form != PICTOR_FORM_FP32_SPLIT) return fail(PICTOR_ERR_ARG, "unknown form");
     if (cfg->fdtd != PICTOR_FDTD_YEE && cfg->fdtd != PICTOR_FDTD_LEHE) return fail(PICTOR_ERR_ARG, "unknown fdtd");
+    if (cfg->geometry != PICTOR_GEOM_CARTESIAN && cfg->geometry != PICTOR_GEOM_CYL)
+        return fail(PICTOR_ERR_ARG, "unknown geometry");
+    const bool cyl = cfg->geometry == PICTOR_GEOM_CYL;
+    if (cyl && cfg->form != PICTOR_FORM_FP64) return fail(PICTOR_ERR_ARG, "the cylindrical geometry runs in the FP64 form");
+    if (cyl && cfg->ny % 2) return fail(PICTOR_ERR_ARG, "the cylindrical grid needs an even number of theta cells (ny)");
     if (!(cfg->c > 0.0 && cfg->c < 1.0)) return fail(PICTOR_ERR_ARG, "Courant number c must lie in (0,1)");
     if (cfg->epc < 1 || cfg->c_ompe <= 0 || cfg->mi_me <= 0) return fail(PICTOR_ERR_ARG, "invalid epc, c_ompe or mi_me");
-    int T = cfg->tile ? cfg->tile : choose_tile(cfg);
-    if (cfg->ndim == 2 && T != 8 && T != 16 && T != 32) return fail(PICTOR_ERR_ARG, "2-D tile must be 8, 16 or 32 and divide nx, ny");
-    if (cfg->ndim == 3 && T != 4 && T != 8) return fail(PICTOR_ERR_ARG, "3-D tile must be 4 or 8 and divide nx, ny, nz");
-    if (cfg->nx % T || cfg->ny % T || (cfg->ndim == 3 && cfg->nz % T)) return fail(PICTOR_ERR_ARG, "tile must divide the grid");
+    // the cylindrical path keeps its particles unsorted: one tile over the grid
+    int T = cyl ? 1 : (cfg->tile ? cfg->tile : choose_tile(cfg));
+    if (!cyl) {
+        if (cfg->ndim == 2 && T != 8 && T != 16 && T != 32) return fail(PICTOR_ERR_ARG, "2-D tile must be 8, 16 or 32 and divide nx, ny");
+        if (cfg->ndim == 3 && T != 4 && T != 8) return fail(PICTOR_ERR_ARG, "3-D tile must be 4 or 8 and divide nx, ny, nz");
+        if (cfg->nx % T || cfg->ny % T || (cfg->ndim == 3 && cfg->nz % T)) return fail(PICTOR_ERR_ARG, "tile must divide the grid");
+    }
 
     pictor_ctx *c = new pictor_ctx();
     c->cfg = *cfg;
@@ -477,9 +485,10 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
         L[9] = dx; L[10] = dy; L[11] = dz;
     }
 
+    c->cyl = cyl;
     TileP &tp = c->tp;
-    tp.T = T;
-    tp.ntx = g.nx / T; tp.nty = g.ny / T; tp.ntz = (g.ndim == 3) ? g.nz / T : 1;
+    tp.T = cyl ? g.nx : T;
+    tp.ntx = cyl ? 1 : g.nx / T; tp.nty = cyl ? 1 : g.ny / T; tp.ntz = (g.ndim == 3 && !cyl) ? g.nz / T : 1;
     tp.ntiles = tp.ntx * tp.nty * tp.ntz;
     tp.ND = (g.ndim == 2) ? 9 : 27;
     tp.nsp = (c->form == PICTOR_FORM_FP64) ? 1 : ph.nflvr;
@@ -511,12 +520,17 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     for (int i = 0; i < 8; i++)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) return cleanup_fail(fail(PICTOR_ERR_HIP, "event creation failed"));
     // empty layout so that a step without particles is valid
-    std::vector<long long> zero((size_t)tp.ntiles * tp.nsp, 0);
-    compute_layout(c, zero);
-    int rc = upload_layout(c);
-    if (!rc) rc = ensure_buffer(c, 0, c->layout_cap);
-    if (!rc) rc = ensure_buffer(c, 1, c->layout_cap);
-    if (!rc) rc = ensure_spill(c, 65536);
+    int rc = PICTOR_OK;
+    if (cyl) {
+        rc = cyl_setup(c);
+    } else {
+        std::vector<long long> zero((size_t)tp.ntiles * tp.nsp, 0);
+        compute_layout(c, zero);
+        rc = upload_layout(c);
+        if (!rc) rc = ensure_buffer(c, 0, c->layout_cap);
+        if (!rc) rc = ensure_buffer(c, 1, c->layout_cap);
+        if (!rc) rc = ensure_spill(c, 65536);
+    }
     c->partial_len = std::max(1024, tp.ntiles) * 5;
     if (!rc && (dalloc(&c->d_partial, c->partial_len) != hipSuccess || dalloc(&c->d_rho, g.ncell) != hipSuccess))
         rc = fail(PICTOR_ERR_HIP, "diagnostic allocation failed");
@@ -538,6 +552,7 @@ int pictor_finalize(pictor_ctx *c)
     dfree(c->d_spill_count); dfree(c->d_err);
     for (int i = 0; i < 2; i++) { free_f64(c->pf[i]); free_split(c->ps[i]); }
     free_f64(c->spill_f); free_split(c->spill_s);
+    cyl_release(c);
     dfree(c->d_partial); dfree(c->d_rho);
     if (c->h_flags) hipHostFree(c->h_flags);
     for (int i = 0; i < 8; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
@@ -616,6 +631,7 @@ int pictor_upload_particles(pictor_ctx *c, int64_t n, const double *q, const dou
     if (n > 0 && (!q || !x || !y || !z || !u || !v || !w || !flv)) return fail(PICTOR_ERR_ARG, "null particle pointer");
     if (n >= 0x7fffffffLL) return fail(PICTOR_ERR_ARG, "more than 2^31-1 particles per device");
     c->pending_swap = false;
+    if (c->cyl) return cyl_upload_particles(c, n, q, x, y, z, u, v, w, flv);
     const TileP &tp = c->tp;
     const size_t ng = (size_t)tp.ntiles * tp.nsp;
     PartF64 raw{};
@@ -676,6 +692,7 @@ int pictor_particle_count(pictor_ctx *c, int64_t *n)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
+    if (c->cyl) { *n = c->np; return PICTOR_OK; }
     std::vector<long long> counts;
     rc = group_counts(c, c->cur, counts);
     if (rc) return rc;
@@ -691,6 +708,12 @@ int pictor_download_particles(pictor_ctx *c, int64_t cap, double *q, double *x, 
     int rc = check_ctx(c);
     if (rc) return rc;
     if (c->pending_swap) return fail(PICTOR_ERR_ARG, "download between move_deposit and exchange_particles");
+    if (c->cyl) {
+        long long no = 0;
+        rc = cyl_download_particles(c, cap, q, x, y, z, u, v, w, flv, tile, &no);
+        if (n_out) *n_out = no;
+        return rc;
+    }
     const TileP &tp = c->tp;
     const size_t ng = (size_t)tp.ntiles * tp.nsp;
     std::vector<long long> counts;
@@ -735,13 +758,18 @@ int pictor_download_particles(pictor_ctx *c, int64_t cap, double *q, double *x, 
 int pictor_update_b_half(pictor_ctx *c)
 {
     int rc = check_ctx(c);
-    return rc ? rc : do_b_half(c);
+    return rc ? rc : (c->cyl ? cyl_b_half(c) : do_b_half(c));
 }
 
 int pictor_move_deposit(pictor_ctx *c)
 {
     int rc = check_ctx(c);
-    return rc ? rc : do_push(c, false);
+    if (rc) return rc;
+    if (c->cyl) {
+        if (c->pending_swap) return fail(PICTOR_ERR_ARG, "pictor_move_deposit called twice without pictor_exchange_particles");
+        return cyl_push(c, false);
+    }
+    return do_push(c, false);
 }
 
 int pictor_exchange_particles(pictor_ctx *c)
@@ -749,25 +777,25 @@ int pictor_exchange_particles(pictor_ctx *c)
     int rc = check_ctx(c);
     if (rc) return rc;
     if (!c->pending_swap) return fail(PICTOR_ERR_ARG, "pictor_exchange_particles without a preceding pictor_move_deposit");
-    return commit_push(c);
+    return c->cyl ? cyl_exchange(c) : commit_push(c);
 }
 
 int pictor_update_e(pictor_ctx *c)
 {
     int rc = check_ctx(c);
-    return rc ? rc : do_e_phase(c, true, false);
+    return rc ? rc : (c->cyl ? cyl_e_phase(c, true, false) : do_e_phase(c, true, false));
 }
 
 int pictor_current_halo_add(pictor_ctx *c)
 {
     int rc = check_ctx(c);
-    return rc ? rc : do_fold(c);
+    return rc ? rc : (c->cyl ? cyl_current_halo(c) : do_fold(c));
 }
 
 int pictor_add_current(pictor_ctx *c)
 {
     int rc = check_ctx(c);
-    return rc ? rc : do_e_phase(c, false, true);
+    return rc ? rc : (c->cyl ? cyl_e_phase(c, false, true) : do_e_phase(c, false, true));
 }
 
 int pictor_field_halo(pictor_ctx *c, int32_t which)
@@ -775,6 +803,7 @@ int pictor_field_halo(pictor_ctx *c, int32_t which)
     int rc = check_ctx(c);
     if (rc) return rc;
     if (which != 0 && which != 1) return fail(PICTOR_ERR_ARG, "which must be 0 (E) or 1 (B)");
+    if (c->cyl) return cyl_field_halo(c, which);
     void **F = which == 0 ? c->E : c->B;
     if (c->form == PICTOR_FORM_FP64) {
         double *P[3] = {(double *)F[0], (double *)F[1], (double *)F[2]};
@@ -790,7 +819,7 @@ int pictor_field_halo(pictor_ctx *c, int32_t which)
 int pictor_reset_current(pictor_ctx *c)
 {
     int rc = check_ctx(c);
-    return rc ? rc : do_reset_current(c);
+    return rc ? rc : (c->cyl ? cyl_reset_current(c) : do_reset_current(c));
 }
 
 int pictor_sort_tiles(pictor_ctx *c)
@@ -798,6 +827,7 @@ int pictor_sort_tiles(pictor_ctx *c)
     int rc = check_ctx(c);
     if (rc) return rc;
     if (c->pending_swap) return fail(PICTOR_ERR_ARG, "sort between move_deposit and exchange_particles");
+    if (c->cyl) return PICTOR_OK;   // the cylindrical path keeps the caller's order
     return relayout(c, 0);
 }
 
@@ -806,6 +836,23 @@ int pictor_step(pictor_ctx *c, int32_t nsteps)
     int rc = check_ctx(c);
     if (rc) return rc;
     if (c->pending_swap) return fail(PICTOR_ERR_ARG, "pictor_step inside a split step");
+    if (c->cyl) {
+        // MAIN.F90:42-284 with the cylindrical routines
+        for (int s = 0; s < nsteps; s++) {
+            HIPTRY(hipEventRecord(c->ev[4], c->stream));
+            if ((rc = cyl_reset_current(c)) || (rc = cyl_b_half(c)) || (rc = cyl_push(c, true))) return rc;
+            if ((rc = cyl_exchange(c))) return rc;
+            if ((rc = cyl_b_half(c)) || (rc = cyl_e_phase(c, true, false)) || (rc = cyl_current_halo(c)) ||
+                (rc = cyl_e_phase(c, false, true)))
+                return rc;
+            HIPTRY(hipEventRecord(c->ev[5], c->stream));
+            HIPTRY(hipEventSynchronize(c->ev[5]));
+            float ms = 0, f2 = 0;
+            if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) { c->t_ms[0] += ms; c->t_n[0]++; }
+            if (hipEventElapsedTime(&f2, c->ev[4], c->ev[5]) == hipSuccess) { c->t_ms[2] += f2; c->t_n[2]++; }
+        }
+        return PICTOR_OK;
+    }
     for (int s = 0; s < nsteps; s++) {
         HIPTRY(hipEventRecord(c->ev[4], c->stream));
         if ((rc = do_reset_current(c))) return rc;
@@ -842,6 +889,7 @@ int pictor_diagnostics(pictor_ctx *c, double out[9])
     int rc = check_ctx(c);
     if (rc) return rc;
     if (c->pending_swap) return fail(PICTOR_ERR_ARG, "diagnostics between move_deposit and exchange_particles");
+    if (c->cyl) return fail(PICTOR_ERR_ARG, "pictor_diagnostics is available for the Cartesian geometry");
     const int nblk = 1024;
     std::vector<double> part((size_t)std::max(nblk * 2, c->tp.ntiles * 5));
     // field energies

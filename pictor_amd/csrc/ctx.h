@@ -6,6 +6,7 @@
 
 #include "../../include/pictor_hip.h"
 #include "pictor_common.h"
+#include "cyl.h"
 
 namespace pictor {
 
@@ -94,6 +95,13 @@ struct pictor_ctx {
     long long t_n[3] = {0, 0, 0};
 
     long long np = 0;
+
+    // cylindrical geometry: particles in the caller's order (cyl.hip)
+    bool cyl = false;
+    pictor::CylP cp{};
+    PartF64 pcyl{};
+    int *d_cerr = nullptr;
+    unsigned long long *d_nsub = nullptr;
 
     pictor::AdjEngineBase *adj = nullptr;   // created by pictor_adjoint_forward
     int adj_engine = 0;                      // 0 tile-sorted (FP32 split form), 1 unsorted

@@ -404,30 +404,17 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                 const float z1 = z0 + wn * gg * idz_;
                 int di = (int)floorf(x1), dj = (int)floorf(y1);
                 const int dk = (int)floorf(z1);
-                const float xr = fminf((float)(min(0, di) + 1), fmaxf((float)max(0, di), 0.5f * (x0 + x1)));
-                const float yr = fminf((float)(min(0, dj) + 1), fmaxf((float)max(0, dj), 0.5f * (y0 + y1)));
-                const float zr = fminf((float)(min(0, dk) + 1), fmaxf((float)max(0, dk), 0.5f * (z0 + z1)));
-                // the two zigzag half-segments: 8 node values each, as exact
-                // integer counts of 2^-36 units
+                // stored offsets; a position rounded onto the upper face belongs to the next cell
+                ox = x1 - (float)di;
+                if (ox >= 1.0f) { ox = 0.0f; di += 1; }
+                oy = y1 - (float)dj;
+                if (oy >= 1.0f) { oy = 0.0f; dj += 1; }
+                oz = z1 - (float)dk;
+                if (oz >= 1.0f) oz = 0.0f;
+                // the two zigzag half-segments to the stored position: 8 node values
+                // each, as exact integer counts of 2^-36 units
                 double a[8], b[8];
-                {
-                    const float Fx = qs * (xr - x0), Fy = qs * (yr - y0), Fz = qs * (zr - z0);
-                    const float Wx = 0.5f * (x0 + xr), Wy = 0.5f * (y0 + yr);
-                    const float zl = Fz * (1.0f - Wx), zh = Fz * Wx;
-                    a[0] = quant(Fx * (1.0f - Wy)); a[1] = quant(Fx * Wy);
-                    a[2] = quant(Fy * (1.0f - Wx)); a[3] = quant(Fy * Wx);
-                    a[4] = quant(zl * (1.0f - Wy)); a[5] = quant(zh * (1.0f - Wy));
-                    a[6] = quant(zl * Wy); a[7] = quant(zh * Wy);
-                }
-                {
-                    const float Fx = qs * (x1 - xr), Fy = qs * (y1 - yr), Fz = qs * (z1 - zr);
-                    const float Wx = 0.5f * (x1 + xr) - (float)di, Wy = 0.5f * (y1 + yr) - (float)dj;
-                    const float zl = Fz * (1.0f - Wx), zh = Fz * Wx;
-                    b[0] = quant(Fx * (1.0f - Wy)); b[1] = quant(Fx * Wy);
-                    b[2] = quant(Fy * (1.0f - Wx)); b[3] = quant(Fy * Wx);
-                    b[4] = quant(zl * (1.0f - Wy)); b[5] = quant(zh * (1.0f - Wy));
-                    b[6] = quant(zl * Wy); b[7] = quant(zh * Wy);
-                }
+                zigzag2d_split((double)qs, x0, y0, z0, ox, oy, z1, di, dj, a, b);
                 // a particle that stays in its cell deposits both halves on the same
                 // 8 nodes: add them first (exact) and halve the LDS atomics
                 const bool cross = (di != 0) || (dj != 0);
@@ -457,12 +444,6 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                     atomicAdd(jp + 2 * WW + W, b[6]);
                     atomicAdd(jp + 2 * WW + W + 1, b[7]);
                 }
-                ox = x1 - (float)di;
-                if (ox >= 1.0f) { ox = 0.0f; di += 1; }
-                oy = y1 - (float)dj;
-                if (oy >= 1.0f) { oy = 0.0f; dj += 1; }
-                oz = z1 - (float)dk;
-                if (oz >= 1.0f) oz = 0.0f;
                 int nci = ci + di, ncj = cj + dj;
                 nci = nci < 0 ? nci + nx : (nci >= nx ? nci - nx : nci);
                 ncj = ncj < 0 ? ncj + ny : (ncj >= ny ? ncj - ny : ncj);

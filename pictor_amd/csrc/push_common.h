@@ -49,6 +49,44 @@ __device__ inline int norm_dir(int dd, int n)
 // FP32 -> exact integer count of 2^-36 units (the argument is already scaled by 2^36)
 __device__ inline double quant(float v) { return (double)__builtin_rintf(v); }
 
+// The two zigzag half-segments of the 2-D split form (movdep.F90:177-237)
+// from the old offsets (x0, y0, z0) to the new position as stored: offsets
+// (ox, oy) in the cell displaced by (di, dj), z advanced to z1.  Evaluated in
+// FP64, every relay point, flux F and weight W is exact for FP32 inputs and
+// each node value is rounded once before its rounding to a 2^-36 count, so
+// the deposited Jx, Jy carry exactly the charge moved between the stored
+// positions up to that last rounding.  qs: charge times 2^36.  a: the first
+// half-segment (nodes of the old cell), b: the second (nodes of the new cell),
+// each Jx (i,j), (i,j+1); Jy (i,j), (i+1,j); Jz (i,j), (i+1,j), (i,j+1), (i+1,j+1).
+__device__ inline void zigzag2d_split(double qs, float x0, float y0, float z0, float ox, float oy, float z1, int di,
+                                      int dj, double a[8], double b[8])
+{
+    const double X0 = x0, Y0 = y0, Z0 = z0;
+    const double X1 = (double)ox + (double)di, Y1 = (double)oy + (double)dj, Z1 = z1;
+    const int dk = (int)floor(Z1);
+    const double xr = dmin((double)(min(0, di) + 1), dmax((double)max(0, di), 0.5 * (X0 + X1)));
+    const double yr = dmin((double)(min(0, dj) + 1), dmax((double)max(0, dj), 0.5 * (Y0 + Y1)));
+    const double zr = dmin((double)(min(0, dk) + 1), dmax((double)max(0, dk), 0.5 * (Z0 + Z1)));
+    {
+        const double Fx = qs * (xr - X0), Fy = qs * (yr - Y0), Fz = qs * (zr - Z0);
+        const double Wx = 0.5 * (X0 + xr), Wy = 0.5 * (Y0 + yr);
+        const double zl = Fz * (1.0 - Wx), zh = Fz * Wx;
+        a[0] = rint(Fx * (1.0 - Wy)); a[1] = rint(Fx * Wy);
+        a[2] = rint(Fy * (1.0 - Wx)); a[3] = rint(Fy * Wx);
+        a[4] = rint(zl * (1.0 - Wy)); a[5] = rint(zh * (1.0 - Wy));
+        a[6] = rint(zl * Wy); a[7] = rint(zh * Wy);
+    }
+    {
+        const double Fx = qs * (X1 - xr), Fy = qs * (Y1 - yr), Fz = qs * (Z1 - zr);
+        const double Wx = 0.5 * (X1 + xr) - (double)di, Wy = 0.5 * (Y1 + yr) - (double)dj;
+        const double zl = Fz * (1.0 - Wx), zh = Fz * Wx;
+        b[0] = rint(Fx * (1.0 - Wy)); b[1] = rint(Fx * Wy);
+        b[2] = rint(Fy * (1.0 - Wx)); b[3] = rint(Fy * Wx);
+        b[4] = rint(zl * (1.0 - Wy)); b[5] = rint(zh * (1.0 - Wy));
+        b[6] = rint(zl * Wy); b[7] = rint(zh * Wy);
+    }
+}
+
 // byte-offset accessors: a uniform base plus a 32-bit lane offset lets the
 // compiler use SGPR-base addressing (one VALU op per access)
 template <typename T>

@@ -12,9 +12,13 @@ module pictor_cabi
    integer(c_int), parameter :: PICTOR_ERR_CAPACITY = 12
    integer(c_int32_t), parameter :: PICTOR_FORM_FP64 = 0
    integer(c_int32_t), parameter :: PICTOR_FORM_FP32_SPLIT = 1
+   integer(c_int32_t), parameter :: PICTOR_FDTD_YEE = 0
+   integer(c_int32_t), parameter :: PICTOR_FDTD_LEHE = 1
+   integer(c_int32_t), parameter :: PICTOR_GEOM_CARTESIAN = 0
+   integer(c_int32_t), parameter :: PICTOR_GEOM_CYL = 1
 
    type, bind(C) :: pictor_config
-      integer(c_int32_t) :: ndim, nx, ny, nz, epc, form, tile, fdtd
+      integer(c_int32_t) :: ndim, nx, ny, nz, epc, form, tile, fdtd, geometry, reserved
       real(c_double) :: c, c_ompe, g0, mi_me, capacity_factor
    end type pictor_config
 

@@ -12,7 +12,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import FDTD_LEHE, FDTD_YEE, FORM_FP32_SPLIT, FORM_FP64, PictorConfig, check, lib
+from ._lib import (FDTD_LEHE, FDTD_YEE, FORM_FP32_SPLIT, FORM_FP64, GEOM_CARTESIAN, GEOM_CYL, PictorConfig, check,
+                   lib)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -33,11 +34,15 @@ class Simulation:
     """
 
     def __init__(self, nx, ny, nz=1, ndim=2, epc=16, c=0.45, c_ompe=10.0, g0=1.0, mi_me=1.0,
-                 form="fp64", tile=0, capacity_factor=0.0, fdtd="yee"):
+                 form="fp64", tile=0, capacity_factor=0.0, fdtd="yee", geometry="cartesian"):
         if fdtd not in ("yee", "lehe"):
             raise ValueError("fdtd must be 'yee' or 'lehe'")
+        if geometry not in ("cartesian", "cyl"):
+            raise ValueError("geometry must be 'cartesian' or 'cyl'")
+        self.geometry = geometry
         self.cfg = PictorConfig(ndim, nx, ny, nz, epc, FORM_FP64 if form == "fp64" else FORM_FP32_SPLIT,
-                                tile, FDTD_LEHE if fdtd == "lehe" else FDTD_YEE, c, c_ompe, g0, mi_me,
+                                tile, FDTD_LEHE if fdtd == "lehe" else FDTD_YEE,
+                                GEOM_CYL if geometry == "cyl" else GEOM_CARTESIAN, 0, c, c_ompe, g0, mi_me,
                                 capacity_factor)
         if form not in ("fp64", "fp32_split"):
             raise ValueError("form must be 'fp64' or 'fp32_split'")
