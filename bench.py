@@ -305,6 +305,86 @@ def adjoint_report(args):
     return out
 
 
+CYL_CONFIGS = {
+    # name: (ndim, nr, ntheta, nz, particles per cell per species, c) -- particles uniform in index
+    # space.  The cells next to the axis are 0.5 dtheta wide in theta, so the reference's scheme
+    # needs c < 1/sqrt(1 + (ntheta/pi)^2 (+1)): 8 theta cells and c = 0.3.
+    "cyl2d_4096": (2, 4096, 8, 1, 128, 0.3),
+    "cyl3d_c4": (3, 512, 8, 4096, 4, 0.3),
+}
+
+
+def cyl_report(args):
+    """The cylindrical geometry (the reference's -Dcyl build with the axis):
+    FP64 steps of an electron-positron plasma filling the grid uniformly in
+    index space (particles per cell constant in r), momenta Gaussian 0.1c.
+    value = particles * steps / wall time; roofline of k_cyl_push with the
+    FP64 record (108 B per particle); cpu_baseline: the cylindrical oracle
+    (one thread) on a bounded sample."""
+    import numpy as np
+
+    from oracle.pyoracle import CylOracle
+    from pictor_amd import Simulation
+
+    ndim, nr, nth, nz, epc, c = CYL_CONFIGS[args.cyl]
+    rng = np.random.default_rng(12345)
+    ncell = nr * nth * (nz if ndim == 3 else 1)
+    n = 2 * epc * ncell
+    rmax = nr - 6.0
+    half = n // 2
+    x = 3.5 + rng.uniform(0.0, rmax, half)
+    y = 3.0 + rng.uniform(0.0, nth, half)
+    z = rng.uniform(1.0, 2.0, half) if ndim == 2 else rng.uniform(3.0, nz + 3.0, half)
+    P = {"x": np.repeat(x, 2), "y": np.repeat(y, 2), "z": np.repeat(z, 2)}
+    P["u"], P["v"], P["w"] = rng.normal(0.0, 0.1, (3, 2 * half))
+    P["flv"] = np.tile(np.array([1, 2], dtype=np.int32), half)
+    P["q"] = np.where(P["flv"] == 1, 1.0, -1.0)
+    npart = len(P["x"])
+    print(f"# cyl {args.cyl}: {npart} particles", file=sys.stderr, flush=True)
+    with Simulation(nr, nth, nz, ndim=ndim, c=c, epc=epc, form="fp64", geometry="cyl") as sim:
+        sim.upload_particles(P["q"], P["x"], P["y"], P["z"], P["u"], P["v"], P["w"], P["flv"])
+        sim.step(args.warmup)
+        sim.synchronize()
+        sim.timer_reset()
+        t0 = time.time()
+        sim.step(args.steps)
+        sim.synchronize()
+        wall = time.time() - t0
+        push_ms, npush = sim.timer("push")
+    push_ms /= max(npush, 1)
+    value = npart * args.steps / wall
+    mx, my, mz = nr + 5, nth + 5, (nz + 5 if ndim == 3 else 1)
+    algo = npart * BYTES_PER_PARTICLE["fp64"] + mx * my * mz * GRID_BYTES["fp64"]
+    achieved = algo / (push_ms * 1e-3)
+    cpu = None
+    if args.cpu_steps != 0:
+        # a bounded sample: the first particles of the same plasma on the same grid
+        m = min(npart, 2_000_000)
+        o = CylOracle(ndim, nr, nth, nz, c=c, epc=epc, cap=m + 16)
+        o.load_particles(*(P[k][:m] for k in ("q", "x", "y", "z", "u", "v", "w", "flv")))
+        o.step(0)
+        steps = 10 if ndim == 2 else 4
+        t0 = time.time()
+        o.run(1, steps)
+        dt = time.time() - t0
+        cpu = {"value": m * steps / dt, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{steps} steps of the first {m} particles of the same plasma on the same grid "
+                         f"(cylindrical FP64 oracle, one thread) after 1 warm-up step, {dt:.1f}s"}
+    return {
+        "metric": "particle-updates/s/GPU, cylindrical (r, theta, z) FP64 step",
+        "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.cyl}: {ndim}-D cylindrical grid nr={nr} ntheta={nth}"
+                               + (f" nz={nz}" if ndim == 3 else "") + f", {epc} ppc/species, {npart} particles, "
+                               f"momenta N(0, 0.1) per component, c={c}, axis included, conducting outer wall",
+                   "form": "fp64", "particles": npart},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_cyl_push", "push_ms": push_ms},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,6 +409,8 @@ def main():
     ap.add_argument("--adjoint-engine", default="sorted", choices=("sorted", "unsorted"))
     ap.add_argument("--adjoint-forms", default="fp32_split,fp64")
     ap.add_argument("--adjoint-fd", type=int, default=1, help="finite-difference check of the drift gradient")
+    ap.add_argument("--cyl", default=None, choices=sorted(CYL_CONFIGS),
+                    help="instead of the Weibel step: time the cylindrical geometry on this grid")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -338,6 +420,9 @@ def main():
 
     if args.adjoint:
         print(json.dumps(adjoint_report(args)), flush=True)
+        return
+    if args.cyl:
+        print(json.dumps(cyl_report(args)), flush=True)
         return
 
     import numpy as np  # noqa: F401

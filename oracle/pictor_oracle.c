@@ -1602,9 +1602,10 @@ int orc_cyl_move_deposit(const orc_params *p, orc_cyl *cy, orc_ws *w, orc_fields
             const int64_t m = off + n;
             if (s->x[m] >= mx - 3) { rc = ORC_ERR_ARG; break; }   /* the outer wall region */
             const int ip = (int)s->x[m], jp = (int)s->y[m];
-            /* near the axis one step can cross several theta cells; the deposit
-             * window of the arrays ends at rows 1 and my - 1 */
-            if (jp < 1 || jp > my - 1) { rc = ORC_ERR_ARG; break; }
+            /* near the axis one step can cross several theta cells (up to ny/4):
+             * the second half-segment's cell row is folded by ny into the arrays,
+             * a periodic image of the same nodes */
+            const int jq = (jp < 1) ? jp + p->ny : (jp > my - 1 ? jp - p->ny : jp);
             const int kp = two ? 1 : (int)s->z[m];
             const double xr = dmin((double)(imin2(ip0[n], ip) + 1), dmax((double)imax2(ip0[n], ip), 0.5 * (x0[n] + s->x[m])));
             const double yr = dmin((double)(imin2(jp0[n], jp) + 1), dmax((double)imax2(jp0[n], jp), 0.5 * (y0[n] + s->y[m])));
@@ -1628,7 +1629,7 @@ int orc_cyl_move_deposit(const orc_params *p, orc_cyl *cy, orc_ws *w, orc_fields
             const double Wy2 = 0.5 * (s->y[m] + yr) - jp;
             const double Wz2 = two ? 0.0 : 0.5 * (s->z[m] + zr) - kp;
             if (cy->second_half_new_cell)
-                sidx = two ? (int64_t)(jp - 1) * mx + ip : (int64_t)(kp - 1) * mx * my + (int64_t)(jp - 1) * mx + ip;
+                sidx = two ? (int64_t)(jq - 1) * mx + ip : (int64_t)(kp - 1) * mx * my + (int64_t)(jq - 1) * mx + ip;
             vj = VJ + nj * (sidx - 1);
             for (int nn = 0; nn < nj; nn++)
                 vj[nn] = vj[nn] + (wtFx[nn] * Fx2 + wtFy[nn] * Fy2 + wtFz[nn] * Fz2) * (Jwtx1[nn] + Jwtx2[nn] * Wx2)

@@ -54,8 +54,30 @@ __device__ inline void vbin_of(int v, int nsp, int &s, int &d)
     d = q < 4 ? q : q + 1;
 }
 
+// one particle of the reverse sweep: its state before the step, the slot it
+// went to, and the adjoint of its state after the step (read at that slot)
+struct AdjRec {
+    unsigned slot;
+    int s, to, cell;
+    float ox, oy, oz, u, v, w;
+    float a[6];
+};
+
+__device__ inline void load_rec(const PartSplit &st, const EPart<float> &ain, AdjRec &r)
+{
+    const unsigned sl = r.slot;
+    r.cell = st.cell[sl];
+    r.ox = st.ox[sl]; r.oy = st.oy[sl]; r.oz = st.oz[sl];
+    r.u = st.u[sl]; r.v = st.v[sl]; r.w = st.w[sl];
+    if (r.to >= 0) {
+        const int t = r.to;
+        r.a[0] = ain.ox[t]; r.a[1] = ain.oy[t]; r.a[2] = ain.oz[t];
+        r.a[3] = ain.u[t]; r.a[4] = ain.v[t]; r.a[5] = ain.w[t];
+    }
+}
+
 template <int T, int BS>
-__global__ __launch_bounds__(BS) void k_fa_adjoint(GridP g, TileP tp, AdjP ap, const float *__restrict__ Ex,
+__global__ __launch_bounds__(BS, 1024 / BS) void k_fa_adjoint(GridP g, TileP tp, AdjP ap, const float *__restrict__ Ex,
                                                    const float *__restrict__ Ey, const float *__restrict__ Ez,
                                                    const float *__restrict__ Bx, const float *__restrict__ By,
                                                    const float *__restrict__ Bz, const double *__restrict__ Ebx,
@@ -108,30 +130,58 @@ __global__ __launch_bounds__(BS) void k_fa_adjoint(GridP g, TileP tp, AdjP ap, c
     for (int e = threadIdx.x; e < 6 * NN; e += BS) sG[e] = 0.0f;
     __syncthreads();
     const int total = sStart[NV];
-    for (int k = (int)threadIdx.x; k < total; k += BS) {
+    // software pipeline: the next particle's state and incoming adjoint are
+    // loaded while the current one is processed; its output slot (which the
+    // incoming adjoint's address depends on) was loaded one iteration earlier
+    auto slot_of = [&](int k, unsigned &slot, int &s) {
         int v = 0;
         while (k >= sStart[v + 1]) v++;
-        const unsigned slot = sSrc[v] + (unsigned)(k - sStart[v]);
-        const int s = sVs[v];
-        const int cell = st.cell[slot];
-        const int ci = cell & 0xFFFF, cj = cell >> 16;
+        slot = sSrc[v] + (unsigned)(k - sStart[v]);
+        s = sVs[v];
+    };
+    AdjRec nxt{};
+    int to_ahead = -1;   // output slot of particle k + 2 BS
+    if ((int)threadIdx.x < total) {
+        slot_of(threadIdx.x, nxt.slot, nxt.s);
+        nxt.to = dst[nxt.slot];
+        load_rec(st, ain, nxt);
+    }
+    if ((int)threadIdx.x + BS < total) {
+        unsigned sl;
+        int ss;
+        slot_of(threadIdx.x + BS, sl, ss);
+        to_ahead = dst[sl];
+    }
+    for (int k = (int)threadIdx.x; k < total; k += BS) {
+        const AdjRec cur = nxt;
+        if (k + BS < total) {
+            slot_of(k + BS, nxt.slot, nxt.s);
+            nxt.to = to_ahead;
+            load_rec(st, ain, nxt);
+        }
+        if (k + 2 * BS < total) {
+            unsigned sl;
+            int ss;
+            slot_of(k + 2 * BS, sl, ss);
+            to_ahead = dst[sl];
+        }
+        const int ci = cur.cell & 0xFFFF, cj = cur.cell >> 16;
         const int ii0 = ci - cx0, jj0 = cj - cy0;
-        const int to = dst[slot];
-        if (ii0 < 2 || ii0 > T + 1 || jj0 < 2 || jj0 > T + 1 || to < 0) continue;   // excluded by the forward checks
+        if (ii0 < 2 || ii0 > T + 1 || jj0 < 2 || jj0 > T + 1 || cur.to < 0) continue;   // excluded by the forward checks
         const int lc = (jj0 - 2) * T + (ii0 - 2);
         const float4 gc[6] = {sC[6 * lc], sC[6 * lc + 1], sC[6 * lc + 2], sC[6 * lc + 3], sC[6 * lc + 4], sC[6 * lc + 5]};
         PState<float> q;
-        push_fwd<float>(ap, s, st.ox[slot], st.oy[slot], st.oz[slot], st.u[slot], st.v[slot], st.w[slot], gc, q);
+        push_fwd<float>(ap, cur.s, cur.ox, cur.oy, cur.oz, cur.u, cur.v, cur.w, gc, q);
         const int e0 = jj0 * W + ii0;
         const int e2 = e0 + q.d[1] * W + q.d[0];
         const float jb1[8] = {sJb[e0], sJb[e0 + W], sJb[WW + e0], sJb[WW + e0 + 1],
                               sJb[2 * WW + e0], sJb[2 * WW + e0 + 1], sJb[2 * WW + e0 + W], sJb[2 * WW + e0 + W + 1]};
         const float jb2[8] = {sJb[e2], sJb[e2 + W], sJb[WW + e2], sJb[WW + e2 + 1],
                               sJb[2 * WW + e2], sJb[2 * WW + e2 + 1], sJb[2 * WW + e2 + W], sJb[2 * WW + e2 + W + 1]};
-        const float x1b[3] = {ain.ox[to], ain.oy[to], ain.oz[to]};
-        const float u1b[3] = {ain.u[to], ain.v[to], ain.w[to]};
+        const float x1b[3] = {cur.a[0], cur.a[1], cur.a[2]};
+        const float u1b[3] = {cur.a[3], cur.a[4], cur.a[5]};
         float x0b[3], unb[3], PEb[3], PBb[3];
-        reverse<float>(ap, s, q, gc, jb1, jb2, x1b, u1b, x0b, unb, PEb, PBb);
+        reverse<float>(ap, cur.s, q, gc, jb1, jb2, x1b, u1b, x0b, unb, PEb, PBb);
         const int n0 = (jj0 - 2) * N1 + (ii0 - 2);
         const int nc[4] = {n0, n0 + 1, n0 + N1, n0 + N1 + 1};
 #pragma unroll
@@ -144,6 +194,7 @@ __global__ __launch_bounds__(BS) void k_fa_adjoint(GridP g, TileP tp, AdjP ap, c
             atomicAdd(&sG[4 * NN + nc[m]], w * PBb[1]);
             atomicAdd(&sG[5 * NN + nc[m]], w * PBb[2]);
         }
+        const unsigned slot = cur.slot;
         aout.ox[slot] = x0b[0]; aout.oy[slot] = x0b[1]; aout.oz[slot] = x0b[2];
         aout.u[slot] = unb[0]; aout.v[slot] = unb[1]; aout.w[slot] = unb[2];
     }

@@ -417,6 +417,16 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     const bool cyl = cfg->geometry == PICTOR_GEOM_CYL;
     if (cyl && cfg->form != PICTOR_FORM_FP64) return fail(PICTOR_ERR_ARG, "the cylindrical geometry runs in the FP64 form");
     if (cyl && cfg->ny % 2) return fail(PICTOR_ERR_ARG, "the cylindrical grid needs an even number of theta cells (ny)");
+    if (cyl) {
+        // the narrowest cell of the r-theta grid is the arc 0.5 * dtheta at r = 1/2 (grid index 4):
+        // the FDTD step is stable for c < 1 / sqrt(1 + 1 / (0.5 dtheta)^2 (+ 1 in 3-D))
+        const double arc = 0.5 * (2.0 * 3.14159265358979323846264 / cfg->ny);
+        const double lim = 1.0 / std::sqrt(1.0 + 1.0 / (arc * arc) + (cfg->ndim == 3 ? 1.0 : 0.0));
+        if (cfg->c >= lim)
+            return fail(PICTOR_ERR_ARG, "cylindrical grid: c must be below the Courant limit of the cells next to the "
+                                        "axis, 1/sqrt(1 + (ny/pi)^2" + std::string(cfg->ndim == 3 ? " + 1" : "") +
+                                        ") = " + std::to_string(lim));
+    }
     if (!(cfg->c > 0.0 && cfg->c < 1.0)) return fail(PICTOR_ERR_ARG, "Courant number c must lie in (0,1)");
     if (cfg->epc < 1 || cfg->c_ompe <= 0 || cfg->mi_me <= 0) return fail(PICTOR_ERR_ARG, "invalid epc, c_ompe or mi_me");
     // the cylindrical path keeps its particles unsorted: one tile over the grid

@@ -18,7 +18,7 @@ struct CylP {
     int substep_limit;
 };
 
-enum CylErr : int { kCylWall = 1, kCylSubsteps = 2, kCylTheta = 4 };
+enum CylErr : int { kCylWall = 1, kCylSubsteps = 2 };
 
 void cyl_params(const GridP &g, const PhysP &ph, int fdtd, CylP &cp);
 

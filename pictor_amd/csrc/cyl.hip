@@ -562,14 +562,14 @@ __global__ __launch_bounds__(256) void k_cyl_push(GridP g, PhysP ph, CylP cp, co
         const double xr = fmin((double)(min(ip0, ip) + 1), fmax((double)max(ip0, ip), 0.5 * (x0 + xn)));
         const double yr = fmin((double)(min(jp0, jp) + 1), fmax((double)max(jp0, jp), 0.5 * (y0 + yn)));
         const double zr = fmin((double)(min(kp0, kp) + 1), fmax((double)max(kp0, kp), 0.5 * (z0 + zn)));
-        // near the axis one step can cross several theta cells: the deposit
-        // window of the arrays ends at rows 1 and my - 1
-        const bool jok = jp >= 1 && jp <= g.my - 1;
-        if (!jok) atomicOr(err, (int)kCylTheta);
-        if (xn < g.mx - 3 && jok) {
+        // near the axis one step can cross several theta cells (up to ny/4): the
+        // second half-segment's cell row is folded by ny into the arrays, a
+        // periodic image of the same nodes
+        const int jq = jp < 1 ? jp + g.ny : (jp > g.my - 1 ? jp - g.ny : jp);
+        if (xn < g.mx - 3) {
             vecj_half<THREE>(g, J, ip0, jp0, kp0, qthis * (xr - x0), qthis * (yr - y0), qthis * (zr - z0),
                              0.5 * (x0 + xr) - ip0, 0.5 * (y0 + yr) - jp0, THREE ? 0.5 * (z0 + zr) - kp0 : 0.0);
-            vecj_half<THREE>(g, J, ip, jp, kp, qthis * (xn - xr), qthis * (yn - yr), qthis * (zn - zr),
+            vecj_half<THREE>(g, J, ip, jq, kp, qthis * (xn - xr), qthis * (yn - yr), qthis * (zn - zr),
                              0.5 * (xn + xr) - ip, 0.5 * (yn + yr) - jp, THREE ? 0.5 * (zn + zr) - kp : 0.0);
         }
     }
@@ -795,7 +795,6 @@ int cyl_exchange(pictor_ctx *c)
     c->pending_swap = false;
     if (err & kCylWall) return fail(PICTOR_ERR_BOUNDS, "a particle reached the outer wall region (x >= mx - 3)");
     if (err & kCylSubsteps) return fail(PICTOR_ERR_BOUNDS, "the axis sub-steps of a particle did not finish");
-    if (err & kCylTheta) return fail(PICTOR_ERR_BOUNDS, "a particle crossed more theta cells in one step than the ghost layers hold");
     return PICTOR_OK;
 }
 

@@ -11,7 +11,10 @@ Bars:
   * 20 full steps: fields and particles within 1e-9 relative;
   * the GPU run keeps the discrete Gauss residual r dtheta div E - rho
     constant node by node away from the axis (a size-independent property,
-    checked on a larger grid than the oracle comparison).
+    checked on a larger grid than the oracle comparison);
+  * a Courant number at or above the limit of the cells next to the axis
+    (0.5 dtheta wide in theta), where the reference's scheme diverges, is
+    rejected at pictor_init.
 """
 import numpy as np
 import pytest
@@ -26,10 +29,14 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
 
 
+# c below the Courant limit of the cells next to the axis (0.193 for 16 theta cells in 2-D, 0.189 in 3-D)
+C = 0.15
+
+
 def _pair(ndim, nr, nth, nz=1, fdtd="yee", seed=1, n=0, rmax=None, epc=4):
     say(f"cyl ndim={ndim} nr={nr} ntheta={nth} nz={nz} fdtd={fdtd} seed={seed} n={n}")
-    o = CylOracle(ndim, nr, nth, nz, epc=epc, fdtd=fdtd, cap=max(n, 1) + 16)
-    sim = Simulation(nr, nth, nz, ndim=ndim, epc=epc, form="fp64", fdtd=fdtd, geometry="cyl")
+    o = CylOracle(ndim, nr, nth, nz, c=C, epc=epc, fdtd=fdtd, cap=max(n, 1) + 16)
+    sim = Simulation(nr, nth, nz, ndim=ndim, c=C, epc=epc, form="fp64", fdtd=fdtd, geometry="cyl")
     rng = np.random.default_rng(seed)
     P = None
     if n:
@@ -79,7 +86,7 @@ def _push_once(o, sim):
 
 @pytest.mark.parametrize("ndim", [2, 3])
 def test_cyl_push_matches_oracle(ndim):
-    o, sim, rng, P = _pair(ndim, 24, 32, 12 if ndim == 3 else 1, n=6000, rmax=16.0)
+    o, sim, rng, P = _pair(ndim, 24, 16, 12 if ndim == 3 else 1, n=6000, rmax=16.0)
     _random_fields(o, sim, rng, amp=2e-2)
     # fields with consistent ghosts and axis values, as after a step
     sim.UpdateBfieldHalf(); o.update_b_half(); o.field_halo("B")
@@ -124,15 +131,15 @@ def test_cyl_steps_match_oracle(ndim):
 
 
 def test_cyl_gpu_run_conserves_charge():
-    # 2-D r-theta, 64 x 64 cells, 40 steps: G constant at nodes 5 <= i <= mx-5
-    o = CylOracle(2, 64, 64, epc=8, cap=16)
+    # 2-D r-theta, 96 x 16 cells, 60 steps: G constant at nodes 5 <= i <= mx-5
+    o = CylOracle(2, 96, 16, c=C, epc=8, cap=16)
     rng = np.random.default_rng(4)
-    P = plasma(o, 100000, rng, 40.0)
-    sim = Simulation(64, 64, 1, ndim=2, epc=8, form="fp64", geometry="cyl")
+    P = plasma(o, 100000, rng, 80.0)
+    sim = Simulation(96, 16, 1, ndim=2, c=C, epc=8, form="fp64", geometry="cyl")
     sim.upload_particles(*P)
     o.load_particles(*P)
     G0 = gauss(o)[interior(o)]
-    sim.step(40)
+    sim.step(60)
     F = sim.download_fields()
     o.set_fields(**{k: F[k] for k in FIELDS})
     A = sim.download_particles()
@@ -145,11 +152,18 @@ def test_cyl_gpu_run_conserves_charge():
 
 
 def test_cyl_rejects_a_particle_at_the_wall():
-    sim = Simulation(16, 16, 1, ndim=2, epc=4, form="fp64", geometry="cyl")
-    x = np.array([17.9])                # mx - 3 = 18: the first push moves it past the wall limit
-    P = dict(q=np.ones(1), x=x, y=np.array([5.0]), z=np.array([1.5]), u=np.array([0.9]), v=np.zeros(1),
+    sim = Simulation(16, 16, 1, ndim=2, c=C, epc=4, form="fp64", geometry="cyl")
+    x = np.array([17.95])               # mx - 3 = 18: the first push moves it past the wall limit
+    P = dict(q=np.ones(1), x=x, y=np.array([5.0]), z=np.array([1.5]), u=np.array([2.0]), v=np.zeros(1),
              w=np.zeros(1), flv=np.ones(1, dtype=np.int32))
     sim.upload_particles(**P)
     with pytest.raises(RuntimeError):
         sim.step(1)
     sim.close()
+
+
+@pytest.mark.parametrize("ndim,ny,c", [(2, 16, 0.2), (3, 16, 0.19), (2, 8, 0.37)])
+def test_cyl_rejects_a_courant_number_above_the_axis_limit(ndim, ny, c):
+    # the cells next to the axis are 0.5 dtheta wide in theta: the reference's scheme diverges there
+    with pytest.raises(RuntimeError):
+        Simulation(16, ny, 8 if ndim == 3 else 1, ndim=ndim, c=c, epc=4, form="fp64", geometry="cyl")
