@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_eng_push(AdjP ap, const typename V4<R>:
     int ei = di, ej = dj;   // cell of the second half-segment
     if constexpr (std::is_same<R, float>::value) {
         // the split form deposits as the production kernel: FP64 to the stored position
-        pushdev::zigzag2d_split(ap.qs[s], in.ox[p], in.oy[p], in.oz[p], ox, oy, q.x1[2], di, dj, a, b);
+        pushdev::zigzag2d_split(sp2(ap.qs, s), in.ox[p], in.oy[p], in.oz[p], ox, oy, q.x1[2], di, dj, a, b);
     } else {
         seg_values<R>(q.F1, q.W1, a);
         seg_values<R>(q.F2, q.W2, b);

@@ -129,6 +129,9 @@ struct AdjEngine : AdjEngineBase {
     R *E[3] = {}, *B[3] = {};
     double *J[3] = {};
     std::vector<R *> ckE, ckB;          // per checkpoint: 3 + 3 arrays
+    // FP32 split form: the low part of E (E = hi + lo), as the context keeps it
+    float *Elo[3] = {};
+    std::vector<float *> ckElo;
     double *Eb[3] = {}, *Bb[3] = {}, *gE[3] = {}, *gB[3] = {};
     double *partial = nullptr;
     int *d_err = nullptr;
@@ -175,6 +178,9 @@ struct AdjEngine : AdjEngineBase {
         for (R *p : ckE) dfree(p);
         for (R *p : ckB) dfree(p);
         ckE.clear(); ckB.clear();
+        for (int i = 0; i < 3; i++) { dfree(Elo[i]); Elo[i] = nullptr; }
+        for (float *p : ckElo) dfree(p);
+        ckElo.clear();
         for (auto &p : ck) free_part(p);
         for (auto &p : st) free_part(p);
         ck.clear(); st.clear();
@@ -238,6 +244,12 @@ struct AdjEngine : AdjEngineBase {
             HIPTRY(dalloc(&ckE[i], ncell)); HIPTRY(dalloc(&ckB[i], ncell));
         }
         bytes += (long long)nseg * 6 * ncell * sizeof(R);
+        if (f32) {
+            for (int i = 0; i < 3; i++) HIPTRY(dalloc(&Elo[i], ncell));
+            ckElo.assign((size_t)nseg * 3, nullptr);
+            for (size_t i = 0; i < ckElo.size(); i++) HIPTRY(dalloc(&ckElo[i], ncell));
+            bytes += (long long)(nseg + 1) * 3 * ncell * 4;
+        }
         if (!fast) {
             ck.assign(nseg, EPart<R>{});
             for (auto &p : ck) HIPTRY(alloc_part(p, n, bytes));
@@ -425,8 +437,9 @@ struct AdjEngine : AdjEngineBase {
         launch_update_b_half<float>(c->g, c->ph, fE, fB, s);
         launch_ghost_copy<float>(c->g, fB, s);
         launch_current_fold<double>(c->g, J, s);
-        launch_update_e_add_current<float, double>(c->g, c->ph, fE, fB, J, ctx_jscale, s);
+        launch_update_e_pair(c->g, c->ph, fE, Elo, fB, J, ctx_jscale, true, true, s);
         launch_ghost_copy<float>(c->g, fE, s);
+        launch_ghost_copy<float>(c->g, Elo, s);
         HIPTRY(hipGetLastError());
         return PICTOR_OK;
     }
@@ -468,7 +481,12 @@ struct AdjEngine : AdjEngineBase {
         launch_update_b_half<R>(c->g, c->ph, E, B, s);
         launch_ghost_copy<R>(c->g, B, s);
         launch_current_fold<double>(c->g, J, s);
-        launch_update_e_add_current<R, double>(c->g, c->ph, E, B, J, ctx_jscale, s);
+        if constexpr (std::is_same<R, float>::value) {
+            launch_update_e_pair(c->g, c->ph, E, Elo, B, J, ctx_jscale, true, true, s);
+            launch_ghost_copy<float>(c->g, Elo, s);
+        } else {
+            launch_update_e_add_current<R, double>(c->g, c->ph, E, B, J, ctx_jscale, s);
+        }
         launch_ghost_copy<R>(c->g, E, s);
         HIPTRY(hipGetLastError());
         return PICTOR_OK;
@@ -510,6 +528,7 @@ struct AdjEngine : AdjEngineBase {
         for (int i = 0; i < 3; i++) {
             HIPTRY(hipMemcpyAsync(ckE[3 * k + i], E[i], ncell * sizeof(R), hipMemcpyDeviceToDevice, c->stream));
             HIPTRY(hipMemcpyAsync(ckB[3 * k + i], B[i], ncell * sizeof(R), hipMemcpyDeviceToDevice, c->stream));
+            if (Elo[i]) HIPTRY(hipMemcpyAsync(ckElo[3 * k + i], Elo[i], ncell * 4, hipMemcpyDeviceToDevice, c->stream));
         }
         return PICTOR_OK;
     }
@@ -519,6 +538,7 @@ struct AdjEngine : AdjEngineBase {
         for (int i = 0; i < 3; i++) {
             HIPTRY(hipMemcpyAsync(E[i], ckE[3 * k + i], ncell * sizeof(R), hipMemcpyDeviceToDevice, c->stream));
             HIPTRY(hipMemcpyAsync(B[i], ckB[3 * k + i], ncell * sizeof(R), hipMemcpyDeviceToDevice, c->stream));
+            if (Elo[i]) HIPTRY(hipMemcpyAsync(Elo[i], ckElo[3 * k + i], ncell * 4, hipMemcpyDeviceToDevice, c->stream));
         }
         return PICTOR_OK;
     }
@@ -530,8 +550,10 @@ struct AdjEngine : AdjEngineBase {
         for (int i = 0; i < 3; i++) {
             HIPTRY(hipMemcpyAsync(E[i], c->E[i], ncell * sizeof(R), hipMemcpyDeviceToDevice, s));
             HIPTRY(hipMemcpyAsync(B[i], c->B[i], ncell * sizeof(R), hipMemcpyDeviceToDevice, s));
+            if (Elo[i]) HIPTRY(hipMemcpyAsync(Elo[i], c->Elo[i], ncell * 4, hipMemcpyDeviceToDevice, s));
         }
         launch_ghost_copy<R>(c->g, E, s);
+        if (Elo[0]) launch_ghost_copy<float>(c->g, Elo, s);
         launch_ghost_copy<R>(c->g, B, s);
         int rc;
         if ((rc = reset_flags())) return rc;

@@ -37,6 +37,10 @@ template <typename R> __device__ inline R floor_(R v);
 template <> __device__ inline float floor_<float>(float v) { return floorf(v); }
 template <> __device__ inline double floor_<double>(double v) { return floor(v); }
 
+// per-species constant of a kernel argument: a select, so that a per-lane
+// species does not index the argument segment (a dependent memory load)
+__device__ inline double sp2(const double (&a)[2], int s) { return s == 0 ? a[0] : a[1]; }
+
 template <typename R> __device__ inline R vmin(R a, R b) { return b < a ? b : a; }
 template <typename R> __device__ inline R vmax(R a, R b) { return b > a ? b : a; }
 
@@ -60,7 +64,7 @@ __device__ inline void push_fwd(const AdjP &ap, int s, R x0, R y0, R z0, R un, R
 {
 #pragma clang fp contract(fast)
     const R c = (R)ap.c, sqc = (R)ap.sqc, cinv = (R)ap.cinv;
-    const R qm = (R)ap.qm[s], qs = (R)ap.qs[s];
+    const R qm = (R)sp2(ap.qm, s), qs = (R)sp2(ap.qs, s);
     q.x0 = x0; q.y0 = y0;
     const R w1 = ((R)1 - x0) * ((R)1 - y0), w2 = x0 * ((R)1 - y0);
     const R w3 = ((R)1 - x0) * y0, w4 = x0 * y0;
@@ -146,7 +150,7 @@ __device__ inline void reverse(const AdjP &ap, int s, const PState<R> &q, const 
                                const R jb1[8], const R jb2[8], const R x1b_in[3], const R u1b_in[3], R x0b[3],
                                R unb[3], R PEb[3], R PBb[3])
 {
-    const R qs = (R)ap.qs[s], qm = (R)ap.qm[s];
+    const R qs = (R)sp2(ap.qs, s), qm = (R)sp2(ap.qm, s);
     const R c = (R)ap.c, sqc = (R)ap.sqc, cinv = (R)ap.cinv;
     R x1b[3] = {x1b_in[0], x1b_in[1], x1b_in[2]};
     R u1b[3] = {u1b_in[0], u1b_in[1], u1b_in[2]};

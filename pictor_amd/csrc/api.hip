@@ -364,10 +364,9 @@ int do_e_phase(pictor_ctx *c, bool curl, bool add)
         double *J[3];
         field_ptrs(c, E, B);
         for (int i = 0; i < 3; i++) J[i] = (double *)c->J[i];
-        if (curl && add) launch_update_e_add_current<float, double>(c->g, c->ph, E, B, J, jscale(c), c->stream);
-        else if (curl) launch_update_e<float>(c->g, c->ph, E, B, c->stream);
-        else launch_add_current<float, double>(c->g, c->ph, E, J, jscale(c), c->stream);
+        launch_update_e_pair(c->g, c->ph, E, c->Elo, B, J, jscale(c), curl, add, c->stream);
         launch_ghost_copy<float>(c->g, E, c->stream);
+        launch_ghost_copy<float>(c->g, c->Elo, c->stream);
     }
     HIPTRY(hipGetLastError());
     return PICTOR_OK;
@@ -407,9 +406,15 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     *out = nullptr;
     if (cfg->ndim != 2 && cfg->ndim != 3) return fail(PICTOR_ERR_ARG, "ndim must be 2 or 3");
     if (cfg->nx < 8 || cfg->ny < 8 || (cfg->ndim == 3 && cfg->nz < 8)) return fail(PICTOR_ERR_ARG, "nx, ny (, nz) must be >= 8");
-    if (cfg->ndim == 2 && (cfg->nx > 32768 || cfg->ny > 32768)) return fail(PICTOR_ERR_ARG, "2-D nx, ny must be <= 32768");
-    if (cfg->ndim == 3 && (cfg->nx > 2048 || cfg->ny > 2048 || cfg->nz > 512))
+    // the packed cell word of the split form bounds the Cartesian grids; the cylindrical
+    // path keeps FP64 coordinates
+    const bool cyl_geom = cfg->geometry == PICTOR_GEOM_CYL;
+    if (!cyl_geom && cfg->ndim == 2 && (cfg->nx > 32768 || cfg->ny > 32768))
+        return fail(PICTOR_ERR_ARG, "2-D nx, ny must be <= 32768");
+    if (!cyl_geom && cfg->ndim == 3 && (cfg->nx > 2048 || cfg->ny > 2048 || cfg->nz > 512))
         return fail(PICTOR_ERR_ARG, "3-D nx, ny must be <= 2048 and nz <= 512");
+    if (cyl_geom && ((long long)cfg->nx + 5) * (cfg->ny + 5) * (cfg->ndim == 3 ? cfg->nz + 5 : 1) > (1LL << 31))
+        return fail(PICTOR_ERR_ARG, "cylindrical grid larger than 2^31 cells");
     if (cfg->form != PICTOR_FORM_FP64 && cfg->form != PICTOR_FORM_FP32_SPLIT) return fail(PICTOR_ERR_ARG, "unknown form");
     if (cfg->fdtd != PICTOR_FDTD_YEE && cfg->fdtd != PICTOR_FDTD_LEHE) return fail(PICTOR_ERR_ARG, "unknown fdtd");
     if (cfg->geometry != PICTOR_GEOM_CARTESIAN && cfg->geometry != PICTOR_GEOM_CYL)
@@ -516,6 +521,11 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
         hipMemsetAsync(c->E[i], 0, g.ncell * rs, c->stream);
         hipMemsetAsync(c->B[i], 0, g.ncell * rs, c->stream);
         hipMemsetAsync(c->J[i], 0, g.ncell * 8, c->stream);
+        if (c->form == PICTOR_FORM_FP32_SPLIT) {
+            if (hipMalloc(&c->Elo[i], g.ncell * sizeof(float)) != hipSuccess)
+                return cleanup_fail(fail(PICTOR_ERR_HIP, "field allocation failed"));
+            hipMemsetAsync(c->Elo[i], 0, g.ncell * sizeof(float), c->stream);
+        }
     }
     const size_t nb = (size_t)tp.ntiles * tp.NB;
     if (dalloc(&c->d_base, nb) || dalloc(&c->d_cap, nb) || dalloc(&c->d_cnt[0], nb) || dalloc(&c->d_cnt[1], nb) ||
@@ -557,7 +567,7 @@ int pictor_finalize(pictor_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     delete c->adj;
     c->adj = nullptr;
-    for (int i = 0; i < 3; i++) { dfree(c->E[i]); dfree(c->B[i]); dfree(c->J[i]); }
+    for (int i = 0; i < 3; i++) { dfree(c->E[i]); dfree(c->B[i]); dfree(c->J[i]); dfree(c->Elo[i]); }
     dfree(c->d_base); dfree(c->d_cap); dfree(c->d_cnt[0]); dfree(c->d_cnt[1]);
     dfree(c->d_spill_count); dfree(c->d_err);
     for (int i = 0; i < 2; i++) { free_f64(c->pf[i]); free_split(c->ps[i]); }
@@ -598,7 +608,8 @@ int pictor_upload_fields(pictor_ctx *c, const double *ex, const double *ey, cons
             HIPTRY(hipMemcpyAsync(dst[i], src[i], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
         } else {
             HIPTRY(hipMemcpyAsync(tmp, src[i], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-            launch_convert_fields_f32(tmp, (float *)dst[i], (long long)n, c->stream);
+            if (i < 3) launch_pair_from_f64(tmp, (float *)dst[i], c->Elo[i], (long long)n, c->stream);
+            else launch_convert_fields_f32(tmp, (float *)dst[i], (long long)n, c->stream);
         }
     }
     HIPTRY(hipStreamSynchronize(c->stream));
@@ -621,7 +632,8 @@ int pictor_download_fields(pictor_ctx *c, double *ex, double *ey, double *ez, do
         if (c->form == PICTOR_FORM_FP64) {
             HIPTRY(hipMemcpyAsync(dst[i], src[i], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         } else {
-            if (i < 6) launch_convert_fields_f64((const float *)src[i], tmp, (long long)n, c->stream);
+            if (i < 3) launch_pair_to_f64((const float *)src[i], c->Elo[i], tmp, (long long)n, c->stream);
+            else if (i < 6) launch_convert_fields_f64((const float *)src[i], tmp, (long long)n, c->stream);
             else launch_convert_jfix((const double *)src[i], tmp, (long long)n, jscale(c), c->stream);
             HIPTRY(hipMemcpyAsync(dst[i], tmp, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
             HIPTRY(hipStreamSynchronize(c->stream));
@@ -821,6 +833,7 @@ int pictor_field_halo(pictor_ctx *c, int32_t which)
     } else {
         float *P[3] = {(float *)F[0], (float *)F[1], (float *)F[2]};
         launch_ghost_copy<float>(c->g, P, c->stream);
+        if (which == 0) launch_ghost_copy<float>(c->g, c->Elo, c->stream);
     }
     HIPTRY(hipGetLastError());
     return PICTOR_OK;
@@ -910,7 +923,7 @@ int pictor_diagnostics(pictor_ctx *c, double out[9])
     } else {
         float *E[3], *B[3];
         field_ptrs(c, E, B);
-        launch_field_energy<float>(c->g, E, B, c->d_partial, nblk, c->stream);
+        launch_field_energy<float>(c->g, E, B, c->d_partial, nblk, c->stream, c->Elo);
     }
     HIPTRY(hipMemcpyAsync(part.data(), c->d_partial, nblk * 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipStreamSynchronize(c->stream));
@@ -942,7 +955,7 @@ int pictor_diagnostics(pictor_ctx *c, double out[9])
     } else {
         float *E[3], *B[3];
         field_ptrs(c, E, B);
-        launch_gauss<float>(c->g, E, c->d_rho, c->d_partial, nblk, c->stream);
+        launch_gauss<float>(c->g, E, c->d_rho, c->d_partial, nblk, c->stream, c->Elo);
     }
     HIPTRY(hipGetLastError());
     HIPTRY(hipMemcpyAsync(part.data(), c->d_partial, nblk * 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -62,6 +62,8 @@ struct pictor_ctx {
     void *E[3] = {nullptr, nullptr, nullptr};
     void *B[3] = {nullptr, nullptr, nullptr};
     void *J[3] = {nullptr, nullptr, nullptr};
+    // split form: the low FP32 part of E (E = hi + lo, hi in E[]), see launch_update_e_pair
+    float *Elo[3] = {nullptr, nullptr, nullptr};
 
     // tile bins
     std::vector<long long> h_base;

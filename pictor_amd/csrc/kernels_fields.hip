@@ -154,6 +154,88 @@ __global__ __launch_bounds__(BX *BY) void k_update_e(GridP g, R fc, R idx_, R id
     Ex[o] = ex; Ey[o] = ey; Ez[o] = ez;
 }
 
+// FP32 split form: E is kept as a pair of FP32 arrays (hi, lo) whose sum
+// carries the field to about 48 bits.  The update runs in FP64: the curl
+// differences of the FP32 B values are exact, so the discrete divergence of
+// the curl terms cancels to FP64 rounding, and the fixed-point current
+// enters exactly scaled.  Everything that reads E for the push or the B
+// update reads hi, an ordinary FP32 field.
+// the value a double takes after being stored as an FP32 (hi, lo) pair and read back
+__device__ __forceinline__ double pair_round(double x)
+{
+    const float h = (float)x;
+    const float l = (float)(x - (double)h);
+    return (double)h + (double)l;
+}
+
+template <bool CURL, bool ADD>
+__global__ __launch_bounds__(BX *BY) void k_update_e_pair(GridP g, double fc, double idx_, double idy_, double idz_,
+                                                          const float *__restrict__ Bx, const float *__restrict__ By,
+                                                          const float *__restrict__ Bz, float *__restrict__ Ex,
+                                                          float *__restrict__ Ey, float *__restrict__ Ez,
+                                                          float *__restrict__ Lx, float *__restrict__ Ly,
+                                                          float *__restrict__ Lz, const double *__restrict__ Jx,
+                                                          const double *__restrict__ Jy, const double *__restrict__ Jz,
+                                                          double jscale)
+{
+    int i = 3 + blockIdx.x * BX + threadIdx.x;
+    int j = 3 + blockIdx.y * BY + threadIdx.y;
+    int k = (g.ndim == 3) ? 3 + (int)blockIdx.z : 1;
+    if (i > g.mx - 3 || j > g.my - 3) return;
+    long long o = fidx(g, i, j, k);
+    long long oi = o - 1, oj = o - g.mx, ok = o - (long long)g.mx * g.my;
+    double ex = (double)Ex[o] + (double)Lx[o], ey = (double)Ey[o] + (double)Ly[o], ez = (double)Ez[o] + (double)Lz[o];
+    if (CURL) {
+        const double bxo = Bx[o], byo = By[o], bzo = Bz[o];
+        if (g.ndim == 2) {
+            ex = ex + fc * idy_ * (bzo - (double)Bz[oj]);
+            ey = ey - fc * idx_ * (bzo - (double)Bz[oi]);
+            ez = ez + fc * idx_ * (byo - (double)By[oi]) - fc * idy_ * (bxo - (double)Bx[oj]);
+        } else {
+            ex = ex + fc * idy_ * (bzo - (double)Bz[oj]) - fc * idz_ * (byo - (double)By[ok]);
+            ey = ey + fc * idz_ * (bxo - (double)Bx[ok]) - fc * idx_ * (bzo - (double)Bz[oi]);
+            ez = ez + fc * idx_ * (byo - (double)By[oi]) - fc * idy_ * (bxo - (double)Bx[oj]);
+        }
+    }
+    if (CURL && ADD) {
+        // round through the stored pair between the curl and the current, as the
+        // separate UpdateEfield / AddCurrent phases do: the fused step and the
+        // phase-by-phase host then give the same bits
+        ex = pair_round(ex); ey = pair_round(ey); ez = pair_round(ez);
+    }
+    if (ADD) {
+        bool inner = (g.ndim == 2) || (k >= 3 && k <= g.mz - 3);
+        if (inner) {
+            ex = ex - (Jx[o] * jscale) * idy_ * idz_;
+            ey = ey - (Jy[o] * jscale) * idx_ * idz_;
+            ez = ez - (Jz[o] * jscale) * idy_ * idx_;
+        }
+    }
+    const float hx = (float)ex, hy = (float)ey, hz = (float)ez;
+    Ex[o] = hx; Ey[o] = hy; Ez[o] = hz;
+    Lx[o] = (float)(ex - (double)hx); Ly[o] = (float)(ey - (double)hy); Lz[o] = (float)(ez - (double)hz);
+}
+
+// FP64 field <-> FP32 (hi, lo) pair
+__global__ void k_pair_from_f64(const double *__restrict__ src, float *__restrict__ hi, float *__restrict__ lo,
+                                long long n)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double v = src[t];
+    const float h = (float)v;
+    hi[t] = h;
+    lo[t] = (float)(v - (double)h);
+}
+
+__global__ void k_pair_to_f64(const float *__restrict__ hi, const float *__restrict__ lo, double *__restrict__ dst,
+                              long long n)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    dst[t] = (double)hi[t] + (double)lo[t];
+}
+
 // Ghost layers 1..2 and m-2..m take the value of their periodic image
 // (comm_fld.F90:75-130 on one rank; corners end as the doubly shifted image).
 template <typename R>
@@ -223,7 +305,8 @@ __global__ __launch_bounds__(256) void k_fill_zero(T *p, long long n)
 // CalcFldEnergy, savedata_routines.F90:310-327: per-block FP64 partial sums
 template <typename R>
 __global__ __launch_bounds__(256) void k_field_energy(GridP g, const R *Ex, const R *Ey, const R *Ez, const R *Bx,
-                                                      const R *By, const R *Bz, double *partial)
+                                                      const R *By, const R *Bz, const R *Lx, const R *Ly,
+                                                      const R *Lz, double *partial)
 {
     __shared__ double sb[256], se[256];
     long long nphys = (long long)g.nx * g.ny * ((g.ndim == 3) ? g.nz : 1);
@@ -236,6 +319,7 @@ __global__ __launch_bounds__(256) void k_field_energy(GridP g, const R *Ex, cons
         int k = (g.ndim == 3) ? (int)(r / g.ny) + 3 : 1;
         long long o = fidx(g, i, j, k);
         double bx = Bx[o], by = By[o], bz = Bz[o], ex = Ex[o], ey = Ey[o], ez = Ez[o];
+        if (Lx) { ex = ex + (double)Lx[o]; ey = ey + (double)Ly[o]; ez = ez + (double)Lz[o]; }
         be += (bx * bx + by * by + bz * bz);
         ee += (ex * ex + ey * ey + ez * ez);
     }
@@ -251,8 +335,8 @@ __global__ __launch_bounds__(256) void k_field_energy(GridP g, const R *Ex, cons
 // max |div E - rho| and max |rho| over physical nodes (CalcDivE stencil,
 // savedata_routines.F90:329-355)
 template <typename R>
-__global__ __launch_bounds__(256) void k_gauss(GridP g, const R *Ex, const R *Ey, const R *Ez, const double *rho,
-                                               double *partial)
+__global__ __launch_bounds__(256) void k_gauss(GridP g, const R *Ex, const R *Ey, const R *Ez, const R *Lx,
+                                               const R *Ly, const R *Lz, const double *rho, double *partial)
 {
     __shared__ double sr[256], sa[256];
     long long nphys = (long long)g.nx * g.ny * ((g.ndim == 3) ? g.nz : 1);
@@ -264,8 +348,17 @@ __global__ __launch_bounds__(256) void k_gauss(GridP g, const R *Ex, const R *Ey
         int j = (int)(r % g.ny) + 3;
         int k = (g.ndim == 3) ? (int)(r / g.ny) + 3 : 1;
         long long o = fidx(g, i, j, k);
-        double d = (double)Ex[o] - (double)Ex[o - 1] + (double)Ey[o] - (double)Ey[o - g.mx];
-        if (g.ndim == 3) d = d + (double)Ez[o] - (double)Ez[o - (long long)g.mx * g.my];
+        const long long ok = o - (long long)g.mx * g.my;
+        double d;
+        if (Lx) {
+            // (hi + lo) of each value first: the pair is one field value
+            d = ((double)Ex[o] + (double)Lx[o]) - ((double)Ex[o - 1] + (double)Lx[o - 1]) +
+                ((double)Ey[o] + (double)Ly[o]) - ((double)Ey[o - g.mx] + (double)Ly[o - g.mx]);
+            if (g.ndim == 3) d = d + ((double)Ez[o] + (double)Lz[o]) - ((double)Ez[ok] + (double)Lz[ok]);
+        } else {
+            d = (double)Ex[o] - (double)Ex[o - 1] + (double)Ey[o] - (double)Ey[o - g.mx];
+            if (g.ndim == 3) d = d + (double)Ez[o] - (double)Ez[ok];
+        }
         double res = fabs(d - rho[o]);
         rm = fmax(rm, res);
         am = fmax(am, fabs(rho[o]));
@@ -374,16 +467,46 @@ void launch_fill_zero(T *p, long long n, hipStream_t s)
 }
 
 template <typename R>
-void launch_field_energy(const GridP &g, R *const E[3], R *const B[3], double *partial, int nblk, hipStream_t s)
+void launch_field_energy(const GridP &g, R *const E[3], R *const B[3], double *partial, int nblk, hipStream_t s,
+                         R *const *L)
 {
     hipLaunchKernelGGL(k_field_energy<R>, dim3(nblk), dim3(256), 0, s, g, E[0], E[1], E[2], B[0], B[1], B[2],
+                       L ? L[0] : (const R *)nullptr, L ? L[1] : (const R *)nullptr, L ? L[2] : (const R *)nullptr,
                        partial);
 }
 
 template <typename R>
-void launch_gauss(const GridP &g, R *const E[3], const double *rho, double *partial, int nblk, hipStream_t s)
+void launch_gauss(const GridP &g, R *const E[3], const double *rho, double *partial, int nblk, hipStream_t s,
+                  R *const *L)
 {
-    hipLaunchKernelGGL(k_gauss<R>, dim3(nblk), dim3(256), 0, s, g, E[0], E[1], E[2], rho, partial);
+    hipLaunchKernelGGL(k_gauss<R>, dim3(nblk), dim3(256), 0, s, g, E[0], E[1], E[2], L ? L[0] : (const R *)nullptr,
+                       L ? L[1] : (const R *)nullptr, L ? L[2] : (const R *)nullptr, rho, partial);
+}
+
+void launch_update_e_pair(const GridP &g, const PhysP &ph, float *const E[3], float *const L[3], float *const B[3],
+                          double *const J[3], double jscale, bool curl, bool add, hipStream_t s)
+{
+    const float *b0 = B ? B[0] : nullptr, *b1 = B ? B[1] : nullptr, *b2 = B ? B[2] : nullptr;
+    const double *j0 = J ? J[0] : nullptr, *j1 = J ? J[1] : nullptr, *j2 = J ? J[2] : nullptr;
+    if (curl && add)
+        hipLaunchKernelGGL((k_update_e_pair<true, true>), phys_grid(g), dim3(BX, BY), 0, s, g, ph.fldc, ph.inv_dx,
+                           ph.inv_dy, ph.inv_dz, b0, b1, b2, E[0], E[1], E[2], L[0], L[1], L[2], j0, j1, j2, jscale);
+    else if (curl)
+        hipLaunchKernelGGL((k_update_e_pair<true, false>), phys_grid(g), dim3(BX, BY), 0, s, g, ph.fldc, ph.inv_dx,
+                           ph.inv_dy, ph.inv_dz, b0, b1, b2, E[0], E[1], E[2], L[0], L[1], L[2], j0, j1, j2, jscale);
+    else
+        hipLaunchKernelGGL((k_update_e_pair<false, true>), phys_grid(g), dim3(BX, BY), 0, s, g, ph.fldc, ph.inv_dx,
+                           ph.inv_dy, ph.inv_dz, b0, b1, b2, E[0], E[1], E[2], L[0], L[1], L[2], j0, j1, j2, jscale);
+}
+
+void launch_pair_from_f64(const double *src, float *hi, float *lo, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pair_from_f64, dim3(nblocks(n, 256)), dim3(256), 0, s, src, hi, lo, n);
+}
+
+void launch_pair_to_f64(const float *hi, const float *lo, double *dst, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pair_to_f64, dim3(nblocks(n, 256)), dim3(256), 0, s, hi, lo, dst, n);
 }
 
 void launch_convert_fields_f32(const double *src, float *dst, long long n, hipStream_t s)
@@ -422,9 +545,13 @@ template void launch_fill_zero<double>(double *, long long, hipStream_t);
 template void launch_fill_zero<float>(float *, long long, hipStream_t);
 template void launch_fill_zero<long long>(long long *, long long, hipStream_t);
 template void launch_fill_zero<int>(int *, long long, hipStream_t);
-template void launch_field_energy<double>(const GridP &, double *const[3], double *const[3], double *, int, hipStream_t);
-template void launch_field_energy<float>(const GridP &, float *const[3], float *const[3], double *, int, hipStream_t);
-template void launch_gauss<double>(const GridP &, double *const[3], const double *, double *, int, hipStream_t);
-template void launch_gauss<float>(const GridP &, float *const[3], const double *, double *, int, hipStream_t);
+template void launch_field_energy<double>(const GridP &, double *const[3], double *const[3], double *, int, hipStream_t,
+                                          double *const *);
+template void launch_field_energy<float>(const GridP &, float *const[3], float *const[3], double *, int, hipStream_t,
+                                         float *const *);
+template void launch_gauss<double>(const GridP &, double *const[3], const double *, double *, int, hipStream_t,
+                                   double *const *);
+template void launch_gauss<float>(const GridP &, float *const[3], const double *, double *, int, hipStream_t,
+                                  float *const *);
 
 }  // namespace pictor

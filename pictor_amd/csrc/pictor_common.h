@@ -114,9 +114,16 @@ void launch_current_fold(const GridP &g, JT *const J[3], hipStream_t s);
 template <typename T>
 void launch_fill_zero(T *p, long long n, hipStream_t s);
 template <typename R>
-void launch_field_energy(const GridP &g, R *const E[3], R *const B[3], double *partial, int nblk, hipStream_t s);
+void launch_field_energy(const GridP &g, R *const E[3], R *const B[3], double *partial, int nblk, hipStream_t s,
+                         R *const *L = nullptr);
 template <typename R>
-void launch_gauss(const GridP &g, R *const E[3], const double *rho, double *partial, int nblk, hipStream_t s);
+void launch_gauss(const GridP &g, R *const E[3], const double *rho, double *partial, int nblk, hipStream_t s,
+                  R *const *L = nullptr);
+// FP32 split form: E as an FP32 (hi, lo) pair updated in FP64 (L = lo); B may be null when !curl, J when !add
+void launch_update_e_pair(const GridP &g, const PhysP &ph, float *const E[3], float *const L[3], float *const B[3],
+                          double *const J[3], double jscale, bool curl, bool add, hipStream_t s);
+void launch_pair_from_f64(const double *src, float *hi, float *lo, long long n, hipStream_t s);
+void launch_pair_to_f64(const float *hi, const float *lo, double *dst, long long n, hipStream_t s);
 void launch_convert_fields_f32(const double *src, float *dst, long long n, hipStream_t s);
 void launch_convert_fields_f64(const float *src, double *dst, long long n, hipStream_t s);
 void launch_convert_jfix(const double *src, double *dst, long long n, double scale, hipStream_t s);
