@@ -191,6 +191,14 @@ def accuracy_report(args, ndim, nx, ny, nz, epc, P):
         t0 = time.time()
         series[form] = _series_gpu(form, ndim, nx, ny, nz, epc, P, steps, every)
         print(f"# {form}: {steps} steps in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    # the sensitivity floor: the FP64 form from momenta rounded to FP32 (a 2^-24 relative
+    # change of the input); after saturation the Weibel filaments amplify any such change
+    Pr = dict(P)
+    for k in ("u", "v", "w"):
+        Pr[k] = np.asarray(P[k]).astype(np.float32).astype(np.float64)
+    t0 = time.time()
+    series["fp64_f32_momenta"] = _series_gpu("fp64", ndim, nx, ny, nz, epc, Pr, steps, every)
+    print(f"# fp64 from FP32-rounded momenta: {steps} steps in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     if args.cpu_steps != 0:
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
         t0 = time.time()
@@ -243,7 +251,7 @@ def adjoint_report(args):
     from pictor_amd.adjoint import ParticleStepObjective, drift_boost, drift_signs
 
     nx = ny = args.adjoint_n
-    epc, drift, uth, nsteps, seg = args.adjoint_epc, 0.5, 0.01, args.steps, args.segment
+    epc, drift, uth, nsteps, seg = args.adjoint_epc, 0.5, 0.01, args.adjoint_steps, args.segment
     P0 = setup_weibel(nx, ny, epc=epc, drift=0.0, uth=uth, seed=12345)
     sgn = torch.from_numpy(drift_signs(len(P0["x"]), epc))
     th = [torch.from_numpy(P0[k]) for k in ("u", "v", "w")]
@@ -429,6 +437,7 @@ def main():
                     help="instead of timing the step: the adjoint workload (gradients, FD check, cost ratios) "
                          "over --steps steps with checkpoints every --segment")
     ap.add_argument("--segment", type=int, default=20)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-n", type=int, default=512)
     ap.add_argument("--adjoint-epc", type=int, default=16)
     ap.add_argument("--adjoint-engine", default="sorted", choices=("sorted", "unsorted"))

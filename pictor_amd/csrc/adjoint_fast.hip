@@ -10,6 +10,8 @@
 // fields accumulated in LDS and flushed with global atomics.  A particle's
 // incoming adjoint is read at its output slot, its outgoing adjoint written
 // at its input slot.
+#include <type_traits>
+
 #include "adjoint.h"
 #include "adjoint_math.h"
 #include "push_common.h"
@@ -54,30 +56,25 @@ __device__ inline void vbin_of(int v, int nsp, int &s, int &d)
     d = q < 4 ? q : q + 1;
 }
 
-// one particle of the reverse sweep: its state before the step, the slot it
-// went to, and the adjoint of its state after the step (read at that slot)
+// one particle of the reverse sweep: its slot, species, state before the step
+// and the slot it went to (where the adjoint of its state after the step is)
 struct AdjRec {
     unsigned slot;
     int s, to, cell;
     float ox, oy, oz, u, v, w;
-    float a[6];
 };
 
-__device__ inline void load_rec(const PartSplit &st, const EPart<float> &ain, AdjRec &r)
+__device__ inline void load_rec(const PartSplit &st, const int *dst, AdjRec &r)
 {
     const unsigned sl = r.slot;
+    r.to = dst[sl];
     r.cell = st.cell[sl];
     r.ox = st.ox[sl]; r.oy = st.oy[sl]; r.oz = st.oz[sl];
     r.u = st.u[sl]; r.v = st.v[sl]; r.w = st.w[sl];
-    if (r.to >= 0) {
-        const int t = r.to;
-        r.a[0] = ain.ox[t]; r.a[1] = ain.oy[t]; r.a[2] = ain.oz[t];
-        r.a[3] = ain.u[t]; r.a[4] = ain.v[t]; r.a[5] = ain.w[t];
-    }
 }
 
 template <int T, int BS>
-__global__ __launch_bounds__(BS, 1024 / BS) void k_fa_adjoint(GridP g, TileP tp, AdjP ap, const float *__restrict__ Ex,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1024 / BS, 1024 / BS))) void k_fa_adjoint(GridP g, TileP tp, AdjP ap, const float *__restrict__ Ex,
                                                    const float *__restrict__ Ey, const float *__restrict__ Ez,
                                                    const float *__restrict__ Bx, const float *__restrict__ By,
                                                    const float *__restrict__ Bz, const double *__restrict__ Ebx,
@@ -94,7 +91,11 @@ __global__ __launch_bounds__(BS, 1024 / BS) void k_fa_adjoint(GridP g, TileP tp,
     constexpr int MB = 18;
     __shared__ float4 sC[6 * T * T];
     __shared__ float sJb[3 * WW];
-    __shared__ float sG[6 * NN];
+    // FP64: an LDS FP32 atomic add runs at about 1/8 the rate of an FP64 one on
+    // gfx950 (0.33 vs 2.6 lanes per CU per clock); the 32 x 32 tile keeps FP32,
+    // whose FP64 window would not fit beside its corner staging in 160 KB
+    using GA = typename std::conditional<(T <= 16), double, float>::type;
+    __shared__ GA sG[6 * NN];
     __shared__ int sStart[MB + 1], sVs[MB];
     __shared__ unsigned sSrc[MB];
     const int tile = tile_remap(tp, blockIdx.x);
@@ -127,76 +128,83 @@ __global__ __launch_bounds__(BS, 1024 / BS) void k_fa_adjoint(GridP g, TileP tp,
         sJb[WW + e] = js * (float)Eby[o];
         sJb[2 * WW + e] = js * (float)Ebz[o];
     }
-    for (int e = threadIdx.x; e < 6 * NN; e += BS) sG[e] = 0.0f;
+    for (int e = threadIdx.x; e < 6 * NN; e += BS) sG[e] = (GA)0;
     __syncthreads();
     const int total = sStart[NV];
-    // software pipeline: the next particle's state and incoming adjoint are
-    // loaded while the current one is processed; its output slot (which the
-    // incoming adjoint's address depends on) was loaded one iteration earlier
+    // software pipeline: the next particle's state and output slot are loaded
+    // while the current one is processed; the current one's incoming adjoint
+    // (whose address is its output slot) is loaded first and used only after
+    // the recomputed forward push
+    // bin of position k: the last v < NV with sStart[v] <= k (an empty bin shares
+    // its start with the next one).  A fixed-depth search with no loop: a loop
+    // inside the particle loop makes the compiler wait for every outstanding
+    // load before entering it, which would undo the pipelining.
+    static_assert(MB <= 32, "bin search depth");
     auto slot_of = [&](int k, unsigned &slot, int &s) {
         int v = 0;
-        while (k >= sStart[v + 1]) v++;
+#pragma unroll
+        for (int step = 16; step >= 1; step >>= 1) {
+            const int c = v + step;
+            const int sv = sStart[c < NV ? c : NV - 1];
+            v = ((c < NV) & (sv <= k)) ? c : v;
+        }
         slot = sSrc[v] + (unsigned)(k - sStart[v]);
         s = sVs[v];
     };
     AdjRec nxt{};
-    int to_ahead = -1;   // output slot of particle k + 2 BS
     if ((int)threadIdx.x < total) {
         slot_of(threadIdx.x, nxt.slot, nxt.s);
-        nxt.to = dst[nxt.slot];
-        load_rec(st, ain, nxt);
+        load_rec(st, dst, nxt);
     }
-    if ((int)threadIdx.x + BS < total) {
-        unsigned sl;
-        int ss;
-        slot_of(threadIdx.x + BS, sl, ss);
-        to_ahead = dst[sl];
-    }
+    // the loads below are unconditional (clamped indices): a load under a branch
+    // makes the compiler wait for all outstanding loads where the paths join
     for (int k = (int)threadIdx.x; k < total; k += BS) {
         const AdjRec cur = nxt;
-        if (k + BS < total) {
-            slot_of(k + BS, nxt.slot, nxt.s);
-            nxt.to = to_ahead;
-            load_rec(st, ain, nxt);
-        }
-        if (k + 2 * BS < total) {
-            unsigned sl;
-            int ss;
-            slot_of(k + 2 * BS, sl, ss);
-            to_ahead = dst[sl];
-        }
+        const int t = cur.to >= 0 ? cur.to : 0;
+        const float a[6] = {ain.ox[t], ain.oy[t], ain.oz[t], ain.u[t], ain.v[t], ain.w[t]};
+        slot_of(k + BS < total ? k + BS : k, nxt.slot, nxt.s);
+        load_rec(st, dst, nxt);
         const int ci = cur.cell & 0xFFFF, cj = cur.cell >> 16;
-        const int ii0 = ci - cx0, jj0 = cj - cy0;
-        if (ii0 < 2 || ii0 > T + 1 || jj0 < 2 || jj0 > T + 1 || cur.to < 0) continue;   // excluded by the forward checks
+        int ii0 = ci - cx0, jj0 = cj - cy0;
+        // a particle the forward checks excluded goes through the same
+        // instructions on a clamped cell and contributes zeros (no branch: see above)
+        const bool ok = ii0 >= 2 && ii0 <= T + 1 && jj0 >= 2 && jj0 <= T + 1 && cur.to >= 0;
+        ii0 = min(max(ii0, 2), T + 1);
+        jj0 = min(max(jj0, 2), T + 1);
         const int lc = (jj0 - 2) * T + (ii0 - 2);
         const float4 gc[6] = {sC[6 * lc], sC[6 * lc + 1], sC[6 * lc + 2], sC[6 * lc + 3], sC[6 * lc + 4], sC[6 * lc + 5]};
         PState<float> q;
         push_fwd<float>(ap, cur.s, cur.ox, cur.oy, cur.oz, cur.u, cur.v, cur.w, gc, q);
         const int e0 = jj0 * W + ii0;
-        const int e2 = e0 + q.d[1] * W + q.d[0];
+        const int e2 = e0 + min(max(q.d[1], -1), 1) * W + min(max(q.d[0], -1), 1);
         const float jb1[8] = {sJb[e0], sJb[e0 + W], sJb[WW + e0], sJb[WW + e0 + 1],
                               sJb[2 * WW + e0], sJb[2 * WW + e0 + 1], sJb[2 * WW + e0 + W], sJb[2 * WW + e0 + W + 1]};
         const float jb2[8] = {sJb[e2], sJb[e2 + W], sJb[WW + e2], sJb[WW + e2 + 1],
                               sJb[2 * WW + e2], sJb[2 * WW + e2 + 1], sJb[2 * WW + e2 + W], sJb[2 * WW + e2 + W + 1]};
-        const float x1b[3] = {cur.a[0], cur.a[1], cur.a[2]};
-        const float u1b[3] = {cur.a[3], cur.a[4], cur.a[5]};
+        const float x1b[3] = {a[0], a[1], a[2]};
+        const float u1b[3] = {a[3], a[4], a[5]};
         float x0b[3], unb[3], PEb[3], PBb[3];
         reverse<float>(ap, cur.s, q, gc, jb1, jb2, x1b, u1b, x0b, unb, PEb, PBb);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            PEb[c] = ok ? PEb[c] : 0.0f;
+            PBb[c] = ok ? PBb[c] : 0.0f;
+        }
         const int n0 = (jj0 - 2) * N1 + (ii0 - 2);
         const int nc[4] = {n0, n0 + 1, n0 + N1, n0 + N1 + 1};
 #pragma unroll
         for (int m = 0; m < 4; m++) {
-            const float w = q.w[m];
-            atomicAdd(&sG[nc[m]], w * PEb[0]);
-            atomicAdd(&sG[NN + nc[m]], w * PEb[1]);
-            atomicAdd(&sG[2 * NN + nc[m]], w * PEb[2]);
-            atomicAdd(&sG[3 * NN + nc[m]], w * PBb[0]);
-            atomicAdd(&sG[4 * NN + nc[m]], w * PBb[1]);
-            atomicAdd(&sG[5 * NN + nc[m]], w * PBb[2]);
+            const float w = ok ? q.w[m] : 0.0f;
+            atomicAdd(&sG[nc[m]], (GA)(w * PEb[0]));
+            atomicAdd(&sG[NN + nc[m]], (GA)(w * PEb[1]));
+            atomicAdd(&sG[2 * NN + nc[m]], (GA)(w * PEb[2]));
+            atomicAdd(&sG[3 * NN + nc[m]], (GA)(w * PBb[0]));
+            atomicAdd(&sG[4 * NN + nc[m]], (GA)(w * PBb[1]));
+            atomicAdd(&sG[5 * NN + nc[m]], (GA)(w * PBb[2]));
         }
         const unsigned slot = cur.slot;
-        aout.ox[slot] = x0b[0]; aout.oy[slot] = x0b[1]; aout.oz[slot] = x0b[2];
-        aout.u[slot] = unb[0]; aout.v[slot] = unb[1]; aout.w[slot] = unb[2];
+        aout.ox[slot] = ok ? x0b[0] : 0.0f; aout.oy[slot] = ok ? x0b[1] : 0.0f; aout.oz[slot] = ok ? x0b[2] : 0.0f;
+        aout.u[slot] = ok ? unb[0] : 0.0f; aout.v[slot] = ok ? unb[1] : 0.0f; aout.w[slot] = ok ? unb[2] : 0.0f;
     }
     __syncthreads();
     // node (gi, gj) of the tile is reference node (3 + tx T + gi, 3 + ty T + gj)
@@ -205,8 +213,8 @@ __global__ __launch_bounds__(BS, 1024 / BS) void k_fa_adjoint(GridP g, TileP tp,
         const long long o = (long long)(2 + tx * T + e % N1) + (long long)mx * (2 + ty * T + e / N1);
 #pragma unroll
         for (int cc = 0; cc < 6; cc++) {
-            const float val = sG[cc * NN + e];
-            if (val != 0.0f) unsafeAtomicAdd(&gdst[cc][o], (double)val);
+            const double val = (double)sG[cc * NN + e];
+            if (val != 0.0) unsafeAtomicAdd(&gdst[cc][o], val);
         }
     }
 }
