@@ -424,12 +424,12 @@ struct AdjEngine : AdjEngineBase {
         float *const *fB = reinterpret_cast<float *const *>(B);
         launch_update_b_half<float>(c->g, c->ph, fE, fB, s);
         launch_ghost_copy<float>(c->g, fB, s);
-        if (j >= 0)
-            for (int i = 0; i < 3; i++) {
-                HIPTRY(hipMemcpyAsync(sfE[3 * j + i], fE[i], ncell * 4, hipMemcpyDeviceToDevice, s));
-                HIPTRY(hipMemcpyAsync(sfB[3 * j + i], fB[i], ncell * 4, hipMemcpyDeviceToDevice, s));
-            }
-        for (int i = 0; i < 3; i++) launch_fill_zero<double>(J[i], ncell, s);
+        if (j >= 0) {
+            float *dst[6] = {sfE[3 * j], sfE[3 * j + 1], sfE[3 * j + 2], sfB[3 * j], sfB[3 * j + 1], sfB[3 * j + 2]};
+            float *src[6] = {fE[0], fE[1], fE[2], fB[0], fB[1], fB[2]};
+            launch_copy_n<float>(dst, src, 6, ncell, s);
+        }
+        launch_fill_zero_n<double>(J, 3, ncell, s);
         BinsP bins{d_base, d_cap, cin, cout};
         SpillP sp{d_spill, spill_cap, d_err};
         launch_push2d_split_track(c->tp.T, c->g, c->ph, c->tp, fE, fB, J, in, out, bins, spill, sp,
@@ -438,8 +438,7 @@ struct AdjEngine : AdjEngineBase {
         launch_ghost_copy<float>(c->g, fB, s);
         launch_current_fold<double>(c->g, J, s);
         launch_update_e_pair(c->g, c->ph, fE, Elo, fB, J, ctx_jscale, true, true, s);
-        launch_ghost_copy<float>(c->g, fE, s);
-        launch_ghost_copy<float>(c->g, Elo, s);
+        launch_ghost_copy<float>(c->g, fE, s, Elo);
         HIPTRY(hipGetLastError());
         return PICTOR_OK;
     }
@@ -474,7 +473,7 @@ struct AdjEngine : AdjEngineBase {
         hipStream_t s = c->stream;
         launch_update_b_half<R>(c->g, c->ph, E, B, s);
         launch_ghost_copy<R>(c->g, B, s);
-        for (int i = 0; i < 3; i++) launch_fill_zero<double>(J[i], ncell, s);
+        launch_fill_zero_n<double>(J, 3, ncell, s);
         void *nE = nodE[j < 0 ? 0 : j], *nB = nodB[j < 0 ? 0 : j];
         launch_eng_nodes<R>(ap, E, B, nE, nB, s);
         launch_eng_push<R>(ap, nE, nB, in, out, J, n, s);
@@ -505,17 +504,13 @@ struct AdjEngine : AdjEngineBase {
         pb.fldc = -c->ph.fld_halfc;      // E kernel (curl only) as the transpose of a B half step
         launch_ghost_copy<double>(g, Eb, s);
         launch_update_b_half<double>(g, pa, Eb, Bb, s);
-        for (int i = 0; i < 3; i++) {
-            launch_fill_zero<double>(gE[i], ncell, s);
-            launch_fill_zero<double>(gB[i], ncell, s);
-        }
+        double *gEB[6] = {gE[0], gE[1], gE[2], gB[0], gB[1], gB[2]};
+        launch_fill_zero_n<double>(gEB, 6, ncell, s);
         particle_adjoint();
         launch_ghost_copy<double>(g, Bb, s);
         launch_update_e<double>(g, pb, Eb, Bb, s);
-        launch_current_fold<double>(g, gE, s);
-        launch_current_fold<double>(g, gB, s);
-        launch_ghost_copy<double>(g, gE, s);
-        launch_ghost_copy<double>(g, gB, s);
+        launch_current_fold<double>(g, gE, s, gB);
+        launch_ghost_copy<double>(g, gE, s, gB);
         launch_node_to_stag(ap, gE, gB, Eb, Bb, s);
         launch_ghost_copy<double>(g, Bb, s);
         launch_update_e<double>(g, pb, Eb, Bb, s);

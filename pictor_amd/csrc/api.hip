@@ -333,7 +333,8 @@ int do_push(pictor_ctx *c, bool timed)
 
 int do_reset_current(pictor_ctx *c)
 {
-    for (int i = 0; i < 3; i++) launch_fill_zero<double>((double *)c->J[i], c->g.ncell, c->stream);
+    double *J[3] = {(double *)c->J[0], (double *)c->J[1], (double *)c->J[2]};
+    launch_fill_zero_n<double>(J, 3, c->g.ncell, c->stream);
     HIPTRY(hipGetLastError());
     return PICTOR_OK;
 }
@@ -365,8 +366,7 @@ int do_e_phase(pictor_ctx *c, bool curl, bool add)
         field_ptrs(c, E, B);
         for (int i = 0; i < 3; i++) J[i] = (double *)c->J[i];
         launch_update_e_pair(c->g, c->ph, E, c->Elo, B, J, jscale(c), curl, add, c->stream);
-        launch_ghost_copy<float>(c->g, E, c->stream);
-        launch_ghost_copy<float>(c->g, c->Elo, c->stream);
+        launch_ghost_copy<float>(c->g, E, c->stream, c->Elo);
     }
     HIPTRY(hipGetLastError());
     return PICTOR_OK;
@@ -832,8 +832,7 @@ int pictor_field_halo(pictor_ctx *c, int32_t which)
         launch_ghost_copy<double>(c->g, P, c->stream);
     } else {
         float *P[3] = {(float *)F[0], (float *)F[1], (float *)F[2]};
-        launch_ghost_copy<float>(c->g, P, c->stream);
-        if (which == 0) launch_ghost_copy<float>(c->g, c->Elo, c->stream);
+        launch_ghost_copy<float>(c->g, P, c->stream, which == 0 ? c->Elo : nullptr);
     }
     HIPTRY(hipGetLastError());
     return PICTOR_OK;

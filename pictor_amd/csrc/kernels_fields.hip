@@ -239,8 +239,11 @@ __global__ void k_pair_to_f64(const float *__restrict__ hi, const float *__restr
 // Ghost layers 1..2 and m-2..m take the value of their periodic image
 // (comm_fld.F90:75-130 on one rank; corners end as the doubly shifted image).
 template <typename R>
-__global__ __launch_bounds__(256) void k_ghost_copy(GridP g, R *__restrict__ F0, R *__restrict__ F1, R *__restrict__ F2)
+__global__ __launch_bounds__(256) void k_ghost_copy(GridP g, Ptrs6<R> F)
 {
+    R *__restrict__ F0 = F.p[3 * blockIdx.y];
+    R *__restrict__ F1 = F.p[3 * blockIdx.y + 1];
+    R *__restrict__ F2 = F.p[3 * blockIdx.y + 2];
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.ncell) return;
     int i = (int)(t % g.mx) + 1;
@@ -261,9 +264,11 @@ __global__ __launch_bounds__(256) void k_ghost_copy(GridP g, R *__restrict__ F0,
 // UpdateCurrentAllFaces for one axis: 3..5 += m-2..m and m-4..m-3 += 1..2.
 // Axis 0 spans the full cross-section, axes 1 and 2 the x range 3..mx-3.
 template <typename JT>
-__global__ __launch_bounds__(256) void k_current_fold(GridP g, int axis, JT *__restrict__ J0, JT *__restrict__ J1,
-                                                      JT *__restrict__ J2)
+__global__ __launch_bounds__(256) void k_current_fold(GridP g, int axis, Ptrs6<JT> J)
 {
+    JT *__restrict__ J0 = J.p[3 * blockIdx.y];
+    JT *__restrict__ J1 = J.p[3 * blockIdx.y + 1];
+    JT *__restrict__ J2 = J.p[3 * blockIdx.y + 2];
     // target layers: 5 planes normal to `axis`
     int m[3] = {g.mx, g.my, g.mz};
     int n[3] = {g.nx, g.ny, g.nz};
@@ -295,11 +300,22 @@ __global__ __launch_bounds__(256) void k_current_fold(GridP g, int axis, JT *__r
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_fill_zero(T *p, long long n)
+__global__ __launch_bounds__(256) void k_fill_zero(Ptrs6<T> P, long long n)
 {
+    T *p = P.p[blockIdx.y];
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
     for (; t < n; t += stride) p[t] = T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_n(Ptrs6<T> D, Ptrs6<T> S, long long n)
+{
+    T *d = D.p[blockIdx.y];
+    const T *src = S.p[blockIdx.y];
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) d[t] = src[t];
 }
 
 // CalcFldEnergy, savedata_routines.F90:310-327: per-block FP64 partial sums
@@ -439,31 +455,52 @@ void launch_add_current(const GridP &g, const PhysP &ph, R *const E[3], JT *cons
 }
 
 template <typename R>
-void launch_ghost_copy(const GridP &g, R *const F[3], hipStream_t s)
+void launch_ghost_copy(const GridP &g, R *const F[3], hipStream_t s, R *const G[3])
 {
-    hipLaunchKernelGGL(k_ghost_copy<R>, dim3(nblocks(g.ncell, 256)), dim3(256), 0, s, g, F[0], F[1], F[2]);
+    Ptrs6<R> P{{F[0], F[1], F[2], G ? G[0] : nullptr, G ? G[1] : nullptr, G ? G[2] : nullptr}};
+    hipLaunchKernelGGL(k_ghost_copy<R>, dim3(nblocks(g.ncell, 256), G ? 2 : 1), dim3(256), 0, s, g, P);
 }
 
 template <typename JT>
-void launch_current_fold(const GridP &g, JT *const J[3], hipStream_t s)
+void launch_current_fold(const GridP &g, JT *const J[3], hipStream_t s, JT *const K[3])
 {
+    Ptrs6<JT> P{{J[0], J[1], J[2], K ? K[0] : nullptr, K ? K[1] : nullptr, K ? K[2] : nullptr}};
     int naxes = (g.ndim == 3) ? 3 : 2;
     for (int a = 0; a < naxes; a++) {
         long long ext[3] = {g.mx, g.my, g.mz};
         if (a != 0) ext[0] = g.mx - 5;
         ext[a] = 5;
         long long total = ext[0] * ext[1] * ext[2];
-        hipLaunchKernelGGL(k_current_fold<JT>, dim3(nblocks(total, 256)), dim3(256), 0, s, g, a, J[0], J[1], J[2]);
+        hipLaunchKernelGGL(k_current_fold<JT>, dim3(nblocks(total, 256), K ? 2 : 1), dim3(256), 0, s, g, a, P);
     }
+}
+
+template <typename T>
+void launch_fill_zero_n(T *const *p, int count, long long n, hipStream_t s)
+{
+    unsigned nb = nblocks(n, 256);
+    if (nb > 65536u) nb = 65536u;
+    if (nb == 0 || count < 1) return;
+    Ptrs6<T> P{};
+    for (int i = 0; i < count && i < 6; i++) P.p[i] = p[i];
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3(nb, count < 6 ? count : 6), dim3(256), 0, s, P, n);
 }
 
 template <typename T>
 void launch_fill_zero(T *p, long long n, hipStream_t s)
 {
+    launch_fill_zero_n<T>(&p, 1, n, s);
+}
+
+template <typename T>
+void launch_copy_n(T *const *dst, T *const *src, int count, long long n, hipStream_t s)
+{
     unsigned nb = nblocks(n, 256);
     if (nb > 65536u) nb = 65536u;
-    if (nb == 0) return;
-    hipLaunchKernelGGL(k_fill_zero<T>, dim3(nb), dim3(256), 0, s, p, n);
+    if (nb == 0 || count < 1) return;
+    Ptrs6<T> D{}, S{};
+    for (int i = 0; i < count && i < 6; i++) { D.p[i] = dst[i]; S.p[i] = src[i]; }
+    hipLaunchKernelGGL(k_copy_n<T>, dim3(nb, count < 6 ? count : 6), dim3(256), 0, s, D, S, n);
 }
 
 template <typename R>
@@ -538,9 +575,13 @@ template void launch_add_current<double, double>(const GridP &, const PhysP &, d
                                                  double, hipStream_t);
 template void launch_add_current<float, double>(const GridP &, const PhysP &, float *const[3], double *const[3],
                                                 double, hipStream_t);
-template void launch_ghost_copy<double>(const GridP &, double *const[3], hipStream_t);
-template void launch_ghost_copy<float>(const GridP &, float *const[3], hipStream_t);
-template void launch_current_fold<double>(const GridP &, double *const[3], hipStream_t);
+template void launch_ghost_copy<double>(const GridP &, double *const[3], hipStream_t, double *const[3]);
+template void launch_ghost_copy<float>(const GridP &, float *const[3], hipStream_t, float *const[3]);
+template void launch_current_fold<double>(const GridP &, double *const[3], hipStream_t, double *const[3]);
+template void launch_fill_zero_n<double>(double *const *, int, long long, hipStream_t);
+template void launch_fill_zero_n<float>(float *const *, int, long long, hipStream_t);
+template void launch_copy_n<float>(float *const *, float *const *, int, long long, hipStream_t);
+template void launch_copy_n<double>(double *const *, double *const *, int, long long, hipStream_t);
 template void launch_fill_zero<double>(double *, long long, hipStream_t);
 template void launch_fill_zero<float>(float *, long long, hipStream_t);
 template void launch_fill_zero<long long>(long long *, long long, hipStream_t);

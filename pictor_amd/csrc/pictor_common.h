@@ -107,12 +107,22 @@ void launch_update_e_add_current(const GridP &g, const PhysP &ph, R *const E[3],
 template <typename R, typename JT>
 void launch_add_current(const GridP &g, const PhysP &ph, R *const E[3], JT *const J[3], double jscale,
                         hipStream_t s);
+// up to six arrays handed to one launch (blockIdx.y selects the array or set of three)
+template <typename T>
+struct Ptrs6 {
+    T *p[6];
+};
+// G / K: an optional second set of three arrays treated in the same launch
 template <typename R>
-void launch_ghost_copy(const GridP &g, R *const F[3], hipStream_t s);
+void launch_ghost_copy(const GridP &g, R *const F[3], hipStream_t s, R *const G[3] = nullptr);
 template <typename JT>
-void launch_current_fold(const GridP &g, JT *const J[3], hipStream_t s);
+void launch_current_fold(const GridP &g, JT *const J[3], hipStream_t s, JT *const K[3] = nullptr);
 template <typename T>
 void launch_fill_zero(T *p, long long n, hipStream_t s);
+template <typename T>
+void launch_fill_zero_n(T *const *p, int count, long long n, hipStream_t s);   // count <= 6
+template <typename T>
+void launch_copy_n(T *const *dst, T *const *src, int count, long long n, hipStream_t s);   // count <= 6
 template <typename R>
 void launch_field_energy(const GridP &g, R *const E[3], R *const B[3], double *partial, int nblk, hipStream_t s,
                          R *const *L = nullptr);
