@@ -418,6 +418,24 @@ def cyl_report(args):
     }
 
 
+def atomics_report(iters=4096):
+    """Atomic throughput behind the accumulator choices: the split push's current
+    window (LDS FP64), the adjoint's field-adjoint window (LDS FP64, FP32 only for
+    the 32 x 32 tile), the tile flush (global FP64)."""
+    import ctypes
+
+    from pictor_amd._lib import check, lib
+
+    out = (ctypes.c_double * 16)()
+    check(lib().pictor_probe_atomics(iters, out))
+    names = ("lds_f32_add", "lds_f64_add", "lds_u32_add", "lds_u64_add", "global_f64_add", "global_f32_add")
+    return {"workload": f"atomic adds at pseudo-random addresses, 4 workgroups x 256 lanes per CU, {iters} each; "
+                        "LDS window 32 KiB, global array 4 Mi elements",
+            "cus": int(out[6]), "clock_hz": out[7],
+            "lane_ops_per_s": {k: out[i] for i, k in enumerate(names)},
+            "lane_ops_per_cu_clock": {k: out[8 + i] for i, k in enumerate(names)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -445,6 +463,8 @@ def main():
     ap.add_argument("--adjoint-fd", type=int, default=1, help="finite-difference check of the drift gradient")
     ap.add_argument("--cyl", default=None, choices=sorted(CYL_CONFIGS),
                     help="instead of the Weibel step: time the cylindrical geometry on this grid")
+    ap.add_argument("--atomics", action="store_true",
+                    help="instead of the Weibel step: LDS versus global atomic throughput (pictor_probe_atomics)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -457,6 +477,9 @@ def main():
         return
     if args.cyl:
         print(json.dumps(cyl_report(args)), flush=True)
+        return
+    if args.atomics:
+        print(json.dumps(atomics_report()), flush=True)
         return
 
     import numpy as np  # noqa: F401

@@ -175,6 +175,14 @@ int pictor_adjoint_stats(pictor_ctx *ctx, double out[10]);
 int pictor_adjoint_engine(pictor_ctx *ctx, int32_t engine);
 int pictor_adjoint_release(pictor_ctx *ctx);
 
+/* ---- measurement (no counterpart in the reference) ----
+ * Atomic throughput on the current device: every lane of 4 workgroups per CU
+ * adds `iters` times at pseudo-random addresses.  out[0..5]: lane-ops per
+ * second of LDS FP32, LDS FP64, LDS u32, LDS u64 atomics (32 KiB window) and
+ * global FP64, FP32 atomics (4 Mi-element array); out[8..13] the same per CU
+ * per clock; out[6] CUs, out[7] clock (Hz). */
+int pictor_probe_atomics(int32_t iters, double out[16]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,7 +33,7 @@ EXPORTS = (
     "pictor_reset_current", "pictor_sort_tiles", "pictor_step", "pictor_synchronize",
     "pictor_diagnostics", "pictor_timer_reset", "pictor_timer_get", "pictor_memory_bytes",
     "pictor_setup_weibel", "pictor_adjoint_forward", "pictor_adjoint_backward", "pictor_adjoint_stats",
-    "pictor_adjoint_engine", "pictor_adjoint_release",
+    "pictor_adjoint_engine", "pictor_adjoint_release", "pictor_probe_atomics",
 )
 
 
@@ -95,6 +95,7 @@ def lib():
     L.pictor_adjoint_stats.argtypes = [c_vp, c_dp]
     L.pictor_adjoint_release.argtypes = [c_vp]
     L.pictor_adjoint_engine.argtypes = [c_vp, i32]
+    L.pictor_probe_atomics.argtypes = [i32, c_dp]
     for name in EXPORTS:
         if name != "pictor_last_error":
             getattr(L, name).restype = ctypes.c_int
