@@ -57,10 +57,11 @@ def measured_traffic(form, workload, tile, ndim):
     return None
 
 
-def run_gpu(form, ndim, nx, ny, nz, epc, P, steps, warmup, calibrate=False, tile=0):
+def run_gpu(form, ndim, nx, ny, nz, epc, P, steps, warmup, calibrate=False, tile=0, curr_filter=0):
     from pictor_amd import Simulation
 
-    sim = Simulation(nx, ny, nz, ndim=ndim, epc=epc, c=0.45, c_ompe=10.0, form=form, tile=tile)
+    sim = Simulation(nx, ny, nz, ndim=ndim, epc=epc, c=0.45, c_ompe=10.0, form=form, tile=tile,
+                     curr_filter=curr_filter)
     t0 = time.time()
     sim.upload_particles(**P)
     sim.synchronize()
@@ -447,6 +448,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--calibrate", action="store_true", help="also run plain copy kernels (counter scaling)")
     ap.add_argument("--tile", type=int, default=0, help="tile edge in cells (0 = library default)")
+    ap.add_argument("--curr-filter", type=int, default=0, help="passes of the 1-2-1 current filter per step")
     ap.add_argument("--accuracy", action="store_true",
                     help="instead of timing: run --steps steps of both forms (and the oracle unless "
                          "--cpu-steps 0) and report field energy, growth rate and Gauss residual errors")
@@ -497,7 +499,8 @@ def main():
     forms = [f for f in args.forms.split(",") if f]
     res = {}
     for form in forms:
-        res[form] = run_gpu(form, ndim, nx, ny, nz, epc, P, args.steps, args.warmup, args.calibrate, args.tile)
+        res[form] = run_gpu(form, ndim, nx, ny, nz, epc, P, args.steps, args.warmup, args.calibrate, args.tile,
+                            args.curr_filter)
         r = res[form]
         r["roofline"]["traffic"] = measured_traffic(form, args.config, r["tile"], ndim)
         print(f"# {form}: {r['value']:.4e} particle-updates/s, {r['ms_per_step']:.3f} ms/step, "
@@ -533,7 +536,7 @@ def main():
                                + ("x".join(str(v) for v in ((nx, ny, nz) if ndim == 3 else (nx, ny))))
                                + f", {epc} ppc/species, 2 species, {r['particles']} particles, drift +-{drift}c, "
                                f"thermal {uth}c, c=0.45, c/wp=10 cells",
-                   "form": head, "tile": r["tile"], "particles": r["particles"]},
+                   "form": head, "tile": r["tile"], "particles": r["particles"], "curr_filter": args.curr_filter},
         "roofline": r["roofline"],
         "cpu_baseline": cpu,
     }

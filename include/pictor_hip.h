@@ -50,7 +50,7 @@ extern "C" {
 
 /* Run configuration: the compile-time parameters of parameters.F90:5-87 and
  * src/vars.F90:10-18 as a runtime struct.  grid_dx = grid_dy = grid_dz = 1,
- * EMspeedBoost = 1, curr_filter = 0, periodic faces on every side. */
+ * EMspeedBoost = 1, periodic faces on every side. */
 typedef struct pictor_config {
     int32_t ndim;          /* 2 (-DtwoD) or 3                                */
     int32_t nx, ny, nz;    /* physical cells; nz ignored in 2-D              */
@@ -59,7 +59,8 @@ typedef struct pictor_config {
     int32_t tile;          /* tile edge in cells (0 = choose); must divide nx, ny (nz) */
     int32_t fdtd;          /* PICTOR_FDTD_*: the B half step of em_update.F90 or em_update_lehe.F90 */
     int32_t geometry;      /* PICTOR_GEOM_* (the reference's -Dcyl switch)   */
-    int32_t reserved;      /* 0 */
+    int32_t curr_filter;   /* passes of the 1-2-1 current filter per step (parameters.F90:22,
+                              smoothen_current current.F90:19-30); Cartesian only */
     double c;              /* parameters.F90:20 */
     double c_ompe;         /* parameters.F90:19 */
     double g0;             /* parameters.F90:16 */
@@ -109,6 +110,12 @@ int pictor_exchange_particles(pictor_ctx *ctx);
 int pictor_update_e(pictor_ctx *ctx);
 /* UpdateCurrentAllFaces: comm_fld.F90:15-72 (ghost current added into owners) */
 int pictor_current_halo_add(pictor_ctx *ctx);
+/* smoothen_current: current.F90:19-30 (MAIN.F90:158-162), cfg.curr_filter passes of
+ * the 1-2-1 moving average over the physical cells of the periodic box, along x,
+ * y (and z) in turn.  The physical cells match the reference bit for bit in the
+ * FP64 form; J's ghost layers are left as they were (nothing reads them before
+ * the next pictor_reset_current).  No-op for curr_filter = 0. */
+int pictor_smoothen_current(pictor_ctx *ctx);
 /* AddCurrent + SendRecvFlds(E): em_update.F90:79-102 */
 int pictor_add_current(pictor_ctx *ctx);
 /* SendRecvFlds on E (which = 0) or B (which = 1): comm_fld.F90:75-130 */

@@ -408,6 +408,56 @@ void orc_current_halo_add(const orc_params *p, double *jx, double *jy, double *j
 }
 
 /* ------------------------------------------------------------------ */
+/* smoothen_current, current.F90:19-30.  Each pass is MovingAverageFilter */
+/* (:55-75): SendRecvFldsYZ then MovingAverageX (:77-104) over            */
+/* i = 2..mx-2, j = 2..my-2, k = k1..k2; SendRecvFldsZX then              */
+/* MovingAverageY (:106-134); in 3-D SendRecvFldsXY then MovingAverageZ   */
+/* (:136-163) over k = 2..mz-2 (k1..k2 = 2..mz-2 in 3-D, 1..1 in 2-D).    */
+/* Every component is filtered into a scratch array over the box and     */
+/* copied back (Fld_filtered).  Weights wtm1, wt0, wtp1 = 1/4, 1/2, 1/4   */
+/* (vars.F90:208-210).  CurrentFilter_BC (:165-) folds currents only at   */
+/* particle boundaries, which a periodic box does not have.               */
+/* ------------------------------------------------------------------ */
+
+static void moving_average_1(const orc_params *p, double *F, double *T, int axis, int k1, int k2)
+{
+    const double wtm1 = 0.25, wt0 = 0.5, wtp1 = 0.25;
+    const int i1 = 2, i2 = p->mx - 2, j1 = 2, j2 = p->my - 2;
+    for (int k = k1; k <= k2; k++)
+        for (int j = j1; j <= j2; j++)
+            for (int i = i1; i <= i2; i++) {
+                int m[3] = {i, j, k}, q[3] = {i, j, k};
+                m[axis] -= 1;
+                q[axis] += 1;
+                T[IX(p, i, j, k)] = wtm1 * F[IX(p, m[0], m[1], m[2])] + wt0 * F[IX(p, i, j, k)] +
+                                    wtp1 * F[IX(p, q[0], q[1], q[2])];
+            }
+    for (int k = k1; k <= k2; k++)
+        for (int j = j1; j <= j2; j++)
+            for (int i = i1; i <= i2; i++) F[IX(p, i, j, k)] = T[IX(p, i, j, k)];
+}
+
+void orc_smoothen_current(const orc_params *p, orc_fields *f)
+{
+    if (p->curr_filter <= 0) return;
+    double *T = (double *)calloc((size_t)p->mx * p->my * p->mz, sizeof(double));
+    if (!T) return;
+    double *J[3] = {f->Jx, f->Jy, f->Jz};
+    const int k1 = (p->ndim == 3) ? 2 : 1, k2 = (p->ndim == 3) ? p->mz - 2 : 1;
+    for (int ni = 1; ni <= p->curr_filter; ni++) {
+        for (int c = 0; c < 3; c++) halo_copy_axis(p, J[c], 0);
+        for (int c = 0; c < 3; c++) moving_average_1(p, J[c], T, 0, k1, k2);
+        for (int c = 0; c < 3; c++) halo_copy_axis(p, J[c], 1);
+        for (int c = 0; c < 3; c++) moving_average_1(p, J[c], T, 1, k1, k2);
+        if (p->ndim == 3) {
+            for (int c = 0; c < 3; c++) halo_copy_axis(p, J[c], 2);
+            for (int c = 0; c < 3; c++) moving_average_1(p, J[c], T, 2, 2, p->mz - 2);
+        }
+    }
+    free(T);
+}
+
+/* ------------------------------------------------------------------ */
 /* Gather of node-averaged corner values, movdep_routines.F90:6-223     */
 /* ------------------------------------------------------------------ */
 
@@ -936,6 +986,7 @@ int orc_step(const orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, in
     orc_update_e(p, f);
     orc_field_halo(p, f->Ex, f->Ey, f->Ez);
     orc_current_halo_add(p, f->Jx, f->Jy, f->Jz);
+    orc_smoothen_current(p, f);
     orc_add_current(p, f);
     orc_field_halo(p, f->Ex, f->Ey, f->Ez);
     if (p->reorder_period > 0 && t % p->reorder_period == 0) rc = orc_reorder(p, s);

@@ -35,7 +35,7 @@ typedef struct orc_params {
     int32_t reorder_period;  /* parameters.F90:71 */
     double flvrqm[8];        /* initialise.F90:84-85 */
     int32_t fdtd;            /* 0 = Yee (em_update.F90), 1 = Lehe (em_update_lehe.F90) */
-    int32_t pad_;
+    int32_t curr_filter;     /* passes of smoothen_current per step, parameters.F90:22 */
     /* em_update_lehe.F90:8-25: alpha_x, alpha_y, alpha_z, beta_xy, beta_xz, beta_yx,
      * beta_yz, beta_zx, beta_zy, delta_x, delta_y, delta_z */
     double lehe[12];
@@ -75,6 +75,10 @@ void orc_reset_current(const orc_params *p, orc_fields *f);
 /* Periodic single-rank halos, comm_fld.F90 */
 void orc_field_halo(const orc_params *p, double *fx, double *fy, double *fz);
 void orc_current_halo_add(const orc_params *p, double *jx, double *jy, double *jz);
+/* smoothen_current, current.F90:19-30: p->curr_filter passes of the 1-2-1
+ * moving average (MovingAverageFilter, :55-75) with the per-axis periodic
+ * ghost exchanges of one rank */
+void orc_smoothen_current(const orc_params *p, orc_fields *f);
 /* Particle mover, movdep.F90 + movdep_routines.F90 */
 void orc_gather_vec_e(const orc_params *p, orc_ws *w, const orc_fields *f);
 void orc_gather_vec_b(const orc_params *p, orc_ws *w, const orc_fields *f);

@@ -30,7 +30,7 @@ class OrcParams(ctypes.Structure):
             "xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "xlen", "ylen", "zlen",
             "Bx_ext0", "By_ext0", "Bz_ext0")]
         + [("nflvr", ctypes.c_int32), ("reorder_period", ctypes.c_int32), ("flvrqm", ctypes.c_double * 8)]
-        + [("fdtd", ctypes.c_int32), ("pad_", ctypes.c_int32), ("lehe", ctypes.c_double * 12)]
+        + [("fdtd", ctypes.c_int32), ("curr_filter", ctypes.c_int32), ("lehe", ctypes.c_double * 12)]
     )
 
 
@@ -73,6 +73,7 @@ def lib():
             getattr(L, name).argtypes = [P, F]
         L.orc_field_halo.argtypes = [P, c_dp, c_dp, c_dp]
         L.orc_current_halo_add.argtypes = [P, c_dp, c_dp, c_dp]
+        L.orc_smoothen_current.argtypes = [P, F]
         L.orc_gather_vec_e.argtypes = [P, W, F]
         L.orc_gather_vec_b.argtypes = [P, W, F]
         L.orc_move_deposit.argtypes = [P, W, F, S]
@@ -112,7 +113,7 @@ class Oracle:
     FIELD_NAMES = ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz")
 
     def __init__(self, ndim=2, nx=16, ny=16, nz=1, c=0.45, c_ompe=10.0, g0=1.0, mi_me=1.0, epc=16,
-                 nthreads=1, cap=1024, reorder_period=20, fdtd="yee"):
+                 nthreads=1, cap=1024, reorder_period=20, fdtd="yee", curr_filter=0):
         L = lib()
         self.p = OrcParams()
         rc = L.orc_params_init(ctypes.byref(self.p), ndim, nx, ny, nz, c, c_ompe, g0, mi_me, epc)
@@ -123,6 +124,7 @@ class Oracle:
         elif fdtd != "yee":
             raise ValueError("fdtd must be 'yee' or 'lehe'")
         self.p.reorder_period = reorder_period
+        self.p.curr_filter = curr_filter
         self.shape = (self.p.mx, self.p.my, self.p.mz)
         self.F = {n: np.zeros(self.shape, order="F") for n in self.FIELD_NAMES}
         self.f = OrcFields(*[_ptr(self.F[n]) for n in self.FIELD_NAMES])
@@ -191,6 +193,9 @@ class Oracle:
 
     def current_halo_add(self):
         lib().orc_current_halo_add(ctypes.byref(self.p), *[_ptr(self.F[n]) for n in ("Jx", "Jy", "Jz")])
+
+    def smoothen_current(self):
+        lib().orc_smoothen_current(ctypes.byref(self.p), ctypes.byref(self.f))
 
     def move_deposit(self):
         lib().orc_move_deposit(ctypes.byref(self.p), self.ws, ctypes.byref(self.f), ctypes.byref(self.s))

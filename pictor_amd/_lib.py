@@ -29,7 +29,7 @@ EXPORTS = (
     "pictor_upload_fields", "pictor_download_fields", "pictor_upload_particles",
     "pictor_particle_count", "pictor_download_particles", "pictor_update_b_half",
     "pictor_move_deposit", "pictor_exchange_particles", "pictor_update_e",
-    "pictor_current_halo_add", "pictor_add_current", "pictor_field_halo",
+    "pictor_current_halo_add", "pictor_smoothen_current", "pictor_add_current", "pictor_field_halo",
     "pictor_reset_current", "pictor_sort_tiles", "pictor_step", "pictor_synchronize",
     "pictor_diagnostics", "pictor_timer_reset", "pictor_timer_get", "pictor_memory_bytes",
     "pictor_setup_weibel", "pictor_adjoint_forward", "pictor_adjoint_backward", "pictor_adjoint_stats",
@@ -41,7 +41,7 @@ class PictorConfig(ctypes.Structure):
     _fields_ = [
         ("ndim", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
         ("epc", ctypes.c_int32), ("form", ctypes.c_int32), ("tile", ctypes.c_int32), ("fdtd", ctypes.c_int32),
-        ("geometry", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("geometry", ctypes.c_int32), ("curr_filter", ctypes.c_int32),
         ("c", ctypes.c_double), ("c_ompe", ctypes.c_double), ("g0", ctypes.c_double), ("mi_me", ctypes.c_double),
         ("capacity_factor", ctypes.c_double),
     ]
@@ -80,7 +80,7 @@ def lib():
     L.pictor_particle_count.argtypes = [c_vp, ctypes.POINTER(i64)]
     L.pictor_download_particles.argtypes = [c_vp, i64] + [c_dp] * 7 + [c_ip, c_ip, ctypes.POINTER(i64)]
     for name in ("pictor_update_b_half", "pictor_move_deposit", "pictor_exchange_particles", "pictor_update_e",
-                 "pictor_current_halo_add", "pictor_add_current", "pictor_reset_current", "pictor_sort_tiles",
+                 "pictor_current_halo_add", "pictor_smoothen_current", "pictor_add_current", "pictor_reset_current", "pictor_sort_tiles",
                  "pictor_synchronize", "pictor_timer_reset"):
         getattr(L, name).argtypes = [c_vp]
     L.pictor_field_halo.argtypes = [c_vp, i32]

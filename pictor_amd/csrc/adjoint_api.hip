@@ -133,6 +133,10 @@ struct AdjEngine : AdjEngineBase {
     float *Elo[3] = {};
     std::vector<float *> ckElo;
     double *Eb[3] = {}, *Bb[3] = {}, *gE[3] = {}, *gB[3] = {};
+    // current filter (curr_filter > 0): scratch, and the filtered adjoint current
+    // the particle reverse step reads (Jadj = Ebf, else Eb)
+    double *Jt[3] = {}, *Ebf[3] = {};
+    double *Jadj[3] = {};
     double *partial = nullptr;
     int *d_err = nullptr;
     double objective = 0;
@@ -173,7 +177,9 @@ struct AdjEngine : AdjEngineBase {
         if (c && c->stream) (void)hipStreamSynchronize(c->stream);
         for (int i = 0; i < 3; i++) {
             dfree(E[i]); dfree(B[i]); dfree(J[i]); dfree(Eb[i]); dfree(Bb[i]); dfree(gE[i]); dfree(gB[i]);
+            dfree(Jt[i]); dfree(Ebf[i]);
             E[i] = B[i] = nullptr; J[i] = Eb[i] = Bb[i] = gE[i] = gB[i] = nullptr;
+            Jt[i] = Ebf[i] = Jadj[i] = nullptr;
         }
         for (R *p : ckE) dfree(p);
         for (R *p : ckB) dfree(p);
@@ -236,8 +242,10 @@ struct AdjEngine : AdjEngineBase {
             HIPTRY(dalloc(&E[i], ncell)); HIPTRY(dalloc(&B[i], ncell)); HIPTRY(dalloc(&J[i], ncell));
             HIPTRY(dalloc(&Eb[i], ncell)); HIPTRY(dalloc(&Bb[i], ncell));
             HIPTRY(dalloc(&gE[i], ncell)); HIPTRY(dalloc(&gB[i], ncell));
+            if (c->curr_filter > 0) { HIPTRY(dalloc(&Jt[i], ncell)); HIPTRY(dalloc(&Ebf[i], ncell)); }
+            Jadj[i] = c->curr_filter > 0 ? Ebf[i] : Eb[i];
         }
-        bytes += ncell * (6 * (long long)sizeof(R) + 15 * 8);
+        bytes += ncell * (6 * (long long)sizeof(R) + 15 * 8 + (c->curr_filter > 0 ? 48 : 0));
         ckE.assign((size_t)nseg * 3, nullptr);
         ckB.assign((size_t)nseg * 3, nullptr);
         for (size_t i = 0; i < ckE.size(); i++) {
@@ -437,6 +445,7 @@ struct AdjEngine : AdjEngineBase {
         launch_update_b_half<float>(c->g, c->ph, fE, fB, s);
         launch_ghost_copy<float>(c->g, fB, s);
         launch_current_fold<double>(c->g, J, s);
+        if (c->curr_filter > 0) launch_smoothen_current(c->g, c->curr_filter, J, Jt, s);
         launch_update_e_pair(c->g, c->ph, fE, Elo, fB, J, ctx_jscale, true, true, s);
         launch_ghost_copy<float>(c->g, fE, s, Elo);
         HIPTRY(hipGetLastError());
@@ -480,6 +489,7 @@ struct AdjEngine : AdjEngineBase {
         launch_update_b_half<R>(c->g, c->ph, E, B, s);
         launch_ghost_copy<R>(c->g, B, s);
         launch_current_fold<double>(c->g, J, s);
+        if (c->curr_filter > 0) launch_smoothen_current(c->g, c->curr_filter, J, Jt, s);
         if constexpr (std::is_same<R, float>::value) {
             launch_update_e_pair(c->g, c->ph, E, Elo, B, J, ctx_jscale, true, true, s);
             launch_ghost_copy<float>(c->g, Elo, s);
@@ -506,6 +516,13 @@ struct AdjEngine : AdjEngineBase {
         launch_update_b_half<double>(g, pa, Eb, Bb, s);
         double *gEB[6] = {gE[0], gE[1], gE[2], gB[0], gB[1], gB[2]};
         launch_fill_zero_n<double>(gEB, 6, ncell, s);
+        if (c->curr_filter > 0) {
+            // the adjoint current is F^T(-jscale Eb) = F(-jscale Eb): the periodic 1-2-1
+            // filter is symmetric; its ghosts are the images (the fold's transpose)
+            launch_copy_n<double>(Ebf, Eb, 3, ncell, s);
+            launch_smoothen_current(g, c->curr_filter, Ebf, Jt, s);
+            launch_ghost_copy<double>(g, Ebf, s);
+        }
         particle_adjoint();
         launch_ghost_copy<double>(g, Bb, s);
         launch_update_e<double>(g, pb, Eb, Bb, s);
@@ -648,7 +665,7 @@ struct AdjEngine : AdjEngineBase {
                     float *fE[3] = {sfE[3 * j], sfE[3 * j + 1], sfE[3 * j + 2]};
                     float *fB[3] = {sfB[3 * j], sfB[3 * j + 1], sfB[3 * j + 2]};
                     rc = rev_step([&]() {
-                        launch_fa_adjoint(c->g, tp, ap, fE, fB, Eb, d_base, C(j), S(j), dstv[j], fa[src], fa[1 - src],
+                        launch_fa_adjoint(c->g, tp, ap, fE, fB, Jadj, d_base, C(j), S(j), dstv[j], fa[src], fa[1 - src],
                                           gE, gB, s);
                     });
                     if (rc) return rc;
@@ -661,7 +678,7 @@ struct AdjEngine : AdjEngineBase {
                     if ((rc = fwd_step(S(j), st[j + 1], j))) return rc;
                 HIPTRY(hipEventRecord(ev[1], s));
                 for (int j = L - 1; j >= 0; j--) {
-                    rc = rev_step([&]() { launch_eng_adjoint<R>(ap, nodE[j], nodB[j], S(j), Eb, adj, gE, gB, n, s); });
+                    rc = rev_step([&]() { launch_eng_adjoint<R>(ap, nodE[j], nodB[j], S(j), Jadj, adj, gE, gB, n, s); });
                     if (rc) return rc;
                 }
             }

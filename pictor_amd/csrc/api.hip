@@ -348,6 +348,15 @@ int do_fold(pictor_ctx *c)
     return PICTOR_OK;
 }
 
+int do_smoothen(pictor_ctx *c)
+{
+    if (c->curr_filter <= 0) return PICTOR_OK;
+    double *J[3] = {(double *)c->J[0], (double *)c->J[1], (double *)c->J[2]};
+    launch_smoothen_current(c->g, c->curr_filter, J, c->Jt, c->stream);
+    HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
 double jscale(const pictor_ctx *c) { return c->ph.qi * kFixInv; }
 
 int do_e_phase(pictor_ctx *c, bool curl, bool add)
@@ -420,6 +429,8 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     if (cfg->geometry != PICTOR_GEOM_CARTESIAN && cfg->geometry != PICTOR_GEOM_CYL)
         return fail(PICTOR_ERR_ARG, "unknown geometry");
     const bool cyl = cfg->geometry == PICTOR_GEOM_CYL;
+    if (cfg->curr_filter < 0 || cfg->curr_filter > 64) return fail(PICTOR_ERR_ARG, "curr_filter must be 0..64");
+    if (cyl && cfg->curr_filter) return fail(PICTOR_ERR_ARG, "the current filter is for the Cartesian geometry");
     if (cyl && cfg->form != PICTOR_FORM_FP64) return fail(PICTOR_ERR_ARG, "the cylindrical geometry runs in the FP64 form");
     if (cyl && cfg->ny % 2) return fail(PICTOR_ERR_ARG, "the cylindrical grid needs an even number of theta cells (ny)");
     if (cyl) {
@@ -526,7 +537,10 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
                 return cleanup_fail(fail(PICTOR_ERR_HIP, "field allocation failed"));
             hipMemsetAsync(c->Elo[i], 0, g.ncell * sizeof(float), c->stream);
         }
+        if (cfg->curr_filter > 0 && hipMalloc(&c->Jt[i], g.ncell * 8) != hipSuccess)
+            return cleanup_fail(fail(PICTOR_ERR_HIP, "field allocation failed"));
     }
+    c->curr_filter = cfg->curr_filter;
     const size_t nb = (size_t)tp.ntiles * tp.NB;
     if (dalloc(&c->d_base, nb) || dalloc(&c->d_cap, nb) || dalloc(&c->d_cnt[0], nb) || dalloc(&c->d_cnt[1], nb) ||
         dalloc(&c->d_spill_count, 1) || dalloc(&c->d_err, 1))
@@ -567,7 +581,7 @@ int pictor_finalize(pictor_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     delete c->adj;
     c->adj = nullptr;
-    for (int i = 0; i < 3; i++) { dfree(c->E[i]); dfree(c->B[i]); dfree(c->J[i]); dfree(c->Elo[i]); }
+    for (int i = 0; i < 3; i++) { dfree(c->E[i]); dfree(c->B[i]); dfree(c->J[i]); dfree(c->Elo[i]); dfree(c->Jt[i]); }
     dfree(c->d_base); dfree(c->d_cap); dfree(c->d_cnt[0]); dfree(c->d_cnt[1]);
     dfree(c->d_spill_count); dfree(c->d_err);
     for (int i = 0; i < 2; i++) { free_f64(c->pf[i]); free_split(c->ps[i]); }
@@ -814,6 +828,12 @@ int pictor_current_halo_add(pictor_ctx *c)
     return rc ? rc : (c->cyl ? cyl_current_halo(c) : do_fold(c));
 }
 
+int pictor_smoothen_current(pictor_ctx *c)
+{
+    int rc = check_ctx(c);
+    return rc ? rc : do_smoothen(c);
+}
+
 int pictor_add_current(pictor_ctx *c)
 {
     int rc = check_ctx(c);
@@ -883,6 +903,7 @@ int pictor_step(pictor_ctx *c, int32_t nsteps)
         if ((rc = do_push(c, true))) return rc;
         if ((rc = do_b_half(c))) return rc;
         if ((rc = do_fold(c))) return rc;
+        if ((rc = do_smoothen(c))) return rc;
         if ((rc = do_e_phase(c, true, true))) return rc;
         HIPTRY(hipEventRecord(c->ev[3], c->stream));
         HIPTRY(hipEventRecord(c->ev[5], c->stream));

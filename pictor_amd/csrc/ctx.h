@@ -64,6 +64,9 @@ struct pictor_ctx {
     void *J[3] = {nullptr, nullptr, nullptr};
     // split form: the low FP32 part of E (E = hi + lo, hi in E[]), see launch_update_e_pair
     float *Elo[3] = {nullptr, nullptr, nullptr};
+    // smoothen_current: passes per step and the filter's scratch current
+    int curr_filter = 0;
+    double *Jt[3] = {nullptr, nullptr, nullptr};
 
     // tile bins
     std::vector<long long> h_base;

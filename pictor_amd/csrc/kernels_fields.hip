@@ -299,6 +299,38 @@ __global__ __launch_bounds__(256) void k_current_fold(GridP g, int axis, Ptrs6<J
     J2[o] = J2[o] + J2[s];
 }
 
+// One axis pass of the 1-2-1 current filter (MovingAverageX1/Y1/Z1,
+// current.F90:90-163, weights vars.F90:208-210) from src to dst over the
+// physical cells, the neighbours across the periodic boundary read at their
+// images (what SendRecvFldsYZ/ZX/XY put in the ghosts on one rank).  Every
+// other cell is copied, so dst is a complete array.  blockIdx.y selects the
+// component.
+__global__ __launch_bounds__(256) void k_filter_axis(GridP g, int axis, Ptrs6<double> P)
+{
+    const double *__restrict__ src = P.p[blockIdx.y];
+    double *__restrict__ dst = P.p[3 + blockIdx.y];
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.ncell) return;
+    const int i = (int)(t % g.mx) + 1;
+    const long long r = t / g.mx;
+    const int j = (int)(r % g.my) + 1;
+    const int k = (int)(r / g.my) + 1;
+    const bool phys = i >= 3 && i <= g.mx - 3 && j >= 3 && j <= g.my - 3 &&
+                      (g.ndim == 2 || (k >= 3 && k <= g.mz - 3));
+    if (!phys) {
+        dst[t] = src[t];
+        return;
+    }
+    int c = i, n = g.nx;
+    long long stride = 1;
+    if (axis == 1) { c = j; n = g.ny; stride = g.mx; }
+    else if (axis == 2) { c = k; n = g.nz; stride = (long long)g.mx * g.my; }
+    const long long lo = (c == 3) ? t + (long long)(n - 1) * stride : t - stride;
+    const long long hi = (c == 2 + n) ? t - (long long)(n - 1) * stride : t + stride;
+    const double wtm1 = 0.25, wt0 = 0.5, wtp1 = 0.25;
+    dst[t] = wtm1 * src[lo] + wt0 * src[t] + wtp1 * src[hi];
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_fill_zero(Ptrs6<T> P, long long n)
 {
@@ -473,6 +505,20 @@ void launch_current_fold(const GridP &g, JT *const J[3], hipStream_t s, JT *cons
         long long total = ext[0] * ext[1] * ext[2];
         hipLaunchKernelGGL(k_current_fold<JT>, dim3(nblocks(total, 256), K ? 2 : 1), dim3(256), 0, s, g, a, P);
     }
+}
+
+void launch_smoothen_current(const GridP &g, int passes, double *const J[3], double *const T[3], hipStream_t s)
+{
+    const int naxes = (g.ndim == 3) ? 3 : 2;
+    Ptrs6<double> fwd{{J[0], J[1], J[2], T[0], T[1], T[2]}}, back{{T[0], T[1], T[2], J[0], J[1], J[2]}};
+    bool in_t = false;   // where the current values are
+    for (int ni = 0; ni < passes; ni++)
+        for (int a = 0; a < naxes; a++) {
+            hipLaunchKernelGGL(k_filter_axis, dim3(nblocks(g.ncell, 256), 3), dim3(256), 0, s, g, a,
+                               in_t ? back : fwd);
+            in_t = !in_t;
+        }
+    if (in_t) launch_copy_n<double>(J, T, 3, g.ncell, s);
 }
 
 template <typename T>

@@ -123,6 +123,9 @@ template <typename T>
 void launch_fill_zero_n(T *const *p, int count, long long n, hipStream_t s);   // count <= 6
 template <typename T>
 void launch_copy_n(T *const *dst, T *const *src, int count, long long n, hipStream_t s);   // count <= 6
+// smoothen_current: `passes` rounds of the 1-2-1 filter along each axis, J in
+// place, T (three arrays of ncell) as scratch
+void launch_smoothen_current(const GridP &g, int passes, double *const J[3], double *const T[3], hipStream_t s);
 template <typename R>
 void launch_field_energy(const GridP &g, R *const E[3], R *const B[3], double *partial, int nblk, hipStream_t s,
                          R *const *L = nullptr);

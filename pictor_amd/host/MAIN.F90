@@ -2,9 +2,9 @@
 !
 ! Each phase subroutine below keeps the name and position of the call it
 ! replaces in the reference loop and forwards to the C ABI.  Phases the
-! periodic one-GPU run does not need (BCs, output, load balancing, filters)
-! are left out.  Usage:  pictor_main [name=value ...]  with names from
-! parameters.F90 (nx, ny, epc, tfinish, form, fused, fdtd, seed, c, c_ompe,
+! periodic one-GPU run does not need (BCs, output, load balancing) are left
+! out.  Usage:  pictor_main [name=value ...]  with names from parameters.F90
+! (nx, ny, epc, tfinish, form, fused, fdtd, curr_filter, seed, c, c_ompe,
 ! energy_save_period, prtl_reorder_period).
 program MAIN
    use iso_c_binding
@@ -30,6 +30,7 @@ program MAIN
          call UpdateBfieldHalf          ! MAIN.F90:103/114
          call UpdateEfield              ! MAIN.F90:108/115
          call UpdateCurrentAllFaces     ! MAIN.F90:124/129
+         call smoothen_current          ! MAIN.F90:158/162
          call AddCurrent                ! MAIN.F90:171/174, SendRecvFlds(E) :184/190
          call ResetCurrent              ! MAIN.F90:248/251
          call ReorderPrtl               ! MAIN.F90:261/264
@@ -71,6 +72,7 @@ contains
          case ('form'); read (val, *, iostat=ios) form
          case ('fused'); read (val, *, iostat=ios) fused
          case ('fdtd'); read (val, *, iostat=ios) fdtd
+         case ('curr_filter'); read (val, *, iostat=ios) curr_filter
          case ('seed'); read (val, *, iostat=ios) seed
          case ('c'); read (val, *, iostat=ios) c
          case ('c_ompe'); read (val, *, iostat=ios) c_ompe
@@ -95,7 +97,8 @@ contains
       allocate (Jx(mx, my, mz), Jy(mx, my, mz), Jz(mx, my, mz))
       Ex = 0; Ey = 0; Ez = 0; Bx = 0; By = 0; Bz = 0; Jx = 0; Jy = 0; Jz = 0
       call InitSetup
-      cfg = pictor_config(2, nx, ny, 1, epc, form, 0, fdtd, PICTOR_GEOM_CARTESIAN, 0, c, c_ompe, g0, mi_me, 0.0d0)
+      cfg = pictor_config(2, nx, ny, 1, epc, form, 0, fdtd, PICTOR_GEOM_CARTESIAN, curr_filter, c, c_ompe, g0, mi_me, &
+                          0.0d0)
       call check(pictor_init(cfg, gpu), 'pictor_init')
       call check(pictor_upload_fields(gpu, Ex, Ey, Ez, Bx, By, Bz), 'pictor_upload_fields')
       call check(pictor_upload_particles(gpu, used_prtl_arr_size, qp, xp, yp, zp, up, vp, wp, flvp), &
@@ -122,6 +125,10 @@ contains
    subroutine UpdateCurrentAllFaces
       call check(pictor_current_halo_add(gpu), 'UpdateCurrentAllFaces')
    end subroutine UpdateCurrentAllFaces
+
+   subroutine smoothen_current
+      call check(pictor_smoothen_current(gpu), 'smoothen_current')
+   end subroutine smoothen_current
 
    subroutine AddCurrent
       call check(pictor_add_current(gpu), 'AddCurrent')

@@ -17,7 +17,7 @@ module parameters
    real(psn) :: mi_me = 1.0_psn                      ! pair plasma
    real(psn) :: c_ompe = 10.0_psn                    ! cells per skin depth
    real(psn) :: c = 0.45_psn                         ! Courant number
-   integer, parameter :: curr_filter = 0
+   integer :: curr_filter = 0                        ! passes of the 1-2-1 current filter per step
    integer :: prtl_reorder_period = 20               ! counting sort into tiles every 20 steps
    integer :: energy_save_period = 10                ! print CalcEnergy every this many steps
    ! run-time choices of the GPU path

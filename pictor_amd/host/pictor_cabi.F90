@@ -18,7 +18,7 @@ module pictor_cabi
    integer(c_int32_t), parameter :: PICTOR_GEOM_CYL = 1
 
    type, bind(C) :: pictor_config
-      integer(c_int32_t) :: ndim, nx, ny, nz, epc, form, tile, fdtd, geometry, reserved
+      integer(c_int32_t) :: ndim, nx, ny, nz, epc, form, tile, fdtd, geometry, curr_filter
       real(c_double) :: c, c_ompe, g0, mi_me, capacity_factor
    end type pictor_config
 
@@ -91,6 +91,10 @@ module pictor_cabi
          type(c_ptr), value :: ctx
       end function
       integer(c_int) function pictor_current_halo_add(ctx) bind(C, name="pictor_current_halo_add")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function pictor_smoothen_current(ctx) bind(C, name="pictor_smoothen_current")
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
       end function

@@ -34,7 +34,7 @@ class Simulation:
     """
 
     def __init__(self, nx, ny, nz=1, ndim=2, epc=16, c=0.45, c_ompe=10.0, g0=1.0, mi_me=1.0,
-                 form="fp64", tile=0, capacity_factor=0.0, fdtd="yee", geometry="cartesian"):
+                 form="fp64", tile=0, capacity_factor=0.0, fdtd="yee", geometry="cartesian", curr_filter=0):
         if fdtd not in ("yee", "lehe"):
             raise ValueError("fdtd must be 'yee' or 'lehe'")
         if geometry not in ("cartesian", "cyl"):
@@ -42,7 +42,7 @@ class Simulation:
         self.geometry = geometry
         self.cfg = PictorConfig(ndim, nx, ny, nz, epc, FORM_FP64 if form == "fp64" else FORM_FP32_SPLIT,
                                 tile, FDTD_LEHE if fdtd == "lehe" else FDTD_YEE,
-                                GEOM_CYL if geometry == "cyl" else GEOM_CARTESIAN, 0, c, c_ompe, g0, mi_me,
+                                GEOM_CYL if geometry == "cyl" else GEOM_CARTESIAN, curr_filter, c, c_ompe, g0, mi_me,
                                 capacity_factor)
         if form not in ("fp64", "fp32_split"):
             raise ValueError("form must be 'fp64' or 'fp32_split'")
@@ -127,6 +127,9 @@ class Simulation:
     def UpdateCurrentAllFaces(self):     # comm_fld.F90:15-33
         check(lib().pictor_current_halo_add(self._ctx))
 
+    def smoothen_current(self):          # current.F90:19-30 (curr_filter passes)
+        check(lib().pictor_smoothen_current(self._ctx))
+
     def AddCurrent(self):                # em_update.F90:79-102 + SendRecvFlds(E)
         check(lib().pictor_add_current(self._ctx))
 
@@ -148,6 +151,7 @@ class Simulation:
         self.UpdateBfieldHalf()
         self.UpdateEfield()
         self.UpdateCurrentAllFaces()
+        self.smoothen_current()
         self.AddCurrent()
         self.t += 1
 

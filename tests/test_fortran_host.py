@@ -36,18 +36,19 @@ def _energies(out):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("fused", [0, 1])
-def test_fortran_host_matches_python_host(fused):
+@pytest.mark.parametrize("fused,curr_filter", [(0, 0), (1, 0), (0, 2)])
+def test_fortran_host_matches_python_host(fused, curr_filter):
     exe = os.path.join(HOST, "pictor_main")
     assert os.path.exists(exe), "pictor_main is not built (__graft_entry__.build)"
-    print(f"[case] fortran host fused={fused}", flush=True)
+    print(f"[case] fortran host fused={fused} curr_filter={curr_filter}", flush=True)
     r = subprocess.run([exe, "nx=32", "ny=32", "epc=8", "tfinish=20", "form=1", f"fused={fused}",
-                        "energy_save_period=10", "seed=7"], capture_output=True, text=True, timeout=120)
+                        f"curr_filter={curr_filter}", "energy_save_period=10", "seed=7"],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     F = _energies(r.stdout)
     from pictor_amd import Simulation, setup_weibel
     P = setup_weibel(32, 32, epc=8, drift=0.5, uth=0.01, seed=7)
-    with Simulation(32, 32, epc=8, form="fp32_split") as sim:
+    with Simulation(32, 32, epc=8, form="fp32_split", curr_filter=curr_filter) as sim:
         sim.upload_particles(**P)
         for t in (10, 20):
             sim.step(10)
