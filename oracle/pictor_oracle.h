@@ -58,6 +58,7 @@ typedef struct orc_ws orc_ws;   /* VecE/VecB/VecJ work arrays (mem_fld.F90:6-32)
 #define ORC_OK 0
 #define ORC_ERR_CAPACITY 12     /* same meaning as the reference's Abort(12) */
 #define ORC_ERR_ARG 1
+#define ORC_ERR_BOUNDS 3        /* a particle reached the wall of the cylindrical grid */
 
 int  orc_params_init(orc_params *p, int ndim, int nx, int ny, int nz, double c,
                      double c_ompe, double g0, double mi_me, int epc);
@@ -126,6 +127,32 @@ void orc_cyl_current_halo_add(const orc_params *p, double *jx, double *jy, doubl
 int  orc_cyl_move_deposit(const orc_params *p, orc_cyl *cy, orc_ws *w, orc_fields *f, orc_particles *s);
 void orc_cyl_exchange_particles(const orc_params *p, orc_particles *s);
 int  orc_cyl_step(const orc_params *p, orc_cyl *cy, orc_ws *w, orc_fields *f, orc_particles *s);
+
+/* ---- azimuthal-mode cylindrical form (m = 0, 1), pictor_oracle_cylm.c ----
+ * No reference counterpart: the reference's cylindrical grid is full r-theta-z.
+ * Fields (mx, 3, mz) per component, slot 0 = m 0, 1 = m 1 cos, 2 = m 1 sin;
+ * particles: x = radial grid coordinate (r = (x - 3.5) dr), y = theta in
+ * radians, z = axial grid coordinate; momenta in the local (r, theta, z) frame. */
+typedef struct orc_cylm {
+    int32_t nr, nz, mx, mz;
+    double dr, inv_dr, inv_dz;
+    double c, fldc, fld_halfc, cinv, sqc;
+    double ompe, qi;
+    double flvrqm[2];
+    double zmin, zmax, zlen;
+} orc_cylm;
+
+int orc_cylm_init(orc_cylm *m, int nr, int nz, double dr, double c, double c_ompe, double g0, double mi_me, int epc);
+void orc_cylm_b_half(const orc_cylm *m, orc_fields *f);
+void orc_cylm_e_phase(const orc_cylm *m, orc_fields *f, int curl, int add);
+void orc_cylm_current_halo(const orc_cylm *m, orc_fields *f);
+void orc_cylm_reset_current(const orc_cylm *m, orc_fields *f);
+int orc_cylm_move_deposit(const orc_cylm *m, orc_fields *f, orc_particles *s);
+int orc_cylm_step(const orc_cylm *m, orc_fields *f, orc_particles *s);
+void orc_cylm_energy(const orc_cylm *m, const orc_fields *f, double out[2]);
+void orc_cylm_gauss(const orc_cylm *m, const orc_fields *f, const orc_particles *s, double *g);
+int64_t orc_cylm_load(const orc_cylm *m, orc_particles *s, int nr_p, int k0, int nzc, int ppc, int epc,
+                      uint64_t seed, int64_t shift);
 
 #ifdef __cplusplus
 }

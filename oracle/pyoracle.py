@@ -313,3 +313,120 @@ class CylOracle(Oracle):
     def radius(self, i):
         """r of grid index i (1-based, the reference's i)"""
         return (i - 3.0) + self.cy.rmin
+
+
+class OrcCylm(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int32) for n in ("nr", "nz", "mx", "mz")]
+                + [(n, ctypes.c_double) for n in ("dr", "inv_dr", "inv_dz", "c", "fldc", "fld_halfc", "cinv", "sqc",
+                                                  "ompe", "qi")]
+                + [("flvrqm", ctypes.c_double * 2)]
+                + [(n, ctypes.c_double) for n in ("zmin", "zmax", "zlen")])
+
+
+class CylModalOracle:
+    """FP64 statement of the azimuthal-mode (m = 0, 1) cylindrical step (no
+    reference counterpart; built from the reference's r-z staggering, metric
+    factors and axis rules).  Fields: numpy arrays (mx, 3, mz) in Fortran order,
+    slot 0 = m 0, 1 = m 1 cos, 2 = m 1 sin; particles x = radial grid coordinate,
+    y = theta (radians), z = axial grid coordinate."""
+
+    FIELD_NAMES = Oracle.FIELD_NAMES
+
+    def __init__(self, nr=16, nz=16, dr=1.0, c=0.45, c_ompe=10.0, g0=1.0, mi_me=1e30, epc=16, cap=1024):
+        L = lib()
+        P = ctypes.POINTER(OrcCylm)
+        F = ctypes.POINTER(OrcFields)
+        S = ctypes.POINTER(OrcParticles)
+        if not getattr(L, "_cylm_declared", False):
+            L.orc_cylm_init.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+            for n in ("orc_cylm_b_half", "orc_cylm_current_halo", "orc_cylm_reset_current"):
+                getattr(L, n).argtypes = [P, F]
+            L.orc_cylm_e_phase.argtypes = [P, F, ctypes.c_int, ctypes.c_int]
+            L.orc_cylm_move_deposit.argtypes = [P, F, S]
+            L.orc_cylm_step.argtypes = [P, F, S]
+            L.orc_cylm_energy.argtypes = [P, F, c_dp]
+            L.orc_cylm_gauss.argtypes = [P, F, S, c_dp]
+            L.orc_cylm_load.argtypes = [P, S, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_uint64, ctypes.c_int64]
+            L.orc_cylm_load.restype = ctypes.c_int64
+            L._cylm_declared = True
+        self.m = OrcCylm()
+        if L.orc_cylm_init(ctypes.byref(self.m), nr, nz, dr, c, c_ompe, g0, mi_me, epc):
+            raise ValueError("invalid modal configuration")
+        self.epc = epc
+        self.shape = (self.m.mx, 3, self.m.mz)
+        self.F = {n: np.zeros(self.shape, order="F") for n in self.FIELD_NAMES}
+        self.f = OrcFields(*[_ptr(self.F[n]) for n in self.FIELD_NAMES])
+        self._alloc(cap)
+
+    def _alloc(self, cap):
+        self.P = {n: np.zeros(cap) for n in ("q", "x", "y", "z", "u", "v", "w")}
+        self.P["flv"] = np.zeros(cap, dtype=np.int32)
+        self.s = OrcParticles(cap, 0, *[_ptr(self.P[n]) for n in ("q", "x", "y", "z", "u", "v", "w")],
+                              _ptr(self.P["flv"], c_ip))
+
+    def load_particles(self, q, x, y, z, u, v, w, flv, cap=None):
+        n = len(x)
+        self._alloc(max(cap or 0, n + 1024))
+        for name, arr in zip(("q", "x", "y", "z", "u", "v", "w"), (q, x, y, z, u, v, w)):
+            self.P[name][:n] = arr
+        self.P["flv"][:n] = flv
+        self.s.used = n
+
+    def load_plasma(self, nr_p, k0, nzc, ppc, seed=1, shift=0, cap=None):
+        """uniform cold electrons (orc_cylm_load); returns the particle arrays"""
+        need = nr_p * nzc * ppc
+        self._alloc(max(cap or 0, need + 1024))
+        lib().orc_cylm_load(ctypes.byref(self.m), ctypes.byref(self.s), nr_p, k0, nzc, ppc, self.epc, seed, shift)
+        return self.particles(live_only=False)
+
+    def particles(self, live_only=True):
+        n = self.s.used
+        out = {k: v[:n].copy() for k, v in self.P.items()}
+        if live_only:
+            keep = out["flv"] != 0
+            out = {k: v[keep] for k, v in out.items()}
+        return out
+
+    def set_fields(self, **arrays):
+        for k, v in arrays.items():
+            self.F[k][...] = v
+
+    def b_half(self):
+        lib().orc_cylm_b_half(ctypes.byref(self.m), ctypes.byref(self.f))
+
+    def e_phase(self, curl=True, add=False):
+        lib().orc_cylm_e_phase(ctypes.byref(self.m), ctypes.byref(self.f), int(curl), int(add))
+
+    def current_halo(self):
+        lib().orc_cylm_current_halo(ctypes.byref(self.m), ctypes.byref(self.f))
+
+    def reset_current(self):
+        lib().orc_cylm_reset_current(ctypes.byref(self.m), ctypes.byref(self.f))
+
+    def move_deposit(self):
+        if lib().orc_cylm_move_deposit(ctypes.byref(self.m), ctypes.byref(self.f), ctypes.byref(self.s)):
+            raise RuntimeError("modal push: a particle reached the outer wall")
+
+    def step(self):
+        if lib().orc_cylm_step(ctypes.byref(self.m), ctypes.byref(self.f), ctypes.byref(self.s)):
+            raise RuntimeError("modal oracle step failed")
+
+    def run(self, nsteps):
+        for _ in range(nsteps):
+            self.step()
+
+    def energy(self):
+        out = np.zeros(2)
+        lib().orc_cylm_energy(ctypes.byref(self.m), ctypes.byref(self.f), _ptr(out))
+        return out
+
+    def gauss(self):
+        g = np.zeros(self.shape, order="F")
+        lib().orc_cylm_gauss(ctypes.byref(self.m), ctypes.byref(self.f), ctypes.byref(self.s), _ptr(g))
+        return g
+
+    def radius(self, i):
+        """r of grid index i (1-based)"""
+        return (i - 3.5) * self.m.dr
