@@ -47,6 +47,14 @@ extern "C" {
                                     form; src/cyl/cyl_em_update.F90,
                                     cyl_movdep.F90; conducting wall beyond
                                     r = nx - 1/2                              */
+#define PICTOR_GEOM_CYL_MODAL 2  /* no reference counterpart: the cylinder in
+                                    azimuthal modes m = 0, 1 (ndim 3, nx = Nr,
+                                    ny = 2 modes, nz = Nz), the reference's r-z
+                                    staggering, axis rules and push; fields
+                                    (nx+5, 3, nz+5) with slot 0 = m 0, 1 = m 1
+                                    cos, 2 = m 1 sin; particles y = theta in
+                                    radians; radial cell cfg.dr axial cells;
+                                    FP64, Yee, periodic z (or the moving window) */
 
 /* Run configuration: the compile-time parameters of parameters.F90:5-87 and
  * src/vars.F90:10-18 as a runtime struct.  grid_dx = grid_dy = grid_dz = 1,
@@ -66,6 +74,7 @@ typedef struct pictor_config {
     double g0;             /* parameters.F90:16 */
     double mi_me;          /* parameters.F90:18 */
     double capacity_factor;/* particle storage per tile / particles at layout time (0 = 1.25) */
+    double dr;             /* PICTOR_GEOM_CYL_MODAL: radial cell in axial cells (0 = 1) */
 } pictor_config;
 
 typedef struct pictor_ctx pictor_ctx;
@@ -181,6 +190,24 @@ int pictor_adjoint_stats(pictor_ctx *ctx, double out[10]);
  * unsorted engine.  Takes effect at the next pictor_adjoint_forward. */
 int pictor_adjoint_engine(pictor_ctx *ctx, int32_t engine);
 int pictor_adjoint_release(pictor_ctx *ctx);
+
+/* ---- the azimuthal-mode cylinder (PICTOR_GEOM_CYL_MODAL; no counterpart in
+ * the reference, which has no modes, laser or moving window) ----
+ * Cold electrons of uniform density (weight epc per unit volume): ppc per cell
+ * in the radial cells 4..4+nr_plasma-1 and axial cells k0..k0+nzc-1, (r, z)
+ * uniform in the cell, ppc evenly spaced angles with a random offset per cell;
+ * counter-based (seed), so a cell always gets the same particles. */
+int pictor_cyl_plasma(pictor_ctx *ctx, int32_t nr_plasma, int32_t k0, int32_t nzc, int32_t ppc, uint64_t seed);
+/* Adds a Gaussian pulse, linearly polarised along x and moving along +z, at
+ * focus: E_x = a0 c w0dt exp(-r^2/w0^2 - (z - zc)^2/(c tau)^2) cos(k0 (z - zc)),
+ * B_y = E_x, into the m = 1 slots.  omega0 in omega_p, tau in 1/omega_p (c tau
+ * is the 1/e half length of the field), w0 in c/omega_p, zc in axial cells. */
+int pictor_cyl_laser(pictor_ctx *ctx, double a0, double omega0, double tau, double w0, double zc);
+/* Moving window: after every fused step the fields and particles move back by
+ * one axial cell whenever the window (speed c) has advanced one; the front
+ * cells restart empty and receive fresh plasma (nr_plasma, ppc, seed as
+ * pictor_cyl_plasma).  on = 0 stops it. */
+int pictor_moving_window(pictor_ctx *ctx, int32_t on, int32_t nr_plasma, int32_t ppc, uint64_t seed);
 
 /* ---- measurement (no counterpart in the reference) ----
  * Atomic throughput on the current device: every lane of 4 workgroups per CU

@@ -153,6 +153,8 @@ void orc_cylm_energy(const orc_cylm *m, const orc_fields *f, double out[2]);
 void orc_cylm_gauss(const orc_cylm *m, const orc_fields *f, const orc_particles *s, double *g);
 int64_t orc_cylm_load(const orc_cylm *m, orc_particles *s, int nr_p, int k0, int nzc, int ppc, int epc,
                       uint64_t seed, int64_t shift);
+void orc_cylm_shift(const orc_cylm *m, orc_fields *f, orc_particles *s, int nr_p, int ppc, int epc, uint64_t seed,
+                    int64_t shift);
 
 #ifdef __cplusplus
 }

@@ -462,3 +462,26 @@ int64_t orc_cylm_load(const orc_cylm *m, orc_particles *s, int nr_p, int k0, int
         }
     return added;
 }
+
+/* the moving window's shift by one axial cell: fields (k) <- (k + 1) with the
+ * front row k = mz - 3 and the ghosts above it zero, then the z halos and the
+ * wall; particles z -= 1, those below z = 3 dropped (flavour 0); fresh plasma
+ * in the front row (orc_cylm_load with the window's shift count) */
+void orc_cylm_shift(const orc_cylm *m, orc_fields *f, orc_particles *s, int nr_p, int ppc, int epc, uint64_t seed,
+                    int64_t shift)
+{
+    double *F[6] = {f->Ex, f->Ey, f->Ez, f->Bx, f->By, f->Bz};
+    for (int c = 0; c < 6; c++)
+        for (int k = 1; k <= m->mz; k++)
+            for (int sl = 0; sl < 3; sl++)
+                for (int i = 1; i <= m->mx; i++)
+                    F[c][MI(m, i, sl, k)] = (k + 1 <= m->mz - 3) ? F[c][MI(m, i, sl, k + 1)] : 0.0;
+    for (int64_t n = 0; n < s->used; n++) {
+        if (s->flv[n] == 0) continue;
+        s->z[n] = s->z[n] - 1.0;
+        if (s->z[n] < m->zmin) s->flv[n] = 0;
+    }
+    if (nr_p > 0 && ppc > 0) orc_cylm_load(m, s, nr_p, m->mz - 3, 1, ppc, epc, seed, shift);
+    cylm_e_axis(m, f);
+    for (int c = 0; c < 6; c++) { cylm_zhalo(m, F[c]); cylm_outer_zero(m, F[c]); }
+}

@@ -350,6 +350,8 @@ class CylModalOracle:
             L.orc_cylm_load.argtypes = [P, S, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_uint64, ctypes.c_int64]
             L.orc_cylm_load.restype = ctypes.c_int64
+            L.orc_cylm_shift.argtypes = [P, F, S, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                         ctypes.c_int64]
             L._cylm_declared = True
         self.m = OrcCylm()
         if L.orc_cylm_init(ctypes.byref(self.m), nr, nz, dr, c, c_ompe, g0, mi_me, epc):
@@ -416,6 +418,11 @@ class CylModalOracle:
     def run(self, nsteps):
         for _ in range(nsteps):
             self.step()
+
+    def shift(self, nr_p, ppc, seed, shift):
+        """the moving window's shift by one axial cell (shift = the shift count after it)"""
+        lib().orc_cylm_shift(ctypes.byref(self.m), ctypes.byref(self.f), ctypes.byref(self.s), nr_p, ppc, self.epc,
+                             seed, shift)
 
     def energy(self):
         out = np.zeros(2)

@@ -22,6 +22,7 @@ FDTD_YEE = 0
 FDTD_LEHE = 1
 GEOM_CARTESIAN = 0
 GEOM_CYL = 1
+GEOM_CYL_MODAL = 2
 
 # every entry point of include/pictor_hip.h
 EXPORTS = (
@@ -33,7 +34,8 @@ EXPORTS = (
     "pictor_reset_current", "pictor_sort_tiles", "pictor_step", "pictor_synchronize",
     "pictor_diagnostics", "pictor_timer_reset", "pictor_timer_get", "pictor_memory_bytes",
     "pictor_setup_weibel", "pictor_adjoint_forward", "pictor_adjoint_backward", "pictor_adjoint_stats",
-    "pictor_adjoint_engine", "pictor_adjoint_release", "pictor_probe_atomics",
+    "pictor_adjoint_engine", "pictor_adjoint_release", "pictor_cyl_plasma", "pictor_cyl_laser",
+    "pictor_moving_window", "pictor_probe_atomics",
 )
 
 
@@ -43,7 +45,7 @@ class PictorConfig(ctypes.Structure):
         ("epc", ctypes.c_int32), ("form", ctypes.c_int32), ("tile", ctypes.c_int32), ("fdtd", ctypes.c_int32),
         ("geometry", ctypes.c_int32), ("curr_filter", ctypes.c_int32),
         ("c", ctypes.c_double), ("c_ompe", ctypes.c_double), ("g0", ctypes.c_double), ("mi_me", ctypes.c_double),
-        ("capacity_factor", ctypes.c_double),
+        ("capacity_factor", ctypes.c_double), ("dr", ctypes.c_double),
     ]
 
 
@@ -96,6 +98,9 @@ def lib():
     L.pictor_adjoint_release.argtypes = [c_vp]
     L.pictor_adjoint_engine.argtypes = [c_vp, i32]
     L.pictor_probe_atomics.argtypes = [i32, c_dp]
+    L.pictor_cyl_plasma.argtypes = [c_vp, i32, i32, i32, i32, ctypes.c_uint64]
+    L.pictor_cyl_laser.argtypes = [c_vp] + [ctypes.c_double] * 5
+    L.pictor_moving_window.argtypes = [c_vp, i32, i32, i32, ctypes.c_uint64]
     for name in EXPORTS:
         if name != "pictor_last_error":
             getattr(L, name).restype = ctypes.c_int

@@ -414,10 +414,12 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     if (!cfg || !out) return fail(PICTOR_ERR_ARG, "null argument");
     *out = nullptr;
     if (cfg->ndim != 2 && cfg->ndim != 3) return fail(PICTOR_ERR_ARG, "ndim must be 2 or 3");
-    if (cfg->nx < 8 || cfg->ny < 8 || (cfg->ndim == 3 && cfg->nz < 8)) return fail(PICTOR_ERR_ARG, "nx, ny (, nz) must be >= 8");
+    const bool ny_modes = cfg->geometry == PICTOR_GEOM_CYL_MODAL;   // ny counts modes there, not cells
+    if (cfg->nx < 8 || (!ny_modes && cfg->ny < 8) || (cfg->ndim == 3 && cfg->nz < 8))
+        return fail(PICTOR_ERR_ARG, "nx, ny (, nz) must be >= 8");
     // the packed cell word of the split form bounds the Cartesian grids; the cylindrical
     // path keeps FP64 coordinates
-    const bool cyl_geom = cfg->geometry == PICTOR_GEOM_CYL;
+    const bool cyl_geom = cfg->geometry == PICTOR_GEOM_CYL || cfg->geometry == PICTOR_GEOM_CYL_MODAL;
     if (!cyl_geom && cfg->ndim == 2 && (cfg->nx > 32768 || cfg->ny > 32768))
         return fail(PICTOR_ERR_ARG, "2-D nx, ny must be <= 32768");
     if (!cyl_geom && cfg->ndim == 3 && (cfg->nx > 2048 || cfg->ny > 2048 || cfg->nz > 512))
@@ -426,14 +428,27 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
         return fail(PICTOR_ERR_ARG, "cylindrical grid larger than 2^31 cells");
     if (cfg->form != PICTOR_FORM_FP64 && cfg->form != PICTOR_FORM_FP32_SPLIT) return fail(PICTOR_ERR_ARG, "unknown form");
     if (cfg->fdtd != PICTOR_FDTD_YEE && cfg->fdtd != PICTOR_FDTD_LEHE) return fail(PICTOR_ERR_ARG, "unknown fdtd");
-    if (cfg->geometry != PICTOR_GEOM_CARTESIAN && cfg->geometry != PICTOR_GEOM_CYL)
+    if (cfg->geometry != PICTOR_GEOM_CARTESIAN && cfg->geometry != PICTOR_GEOM_CYL &&
+        cfg->geometry != PICTOR_GEOM_CYL_MODAL)
         return fail(PICTOR_ERR_ARG, "unknown geometry");
-    const bool cyl = cfg->geometry == PICTOR_GEOM_CYL;
+    const bool modal = cfg->geometry == PICTOR_GEOM_CYL_MODAL;
+    if (modal) {
+        if (cfg->ndim != 3 || cfg->ny != 2) return fail(PICTOR_ERR_ARG, "the modal cylinder needs ndim 3 and ny = 2 (modes m = 0, 1)");
+        if (cfg->form != PICTOR_FORM_FP64 || cfg->fdtd != PICTOR_FDTD_YEE || cfg->curr_filter)
+            return fail(PICTOR_ERR_ARG, "the modal cylinder runs the FP64 form with the Yee solver and no current filter");
+        if (cfg->dr < 0.0) return fail(PICTOR_ERR_ARG, "dr must be >= 0");
+        // the m = 1 terms near the axis act like a theta cell of dr / 2 at r = dr / 2
+        const double dr = cfg->dr > 0.0 ? cfg->dr : 1.0;
+        const double lim = 1.0 / std::sqrt(1.0 + 1.0 / (dr * dr) + 4.0 / (dr * dr));
+        if (cfg->c >= lim)
+            return fail(PICTOR_ERR_ARG, "modal cylinder: c must be below 1/sqrt(1 + 5/dr^2) = " + std::to_string(lim));
+    }
+    const bool cyl = cfg->geometry == PICTOR_GEOM_CYL || modal;
     if (cfg->curr_filter < 0 || cfg->curr_filter > 64) return fail(PICTOR_ERR_ARG, "curr_filter must be 0..64");
     if (cyl && cfg->curr_filter) return fail(PICTOR_ERR_ARG, "the current filter is for the Cartesian geometry");
     if (cyl && cfg->form != PICTOR_FORM_FP64) return fail(PICTOR_ERR_ARG, "the cylindrical geometry runs in the FP64 form");
-    if (cyl && cfg->ny % 2) return fail(PICTOR_ERR_ARG, "the cylindrical grid needs an even number of theta cells (ny)");
-    if (cyl) {
+    if (cyl && !modal && cfg->ny % 2) return fail(PICTOR_ERR_ARG, "the cylindrical grid needs an even number of theta cells (ny)");
+    if (cyl && !modal) {
         // the narrowest cell of the r-theta grid is the arc 0.5 * dtheta at r = 1/2 (grid index 4):
         // the FDTD step is stable for c < 1 / sqrt(1 + 1 / (0.5 dtheta)^2 (+ 1 in 3-D))
         const double arc = 0.5 * (2.0 * 3.14159265358979323846264 / cfg->ny);
@@ -463,6 +478,7 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     g.ndim = cfg->ndim;
     g.nx = cfg->nx; g.ny = cfg->ny; g.nz = (cfg->ndim == 2) ? 1 : cfg->nz;
     g.mx = g.nx + 5; g.my = g.ny + 5; g.mz = (cfg->ndim == 2) ? 1 : g.nz + 5;
+    if (modal) { g.ny = 3; g.my = 3; }   // the middle index holds the three mode slots
     g.ncell = (long long)g.mx * g.my * g.mz;
 
     // InitScale, initialise.F90:44-53; vars.F90:220-223; subdomains.F90:52-67
@@ -512,6 +528,8 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     }
 
     c->cyl = cyl;
+    c->cylm = modal;
+    if (modal) cylm_params(*cfg, ph, c->cq);
     TileP &tp = c->tp;
     tp.T = cyl ? g.nx : T;
     tp.ntx = cyl ? 1 : g.nx / T; tp.nty = cyl ? 1 : g.ny / T; tp.ntz = (g.ndim == 3 && !cyl) ? g.nz / T : 1;
@@ -555,7 +573,9 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) return cleanup_fail(fail(PICTOR_ERR_HIP, "event creation failed"));
     // empty layout so that a step without particles is valid
     int rc = PICTOR_OK;
-    if (cyl) {
+    if (modal) {
+        rc = cylm_setup(c);
+    } else if (cyl) {
         rc = cyl_setup(c);
     } else {
         std::vector<long long> zero((size_t)tp.ntiles * tp.nsp, 0);
@@ -586,6 +606,7 @@ int pictor_finalize(pictor_ctx *c)
     dfree(c->d_spill_count); dfree(c->d_err);
     for (int i = 0; i < 2; i++) { free_f64(c->pf[i]); free_split(c->ps[i]); }
     free_f64(c->spill_f); free_split(c->spill_s);
+    if (c->cylm) cylm_release(c);
     cyl_release(c);
     dfree(c->d_partial); dfree(c->d_rho);
     if (c->h_flags) hipHostFree(c->h_flags);
@@ -667,7 +688,7 @@ int pictor_upload_particles(pictor_ctx *c, int64_t n, const double *q, const dou
     if (n > 0 && (!q || !x || !y || !z || !u || !v || !w || !flv)) return fail(PICTOR_ERR_ARG, "null particle pointer");
     if (n >= 0x7fffffffLL) return fail(PICTOR_ERR_ARG, "more than 2^31-1 particles per device");
     c->pending_swap = false;
-    if (c->cyl) return cyl_upload_particles(c, n, q, x, y, z, u, v, w, flv);
+    if (c->cyl) return c->cylm ? cylm_upload_particles(c, n, q, x, y, z, u, v, w, flv) : cyl_upload_particles(c, n, q, x, y, z, u, v, w, flv);
     const TileP &tp = c->tp;
     const size_t ng = (size_t)tp.ntiles * tp.nsp;
     PartF64 raw{};
@@ -794,7 +815,7 @@ int pictor_download_particles(pictor_ctx *c, int64_t cap, double *q, double *x, 
 int pictor_update_b_half(pictor_ctx *c)
 {
     int rc = check_ctx(c);
-    return rc ? rc : (c->cyl ? cyl_b_half(c) : do_b_half(c));
+    return rc ? rc : (c->cyl ? (c->cylm ? cylm_b_half(c) : cyl_b_half(c)) : do_b_half(c));
 }
 
 int pictor_move_deposit(pictor_ctx *c)
@@ -803,7 +824,7 @@ int pictor_move_deposit(pictor_ctx *c)
     if (rc) return rc;
     if (c->cyl) {
         if (c->pending_swap) return fail(PICTOR_ERR_ARG, "pictor_move_deposit called twice without pictor_exchange_particles");
-        return cyl_push(c, false);
+        return c->cylm ? cylm_push(c, false) : cyl_push(c, false);
     }
     return do_push(c, false);
 }
@@ -813,19 +834,19 @@ int pictor_exchange_particles(pictor_ctx *c)
     int rc = check_ctx(c);
     if (rc) return rc;
     if (!c->pending_swap) return fail(PICTOR_ERR_ARG, "pictor_exchange_particles without a preceding pictor_move_deposit");
-    return c->cyl ? cyl_exchange(c) : commit_push(c);
+    return c->cyl ? (c->cylm ? cylm_exchange(c) : cyl_exchange(c)) : commit_push(c);
 }
 
 int pictor_update_e(pictor_ctx *c)
 {
     int rc = check_ctx(c);
-    return rc ? rc : (c->cyl ? cyl_e_phase(c, true, false) : do_e_phase(c, true, false));
+    return rc ? rc : (c->cyl ? (c->cylm ? cylm_e_phase(c, true, false) : cyl_e_phase(c, true, false)) : do_e_phase(c, true, false));
 }
 
 int pictor_current_halo_add(pictor_ctx *c)
 {
     int rc = check_ctx(c);
-    return rc ? rc : (c->cyl ? cyl_current_halo(c) : do_fold(c));
+    return rc ? rc : (c->cyl ? (c->cylm ? cylm_current_halo(c) : cyl_current_halo(c)) : do_fold(c));
 }
 
 int pictor_smoothen_current(pictor_ctx *c)
@@ -837,7 +858,7 @@ int pictor_smoothen_current(pictor_ctx *c)
 int pictor_add_current(pictor_ctx *c)
 {
     int rc = check_ctx(c);
-    return rc ? rc : (c->cyl ? cyl_e_phase(c, false, true) : do_e_phase(c, false, true));
+    return rc ? rc : (c->cyl ? (c->cylm ? cylm_e_phase(c, false, true) : cyl_e_phase(c, false, true)) : do_e_phase(c, false, true));
 }
 
 int pictor_field_halo(pictor_ctx *c, int32_t which)
@@ -845,7 +866,7 @@ int pictor_field_halo(pictor_ctx *c, int32_t which)
     int rc = check_ctx(c);
     if (rc) return rc;
     if (which != 0 && which != 1) return fail(PICTOR_ERR_ARG, "which must be 0 (E) or 1 (B)");
-    if (c->cyl) return cyl_field_halo(c, which);
+    if (c->cyl) return c->cylm ? cylm_field_halo(c, which) : cyl_field_halo(c, which);
     void **F = which == 0 ? c->E : c->B;
     if (c->form == PICTOR_FORM_FP64) {
         double *P[3] = {(double *)F[0], (double *)F[1], (double *)F[2]};
@@ -861,7 +882,7 @@ int pictor_field_halo(pictor_ctx *c, int32_t which)
 int pictor_reset_current(pictor_ctx *c)
 {
     int rc = check_ctx(c);
-    return rc ? rc : (c->cyl ? cyl_reset_current(c) : do_reset_current(c));
+    return rc ? rc : (c->cyl ? (c->cylm ? cylm_reset_current(c) : cyl_reset_current(c)) : do_reset_current(c));
 }
 
 int pictor_sort_tiles(pictor_ctx *c)
@@ -882,11 +903,19 @@ int pictor_step(pictor_ctx *c, int32_t nsteps)
         // MAIN.F90:42-284 with the cylindrical routines
         for (int s = 0; s < nsteps; s++) {
             HIPTRY(hipEventRecord(c->ev[4], c->stream));
-            if ((rc = cyl_reset_current(c)) || (rc = cyl_b_half(c)) || (rc = cyl_push(c, true))) return rc;
-            if ((rc = cyl_exchange(c))) return rc;
-            if ((rc = cyl_b_half(c)) || (rc = cyl_e_phase(c, true, false)) || (rc = cyl_current_halo(c)) ||
-                (rc = cyl_e_phase(c, false, true)))
-                return rc;
+            if (c->cylm) {
+                if ((rc = cylm_reset_current(c)) || (rc = cylm_b_half(c)) || (rc = cylm_push(c, true))) return rc;
+                if ((rc = cylm_exchange(c))) return rc;
+                if ((rc = cylm_b_half(c)) || (rc = cylm_e_phase(c, true, false)) || (rc = cylm_current_halo(c)) ||
+                    (rc = cylm_e_phase(c, false, true)) || (rc = cylm_window_after_step(c)))
+                    return rc;
+            } else {
+                if ((rc = cyl_reset_current(c)) || (rc = cyl_b_half(c)) || (rc = cyl_push(c, true))) return rc;
+                if ((rc = cyl_exchange(c))) return rc;
+                if ((rc = cyl_b_half(c)) || (rc = cyl_e_phase(c, true, false)) || (rc = cyl_current_halo(c)) ||
+                    (rc = cyl_e_phase(c, false, true)))
+                    return rc;
+            }
             HIPTRY(hipEventRecord(c->ev[5], c->stream));
             HIPTRY(hipEventSynchronize(c->ev[5]));
             float ms = 0, f2 = 0;
@@ -927,12 +956,45 @@ int pictor_synchronize(pictor_ctx *c)
     return PICTOR_OK;
 }
 
+int pictor_cyl_plasma(pictor_ctx *c, int32_t nr_plasma, int32_t k0, int32_t nzc, int32_t ppc, uint64_t seed)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->cylm) return fail(PICTOR_ERR_ARG, "pictor_cyl_plasma is for the modal cylinder");
+    return cylm_inject(c, nr_plasma, k0, nzc, ppc, seed, c->cm.win_shift);
+}
+
+int pictor_cyl_laser(pictor_ctx *c, double a0, double omega0, double tau, double w0, double zc)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->cylm) return fail(PICTOR_ERR_ARG, "pictor_cyl_laser is for the modal cylinder");
+    if (!(omega0 > 0.0 && tau > 0.0 && w0 > 0.0)) return fail(PICTOR_ERR_ARG, "omega0, tau and w0 must be > 0");
+    return cylm_laser(c, a0, omega0, tau, w0, zc);
+}
+
+int pictor_moving_window(pictor_ctx *c, int32_t on, int32_t nr_plasma, int32_t ppc, uint64_t seed)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->cylm) return fail(PICTOR_ERR_ARG, "pictor_moving_window is for the modal cylinder");
+    if (on && (nr_plasma < 0 || nr_plasma > c->g.nx - 1 || ppc < 0))
+        return fail(PICTOR_ERR_ARG, "nr_plasma must lie in 0..nx-1 and ppc >= 0");
+    CylmState &st = c->cm;
+    st.win_on = on != 0;
+    st.win_nrp = nr_plasma;
+    st.win_ppc = ppc;
+    st.win_seed = seed;
+    return PICTOR_OK;
+}
+
 int pictor_diagnostics(pictor_ctx *c, double out[9])
 {
     int rc = check_ctx(c);
     if (rc) return rc;
     if (c->pending_swap) return fail(PICTOR_ERR_ARG, "diagnostics between move_deposit and exchange_particles");
-    if (c->cyl) return fail(PICTOR_ERR_ARG, "pictor_diagnostics is available for the Cartesian geometry");
+    if (c->cylm) return cylm_diagnostics(c, out);
+    if (c->cyl) return fail(PICTOR_ERR_ARG, "pictor_diagnostics is available for the Cartesian and modal geometries");
     const int nblk = 1024;
     std::vector<double> part((size_t)std::max(nblk * 2, c->tp.ntiles * 5));
     // field energies

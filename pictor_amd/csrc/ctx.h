@@ -7,6 +7,7 @@
 #include "../../include/pictor_hip.h"
 #include "pictor_common.h"
 #include "cyl.h"
+#include "cylm.h"
 
 namespace pictor {
 
@@ -107,6 +108,11 @@ struct pictor_ctx {
     PartF64 pcyl{};
     int *d_cerr = nullptr;
     unsigned long long *d_nsub = nullptr;
+    // azimuthal-mode cylinder (cylm.hip): set together with cyl, which routes
+    // the phase calls; the modal functions then take over
+    bool cylm = false;
+    pictor::CylmP cq{};
+    pictor::CylmState cm{};
 
     pictor::AdjEngineBase *adj = nullptr;   // created by pictor_adjoint_forward
     int adj_engine = 0;                      // 0 tile-sorted (FP32 split form), 1 unsorted

@@ -1,0 +1,49 @@
+// cylm.h -- azimuthal-mode (m = 0, 1) cylindrical geometry (cylm.hip).  FP64.
+#pragma once
+#include "pictor_common.h"
+
+struct pictor_ctx;
+struct pictor_config;
+
+namespace pictor {
+
+// scalars of the modal grid: lengths in axial cells, radial cell dr
+struct CylmP {
+    double dr, inv_dr, inv_dz;
+    double c, fldc, fld_halfc, cinv, sqc;
+    double qi;
+    double flvrqm[2];
+    double zmin, zmax, zlen;
+};
+
+// particle storage and moving-window state of the modal path
+struct CylmState {
+    long long cap = 0;                   // allocated particle slots (c->np slots in use, holes included)
+    int *d_err = nullptr;
+    unsigned long long *d_count = nullptr;
+    double *d_partial = nullptr;
+    bool win_on = false;
+    long long win_steps = 0, win_shift = 0;
+    int win_nrp = 0, win_ppc = 0;
+    unsigned long long win_seed = 0;
+};
+
+void cylm_params(const pictor_config &cfg, const PhysP &ph, CylmP &q);
+
+}  // namespace pictor
+
+int cylm_setup(pictor_ctx *c);
+void cylm_release(pictor_ctx *c);
+int cylm_upload_particles(pictor_ctx *c, long long n, const double *q, const double *x, const double *y,
+                          const double *z, const double *u, const double *v, const double *w, const int *flv);
+int cylm_b_half(pictor_ctx *c);
+int cylm_e_phase(pictor_ctx *c, bool curl, bool add);
+int cylm_current_halo(pictor_ctx *c);
+int cylm_field_halo(pictor_ctx *c, int which);
+int cylm_reset_current(pictor_ctx *c);
+int cylm_push(pictor_ctx *c, bool timed);
+int cylm_exchange(pictor_ctx *c);
+int cylm_inject(pictor_ctx *c, int nrp, int k0, int nzc, int ppc, unsigned long long seed, long long shift);
+int cylm_laser(pictor_ctx *c, double a0, double omega0, double tau, double w0, double zc);
+int cylm_window_after_step(pictor_ctx *c);
+int cylm_diagnostics(pictor_ctx *c, double out[9]);

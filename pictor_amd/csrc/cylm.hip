@@ -1,0 +1,788 @@
+// cylm.hip -- azimuthal-mode (m = 0, 1) cylindrical geometry, FP64.
+//
+// The reference's cylindrical build is a full r-theta-z grid (cyl.hip here);
+// it has no modal form, no laser and no moving window.  This form keeps that
+// grid's r and z staggering, metric factors, axis rules and local-frame push,
+// and replaces theta by F = F0 + F1c cos(theta) + F1s sin(theta):
+//   * theta differences (F(j) - F(j-1)) / dtheta of cyl_em_update.F90 become
+//     the exact derivative, (dF/dtheta)_c = F1s, (dF/dtheta)_s = -F1c;
+//   * the axis fills map theta to theta + pi: each mode by its parity (-1)^m;
+//     the Bz axis integral acts on m = 0 only;
+//   * AddCurrent divides by the circle, 2 pi for m = 0 and pi for m = 1;
+//   * the deposit splits the move into the r-z zigzag at the old angle
+//     (every slot weighted by its phase) and the angular move at the new
+//     (r, z), which keeps each mode's discrete continuity equation exact.
+// Arrays (mx, 3, mz): slot 0 = m 0, 1 = m 1 cos, 2 = m 1 sin; element
+// (i, s, k), 1-based i and k, at (i - 1) + mx (s + 3 (k - 1)).  Lengths in
+// axial cells (dz = 1); the radial cell is dr.  Particles: x = radial grid
+// coordinate (r = (x - 3.5) dr), y = theta in radians, z = axial coordinate,
+// momenta in the local (r, theta, z) frame; flavour 2 = electrons.
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "ctx.h"
+
+namespace pictor {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ __forceinline__ long long mi(const GridP &g, int i, int s, int k)
+{
+    return (long long)(i - 1) + (long long)g.mx * (s + 3LL * (k - 1));
+}
+
+__device__ __forceinline__ double rad(const CylmP &q, double i) { return (i - 3.5) * q.dr; }
+
+__device__ __forceinline__ double parity(int s) { return s == 0 ? 1.0 : -1.0; }
+
+// UpdateEfield (cyl_em_update.F90:7-46, 3-D): i = 4..mx-3, k = 3..mz-3, every slot
+__global__ void k_cylm_update_e(GridP g, CylmP q, double *__restrict__ Ex, double *__restrict__ Ey,
+                                double *__restrict__ Ez, const double *__restrict__ Bx, const double *__restrict__ By,
+                                const double *__restrict__ Bz)
+{
+    const int ni = g.mx - 6, nk = g.mz - 5;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)ni * 3 * nk) return;
+    const int i = 4 + (int)(t % ni);
+    t /= ni;
+    const int s = (int)(t % 3);
+    const int k = 3 + (int)(t / 3);
+    const long long o = mi(g, i, s, k), sk = 3LL * g.mx, ss = g.mx;
+    const double fldc = q.fldc, idr = q.inv_dr, idz = q.inv_dz;
+    const double r = rad(q, i), rp = r + 0.5 * q.dr, rm = r - 0.5 * q.dr;
+    const double dBz = s == 0 ? 0.0 : (s == 1 ? Bz[o + ss] : -Bz[o - ss]);
+    const double dBr = s == 0 ? 0.0 : (s == 1 ? Bx[o + ss] : -Bx[o - ss]);
+    Ex[o] = Ex[o] + fldc * dBz / rp - fldc * idz * (By[o] - By[o - sk]);
+    Ey[o] = Ey[o] + fldc * idz * (Bx[o] - Bx[o - sk]) - fldc * idr * (Bz[o] - Bz[o - 1]);
+    Ez[o] = Ez[o] + fldc * idr * (rp * By[o] - rm * By[o - 1]) / r - fldc * dBr / r;
+}
+
+// AddCurrent (cyl_em_update.F90:114-140) over the circle of each mode
+__global__ void k_cylm_add_current(GridP g, CylmP q, double *__restrict__ Ex, double *__restrict__ Ey,
+                                   double *__restrict__ Ez, const double *__restrict__ Jx,
+                                   const double *__restrict__ Jy, const double *__restrict__ Jz)
+{
+    const int ni = g.mx - 6, nk = g.mz - 5;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)ni * 3 * nk) return;
+    const int i = 4 + (int)(t % ni);
+    t /= ni;
+    const int s = (int)(t % 3);
+    const int k = 3 + (int)(t / 3);
+    const long long o = mi(g, i, s, k);
+    const double th = s == 0 ? 2.0 * kPi : kPi;
+    const double r = rad(q, i), rp = r + 0.5 * q.dr;
+    Ex[o] = Ex[o] - Jx[o] * q.inv_dz / (rp * th);
+    Ey[o] = Ey[o] - Jy[o] * q.inv_dr * q.inv_dz / th;
+    Ez[o] = Ez[o] - Jz[o] * q.inv_dr / (r * th);
+}
+
+// E axis fills (cyl_em_update.F90:143-180), theta + pi as the mode parity
+__global__ void k_cylm_e_axis(GridP g, double *__restrict__ Ex, double *__restrict__ Ey, double *__restrict__ Ez)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3LL * g.mz) return;
+    const int s = (int)(t % 3), k = 1 + (int)(t / 3);
+    const double p = parity(s);
+    Ex[mi(g, 3, s, k)] = 0.5 * (Ex[mi(g, 4, s, k)] - p * Ex[mi(g, 4, s, k)]);
+    Ex[mi(g, 2, s, k)] = -(p * Ex[mi(g, 4, s, k)]);
+    Ey[mi(g, 3, s, k)] = -(p * Ey[mi(g, 4, s, k)]);
+    Ez[mi(g, 3, s, k)] = p * Ez[mi(g, 4, s, k)];
+}
+
+// UpdateBfieldHalf (cyl_em_update.F90:49-110, 3-D): i = 4..mx-1, k = 1..mz-1
+__global__ void k_cylm_update_b(GridP g, CylmP q, double *__restrict__ Bx, double *__restrict__ By,
+                                double *__restrict__ Bz, const double *__restrict__ Ex, const double *__restrict__ Ey,
+                                const double *__restrict__ Ez)
+{
+    const int ni = g.mx - 4, nk = g.mz - 1;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)ni * 3 * nk) return;
+    const int i = 4 + (int)(t % ni);
+    t /= ni;
+    const int s = (int)(t % 3);
+    const int k = 1 + (int)(t / 3);
+    const long long o = mi(g, i, s, k), sk = 3LL * g.mx, ss = g.mx;
+    const double h = q.fld_halfc, idr = q.inv_dr, idz = q.inv_dz;
+    const double r = rad(q, i), rph = r + 0.5 * q.dr, rp = r + q.dr;
+    const double dEz = s == 0 ? 0.0 : (s == 1 ? Ez[o + ss] : -Ez[o - ss]);
+    const double dEr = s == 0 ? 0.0 : (s == 1 ? Ex[o + ss] : -Ex[o - ss]);
+    Bx[o] = Bx[o] - h * dEz / r + h * idz * (Ey[o + sk] - Ey[o]);
+    By[o] = By[o] - h * idz * (Ex[o + sk] - Ex[o]) + h * idr * (Ez[o + 1] - Ez[o]);
+    Bz[o] = Bz[o] - h * (idr * (rp * Ey[o + 1] - r * Ey[o]) - dEr) / rph;
+}
+
+// Bz on the axis (cyl_em_update.F90:231-250, the m = 0 circulation of E_theta at
+// r = dr/2 over the disk, 4/dr), then the fills of :182-229 by parity
+__global__ void k_cylm_b_axis(GridP g, CylmP q, double *__restrict__ Bx, double *__restrict__ By,
+                              double *__restrict__ Bz, const double *__restrict__ Ey)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3LL * g.mz) return;
+    const int s = (int)(t % 3), k = 1 + (int)(t / 3);
+    if (s == 0 && k <= g.mz - 1) {
+        const long long a = mi(g, 3, 0, k);
+        Bz[a] = Bz[a] - q.fld_halfc * (4.0 * q.inv_dr) * Ey[mi(g, 4, 0, k)];
+    }
+    const double p = parity(s);
+    By[mi(g, 3, s, k)] = 0.5 * (By[mi(g, 4, s, k)] - p * By[mi(g, 4, s, k)]);
+    By[mi(g, 2, s, k)] = -(p * By[mi(g, 4, s, k)]);
+    Bx[mi(g, 3, s, k)] = -(p * Bx[mi(g, 4, s, k)]);
+    Bz[mi(g, 2, s, k)] = p * Bz[mi(g, 4, s, k)];
+}
+
+// periodic z ghosts over x = 3..mx-3, then the wall (i >= mx-2 zero); three arrays
+__global__ void k_cylm_zhalo(GridP g, double *__restrict__ F0, double *__restrict__ F1, double *__restrict__ F2)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.mx * 3 * 5) return;
+    const int i = 1 + (int)(t % g.mx);
+    const long long r = t / g.mx;
+    const int s = (int)(r % 3), l = (int)(r / 3);   // l: 0,1 low ghosts, 2..4 high ghosts
+    if (i < 3 || i > g.mx - 3) return;
+    const int k = l < 2 ? 1 + l : g.mz - 2 + (l - 2);
+    const int src = l < 2 ? k + g.nz : k - g.nz;
+    double *F[3] = {F0, F1, F2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) F[c][mi(g, i, s, k)] = F[c][mi(g, i, s, src)];
+}
+
+__global__ void k_cylm_wall(GridP g, double *__restrict__ F0, double *__restrict__ F1, double *__restrict__ F2)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3LL * 3 * g.mz) return;
+    const int i = g.mx - 2 + (int)(t % 3);
+    const long long r = t / 3;
+    const int s = (int)(r % 3), k = 1 + (int)(r / 3);
+    F0[mi(g, i, s, k)] = 0.0;
+    F1[mi(g, i, s, k)] = 0.0;
+    F2[mi(g, i, s, k)] = 0.0;
+}
+
+// AxisCurrentBC (cyl_movdep.F90:451-465) at fixed theta: on every slot alike
+__global__ void k_cylm_axis_current(GridP g, double *__restrict__ Jx, double *__restrict__ Jy,
+                                    double *__restrict__ Jz)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3LL * g.mz) return;
+    const int s = (int)(t % 3), k = 1 + (int)(t / 3);
+    Jy[mi(g, 4, s, k)] = Jy[mi(g, 4, s, k)] - Jy[mi(g, 3, s, k)];
+    Jz[mi(g, 4, s, k)] = Jz[mi(g, 4, s, k)] - Jz[mi(g, 3, s, k)];
+    Jx[mi(g, 3, s, k)] = 0.0;
+    Jy[mi(g, 2, s, k)] = 0.0;
+    Jz[mi(g, 2, s, k)] = 0.0;
+}
+
+// periodic z fold of J: 3..5 += mz-2..mz, then mz-4..mz-3 += 1..2 (two launches)
+__global__ void k_cylm_zfold(GridP g, int high, double *__restrict__ J0, double *__restrict__ J1,
+                             double *__restrict__ J2)
+{
+    const int nl = high ? 2 : 3;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.mx * 3 * nl) return;
+    const int i = 1 + (int)(t % g.mx);
+    const long long r = t / g.mx;
+    const int s = (int)(r % 3), l = (int)(r / 3);
+    const int k = high ? g.mz - 4 + l : 3 + l;
+    const int src = high ? k - g.nz : k + g.nz;
+    double *J[3] = {J0, J1, J2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) J[c][mi(g, i, s, k)] = J[c][mi(g, i, s, k)] + J[c][mi(g, i, s, src)];
+}
+
+// node values of the staggered fields (no theta averaging: the modes are exact in theta)
+__device__ inline void node_vals(const GridP &g, const double *const *E, const double *const *B, int i, int s, int k,
+                                 double e[3], double b[3])
+{
+    const long long o = mi(g, i, s, k), sk = 3LL * g.mx;
+    e[0] = (E[0][o - 1] + E[0][o]) * 0.5;
+    e[1] = E[1][o];
+    e[2] = (E[2][o - sk] + E[2][o]) * 0.5;
+    b[0] = (B[0][o - sk] + B[0][o]) * 0.5;
+    b[1] = (B[1][o - 1 - sk] + B[1][o - 1] + B[1][o - sk] + B[1][o]) * 0.25;
+    b[2] = (B[2][o - 1] + B[2][o]) * 0.5;
+}
+
+__device__ inline void gather(const GridP &g, const double *const *E, const double *const *B, double x, double th,
+                              double z, double pe[3], double pb[3])
+{
+    const int ip = (int)x, kp = (int)z;
+    const double dx = x - ip, dz = z - kp;
+    const double ph[3] = {1.0, cos(th), sin(th)};
+    for (int c = 0; c < 3; c++) { pe[c] = 0.0; pb[c] = 0.0; }
+    for (int s = 0; s < 3; s++) {
+        double se[3] = {0.0, 0.0, 0.0}, sb[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int nn = 0; nn < 4; nn++) {
+            const int a = nn & 1, b = nn >> 1;
+            const double wt = (a ? dx : 1.0 - dx) * (b ? dz : 1.0 - dz);
+            double e[3], bb[3];
+            node_vals(g, E, B, ip + a, s, kp + b, e, bb);
+            for (int c = 0; c < 3; c++) { se[c] = se[c] + wt * e[c]; sb[c] = sb[c] + wt * bb[c]; }
+        }
+        for (int c = 0; c < 3; c++) { pe[c] = pe[c] + ph[s] * se[c]; pb[c] = pb[c] + ph[s] * sb[c]; }
+    }
+}
+
+// MoveDepositPrtl: gather, Boris, the local-frame move and rotation (cyl_movdep.F90:14-289),
+// then the modal deposit (r-z zigzag at the old angle, angular move at the new r, z)
+__global__ __launch_bounds__(256) void k_cylm_push(GridP g, CylmP q, const double *__restrict__ Ex,
+                                                   const double *__restrict__ Ey, const double *__restrict__ Ez,
+                                                   const double *__restrict__ Bx, const double *__restrict__ By,
+                                                   const double *__restrict__ Bz, double *__restrict__ Jx,
+                                                   double *__restrict__ Jy, double *__restrict__ Jz, PartF64 P,
+                                                   long long n, int *__restrict__ err)
+{
+    const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const int flv = P.flv[m];
+    if (flv == 0) return;
+    const double *E[3] = {Ex, Ey, Ez}, *B[3] = {Bx, By, Bz};
+    const double x0 = P.x[m], th0 = P.y[m], z0 = P.z[m];
+    const double qm = q.flvrqm[flv - 1] * 0.5;
+    const double c = q.c, sqc = q.sqc, cinv = q.cinv;
+    double pe[3], pb[3];
+    gather(g, E, B, x0, th0, z0, pe, pb);
+    const double pEx = pe[0] * qm, pEy = pe[1] * qm, pEz = pe[2] * qm;
+    double pBx = pb[0] * qm, pBy = pb[1] * qm, pBz = pb[2] * qm;
+    double u0 = c * P.u[m] + pEx;
+    double v0 = c * P.v[m] + pEy;
+    double w0 = c * P.w[m] + pEz;
+    double gg = 1.0 / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+    pBx = gg * pBx; pBy = gg * pBy; pBz = gg * pBz;
+    const double ff = 2.0 / (1.0 + pBx * pBx + pBy * pBy + pBz * pBz);
+    double u1 = (u0 + v0 * pBz - w0 * pBy) * ff;
+    double v1 = (v0 + w0 * pBx - u0 * pBz) * ff;
+    const double w1 = (w0 + u0 * pBy - v0 * pBx) * ff;
+    u0 = u0 + v1 * pBz - w1 * pBy + pEx;
+    v0 = v0 + w1 * pBx - u1 * pBz + pEy;
+    w0 = w0 + u1 * pBy - v1 * pBx + pEz;
+    u1 = u0 * cinv;
+    v1 = v0 * cinv;
+    const double wn = w0 * cinv;
+    gg = sqc / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+    const double X = rad(q, x0) + u1 * gg, Y = v1 * gg;
+    const double r1 = sqrt(X * X + Y * Y);
+    const double x1 = r1 * q.inv_dr + 3.5;
+    const double th1 = th0 + atan2(Y, X);
+    const double z1 = z0 + wn * gg * q.inv_dz;
+    const double ir = 1.0 / r1;
+    const double un = (X * u1 + Y * v1) * ir;
+    const double vn = (-Y * u1 + X * v1) * ir;
+    if (x1 >= g.mx - 3) {
+        atomicOr(err, 1);
+    } else {
+        const double qthis = P.q[m] * q.qi;
+        const int i1 = (int)floor(x0), i2 = (int)floor(x1), k1 = (int)floor(z0), k2 = (int)floor(z1);
+        const double xr = fmin((double)(min(i1, i2) + 1), fmax((double)max(i1, i2), 0.5 * (x0 + x1)));
+        const double zr = fmin((double)(min(k1, k2) + 1), fmax((double)max(k1, k2), 0.5 * (z0 + z1)));
+        const double Fx1 = qthis * (xr - x0), Fz1 = qthis * (zr - z0);
+        const double Fx2 = qthis * (x1 - xr), Fz2 = qthis * (z1 - zr);
+        const double Wx1 = 0.5 * (x0 + xr) - i1, Wz1 = 0.5 * (z0 + zr) - k1;
+        const double Wx2 = 0.5 * (x1 + xr) - i2, Wz2 = 0.5 * (z1 + zr) - k2;
+        const double pa[3] = {1.0, cos(th0), sin(th0)};
+        const long long sk = 3LL * g.mx;
+        for (int s = 0; s < 3; s++) {
+            const long long a = mi(g, i1, s, k1), b = mi(g, i2, s, k2);
+            unsafeAtomicAdd(&Jx[a], pa[s] * Fx1 * (1.0 - Wz1));
+            unsafeAtomicAdd(&Jx[a + sk], pa[s] * Fx1 * Wz1);
+            unsafeAtomicAdd(&Jz[a], pa[s] * Fz1 * (1.0 - Wx1));
+            unsafeAtomicAdd(&Jz[a + 1], pa[s] * Fz1 * Wx1);
+            unsafeAtomicAdd(&Jx[b], pa[s] * Fx2 * (1.0 - Wz2));
+            unsafeAtomicAdd(&Jx[b + sk], pa[s] * Fx2 * Wz2);
+            unsafeAtomicAdd(&Jz[b], pa[s] * Fz2 * (1.0 - Wx2));
+            unsafeAtomicAdd(&Jz[b + 1], pa[s] * Fz2 * Wx2);
+        }
+        const double wx = x1 - i2, wz = z1 - k2;
+        const double pt[3] = {qthis * (th1 - th0), qthis * (sin(th1) - sin(th0)), -(qthis * (cos(th1) - cos(th0)))};
+        for (int s = 0; s < 3; s++) {
+            const long long b = mi(g, i2, s, k2);
+            unsafeAtomicAdd(&Jy[b], pt[s] * (1.0 - wx) * (1.0 - wz));
+            unsafeAtomicAdd(&Jy[b + 1], pt[s] * wx * (1.0 - wz));
+            unsafeAtomicAdd(&Jy[b + sk], pt[s] * (1.0 - wx) * wz);
+            unsafeAtomicAdd(&Jy[b + 1 + sk], pt[s] * wx * wz);
+        }
+    }
+    double zn = z1, tn = th1;
+    if (zn > q.zmax) zn = zn - q.zlen;
+    else if (zn < q.zmin) zn = q.zlen + zn;
+    if (tn >= 2.0 * kPi) tn = tn - 2.0 * kPi;
+    else if (tn < 0.0) tn = tn + 2.0 * kPi;
+    P.x[m] = x1; P.y[m] = tn; P.z[m] = zn;
+    P.u[m] = un; P.v[m] = vn; P.w[m] = wn;
+}
+
+__device__ inline uint64_t mix64d(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ inline double u01d(uint64_t seed, uint64_t idx, uint64_t stream)
+{
+    const uint64_t h = mix64d(mix64d(seed ^ (idx * 0xD1342543DE82EF95ULL)) + stream * 0x9E3779B97F4A7C15ULL);
+    return (double)(h >> 24) * (1.0 / 1099511627776.0);
+}
+
+// uniform cold electrons: ppc per cell of i = 4..4+nrp-1, k = k0..k0+nzc-1, (r, z)
+// uniform in the cell, ppc evenly spaced angles with a per-cell random offset,
+// weight epc 2 pi r dr / ppc; counter-based, so a cell (shifted by `shift`
+// window moves) always gets the same particles.  Appended at base + index.
+__global__ void k_cylm_inject(GridP g, CylmP q, PartF64 P, long long base, int nrp, int k0, int nzc, int ppc,
+                              int epc, unsigned long long seed, long long shift)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)nrp * nzc * ppc;
+    if (t >= total) return;
+    const int p = (int)(t % ppc);
+    const long long cidx = t / ppc;
+    const int i = 4 + (int)(cidx % nrp), k = k0 + (int)(cidx / nrp);
+    const uint64_t cell = (uint64_t)((k - 3 + shift) * (long long)g.nx + (i - 4));
+    const double off = u01d(seed, cell, 7);
+    const uint64_t id = cell * (uint64_t)ppc + (uint64_t)p;
+    const double x = i + u01d(seed, id, 1), z = k + u01d(seed, id, 2);
+    const double th = 2.0 * kPi * ((p + off) / ppc);
+    const long long n = base + t;
+    P.x[n] = x; P.y[n] = th; P.z[n] = z;
+    P.u[n] = 0.0; P.v[n] = 0.0; P.w[n] = 0.0;
+    P.q[n] = -(epc * 2.0 * kPi * rad(q, x) * q.dr / ppc);
+    P.flv[n] = 2;
+}
+
+// a linearly polarised (x) Gaussian pulse moving along +z, at focus (no curvature):
+// E_x = E0 exp(-r^2/w0^2 - (z - zc)^2/L^2) cos(k0 (z - zc)), B_y = E_x at B's
+// stagger, added to the m = 1 slots: E_r,c = E_x, E_theta,s = -E_x, B_r,s = B_y,
+// B_theta,c = B_y.  E0 = a0 c w0dt (the quiver amplitude of gamma beta is a0).
+__global__ void k_cylm_laser(GridP g, CylmP q, double E0, double k0, double w0, double L, double zc,
+                             double *__restrict__ Ex, double *__restrict__ Ey, double *__restrict__ Bx,
+                             double *__restrict__ By)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.mx * g.mz) return;
+    const int i = 1 + (int)(t % g.mx), k = 1 + (int)(t / g.mx);
+    if (i < 4 || i > g.mx - 3) return;
+    auto prof = [&](double r, double z) {
+        const double dz = z - zc;
+        return E0 * exp(-(r * r) / (w0 * w0) - (dz * dz) / (L * L)) * cos(k0 * dz);
+    };
+    // staggers: E_r at r(i) + dr/2, z(k); E_theta at r(i), z(k); B_r at r(i), z(k) + 1/2;
+    // B_theta at r(i) + dr/2, z(k) + 1/2 (the z centring of the Yee cell).  Between steps
+    // B is half a step behind E (the E update uses B after both half pushes), so B is the
+    // pulse half a step earlier, c/2 cells further back: a purely forward wave
+    const double r = rad(q, i), rh = r + 0.5 * q.dr, z = (double)k, zh = k + 0.5 + 0.5 * q.c;
+    Ex[mi(g, i, 1, k)] += prof(rh, z);
+    Ey[mi(g, i, 2, k)] += -prof(r, z);
+    Bx[mi(g, i, 2, k)] += prof(r, zh);
+    By[mi(g, i, 1, k)] += prof(rh, zh);
+}
+
+// the moving window: every slot and z row moves one cell back, dst(k) = src(k + 1)
+// from a copy; the front row k = mz - 3 and the ghosts above it restart at zero
+__global__ void k_cylm_shift_fields(GridP g, Ptrs6<double> P)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.ncell) return;
+    const double *__restrict__ src = P.p[blockIdx.y];
+    double *__restrict__ dst = P.p[3 + blockIdx.y];
+    const int k = 1 + (int)(t / (3LL * g.mx));
+    dst[t] = (k + 1 <= g.mz - 3) ? src[t + 3LL * g.mx] : 0.0;
+}
+
+// particles: z -= 1; those below the window's start are dropped (flavour 0)
+__global__ void k_cylm_shift_particles(CylmP q, PartF64 P, long long n)
+{
+    const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n || P.flv[m] == 0) return;
+    const double z = P.z[m] - 1.0;
+    P.z[m] = z;
+    if (z < q.zmin) P.flv[m] = 0;
+}
+
+// 1/2 sum B^2, E^2 with the volume 2 pi r dr (pi for the m = 1 slots) of each stagger
+__global__ void k_cylm_energy(GridP g, CylmP q, const double *__restrict__ Ex, const double *__restrict__ Ey,
+                              const double *__restrict__ Ez, const double *__restrict__ Bx,
+                              const double *__restrict__ By, const double *__restrict__ Bz, double *partial)
+{
+    __shared__ double sb[256], se[256];
+    const int ni = g.mx - 6, nk = g.mz - 5;
+    const long long total = (long long)ni * 3 * nk;
+    double be = 0.0, ee = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        long long r0 = t;
+        const int i = 4 + (int)(r0 % ni);
+        r0 /= ni;
+        const int s = (int)(r0 % 3), k = 3 + (int)(r0 / 3);
+        const long long o = mi(g, i, s, k);
+        const double wm = (s == 0 ? 2.0 * kPi : kPi) * q.dr;
+        const double r = rad(q, i), rh = r + 0.5 * q.dr;
+        ee += wm * (rh * Ex[o] * Ex[o] + r * Ey[o] * Ey[o] + r * Ez[o] * Ez[o]);
+        be += wm * (r * Bx[o] * Bx[o] + rh * By[o] * By[o] + rh * Bz[o] * Bz[o]);
+    }
+    sb[threadIdx.x] = be;
+    se[threadIdx.x] = ee;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) { sb[threadIdx.x] += sb[threadIdx.x + w]; se[threadIdx.x] += se[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = 0.5 * sb[0]; partial[2 * blockIdx.x + 1] = 0.5 * se[0]; }
+}
+
+// kinetic energy sum (gamma - 1) |q| per flavour (CalcPrtlEnergy, savedata_routines.F90:283-308
+// without the mass factor) and the live count
+__global__ void k_cylm_kinetic(PartF64 P, long long n, double *partial)
+{
+    __shared__ double s1[256], s2[256], sn[256];
+    double a = 0.0, b = 0.0, cnt = 0.0;
+    for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (long long)gridDim.x * blockDim.x) {
+        const int f = P.flv[m];
+        if (f == 0) continue;
+        const double u = P.u[m], v = P.v[m], w = P.w[m];
+        const double e = (sqrt(1.0 + u * u + v * v + w * w) - 1.0) * fabs(P.q[m]);
+        if (f == 1) a += e; else b += e;
+        cnt += 1.0;
+    }
+    s1[threadIdx.x] = a; s2[threadIdx.x] = b; sn[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s1[threadIdx.x] += s1[threadIdx.x + w]; s2[threadIdx.x] += s2[threadIdx.x + w]; sn[threadIdx.x] += sn[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[3 * blockIdx.x] = s1[0]; partial[3 * blockIdx.x + 1] = s2[0]; partial[3 * blockIdx.x + 2] = sn[0];
+    }
+}
+
+// stream compaction of the live particles into `out` (order not kept); count in *cnt
+__global__ void k_cylm_compact(PartF64 P, long long n, PartF64 O, unsigned long long *cnt)
+{
+    const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n || P.flv[m] == 0) return;
+    const unsigned long long d = atomicAdd(cnt, 1ull);
+    O.q[d] = P.q[m]; O.x[d] = P.x[m]; O.y[d] = P.y[m]; O.z[d] = P.z[m];
+    O.u[d] = P.u[m]; O.v[d] = P.v[m]; O.w[d] = P.w[m]; O.flv[d] = P.flv[m];
+}
+
+void fields(pictor_ctx *c, double *E[3], double *B[3], double *J[3])
+{
+    for (int i = 0; i < 3; i++) {
+        E[i] = (double *)c->E[i];
+        B[i] = (double *)c->B[i];
+        J[i] = (double *)c->J[i];
+    }
+}
+
+inline int nblocks(long long n, int bs) { return (int)std::max<long long>(1, (n + bs - 1) / bs); }
+dim3 blocks_of(long long n) { return dim3(nblocks(n, 256)); }
+
+void halo(pictor_ctx *c, double *const F[3])
+{
+    const GridP &g = c->g;
+    hipLaunchKernelGGL(k_cylm_zhalo, blocks_of((long long)g.mx * 15), dim3(256), 0, c->stream, g, F[0], F[1], F[2]);
+    hipLaunchKernelGGL(k_cylm_wall, blocks_of(9LL * g.mz), dim3(256), 0, c->stream, g, F[0], F[1], F[2]);
+}
+
+void free_parts(PartF64 &p)
+{
+    dfree(p.q); dfree(p.x); dfree(p.y); dfree(p.z); dfree(p.u); dfree(p.v); dfree(p.w); dfree(p.flv);
+    p = PartF64{};
+}
+
+int alloc_parts(PartF64 &p, long long cap)
+{
+    double **a[7] = {&p.q, &p.x, &p.y, &p.z, &p.u, &p.v, &p.w};
+    for (auto x : a) HIPTRY(dalloc(x, cap));
+    HIPTRY(dalloc(&p.flv, cap));
+    return PICTOR_OK;
+}
+
+// room for `extra` more particles: compact first, then grow (x1.25) if still short
+int ensure_room(pictor_ctx *c, long long extra)
+{
+    CylmState &st = c->cm;
+    if (c->np + extra <= st.cap) return PICTOR_OK;
+    long long live = c->np;
+    if (c->np > 0) {
+        PartF64 O{};
+        const long long cap2 = std::max<long long>(st.cap, (long long)((c->np + extra) * 1.25) + 1024);
+        int rc = alloc_parts(O, cap2);
+        if (rc) return rc;
+        HIPTRY(hipMemsetAsync(st.d_count, 0, sizeof(unsigned long long), c->stream));
+        hipLaunchKernelGGL(k_cylm_compact, blocks_of(c->np), dim3(256), 0, c->stream, c->pcyl, c->np, O, st.d_count);
+        unsigned long long h = 0;
+        HIPTRY(hipMemcpyAsync(&h, st.d_count, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPTRY(hipStreamSynchronize(c->stream));
+        free_parts(c->pcyl);
+        c->pcyl = O;
+        st.cap = cap2;
+        live = (long long)h;
+    } else {
+        free_parts(c->pcyl);
+        const long long cap2 = (long long)(extra * 1.25) + 1024;
+        int rc = alloc_parts(c->pcyl, cap2);
+        if (rc) return rc;
+        st.cap = cap2;
+        live = 0;
+    }
+    c->np = live;
+    return PICTOR_OK;
+}
+
+}  // namespace
+
+void cylm_params(const pictor_config &cfg, const PhysP &ph, CylmP &q)
+{
+    q.dr = cfg.dr > 0.0 ? cfg.dr : 1.0;
+    q.inv_dr = 1.0 / q.dr;
+    q.inv_dz = 1.0;
+    q.c = ph.c; q.fldc = ph.fldc; q.fld_halfc = ph.fld_halfc; q.cinv = ph.cinv; q.sqc = ph.sqc;
+    q.qi = ph.qi;
+    q.flvrqm[0] = ph.flvrqm[0];
+    q.flvrqm[1] = ph.flvrqm[1];
+    q.zmin = 3.0; q.zmax = 3.0 + cfg.nz; q.zlen = cfg.nz;
+}
+
+}  // namespace pictor
+
+using namespace pictor;
+
+int cylm_setup(pictor_ctx *c)
+{
+    CylmState &st = c->cm;
+    if (!st.d_err) {
+        HIPTRY(dalloc(&st.d_err, 1));
+        HIPTRY(dalloc(&st.d_count, 1));
+        HIPTRY(dalloc(&st.d_partial, 3 * 1024));
+        HIPTRY(hipMemsetAsync(st.d_err, 0, sizeof(int), c->stream));
+    }
+    return PICTOR_OK;
+}
+
+void cylm_release(pictor_ctx *c)
+{
+    CylmState &st = c->cm;
+    free_parts(c->pcyl);
+    dfree(st.d_err); dfree(st.d_count); dfree(st.d_partial);
+    st = CylmState{};
+    c->np = 0;
+}
+
+int cylm_upload_particles(pictor_ctx *c, long long n, const double *q, const double *x, const double *y,
+                          const double *z, const double *u, const double *v, const double *w, const int *flv)
+{
+    const GridP &g = c->g;
+    std::vector<long long> keep;
+    keep.reserve(n);
+    for (long long i = 0; i < n; i++) {
+        if (flv[i] == 0) continue;
+        if (flv[i] != 1 && flv[i] != 2) return fail(PICTOR_ERR_ARG, "flavour must be 0, 1 or 2");
+        if (!(x[i] >= 3.5 && x[i] < g.mx - 3)) return fail(PICTOR_ERR_ARG, "particle outside 3.5 <= x < mx-3");
+        if (!(y[i] >= 0.0 && y[i] < 2.0 * kPi)) return fail(PICTOR_ERR_ARG, "theta (y) must lie in [0, 2 pi)");
+        if (!(z[i] >= 3.0 && z[i] <= g.mz - 2)) return fail(PICTOR_ERR_ARG, "particle outside 3 <= z <= mz-2");
+        keep.push_back(i);
+    }
+    const long long m = (long long)keep.size();
+    free_parts(c->pcyl);
+    c->cm.cap = 0;
+    c->np = 0;
+    int rc = cylm_setup(c);
+    if (rc) return rc;
+    if ((rc = ensure_room(c, m))) return rc;
+    PartF64 &p = c->pcyl;
+    double *arr[7] = {p.q, p.x, p.y, p.z, p.u, p.v, p.w};
+    const double *src[7] = {q, x, y, z, u, v, w};
+    std::vector<double> buf(std::max<long long>(m, 1));
+    for (int a = 0; a < 7; a++) {
+        for (long long e = 0; e < m; e++) buf[e] = src[a][keep[e]];
+        HIPTRY(hipMemcpy(arr[a], buf.data(), m * sizeof(double), hipMemcpyHostToDevice));
+    }
+    std::vector<int> fb(std::max<long long>(m, 1));
+    for (long long e = 0; e < m; e++) fb[e] = flv[keep[e]];
+    HIPTRY(hipMemcpy(p.flv, fb.data(), m * sizeof(int), hipMemcpyHostToDevice));
+    c->np = m;
+    return PICTOR_OK;
+}
+
+int cylm_b_half(pictor_ctx *c)
+{
+    const GridP &g = c->g;
+    double *E[3], *B[3], *J[3];
+    fields(c, E, B, J);
+    const long long nb = (long long)(g.mx - 4) * 3 * (g.mz - 1);
+    hipLaunchKernelGGL(k_cylm_update_b, blocks_of(nb), dim3(256), 0, c->stream, g, c->cq, B[0], B[1], B[2], E[0], E[1],
+                       E[2]);
+    hipLaunchKernelGGL(k_cylm_b_axis, blocks_of(3LL * g.mz), dim3(256), 0, c->stream, g, c->cq, B[0], B[1], B[2], E[1]);
+    halo(c, B);
+    HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
+int cylm_e_phase(pictor_ctx *c, bool curl, bool add)
+{
+    const GridP &g = c->g;
+    double *E[3], *B[3], *J[3];
+    fields(c, E, B, J);
+    const long long ne = (long long)(g.mx - 6) * 3 * (g.mz - 5);
+    if (curl)
+        hipLaunchKernelGGL(k_cylm_update_e, blocks_of(ne), dim3(256), 0, c->stream, g, c->cq, E[0], E[1], E[2], B[0],
+                           B[1], B[2]);
+    if (add)
+        hipLaunchKernelGGL(k_cylm_add_current, blocks_of(ne), dim3(256), 0, c->stream, g, c->cq, E[0], E[1], E[2],
+                           J[0], J[1], J[2]);
+    hipLaunchKernelGGL(k_cylm_e_axis, blocks_of(3LL * g.mz), dim3(256), 0, c->stream, g, E[0], E[1], E[2]);
+    halo(c, E);
+    HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
+int cylm_current_halo(pictor_ctx *c)
+{
+    const GridP &g = c->g;
+    double *E[3], *B[3], *J[3];
+    fields(c, E, B, J);
+    hipLaunchKernelGGL(k_cylm_axis_current, blocks_of(3LL * g.mz), dim3(256), 0, c->stream, g, J[0], J[1], J[2]);
+    hipLaunchKernelGGL(k_cylm_zfold, blocks_of(9LL * g.mx), dim3(256), 0, c->stream, g, 0, J[0], J[1], J[2]);
+    hipLaunchKernelGGL(k_cylm_zfold, blocks_of(6LL * g.mx), dim3(256), 0, c->stream, g, 1, J[0], J[1], J[2]);
+    HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
+int cylm_field_halo(pictor_ctx *c, int which)
+{
+    double *E[3], *B[3], *J[3];
+    fields(c, E, B, J);
+    if (which == 0) {
+        hipLaunchKernelGGL(k_cylm_e_axis, blocks_of(3LL * c->g.mz), dim3(256), 0, c->stream, c->g, E[0], E[1], E[2]);
+        halo(c, E);
+    } else {
+        halo(c, B);
+    }
+    HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
+int cylm_reset_current(pictor_ctx *c)
+{
+    double *E[3], *B[3], *J[3];
+    fields(c, E, B, J);
+    launch_fill_zero_n<double>(J, 3, c->g.ncell, c->stream);
+    HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
+int cylm_push(pictor_ctx *c, bool timed)
+{
+    double *E[3], *B[3], *J[3];
+    fields(c, E, B, J);
+    if (timed) HIPTRY(hipEventRecord(c->ev[0], c->stream));
+    if (c->np > 0)
+        hipLaunchKernelGGL(k_cylm_push, blocks_of(c->np), dim3(256), 0, c->stream, c->g, c->cq, E[0], E[1], E[2], B[0],
+                           B[1], B[2], J[0], J[1], J[2], c->pcyl, c->np, c->cm.d_err);
+    if (timed) HIPTRY(hipEventRecord(c->ev[1], c->stream));
+    HIPTRY(hipGetLastError());
+    c->pending_swap = true;
+    return PICTOR_OK;
+}
+
+int cylm_exchange(pictor_ctx *c)
+{
+    int err = 0;
+    HIPTRY(hipMemcpyAsync(&err, c->cm.d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    c->pending_swap = false;
+    if (err) return fail(PICTOR_ERR_BOUNDS, "modal cylinder: a particle reached the outer wall");
+    return PICTOR_OK;
+}
+
+int cylm_inject(pictor_ctx *c, int nrp, int k0, int nzc, int ppc, unsigned long long seed, long long shift)
+{
+    const GridP &g = c->g;
+    if (nrp < 1 || nrp > g.nx - 1 || ppc < 1 || nzc < 1 || k0 < 3 || k0 + nzc - 1 > g.mz - 3)
+        return fail(PICTOR_ERR_ARG, "plasma cells outside the grid (4 <= i < 4 + nr_plasma <= mx - 3, 3 <= k <= mz - 3)");
+    int rc = cylm_setup(c);
+    if (rc) return rc;
+    const long long add = (long long)nrp * nzc * ppc;
+    if ((rc = ensure_room(c, add))) return rc;
+    hipLaunchKernelGGL(k_cylm_inject, blocks_of(add), dim3(256), 0, c->stream, g, c->cq, c->pcyl, c->np, nrp, k0, nzc,
+                       ppc, c->cfg.epc, seed, shift);
+    HIPTRY(hipGetLastError());
+    c->np += add;
+    return PICTOR_OK;
+}
+
+int cylm_laser(pictor_ctx *c, double a0, double omega0, double tau, double w0, double zc)
+{
+    // code units: one axial cell = dz; omega_p dt = ompe, c cells per step
+    const double cc = c->ph.c, ompe = cc / c->cfg.c_ompe;
+    const double w0dt = omega0 * ompe;                 // laser phase advance per step
+    const double E0 = a0 * cc * w0dt;
+    const double k0 = w0dt / cc;                       // rad per axial cell
+    const double cells_per_skin = cc / ompe;           // c / omega_p in axial cells
+    double *E[3], *B[3], *J[3];
+    fields(c, E, B, J);
+    hipLaunchKernelGGL(k_cylm_laser, blocks_of((long long)c->g.mx * c->g.mz), dim3(256), 0, c->stream, c->g, c->cq, E0,
+                       k0, w0 * cells_per_skin, tau * cells_per_skin, zc, E[0], E[1], B[0], B[1]);
+    HIPTRY(hipGetLastError());
+    int rc = cylm_field_halo(c, 0);
+    return rc ? rc : cylm_field_halo(c, 1);
+}
+
+// after a step: advance the window by whole cells at c (one cell per 1/c steps)
+int cylm_window_after_step(pictor_ctx *c)
+{
+    CylmState &st = c->cm;
+    if (!st.win_on) return PICTOR_OK;
+    st.win_steps += 1;
+    while ((double)st.win_steps * c->ph.c >= (double)(st.win_shift + 1)) {
+        double *E[3], *B[3], *J[3];
+        fields(c, E, B, J);
+        // J (zeroed again by the next step's ResetCurrent) holds the copy being shifted
+        for (double **F : {E, B}) {
+            launch_copy_n<double>(J, F, 3, c->g.ncell, c->stream);
+            Ptrs6<double> P{{J[0], J[1], J[2], F[0], F[1], F[2]}};
+            hipLaunchKernelGGL(k_cylm_shift_fields, dim3(nblocks(c->g.ncell, 256), 3), dim3(256), 0, c->stream, c->g, P);
+        }
+        launch_fill_zero_n<double>(J, 3, c->g.ncell, c->stream);
+        if (c->np > 0)
+            hipLaunchKernelGGL(k_cylm_shift_particles, blocks_of(c->np), dim3(256), 0, c->stream, c->cq, c->pcyl, c->np);
+        st.win_shift += 1;
+        int rc = PICTOR_OK;
+        if (st.win_nrp > 0 && st.win_ppc > 0 &&
+            (rc = cylm_inject(c, st.win_nrp, c->g.mz - 3, 1, st.win_ppc, st.win_seed, st.win_shift)))
+            return rc;
+        if ((rc = cylm_field_halo(c, 0)) || (rc = cylm_field_halo(c, 1))) return rc;
+    }
+    return PICTOR_OK;
+}
+
+int cylm_diagnostics(pictor_ctx *c, double out[9])
+{
+    double *E[3], *B[3], *J[3];
+    fields(c, E, B, J);
+    CylmState &st = c->cm;
+    int rc = cylm_setup(c);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_cylm_energy, dim3(1024), dim3(256), 0, c->stream, c->g, c->cq, E[0], E[1], E[2], B[0], B[1],
+                       B[2], st.d_partial);
+    std::vector<double> h(3 * 1024);
+    HIPTRY(hipMemcpyAsync(h.data(), st.d_partial, 2 * 1024 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    double be = 0, ee = 0;
+    for (int b = 0; b < 1024; b++) { be += h[2 * b]; ee += h[2 * b + 1]; }
+    double k1 = 0, k2 = 0, cnt = 0;
+    if (c->np > 0) {
+        hipLaunchKernelGGL(k_cylm_kinetic, dim3(1024), dim3(256), 0, c->stream, c->pcyl, c->np, st.d_partial);
+        HIPTRY(hipMemcpyAsync(h.data(), st.d_partial, 3 * 1024 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPTRY(hipStreamSynchronize(c->stream));
+        for (int b = 0; b < 1024; b++) { k1 += h[3 * b]; k2 += h[3 * b + 1]; cnt += h[3 * b + 2]; }
+    }
+    for (int i = 0; i < 9; i++) out[i] = 0.0;
+    out[0] = be; out[1] = ee; out[2] = k1; out[3] = k2;
+    (void)cnt;
+    return PICTOR_OK;
+}

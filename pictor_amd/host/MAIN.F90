@@ -98,7 +98,7 @@ contains
       Ex = 0; Ey = 0; Ez = 0; Bx = 0; By = 0; Bz = 0; Jx = 0; Jy = 0; Jz = 0
       call InitSetup
       cfg = pictor_config(2, nx, ny, 1, epc, form, 0, fdtd, PICTOR_GEOM_CARTESIAN, curr_filter, c, c_ompe, g0, mi_me, &
-                          0.0d0)
+                          0.0d0, 0.0d0)
       call check(pictor_init(cfg, gpu), 'pictor_init')
       call check(pictor_upload_fields(gpu, Ex, Ey, Ez, Bx, By, Bz), 'pictor_upload_fields')
       call check(pictor_upload_particles(gpu, used_prtl_arr_size, qp, xp, yp, zp, up, vp, wp, flvp), &

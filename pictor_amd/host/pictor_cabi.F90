@@ -16,10 +16,11 @@ module pictor_cabi
    integer(c_int32_t), parameter :: PICTOR_FDTD_LEHE = 1
    integer(c_int32_t), parameter :: PICTOR_GEOM_CARTESIAN = 0
    integer(c_int32_t), parameter :: PICTOR_GEOM_CYL = 1
+   integer(c_int32_t), parameter :: PICTOR_GEOM_CYL_MODAL = 2
 
    type, bind(C) :: pictor_config
       integer(c_int32_t) :: ndim, nx, ny, nz, epc, form, tile, fdtd, geometry, curr_filter
-      real(c_double) :: c, c_ompe, g0, mi_me, capacity_factor
+      real(c_double) :: c, c_ompe, g0, mi_me, capacity_factor, dr
    end type pictor_config
 
    interface

@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (FDTD_LEHE, FDTD_YEE, FORM_FP32_SPLIT, FORM_FP64, GEOM_CARTESIAN, GEOM_CYL, PictorConfig, check,
+from ._lib import (FDTD_LEHE, FDTD_YEE, FORM_FP32_SPLIT, FORM_FP64, GEOM_CARTESIAN, GEOM_CYL, GEOM_CYL_MODAL, PictorConfig, check,
                    lib)
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -34,16 +34,16 @@ class Simulation:
     """
 
     def __init__(self, nx, ny, nz=1, ndim=2, epc=16, c=0.45, c_ompe=10.0, g0=1.0, mi_me=1.0,
-                 form="fp64", tile=0, capacity_factor=0.0, fdtd="yee", geometry="cartesian", curr_filter=0):
+                 form="fp64", tile=0, capacity_factor=0.0, fdtd="yee", geometry="cartesian", curr_filter=0, dr=0.0):
         if fdtd not in ("yee", "lehe"):
             raise ValueError("fdtd must be 'yee' or 'lehe'")
-        if geometry not in ("cartesian", "cyl"):
-            raise ValueError("geometry must be 'cartesian' or 'cyl'")
+        if geometry not in ("cartesian", "cyl", "cyl_modal"):
+            raise ValueError("geometry must be 'cartesian', 'cyl' or 'cyl_modal'")
         self.geometry = geometry
         self.cfg = PictorConfig(ndim, nx, ny, nz, epc, FORM_FP64 if form == "fp64" else FORM_FP32_SPLIT,
                                 tile, FDTD_LEHE if fdtd == "lehe" else FDTD_YEE,
-                                GEOM_CYL if geometry == "cyl" else GEOM_CARTESIAN, curr_filter, c, c_ompe, g0, mi_me,
-                                capacity_factor)
+                                {"cyl": GEOM_CYL, "cyl_modal": GEOM_CYL_MODAL}.get(geometry, GEOM_CARTESIAN),
+                                curr_filter, c, c_ompe, g0, mi_me, capacity_factor, dr)
         if form not in ("fp64", "fp32_split"):
             raise ValueError("form must be 'fp64' or 'fp32_split'")
         self.form = form
@@ -126,6 +126,18 @@ class Simulation:
 
     def UpdateCurrentAllFaces(self):     # comm_fld.F90:15-33
         check(lib().pictor_current_halo_add(self._ctx))
+
+    # ---- the azimuthal-mode cylinder (geometry "cyl_modal") ----
+    def cyl_plasma(self, nr_plasma, k0, nzc, ppc, seed=1):
+        """cold uniform electrons in radial cells 4..4+nr_plasma-1, axial cells k0..k0+nzc-1"""
+        check(lib().pictor_cyl_plasma(self._ctx, nr_plasma, k0, nzc, ppc, seed))
+
+    def cyl_laser(self, a0, omega0, tau, w0, zc):
+        """Gaussian x-polarised pulse along +z (omega0 in omega_p, tau in 1/omega_p, w0 in c/omega_p, zc in cells)"""
+        check(lib().pictor_cyl_laser(self._ctx, a0, omega0, tau, w0, zc))
+
+    def moving_window(self, on=True, nr_plasma=0, ppc=0, seed=1):
+        check(lib().pictor_moving_window(self._ctx, 1 if on else 0, nr_plasma, ppc, seed))
 
     def smoothen_current(self):          # current.F90:19-30 (curr_filter passes)
         check(lib().pictor_smoothen_current(self._ctx))

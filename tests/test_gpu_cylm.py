@@ -1,0 +1,234 @@
+"""The azimuthal-mode (m = 0, 1) cylinder on the GPU vs its oracle, through the C ABI.
+
+Bars:
+  * field updates, axis parity fills, z halos and wall: bit for bit;
+  * the plasma loader: the same particles bit for bit (counter-based);
+  * one push from identical state: positions and momenta within 1e-13
+    relative (device vs host cos, sin, atan2), current within 1e-12 of its
+    maximum (atomic order); 20 steps: fields and particles within 1e-9;
+  * the moving window: 12 steps with shifts and injection as the oracle's;
+  * the GPU run keeps the modal Gauss law node by node (1e-12);
+  * a laser pulse in vacuum keeps its energy and moves at the group speed;
+    a test electron in a weak pulse quivers with gamma beta_x = a0 (5 %);
+  * a Courant number above the limit of the m = 1 terms next to the axis is
+    rejected.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import CylModalOracle
+from pictor_amd import Simulation
+from tests._util import say
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
+
+
+def _pair(nr, nz, dr=1.0, c=0.3, epc=4, cap=4096):
+    say(f"[case] cylm nr={nr} nz={nz} dr={dr} c={c}")
+    o = CylModalOracle(nr, nz, dr=dr, c=c, c_ompe=10.0, epc=epc, cap=cap)
+    sim = Simulation(nr, 2, nz, ndim=3, epc=epc, c=c, c_ompe=10.0, form="fp64", geometry="cyl_modal", dr=dr,
+                     mi_me=1e30)
+    return o, sim
+
+
+def _smooth_fields(o, rng, amp=1e-3):
+    mx, _, mz = o.shape
+    i = np.arange(mx)[:, None] + 1.0
+    k = np.arange(mz)[None, :] + 1.0
+    F = {}
+    for n in FIELDS:
+        a = np.zeros(o.shape, order="F")
+        for s in range(3):
+            p, q, ph = rng.normal(size=3)
+            a[:, s, :] = amp * np.cos(0.3 * p * i + 0.2 * q * k + ph) * np.exp(-i / (0.7 * mx))
+        F[n] = a
+    return F
+
+
+def _upload_fields(o, sim, F):
+    o.set_fields(**F)
+    sim.upload_fields(*[np.asfortranarray(F[n]) for n in FIELDS])
+
+
+def test_field_updates_bitwise():
+    o, sim = _pair(16, 20, dr=2.0, c=0.5)
+    with sim:
+        _upload_fields(o, sim, _smooth_fields(o, np.random.default_rng(1)))
+        for _ in range(3):
+            sim.UpdateBfieldHalf(); o.b_half()
+            sim.UpdateEfield(); o.e_phase(curl=True)
+        G = sim.download_fields()
+    for n in FIELDS:
+        assert np.array_equal(G[n], o.F[n]), f"{n}: max diff {np.abs(G[n] - o.F[n]).max()}"
+
+
+def test_plasma_loader_bitwise():
+    o, sim = _pair(24, 16, dr=1.5, c=0.3)
+    with sim:
+        sim.cyl_plasma(12, 5, 9, 8, seed=77)
+        A = sim.download_particles()
+    B = o.load_plasma(12, 5, 9, 8, seed=77)
+    assert len(A["x"]) == len(B["x"]) == 12 * 9 * 8
+    for k in ("q", "x", "y", "z", "u", "v", "w", "flv"):
+        assert np.array_equal(A[k], B[k]), k
+
+
+def _thermal(o, rng, spread=0.1):
+    P = o.particles(live_only=False)
+    n = len(P["x"])
+    for k in "uvw":
+        P[k] = rng.normal(0.0, spread, n)
+    return P
+
+
+def test_push_and_steps_match_oracle():
+    o, sim = _pair(24, 24, dr=2.0, c=0.5, epc=4)
+    rng = np.random.default_rng(5)
+    o.load_plasma(14, 3, 24, 8, seed=5)
+    P = _thermal(o, rng)
+    o.load_particles(P["q"], P["x"], P["y"], P["z"], P["u"], P["v"], P["w"], P["flv"])
+    with sim:
+        _upload_fields(o, sim, _smooth_fields(o, rng, amp=5e-3))
+        sim.upload_particles(**P)
+        sim.ResetCurrent(); o.reset_current()
+        sim.MoveDepositPrtl(); sim.ExchangePrtl(); o.move_deposit()
+        A, B = sim.download_particles(), o.particles()
+        for k in ("x", "y", "z", "u", "v", "w"):
+            d = np.abs(A[k] - B[k]) / np.maximum(np.abs(B[k]), 1.0)
+            assert d.max() <= 1e-13, (k, d.max())
+        G = sim.download_fields()
+        for k in ("Jx", "Jy", "Jz"):
+            scale = np.abs(o.F[k]).max()
+            assert scale > 0 and np.abs(G[k] - o.F[k]).max() <= 1e-12 * scale, k
+        # 20 full steps from the same state
+        _upload_fields(o, sim, _smooth_fields(o, np.random.default_rng(6), amp=5e-3))
+        sim.upload_particles(**B)
+        sim.step(20)
+        o.run(20)
+        G = sim.download_fields()
+        for k in FIELDS:
+            scale = np.abs(o.F[k]).max()
+            d = np.abs(G[k] - o.F[k]).max() / scale
+            say(f"{k}: max diff / max {d:.2e}")
+            assert d <= 1e-9, k
+        A, B = sim.download_particles(), o.particles()
+        for k in ("x", "y", "z", "u", "v", "w"):
+            assert np.abs(A[k] - B[k]).max() <= 1e-9 * max(np.abs(B[k]).max(), 1.0), k
+
+
+def test_moving_window_matches_oracle():
+    o, sim = _pair(20, 24, dr=2.0, c=0.45, epc=4, cap=20000)
+    rng = np.random.default_rng(9)
+    o.load_plasma(10, 3, 24, 4, seed=13, cap=20000)
+    P = _thermal(o, rng, 0.05)
+    o.load_particles(P["q"], P["x"], P["y"], P["z"], P["u"], P["v"], P["w"], P["flv"], cap=20000)
+    with sim:
+        sim.upload_particles(**P)
+        sim.moving_window(True, nr_plasma=10, ppc=4, seed=13)
+        nsteps, shifts = 12, 0
+        sim.step(nsteps)
+        for t in range(1, nsteps + 1):
+            o.step()
+            while t * 0.45 >= shifts + 1:
+                shifts += 1
+                o.shift(10, 4, 13, shifts)
+        G = sim.download_fields()
+        A, B = sim.download_particles(), o.particles(live_only=False)
+    say(f"window shifts {shifts}")
+    assert shifts == 5
+    assert len(A["x"]) == len(B["x"])
+    assert np.array_equal(A["flv"], B["flv"])
+    live = B["flv"] != 0
+    for k in ("x", "y", "z", "u", "v", "w"):
+        assert np.abs(A[k][live] - B[k][live]).max() <= 1e-9 * max(np.abs(B[k][live]).max(), 1.0), k
+    for k in FIELDS:
+        scale = np.abs(o.F[k]).max()
+        assert scale > 0 and np.abs(G[k] - o.F[k]).max() <= 1e-9 * scale, k
+
+
+def test_gpu_run_keeps_modal_gauss_law():
+    o, sim = _pair(32, 40, dr=2.0, c=0.5, epc=4, cap=60000)
+    rng = np.random.default_rng(2)
+    o.load_plasma(20, 3, 40, 8, seed=2, cap=60000)
+    P = _thermal(o, rng, 0.1)
+    o.load_particles(P["q"], P["x"], P["y"], P["z"], P["u"], P["v"], P["w"], P["flv"], cap=60000)
+    G0 = o.gauss()
+    with sim:
+        sim.upload_particles(**P)
+        sim.step(40)
+        F = sim.download_fields()
+        A = sim.download_particles()
+    o.set_fields(**{k: F[k] for k in FIELDS})
+    o.load_particles(A["q"], A["x"], A["y"], A["z"], A["u"], A["v"], A["w"], A["flv"], cap=60000)
+    G1 = o.gauss()
+    for s in range(3):
+        d = np.abs(G1 - G0)[:, s, :].max() / np.abs(G0[:, s, :]).max()
+        say(f"slot {s}: Gauss change / max {d:.2e}")
+        assert d <= 1e-12, s
+
+
+def _laser_run(a0, with_electron, steps):
+    nr, nz, dr, c = 48, 384, 4.0, 0.8
+    sim = Simulation(nr, 2, nz, ndim=3, epc=1, c=c, c_ompe=20.0, form="fp64", geometry="cyl_modal", dr=dr, mi_me=1e30)
+    # omega0 = 10 omega_p: k0 = 0.5 rad per axial cell; c tau = 40 cells, w0 = 120 cells
+    zc = 110.0
+    sim.cyl_laser(a0, 10.0, 2.0, 6.0, zc)
+    if with_electron:
+        P = dict(q=np.array([-1e-9]), x=np.array([3.6]), y=np.array([0.0]), z=np.array([250.0]),
+                 u=np.zeros(1), v=np.zeros(1), w=np.zeros(1), flv=np.array([2], dtype=np.int32))
+        sim.upload_particles(**P)
+    return sim, zc
+
+
+def test_laser_in_vacuum_keeps_energy_and_moves_forward():
+    say("[case] laser in vacuum")
+    sim, zc = _laser_run(0.1, False, 0)
+    with sim:
+        e0 = sim.diagnostics()
+        F0 = sim.download_fields()
+        sim.step(150)
+        e1 = sim.diagnostics()
+        F1 = sim.download_fields()
+    w0 = e0["B_energy"] + e0["E_energy"]
+    w1 = e1["B_energy"] + e1["E_energy"]
+    say(f"field energy {w0:.6e} -> {w1:.6e}")
+    # a purely forward pulse from the first step (B set half a step behind E): no
+    # backward wave, E and B energies stay equal
+    assert w0 > 0 and abs(w1 - w0) <= 1e-3 * w0
+    assert abs(e1["B_energy"] - e1["E_energy"]) <= 2e-3 * w1
+    # envelope centre on the axis row of E_r (m = 1 cos), then the distance moved
+    def centre(F):
+        a = F["Ex"][3, 1, :] ** 2 + F["Bx"][3, 2, :] ** 2
+        k = np.arange(len(a)) + 1.0
+        return float((a * k).sum() / a.sum())
+    moved = centre(F1) - centre(F0)
+    # Yee group speed along z for k0 = 0.5 at c = 0.8: d omega/dk of sin(w/2) = c sin(k/2)
+    vg = 0.8 * math.cos(0.25) / math.sqrt(1.0 - (0.8 * math.sin(0.25)) ** 2)
+    say(f"moved {moved:.2f} cells, group speed x 150 = {150 * vg:.2f}")
+    assert abs(moved - 150 * vg) <= 0.03 * 150 * vg
+
+
+def test_electron_quivers_with_a0():
+    say("[case] electron in a weak pulse")
+    a0 = 0.02
+    sim, _ = _laser_run(a0, True, 0)
+    umax = 0.0
+    with sim:
+        for _ in range(60):
+            sim.step(5)
+            P = sim.download_particles()
+            live = P["flv"] != 0
+            ux = P["u"][live] * np.cos(P["y"][live]) - P["v"][live] * np.sin(P["y"][live])
+            umax = max(umax, float(np.abs(ux).max()))
+    say(f"max |gamma beta_x| {umax:.5f} for a0 {a0}")
+    assert abs(umax - a0) <= 0.05 * a0
+
+
+@pytest.mark.parametrize("dr,c", [(1.0, 0.41), (4.0, 0.88)])
+def test_courant_above_the_axis_limit_is_rejected(dr, c):
+    with pytest.raises(RuntimeError):
+        Simulation(16, 2, 16, ndim=3, epc=1, c=c, form="fp64", geometry="cyl_modal", dr=dr)
