@@ -419,6 +419,84 @@ def cyl_report(args):
     }
 
 
+LWFA_CONFIGS = {
+    # name: (nr, nz, dr in axial cells, c/omega_p in axial cells, radial plasma cells, particles per cell, c)
+    # dz = 0.05 c/omega_p, dr = 0.2 c/omega_p; c dt = 0.8 dz sits below the m = 1 axis limit
+    # 1/sqrt(1 + 5/dr^2) = 0.873
+    "lwfa_4096x512": (512, 4096, 4.0, 20.0, 480, 16, 0.8),
+    "lwfa_1024x128": (128, 1024, 4.0, 20.0, 112, 16, 0.8),
+}
+
+
+def lwfa_report(args):
+    """Laser wakefield in the azimuthal-mode (m = 0, 1) cylinder: cold electrons
+    at n/n_c = 0.005 (omega_0 = 14.14 omega_p) over immobile ions, 16 particles
+    per (r, z) cell, a linearly polarised Gaussian pulse a0 = 2, tau = 10/omega_p,
+    w0 = 15 c/omega_p, and the moving window (one axial cell per 1/c steps, new
+    plasma injected at the front).  value = particles * steps / wall time of the
+    timed steps (whole steps: fields, push + modal deposit, halos, window shifts
+    and injection).  roofline: k_cylm_push, 108 B per particle (the FP64 record
+    read and written) + 72 B per mode cell (E, B read, J written).
+    cpu_baseline: the modal oracle (one thread) on a 256-cell axial slab of the
+    same plasma."""
+    import numpy as np
+
+    from oracle.pyoracle import CylModalOracle
+    from pictor_amd import Simulation
+
+    nr, nz, dr, cps, nrp, ppc, c = LWFA_CONFIGS[args.lwfa]
+    a0, omega0, tau, w0 = 2.0, (1.0 / 0.005) ** 0.5, 10.0, 15.0
+    zc = nz - 3.0 * tau * cps            # the pulse centre three lengths behind the front
+    with Simulation(nr, 2, nz, ndim=3, epc=ppc, c=c, c_ompe=cps, mi_me=1e30, form="fp64",
+                    geometry="cyl_modal", dr=dr) as sim:
+        sim.cyl_plasma(nrp, 3, nz, ppc, seed=2024)
+        sim.cyl_laser(a0, omega0, tau, w0, zc)
+        sim.moving_window(True, nr_plasma=nrp, ppc=ppc, seed=2024)
+        npart = nrp * nz * ppc
+        print(f"# lwfa {args.lwfa}: {npart} particles", file=sys.stderr, flush=True)
+        sim.step(args.warmup)
+        sim.synchronize()
+        sim.timer_reset()
+        t0 = time.time()
+        sim.step(args.steps)
+        sim.synchronize()
+        wall = time.time() - t0
+        push_ms, npush = sim.timer("push")
+        d = sim.diagnostics()
+    push_ms /= max(npush, 1)
+    value = npart * args.steps / wall
+    ncell = (nr + 5) * 3 * (nz + 5)
+    algo = npart * BYTES_PER_PARTICLE["fp64"] + ncell * GRID_BYTES["fp64"]
+    achieved = algo / (push_ms * 1e-3)
+    cpu = None
+    if args.cpu_steps != 0:
+        nzs, steps = 256, 20
+        cap = nrp * nzs * ppc + 16
+        o = CylModalOracle(nr, nzs, dr=dr, c=c, c_ompe=cps, epc=ppc, cap=cap)
+        o.load_plasma(nrp, 3, nzs, ppc, seed=2024, cap=cap)
+        o.step()
+        t0 = time.time()
+        o.run(steps)
+        dt = time.time() - t0
+        m = nrp * nzs * ppc
+        cpu = {"value": m * steps / dt, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{steps} steps of the same plasma on a {nr} x {nzs} slab ({m} particles) "
+                         f"with the modal oracle, one thread, after 1 warm-up step, {dt:.1f}s"}
+    return {
+        "metric": "particle-updates/s/GPU, cylindrical laser wakefield (modes m = 0, 1) FP64 step",
+        "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.lwfa}: modal cylinder nz={nz} nr={nr}, dz=0.05, dr=0.2 c/omega_p, "
+                               f"c dt = {c} dz, n/n_c=0.005, immobile ions, {ppc} ppc in {nrp} radial cells, "
+                               f"a0={a0}, tau={tau}/omega_p, w0={w0} c/omega_p, moving window, Yee",
+                   "form": "fp64", "particles": npart, "energies": {k: d[k] for k in ("E_energy", "B_energy")}},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_cylm_push", "push_ms": push_ms},
+        "cpu_baseline": cpu,
+    }
+
+
 def atomics_report(iters=4096):
     """Atomic throughput behind the accumulator choices: the split push's current
     window (LDS FP64), the adjoint's field-adjoint window (LDS FP64, FP32 only for
@@ -465,6 +543,8 @@ def main():
     ap.add_argument("--adjoint-fd", type=int, default=1, help="finite-difference check of the drift gradient")
     ap.add_argument("--cyl", default=None, choices=sorted(CYL_CONFIGS),
                     help="instead of the Weibel step: time the cylindrical geometry on this grid")
+    ap.add_argument("--lwfa", default=None, choices=sorted(LWFA_CONFIGS),
+                    help="instead of the Weibel step: time the laser wakefield in the modal cylinder")
     ap.add_argument("--atomics", action="store_true",
                     help="instead of the Weibel step: LDS versus global atomic throughput (pictor_probe_atomics)")
     args = ap.parse_args()
@@ -482,6 +562,9 @@ def main():
         return
     if args.atomics:
         print(json.dumps(atomics_report()), flush=True)
+        return
+    if args.lwfa:
+        print(json.dumps(lwfa_report(args)), flush=True)
         return
 
     import numpy as np  # noqa: F401

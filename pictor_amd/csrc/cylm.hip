@@ -17,6 +17,7 @@
 // axial cells (dz = 1); the radial cell is dr.  Particles: x = radial grid
 // coordinate (r = (x - 3.5) dr), y = theta in radians, z = axial coordinate,
 // momenta in the local (r, theta, z) frame; flavour 2 = electrons.
+#include <climits>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -226,8 +227,63 @@ __device__ inline void gather(const GridP &g, const double *const *E, const doub
     }
 }
 
+// the modal deposit of one particle (cyl_movdep.F90's zigzag split, per mode): the r-z
+// zigzag at the old angle, each slot weighted by its phase, then the angular move at
+// the new (r, z); add(component, slot, i, k, value) receives every contribution
+struct DepMove {
+    int i1, i2, k1, k2;
+    double Fx1, Fz1, Fx2, Fz2, Wx1, Wz1, Wx2, Wz2, wx, wz;
+    double pa[3], pt[3];
+};
+
+template <class Add>
+__device__ inline void modal_deposit(const DepMove &d, Add add)
+{
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        add(0, s, d.i1, d.k1, d.pa[s] * d.Fx1 * (1.0 - d.Wz1));
+        add(0, s, d.i1, d.k1 + 1, d.pa[s] * d.Fx1 * d.Wz1);
+        add(2, s, d.i1, d.k1, d.pa[s] * d.Fz1 * (1.0 - d.Wx1));
+        add(2, s, d.i1 + 1, d.k1, d.pa[s] * d.Fz1 * d.Wx1);
+        add(0, s, d.i2, d.k2, d.pa[s] * d.Fx2 * (1.0 - d.Wz2));
+        add(0, s, d.i2, d.k2 + 1, d.pa[s] * d.Fx2 * d.Wz2);
+        add(2, s, d.i2, d.k2, d.pa[s] * d.Fz2 * (1.0 - d.Wx2));
+        add(2, s, d.i2 + 1, d.k2, d.pa[s] * d.Fz2 * d.Wx2);
+    }
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        add(1, s, d.i2, d.k2, d.pt[s] * (1.0 - d.wx) * (1.0 - d.wz));
+        add(1, s, d.i2 + 1, d.k2, d.pt[s] * d.wx * (1.0 - d.wz));
+        add(1, s, d.i2, d.k2 + 1, d.pt[s] * (1.0 - d.wx) * d.wz);
+        add(1, s, d.i2 + 1, d.k2 + 1, d.pt[s] * d.wx * d.wz);
+    }
+}
+
+__device__ inline int wave_min(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// mode cells of the block's deposit window held in LDS: 9 accumulators (3 components x
+// 3 slots) per (i, k) cell, FP64.  A window of at most kDepCells / 4 cells keeps 4
+// interleaved copies of each accumulator (lane l adds into copy l % 4): the particles of
+// one cell sit in adjacent lanes and would otherwise serialise on the same LDS words
+constexpr int kDepCells = 448;
+constexpr int kDepCopies = 4;
+
 // MoveDepositPrtl: gather, Boris, the local-frame move and rotation (cyl_movdep.F90:14-289),
-// then the modal deposit (r-z zigzag at the old angle, angular move at the new r, z)
+// then the modal deposit.  The particles keep the loader's cell order, so the 256 of a
+// block deposit into a small (i, k) window: it is accumulated in LDS and added to J once
+// per cell and accumulator; a block whose window exceeds kDepCells cells adds every
+// contribution to J directly.
 __global__ __launch_bounds__(256) void k_cylm_push(GridP g, CylmP q, const double *__restrict__ Ex,
                                                    const double *__restrict__ Ey, const double *__restrict__ Ez,
                                                    const double *__restrict__ Bx, const double *__restrict__ By,
@@ -235,83 +291,110 @@ __global__ __launch_bounds__(256) void k_cylm_push(GridP g, CylmP q, const doubl
                                                    double *__restrict__ Jy, double *__restrict__ Jz, PartF64 P,
                                                    long long n, int *__restrict__ err)
 {
+    __shared__ double sJ[9 * kDepCells];
+    __shared__ int sred[4][4];
     const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n) return;
-    const int flv = P.flv[m];
-    if (flv == 0) return;
-    const double *E[3] = {Ex, Ey, Ez}, *B[3] = {Bx, By, Bz};
-    const double x0 = P.x[m], th0 = P.y[m], z0 = P.z[m];
-    const double qm = q.flvrqm[flv - 1] * 0.5;
-    const double c = q.c, sqc = q.sqc, cinv = q.cinv;
-    double pe[3], pb[3];
-    gather(g, E, B, x0, th0, z0, pe, pb);
-    const double pEx = pe[0] * qm, pEy = pe[1] * qm, pEz = pe[2] * qm;
-    double pBx = pb[0] * qm, pBy = pb[1] * qm, pBz = pb[2] * qm;
-    double u0 = c * P.u[m] + pEx;
-    double v0 = c * P.v[m] + pEy;
-    double w0 = c * P.w[m] + pEz;
-    double gg = 1.0 / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
-    pBx = gg * pBx; pBy = gg * pBy; pBz = gg * pBz;
-    const double ff = 2.0 / (1.0 + pBx * pBx + pBy * pBy + pBz * pBz);
-    double u1 = (u0 + v0 * pBz - w0 * pBy) * ff;
-    double v1 = (v0 + w0 * pBx - u0 * pBz) * ff;
-    const double w1 = (w0 + u0 * pBy - v0 * pBx) * ff;
-    u0 = u0 + v1 * pBz - w1 * pBy + pEx;
-    v0 = v0 + w1 * pBx - u1 * pBz + pEy;
-    w0 = w0 + u1 * pBy - v1 * pBx + pEz;
-    u1 = u0 * cinv;
-    v1 = v0 * cinv;
-    const double wn = w0 * cinv;
-    gg = sqc / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
-    const double X = rad(q, x0) + u1 * gg, Y = v1 * gg;
-    const double r1 = sqrt(X * X + Y * Y);
-    const double x1 = r1 * q.inv_dr + 3.5;
-    const double th1 = th0 + atan2(Y, X);
-    const double z1 = z0 + wn * gg * q.inv_dz;
-    const double ir = 1.0 / r1;
-    const double un = (X * u1 + Y * v1) * ir;
-    const double vn = (-Y * u1 + X * v1) * ir;
-    if (x1 >= g.mx - 3) {
-        atomicOr(err, 1);
-    } else {
-        const double qthis = P.q[m] * q.qi;
-        const int i1 = (int)floor(x0), i2 = (int)floor(x1), k1 = (int)floor(z0), k2 = (int)floor(z1);
-        const double xr = fmin((double)(min(i1, i2) + 1), fmax((double)max(i1, i2), 0.5 * (x0 + x1)));
-        const double zr = fmin((double)(min(k1, k2) + 1), fmax((double)max(k1, k2), 0.5 * (z0 + z1)));
-        const double Fx1 = qthis * (xr - x0), Fz1 = qthis * (zr - z0);
-        const double Fx2 = qthis * (x1 - xr), Fz2 = qthis * (z1 - zr);
-        const double Wx1 = 0.5 * (x0 + xr) - i1, Wz1 = 0.5 * (z0 + zr) - k1;
-        const double Wx2 = 0.5 * (x1 + xr) - i2, Wz2 = 0.5 * (z1 + zr) - k2;
-        const double pa[3] = {1.0, cos(th0), sin(th0)};
-        const long long sk = 3LL * g.mx;
-        for (int s = 0; s < 3; s++) {
-            const long long a = mi(g, i1, s, k1), b = mi(g, i2, s, k2);
-            unsafeAtomicAdd(&Jx[a], pa[s] * Fx1 * (1.0 - Wz1));
-            unsafeAtomicAdd(&Jx[a + sk], pa[s] * Fx1 * Wz1);
-            unsafeAtomicAdd(&Jz[a], pa[s] * Fz1 * (1.0 - Wx1));
-            unsafeAtomicAdd(&Jz[a + 1], pa[s] * Fz1 * Wx1);
-            unsafeAtomicAdd(&Jx[b], pa[s] * Fx2 * (1.0 - Wz2));
-            unsafeAtomicAdd(&Jx[b + sk], pa[s] * Fx2 * Wz2);
-            unsafeAtomicAdd(&Jz[b], pa[s] * Fz2 * (1.0 - Wx2));
-            unsafeAtomicAdd(&Jz[b + 1], pa[s] * Fz2 * Wx2);
+    const int flv = m < n ? P.flv[m] : 0;
+    bool dep = false;
+    DepMove d;
+    if (flv != 0) {
+        const double *E[3] = {Ex, Ey, Ez}, *B[3] = {Bx, By, Bz};
+        const double x0 = P.x[m], th0 = P.y[m], z0 = P.z[m];
+        const double qm = q.flvrqm[flv - 1] * 0.5;
+        const double c = q.c, sqc = q.sqc, cinv = q.cinv;
+        double pe[3], pb[3];
+        gather(g, E, B, x0, th0, z0, pe, pb);
+        const double pEx = pe[0] * qm, pEy = pe[1] * qm, pEz = pe[2] * qm;
+        double pBx = pb[0] * qm, pBy = pb[1] * qm, pBz = pb[2] * qm;
+        double u0 = c * P.u[m] + pEx;
+        double v0 = c * P.v[m] + pEy;
+        double w0 = c * P.w[m] + pEz;
+        double gg = 1.0 / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+        pBx = gg * pBx; pBy = gg * pBy; pBz = gg * pBz;
+        const double ff = 2.0 / (1.0 + pBx * pBx + pBy * pBy + pBz * pBz);
+        double u1 = (u0 + v0 * pBz - w0 * pBy) * ff;
+        double v1 = (v0 + w0 * pBx - u0 * pBz) * ff;
+        const double w1 = (w0 + u0 * pBy - v0 * pBx) * ff;
+        u0 = u0 + v1 * pBz - w1 * pBy + pEx;
+        v0 = v0 + w1 * pBx - u1 * pBz + pEy;
+        w0 = w0 + u1 * pBy - v1 * pBx + pEz;
+        u1 = u0 * cinv;
+        v1 = v0 * cinv;
+        const double wn = w0 * cinv;
+        gg = sqc / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+        const double X = rad(q, x0) + u1 * gg, Y = v1 * gg;
+        const double r1 = sqrt(X * X + Y * Y);
+        const double x1 = r1 * q.inv_dr + 3.5;
+        const double th1 = th0 + atan2(Y, X);
+        const double z1 = z0 + wn * gg * q.inv_dz;
+        const double ir = 1.0 / r1;
+        const double un = (X * u1 + Y * v1) * ir;
+        const double vn = (-Y * u1 + X * v1) * ir;
+        if (x1 >= g.mx - 3) {
+            atomicOr(err, 1);
+        } else {
+            dep = true;
+            const double qthis = P.q[m] * q.qi;
+            d.i1 = (int)floor(x0); d.i2 = (int)floor(x1); d.k1 = (int)floor(z0); d.k2 = (int)floor(z1);
+            const double xr = fmin((double)(min(d.i1, d.i2) + 1), fmax((double)max(d.i1, d.i2), 0.5 * (x0 + x1)));
+            const double zr = fmin((double)(min(d.k1, d.k2) + 1), fmax((double)max(d.k1, d.k2), 0.5 * (z0 + z1)));
+            d.Fx1 = qthis * (xr - x0); d.Fz1 = qthis * (zr - z0);
+            d.Fx2 = qthis * (x1 - xr); d.Fz2 = qthis * (z1 - zr);
+            d.Wx1 = 0.5 * (x0 + xr) - d.i1; d.Wz1 = 0.5 * (z0 + zr) - d.k1;
+            d.Wx2 = 0.5 * (x1 + xr) - d.i2; d.Wz2 = 0.5 * (z1 + zr) - d.k2;
+            d.wx = x1 - d.i2; d.wz = z1 - d.k2;
+            d.pa[0] = 1.0; d.pa[1] = cos(th0); d.pa[2] = sin(th0);
+            d.pt[0] = qthis * (th1 - th0);
+            d.pt[1] = qthis * (sin(th1) - sin(th0));
+            d.pt[2] = -(qthis * (cos(th1) - cos(th0)));
         }
-        const double wx = x1 - i2, wz = z1 - k2;
-        const double pt[3] = {qthis * (th1 - th0), qthis * (sin(th1) - sin(th0)), -(qthis * (cos(th1) - cos(th0)))};
-        for (int s = 0; s < 3; s++) {
-            const long long b = mi(g, i2, s, k2);
-            unsafeAtomicAdd(&Jy[b], pt[s] * (1.0 - wx) * (1.0 - wz));
-            unsafeAtomicAdd(&Jy[b + 1], pt[s] * wx * (1.0 - wz));
-            unsafeAtomicAdd(&Jy[b + sk], pt[s] * (1.0 - wx) * wz);
-            unsafeAtomicAdd(&Jy[b + 1 + sk], pt[s] * wx * wz);
-        }
+        double zn = z1, tn = th1;
+        if (zn > q.zmax) zn = zn - q.zlen;
+        else if (zn < q.zmin) zn = q.zlen + zn;
+        if (tn >= 2.0 * kPi) tn = tn - 2.0 * kPi;
+        else if (tn < 0.0) tn = tn + 2.0 * kPi;
+        P.x[m] = x1; P.y[m] = tn; P.z[m] = zn;
+        P.u[m] = un; P.v[m] = vn; P.w[m] = wn;
     }
-    double zn = z1, tn = th1;
-    if (zn > q.zmax) zn = zn - q.zlen;
-    else if (zn < q.zmin) zn = q.zlen + zn;
-    if (tn >= 2.0 * kPi) tn = tn - 2.0 * kPi;
-    else if (tn < 0.0) tn = tn + 2.0 * kPi;
-    P.x[m] = x1; P.y[m] = tn; P.z[m] = zn;
-    P.u[m] = un; P.v[m] = vn; P.w[m] = wn;
+    // the block's window: i in [ilo, ihi + 1], k in [klo, khi + 1]
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ilo = wave_min(dep ? min(d.i1, d.i2) : INT_MAX), ihi = wave_max(dep ? max(d.i1, d.i2) : INT_MIN);
+    const int klo = wave_min(dep ? min(d.k1, d.k2) : INT_MAX), khi = wave_max(dep ? max(d.k1, d.k2) : INT_MIN);
+    if (lane == 0) { sred[wv][0] = ilo; sred[wv][1] = ihi; sred[wv][2] = klo; sred[wv][3] = khi; }
+    __syncthreads();
+    int bil = INT_MAX, bih = INT_MIN, bkl = INT_MAX, bkh = INT_MIN;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        bil = min(bil, sred[w][0]); bih = max(bih, sred[w][1]);
+        bkl = min(bkl, sred[w][2]); bkh = max(bkh, sred[w][3]);
+    }
+    if (bil > bih) return;                      // nothing to deposit in this block (uniform)
+    const int wi = bih + 2 - bil, wk = bkh + 2 - bkl;
+    double *J[3] = {Jx, Jy, Jz};
+    if ((long long)wi * wk > kDepCells) {       // scattered block: straight to J
+        if (dep)
+            modal_deposit(d, [&](int cc, int s, int i, int k, double v) { unsafeAtomicAdd(&J[cc][mi(g, i, s, k)], v); });
+        return;
+    }
+    const int cells = wi * wk;
+    int R = kDepCopies;                         // copies per accumulator: cells * R <= kDepCells
+    while (R > 1 && cells * R > kDepCells) R >>= 1;
+    const int rl = lane & (R - 1);
+    for (int e = threadIdx.x; e < 9 * cells * R; e += 256) sJ[e] = 0.0;
+    __syncthreads();
+    if (dep)
+        modal_deposit(d, [&](int cc, int s, int i, int k, double v) {
+            atomicAdd(&sJ[((cc * 3 + s) * cells + (k - bkl) * wi + (i - bil)) * R + rl], v);
+        });
+    __syncthreads();
+    for (int e = threadIdx.x; e < 9 * cells; e += 256) {
+        double v = 0.0;
+        for (int r = 0; r < R; r++) v += sJ[e * R + r];
+        if (v == 0.0) continue;
+        const int cs = e / cells, cell = e - cs * cells;
+        const int i = bil + cell % wi, k = bkl + cell / wi;
+        unsafeAtomicAdd(&J[cs / 3][mi(g, i, cs % 3, k)], v);
+    }
 }
 
 __device__ inline uint64_t mix64d(uint64_t z)
