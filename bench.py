@@ -429,7 +429,8 @@ LWFA_CONFIGS = {
 
 
 def lwfa_report(args):
-    """Laser wakefield in the azimuthal-mode (m = 0, 1) cylinder: cold electrons
+    """Laser wakefield in the azimuthal-mode (m = 0, 1) cylinder with the Lehe
+    B update (dispersion along z corrected): cold electrons
     at n/n_c = 0.005 (omega_0 = 14.14 omega_p) over immobile ions, 16 particles
     per (r, z) cell, a linearly polarised Gaussian pulse a0 = 2, tau = 10/omega_p,
     w0 = 15 c/omega_p, and the moving window (one axial cell per 1/c steps, new
@@ -438,9 +439,7 @@ def lwfa_report(args):
     and injection).  roofline: k_cylm_push, 108 B per particle (the FP64 record
     read and written) + 72 B per mode cell (E, B read, J written).
     cpu_baseline: the modal oracle (one thread) on a 256-cell axial slab of the
-    same plasma."""
-    import numpy as np
-
+    same plasma.  FDTD: Lehe (the default) or Yee."""
     from oracle.pyoracle import CylModalOracle
     from pictor_amd import Simulation
 
@@ -448,7 +447,7 @@ def lwfa_report(args):
     a0, omega0, tau, w0 = 2.0, (1.0 / 0.005) ** 0.5, 10.0, 15.0
     zc = nz - 3.0 * tau * cps            # the pulse centre three lengths behind the front
     with Simulation(nr, 2, nz, ndim=3, epc=ppc, c=c, c_ompe=cps, mi_me=1e30, form="fp64",
-                    geometry="cyl_modal", dr=dr) as sim:
+                    geometry="cyl_modal", dr=dr, fdtd=args.lwfa_fdtd) as sim:
         sim.cyl_plasma(nrp, 3, nz, ppc, seed=2024)
         sim.cyl_laser(a0, omega0, tau, w0, zc)
         sim.moving_window(True, nr_plasma=nrp, ppc=ppc, seed=2024)
@@ -472,7 +471,7 @@ def lwfa_report(args):
     if args.cpu_steps != 0:
         nzs, steps = 256, 20
         cap = nrp * nzs * ppc + 16
-        o = CylModalOracle(nr, nzs, dr=dr, c=c, c_ompe=cps, epc=ppc, cap=cap)
+        o = CylModalOracle(nr, nzs, dr=dr, c=c, c_ompe=cps, epc=ppc, cap=cap, fdtd=1 if args.lwfa_fdtd == "lehe" else 0)
         o.load_plasma(nrp, 3, nzs, ppc, seed=2024, cap=cap)
         o.step()
         t0 = time.time()
@@ -489,7 +488,8 @@ def lwfa_report(args):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.lwfa}: modal cylinder nz={nz} nr={nr}, dz=0.05, dr=0.2 c/omega_p, "
                                f"c dt = {c} dz, n/n_c=0.005, immobile ions, {ppc} ppc in {nrp} radial cells, "
-                               f"a0={a0}, tau={tau}/omega_p, w0={w0} c/omega_p, moving window, Yee",
+                               f"a0={a0}, tau={tau}/omega_p, w0={w0} c/omega_p, moving window, "
+                               f"{'Lehe' if args.lwfa_fdtd == 'lehe' else 'Yee'} FDTD",
                    "form": "fp64", "particles": npart, "energies": {k: d[k] for k in ("E_energy", "B_energy")}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_cylm_push", "push_ms": push_ms},
@@ -545,6 +545,7 @@ def main():
                     help="instead of the Weibel step: time the cylindrical geometry on this grid")
     ap.add_argument("--lwfa", default=None, choices=sorted(LWFA_CONFIGS),
                     help="instead of the Weibel step: time the laser wakefield in the modal cylinder")
+    ap.add_argument("--lwfa-fdtd", default="lehe", choices=("lehe", "yee"))
     ap.add_argument("--atomics", action="store_true",
                     help="instead of the Weibel step: LDS versus global atomic throughput (pictor_probe_atomics)")
     args = ap.parse_args()

@@ -53,6 +53,36 @@ int orc_cylm_init(orc_cylm *m, int nr, int nz, double dr, double c, double c_omp
     return ORC_OK;
 }
 
+/* The Lehe B update per mode (cyl_em_update_lehe.F90:6-17, 62-118, 255-289).  The
+ * reference averages the r and theta differences along z with beta_xz = beta_yz = 1/8,
+ * and its z difference is
+ *   (1 - 2 b_zx - 2 b_zy - 3 d_z) dF + b_zx (dF(i+1) + dF(i-1)) + b_zy (dF(j+1) + dF(j-1))
+ *   + d_z (F(k+2) - F(k-1)),  b_zx = (dz/dr)^2 / 8, b_zy = 1 / (8 dtheta^2),
+ *   d_z = (1 - sin^2(pi c / 2) / c^2) / 4.
+ * The modal form keeps beta_rz = beta_thetaz = 1/8 and d_z and sets b_zx = 0 and the
+ * theta pair to 0 (l_zr = l_zm = 0):
+ *   * the r pair F(i+1) - 2 F(i) + F(i-1) reads the axis fills at i = 4 and is not
+ *     symmetric in the r-weighted energy norm: with it the fields next to the axis
+ *     grow without bound at the z Nyquist wavenumber, on the reference's full grid
+ *     as on the modes (b_zx = 1/128 at dr = 4 dz);
+ *   * b_zy (2 cos(m dtheta) - 2) tends to -m^2 / 8 at every radius (the Cartesian
+ *     coefficient applied to the theta index), which would slow an m = 1 pulse.
+ * l_zr and l_zm stay as fields: setting them restores the reference's terms. */
+int orc_cylm_set_fdtd(orc_cylm *m, int fdtd)
+{
+    if (fdtd != 0 && fdtd != 1) return ORC_ERR_ARG;
+    m->fdtd = fdtd;
+    const double pi = 3.14159265358979323846264;
+    const double brz = 0.125, btz = 0.125;
+    const double sn = sin(0.5 * pi * m->c);
+    const double dz = 0.25 * (1.0 - (m->cinv * m->cinv) * (sn * sn));
+    m->l_rr = 1.0 - 2.0 * brz; m->l_rz = brz;
+    m->l_tt = 1.0 - 2.0 * btz; m->l_tz = btz;
+    m->l_zz = 1.0 - 3.0 * dz; m->l_zr = 0.0; m->l_zd = dz;
+    m->l_zm = 0.0;
+    return ORC_OK;
+}
+
 /* radius of grid index i: r(i) = (i - 3.5) dr (cyl_common.F90:23 with rmin = -dr/2) */
 static inline double rad(const orc_cylm *m, double i) { return (i - 3.5) * m->dr; }
 
@@ -146,6 +176,53 @@ static void cylm_update_b_half(const orc_cylm *m, orc_fields *f)
             }
 }
 
+/* the Lehe differences at o = (i, s, k): gradX, gradX1 with the z pair, the theta
+ * derivative (exact per mode) with the z pair, gradZ with the r pair, the mode's
+ * theta term and the wide z pair (term order of cyl_em_update_lehe.F90:255-289) */
+static inline double dth(const orc_cylm *m, const double *F, long long o, int s)
+{
+    return s == 0 ? 0.0 : (s == 1 ? F[o + m->mx] : -F[o - m->mx]);
+}
+static double lehe_gr(const orc_cylm *m, const double *F, long long o)
+{
+    const long long sk = 3LL * m->mx;
+    return m->l_rr * (F[o + 1] - F[o]) + m->l_rz * (F[o + 1 + sk] - F[o + sk]) + m->l_rz * (F[o + 1 - sk] - F[o - sk]);
+}
+static double lehe_gr1(const orc_cylm *m, const double *F, long long o, double rp, double r)
+{
+    const long long sk = 3LL * m->mx;
+    return m->l_rr * (rp * F[o + 1] - r * F[o]) + m->l_rz * (rp * F[o + 1 + sk] - r * F[o + sk])
+           + m->l_rz * (rp * F[o + 1 - sk] - r * F[o - sk]);
+}
+static double lehe_gt(const orc_cylm *m, const double *F, long long o, int s)
+{
+    const long long sk = 3LL * m->mx;
+    return m->l_tt * dth(m, F, o, s) + m->l_tz * dth(m, F, o + sk, s) + m->l_tz * dth(m, F, o - sk, s);
+}
+static double lehe_gz(const orc_cylm *m, const double *F, long long o, int s)
+{
+    const long long sk = 3LL * m->mx;
+    const double lzz = s == 0 ? m->l_zz : m->l_zz - m->l_zm;
+    return lzz * (F[o + sk] - F[o]) + m->l_zr * (F[o + 1 + sk] - F[o + 1]) + m->l_zr * (F[o - 1 + sk] - F[o - 1])
+           + m->l_zd * (F[o + 2 * sk] - F[o - sk]);
+}
+
+/* UpdateBfieldHalf of cyl_em_update_lehe.F90:62-118 per mode, i = 4..mx-1,
+ * k = 2..mz-2 (gradZ reads k - 1 and k + 2; the z halo rewrites k = 1, mz - 1) */
+static void cylm_update_b_half_lehe(const orc_cylm *m, orc_fields *f)
+{
+    const double h = m->fld_halfc, idr = m->inv_dr, idz = m->inv_dz;
+    for (int k = 2; k <= m->mz - 2; k++)
+        for (int s = 0; s < 3; s++)
+            for (int i = 4; i <= m->mx - 1; i++) {
+                const long long o = MI(m, i, s, k);
+                const double r = rad(m, i), rph = r + 0.5 * m->dr, rp = r + m->dr;
+                f->Bx[o] = f->Bx[o] - h * lehe_gt(m, f->Ez, o, s) / r + h * idz * lehe_gz(m, f->Ey, o, s);
+                f->By[o] = f->By[o] - h * idz * lehe_gz(m, f->Ex, o, s) + h * idr * lehe_gr(m, f->Ez, o);
+                f->Bz[o] = f->Bz[o] - h * (idr * lehe_gr1(m, f->Ey, o, rp, r) - lehe_gt(m, f->Ex, o, s)) / rph;
+            }
+}
+
 /* B axis: the Bz integral over the disk r < dr/2 (cyl_em_update.F90:231-250: the
  * circulation of E_theta at r = dr/2 over the disk area, 4 / dr for m = 0; the
  * m = 1 circulation vanishes), then the fills of :182-229 */
@@ -167,7 +244,10 @@ static void cylm_b_axis(const orc_cylm *m, orc_fields *f)
 
 void orc_cylm_b_half(const orc_cylm *m, orc_fields *f)
 {
-    cylm_update_b_half(m, f);
+    if (m->fdtd == 1)
+        cylm_update_b_half_lehe(m, f);
+    else
+        cylm_update_b_half(m, f);
     cylm_b_axis(m, f);
     cylm_zhalo(m, f->Bx); cylm_zhalo(m, f->By); cylm_zhalo(m, f->Bz);
     cylm_outer_zero(m, f->Bx); cylm_outer_zero(m, f->By); cylm_outer_zero(m, f->Bz);

@@ -320,7 +320,9 @@ class OrcCylm(ctypes.Structure):
                 + [(n, ctypes.c_double) for n in ("dr", "inv_dr", "inv_dz", "c", "fldc", "fld_halfc", "cinv", "sqc",
                                                   "ompe", "qi")]
                 + [("flvrqm", ctypes.c_double * 2)]
-                + [(n, ctypes.c_double) for n in ("zmin", "zmax", "zlen")])
+                + [(n, ctypes.c_double) for n in ("zmin", "zmax", "zlen")]
+                + [("fdtd", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                + [(n, ctypes.c_double) for n in ("l_rr", "l_rz", "l_tt", "l_tz", "l_zz", "l_zr", "l_zd", "l_zm")])
 
 
 class CylModalOracle:
@@ -332,7 +334,7 @@ class CylModalOracle:
 
     FIELD_NAMES = Oracle.FIELD_NAMES
 
-    def __init__(self, nr=16, nz=16, dr=1.0, c=0.45, c_ompe=10.0, g0=1.0, mi_me=1e30, epc=16, cap=1024):
+    def __init__(self, nr=16, nz=16, dr=1.0, c=0.45, c_ompe=10.0, g0=1.0, mi_me=1e30, epc=16, cap=1024, fdtd=0):
         L = lib()
         P = ctypes.POINTER(OrcCylm)
         F = ctypes.POINTER(OrcFields)
@@ -343,6 +345,7 @@ class CylModalOracle:
             for n in ("orc_cylm_b_half", "orc_cylm_current_halo", "orc_cylm_reset_current"):
                 getattr(L, n).argtypes = [P, F]
             L.orc_cylm_e_phase.argtypes = [P, F, ctypes.c_int, ctypes.c_int]
+            L.orc_cylm_set_fdtd.argtypes = [P, ctypes.c_int]
             L.orc_cylm_move_deposit.argtypes = [P, F, S]
             L.orc_cylm_step.argtypes = [P, F, S]
             L.orc_cylm_energy.argtypes = [P, F, c_dp]
@@ -354,7 +357,8 @@ class CylModalOracle:
                                          ctypes.c_int64]
             L._cylm_declared = True
         self.m = OrcCylm()
-        if L.orc_cylm_init(ctypes.byref(self.m), nr, nz, dr, c, c_ompe, g0, mi_me, epc):
+        if L.orc_cylm_init(ctypes.byref(self.m), nr, nz, dr, c, c_ompe, g0, mi_me, epc) or \
+                L.orc_cylm_set_fdtd(ctypes.byref(self.m), fdtd):
             raise ValueError("invalid modal configuration")
         self.epc = epc
         self.shape = (self.m.mx, 3, self.m.mz)

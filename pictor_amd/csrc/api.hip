@@ -434,14 +434,19 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     const bool modal = cfg->geometry == PICTOR_GEOM_CYL_MODAL;
     if (modal) {
         if (cfg->ndim != 3 || cfg->ny != 2) return fail(PICTOR_ERR_ARG, "the modal cylinder needs ndim 3 and ny = 2 (modes m = 0, 1)");
-        if (cfg->form != PICTOR_FORM_FP64 || cfg->fdtd != PICTOR_FDTD_YEE || cfg->curr_filter)
-            return fail(PICTOR_ERR_ARG, "the modal cylinder runs the FP64 form with the Yee solver and no current filter");
+        if (cfg->form != PICTOR_FORM_FP64 || cfg->curr_filter)
+            return fail(PICTOR_ERR_ARG, "the modal cylinder runs the FP64 form with no current filter");
         if (cfg->dr < 0.0) return fail(PICTOR_ERR_ARG, "dr must be >= 0");
         // the m = 1 terms near the axis act like a theta cell of dr / 2 at r = dr / 2
         const double dr = cfg->dr > 0.0 ? cfg->dr : 1.0;
         const double lim = 1.0 / std::sqrt(1.0 + 1.0 / (dr * dr) + 4.0 / (dr * dr));
         if (cfg->c >= lim)
             return fail(PICTOR_ERR_ARG, "modal cylinder: c must be below 1/sqrt(1 + 5/dr^2) = " + std::to_string(lim));
+        // the Lehe update raises the z Nyquist difference by (1 - 4 d_z) = sin^2(pi c/2) / c^2;
+        // with the axis terms it stays bounded for sin^2(pi c/2) + 1.25 c^2/dr^2 < 1
+        const double sn = std::sin(0.5 * 3.14159265358979323846 * cfg->c);
+        if (cfg->fdtd == PICTOR_FDTD_LEHE && sn * sn + 1.25 * cfg->c * cfg->c / (dr * dr) >= 1.0)
+            return fail(PICTOR_ERR_ARG, "modal cylinder with Lehe: c must satisfy sin^2(pi c/2) + 1.25 c^2/dr^2 < 1");
     }
     const bool cyl = cfg->geometry == PICTOR_GEOM_CYL || modal;
     if (cfg->curr_filter < 0 || cfg->curr_filter > 64) return fail(PICTOR_ERR_ARG, "curr_filter must be 0..64");

@@ -14,6 +14,10 @@ struct CylmP {
     double qi;
     double flvrqm[2];
     double zmin, zmax, zlen;
+    // the Lehe B update (fdtd = PICTOR_FDTD_LEHE): z averaging of the r and theta
+    // differences (l_rr, l_rz; l_tt, l_tz) and the z difference with the wide pair (l_zz, l_zd)
+    int lehe;
+    double l_rr, l_rz, l_tt, l_tz, l_zz, l_zd;
 };
 
 // particle storage and moving-window state of the modal path

@@ -115,6 +115,50 @@ __global__ void k_cylm_update_b(GridP g, CylmP q, double *__restrict__ Bx, doubl
     Bz[o] = Bz[o] - h * (idr * (rp * Ey[o + 1] - r * Ey[o]) - dEr) / rph;
 }
 
+// the Lehe differences at o = (i, s, k) (cyl_em_update_lehe.F90:255-289 per mode, term
+// order kept): gradX, gradX1 and the exact theta derivative, each with its z pair;
+// gradZ with the wide z pair.  The reference's r pair and theta pair of gradZ are left
+// out: the r pair reads the axis fills and makes the fields next to the axis grow at
+// the z Nyquist wavenumber, and the theta pair (the Cartesian coefficient on the theta
+// index) would lower every m = 1 z difference by 1/8 at any radius.
+__device__ __forceinline__ double dth(const GridP &g, const double *F, long long o, int s)
+{
+    return s == 0 ? 0.0 : (s == 1 ? F[o + g.mx] : -F[o - g.mx]);
+}
+
+__global__ void k_cylm_update_b_lehe(GridP g, CylmP q, double *__restrict__ Bx, double *__restrict__ By,
+                                     double *__restrict__ Bz, const double *__restrict__ Ex,
+                                     const double *__restrict__ Ey, const double *__restrict__ Ez)
+{
+    // i = 4..mx-1, k = 2..mz-2: gradZ reads k - 1 and k + 2; the z halo rewrites k = 1, mz - 1
+    const int ni = g.mx - 4, nk = g.mz - 3;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)ni * 3 * nk) return;
+    const int i = 4 + (int)(t % ni);
+    t /= ni;
+    const int s = (int)(t % 3);
+    const int k = 2 + (int)(t / 3);
+    const long long o = mi(g, i, s, k), sk = 3LL * g.mx;
+    const double h = q.fld_halfc, idr = q.inv_dr, idz = q.inv_dz;
+    const double r = rad(q, i), rph = r + 0.5 * q.dr, rp = r + q.dr;
+    auto gr = [&](const double *F) {
+        return q.l_rr * (F[o + 1] - F[o]) + q.l_rz * (F[o + 1 + sk] - F[o + sk]) + q.l_rz * (F[o + 1 - sk] - F[o - sk]);
+    };
+    auto gr1 = [&](const double *F) {
+        return q.l_rr * (rp * F[o + 1] - r * F[o]) + q.l_rz * (rp * F[o + 1 + sk] - r * F[o + sk])
+               + q.l_rz * (rp * F[o + 1 - sk] - r * F[o - sk]);
+    };
+    auto gt = [&](const double *F) {
+        return q.l_tt * dth(g, F, o, s) + q.l_tz * dth(g, F, o + sk, s) + q.l_tz * dth(g, F, o - sk, s);
+    };
+    auto gz = [&](const double *F) {
+        return q.l_zz * (F[o + sk] - F[o]) + q.l_zd * (F[o + 2 * sk] - F[o - sk]);
+    };
+    Bx[o] = Bx[o] - h * gt(Ez) / r + h * idz * gz(Ey);
+    By[o] = By[o] - h * idz * gz(Ex) + h * idr * gr(Ez);
+    Bz[o] = Bz[o] - h * (idr * gr1(Ey) - gt(Ex)) / rph;
+}
+
 // Bz on the axis (cyl_em_update.F90:231-250, the m = 0 circulation of E_theta at
 // r = dr/2 over the disk, 4/dr), then the fills of :182-229 by parity
 __global__ void k_cylm_b_axis(GridP g, CylmP q, double *__restrict__ Bx, double *__restrict__ By,
@@ -628,6 +672,15 @@ void cylm_params(const pictor_config &cfg, const PhysP &ph, CylmP &q)
     q.flvrqm[0] = ph.flvrqm[0];
     q.flvrqm[1] = ph.flvrqm[1];
     q.zmin = 3.0; q.zmax = 3.0 + cfg.nz; q.zlen = cfg.nz;
+    // Lehe coefficients (cyl_em_update_lehe.F90:6-17): beta_rz = beta_thetaz = 1/8,
+    // d_z = (1 - sin^2(pi c / 2) / c^2) / 4; no r pair, no theta pair in gradZ
+    q.lehe = cfg.fdtd == PICTOR_FDTD_LEHE ? 1 : 0;
+    const double brz = 0.125, btz = 0.125;
+    const double sn = std::sin(0.5 * kPi * ph.c);
+    const double dz = 0.25 * (1.0 - (ph.cinv * ph.cinv) * (sn * sn));
+    q.l_rr = 1.0 - 2.0 * brz; q.l_rz = brz;
+    q.l_tt = 1.0 - 2.0 * btz; q.l_tz = btz;
+    q.l_zz = 1.0 - 3.0 * dz; q.l_zd = dz;
 }
 
 }  // namespace pictor
@@ -696,9 +749,15 @@ int cylm_b_half(pictor_ctx *c)
     const GridP &g = c->g;
     double *E[3], *B[3], *J[3];
     fields(c, E, B, J);
-    const long long nb = (long long)(g.mx - 4) * 3 * (g.mz - 1);
-    hipLaunchKernelGGL(k_cylm_update_b, blocks_of(nb), dim3(256), 0, c->stream, g, c->cq, B[0], B[1], B[2], E[0], E[1],
-                       E[2]);
+    if (c->cq.lehe) {
+        const long long nb = (long long)(g.mx - 4) * 3 * (g.mz - 3);
+        hipLaunchKernelGGL(k_cylm_update_b_lehe, blocks_of(nb), dim3(256), 0, c->stream, g, c->cq, B[0], B[1], B[2],
+                           E[0], E[1], E[2]);
+    } else {
+        const long long nb = (long long)(g.mx - 4) * 3 * (g.mz - 1);
+        hipLaunchKernelGGL(k_cylm_update_b, blocks_of(nb), dim3(256), 0, c->stream, g, c->cq, B[0], B[1], B[2], E[0],
+                           E[1], E[2]);
+    }
     hipLaunchKernelGGL(k_cylm_b_axis, blocks_of(3LL * g.mz), dim3(256), 0, c->stream, g, c->cq, B[0], B[1], B[2], E[1]);
     halo(c, B);
     HIPTRY(hipGetLastError());

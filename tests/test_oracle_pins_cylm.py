@@ -198,3 +198,142 @@ def test_gauss_law_conserved_per_mode(dr):
         scale = np.abs(G0[:, s, :]).max()
         assert scale > 0
         assert np.abs(G1 - G0)[:, s, :].max() <= 1e-13 * max(scale, 1.0), s
+
+
+# ---- the Lehe B update per mode ----
+# The modal Lehe keeps the reference's z averaging of the r and theta differences
+# (beta = 1/8) and its wide z pair d_z, and leaves out the r pair (b_zx) and the theta
+# pair (b_zy) of the z difference.  The reference's Lehe-cyl restated on the full grid
+# with those two coefficients set to 0 is its reference.
+
+def _lehe_full(nr, nth, nz, c):
+    full = CylOracle(3, nr, nth, nz, c=c, epc=1, cap=16, fdtd="lehe")
+    L = full.cy.lehe
+    L[2] = L[2] + 2.0 * L[7] + 2.0 * L[8]
+    L[7] = 0.0
+    L[8] = 0.0
+    return full
+
+
+def test_lehe_m0_matches_full_grid():
+    full = _lehe_full(12, 8, 12, 0.3)
+    modal = CylModalOracle(12, 12, dr=1.0, c=0.3, epc=1, cap=16, fdtd=1)
+    assert modal.m.l_zd != 0.0 and modal.m.l_rz == 0.125
+    rng = np.random.default_rng(4)
+    mx, mz = modal.shape[0], modal.shape[2]
+    for n in FIELDS:
+        f0 = rng.normal(size=(mx, mz))
+        full.F[n][...] = f0[:, None, :]
+        modal.F[n][...] = 0.0
+        modal.F[n][:, 0, :] = f0
+    for _ in range(3):
+        full.update_b_half(); full.field_halo("B")
+        full.update_e(); full.field_halo("E")
+        modal.b_half()
+        modal.e_phase(curl=True, add=False)
+    for n in FIELDS:
+        a = _phys(modal.F[n][:, 0, :][:, None, :])
+        for j in range(2, full.p.my - 3):
+            b = _phys(full.F[n][:, j:j + 1, :])
+            assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max(), (n, j)
+    # and the Lehe update differs from Yee's
+    yee = CylModalOracle(12, 12, dr=1.0, c=0.3, epc=1, cap=16, fdtd=0)
+    for n in FIELDS:
+        yee.F[n][...] = 0.0
+        yee.F[n][:, 0, :] = rng.normal(size=(mx, mz))
+    lehe = CylModalOracle(12, 12, dr=1.0, c=0.3, epc=1, cap=16, fdtd=1)
+    lehe.set_fields(**{n: yee.F[n].copy() for n in FIELDS})
+    yee.b_half(); lehe.b_half()
+    assert np.abs(_phys(yee.F["By"]) - _phys(lehe.F["By"])).max() > 1e-3
+
+
+def _m1_lehe_error(nth, seed=3):
+    nr, nz = 10, 8
+    full = _lehe_full(nr, nth, nz, 0.05)
+    modal = CylModalOracle(nr, nz, dr=1.0, c=0.05, epc=1, cap=16, fdtd=1)
+    rng = np.random.default_rng(seed)
+    mx, mz = modal.shape[0], modal.shape[2]
+    i = np.arange(mx)[:, None] + 1.0
+    k = np.arange(mz)[None, :] + 1.0
+    for n in FIELDS:
+        a, b, ph = rng.normal(size=3)
+        fc = np.cos(0.3 * a * i + 0.2 * k + ph) * np.ones((mx, mz))
+        fs = np.sin(0.25 * b * i - 0.3 * k + ph) * np.ones((mx, mz))
+        full.F[n][...] = _sample(full, n, fc, fs)[0]
+        modal.F[n][...] = 0.0
+        modal.F[n][:, 1, :] = fc
+        modal.F[n][:, 2, :] = fs
+    before = {n: full.F[n].copy() for n in FIELDS}
+    full.update_b_half()
+    modal.b_half()
+    err = 0.0
+    for n in ("Bx", "By", "Bz"):
+        mod, _ = _sample(full, n, modal.F[n][:, 1, :], modal.F[n][:, 2, :])
+        sl = (slice(4, -4), slice(2, -3), slice(3, -4))
+        d_full = (full.F[n] - before[n])[sl]
+        d_mod = (mod - before[n])[sl]
+        err = max(err, np.abs(d_full - d_mod).max() / np.abs(d_mod).max())
+    return err
+
+
+def test_lehe_m1_converges_to_full_grid_at_second_order():
+    e16, e32, e64 = _m1_lehe_error(16), _m1_lehe_error(32), _m1_lehe_error(64)
+    assert e16 < 0.05
+    assert 3.3 < e16 / e32 < 4.8, (e16, e32)
+    assert 3.3 < e32 / e64 < 4.8, (e32, e64)
+
+
+def _vacuum_energy_halves(dr, c, slots, steps, restore_r_pair=False, seed=0):
+    """largest E^2 + B^2 (over its start value) in the first and in the second half of a
+    vacuum run from white-noise fields.  Near the Courant limit the start-up ratio is
+    large (E^2 + B^2 at one time level is not the Lehe scheme's conserved norm); a
+    bounded run keeps the second half at the first half's level."""
+    o = CylModalOracle(12, 16, dr=dr, c=c, epc=1, cap=16, fdtd=1)
+    if restore_r_pair:   # the reference's b_zx = (dz/dr)^2 / 8 with its share of the central term
+        bzx = 0.125 / dr ** 2
+        o.m.l_zr = bzx
+        o.m.l_zz = o.m.l_zz - 2.0 * bzx
+    rng = np.random.default_rng(seed)
+    for n in FIELDS:
+        for s in slots:
+            o.F[n][4:-4, s, 3:-3] = rng.normal(size=(o.shape[0] - 8, o.shape[2] - 6))
+    o.e_phase(curl=False, add=False)
+    o.b_half(); o.b_half()
+    e0 = o.energy().sum()
+    halves = [0.0, 0.0]
+    for t in range(steps):
+        o.b_half(); o.b_half(); o.e_phase(curl=True, add=False)
+        if t % 100 == 99:
+            e = o.energy().sum()
+            r = e / e0 if np.isfinite(e) else np.inf
+            h = 0 if t < steps // 2 else 1
+            halves[h] = max(halves[h], r)
+    return halves
+
+
+def test_reference_r_pair_makes_the_axis_unstable():
+    # b_zx's r pair F(i+1) - 2 F(i) + F(i-1) reads the axis fills at i = 4
+    a, b = _vacuum_energy_halves(1.0, 0.39, (0,), 2000, restore_r_pair=True)
+    assert b > 1e4 * a
+    a, b = _vacuum_energy_halves(1.0, 0.39, (0,), 2000)
+    assert b <= 1.1 * a
+
+
+def _courant_rule(dr):
+    # sin^2(pi c / 2) + 1.25 c^2 / dr^2 < 1, and the axis limit of the Yee update
+    lo, hi = 0.0, 1.0
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if math.sin(0.5 * math.pi * mid) ** 2 + 1.25 * mid * mid / dr ** 2 < 1.0:
+            lo = mid
+        else:
+            hi = mid
+    return min(lo, 1.0 / math.sqrt(1.0 + 5.0 / dr ** 2))
+
+
+@pytest.mark.parametrize("dr,c", [(4.0, 0.8), (4.0, None), (8.0, None), (2.0, None), (1.0, None)])
+def test_modal_lehe_stays_bounded(dr, c):
+    c = c if c is not None else 0.999 * _courant_rule(dr)
+    for slots in ((0,), (1, 2)):
+        a, b = _vacuum_energy_halves(dr, c, slots, 8000)
+        assert np.isfinite(a) and b <= 1.1 * a, (dr, c, slots, a, b)
