@@ -200,8 +200,10 @@ int pictor_adjoint_release(pictor_ctx *ctx);
 int pictor_cyl_plasma(pictor_ctx *ctx, int32_t nr_plasma, int32_t k0, int32_t nzc, int32_t ppc, uint64_t seed);
 /* Adds a Gaussian pulse, linearly polarised along x and moving along +z, at
  * focus: E_x = a0 c w0dt exp(-r^2/w0^2 - (z - zc)^2/(c tau)^2) cos(k0 (z - zc)),
- * B_y = E_x, into the m = 1 slots.  omega0 in omega_p, tau in 1/omega_p (c tau
- * is the 1/e half length of the field), w0 in c/omega_p, zc in axial cells. */
+ * B_y the same profile half a step earlier (at z + c/2: between steps B is half
+ * a step behind E), into the m = 1 slots: a purely forward pulse.  omega0 in
+ * omega_p, tau in 1/omega_p (c tau is the 1/e half length of the field), w0 in
+ * c/omega_p, zc in axial cells. */
 int pictor_cyl_laser(pictor_ctx *ctx, double a0, double omega0, double tau, double w0, double zc);
 /* Moving window: after every fused step the fields and particles move back by
  * one axial cell whenever the window (speed c) has advanced one; the front

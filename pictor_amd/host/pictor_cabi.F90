@@ -99,6 +99,24 @@ module pictor_cabi
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
       end function
+      ! the azimuthal-mode cylinder (PICTOR_GEOM_CYL_MODAL): plasma, laser, moving window
+      integer(c_int) function pictor_cyl_plasma(ctx, nr_plasma, k0, nzc, ppc, seed) bind(C, name="pictor_cyl_plasma")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: nr_plasma, k0, nzc, ppc
+         integer(c_int64_t), value :: seed
+      end function
+      integer(c_int) function pictor_cyl_laser(ctx, a0, omega0, tau, w0, zc) bind(C, name="pictor_cyl_laser")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: a0, omega0, tau, w0, zc
+      end function
+      integer(c_int) function pictor_moving_window(ctx, on, nr_plasma, ppc, seed) bind(C, name="pictor_moving_window")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: on, nr_plasma, ppc
+         integer(c_int64_t), value :: seed
+      end function
       integer(c_int) function pictor_add_current(ctx) bind(C, name="pictor_add_current")
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
