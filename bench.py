@@ -428,31 +428,23 @@ LWFA_CONFIGS = {
 }
 
 
-def lwfa_report(args):
-    """Laser wakefield in the azimuthal-mode (m = 0, 1) cylinder with the Lehe
-    B update (dispersion along z corrected): cold electrons
-    at n/n_c = 0.005 (omega_0 = 14.14 omega_p) over immobile ions, 16 particles
-    per (r, z) cell, a linearly polarised Gaussian pulse a0 = 2, tau = 10/omega_p,
-    w0 = 15 c/omega_p, and the moving window (one axial cell per 1/c steps, new
-    plasma injected at the front).  value = particles * steps / wall time of the
-    timed steps (whole steps: fields, push + modal deposit, halos, window shifts
-    and injection).  roofline: k_cylm_push, 108 B per particle (the FP64 record
-    read and written) + 72 B per mode cell (E, B read, J written).
-    cpu_baseline: the modal oracle (one thread) on a 256-cell axial slab of the
-    same plasma.  FDTD: Lehe (the default) or Yee."""
-    from oracle.pyoracle import CylModalOracle
+# algorithmic bytes per particle update of the modal push: the FP64 record (q, x, y, z,
+# u, v, w, flavour read; x, y, z, u, v, w written) or the split record (cell word, 2 FP32
+# offsets, 32-bit angle, 3 FP32 momenta, FP32 weight read; all but the weight written)
+LWFA_BYTES_PER_PARTICLE = {"fp64": 7 * 8 + 4 + 6 * 8, "fp32_split": 8 * 4 + 7 * 4}
+
+
+def _lwfa_run(args, form):
     from pictor_amd import Simulation
 
     nr, nz, dr, cps, nrp, ppc, c = LWFA_CONFIGS[args.lwfa]
     a0, omega0, tau, w0 = 2.0, (1.0 / 0.005) ** 0.5, 10.0, 15.0
     zc = nz - 3.0 * tau * cps            # the pulse centre three lengths behind the front
-    with Simulation(nr, 2, nz, ndim=3, epc=ppc, c=c, c_ompe=cps, mi_me=1e30, form="fp64",
+    with Simulation(nr, 2, nz, ndim=3, epc=ppc, c=c, c_ompe=cps, mi_me=1e30, form=form,
                     geometry="cyl_modal", dr=dr, fdtd=args.lwfa_fdtd) as sim:
         sim.cyl_plasma(nrp, 3, nz, ppc, seed=2024)
         sim.cyl_laser(a0, omega0, tau, w0, zc)
         sim.moving_window(True, nr_plasma=nrp, ppc=ppc, seed=2024)
-        npart = nrp * nz * ppc
-        print(f"# lwfa {args.lwfa}: {npart} particles", file=sys.stderr, flush=True)
         sim.step(args.warmup)
         sim.synchronize()
         sim.timer_reset()
@@ -462,11 +454,40 @@ def lwfa_report(args):
         wall = time.time() - t0
         push_ms, npush = sim.timer("push")
         d = sim.diagnostics()
-    push_ms /= max(npush, 1)
-    value = npart * args.steps / wall
+        mem = sim.memory_bytes()
+    return wall, push_ms / max(npush, 1), d, mem
+
+
+def lwfa_report(args):
+    """Laser wakefield in the azimuthal-mode (m = 0, 1) cylinder with the Lehe
+    B update (dispersion along z corrected): cold electrons at n/n_c = 0.005
+    (omega_0 = 14.14 omega_p) over immobile ions, 16 particles per (r, z) cell,
+    a linearly polarised Gaussian pulse a0 = 2, tau = 10/omega_p, w0 = 15
+    c/omega_p, and the moving window (one axial cell per 1/c steps, new plasma
+    injected at the front).  value = particles * steps / wall time of the timed
+    steps of the split form (whole steps: fields, push + modal deposit, halos,
+    window shifts and injection); the FP64 form runs beside it.  roofline:
+    k_cylm_push_split (60 B per particle) or k_cylm_push (108 B) + 72 B per mode
+    cell (E, B read as FP64, J written).  cpu_baseline: the modal oracle (one
+    thread) on a 256-cell axial slab of the same plasma.  FDTD: Lehe (the
+    default) or Yee."""
+    from oracle.pyoracle import CylModalOracle
+
+    nr, nz, dr, cps, nrp, ppc, c = LWFA_CONFIGS[args.lwfa]
+    npart = nrp * nz * ppc
     ncell = (nr + 5) * 3 * (nz + 5)
-    algo = npart * BYTES_PER_PARTICLE["fp64"] + ncell * GRID_BYTES["fp64"]
-    achieved = algo / (push_ms * 1e-3)
+    forms = [f for f in args.forms.split(",") if f]
+    res = {}
+    for form in forms:
+        print(f"# lwfa {args.lwfa} {form}: {npart} particles", file=sys.stderr, flush=True)
+        wall, push_ms, d, mem = _lwfa_run(args, form)
+        algo = npart * LWFA_BYTES_PER_PARTICLE[form] + ncell * GRID_BYTES["fp64"]
+        res[form] = {"value": npart * args.steps / wall, "ms_per_step": wall / args.steps * 1e3, "push_ms": push_ms,
+                     "achieved": algo / (push_ms * 1e-3), "energies": {k: d[k] for k in ("E_energy", "B_energy")},
+                     "device_bytes": mem}
+        print(f"# {form}: {res[form]['ms_per_step']:.3f} ms/step, push {push_ms:.3f} ms", file=sys.stderr, flush=True)
+    main_form = forms[0]
+    r = res[main_form]
     cpu = None
     if args.cpu_steps != 0:
         nzs, steps = 256, 20
@@ -481,20 +502,26 @@ def lwfa_report(args):
         cpu = {"value": m * steps / dt, "unit": "particle-updates/s", "cores": 1, "kind": "port",
                "sample": f"{steps} steps of the same plasma on a {nr} x {nzs} slab ({m} particles) "
                          f"with the modal oracle, one thread, after 1 warm-up step, {dt:.1f}s"}
-    return {
-        "metric": "particle-updates/s/GPU, cylindrical laser wakefield (modes m = 0, 1) FP64 step",
-        "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+    kernel = "k_cylm_push_split" if main_form == "fp32_split" else "k_cylm_push"
+    out = {
+        "metric": "particle-updates/s/GPU, cylindrical laser wakefield (modes m = 0, 1)",
+        "value": r["value"], "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if main_form == "fp32_split" else "f64", "data": "synthetic",
         "config": {"workload": f"{args.lwfa}: modal cylinder nz={nz} nr={nr}, dz=0.05, dr=0.2 c/omega_p, "
                                f"c dt = {c} dz, n/n_c=0.005, immobile ions, {ppc} ppc in {nrp} radial cells, "
-                               f"a0={a0}, tau={tau}/omega_p, w0={w0} c/omega_p, moving window, "
+                               f"a0=2.0, tau=10.0/omega_p, w0=15.0 c/omega_p, moving window, "
                                f"{'Lehe' if args.lwfa_fdtd == 'lehe' else 'Yee'} FDTD",
-                   "form": "fp64", "particles": npart, "energies": {k: d[k] for k in ("E_energy", "B_energy")}},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_cylm_push", "push_ms": push_ms},
+                   "form": main_form, "particles": npart, "energies": r["energies"],
+                   "device_bytes": r["device_bytes"]},
+        "roofline": {"bound": "hbm", "achieved": r["achieved"] / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": r["achieved"] / HBM_PEAK, "traffic": None, "kernel": kernel, "push_ms": r["push_ms"]},
         "cpu_baseline": cpu,
     }
+    for form in forms[1:]:
+        out[f"{form}_form"] = res[form]
+        out[f"{main_form}_over_{form}"] = r["value"] / res[form]["value"]
+    return out
 
 
 def atomics_report(iters=4096):

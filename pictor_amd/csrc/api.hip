@@ -434,8 +434,10 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     const bool modal = cfg->geometry == PICTOR_GEOM_CYL_MODAL;
     if (modal) {
         if (cfg->ndim != 3 || cfg->ny != 2) return fail(PICTOR_ERR_ARG, "the modal cylinder needs ndim 3 and ny = 2 (modes m = 0, 1)");
-        if (cfg->form != PICTOR_FORM_FP64 || cfg->curr_filter)
-            return fail(PICTOR_ERR_ARG, "the modal cylinder runs the FP64 form with no current filter");
+        if (cfg->curr_filter) return fail(PICTOR_ERR_ARG, "the modal cylinder runs without the current filter");
+        // the split record packs i in 12 bits and k in 19 bits of its cell word
+        if (cfg->form == PICTOR_FORM_FP32_SPLIT && (cfg->nx > 4090 || cfg->nz > 500000))
+            return fail(PICTOR_ERR_ARG, "modal cylinder, split form: nx <= 4090 and nz <= 500000");
         if (cfg->dr < 0.0) return fail(PICTOR_ERR_ARG, "dr must be >= 0");
         // the m = 1 terms near the axis act like a theta cell of dr / 2 at r = dr / 2
         const double dr = cfg->dr > 0.0 ? cfg->dr : 1.0;
@@ -451,7 +453,8 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     const bool cyl = cfg->geometry == PICTOR_GEOM_CYL || modal;
     if (cfg->curr_filter < 0 || cfg->curr_filter > 64) return fail(PICTOR_ERR_ARG, "curr_filter must be 0..64");
     if (cyl && cfg->curr_filter) return fail(PICTOR_ERR_ARG, "the current filter is for the Cartesian geometry");
-    if (cyl && cfg->form != PICTOR_FORM_FP64) return fail(PICTOR_ERR_ARG, "the cylindrical geometry runs in the FP64 form");
+    if (cyl && !modal && cfg->form != PICTOR_FORM_FP64)
+        return fail(PICTOR_ERR_ARG, "the cylindrical geometry runs in the FP64 form");
     if (cyl && !modal && cfg->ny % 2) return fail(PICTOR_ERR_ARG, "the cylindrical grid needs an even number of theta cells (ny)");
     if (cyl && !modal) {
         // the narrowest cell of the r-theta grid is the arc 0.5 * dtheta at r = 1/2 (grid index 4):
@@ -547,7 +550,9 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     auto cleanup_fail = [&](int rc) { pictor_finalize(c); return rc; };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup_fail(fail(PICTOR_ERR_HIP, "hipStreamCreate failed"));
-    const size_t rs = (c->form == PICTOR_FORM_FP64) ? sizeof(double) : sizeof(float);
+    // FP64 fields: the FP64 form and the modal cylinder (whose split form keeps FP64 fields)
+    const bool f64 = c->form == PICTOR_FORM_FP64 || c->cylm;
+    const size_t rs = f64 ? sizeof(double) : sizeof(float);
     for (int i = 0; i < 3; i++) {
         if (hipMalloc(&c->E[i], g.ncell * rs) != hipSuccess || hipMalloc(&c->B[i], g.ncell * rs) != hipSuccess ||
             hipMalloc(&c->J[i], g.ncell * 8) != hipSuccess)
@@ -555,7 +560,7 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
         hipMemsetAsync(c->E[i], 0, g.ncell * rs, c->stream);
         hipMemsetAsync(c->B[i], 0, g.ncell * rs, c->stream);
         hipMemsetAsync(c->J[i], 0, g.ncell * 8, c->stream);
-        if (c->form == PICTOR_FORM_FP32_SPLIT) {
+        if (!f64) {
             if (hipMalloc(&c->Elo[i], g.ncell * sizeof(float)) != hipSuccess)
                 return cleanup_fail(fail(PICTOR_ERR_HIP, "field allocation failed"));
             hipMemsetAsync(c->Elo[i], 0, g.ncell * sizeof(float), c->stream);
@@ -640,11 +645,12 @@ int pictor_upload_fields(pictor_ctx *c, const double *ex, const double *ey, cons
     const double *src[6] = {ex, ey, ez, bx, by, bz};
     void *dst[6] = {c->E[0], c->E[1], c->E[2], c->B[0], c->B[1], c->B[2]};
     const size_t n = (size_t)c->g.ncell;
+    const bool f64 = c->form == PICTOR_FORM_FP64 || c->cylm;
     double *tmp = nullptr;
-    if (c->form != PICTOR_FORM_FP64) HIPTRY(dalloc(&tmp, n));
+    if (!f64) HIPTRY(dalloc(&tmp, n));
     for (int i = 0; i < 6; i++) {
         if (!src[i]) return fail(PICTOR_ERR_ARG, "null field pointer");
-        if (c->form == PICTOR_FORM_FP64) {
+        if (f64) {
             HIPTRY(hipMemcpyAsync(dst[i], src[i], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
         } else {
             HIPTRY(hipMemcpyAsync(tmp, src[i], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -665,11 +671,12 @@ int pictor_download_fields(pictor_ctx *c, double *ex, double *ey, double *ez, do
     double *dst[9] = {ex, ey, ez, bx, by, bz, jx, jy, jz};
     void *src[9] = {c->E[0], c->E[1], c->E[2], c->B[0], c->B[1], c->B[2], c->J[0], c->J[1], c->J[2]};
     const size_t n = (size_t)c->g.ncell;
+    const bool f64 = c->form == PICTOR_FORM_FP64 || c->cylm;
     double *tmp = nullptr;
-    if (c->form != PICTOR_FORM_FP64) HIPTRY(dalloc(&tmp, n));
+    if (!f64) HIPTRY(dalloc(&tmp, n));
     for (int i = 0; i < 9; i++) {
         if (!dst[i]) continue;
-        if (c->form == PICTOR_FORM_FP64) {
+        if (f64) {
             HIPTRY(hipMemcpyAsync(dst[i], src[i], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         } else {
             if (i < 3) launch_pair_to_f64((const float *)src[i], c->Elo[i], tmp, (long long)n, c->stream);
@@ -772,7 +779,8 @@ int pictor_download_particles(pictor_ctx *c, int64_t cap, double *q, double *x, 
     if (c->pending_swap) return fail(PICTOR_ERR_ARG, "download between move_deposit and exchange_particles");
     if (c->cyl) {
         long long no = 0;
-        rc = cyl_download_particles(c, cap, q, x, y, z, u, v, w, flv, tile, &no);
+        rc = c->cylm ? cylm_download_particles(c, cap, q, x, y, z, u, v, w, flv, tile, &no)
+                     : cyl_download_particles(c, cap, q, x, y, z, u, v, w, flv, tile, &no);
         if (n_out) *n_out = no;
         return rc;
     }
@@ -1072,6 +1080,12 @@ int pictor_timer_get(pictor_ctx *c, int32_t id, double *total_ms, int64_t *count
 int pictor_memory_bytes(pictor_ctx *c, int64_t *bytes)
 {
     if (!c || !bytes) return fail(PICTOR_ERR_ARG, "null argument");
+    if (c->cylm) {
+        // FP64 E, B, J; the particle slots (FP64 record 60 B, split record 32 B)
+        const long long per = (c->form == PICTOR_FORM_FP64) ? 60 : 32;
+        *bytes = c->g.ncell * 9 * 8 + c->cm.cap * per;
+        return PICTOR_OK;
+    }
     const long long rs = (c->form == PICTOR_FORM_FP64) ? 8 : 4;
     long long b = c->g.ncell * (6 * rs + 3 * 8 + 8);
     const long long per = (c->form == PICTOR_FORM_FP64) ? 60 : 28;

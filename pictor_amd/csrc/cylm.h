@@ -20,8 +20,20 @@ struct CylmP {
     double l_rr, l_rz, l_tt, l_tz, l_zz, l_zd;
 };
 
+// the split particle record of the modal path (FP32 push, fixed-point current):
+// 32 B read and 28 B written per particle update
+struct PartCylS {
+    uint32_t *cell = nullptr;   // 0: hole; else (flavour 1: bit 31) | k << 12 | i
+    float *ox = nullptr;        // radial offset in the cell, [0, 1) in index units
+    float *oz = nullptr;        // axial offset in the cell, [0, 1)
+    uint32_t *th = nullptr;     // theta in units of 2^-32 turns
+    float *u = nullptr, *v = nullptr, *w = nullptr;   // local-frame momenta
+    float *q = nullptr;         // charge weight (the FP64 record's q)
+};
+
 // particle storage and moving-window state of the modal path
 struct CylmState {
+    PartCylS ps;                         // the split form's particles (FP32_SPLIT); pcyl holds FP64
     long long cap = 0;                   // allocated particle slots (c->np slots in use, holes included)
     int *d_err = nullptr;
     unsigned long long *d_count = nullptr;
@@ -40,6 +52,8 @@ int cylm_setup(pictor_ctx *c);
 void cylm_release(pictor_ctx *c);
 int cylm_upload_particles(pictor_ctx *c, long long n, const double *q, const double *x, const double *y,
                           const double *z, const double *u, const double *v, const double *w, const int *flv);
+int cylm_download_particles(pictor_ctx *c, long long cap, double *q, double *x, double *y, double *z, double *u,
+                            double *v, double *w, int *flv, int *tile, long long *n_out);
 int cylm_b_half(pictor_ctx *c);
 int cylm_e_phase(pictor_ctx *c, bool curl, bool add);
 int cylm_current_halo(pictor_ctx *c);

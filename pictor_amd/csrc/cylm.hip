@@ -20,7 +20,9 @@
 #include <climits>
 #include <cmath>
 #include <string>
+#include <type_traits>
 #include <vector>
+#include <algorithm>
 
 #include "ctx.h"
 
@@ -37,6 +39,32 @@ __device__ __forceinline__ long long mi(const GridP &g, int i, int s, int k)
 __device__ __forceinline__ double rad(const CylmP &q, double i) { return (i - 3.5) * q.dr; }
 
 __device__ __forceinline__ double parity(int s) { return s == 0 ? 1.0 : -1.0; }
+
+// ---- the split record: (i, k) cell word, FP32 offsets, theta in 2^-32 turns ----
+constexpr double kTurnRad = 2.0 * kPi / 4294967296.0;   // radians per 2^-32 turn
+constexpr double kJq = 68719476736.0;                    // 2^36: current counts per charge cell
+constexpr double kJqInv = 1.0 / 68719476736.0;
+
+__device__ __host__ inline uint32_t pack_cell(int i, int k, int flv)
+{
+    return (flv == 1 ? 0x80000000u : 0u) | ((uint32_t)k << 12) | (uint32_t)i;
+}
+__device__ __host__ inline int cell_i(uint32_t w) { return (int)(w & 0xFFFu); }
+__device__ __host__ inline int cell_k(uint32_t w) { return (int)((w >> 12) & 0x7FFFFu); }
+__device__ __host__ inline int cell_flv(uint32_t w) { return (w >> 31) ? 1 : 2; }
+
+// a coordinate as (cell, offset in [0, 1)) with an FP32 offset
+__device__ __host__ inline void split_coord(double x, int &i, float &o)
+{
+    i = (int)floor(x);
+    o = (float)(x - i);
+    if (o >= 1.0f) { i += 1; o = 0.0f; }
+}
+__device__ __host__ inline uint32_t theta_turns(double th)
+{
+    return (uint32_t)(unsigned long long)(long long)llrint(th * (4294967296.0 / (2.0 * kPi)));
+}
+
 
 // UpdateEfield (cyl_em_update.F90:7-46, 3-D): i = 4..mx-3, k = 3..mz-3, every slot
 __global__ void k_cylm_update_e(GridP g, CylmP q, double *__restrict__ Ex, double *__restrict__ Ey,
@@ -441,6 +469,173 @@ __global__ __launch_bounds__(256) void k_cylm_push(GridP g, CylmP q, const doubl
     }
 }
 
+// the FP32 gather: FP64 field values, FP32 sums, at (cell ip, kp; offsets dx, dz) and
+// the phases (1, cos theta, sin theta)
+__device__ inline void gather_f32(const GridP &g, const double *const *E, const double *const *B, int ip, int kp,
+                                  float dx, float dz, float c1, float s1, float pe[3], float pb[3])
+{
+    const float ph[3] = {1.0f, c1, s1};
+    for (int c = 0; c < 3; c++) { pe[c] = 0.0f; pb[c] = 0.0f; }
+    for (int s = 0; s < 3; s++) {
+        float se[3] = {0.0f, 0.0f, 0.0f}, sb[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int nn = 0; nn < 4; nn++) {
+            const int a = nn & 1, b = nn >> 1;
+            const float wt = (a ? dx : 1.0f - dx) * (b ? dz : 1.0f - dz);
+            double e[3], bb[3];
+            node_vals(g, E, B, ip + a, s, kp + b, e, bb);
+            for (int c = 0; c < 3; c++) { se[c] += wt * (float)e[c]; sb[c] += wt * (float)bb[c]; }
+        }
+        for (int c = 0; c < 3; c++) { pe[c] += ph[s] * se[c]; pb[c] += ph[s] * sb[c]; }
+    }
+}
+
+// MoveDepositPrtl of the split form: the push in FP32 on (cell, offset) positions, theta
+// in 2^-32 turns; the deposit in FP64 from the stored old and new positions (exact in
+// FP64), each contribution rounded to a multiple of 2^-36 and summed as an FP64 integer
+// count: every sum is exact, so J does not depend on the order of the atomics.
+// Flags: 1 = a particle reached the wall, 2 = a contribution above 64 charge cells (the
+// fixed-point range).
+__global__ __launch_bounds__(256) void k_cylm_push_split(GridP g, CylmP q, const double *__restrict__ Ex,
+                                                         const double *__restrict__ Ey, const double *__restrict__ Ez,
+                                                         const double *__restrict__ Bx, const double *__restrict__ By,
+                                                         const double *__restrict__ Bz, double *__restrict__ Jx,
+                                                         double *__restrict__ Jy, double *__restrict__ Jz, PartCylS P,
+                                                         long long n, int *__restrict__ err)
+{
+    __shared__ double sJ[9 * kDepCells];
+    __shared__ int sred[4][4];
+    const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t cw = m < n ? P.cell[m] : 0u;
+    bool dep = false;
+    DepMove d;
+    if (cw != 0u) {
+        const double *E[3] = {Ex, Ey, Ez}, *B[3] = {Bx, By, Bz};
+        const int i0 = cell_i(cw), k0 = cell_k(cw), flv = cell_flv(cw);
+        const float ox = P.ox[m], oz = P.oz[m];
+        const uint32_t t0 = P.th[m];
+        float s0, c0;
+        sincosf((float)((double)t0 * kTurnRad), &s0, &c0);
+        float pe[3], pb[3];
+        gather_f32(g, E, B, i0, k0, ox, oz, c0, s0, pe, pb);
+        const float qm = (float)(q.flvrqm[flv - 1] * 0.5);
+        const float c = (float)q.c, sqc = (float)q.sqc, cinv = (float)q.cinv;
+        const float pEx = pe[0] * qm, pEy = pe[1] * qm, pEz = pe[2] * qm;
+        float pBx = pb[0] * qm, pBy = pb[1] * qm, pBz = pb[2] * qm;
+        float u0 = c * P.u[m] + pEx;
+        float v0 = c * P.v[m] + pEy;
+        float w0 = c * P.w[m] + pEz;
+        float gg = rsqrtf(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+        pBx *= gg; pBy *= gg; pBz *= gg;
+        const float ff = 2.0f / (1.0f + pBx * pBx + pBy * pBy + pBz * pBz);
+        float u1 = (u0 + v0 * pBz - w0 * pBy) * ff;
+        float v1 = (v0 + w0 * pBx - u0 * pBz) * ff;
+        const float w1 = (w0 + u0 * pBy - v0 * pBx) * ff;
+        u0 = u0 + v1 * pBz - w1 * pBy + pEx;
+        v0 = v0 + w1 * pBx - u1 * pBz + pEy;
+        w0 = w0 + u1 * pBy - v1 * pBx + pEz;
+        u1 = u0 * cinv;
+        v1 = v0 * cinv;
+        const float wn = w0 * cinv;
+        gg = sqc * rsqrtf(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+        // the move in the local frame: X = r0 + dX, Y; the new radius as r0 + dR with
+        // dR = (dX (2 r0 + dX) + Y^2) / (r0 + r1), so the offset keeps FP32 precision
+        const float dr = (float)q.dr;
+        const float r0 = ((float)(i0 - 3.5) + ox) * dr;
+        const float dX = u1 * gg, Y = v1 * gg, X = r0 + dX;
+        const float r1 = sqrtf(X * X + Y * Y);
+        const float rs = r0 + r1;
+        const float dR = rs > 0.0f ? (dX * (2.0f * r0 + dX) + Y * Y) / rs : r1;
+        const float xo = ox + dR * (float)q.inv_dr, zo = oz + wn * gg * (float)q.inv_dz;
+        const float fx = floorf(xo), fz = floorf(zo);
+        int i2 = i0 + (int)fx, k2 = k0 + (int)fz;
+        float ox2 = xo - fx, oz2 = zo - fz;
+        if (ox2 >= 1.0f) { i2 += 1; ox2 = 0.0f; }
+        if (oz2 >= 1.0f) { k2 += 1; oz2 = 0.0f; }
+        const float ir = r1 > 0.0f ? 1.0f / r1 : 0.0f;
+        const float un = r1 > 0.0f ? (X * u1 + Y * v1) * ir : u1;
+        const float vn = r1 > 0.0f ? (-Y * u1 + X * v1) * ir : v1;
+        const float dth = atan2f(Y, X);
+        const uint32_t t1 = t0 + (uint32_t)(unsigned long long)(long long)llrintf(dth * (float)(4294967296.0 / (2.0 * kPi)));
+        if (i2 >= g.mx - 3) {
+            atomicOr(err, 1);
+        } else {
+            dep = true;
+            const double qthis = (double)P.q[m] * q.qi;
+            const double x0 = i0 + (double)ox, z0 = k0 + (double)oz;
+            const double x1 = i2 + (double)ox2, z1 = k2 + (double)oz2;
+            d.i1 = i0; d.i2 = i2; d.k1 = k0; d.k2 = k2;
+            const double xr = fmin((double)(min(d.i1, d.i2) + 1), fmax((double)max(d.i1, d.i2), 0.5 * (x0 + x1)));
+            const double zr = fmin((double)(min(d.k1, d.k2) + 1), fmax((double)max(d.k1, d.k2), 0.5 * (z0 + z1)));
+            d.Fx1 = qthis * (xr - x0); d.Fz1 = qthis * (zr - z0);
+            d.Fx2 = qthis * (x1 - xr); d.Fz2 = qthis * (z1 - zr);
+            d.Wx1 = 0.5 * (x0 + xr) - d.i1; d.Wz1 = 0.5 * (z0 + zr) - d.k1;
+            d.Wx2 = 0.5 * (x1 + xr) - d.i2; d.Wz2 = 0.5 * (z1 + zr) - d.k2;
+            d.wx = x1 - d.i2; d.wz = z1 - d.k2;
+            // the phases of the stored angles, as the next step's gather sees them
+            float s1, c1;
+            sincosf((float)((double)t1 * kTurnRad), &s1, &c1);
+            d.pa[0] = 1.0; d.pa[1] = c0; d.pa[2] = s0;
+            d.pt[0] = qthis * ((double)(int32_t)(t1 - t0) * kTurnRad);
+            d.pt[1] = qthis * ((double)s1 - (double)s0);
+            d.pt[2] = -(qthis * ((double)c1 - (double)c0));
+        }
+        // periodic z; the stored record
+        if (k2 >= (int)q.zmax) k2 -= (int)q.zlen;
+        else if (k2 < (int)q.zmin) k2 += (int)q.zlen;
+        P.cell[m] = pack_cell(i2, k2, flv);
+        P.ox[m] = ox2; P.oz[m] = oz2; P.th[m] = t1;
+        P.u[m] = un; P.v[m] = vn; P.w[m] = wn;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ilo = wave_min(dep ? min(d.i1, d.i2) : INT_MAX), ihi = wave_max(dep ? max(d.i1, d.i2) : INT_MIN);
+    const int klo = wave_min(dep ? min(d.k1, d.k2) : INT_MAX), khi = wave_max(dep ? max(d.k1, d.k2) : INT_MIN);
+    if (lane == 0) { sred[wv][0] = ilo; sred[wv][1] = ihi; sred[wv][2] = klo; sred[wv][3] = khi; }
+    __syncthreads();
+    int bil = INT_MAX, bih = INT_MIN, bkl = INT_MAX, bkh = INT_MIN;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        bil = min(bil, sred[w][0]); bih = max(bih, sred[w][1]);
+        bkl = min(bkl, sred[w][2]); bkh = max(bkh, sred[w][3]);
+    }
+    if (bil > bih) return;
+    const int wi = bih + 2 - bil, wk = bkh + 2 - bkl;
+    double *J[3] = {Jx, Jy, Jz};
+    bool big = false;
+    auto count = [&](double v) {
+        big = big || !(fabs(v) <= 64.0);
+        return rint(v * kJq);
+    };
+    if ((long long)wi * wk > kDepCells) {
+        if (dep)
+            modal_deposit(d, [&](int cc, int s, int i, int k, double v) {
+                unsafeAtomicAdd(&J[cc][mi(g, i, s, k)], count(v) * kJqInv);
+            });
+        if (big) atomicOr(err, 2);
+        return;
+    }
+    const int cells = wi * wk;
+    int R = kDepCopies;
+    while (R > 1 && cells * R > kDepCells) R >>= 1;
+    const int rl = lane & (R - 1);
+    for (int e = threadIdx.x; e < 9 * cells * R; e += 256) sJ[e] = 0.0;
+    __syncthreads();
+    if (dep)
+        modal_deposit(d, [&](int cc, int s, int i, int k, double v) {
+            atomicAdd(&sJ[((cc * 3 + s) * cells + (k - bkl) * wi + (i - bil)) * R + rl], count(v));
+        });
+    if (big) atomicOr(err, 2);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 9 * cells; e += 256) {
+        double v = 0.0;
+        for (int r = 0; r < R; r++) v += sJ[e * R + r];
+        if (v == 0.0) continue;
+        const int cs = e / cells, cell = e - cs * cells;
+        const int i = bil + cell % wi, k = bkl + cell / wi;
+        unsafeAtomicAdd(&J[cs / 3][mi(g, i, cs % 3, k)], v * kJqInv);
+    }
+}
+
 __device__ inline uint64_t mix64d(uint64_t z)
 {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -456,26 +651,57 @@ __device__ inline double u01d(uint64_t seed, uint64_t idx, uint64_t stream)
 // uniform cold electrons: ppc per cell of i = 4..4+nrp-1, k = k0..k0+nzc-1, (r, z)
 // uniform in the cell, ppc evenly spaced angles with a per-cell random offset,
 // weight epc 2 pi r dr / ppc; counter-based, so a cell (shifted by `shift`
-// window moves) always gets the same particles.  Appended at base + index.
-__global__ void k_cylm_inject(GridP g, CylmP q, PartF64 P, long long base, int nrp, int k0, int nzc, int ppc,
-                              int epc, unsigned long long seed, long long shift)
+// window moves) always gets the same particles.
+__device__ inline void plasma_particle(const GridP &g, const CylmP &q, long long t, int nrp, int k0, int ppc, int epc,
+                                       unsigned long long seed, long long shift, double &x, double &th, double &z,
+                                       double &qq)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long total = (long long)nrp * nzc * ppc;
-    if (t >= total) return;
     const int p = (int)(t % ppc);
     const long long cidx = t / ppc;
     const int i = 4 + (int)(cidx % nrp), k = k0 + (int)(cidx / nrp);
     const uint64_t cell = (uint64_t)((k - 3 + shift) * (long long)g.nx + (i - 4));
     const double off = u01d(seed, cell, 7);
     const uint64_t id = cell * (uint64_t)ppc + (uint64_t)p;
-    const double x = i + u01d(seed, id, 1), z = k + u01d(seed, id, 2);
-    const double th = 2.0 * kPi * ((p + off) / ppc);
+    x = i + u01d(seed, id, 1);
+    z = k + u01d(seed, id, 2);
+    th = 2.0 * kPi * ((p + off) / ppc);
+    qq = -(epc * 2.0 * kPi * rad(q, x) * q.dr / ppc);
+}
+
+// appended at base + index
+__global__ void k_cylm_inject(GridP g, CylmP q, PartF64 P, long long base, int nrp, int k0, int nzc, int ppc,
+                              int epc, unsigned long long seed, long long shift)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)nrp * nzc * ppc;
+    if (t >= total) return;
+    double x, th, z, qq;
+    plasma_particle(g, q, t, nrp, k0, ppc, epc, seed, shift, x, th, z, qq);
     const long long n = base + t;
     P.x[n] = x; P.y[n] = th; P.z[n] = z;
     P.u[n] = 0.0; P.v[n] = 0.0; P.w[n] = 0.0;
-    P.q[n] = -(epc * 2.0 * kPi * rad(q, x) * q.dr / ppc);
+    P.q[n] = qq;
     P.flv[n] = 2;
+}
+
+__global__ void k_cylm_inject_split(GridP g, CylmP q, PartCylS P, long long base, int nrp, int k0, int nzc, int ppc,
+                                    int epc, unsigned long long seed, long long shift)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)nrp * nzc * ppc;
+    if (t >= total) return;
+    double x, th, z, qq;
+    plasma_particle(g, q, t, nrp, k0, ppc, epc, seed, shift, x, th, z, qq);
+    const long long n = base + t;
+    int i, k;
+    float ox, oz;
+    split_coord(x, i, ox);
+    split_coord(z, k, oz);
+    P.cell[n] = pack_cell(i, k, 2);
+    P.ox[n] = ox; P.oz[n] = oz;
+    P.th[n] = theta_turns(th);
+    P.u[n] = 0.0f; P.v[n] = 0.0f; P.w[n] = 0.0f;
+    P.q[n] = (float)qq;
 }
 
 // a linearly polarised (x) Gaussian pulse moving along +z, at focus (no curvature):
@@ -527,6 +753,17 @@ __global__ void k_cylm_shift_particles(CylmP q, PartF64 P, long long n)
     if (z < q.zmin) P.flv[m] = 0;
 }
 
+// the same on the split record: k -= 1; a particle with k - 1 < zmin (z < zmin) is dropped
+__global__ void k_cylm_shift_split(CylmP q, PartCylS P, long long n)
+{
+    const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const uint32_t w = P.cell[m];
+    if (w == 0u) return;
+    const int k = cell_k(w) - 1;
+    P.cell[m] = k < (int)q.zmin ? 0u : pack_cell(cell_i(w), k, cell_flv(w));
+}
+
 // 1/2 sum B^2, E^2 with the volume 2 pi r dr (pi for the m = 1 slots) of each stagger
 __global__ void k_cylm_energy(GridP g, CylmP q, const double *__restrict__ Ex, const double *__restrict__ Ey,
                               const double *__restrict__ Ez, const double *__restrict__ Bx,
@@ -558,18 +795,49 @@ __global__ void k_cylm_energy(GridP g, CylmP q, const double *__restrict__ Ex, c
     if (threadIdx.x == 0) { partial[2 * blockIdx.x] = 0.5 * sb[0]; partial[2 * blockIdx.x + 1] = 0.5 * se[0]; }
 }
 
+// the two particle records behind one interface: liveness, flavour, momenta, weight, copy
+struct RecF64 {
+    PartF64 P;
+    __device__ bool live(long long m) const { return P.flv[m] != 0; }
+    __device__ int flv(long long m) const { return P.flv[m]; }
+    __device__ void mom(long long m, double &u, double &v, double &w, double &q) const
+    {
+        u = P.u[m]; v = P.v[m]; w = P.w[m]; q = P.q[m];
+    }
+    __device__ void copy_to(long long m, const RecF64 &O, long long d) const
+    {
+        O.P.q[d] = P.q[m]; O.P.x[d] = P.x[m]; O.P.y[d] = P.y[m]; O.P.z[d] = P.z[m];
+        O.P.u[d] = P.u[m]; O.P.v[d] = P.v[m]; O.P.w[d] = P.w[m]; O.P.flv[d] = P.flv[m];
+    }
+};
+struct RecSplit {
+    PartCylS P;
+    __device__ bool live(long long m) const { return P.cell[m] != 0u; }
+    __device__ int flv(long long m) const { return cell_flv(P.cell[m]); }
+    __device__ void mom(long long m, double &u, double &v, double &w, double &q) const
+    {
+        u = P.u[m]; v = P.v[m]; w = P.w[m]; q = P.q[m];
+    }
+    __device__ void copy_to(long long m, const RecSplit &O, long long d) const
+    {
+        O.P.cell[d] = P.cell[m]; O.P.ox[d] = P.ox[m]; O.P.oz[d] = P.oz[m]; O.P.th[d] = P.th[m];
+        O.P.u[d] = P.u[m]; O.P.v[d] = P.v[m]; O.P.w[d] = P.w[m]; O.P.q[d] = P.q[m];
+    }
+};
+
 // kinetic energy sum (gamma - 1) |q| per flavour (CalcPrtlEnergy, savedata_routines.F90:283-308
 // without the mass factor) and the live count
-__global__ void k_cylm_kinetic(PartF64 P, long long n, double *partial)
+template <class R>
+__global__ void k_cylm_kinetic(R P, long long n, double *partial)
 {
     __shared__ double s1[256], s2[256], sn[256];
     double a = 0.0, b = 0.0, cnt = 0.0;
     for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (long long)gridDim.x * blockDim.x) {
-        const int f = P.flv[m];
-        if (f == 0) continue;
-        const double u = P.u[m], v = P.v[m], w = P.w[m];
-        const double e = (sqrt(1.0 + u * u + v * v + w * w) - 1.0) * fabs(P.q[m]);
-        if (f == 1) a += e; else b += e;
+        if (!P.live(m)) continue;
+        double u, v, w, qq;
+        P.mom(m, u, v, w, qq);
+        const double e = (sqrt(1.0 + u * u + v * v + w * w) - 1.0) * fabs(qq);
+        if (P.flv(m) == 1) a += e; else b += e;
         cnt += 1.0;
     }
     s1[threadIdx.x] = a; s2[threadIdx.x] = b; sn[threadIdx.x] = cnt;
@@ -585,14 +853,53 @@ __global__ void k_cylm_kinetic(PartF64 P, long long n, double *partial)
     }
 }
 
-// stream compaction of the live particles into `out` (order not kept); count in *cnt
-__global__ void k_cylm_compact(PartF64 P, long long n, PartF64 O, unsigned long long *cnt)
+// order-keeping compaction of the live particles: the deposit window relies on the
+// loader's cell order.  Pass 1 counts the live particles of each span of kSpan; the host
+// scans the counts; pass 2 writes each span's particles at its offset, in order.
+constexpr int kSpan = 1024;
+
+template <class R>
+__global__ void k_cylm_span_count(R P, long long n, int *cnt)
 {
-    const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n || P.flv[m] == 0) return;
-    const unsigned long long d = atomicAdd(cnt, 1ull);
-    O.q[d] = P.q[m]; O.x[d] = P.x[m]; O.y[d] = P.y[m]; O.z[d] = P.z[m];
-    O.u[d] = P.u[m]; O.v[d] = P.v[m]; O.w[d] = P.w[m]; O.flv[d] = P.flv[m];
+    __shared__ int sw[4];
+    const long long base = (long long)blockIdx.x * kSpan;
+    int c = 0;
+    for (int j = threadIdx.x; j < kSpan; j += 256) {
+        const long long m = base + j;
+        if (m < n && P.live(m)) c++;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = sw[0] + sw[1] + sw[2] + sw[3];
+}
+
+template <class R>
+__global__ void k_cylm_compact_spans(R P, long long n, const long long *__restrict__ off, R O)
+{
+    // thread t owns the 4 consecutive particles base + 4t .. 4t + 3
+    __shared__ int ss[256];
+    const long long base = (long long)blockIdx.x * kSpan + 4LL * threadIdx.x;
+    int f[4], c = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const long long m = base + j;
+        f[j] = (m < n && P.live(m)) ? 1 : 0;
+        c += f[j];
+    }
+    ss[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {     // inclusive scan (Hillis-Steele)
+        const int v = (int)threadIdx.x >= o ? ss[threadIdx.x - o] : 0;
+        __syncthreads();
+        ss[threadIdx.x] += v;
+        __syncthreads();
+    }
+    long long d = off[blockIdx.x] + ss[threadIdx.x] - c;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (f[j]) P.copy_to(base + j, O, d++);
 }
 
 void fields(pictor_ctx *c, double *E[3], double *B[3], double *J[3])
@@ -628,36 +935,80 @@ int alloc_parts(PartF64 &p, long long cap)
     return PICTOR_OK;
 }
 
-// room for `extra` more particles: compact first, then grow (x1.25) if still short
-int ensure_room(pictor_ctx *c, long long extra)
+void free_parts(PartCylS &p)
+{
+    dfree(p.cell); dfree(p.ox); dfree(p.oz); dfree(p.th); dfree(p.u); dfree(p.v); dfree(p.w); dfree(p.q);
+    p = PartCylS{};
+}
+
+int alloc_parts(PartCylS &p, long long cap)
+{
+    float **a[6] = {&p.ox, &p.oz, &p.u, &p.v, &p.w, &p.q};
+    for (auto x : a) HIPTRY(dalloc(x, cap));
+    HIPTRY(dalloc(&p.cell, cap));
+    HIPTRY(dalloc(&p.th, cap));
+    return PICTOR_OK;
+}
+
+bool split_form(const pictor_ctx *c) { return c->form == PICTOR_FORM_FP32_SPLIT; }
+
+void free_particles(pictor_ctx *c)
+{
+    free_parts(c->pcyl);
+    free_parts(c->cm.ps);
+}
+
+// order-keeping compaction of `rec` (n slots) into `out`; returns the live count
+template <class R>
+int compact_into(pictor_ctx *c, const R &rec, long long n, const R &out, long long &live)
+{
+    const long long nb = (n + kSpan - 1) / kSpan;
+    int *d_cnt = nullptr;
+    long long *d_off = nullptr;
+    HIPTRY(dalloc(&d_cnt, nb));
+    HIPTRY(dalloc(&d_off, nb));
+    hipLaunchKernelGGL(k_cylm_span_count<R>, dim3((unsigned)nb), dim3(256), 0, c->stream, rec, n, d_cnt);
+    std::vector<int> h(nb);
+    HIPTRY(hipMemcpyAsync(h.data(), d_cnt, nb * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    std::vector<long long> off(nb);
+    long long a = 0;
+    for (long long b = 0; b < nb; b++) { off[b] = a; a += h[b]; }
+    HIPTRY(hipMemcpyAsync(d_off, off.data(), nb * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_cylm_compact_spans<R>, dim3((unsigned)nb), dim3(256), 0, c->stream, rec, n, d_off, out);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipStreamSynchronize(c->stream));
+    dfree(d_cnt);
+    dfree(d_off);
+    live = a;
+    return PICTOR_OK;
+}
+
+// room for `extra` more particles: compact first (order kept), then grow (x1.25) if short
+template <class P>
+int ensure_room_rec(pictor_ctx *c, P &cur, long long extra)
 {
     CylmState &st = c->cm;
     if (c->np + extra <= st.cap) return PICTOR_OK;
-    long long live = c->np;
+    const long long cap2 = std::max<long long>(st.cap, (long long)((c->np + extra) * 1.25) + 1024);
+    P O{};
+    int rc = alloc_parts(O, cap2);
+    if (rc) return rc;
+    long long live = 0;
     if (c->np > 0) {
-        PartF64 O{};
-        const long long cap2 = std::max<long long>(st.cap, (long long)((c->np + extra) * 1.25) + 1024);
-        int rc = alloc_parts(O, cap2);
-        if (rc) return rc;
-        HIPTRY(hipMemsetAsync(st.d_count, 0, sizeof(unsigned long long), c->stream));
-        hipLaunchKernelGGL(k_cylm_compact, blocks_of(c->np), dim3(256), 0, c->stream, c->pcyl, c->np, O, st.d_count);
-        unsigned long long h = 0;
-        HIPTRY(hipMemcpyAsync(&h, st.d_count, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-        HIPTRY(hipStreamSynchronize(c->stream));
-        free_parts(c->pcyl);
-        c->pcyl = O;
-        st.cap = cap2;
-        live = (long long)h;
-    } else {
-        free_parts(c->pcyl);
-        const long long cap2 = (long long)(extra * 1.25) + 1024;
-        int rc = alloc_parts(c->pcyl, cap2);
-        if (rc) return rc;
-        st.cap = cap2;
-        live = 0;
+        using R = typename std::conditional<std::is_same<P, PartF64>::value, RecF64, RecSplit>::type;
+        if ((rc = compact_into(c, R{cur}, c->np, R{O}, live))) { free_parts(O); return rc; }
     }
+    free_parts(cur);
+    cur = O;
+    st.cap = cap2;
     c->np = live;
     return PICTOR_OK;
+}
+
+int ensure_room(pictor_ctx *c, long long extra)
+{
+    return split_form(c) ? ensure_room_rec(c, c->cm.ps, extra) : ensure_room_rec(c, c->pcyl, extra);
 }
 
 }  // namespace
@@ -702,7 +1053,7 @@ int cylm_setup(pictor_ctx *c)
 void cylm_release(pictor_ctx *c)
 {
     CylmState &st = c->cm;
-    free_parts(c->pcyl);
+    free_particles(c);
     dfree(st.d_err); dfree(st.d_count); dfree(st.d_partial);
     st = CylmState{};
     c->np = 0;
@@ -723,12 +1074,34 @@ int cylm_upload_particles(pictor_ctx *c, long long n, const double *q, const dou
         keep.push_back(i);
     }
     const long long m = (long long)keep.size();
-    free_parts(c->pcyl);
+    free_particles(c);
     c->cm.cap = 0;
     c->np = 0;
     int rc = cylm_setup(c);
     if (rc) return rc;
     if ((rc = ensure_room(c, m))) return rc;
+    if (split_form(c)) {
+        // (cell, FP32 offset) per coordinate, theta in 2^-32 turns, FP32 momenta and weight
+        PartCylS &p = c->cm.ps;
+        std::vector<uint32_t> cw(std::max<long long>(m, 1)), tw(std::max<long long>(m, 1));
+        std::vector<float> f[6];
+        for (auto &v : f) v.resize(std::max<long long>(m, 1));
+        for (long long e = 0; e < m; e++) {
+            const long long j = keep[e];
+            int ii, kk;
+            split_coord(x[j], ii, f[0][e]);
+            split_coord(z[j], kk, f[1][e]);
+            cw[e] = pack_cell(ii, kk, flv[j]);
+            tw[e] = theta_turns(y[j]);
+            f[2][e] = (float)u[j]; f[3][e] = (float)v[j]; f[4][e] = (float)w[j]; f[5][e] = (float)q[j];
+        }
+        float *dst[6] = {p.ox, p.oz, p.u, p.v, p.w, p.q};
+        for (int a = 0; a < 6; a++) HIPTRY(hipMemcpy(dst[a], f[a].data(), m * sizeof(float), hipMemcpyHostToDevice));
+        HIPTRY(hipMemcpy(p.cell, cw.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIPTRY(hipMemcpy(p.th, tw.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice));
+        c->np = m;
+        return PICTOR_OK;
+    }
     PartF64 &p = c->pcyl;
     double *arr[7] = {p.q, p.x, p.y, p.z, p.u, p.v, p.w};
     const double *src[7] = {q, x, y, z, u, v, w};
@@ -822,7 +1195,10 @@ int cylm_push(pictor_ctx *c, bool timed)
     double *E[3], *B[3], *J[3];
     fields(c, E, B, J);
     if (timed) HIPTRY(hipEventRecord(c->ev[0], c->stream));
-    if (c->np > 0)
+    if (c->np > 0 && split_form(c))
+        hipLaunchKernelGGL(k_cylm_push_split, blocks_of(c->np), dim3(256), 0, c->stream, c->g, c->cq, E[0], E[1], E[2],
+                           B[0], B[1], B[2], J[0], J[1], J[2], c->cm.ps, c->np, c->cm.d_err);
+    else if (c->np > 0)
         hipLaunchKernelGGL(k_cylm_push, blocks_of(c->np), dim3(256), 0, c->stream, c->g, c->cq, E[0], E[1], E[2], B[0],
                            B[1], B[2], J[0], J[1], J[2], c->pcyl, c->np, c->cm.d_err);
     if (timed) HIPTRY(hipEventRecord(c->ev[1], c->stream));
@@ -837,7 +1213,10 @@ int cylm_exchange(pictor_ctx *c)
     HIPTRY(hipMemcpyAsync(&err, c->cm.d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipStreamSynchronize(c->stream));
     c->pending_swap = false;
-    if (err) return fail(PICTOR_ERR_BOUNDS, "modal cylinder: a particle reached the outer wall");
+    if (err & 1) return fail(PICTOR_ERR_BOUNDS, "modal cylinder: a particle reached the outer wall");
+    if (err & 2)
+        return fail(PICTOR_ERR_CAPACITY, "modal cylinder, split form: a current contribution above 64 charge cells "
+                                         "(the fixed-point range); use the FP64 form");
     return PICTOR_OK;
 }
 
@@ -850,8 +1229,12 @@ int cylm_inject(pictor_ctx *c, int nrp, int k0, int nzc, int ppc, unsigned long 
     if (rc) return rc;
     const long long add = (long long)nrp * nzc * ppc;
     if ((rc = ensure_room(c, add))) return rc;
-    hipLaunchKernelGGL(k_cylm_inject, blocks_of(add), dim3(256), 0, c->stream, g, c->cq, c->pcyl, c->np, nrp, k0, nzc,
-                       ppc, c->cfg.epc, seed, shift);
+    if (split_form(c))
+        hipLaunchKernelGGL(k_cylm_inject_split, blocks_of(add), dim3(256), 0, c->stream, g, c->cq, c->cm.ps, c->np, nrp,
+                           k0, nzc, ppc, c->cfg.epc, seed, shift);
+    else
+        hipLaunchKernelGGL(k_cylm_inject, blocks_of(add), dim3(256), 0, c->stream, g, c->cq, c->pcyl, c->np, nrp, k0,
+                           nzc, ppc, c->cfg.epc, seed, shift);
     HIPTRY(hipGetLastError());
     c->np += add;
     return PICTOR_OK;
@@ -890,7 +1273,9 @@ int cylm_window_after_step(pictor_ctx *c)
             hipLaunchKernelGGL(k_cylm_shift_fields, dim3(nblocks(c->g.ncell, 256), 3), dim3(256), 0, c->stream, c->g, P);
         }
         launch_fill_zero_n<double>(J, 3, c->g.ncell, c->stream);
-        if (c->np > 0)
+        if (c->np > 0 && split_form(c))
+            hipLaunchKernelGGL(k_cylm_shift_split, blocks_of(c->np), dim3(256), 0, c->stream, c->cq, c->cm.ps, c->np);
+        else if (c->np > 0)
             hipLaunchKernelGGL(k_cylm_shift_particles, blocks_of(c->np), dim3(256), 0, c->stream, c->cq, c->pcyl, c->np);
         st.win_shift += 1;
         int rc = PICTOR_OK;
@@ -918,7 +1303,12 @@ int cylm_diagnostics(pictor_ctx *c, double out[9])
     for (int b = 0; b < 1024; b++) { be += h[2 * b]; ee += h[2 * b + 1]; }
     double k1 = 0, k2 = 0, cnt = 0;
     if (c->np > 0) {
-        hipLaunchKernelGGL(k_cylm_kinetic, dim3(1024), dim3(256), 0, c->stream, c->pcyl, c->np, st.d_partial);
+        if (split_form(c))
+            hipLaunchKernelGGL(k_cylm_kinetic<RecSplit>, dim3(1024), dim3(256), 0, c->stream, RecSplit{c->cm.ps}, c->np,
+                               st.d_partial);
+        else
+            hipLaunchKernelGGL(k_cylm_kinetic<RecF64>, dim3(1024), dim3(256), 0, c->stream, RecF64{c->pcyl}, c->np,
+                               st.d_partial);
         HIPTRY(hipMemcpyAsync(h.data(), st.d_partial, 3 * 1024 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPTRY(hipStreamSynchronize(c->stream));
         for (int b = 0; b < 1024; b++) { k1 += h[3 * b]; k2 += h[3 * b + 1]; cnt += h[3 * b + 2]; }
@@ -926,5 +1316,49 @@ int cylm_diagnostics(pictor_ctx *c, double out[9])
     for (int i = 0; i < 9; i++) out[i] = 0.0;
     out[0] = be; out[1] = ee; out[2] = k1; out[3] = k2;
     (void)cnt;
+    return PICTOR_OK;
+}
+
+int cylm_download_particles(pictor_ctx *c, long long cap, double *q, double *x, double *y, double *z, double *u,
+                            double *v, double *w, int *flv, int *tile, long long *n_out)
+{
+    const long long n = c->np;
+    if (n_out) *n_out = n;
+    if (n > cap) return fail(PICTOR_ERR_ARG, "output arrays too small");
+    HIPTRY(hipStreamSynchronize(c->stream));
+    if (n == 0) return PICTOR_OK;
+    if (tile) std::fill(tile, tile + n, 0);
+    if (!split_form(c)) {
+        const PartF64 &p = c->pcyl;
+        double *hd[7] = {q, x, y, z, u, v, w};
+        double *dd[7] = {p.q, p.x, p.y, p.z, p.u, p.v, p.w};
+        for (int i = 0; i < 7; i++)
+            if (hd[i]) HIPTRY(hipMemcpy(hd[i], dd[i], n * sizeof(double), hipMemcpyDeviceToHost));
+        if (flv) HIPTRY(hipMemcpy(flv, p.flv, n * sizeof(int), hipMemcpyDeviceToHost));
+        return PICTOR_OK;
+    }
+    // the split record in the FP64 layout: x = i + ox, z = k + oz (exact), theta = turns * 2 pi / 2^32
+    const PartCylS &p = c->cm.ps;
+    std::vector<uint32_t> cw(n), tw(n);
+    std::vector<float> f[6];
+    const float *src[6] = {p.ox, p.oz, p.u, p.v, p.w, p.q};
+    for (int a = 0; a < 6; a++) {
+        f[a].resize(n);
+        HIPTRY(hipMemcpy(f[a].data(), src[a], n * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    HIPTRY(hipMemcpy(cw.data(), p.cell, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPTRY(hipMemcpy(tw.data(), p.th, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (long long e = 0; e < n; e++) {
+        const uint32_t wd = cw[e];
+        const bool live = wd != 0u;
+        if (x) x[e] = live ? cell_i(wd) + (double)f[0][e] : 0.0;
+        if (z) z[e] = live ? cell_k(wd) + (double)f[1][e] : 0.0;
+        if (y) y[e] = (double)tw[e] * kTurnRad;
+        if (u) u[e] = f[2][e];
+        if (v) v[e] = f[3][e];
+        if (w) w[e] = f[4][e];
+        if (q) q[e] = f[5][e];
+        if (flv) flv[e] = live ? cell_flv(wd) : 0;
+    }
     return PICTOR_OK;
 }

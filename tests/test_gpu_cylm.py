@@ -13,7 +13,13 @@ Bars:
   * the Lehe B update: bit for bit with the oracle; a pulse at k0 = 1 rad per cell
     moves at the Lehe group speed (2 %), 15 % faster than the Yee one;
   * a Courant number above the limit of the m = 1 terms next to the axis (and
-    above the Lehe rule) is rejected.
+    above the Lehe rule) is rejected;
+  * the split form: the loader gives the FP64 loader's particles rounded to the
+    split record; one push from the same (rounded) state matches the oracle to
+    4e-6 (positions, momenta) and 2e-5 (J_r, J_z; 1e-4 for J_theta); 20 steps stay within 1e-5 of
+    the FP64 form; runs are bitwise deterministic, the moving window included;
+    the modal Gauss law holds to 1e-6; a contribution beyond the fixed-point
+    range is an error.
 """
 import math
 
@@ -268,3 +274,164 @@ def test_courant_below_the_lehe_rule_is_accepted():
     # sin^2(0.42 pi) + 1.25 * 0.84^2 / 16 = 0.994 < 1; Yee accepts 0.85 as well
     Simulation(16, 2, 16, ndim=3, epc=1, c=0.84, form="fp64", geometry="cyl_modal", dr=4.0, fdtd="lehe").close()
     Simulation(16, 2, 16, ndim=3, epc=1, c=0.85, form="fp64", geometry="cyl_modal", dr=4.0, fdtd="yee").close()
+
+
+# ---- the split form: (cell, FP32 offset) positions, theta in 2^-32 turns, FP32 push,
+# fixed-point current (2^-36 counts summed exactly in FP64) ----
+
+TURN = 2.0 * math.pi / 2.0 ** 32
+
+
+def _split_sim(nr, nz, dr, c, epc=4, fdtd="yee"):
+    return Simulation(nr, 2, nz, ndim=3, epc=epc, c=c, c_ompe=10.0, form="fp32_split", geometry="cyl_modal", dr=dr,
+                      mi_me=1e30, fdtd=fdtd)
+
+
+def test_split_loader_is_the_fp64_loader_rounded():
+    say("[case] split loader")
+    o = CylModalOracle(24, 16, dr=1.5, c=0.3, c_ompe=10.0, epc=5, cap=4096)
+    B = o.load_plasma(12, 5, 9, 8, seed=77)
+    with _split_sim(24, 16, 1.5, 0.3, epc=5) as sim:
+        sim.cyl_plasma(12, 5, 9, 8, seed=77)
+        A = sim.download_particles()
+    assert np.array_equal(A["flv"], B["flv"])
+    for k in ("x", "z"):   # the cell exact, the offset rounded to FP32
+        assert np.array_equal(np.floor(A[k]), np.floor(B[k])), k
+        assert np.abs(A[k] - B[k]).max() <= 2.0 ** -24, k
+    assert np.abs(A["y"] - B["y"]).max() <= TURN
+    np.testing.assert_allclose(A["q"], B["q"], rtol=2.0 ** -24, atol=0)
+
+
+def _split_state(o, rng, spread=0.1):
+    """a thermal plasma in the oracle, rounded to what the split record holds"""
+    P = _thermal(o, rng, spread)
+    P["u"], P["v"], P["w"] = (P[k].astype(np.float32).astype(np.float64) for k in "uvw")
+    P["q"] = P["q"].astype(np.float32).astype(np.float64)
+    for k in ("x", "z"):
+        cell = np.floor(P[k])
+        P[k] = cell + (P[k] - cell).astype(np.float32).astype(np.float64)
+    P["y"] = np.round(P["y"] / TURN) % 2.0 ** 32 * TURN
+    return P
+
+
+def test_split_push_matches_oracle():
+    o = CylModalOracle(24, 24, dr=2.0, c=0.5, c_ompe=10.0, epc=4, cap=8192)
+    rng = np.random.default_rng(15)
+    o.load_plasma(14, 3, 24, 8, seed=5, cap=8192)
+    P = _split_state(o, rng)
+    F = _smooth_fields(o, rng, amp=5e-3)
+    with _split_sim(24, 24, 2.0, 0.5) as sim:
+        sim.upload_particles(**P)
+        A0 = sim.download_particles()
+        for k in ("x", "y", "z", "u", "v", "w", "q"):   # the upload holds the rounded state exactly
+            assert np.array_equal(A0[k], P[k]), k
+        o.load_particles(P["q"], P["x"], P["y"], P["z"], P["u"], P["v"], P["w"], P["flv"], cap=8192)
+        o.set_fields(**F)
+        sim.upload_fields(*[np.asfortranarray(F[n]) for n in FIELDS])
+        sim.ResetCurrent(); o.reset_current()
+        sim.MoveDepositPrtl(); sim.ExchangePrtl(); o.move_deposit()
+        A, B = sim.download_particles(), o.particles()
+        G = sim.download_fields()
+    assert len(A["x"]) == len(B["x"])
+    err = {}
+    for k in ("x", "z"):
+        err[k] = np.abs(A[k] - B[k]).max()
+    err["y"] = np.abs((A["y"] - B["y"] + math.pi) % (2 * math.pi) - math.pi).max()
+    for k in "uvw":
+        err[k] = np.abs(A[k] - B[k]).max() / np.abs(B[k]).max()
+    for k in ("Jx", "Jy", "Jz"):
+        err[k] = np.abs(G[k] - o.F[k]).max() / np.abs(o.F[k]).max()
+    say("split vs oracle after one push: " + ", ".join(f"{k} {v:.2e}" for k, v in err.items()))
+    for k in ("x", "z"):
+        assert err[k] <= 4e-6, k          # cells: FP32 offsets and displacement
+    assert err["y"] <= 4e-6               # radians
+    for k in "uvw":
+        assert err[k] <= 4e-6, k
+    for k in ("Jx", "Jz"):
+        assert err[k] <= 2e-5, k
+    # the m = 1 angular current q (sin t1 - sin t0), -q (cos t1 - cos t0) is taken from the
+    # FP32 phases of the stored angles (the phases the charge density uses, so continuity
+    # stays exact); their difference loses digits for small angular moves
+    assert err["Jy"] <= 1e-4
+
+
+def test_split_steps_stay_close_to_fp64():
+    say("[case] split vs FP64, 20 steps")
+    o = CylModalOracle(24, 32, dr=2.0, c=0.5, c_ompe=10.0, epc=4, cap=20000)
+    rng = np.random.default_rng(21)
+    o.load_plasma(16, 3, 32, 8, seed=9, cap=20000)
+    P = _split_state(o, rng, 0.05)
+    F = _smooth_fields(o, rng, amp=5e-3)
+    out = []
+    for form in ("fp64", "fp32_split"):
+        with Simulation(24, 2, 32, ndim=3, epc=4, c=0.5, c_ompe=10.0, form=form, geometry="cyl_modal", dr=2.0,
+                        mi_me=1e30, fdtd="lehe") as sim:
+            sim.upload_particles(**P)
+            sim.upload_fields(*[np.asfortranarray(F[n]) for n in FIELDS])
+            sim.step(20)
+            out.append((sim.download_fields(), sim.diagnostics()))
+    for k in FIELDS:
+        d = np.abs(out[1][0][k] - out[0][0][k]).max() / np.abs(out[0][0][k]).max()
+        say(f"{k}: split vs FP64 {d:.2e}")
+        assert d <= 1e-5, k
+    for k in ("E_energy", "B_energy", "prtl_energy_2"):
+        assert abs(out[1][1][k] - out[0][1][k]) <= 1e-6 * out[0][1][k], k
+
+
+def _window_run(form, nsteps=16):
+    with Simulation(20, 2, 24, ndim=3, epc=4, c=0.45, c_ompe=10.0, form=form, geometry="cyl_modal", dr=2.0,
+                    mi_me=1e30, fdtd="lehe") as sim:
+        sim.cyl_plasma(10, 3, 24, 4, seed=13)
+        sim.cyl_laser(0.5, 3.0, 0.8, 2.0, 14.0)
+        sim.moving_window(True, nr_plasma=10, ppc=4, seed=13)
+        sim.step(nsteps)
+        return sim.download_fields(), sim.download_particles(), sim.diagnostics()
+
+
+def test_split_is_deterministic_and_follows_fp64_with_the_window():
+    say("[case] split determinism with the moving window")
+    F1, P1, d1 = _window_run("fp32_split")
+    F2, P2, d2 = _window_run("fp32_split")
+    for k in FIELDS + ("Jx", "Jy", "Jz"):
+        assert np.array_equal(F1[k], F2[k]), k
+    for k in P1:
+        assert np.array_equal(P1[k], P2[k]), k
+    F0, P0, d0 = _window_run("fp64")
+    assert len(P0["x"]) == len(P1["x"]) and np.array_equal(P0["flv"], P1["flv"])
+    for k in FIELDS:
+        d = np.abs(F1[k] - F0[k]).max() / np.abs(F0[k]).max()
+        say(f"{k}: split vs FP64 with the window {d:.2e}")
+        assert d <= 1e-5, k
+
+
+def test_split_run_keeps_modal_gauss_law():
+    say("[case] split Gauss law")
+    o = CylModalOracle(32, 40, dr=2.0, c=0.5, c_ompe=10.0, epc=4, cap=60000)
+    rng = np.random.default_rng(2)
+    o.load_plasma(20, 3, 40, 8, seed=2, cap=60000)
+    P = _split_state(o, rng, 0.1)
+    o.load_particles(P["q"], P["x"], P["y"], P["z"], P["u"], P["v"], P["w"], P["flv"], cap=60000)
+    G0 = o.gauss()
+    with _split_sim(32, 40, 2.0, 0.5) as sim:
+        sim.upload_particles(**P)
+        sim.step(40)
+        F = sim.download_fields()
+        A = sim.download_particles()
+    o.set_fields(**{k: F[k] for k in FIELDS})
+    o.load_particles(A["q"], A["x"], A["y"], A["z"], A["u"], A["v"], A["w"], A["flv"], cap=60000)
+    G1 = o.gauss()
+    for s in range(3):
+        d = np.abs(G1 - G0)[:, s, :].max() / np.abs(G0[:, s, :]).max()
+        say(f"slot {s}: Gauss change / max {d:.2e}")
+        assert d <= 1e-6, s
+
+
+def test_split_contribution_above_the_fixed_point_range_is_an_error():
+    o = CylModalOracle(16, 16, dr=1.0, c=0.3, c_ompe=10.0, epc=4, cap=64)
+    with _split_sim(16, 16, 1.0, 0.3) as sim:
+        # one particle whose weight makes q * qi * dx far above 64 charge cells
+        big = -1e9 / o.m.qi
+        sim.upload_particles(q=np.array([big]), x=np.array([8.2]), y=np.array([1.0]), z=np.array([8.5]),
+                             u=np.array([0.9]), v=np.zeros(1), w=np.zeros(1), flv=np.array([2], dtype=np.int32))
+        with pytest.raises(RuntimeError, match="fixed-point"):
+            sim.step(1)
