@@ -524,6 +524,55 @@ def lwfa_report(args):
     return out
 
 
+def lwfa_accuracy_report(args):
+    """The split form against the FP64 form on the laser wakefield (lwfa_1024x128,
+    Lehe, no window, so that the Gauss law can be followed): field energies every
+    `every` steps over `steps` steps, and per mode the change of the discrete Gauss
+    residual (flux of E minus the node charge, the modal oracle's definition) from
+    the start to the end, relative to its largest value."""
+    import numpy as np
+
+    from oracle.pyoracle import CylModalOracle
+    from pictor_amd import Simulation
+
+    name = "lwfa_1024x128"
+    nr, nz, dr, cps, nrp, ppc, c = LWFA_CONFIGS[name]
+    a0, omega0, tau, w0 = 2.0, (1.0 / 0.005) ** 0.5, 10.0, 15.0
+    steps, every = args.steps, args.every
+    series, gauss = {}, {}
+    for form in ("fp64", "fp32_split"):
+        with Simulation(nr, 2, nz, ndim=3, epc=ppc, c=c, c_ompe=cps, mi_me=1e30, form=form, geometry="cyl_modal",
+                        dr=dr, fdtd=args.lwfa_fdtd) as sim:
+            sim.cyl_plasma(nrp, 3, nz, ppc, seed=2024)
+            P0 = sim.download_particles()
+            sim.cyl_laser(a0, omega0, tau, w0, nz / 2.0)
+            F0 = sim.download_fields()
+            rows = []
+            for t in range(0, steps + 1, every):
+                if t:
+                    sim.step(every)
+                d = sim.diagnostics()
+                rows.append((t, d["E_energy"], d["B_energy"], d["prtl_energy_2"]))
+                print(f"# {form} step {t}: E {d['E_energy']:.9e} B {d['B_energy']:.9e}", file=sys.stderr, flush=True)
+            F1 = sim.download_fields()
+            P1 = sim.download_particles()
+        series[form] = rows
+        cap = len(P0["x"]) + 16
+        o = CylModalOracle(nr, nz, dr=dr, c=c, c_ompe=cps, epc=ppc, cap=cap)
+        res = []
+        for F, P in ((F0, P0), (F1, P1)):
+            o.set_fields(**{k: F[k] for k in ("Ex", "Ey", "Ez", "Bx", "By", "Bz")})
+            o.load_particles(P["q"], P["x"], P["y"], P["z"], P["u"], P["v"], P["w"], P["flv"], cap=cap)
+            res.append(o.gauss())
+        gauss[form] = [float(np.abs(res[1] - res[0])[:, s, :].max() / np.abs(res[0][:, s, :]).max()) for s in range(3)]
+    a, b = series["fp64"], series["fp32_split"]
+    rel = {k: max(abs(y[i] - x[i]) / abs(x[i]) for x, y in zip(a, b) if x[i] != 0) for k, i in
+           (("E_energy", 1), ("B_energy", 2), ("prtl_energy_2", 3))}
+    return {"report": "lwfa accuracy: split vs FP64", "config": name, "steps": steps, "every": every,
+            "fdtd": args.lwfa_fdtd, "max_rel_diff": rel, "gauss_change_per_mode": gauss,
+            "series": {f: [list(r) for r in series[f]] for f in series}}
+
+
 def atomics_report(iters=4096):
     """Atomic throughput behind the accumulator choices: the split push's current
     window (LDS FP64), the adjoint's field-adjoint window (LDS FP64, FP32 only for
@@ -573,6 +622,8 @@ def main():
     ap.add_argument("--lwfa", default=None, choices=sorted(LWFA_CONFIGS),
                     help="instead of the Weibel step: time the laser wakefield in the modal cylinder")
     ap.add_argument("--lwfa-fdtd", default="lehe", choices=("lehe", "yee"))
+    ap.add_argument("--lwfa-accuracy", action="store_true",
+                    help="split vs FP64 on lwfa_1024x128 (no window): energies every --every steps, Gauss change")
     ap.add_argument("--atomics", action="store_true",
                     help="instead of the Weibel step: LDS versus global atomic throughput (pictor_probe_atomics)")
     args = ap.parse_args()
@@ -590,6 +641,9 @@ def main():
         return
     if args.atomics:
         print(json.dumps(atomics_report()), flush=True)
+        return
+    if args.lwfa_accuracy:
+        print(json.dumps(lwfa_accuracy_report(args)), flush=True)
         return
     if args.lwfa:
         print(json.dumps(lwfa_report(args)), flush=True)
