@@ -1333,6 +1333,31 @@ void orc_cyl_update_b_half(const orc_params *p, const orc_cyl *cy, orc_fields *f
     cyl_b_axis(p, f);
 }
 
+/* CalcEnergy with -Dcyl (savedata_routines.F90:265-281): CalcFldEnergy_cyl
+ * (cyl/cyl_savedata_routines.F90:8-31, the axis rank: i from 4, r = (i - 3) + rmin
+ * on each component's stagger, times dtheta), then CalcPrtlEnergy as orc_energy.
+ * out: B energy, E energy, flavour 1, flavour 2. */
+void orc_cyl_energy(const orc_params *p, const orc_cyl *cy, const orc_fields *f, const orc_particles *s, double out[4])
+{
+    double be = 0.0, ee = 0.0;
+    const int k0 = p->ndim == 3 ? 3 : 1, k1 = p->ndim == 3 ? p->mz - 3 : 1;
+    for (int k = k0; k <= k1; k++)
+        for (int j = 3; j <= p->my - 3; j++)
+            for (int i = 4; i <= p->mx - 3; i++) {
+                const int64_t o = IX(p, i, j, k);
+                const double r = (i - 3.0) + cy->rmin;
+                const double rp = r + 0.5;
+                be = be + (r * (f->Bx[o] * f->Bx[o]) + rp * (f->By[o] * f->By[o]) + rp * (f->Bz[o] * f->Bz[o])) * cy->dtheta;
+                ee = ee + (rp * (f->Ex[o] * f->Ex[o]) + r * (f->Ey[o] * f->Ey[o]) + r * (f->Ez[o] * f->Ez[o])) * cy->dtheta;
+            }
+    double tmp[4];
+    orc_energy(p, f, s, tmp);   /* its particle part is CalcPrtlEnergy */
+    out[0] = be * 0.5;
+    out[1] = ee * 0.5;
+    out[2] = tmp[2];
+    out[3] = tmp[3];
+}
+
 /* AddCurrent, cyl_em_update.F90:114-140 (RZtwoD = .false.) */
 void orc_cyl_add_current(const orc_params *p, const orc_cyl *cy, orc_fields *f)
 {

@@ -126,6 +126,7 @@ void orc_cyl_field_halo(const orc_params *p, double *fx, double *fy, double *fz)
 void orc_cyl_current_halo_add(const orc_params *p, double *jx, double *jy, double *jz);
 int  orc_cyl_move_deposit(const orc_params *p, orc_cyl *cy, orc_ws *w, orc_fields *f, orc_particles *s);
 void orc_cyl_exchange_particles(const orc_params *p, orc_particles *s);
+void orc_cyl_energy(const orc_params *p, const orc_cyl *cy, const orc_fields *f, const orc_particles *s, double out[4]);
 int  orc_cyl_step(const orc_params *p, orc_cyl *cy, orc_ws *w, orc_fields *f, orc_particles *s);
 
 /* ---- azimuthal-mode cylindrical form (m = 0, 1), pictor_oracle_cylm.c ----

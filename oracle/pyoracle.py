@@ -97,6 +97,7 @@ def lib():
         L.orc_cyl_field_halo.argtypes = [P, c_dp, c_dp, c_dp]
         L.orc_cyl_current_halo_add.argtypes = [P, c_dp, c_dp, c_dp]
         L.orc_cyl_move_deposit.argtypes = [P, C, W, F, S]
+        L.orc_cyl_energy.argtypes = [P, C, F, S, c_dp]
         L.orc_cyl_exchange_particles.argtypes = [P, S]
         L.orc_cyl_step.argtypes = [P, C, W, F, S]
         _lib = L
@@ -273,6 +274,13 @@ class CylOracle(Oracle):
             raise ValueError("the cylindrical grid needs an even number of theta cells")
         self.p.fdtd = {"yee": 0, "lehe": 1}[fdtd]
         self.cy.second_half_new_cell = 1 if second_half_new_cell else 0
+
+    def energy(self):
+        """CalcEnergy with -Dcyl: B energy, E energy (radius-weighted), flavour 1, flavour 2"""
+        out = np.zeros(4)
+        lib().orc_cyl_energy(ctypes.byref(self.p), ctypes.byref(self.cy), ctypes.byref(self.f), ctypes.byref(self.s),
+                             _ptr(out))
+        return out
 
     def update_b_half(self):
         lib().orc_cyl_update_b_half(ctypes.byref(self.p), ctypes.byref(self.cy), ctypes.byref(self.f))

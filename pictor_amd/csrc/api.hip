@@ -1007,7 +1007,7 @@ int pictor_diagnostics(pictor_ctx *c, double out[9])
     if (rc) return rc;
     if (c->pending_swap) return fail(PICTOR_ERR_ARG, "diagnostics between move_deposit and exchange_particles");
     if (c->cylm) return cylm_diagnostics(c, out);
-    if (c->cyl) return fail(PICTOR_ERR_ARG, "pictor_diagnostics is available for the Cartesian and modal geometries");
+    if (c->cyl) return cyl_diagnostics(c, out);
     const int nblk = 1024;
     std::vector<double> part((size_t)std::max(nblk * 2, c->tp.ntiles * 5));
     // field energies

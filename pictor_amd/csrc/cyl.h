@@ -29,6 +29,7 @@ int cyl_setup(pictor_ctx *c);
 void cyl_release(pictor_ctx *c);
 int cyl_upload_particles(pictor_ctx *c, long long n, const double *q, const double *x, const double *y,
                          const double *z, const double *u, const double *v, const double *w, const int *flv);
+int cyl_diagnostics(pictor_ctx *c, double out[9]);
 int cyl_download_particles(pictor_ctx *c, long long cap, double *q, double *x, double *y, double *z, double *u,
                            double *v, double *w, int *flv, int *tile, long long *n_out);
 int cyl_b_half(pictor_ctx *c);

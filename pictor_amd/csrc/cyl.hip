@@ -599,6 +599,52 @@ __global__ void k_cyl_wrap(GridP g, PhysP ph, PartF64 P, long long n)
     }
 }
 
+// CalcEnergy with -Dcyl (savedata_routines.F90:265-281): CalcFldEnergy_cyl
+// (cyl/cyl_savedata_routines.F90:8-31: every component weighted by the radius of its
+// stagger, r = (i - 3) + rmin or r + 1/2, times dtheta, i from 4 on the axis rank) and
+// CalcPrtlEnergy (savedata_routines.F90:283-308).  Per block: B, E, flavour 1, flavour 2.
+__global__ __launch_bounds__(256) void k_cyl_energy(GridP g, PhysP ph, CylP cp, const double *__restrict__ Ex,
+                                                    const double *__restrict__ Ey, const double *__restrict__ Ez,
+                                                    const double *__restrict__ Bx, const double *__restrict__ By,
+                                                    const double *__restrict__ Bz, PartF64 P, long long n,
+                                                    double *partial)
+{
+    __shared__ double sred[4][256];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int ni = g.mx - 6, nj = g.my - 5, nk = g.ndim == 3 ? g.mz - 5 : 1, k0 = g.ndim == 3 ? 3 : 1;
+    const long long ncell = (long long)ni * nj * nk;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += stride) {
+        const int i = 4 + (int)(t % ni);
+        const long long r0 = t / ni;
+        const int j = 3 + (int)(r0 % nj), k = k0 + (int)(r0 / nj);
+        const long long o = cx(g, i, j, k);
+        const double r = (i - 3.0) + cp.rmin, rp = r + 0.5;
+        acc[0] += (r * (Bx[o] * Bx[o]) + rp * (By[o] * By[o]) + rp * (Bz[o] * Bz[o])) * cp.dtheta;
+        acc[1] += (rp * (Ex[o] * Ex[o]) + r * (Ey[o] * Ey[o]) + r * (Ez[o] * Ez[o])) * cp.dtheta;
+    }
+    const double c2 = ph.c * ph.c;
+    for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += stride) {
+        const int flv = P.flv[m];
+        if (flv < 1 || flv > 2) continue;
+        const double u = P.u[m], v = P.v[m], w = P.w[m], q = P.q[m];
+        const double gamma = sqrt(1.0 + u * u + v * v + w * w);
+        double e = 0.0;
+        if (q > 0) e = (gamma - 1.0) * 1.0 * (q * ph.qi / ph.flvrqm[flv - 1]) * c2;
+        if (q < 0) e = (gamma - 1.0) * 1.0 * (ph.mass[1] * c2);
+        acc[1 + flv] += e;
+    }
+    for (int a = 0; a < 4; a++) sred[a][threadIdx.x] = acc[a];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int a = 0; a < 4; a++) sred[a][threadIdx.x] += sred[a][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 4; a++) partial[4 * blockIdx.x + a] = sred[a][0];
+}
+
 }  // namespace
 }  // namespace pictor
 
@@ -826,5 +872,28 @@ int cyl_reset_current(pictor_ctx *c)
 {
     for (int i = 0; i < 3; i++) launch_fill_zero<double>((double *)c->J[i], c->g.ncell, c->stream);
     HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
+int cyl_diagnostics(pictor_ctx *c, double out[9])
+{
+    double *E[3], *B[3], *J[3];
+    fields(c, E, B, J);
+    const int nb = 1024;
+    if (c->partial_len < 4 * nb) return fail(PICTOR_ERR_HIP, "diagnostic buffer too small");
+    hipLaunchKernelGGL(k_cyl_energy, dim3(nb), dim3(256), 0, c->stream, c->g, c->ph, c->cp, E[0], E[1], E[2], B[0], B[1],
+                       B[2], c->pcyl, c->np, c->d_partial);
+    HIPTRY(hipGetLastError());
+    std::vector<double> h(4 * nb);
+    HIPTRY(hipMemcpyAsync(h.data(), c->d_partial, 4 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < 4; k++) a[k] += h[4 * b + k];
+    for (int i = 0; i < 9; i++) out[i] = 0.0;
+    out[0] = 0.5 * a[0];
+    out[1] = 0.5 * a[1];
+    out[2] = a[2];
+    out[3] = a[3];
     return PICTOR_OK;
 }

@@ -127,6 +127,14 @@ def test_cyl_steps_match_oracle(ndim):
     for k in ("x", "y", "z", "u", "v", "w"):
         d = np.abs(A[k] - B[k]).max() / max(np.abs(B[k]).max(), 1.0)
         assert d <= 1e-9, k
+    # CalcEnergy with -Dcyl on the GPU state equals the oracle's on the same state
+    dg = sim.diagnostics()
+    o.set_fields(**{k: G[k] for k in FIELDS})
+    o.load_particles(A["q"], A["x"], A["y"], A["z"], A["u"], A["v"], A["w"], A["flv"])
+    eo = o.energy()
+    for i, key in enumerate(("B_energy", "E_energy", "prtl_energy_1", "prtl_energy_2")):
+        say(f"{key}: GPU {dg[key]:.12e} oracle {eo[i]:.12e}")
+        assert eo[i] > 0 and abs(dg[key] - eo[i]) <= 1e-12 * eo[i], key
     sim.close()
 
 

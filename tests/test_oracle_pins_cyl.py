@@ -21,11 +21,15 @@ wrong index would break:
 * axis fills (:185-260): a uniform Cartesian field is continued across the
   axis exactly.
 """
+import math
+
 import numpy as np
 import pytest
 
 from oracle.pyoracle import CylOracle
 from tests._cyl_util import gauss as _gauss, interior as _interior, plasma as _plasma
+
+FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
 
 
 def _cart(o, x, y):
@@ -266,3 +270,25 @@ def test_axis_fill_continues_a_uniform_cartesian_field():
     assert np.allclose(o.F["Bx"][2, :, 0], B0 * np.cos(th_h), atol=1e-14, rtol=0)
     assert np.allclose(o.F["By"][2, :, 0], -B0 * np.sin(th_j), atol=1e-14, rtol=0)
     assert np.allclose(o.F["By"][1, :, 0], -B0 * np.sin(th_j), atol=1e-14, rtol=0)
+
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_cyl_energy_weights_each_component_by_its_stagger_radius(ndim):
+    # CalcFldEnergy_cyl: a uniform component F0 over the grid gives
+    # 1/2 F0^2 dtheta ny nz_phys * sum of its stagger radius over i = 4..mx-3:
+    # r = i - 3.5 (B_r, E_theta, E_z): (nx - 1)^2 / 2;  r + 1/2 (E_r, B_theta, B_z): nx (nx - 1) / 2
+    nx, ny, nz = 12, 8, 6
+    o = CylOracle(ndim, nx, ny, nz if ndim == 3 else 1, c=0.2, epc=1, cap=16)
+    nzp = nz if ndim == 3 else 1
+    sum_r, sum_rp = (nx - 1) ** 2 / 2.0, nx * (nx - 1) / 2.0
+    dth = 2.0 * math.pi / ny
+    for comp, slot, radial in (("Ex", 1, sum_rp), ("Ey", 1, sum_r), ("Ez", 1, sum_r),
+                               ("Bx", 0, sum_r), ("By", 0, sum_rp), ("Bz", 0, sum_rp)):
+        for n in FIELDS:
+            o.F[n][...] = 0.0
+        o.F[comp][...] = 3.0
+        e = o.energy()
+        want = 0.5 * 9.0 * dth * ny * nzp * radial
+        assert abs(e[slot] - want) <= 1e-12 * want, comp
+        assert e[1 - slot] == 0.0, comp
