@@ -191,6 +191,16 @@ int pictor_adjoint_stats(pictor_ctx *ctx, double out[10]);
 int pictor_adjoint_engine(pictor_ctx *ctx, int32_t engine);
 int pictor_adjoint_release(pictor_ctx *ctx);
 
+/* Push scheme of the cylindrical grid (PICTOR_GEOM_CYL): 0 = Boris, MoveDepositPrtl of
+ * src/cyl/cyl_movdep.F90:14-465 (the algorithm of the reference's GPU kernel,
+ * cyl_movdep_gpu.F90:69-340; the default); 1 = Strang splitting, MoveDepositPrtl of
+ * src/cyl/cyl_movdep_strang.F90:14-584, which the reference's CPU -Dcyl build links
+ * (Makefile:104): half kick, move, second gather at the new point, half kick, axis
+ * sub-steps growing by 1.7.  Its main loop as written keeps the first gather's scaled
+ * fields in the second gather's sums and moves z before the kick; this follows its
+ * MovePrtlAxis, which does neither. */
+int pictor_cyl_push_scheme(pictor_ctx *ctx, int32_t scheme);
+
 /* ---- the azimuthal-mode cylinder (PICTOR_GEOM_CYL_MODAL; no counterpart in
  * the reference, which has no modes, laser or moving window) ----
  * Cold electrons of uniform density (weight epc per unit volume): ppc per cell

@@ -1554,6 +1554,71 @@ static int cyl_move_axis(const orc_params *p, const orc_cyl *cy, const orc_ws *w
     return 1;
 }
 
+/* Boris with the half-kick factors pE, pB (already times q/m and dt) on the
+ * c-scaled momentum (u0, v0, w0), in place (cyl_movdep_strang.F90:172-185);
+ * returns the factor 1 / (c gamma) it scaled pB by */
+static double boris_cscaled(const orc_params *p, double pEx, double pEy, double pEz, double pBx, double pBy, double pBz,
+                          double *u0, double *v0, double *w0)
+{
+    const double sqc = p->sqc;
+    double U = *u0 + pEx, V = *v0 + pEy, W = *w0 + pEz;
+    const double g = 1.0 / sqrt(sqc + U * U + V * V + W * W);
+    pBx = g * pBx; pBy = g * pBy; pBz = g * pBz;
+    const double f = 2.0 / (1.0 + pBx * pBx + pBy * pBy + pBz * pBz);
+    const double u1 = (U + V * pBz - W * pBy) * f;
+    const double v1 = (V + W * pBx - U * pBz) * f;
+    const double w1 = (W + U * pBy - V * pBx) * f;
+    *u0 = U + v1 * pBz - w1 * pBy + pEx;
+    *v0 = V + w1 * pBx - u1 * pBz + pEy;
+    *w0 = W + u1 * pBy - v1 * pBx + pEz;
+    return g;
+}
+
+/* MovePrtlAxis, cyl_movdep_strang.F90:462-566: half kick with the fields at the
+ * start, the straight move of length dt, the rotation of (u, v) into the new local
+ * frame, half kick with the fields at the new point.  push = 2 keeps the written
+ * x1 <= 0 branch, which leaves (u, v) divided by c before the second kick. */
+static int cyl_move_axis_strang(const orc_params *p, const orc_cyl *cy, const orc_ws *w, orc_particles *s, int64_t n,
+                                double x, double y, double z, double dt)
+{
+    double e[3], b[3];
+    cyl_interp(p, w, x, y, z, e, b);
+    const double qm = (s->flv[n] == 0) ? 0.0 : p->flvrqm[s->flv[n] - 1] * 0.25;
+    const double c = p->c, sqc = p->sqc, cinv = p->cinv;
+    double u0 = c * s->u[n], v0 = c * s->v[n], w0 = c * s->w[n];
+    boris_cscaled(p, e[0] * qm * dt, e[1] * qm * dt, e[2] * qm * dt, b[0] * qm * dt, b[1] * qm * dt, b[2] * qm * dt,
+                  &u0, &v0, &w0);
+    double u1 = u0 * cinv, v1 = v0 * cinv;
+    const double w1 = w0 * cinv;
+    const double g = sqc / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+    const double r0 = (x - 3.5) * 1.0;
+    const double x1 = r0 + u1 * g * dt;
+    const double y1 = v1 * g * dt;
+    const double r1 = sqrt(x1 * x1 + y1 * y1);
+    double dy;
+    if (x1 > 0) dy = atan(y1 / x1) * cy->inv_dtheta;
+    else dy = 0.0;
+    if (fabs(dy) > 0.45) return 0;
+    if (x1 > 0) {
+        s->x[n] = r1 / p->inv_dx + 3.5;
+        u1 = (r0 * u0 + (u0 * u1 * g * dt + v0 * y1)) / r1;
+        v1 = (v0 * r0) / r1;
+    } else {
+        s->x[n] = x1 / p->inv_dx + 3.5;
+        if (cy->push != 2) { u1 = u0; v1 = v0; }
+    }
+    s->y[n] = s->y[n] + dy;
+    s->z[n] = s->z[n] + w1 * g * dt * p->inv_dz;
+    cyl_interp(p, w, s->x[n], s->y[n], s->z[n], e, b);
+    u0 = u1; v0 = v1;
+    boris_cscaled(p, e[0] * qm * dt, e[1] * qm * dt, e[2] * qm * dt, b[0] * qm * dt, b[1] * qm * dt, b[2] * qm * dt,
+                  &u0, &v0, &w0);
+    s->u[n] = u0 * cinv;
+    s->v[n] = v0 * cinv;
+    s->w[n] = w0 * cinv;
+    return 1;
+}
+
 /* PrtlAxisBC, cyl_movdep.F90:366-379 */
 static void cyl_axis_bc(const orc_params *p, orc_particles *s, int64_t n)
 {
@@ -1580,12 +1645,13 @@ static int64_t cyl_substep(const orc_params *p, const orc_cyl *cy, const orc_ws 
         if (++tries > cy->substep_limit) return -1;
         double delt = dt;
         if (t + dt > 1.0) delt = 1.0 - t;
-        const int res = cyl_move_axis(p, cy, w, s, n, x0, y0, z0, delt);
+        const int res = cy->push ? cyl_move_axis_strang(p, cy, w, s, n, x0, y0, z0, delt)
+                                 : cyl_move_axis(p, cy, w, s, n, x0, y0, z0, delt);
         if (res == 1) {
             t = t + delt;
             cyl_deposit_pic(p, f, x0, y0, z0, s->x[n], s->y[n], s->z[n], s->q[n]);
             cyl_axis_bc(p, s, n);
-            dt = dt * 1.1;
+            dt = dt * (cy->push ? 1.7 : 1.1);   /* cyl_movdep_strang.F90:427 */
             x0 = s->x[n]; y0 = s->y[n]; z0 = s->z[n];
         } else {
             dt = dt * 0.5;
@@ -1630,7 +1696,54 @@ int orc_cyl_move_deposit(const orc_params *p, orc_cyl *cy, orc_ws *w, orc_fields
             qthis[n] = s->q[m] * qi;
             ui[n] = s->u[m]; vi[n] = s->v[m]; wi[n] = s->w[m];
         }
-        for (int n = 0; n < nb; n++) {
+        for (int n = 0; n < nb && cy->push; n++) {
+            /* MoveDepositPrtl of cyl_movdep_strang.F90:95-294 (Strang splitting, Delzanno
+             * et al. 2013): half kick with the fields at the old point (q/m / 4), the move
+             * with the half-kicked momentum, the rotation into the new local frame, half
+             * kick with the fields at the new point.  As written (push = 2) the second
+             * gather adds onto the first one's scaled fields and the z move uses the
+             * momentum before the kick; push = 1 follows MovePrtlAxis of the same file,
+             * which resets the gather and moves z with the half-kicked momentum. */
+            const int64_t m = off + n;
+            const double qm = (s->flv[m] == 0) ? 0.0 : p->flvrqm[s->flv[m] - 1] * 0.25;
+            double e[3], b[3];
+            ip0[n] = (int)x0[n];
+            jp0[n] = (int)y0[n];
+            kp0[n] = two ? 1 : (int)z0[n];
+            cyl_interp(p, w, x0[n], y0[n], z0[n], e, b);
+            const double pE1[3] = {e[0] * qm, e[1] * qm, e[2] * qm};
+            const double pB1[3] = {b[0] * qm, b[1] * qm, b[2] * qm};
+            double u0 = c * s->u[m], v0 = c * s->v[m], w0 = c * s->w[m];
+            const double g1 = boris_cscaled(p, pE1[0], pE1[1], pE1[2], pB1[0], pB1[1], pB1[2], &u0, &v0, &w0);
+            double u1 = u0 * cinv;
+            const double v1 = v0 * cinv, w1 = w0 * cinv;
+            const double g = sqc / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+            const double r0 = (s->x[m] + r_proc) * 1.0;
+            const double x1 = r0 + u1 * g;
+            const double y1 = v1 * g;
+            double r1 = sqrt(x1 * x1 + y1 * y1);
+            s->x[m] = r1 * p->inv_dx - r_proc;
+            s->y[m] = s->y[m] + atan(y1 / x1) * cy->inv_dtheta;
+            s->z[m] = s->z[m] + (cy->push == 2 ? s->w[m] : w1) * g * p->inv_dz;
+            r1 = 1.0 / r1;
+            u1 = (r0 * u0 + (u0 * u1 * g + v0 * y1)) * r1;
+            const double v2 = (v0 * r0) * r1;
+            if (x0[n] < 4.5) continue;   /* redone by the axis sub-steps */
+            cyl_interp(p, w, s->x[m], s->y[m], s->z[m], e, b);
+            double pE2[3] = {e[0], e[1], e[2]}, pB2[3] = {b[0], b[1], b[2]};
+            if (cy->push == 2)   /* the sums start from the first half's pE and its gamma-scaled pB */
+                for (int a = 0; a < 3; a++) {
+                    pE2[a] = pE1[a] + e[a];
+                    pB2[a] = g1 * pB1[a] + b[a];
+                }
+            u0 = u1; v0 = v2;
+            boris_cscaled(p, pE2[0] * qm, pE2[1] * qm, pE2[2] * qm, pB2[0] * qm, pB2[1] * qm, pB2[2] * qm, &u0, &v0,
+                          &w0);
+            s->u[m] = u0 * cinv;
+            s->v[m] = v0 * cinv;
+            s->w[m] = w0 * cinv;
+        }
+        for (int n = 0; n < nb && !cy->push; n++) {
             const int64_t m = off + n;
             const double qm = (s->flv[m] == 0) ? 0.0 : p->flvrqm[s->flv[m] - 1] * 0.5;
             double e[3], b[3];

@@ -111,7 +111,8 @@ typedef struct orc_cyl {
     double rmin;             /* xborders(0) + rshift */
     double r_proc;           /* xborders(0) + rshift - xmin, cyl_movdep.F90:56 */
     int32_t second_half_new_cell;  /* 1: second zigzag half at the new cell; 0: cyl_movdep.F90:249,252 indexing */
-    int32_t pad_;
+    int32_t push;            /* 0: Boris (cyl_movdep.F90); 1: Strang (cyl_movdep_strang.F90, the
+                                CPU -Dcyl build); 2: Strang as its main loop is written */
     int64_t substep_limit;   /* bound on the axis sub-step loop (unbounded in the reference) */
     int64_t substeps;        /* sub-steps taken by the last orc_cyl_move_deposit */
     double lehe[12];         /* cyl_em_update_lehe.F90:6-17, the order of orc_params.lehe */

@@ -36,7 +36,7 @@ class OrcParams(ctypes.Structure):
 
 class OrcCyl(ctypes.Structure):
     _fields_ = ([(n, ctypes.c_double) for n in ("dtheta", "inv_dtheta", "ax_perm_area", "rshift", "rmin", "r_proc")]
-                + [("second_half_new_cell", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                + [("second_half_new_cell", ctypes.c_int32), ("push", ctypes.c_int32),
                    ("substep_limit", ctypes.c_int64), ("substeps", ctypes.c_int64), ("lehe", ctypes.c_double * 12)])
 
 
@@ -266,7 +266,7 @@ class CylOracle(Oracle):
     (ny cells over 2 pi), z = z.  The outer x ghosts are a conducting wall."""
 
     def __init__(self, ndim=2, nr=16, ntheta=16, nz=1, c=0.45, c_ompe=10.0, g0=1.0, mi_me=1.0, epc=16,
-                 cap=1024, fdtd="yee", second_half_new_cell=True):
+                 cap=1024, fdtd="yee", second_half_new_cell=True, push="boris"):
         super().__init__(ndim, nr, ntheta, nz, c=c, c_ompe=c_ompe, g0=g0, mi_me=mi_me, epc=epc, nthreads=1,
                          cap=cap, reorder_period=0)
         self.cy = OrcCyl()
@@ -274,6 +274,7 @@ class CylOracle(Oracle):
             raise ValueError("the cylindrical grid needs an even number of theta cells")
         self.p.fdtd = {"yee": 0, "lehe": 1}[fdtd]
         self.cy.second_half_new_cell = 1 if second_half_new_cell else 0
+        self.cy.push = {"boris": 0, "strang": 1, "strang_as_written": 2}[push]
 
     def energy(self):
         """CalcEnergy with -Dcyl: B energy, E energy (radius-weighted), flavour 1, flavour 2"""

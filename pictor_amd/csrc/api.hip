@@ -969,6 +969,16 @@ int pictor_synchronize(pictor_ctx *c)
     return PICTOR_OK;
 }
 
+int pictor_cyl_push_scheme(pictor_ctx *c, int32_t scheme)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->cyl || c->cylm) return fail(PICTOR_ERR_ARG, "pictor_cyl_push_scheme is for the cylindrical (r, theta) grid");
+    if (scheme != 0 && scheme != 1) return fail(PICTOR_ERR_ARG, "scheme must be 0 (Boris) or 1 (Strang)");
+    c->cp.strang = scheme;
+    return PICTOR_OK;
+}
+
 int pictor_cyl_plasma(pictor_ctx *c, int32_t nr_plasma, int32_t k0, int32_t nzc, int32_t ppc, uint64_t seed)
 {
     int rc = check_ctx(c);

@@ -16,6 +16,7 @@ struct CylP {
     double lehe[12]; // alpha_x, alpha_y, alpha_z, beta_xy, beta_xz, beta_yx, beta_yz, beta_zx, beta_zy, delta_x..z
     int lehe_on;     // fdtd = Lehe in 3-D
     int substep_limit;
+    int strang;      // 1: the Strang push of cyl_movdep_strang.F90 (the CPU -Dcyl build); 0: cyl_movdep.F90
 };
 
 enum CylErr : int { kCylWall = 1, kCylSubsteps = 2 };

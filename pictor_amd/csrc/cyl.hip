@@ -425,6 +425,74 @@ __device__ inline void deposit_pic(const GridP &g, double qi, double *const *J, 
 // MovePrtlAxis, cyl_movdep.F90:381-446: one straight sub-step of length dt
 // from (x, y, z) with the particle's momenta; false when the azimuthal move
 // exceeds 0.45 cells (nothing is changed then)
+// Boris with half-kick factors pE, pB (times q/m and dt) on the c-scaled momentum, in
+// place (cyl_movdep_strang.F90:172-185); returns the 1 / (c gamma) that scaled pB
+__device__ inline double boris_cscaled(const PhysP &ph, double pEx, double pEy, double pEz, double pBx, double pBy,
+                                       double pBz, double &u0, double &v0, double &w0)
+{
+    const double sqc = ph.sqc;
+    const double U = u0 + pEx, V = v0 + pEy, W = w0 + pEz;
+    const double gg = 1.0 / sqrt(sqc + U * U + V * V + W * W);
+    pBx = gg * pBx; pBy = gg * pBy; pBz = gg * pBz;
+    const double f = 2.0 / (1.0 + pBx * pBx + pBy * pBy + pBz * pBz);
+    const double u1 = (U + V * pBz - W * pBy) * f;
+    const double v1 = (V + W * pBx - U * pBz) * f;
+    const double w1 = (W + U * pBy - V * pBx) * f;
+    u0 = U + v1 * pBz - w1 * pBy + pEx;
+    v0 = V + w1 * pBx - u1 * pBz + pEy;
+    w0 = W + u1 * pBy - v1 * pBx + pEz;
+    return gg;
+}
+
+// MovePrtlAxis of cyl_movdep_strang.F90:462-566 (qm = q/m / 4): half kick with the fields
+// at the start, the straight move of length dt, the rotation of (u, v) into the new
+// local frame, half kick with the fields at the new point
+template <bool THREE>
+__device__ inline bool move_axis_strang(const GridP &g, const PhysP &ph, const CylP &cp, const double *const *E,
+                                        const double *const *B, double qm, double x, double y, double z, double dt,
+                                        double &px, double &py, double &pz, double &pu, double &pv, double &pw)
+{
+    double e[3], b[3];
+    cyl_gather<THREE>(g, E, B, x, y, z, e, b);
+    const double c = ph.c, sqc = ph.sqc, cinv = ph.cinv;
+    double u0 = c * pu, v0 = c * pv, w0 = c * pw;
+    boris_cscaled(ph, e[0] * qm * dt, e[1] * qm * dt, e[2] * qm * dt, b[0] * qm * dt, b[1] * qm * dt, b[2] * qm * dt,
+                  u0, v0, w0);
+    double u1 = u0 * cinv, v1 = v0 * cinv;
+    const double w1 = w0 * cinv;
+    const double gg = sqc / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+    const double r0 = (x - 3.5) * 1.0;
+    const double x1 = r0 + u1 * gg * dt;
+    const double y1 = v1 * gg * dt;
+    const double r1 = sqrt(x1 * x1 + y1 * y1);
+    double dy;
+    if (x1 > 0) dy = atan(y1 / x1) * cp.inv_dtheta;
+    else dy = 0.0;
+    if (fabs(dy) > 0.45) return false;
+    double nx;
+    if (x1 > 0) {
+        nx = r1 / ph.inv_dx + 3.5;
+        u1 = (r0 * u0 + (u0 * u1 * gg * dt + v0 * y1)) / r1;
+        v1 = (v0 * r0) / r1;
+    } else {
+        nx = x1 / ph.inv_dx + 3.5;
+        u1 = u0;
+        v1 = v0;
+    }
+    px = nx;
+    py = py + dy;
+    pz = pz + w1 * gg * dt * ph.inv_dz;
+    cyl_gather<THREE>(g, E, B, px, py, pz, e, b);
+    u0 = u1;
+    v0 = v1;
+    boris_cscaled(ph, e[0] * qm * dt, e[1] * qm * dt, e[2] * qm * dt, b[0] * qm * dt, b[1] * qm * dt, b[2] * qm * dt,
+                  u0, v0, w0);
+    pu = u0 * cinv;
+    pv = v0 * cinv;
+    pw = w0 * cinv;
+    return true;
+}
+
 template <bool THREE>
 __device__ inline bool move_axis(const GridP &g, const PhysP &ph, const CylP &cp, const double *const *E,
                                  const double *const *B, double qm, double x, double y, double z, double dt,
@@ -472,11 +540,11 @@ __device__ inline bool move_axis(const GridP &g, const PhysP &ph, const CylP &cp
     return true;
 }
 
-// MoveDepositPrtl, cyl_movdep.F90:14-289, one thread per particle.  The
-// second zigzag half-segment is deposited at the particle's new cell (the
-// reference indexes it with the old cell, cyl_movdep.F90:249,252, which
-// breaks charge conservation).
-template <bool THREE>
+// MoveDepositPrtl, cyl_movdep.F90:14-289 (STRANG: cyl_movdep_strang.F90:95-294, the
+// push of the reference's CPU -Dcyl build), one thread per particle.  The second
+// zigzag half-segment is deposited at the particle's new cell (the reference indexes it
+// with the old cell, cyl_movdep.F90:249,252, which breaks charge conservation).
+template <bool THREE, bool STRANG>
 __global__ __launch_bounds__(256) void k_cyl_push(GridP g, PhysP ph, CylP cp, const double *__restrict__ Ex,
                                                   const double *__restrict__ Ey, const double *__restrict__ Ez,
                                                   const double *__restrict__ Bx, const double *__restrict__ By,
@@ -493,7 +561,7 @@ __global__ __launch_bounds__(256) void k_cyl_push(GridP g, PhysP ph, CylP cp, co
     double *J[3] = {Jx, Jy, Jz};
     const double x0 = P.x[m], y0 = P.y[m], z0 = P.z[m], q = P.q[m];
     const double ui = P.u[m], vi = P.v[m], wi = P.w[m];
-    const double qm = ph.flvrqm[flv - 1] * 0.5;
+    const double qm = ph.flvrqm[flv - 1] * (STRANG ? 0.25 : 0.5);
     const double c = ph.c, sqc = ph.sqc, cinv = ph.cinv;
     double xn, yn, zn, un, vn, wn;
     if (x0 < 4.5) {
@@ -506,7 +574,9 @@ __global__ __launch_bounds__(256) void k_cyl_push(GridP g, PhysP ph, CylP cp, co
             if (++tries > cp.substep_limit) { atomicOr(err, (int)kCylSubsteps); break; }
             double delt = dt;
             if (t + dt > 1.0) delt = 1.0 - t;
-            if (move_axis<THREE>(g, ph, cp, E, B, qm, sx, sy, sz, delt, xn, yn, zn, un, vn, wn)) {
+            const bool ok = STRANG ? move_axis_strang<THREE>(g, ph, cp, E, B, qm, sx, sy, sz, delt, xn, yn, zn, un, vn, wn)
+                                   : move_axis<THREE>(g, ph, cp, E, B, qm, sx, sy, sz, delt, xn, yn, zn, un, vn, wn);
+            if (ok) {
                 t = t + delt;
                 deposit_pic<THREE>(g, ph.qi, J, sx, sy, sz, xn, yn, zn, q);
                 // PrtlAxisBC, cyl_movdep.F90:366-379
@@ -518,7 +588,7 @@ __global__ __launch_bounds__(256) void k_cyl_push(GridP g, PhysP ph, CylP cp, co
                 }
                 if (yn > ph.ymax) yn = yn - ph.ylen;
                 if (yn < ph.ymin) yn = yn + ph.ylen;
-                dt = dt * 1.1;
+                dt = dt * (STRANG ? 1.7 : 1.1);   // cyl_movdep_strang.F90:427
                 sx = xn; sy = yn; sz = zn;
             } else {
                 dt = dt * 0.5;
@@ -526,6 +596,36 @@ __global__ __launch_bounds__(256) void k_cyl_push(GridP g, PhysP ph, CylP cp, co
         }
         atomicAdd(nsub, (unsigned long long)tries);
     } else {
+      if (STRANG) {
+        // cyl_movdep_strang.F90:95-294 with the gather of the second half reset and the
+        // z move on the half-kicked momentum, as its MovePrtlAxis does (the main loop as
+        // written carries the first half's scaled fields into the second gather's sums
+        // and moves z with the momentum before the kick)
+        double e[3], b[3];
+        cyl_gather<THREE>(g, E, B, x0, y0, z0, e, b);
+        double u0 = c * ui, v0 = c * vi, w0 = c * wi;
+        boris_cscaled(ph, e[0] * qm, e[1] * qm, e[2] * qm, b[0] * qm, b[1] * qm, b[2] * qm, u0, v0, w0);
+        double u1 = u0 * cinv;
+        const double v1 = v0 * cinv, w1 = w0 * cinv;
+        const double gg = sqc / sqrt(sqc + u0 * u0 + v0 * v0 + w0 * w0);
+        const double r0 = (x0 + cp.r_proc) * 1.0;
+        const double x1 = r0 + u1 * gg;
+        const double y1 = v1 * gg;
+        double r1 = sqrt(x1 * x1 + y1 * y1);
+        xn = r1 * ph.inv_dx - cp.r_proc;
+        yn = y0 + atan(y1 / x1) * cp.inv_dtheta;
+        zn = z0 + w1 * gg * ph.inv_dz;
+        r1 = 1.0 / r1;
+        u1 = (r0 * u0 + (u0 * u1 * gg + v0 * y1)) * r1;
+        const double v2 = (v0 * r0) * r1;
+        cyl_gather<THREE>(g, E, B, xn, yn, zn, e, b);
+        u0 = u1;
+        v0 = v2;
+        boris_cscaled(ph, e[0] * qm, e[1] * qm, e[2] * qm, b[0] * qm, b[1] * qm, b[2] * qm, u0, v0, w0);
+        un = u0 * cinv;
+        vn = v0 * cinv;
+        wn = w0 * cinv;
+      } else {
         double pe[3], pb[3];
         cyl_gather<THREE>(g, E, B, x0, y0, z0, pe, pb);
         const double pEx = pe[0] * qm, pEy = pe[1] * qm, pEz = pe[2] * qm;
@@ -555,6 +655,7 @@ __global__ __launch_bounds__(256) void k_cyl_push(GridP g, PhysP ph, CylP cp, co
         r1 = 1.0 / r1;
         un = (x1 * u1 + y1 * v1) * r1;
         vn = (-y1 * u1 + x1 * v1) * r1;
+      }
         // zigzag deposit, cyl_movdep.F90:215-268
         const int ip0 = (int)x0, jp0 = (int)y0, kp0 = THREE ? (int)z0 : 1;
         const int ip = (int)xn, jp = (int)yn, kp = THREE ? (int)zn : 1;
@@ -680,7 +781,9 @@ int cyl_setup(pictor_ctx *c)
 {
     if (c->g.ny % 2 != 0) return fail(PICTOR_ERR_ARG, "the cylindrical grid needs an even number of theta cells (ny)");
     if (c->form != PICTOR_FORM_FP64) return fail(PICTOR_ERR_ARG, "the cylindrical geometry runs in the FP64 form");
+    const int strang = c->cp.strang;   // the push scheme chosen by pictor_cyl_push_scheme survives re-setup
     cyl_params(c->g, c->ph, c->cfg.fdtd, c->cp);
+    c->cp.strang = strang;
     HIPTRY(dalloc(&c->d_cerr, 1));
     HIPTRY(dalloc(&c->d_nsub, 1));
     HIPTRY(hipMemsetAsync(c->d_cerr, 0, sizeof(int), c->stream));
@@ -815,12 +918,19 @@ int cyl_push(pictor_ctx *c, bool timed)
     const long long n = c->np;
     if (timed) HIPTRY(hipEventRecord(c->ev[0], c->stream));
     if (n > 0) {
-        if (g.ndim == 3)
-            hipLaunchKernelGGL(k_cyl_push<true>, dim3(nblocks(n, 256)), dim3(256), 0, c->stream, g, c->ph, c->cp, E[0],
-                               E[1], E[2], B[0], B[1], B[2], J[0], J[1], J[2], c->pcyl, n, c->d_cerr, c->d_nsub);
+        const dim3 gb(nblocks(n, 256)), bs(256);
+        if (g.ndim == 3 && c->cp.strang)
+            hipLaunchKernelGGL((k_cyl_push<true, true>), gb, bs, 0, c->stream, g, c->ph, c->cp, E[0], E[1], E[2], B[0],
+                               B[1], B[2], J[0], J[1], J[2], c->pcyl, n, c->d_cerr, c->d_nsub);
+        else if (g.ndim == 3)
+            hipLaunchKernelGGL((k_cyl_push<true, false>), gb, bs, 0, c->stream, g, c->ph, c->cp, E[0], E[1], E[2], B[0],
+                               B[1], B[2], J[0], J[1], J[2], c->pcyl, n, c->d_cerr, c->d_nsub);
+        else if (c->cp.strang)
+            hipLaunchKernelGGL((k_cyl_push<false, true>), gb, bs, 0, c->stream, g, c->ph, c->cp, E[0], E[1], E[2], B[0],
+                               B[1], B[2], J[0], J[1], J[2], c->pcyl, n, c->d_cerr, c->d_nsub);
         else
-            hipLaunchKernelGGL(k_cyl_push<false>, dim3(nblocks(n, 256)), dim3(256), 0, c->stream, g, c->ph, c->cp, E[0],
-                               E[1], E[2], B[0], B[1], B[2], J[0], J[1], J[2], c->pcyl, n, c->d_cerr, c->d_nsub);
+            hipLaunchKernelGGL((k_cyl_push<false, false>), gb, bs, 0, c->stream, g, c->ph, c->cp, E[0], E[1], E[2], B[0],
+                               B[1], B[2], J[0], J[1], J[2], c->pcyl, n, c->d_cerr, c->d_nsub);
     }
     if (timed) HIPTRY(hipEventRecord(c->ev[1], c->stream));
     const long long nj = (long long)g.my * g.mz;

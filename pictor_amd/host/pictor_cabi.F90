@@ -99,6 +99,12 @@ module pictor_cabi
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
       end function
+      ! push scheme of the cylindrical grid: 0 Boris (cyl_movdep.F90), 1 Strang (cyl_movdep_strang.F90)
+      integer(c_int) function pictor_cyl_push_scheme(ctx, scheme) bind(C, name="pictor_cyl_push_scheme")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: scheme
+      end function
       ! the azimuthal-mode cylinder (PICTOR_GEOM_CYL_MODAL): plasma, laser, moving window
       integer(c_int) function pictor_cyl_plasma(ctx, nr_plasma, k0, nzc, ppc, seed) bind(C, name="pictor_cyl_plasma")
          import :: c_int, c_ptr, c_int32_t, c_int64_t

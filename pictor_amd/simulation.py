@@ -128,6 +128,11 @@ class Simulation:
         check(lib().pictor_current_halo_add(self._ctx))
 
     # ---- the azimuthal-mode cylinder (geometry "cyl_modal") ----
+    def cyl_push_scheme(self, scheme):
+        """push of the cylindrical grid: "boris" (cyl_movdep.F90, the default) or "strang"
+        (cyl_movdep_strang.F90, the reference's CPU -Dcyl build)"""
+        check(lib().pictor_cyl_push_scheme(self._ctx, {"boris": 0, "strang": 1}[scheme]))
+
     def cyl_plasma(self, nr_plasma, k0, nzc, ppc, seed=1):
         """cold uniform electrons in radial cells 4..4+nr_plasma-1, axial cells k0..k0+nzc-1"""
         check(lib().pictor_cyl_plasma(self._ctx, nr_plasma, k0, nzc, ppc, seed))

@@ -1,7 +1,8 @@
 """Cylindrical geometry (the reference's -Dcyl build with the axis) on the GPU
 vs the cylindrical oracle, through the C ABI (needs an MI355X).
 
-Bars:
+Bars (the Boris push of cyl_movdep.F90 and the Strang push of
+cyl_movdep_strang.F90 alike):
   * field updates, axis fills and halos (cyl_em_update.F90, Yee 2-D/3-D and
     Lehe-cyl 3-D) bit-exact against the oracle from identical fields;
   * one push from identical state: positions and momenta within 1e-13
@@ -33,10 +34,11 @@ FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
 C = 0.15
 
 
-def _pair(ndim, nr, nth, nz=1, fdtd="yee", seed=1, n=0, rmax=None, epc=4):
-    say(f"cyl ndim={ndim} nr={nr} ntheta={nth} nz={nz} fdtd={fdtd} seed={seed} n={n}")
-    o = CylOracle(ndim, nr, nth, nz, c=C, epc=epc, fdtd=fdtd, cap=max(n, 1) + 16)
+def _pair(ndim, nr, nth, nz=1, fdtd="yee", seed=1, n=0, rmax=None, epc=4, push="boris"):
+    say(f"cyl ndim={ndim} nr={nr} ntheta={nth} nz={nz} fdtd={fdtd} seed={seed} n={n} push={push}")
+    o = CylOracle(ndim, nr, nth, nz, c=C, epc=epc, fdtd=fdtd, cap=max(n, 1) + 16, push=push)
     sim = Simulation(nr, nth, nz, ndim=ndim, c=C, epc=epc, form="fp64", fdtd=fdtd, geometry="cyl")
+    sim.cyl_push_scheme(push)
     rng = np.random.default_rng(seed)
     P = None
     if n:
@@ -84,9 +86,9 @@ def _push_once(o, sim):
     return ns
 
 
-@pytest.mark.parametrize("ndim", [2, 3])
-def test_cyl_push_matches_oracle(ndim):
-    o, sim, rng, P = _pair(ndim, 24, 16, 12 if ndim == 3 else 1, n=6000, rmax=16.0)
+@pytest.mark.parametrize("ndim,push", [(2, "boris"), (3, "boris"), (2, "strang"), (3, "strang")])
+def test_cyl_push_matches_oracle(ndim, push):
+    o, sim, rng, P = _pair(ndim, 24, 16, 12 if ndim == 3 else 1, n=6000, rmax=16.0, push=push)
     _random_fields(o, sim, rng, amp=2e-2)
     # fields with consistent ghosts and axis values, as after a step
     sim.UpdateBfieldHalf(); o.update_b_half(); o.field_halo("B")
@@ -110,9 +112,9 @@ def test_cyl_push_matches_oracle(ndim):
     sim.close()
 
 
-@pytest.mark.parametrize("ndim", [2, 3])
-def test_cyl_steps_match_oracle(ndim):
-    o, sim, rng, P = _pair(ndim, 20, 16, 10 if ndim == 3 else 1, n=3000, rmax=11.0)
+@pytest.mark.parametrize("ndim,push", [(2, "boris"), (3, "boris"), (2, "strang"), (3, "strang")])
+def test_cyl_steps_match_oracle(ndim, push):
+    o, sim, rng, P = _pair(ndim, 20, 16, 10 if ndim == 3 else 1, n=3000, rmax=11.0, push=push)
     nsteps = 20
     sim.step(nsteps)
     o.run(0, nsteps)

@@ -292,3 +292,85 @@ def test_cyl_energy_weights_each_component_by_its_stagger_radius(ndim):
         want = 0.5 * 9.0 * dth * ny * nzp * radial
         assert abs(e[slot] - want) <= 1e-12 * want, comp
         assert e[1 - slot] == 0.0, comp
+
+
+# ---- the Strang push of the CPU -Dcyl build (cyl_movdep_strang.F90) ----
+
+def test_strang_free_particles_move_on_straight_lines():
+    # no fields: half kick, move, rotation into the new frame, half kick is a straight
+    # line, also for the particles that cross the axis in sub-steps
+    rng = np.random.default_rng(3)
+    o = CylOracle(2, 24, 32, c=0.15, push="strang", cap=600)
+    n = 400
+    r = rng.uniform(0.0, 10.0, n)
+    th = rng.uniform(0.0, 2 * np.pi, n)
+    U = rng.normal(0.0, 0.4, (3, n))
+    ur, ut = _local(th, U[0], U[1])
+    o.load_particles(np.ones(n), 3.5 + r, 3.0 + th / o.cy.dtheta, np.full(n, 1.5), ur, ut, U[2],
+                     np.ones(n, dtype=np.int32))
+    X0, Y0, _ = _cart(o, 3.5 + r, 3.0 + th / o.cy.dtheta)
+    g = np.sqrt(1.0 + (U ** 2).sum(axis=0))
+    steps = 8
+    for _ in range(steps):
+        o.reset_current(); o.move_deposit(); o.exchange()
+    P = o.particles()
+    X, Y, t1 = _cart(o, P["x"], P["y"])
+    # a sub-step that carries a particle past the axis (x1 <= 0) keeps |x1| and drops the
+    # transverse move (cyl_movdep_strang.F90:516-527, as in cyl_movdep.F90): paths that
+    # pass within 0.1 cell of the axis are left out
+    L = steps * o.p.c / g
+    s = np.clip(-(X0 * U[0] + Y0 * U[1]) / np.maximum(U[0] ** 2 + U[1] ** 2, 1e-30), 0.0, L)
+    clear = np.hypot(X0 + s * U[0], Y0 + s * U[1]) > 0.1
+    assert clear.sum() > 390
+    np.testing.assert_allclose(X[clear], (X0 + steps * o.p.c * U[0] / g)[clear], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(Y[clear], (Y0 + steps * o.p.c * U[1] / g)[clear], rtol=0, atol=1e-10)
+    ux = P["u"] * np.cos(t1) - P["v"] * np.sin(t1)
+    np.testing.assert_allclose(ux[clear], U[0][clear], rtol=0, atol=1e-12)
+    assert o.cy.substeps > 0
+
+
+@pytest.mark.parametrize("push,exact", [("boris", True), ("strang", True), ("strang_as_written", False)])
+def test_strang_kick_in_a_uniform_axial_field(push, exact):
+    # a uniform E_z: every particle's c w changes by (q/m) E_z in one step; Strang
+    # applies q/m / 4 four times.  As written the second half starts its gather sums
+    # from the first half's (q/m / 4) E: an extra (q/m)^2 E / 8, 12.5 % for electrons
+    rng = np.random.default_rng(5)
+    o = CylOracle(3, 16, 16, 8, c=0.15, push=push, cap=600, mi_me=1.0)
+    E0 = 1e-3
+    for k in FIELDS:
+        o.F[k][...] = 0.0
+    o.F["Ez"][...] = E0
+    n = 300
+    r = rng.uniform(0.0, 9.0, n)
+    th = rng.uniform(0.0, 2 * np.pi, n)
+    flv = rng.integers(1, 3, n).astype(np.int32)
+    o.load_particles(np.where(flv == 1, 1.0, -1.0), 3.5 + r, 3.0 + th / o.cy.dtheta, rng.uniform(4.0, 8.0, n),
+                     np.zeros(n), np.zeros(n), np.zeros(n), flv)
+    o.reset_current(); o.move_deposit()
+    P = o.particles()
+    qm = np.where(P["flv"] == 1, o.p.flvrqm[0], o.p.flvrqm[1])
+    want = qm * E0 / o.p.c
+    if exact:
+        np.testing.assert_allclose(P["w"], want, rtol=1e-12, atol=0)
+    else:
+        off = P["x"] >= 4.5 + 0.2     # the main loop (the axis sub-steps reset their gathers)
+        extra = qm[off] ** 2 * E0 / 8.0 / o.p.c
+        np.testing.assert_allclose(P["w"][off], want[off] + extra, rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_strang_keeps_the_gauss_residual(ndim):
+    rng = np.random.default_rng(17 + ndim)
+    o = CylOracle(ndim, 20, 16, 8 if ndim == 3 else 1, push="strang")
+    P = list(_plasma(o, 3000, rng, 11.0))
+    nsteps = 12
+    if ndim == 3:
+        P[6] = np.zeros_like(P[6])
+        nsteps = 1
+    o.load_particles(*P)
+    G0 = _gauss(o)[_interior(o)]
+    for t in range(nsteps):
+        o.step(t)
+    G1 = _gauss(o)[_interior(o)]
+    assert np.abs(G1 - G0).max() <= 1e-11 * np.abs(G0).max()   # rounding: about 2e-12
+    assert o.cy.substeps > 0
