@@ -503,6 +503,14 @@ def lwfa_report(args):
                "sample": f"{steps} steps of the same plasma on a {nr} x {nzs} slab ({m} particles) "
                          f"with the modal oracle, one thread, after 1 warm-up step, {dt:.1f}s"}
     kernel = "k_cylm_push_split" if main_form == "fp32_split" else "k_cylm_push"
+    traffic = None   # HBM bytes per launch from the committed counter runs (FETCH_SIZE x 2 + WRITE_SIZE)
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            e = json.load(f).get(kernel)
+        if e and e.get("workload") == args.lwfa:
+            traffic = e["bytes_per_launch"]
+    except (OSError, ValueError):
+        pass
     out = {
         "metric": "particle-updates/s/GPU, cylindrical laser wakefield (modes m = 0, 1)",
         "value": r["value"], "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -515,7 +523,7 @@ def lwfa_report(args):
                    "form": main_form, "particles": npart, "energies": r["energies"],
                    "device_bytes": r["device_bytes"]},
         "roofline": {"bound": "hbm", "achieved": r["achieved"] / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": r["achieved"] / HBM_PEAK, "traffic": None, "kernel": kernel, "push_ms": r["push_ms"]},
+                     "frac": r["achieved"] / HBM_PEAK, "traffic": traffic, "kernel": kernel, "push_ms": r["push_ms"]},
         "cpu_baseline": cpu,
     }
     for form in forms[1:]:
