@@ -445,6 +445,7 @@ def _lwfa_run(args, form):
         sim.cyl_plasma(nrp, 3, nz, ppc, seed=2024)
         sim.cyl_laser(a0, omega0, tau, w0, zc)
         sim.moving_window(True, nr_plasma=nrp, ppc=ppc, seed=2024)
+        sim.cyl_sort_period(args.lwfa_sort)
         sim.step(args.warmup)
         sim.synchronize()
         sim.timer_reset()
@@ -519,7 +520,8 @@ def lwfa_report(args):
         "config": {"workload": f"{args.lwfa}: modal cylinder nz={nz} nr={nr}, dz=0.05, dr=0.2 c/omega_p, "
                                f"c dt = {c} dz, n/n_c=0.005, immobile ions, {ppc} ppc in {nrp} radial cells, "
                                f"a0=2.0, tau=10.0/omega_p, w0=15.0 c/omega_p, moving window, "
-                               f"{'Lehe' if args.lwfa_fdtd == 'lehe' else 'Yee'} FDTD",
+                               f"{'Lehe' if args.lwfa_fdtd == 'lehe' else 'Yee'} FDTD, particles sorted by cell "
+                               f"every {args.lwfa_sort} steps",
                    "form": main_form, "particles": npart, "energies": r["energies"],
                    "device_bytes": r["device_bytes"]},
         "roofline": {"bound": "hbm", "achieved": r["achieved"] / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -630,6 +632,7 @@ def main():
     ap.add_argument("--lwfa", default=None, choices=sorted(LWFA_CONFIGS),
                     help="instead of the Weibel step: time the laser wakefield in the modal cylinder")
     ap.add_argument("--lwfa-fdtd", default="lehe", choices=("lehe", "yee"))
+    ap.add_argument("--lwfa-sort", type=int, default=20, help="steps between sorts of the modal particles by cell")
     ap.add_argument("--lwfa-accuracy", action="store_true",
                     help="split vs FP64 on lwfa_1024x128 (no window): energies every --every steps, Gauss change")
     ap.add_argument("--atomics", action="store_true",

@@ -215,6 +215,12 @@ int pictor_cyl_plasma(pictor_ctx *ctx, int32_t nr_plasma, int32_t k0, int32_t nz
  * omega_p, tau in 1/omega_p (c tau is the 1/e half length of the field), w0 in
  * c/omega_p, zc in axial cells. */
 int pictor_cyl_laser(pictor_ctx *ctx, double a0, double omega0, double tau, double w0, double zc);
+/* Sorting of the modal particles: pictor_sort_tiles (ReorderPrtlArr, mem_prtl.F90:166-301)
+ * orders them by cell, k-major with i fastest (the loader's order), and drops holes, with
+ * a stable radix sort; pictor_cyl_sort_period(ctx, n) makes pictor_step do that every n
+ * steps (0, the default: never).  Strong wakes scatter the particles of neighbouring
+ * array slots over many cells; the push's deposit window needs them in cell order. */
+int pictor_cyl_sort_period(pictor_ctx *ctx, int32_t period);
 /* Moving window: after every fused step the fields and particles move back by
  * one axial cell whenever the window (speed c) has advanced one; the front
  * cells restart empty and receive fresh plasma (nr_plasma, ppc, seed as

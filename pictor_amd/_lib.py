@@ -35,7 +35,7 @@ EXPORTS = (
     "pictor_diagnostics", "pictor_timer_reset", "pictor_timer_get", "pictor_memory_bytes",
     "pictor_setup_weibel", "pictor_adjoint_forward", "pictor_adjoint_backward", "pictor_adjoint_stats",
     "pictor_adjoint_engine", "pictor_adjoint_release", "pictor_cyl_plasma", "pictor_cyl_laser",
-    "pictor_moving_window", "pictor_probe_atomics", "pictor_cyl_push_scheme",
+    "pictor_moving_window", "pictor_probe_atomics", "pictor_cyl_push_scheme", "pictor_cyl_sort_period",
 )
 
 
@@ -100,6 +100,7 @@ def lib():
     L.pictor_probe_atomics.argtypes = [i32, c_dp]
     L.pictor_cyl_plasma.argtypes = [c_vp, i32, i32, i32, i32, ctypes.c_uint64]
     L.pictor_cyl_push_scheme.argtypes = [c_vp, i32]
+    L.pictor_cyl_sort_period.argtypes = [c_vp, i32]
     L.pictor_cyl_laser.argtypes = [c_vp] + [ctypes.c_double] * 5
     L.pictor_moving_window.argtypes = [c_vp, i32, i32, i32, ctypes.c_uint64]
     for name in EXPORTS:

@@ -903,6 +903,7 @@ int pictor_sort_tiles(pictor_ctx *c)
     int rc = check_ctx(c);
     if (rc) return rc;
     if (c->pending_swap) return fail(PICTOR_ERR_ARG, "sort between move_deposit and exchange_particles");
+    if (c->cylm) return cylm_sort(c);   // the modal particles by cell (holes dropped)
     if (c->cyl) return PICTOR_OK;   // the cylindrical path keeps the caller's order
     return relayout(c, 0);
 }
@@ -976,6 +977,17 @@ int pictor_cyl_push_scheme(pictor_ctx *c, int32_t scheme)
     if (!c->cyl || c->cylm) return fail(PICTOR_ERR_ARG, "pictor_cyl_push_scheme is for the cylindrical (r, theta) grid");
     if (scheme != 0 && scheme != 1) return fail(PICTOR_ERR_ARG, "scheme must be 0 (Boris) or 1 (Strang)");
     c->cp.strang = scheme;
+    return PICTOR_OK;
+}
+
+int pictor_cyl_sort_period(pictor_ctx *c, int32_t period)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->cylm) return fail(PICTOR_ERR_ARG, "pictor_cyl_sort_period is for the modal cylinder");
+    if (period < 0) return fail(PICTOR_ERR_ARG, "period must be >= 0");
+    c->cm.sort_period = period;
+    c->cm.steps_since_sort = 0;
     return PICTOR_OK;
 }
 

@@ -34,6 +34,8 @@ struct PartCylS {
 // particle storage and moving-window state of the modal path
 struct CylmState {
     PartCylS ps;                         // the split form's particles (FP32_SPLIT); pcyl holds FP64
+    int sort_period = 0;                 // pictor_step sorts by cell every sort_period steps (0: never)
+    long long steps_since_sort = 0;
     long long cap = 0;                   // allocated particle slots (c->np slots in use, holes included)
     int *d_err = nullptr;
     unsigned long long *d_count = nullptr;
@@ -52,6 +54,7 @@ int cylm_setup(pictor_ctx *c);
 void cylm_release(pictor_ctx *c);
 int cylm_upload_particles(pictor_ctx *c, long long n, const double *q, const double *x, const double *y,
                           const double *z, const double *u, const double *v, const double *w, const int *flv);
+int cylm_sort(pictor_ctx *c);
 int cylm_download_particles(pictor_ctx *c, long long cap, double *q, double *x, double *y, double *z, double *u,
                             double *v, double *w, int *flv, int *tile, long long *n_out);
 int cylm_b_half(pictor_ctx *c);

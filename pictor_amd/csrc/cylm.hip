@@ -24,6 +24,8 @@
 #include <vector>
 #include <algorithm>
 
+#include <hipcub/hipcub.hpp>
+
 #include "ctx.h"
 
 namespace pictor {
@@ -350,6 +352,43 @@ __device__ inline int wave_max(int v)
 // one cell sit in adjacent lanes and would otherwise serialise on the same LDS words
 constexpr int kDepCells = 448;
 constexpr int kDepCopies = 4;
+// when the block's particles spread wider than kDepCells cells (electrons expelled,
+// crossing the axis or trapped in the wake, far from their array neighbours), the window
+// is a box of kBoxI x kBoxK cells around the block's first depositing particle; the
+// particles outside it add to J directly
+constexpr int kBoxI = 28, kBoxK = 4;
+
+// the block's deposit window [bil, bil + wi) x [bkl, bkl + wk); uniform over the block
+__device__ inline void deposit_window(bool dep, const DepMove &d, int (*sred)[4], int *sbox, int &bil, int &bkl,
+                                      int &wi, int &wk, bool &any)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ilo = wave_min(dep ? min(d.i1, d.i2) : INT_MAX), ihi = wave_max(dep ? max(d.i1, d.i2) : INT_MIN);
+    const int klo = wave_min(dep ? min(d.k1, d.k2) : INT_MAX), khi = wave_max(dep ? max(d.k1, d.k2) : INT_MIN);
+    const int first = wave_min(dep ? (int)threadIdx.x : INT_MAX);
+    if (lane == 0) { sred[wv][0] = ilo; sred[wv][1] = ihi; sred[wv][2] = klo; sred[wv][3] = khi; sbox[wv] = first; }
+    __syncthreads();
+    int bih = INT_MIN, bkh = INT_MIN, f = INT_MAX;
+    bil = INT_MAX; bkl = INT_MAX;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        bil = min(bil, sred[w][0]); bih = max(bih, sred[w][1]);
+        bkl = min(bkl, sred[w][2]); bkh = max(bkh, sred[w][3]);
+        f = min(f, sbox[w]);
+    }
+    any = bil <= bih;
+    if (!any) return;
+    wi = bih + 2 - bil;
+    wk = bkh + 2 - bkl;
+    if ((long long)wi * wk <= kDepCells) return;
+    __syncthreads();   // every thread has read sbox
+    if ((int)threadIdx.x == f) { sbox[0] = min(d.i1, d.i2); sbox[1] = min(d.k1, d.k2); }
+    __syncthreads();
+    bil = sbox[0] - 4;
+    bkl = sbox[1] - 1;
+    wi = kBoxI;
+    wk = kBoxK;
+}
 
 // MoveDepositPrtl: gather, Boris, the local-frame move and rotation (cyl_movdep.F90:14-289),
 // then the modal deposit.  The particles keep the loader's cell order, so the 256 of a
@@ -364,7 +403,7 @@ __global__ __launch_bounds__(256) void k_cylm_push(GridP g, CylmP q, const doubl
                                                    long long n, int *__restrict__ err)
 {
     __shared__ double sJ[9 * kDepCells];
-    __shared__ int sred[4][4];
+    __shared__ int sred[4][4], sbox[4];
     const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int flv = m < n ? P.flv[m] : 0;
     bool dep = false;
@@ -428,33 +467,24 @@ __global__ __launch_bounds__(256) void k_cylm_push(GridP g, CylmP q, const doubl
         P.x[m] = x1; P.y[m] = tn; P.z[m] = zn;
         P.u[m] = un; P.v[m] = vn; P.w[m] = wn;
     }
-    // the block's window: i in [ilo, ihi + 1], k in [klo, khi + 1]
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int ilo = wave_min(dep ? min(d.i1, d.i2) : INT_MAX), ihi = wave_max(dep ? max(d.i1, d.i2) : INT_MIN);
-    const int klo = wave_min(dep ? min(d.k1, d.k2) : INT_MAX), khi = wave_max(dep ? max(d.k1, d.k2) : INT_MIN);
-    if (lane == 0) { sred[wv][0] = ilo; sred[wv][1] = ihi; sred[wv][2] = klo; sred[wv][3] = khi; }
-    __syncthreads();
-    int bil = INT_MAX, bih = INT_MIN, bkl = INT_MAX, bkh = INT_MIN;
-#pragma unroll
-    for (int w = 0; w < 4; w++) {
-        bil = min(bil, sred[w][0]); bih = max(bih, sred[w][1]);
-        bkl = min(bkl, sred[w][2]); bkh = max(bkh, sred[w][3]);
-    }
-    if (bil > bih) return;                      // nothing to deposit in this block (uniform)
-    const int wi = bih + 2 - bil, wk = bkh + 2 - bkl;
+    // the block's window: i in [bil, bil + wi), k in [bkl, bkl + wk)
+    const int lane = threadIdx.x & 63;
+    int bil, bkl, wi, wk;
+    bool any;
+    deposit_window(dep, d, sred, sbox, bil, bkl, wi, wk, any);
+    if (!any) return;                           // nothing to deposit in this block (uniform)
     double *J[3] = {Jx, Jy, Jz};
-    if ((long long)wi * wk > kDepCells) {       // scattered block: straight to J
-        if (dep)
-            modal_deposit(d, [&](int cc, int s, int i, int k, double v) { unsafeAtomicAdd(&J[cc][mi(g, i, s, k)], v); });
-        return;
-    }
+    const bool inside = dep && min(d.i1, d.i2) >= bil && max(d.i1, d.i2) + 1 < bil + wi && min(d.k1, d.k2) >= bkl &&
+                        max(d.k1, d.k2) + 1 < bkl + wk;
+    if (dep && !inside)
+        modal_deposit(d, [&](int cc, int s, int i, int k, double v) { unsafeAtomicAdd(&J[cc][mi(g, i, s, k)], v); });
     const int cells = wi * wk;
     int R = kDepCopies;                         // copies per accumulator: cells * R <= kDepCells
     while (R > 1 && cells * R > kDepCells) R >>= 1;
     const int rl = lane & (R - 1);
     for (int e = threadIdx.x; e < 9 * cells * R; e += 256) sJ[e] = 0.0;
     __syncthreads();
-    if (dep)
+    if (inside)
         modal_deposit(d, [&](int cc, int s, int i, int k, double v) {
             atomicAdd(&sJ[((cc * 3 + s) * cells + (k - bkl) * wi + (i - bil)) * R + rl], v);
         });
@@ -504,7 +534,7 @@ __global__ __launch_bounds__(256) void k_cylm_push_split(GridP g, CylmP q, const
                                                          long long n, int *__restrict__ err)
 {
     __shared__ double sJ[9 * kDepCells];
-    __shared__ int sred[4][4];
+    __shared__ int sred[4][4], sbox[4];
     const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t cw = m < n ? P.cell[m] : 0u;
     bool dep = false;
@@ -587,40 +617,30 @@ __global__ __launch_bounds__(256) void k_cylm_push_split(GridP g, CylmP q, const
         P.ox[m] = ox2; P.oz[m] = oz2; P.th[m] = t1;
         P.u[m] = un; P.v[m] = vn; P.w[m] = wn;
     }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int ilo = wave_min(dep ? min(d.i1, d.i2) : INT_MAX), ihi = wave_max(dep ? max(d.i1, d.i2) : INT_MIN);
-    const int klo = wave_min(dep ? min(d.k1, d.k2) : INT_MAX), khi = wave_max(dep ? max(d.k1, d.k2) : INT_MIN);
-    if (lane == 0) { sred[wv][0] = ilo; sred[wv][1] = ihi; sred[wv][2] = klo; sred[wv][3] = khi; }
-    __syncthreads();
-    int bil = INT_MAX, bih = INT_MIN, bkl = INT_MAX, bkh = INT_MIN;
-#pragma unroll
-    for (int w = 0; w < 4; w++) {
-        bil = min(bil, sred[w][0]); bih = max(bih, sred[w][1]);
-        bkl = min(bkl, sred[w][2]); bkh = max(bkh, sred[w][3]);
-    }
-    if (bil > bih) return;
-    const int wi = bih + 2 - bil, wk = bkh + 2 - bkl;
+    const int lane = threadIdx.x & 63;
+    int bil, bkl, wi, wk;
+    bool any;
+    deposit_window(dep, d, sred, sbox, bil, bkl, wi, wk, any);
+    if (!any) return;
     double *J[3] = {Jx, Jy, Jz};
     bool big = false;
     auto count = [&](double v) {
         big = big || !(fabs(v) <= 64.0);
         return rint(v * kJq);
     };
-    if ((long long)wi * wk > kDepCells) {
-        if (dep)
-            modal_deposit(d, [&](int cc, int s, int i, int k, double v) {
-                unsafeAtomicAdd(&J[cc][mi(g, i, s, k)], count(v) * kJqInv);
-            });
-        if (big) atomicOr(err, 2);
-        return;
-    }
+    const bool inside = dep && min(d.i1, d.i2) >= bil && max(d.i1, d.i2) + 1 < bil + wi && min(d.k1, d.k2) >= bkl &&
+                        max(d.k1, d.k2) + 1 < bkl + wk;
+    if (dep && !inside)
+        modal_deposit(d, [&](int cc, int s, int i, int k, double v) {
+            unsafeAtomicAdd(&J[cc][mi(g, i, s, k)], count(v) * kJqInv);
+        });
     const int cells = wi * wk;
     int R = kDepCopies;
     while (R > 1 && cells * R > kDepCells) R >>= 1;
     const int rl = lane & (R - 1);
     for (int e = threadIdx.x; e < 9 * cells * R; e += 256) sJ[e] = 0.0;
     __syncthreads();
-    if (dep)
+    if (inside)
         modal_deposit(d, [&](int cc, int s, int i, int k, double v) {
             atomicAdd(&sJ[((cc * 3 + s) * cells + (k - bkl) * wi + (i - bil)) * R + rl], count(v));
         });
@@ -799,6 +819,10 @@ __global__ void k_cylm_energy(GridP g, CylmP q, const double *__restrict__ Ex, c
 struct RecF64 {
     PartF64 P;
     __device__ bool live(long long m) const { return P.flv[m] != 0; }
+    __device__ uint32_t sort_key(long long m) const
+    {
+        return ((uint32_t)floor(P.z[m]) << 12) | (uint32_t)floor(P.x[m]);
+    }
     __device__ int flv(long long m) const { return P.flv[m]; }
     __device__ void mom(long long m, double &u, double &v, double &w, double &q) const
     {
@@ -813,6 +837,7 @@ struct RecF64 {
 struct RecSplit {
     PartCylS P;
     __device__ bool live(long long m) const { return P.cell[m] != 0u; }
+    __device__ uint32_t sort_key(long long m) const { return P.cell[m] & 0x7FFFFFFFu; }
     __device__ int flv(long long m) const { return cell_flv(P.cell[m]); }
     __device__ void mom(long long m, double &u, double &v, double &w, double &q) const
     {
@@ -1009,6 +1034,75 @@ int ensure_room_rec(pictor_ctx *c, P &cur, long long extra)
 int ensure_room(pictor_ctx *c, long long extra)
 {
     return split_form(c) ? ensure_room_rec(c, c->cm.ps, extra) : ensure_room_rec(c, c->pcyl, extra);
+}
+
+// ReorderPrtlArr (mem_prtl.F90:166-301) for the modal particles: keys k * 4096 + i of the
+// cell (k-major, i fastest: the loader's order), all ones for holes; a stable radix sort
+// of (key, index) pairs, then every array gathered through the permutation
+template <class R>
+__global__ void k_cylm_sort_keys(R P, long long n, uint32_t *key, uint32_t *idx, unsigned long long *live)
+{
+    const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool l = false;
+    if (m < n) {
+        l = P.live(m);
+        key[m] = l ? P.sort_key(m) : 0xFFFFFFFFu;
+        idx[m] = (uint32_t)m;
+    }
+    const unsigned long long b = __ballot(l);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(live, (unsigned long long)__popcll(b));
+}
+
+template <class R>
+__global__ void k_cylm_permute(R P, const uint32_t *__restrict__ idx, long long live, R O)
+{
+    const long long d = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < live) P.copy_to(idx[d], O, d);
+}
+
+template <class P>
+int sort_rec(pictor_ctx *c, P &cur)
+{
+    using R = typename std::conditional<std::is_same<P, PartF64>::value, RecF64, RecSplit>::type;
+    CylmState &st = c->cm;
+    const long long n = c->np;
+    if (n == 0) return PICTOR_OK;
+    if (n > 0x7FFFFFFFLL) return fail(PICTOR_ERR_ARG, "modal sort: more than 2^31 particle slots");
+    uint32_t *kin = nullptr, *kout = nullptr, *iin = nullptr, *iout = nullptr;
+    void *tmp = nullptr;
+    size_t tbytes = 0;
+    int rc = PICTOR_OK;
+    P O{};
+    unsigned long long live = 0;
+    auto done = [&](int r) {
+        dfree(kin); dfree(kout); dfree(iin); dfree(iout); dfree(tmp);
+        if (r) free_parts(O);
+        return r;
+    };
+    if (dalloc(&kin, n) != hipSuccess || dalloc(&kout, n) != hipSuccess || dalloc(&iin, n) != hipSuccess ||
+        dalloc(&iout, n) != hipSuccess)
+        return done(fail(PICTOR_ERR_HIP, "modal sort: key allocation failed"));
+    HIPTRY(hipMemsetAsync(st.d_count, 0, sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_cylm_sort_keys<R>, blocks_of(n), dim3(256), 0, c->stream, R{cur}, n, kin, iin, st.d_count);
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, kin, kout, iin, iout, (int)n, 0, 32, c->stream) !=
+            hipSuccess ||
+        hipMalloc(&tmp, std::max<size_t>(tbytes, 1)) != hipSuccess ||
+        hipcub::DeviceRadixSort::SortPairs(tmp, tbytes, kin, kout, iin, iout, (int)n, 0, 32, c->stream) != hipSuccess)
+        return done(fail(PICTOR_ERR_HIP, "modal sort: radix sort failed"));
+    if (hipMemcpyAsync(&live, st.d_count, sizeof(live), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return done(fail(PICTOR_ERR_HIP, "modal sort: count failed"));
+    if ((rc = alloc_parts(O, st.cap))) return done(rc);
+    if (live > 0)
+        hipLaunchKernelGGL(k_cylm_permute<R>, blocks_of((long long)live), dim3(256), 0, c->stream, R{cur}, iout,
+                           (long long)live, R{O});
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+        return done(fail(PICTOR_ERR_HIP, "modal sort: permute failed"));
+    free_parts(cur);
+    cur = O;
+    c->np = (long long)live;
+    st.steps_since_sort = 0;
+    return done(PICTOR_OK);
 }
 
 }  // namespace
@@ -1258,7 +1352,18 @@ int cylm_laser(pictor_ctx *c, double a0, double omega0, double tau, double w0, d
 }
 
 // after a step: advance the window by whole cells at c (one cell per 1/c steps)
+// after a step: the window's shifts, then the sort by cell when one is due
+static int cylm_window_shifts(pictor_ctx *c);
+
 int cylm_window_after_step(pictor_ctx *c)
+{
+    CylmState &st = c->cm;
+    int rc = cylm_window_shifts(c);
+    if (!rc && st.sort_period > 0 && ++st.steps_since_sort >= st.sort_period) rc = cylm_sort(c);
+    return rc;
+}
+
+static int cylm_window_shifts(pictor_ctx *c)
 {
     CylmState &st = c->cm;
     if (!st.win_on) return PICTOR_OK;
@@ -1361,4 +1466,11 @@ int cylm_download_particles(pictor_ctx *c, long long cap, double *q, double *x, 
         if (flv) flv[e] = live ? cell_flv(wd) : 0;
     }
     return PICTOR_OK;
+}
+
+int cylm_sort(pictor_ctx *c)
+{
+    int rc = cylm_setup(c);
+    if (rc) return rc;
+    return split_form(c) ? sort_rec(c, c->cm.ps) : sort_rec(c, c->pcyl);
 }

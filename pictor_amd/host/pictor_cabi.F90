@@ -105,6 +105,11 @@ module pictor_cabi
          type(c_ptr), value :: ctx
          integer(c_int32_t), value :: scheme
       end function
+      integer(c_int) function pictor_cyl_sort_period(ctx, period) bind(C, name="pictor_cyl_sort_period")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: period
+      end function
       ! the azimuthal-mode cylinder (PICTOR_GEOM_CYL_MODAL): plasma, laser, moving window
       integer(c_int) function pictor_cyl_plasma(ctx, nr_plasma, k0, nzc, ppc, seed) bind(C, name="pictor_cyl_plasma")
          import :: c_int, c_ptr, c_int32_t, c_int64_t

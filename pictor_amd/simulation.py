@@ -133,6 +133,10 @@ class Simulation:
         (cyl_movdep_strang.F90, the reference's CPU -Dcyl build)"""
         check(lib().pictor_cyl_push_scheme(self._ctx, {"boris": 0, "strang": 1}[scheme]))
 
+    def cyl_sort_period(self, period):
+        """modal cylinder: sort the particles by cell (and drop holes) every `period` steps"""
+        check(lib().pictor_cyl_sort_period(self._ctx, period))
+
     def cyl_plasma(self, nr_plasma, k0, nzc, ppc, seed=1):
         """cold uniform electrons in radial cells 4..4+nr_plasma-1, axial cells k0..k0+nzc-1"""
         check(lib().pictor_cyl_plasma(self._ctx, nr_plasma, k0, nzc, ppc, seed))
