@@ -36,6 +36,15 @@ struct CylmState {
     PartCylS ps;                         // the split form's particles (FP32_SPLIT); pcyl holds FP64
     int sort_period = 0;                 // pictor_step sorts by cell every sort_period steps (0: never)
     long long steps_since_sort = 0;
+    // the sort's buffers, kept between sorts: keys and indices (in, out), the radix sort's
+    // scratch, and a spare particle record of the current capacity
+    uint32_t *skey[2] = {nullptr, nullptr}, *sidx[2] = {nullptr, nullptr};
+    long long sbuf_cap = 0;
+    void *stmp = nullptr;
+    size_t stmp_bytes = 0;
+    PartF64 spare64;
+    PartCylS spare32;
+    long long spare_cap = 0;
     long long cap = 0;                   // allocated particle slots (c->np slots in use, holes included)
     int *d_err = nullptr;
     unsigned long long *d_count = nullptr;

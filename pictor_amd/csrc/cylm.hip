@@ -819,9 +819,9 @@ __global__ void k_cylm_energy(GridP g, CylmP q, const double *__restrict__ Ex, c
 struct RecF64 {
     PartF64 P;
     __device__ bool live(long long m) const { return P.flv[m] != 0; }
-    __device__ uint32_t sort_key(long long m) const
+    __device__ uint32_t sort_key(long long m, int ib) const
     {
-        return ((uint32_t)floor(P.z[m]) << 12) | (uint32_t)floor(P.x[m]);
+        return ((uint32_t)floor(P.z[m]) << ib) | (uint32_t)floor(P.x[m]);
     }
     __device__ int flv(long long m) const { return P.flv[m]; }
     __device__ void mom(long long m, double &u, double &v, double &w, double &q) const
@@ -837,7 +837,11 @@ struct RecF64 {
 struct RecSplit {
     PartCylS P;
     __device__ bool live(long long m) const { return P.cell[m] != 0u; }
-    __device__ uint32_t sort_key(long long m) const { return P.cell[m] & 0x7FFFFFFFu; }
+    __device__ uint32_t sort_key(long long m, int ib) const
+    {
+        const uint32_t w = P.cell[m];
+        return ((uint32_t)cell_k(w) << ib) | (uint32_t)cell_i(w);
+    }
     __device__ int flv(long long m) const { return cell_flv(P.cell[m]); }
     __device__ void mom(long long m, double &u, double &v, double &w, double &q) const
     {
@@ -1040,13 +1044,13 @@ int ensure_room(pictor_ctx *c, long long extra)
 // cell (k-major, i fastest: the loader's order), all ones for holes; a stable radix sort
 // of (key, index) pairs, then every array gathered through the permutation
 template <class R>
-__global__ void k_cylm_sort_keys(R P, long long n, uint32_t *key, uint32_t *idx, unsigned long long *live)
+__global__ void k_cylm_sort_keys(R P, long long n, int ib, uint32_t *key, uint32_t *idx, unsigned long long *live)
 {
     const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     bool l = false;
     if (m < n) {
         l = P.live(m);
-        key[m] = l ? P.sort_key(m) : 0xFFFFFFFFu;
+        key[m] = l ? P.sort_key(m, ib) : 0xFFFFFFFFu;
         idx[m] = (uint32_t)m;
     }
     const unsigned long long b = __ballot(l);
@@ -1060,6 +1064,11 @@ __global__ void k_cylm_permute(R P, const uint32_t *__restrict__ idx, long long 
     if (d < live) P.copy_to(idx[d], O, d);
 }
 
+PartF64 &spare_of(CylmState &st, PartF64 *) { return st.spare64; }
+PartCylS &spare_of(CylmState &st, PartCylS *) { return st.spare32; }
+template <class P>
+P &spare_of(CylmState &st) { return spare_of(st, (P *)nullptr); }
+
 template <class P>
 int sort_rec(pictor_ctx *c, P &cur)
 {
@@ -1068,41 +1077,59 @@ int sort_rec(pictor_ctx *c, P &cur)
     const long long n = c->np;
     if (n == 0) return PICTOR_OK;
     if (n > 0x7FFFFFFFLL) return fail(PICTOR_ERR_ARG, "modal sort: more than 2^31 particle slots");
-    uint32_t *kin = nullptr, *kout = nullptr, *iin = nullptr, *iout = nullptr;
-    void *tmp = nullptr;
-    size_t tbytes = 0;
-    int rc = PICTOR_OK;
-    P O{};
-    unsigned long long live = 0;
-    auto done = [&](int r) {
-        dfree(kin); dfree(kout); dfree(iin); dfree(iout); dfree(tmp);
-        if (r) free_parts(O);
-        return r;
-    };
-    if (dalloc(&kin, n) != hipSuccess || dalloc(&kout, n) != hipSuccess || dalloc(&iin, n) != hipSuccess ||
-        dalloc(&iout, n) != hipSuccess)
-        return done(fail(PICTOR_ERR_HIP, "modal sort: key allocation failed"));
+    // key = k << ib | i over ib = ceil(log2 mx) bits; holes (all ones) sort last with end_bit
+    int ib = 1;
+    while ((1 << ib) < c->g.mx) ib++;
+    int end_bit = ib + 1;
+    while ((1LL << end_bit) <= ((long long)c->g.mz << ib)) end_bit++;
+    if (end_bit > 32) return fail(PICTOR_ERR_ARG, "modal sort: grid too large for 32-bit keys");
+    end_bit = std::min(32, end_bit + 1);
+    if (st.sbuf_cap < n) {
+        for (int a = 0; a < 2; a++) { dfree(st.skey[a]); dfree(st.sidx[a]); st.skey[a] = st.sidx[a] = nullptr; }
+        st.sbuf_cap = 0;
+        const long long cap = std::max<long long>(n, st.cap);
+        for (int a = 0; a < 2; a++)
+            if (dalloc(&st.skey[a], cap) != hipSuccess || dalloc(&st.sidx[a], cap) != hipSuccess)
+                return fail(PICTOR_ERR_HIP, "modal sort: key allocation failed");
+        st.sbuf_cap = cap;
+    }
+    size_t need = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, need, st.skey[0], st.skey[1], st.sidx[0], st.sidx[1], (int)n, 0,
+                                           end_bit, c->stream) != hipSuccess)
+        return fail(PICTOR_ERR_HIP, "modal sort: radix sort sizing failed");
+    if (st.stmp_bytes < need) {
+        dfree(st.stmp);
+        st.stmp = nullptr;
+        st.stmp_bytes = 0;
+        if (hipMalloc(&st.stmp, need) != hipSuccess) return fail(PICTOR_ERR_HIP, "modal sort: scratch allocation failed");
+        st.stmp_bytes = need;
+    }
+    P &O = spare_of<P>(st);
+    if (st.spare_cap != st.cap) {
+        free_parts(O);
+        st.spare_cap = 0;
+        int rc = alloc_parts(O, st.cap);
+        if (rc) return rc;
+        st.spare_cap = st.cap;
+    }
     HIPTRY(hipMemsetAsync(st.d_count, 0, sizeof(unsigned long long), c->stream));
-    hipLaunchKernelGGL(k_cylm_sort_keys<R>, blocks_of(n), dim3(256), 0, c->stream, R{cur}, n, kin, iin, st.d_count);
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, kin, kout, iin, iout, (int)n, 0, 32, c->stream) !=
-            hipSuccess ||
-        hipMalloc(&tmp, std::max<size_t>(tbytes, 1)) != hipSuccess ||
-        hipcub::DeviceRadixSort::SortPairs(tmp, tbytes, kin, kout, iin, iout, (int)n, 0, 32, c->stream) != hipSuccess)
-        return done(fail(PICTOR_ERR_HIP, "modal sort: radix sort failed"));
-    if (hipMemcpyAsync(&live, st.d_count, sizeof(live), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess)
-        return done(fail(PICTOR_ERR_HIP, "modal sort: count failed"));
-    if ((rc = alloc_parts(O, st.cap))) return done(rc);
+    hipLaunchKernelGGL(k_cylm_sort_keys<R>, blocks_of(n), dim3(256), 0, c->stream, R{cur}, n, ib, st.skey[0],
+                       st.sidx[0], st.d_count);
+    size_t tb = st.stmp_bytes;
+    if (hipcub::DeviceRadixSort::SortPairs(st.stmp, tb, st.skey[0], st.skey[1], st.sidx[0], st.sidx[1], (int)n, 0,
+                                           end_bit, c->stream) != hipSuccess)
+        return fail(PICTOR_ERR_HIP, "modal sort: radix sort failed");
+    unsigned long long live = 0;
+    HIPTRY(hipMemcpyAsync(&live, st.d_count, sizeof(live), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
     if (live > 0)
-        hipLaunchKernelGGL(k_cylm_permute<R>, blocks_of((long long)live), dim3(256), 0, c->stream, R{cur}, iout,
+        hipLaunchKernelGGL(k_cylm_permute<R>, blocks_of((long long)live), dim3(256), 0, c->stream, R{cur}, st.sidx[1],
                            (long long)live, R{O});
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
-        return done(fail(PICTOR_ERR_HIP, "modal sort: permute failed"));
-    free_parts(cur);
-    cur = O;
+    HIPTRY(hipGetLastError());
+    std::swap(cur, O);   // the old record becomes the spare (same capacity)
     c->np = (long long)live;
     st.steps_since_sort = 0;
-    return done(PICTOR_OK);
+    return PICTOR_OK;
 }
 
 }  // namespace
@@ -1148,6 +1175,10 @@ void cylm_release(pictor_ctx *c)
 {
     CylmState &st = c->cm;
     free_particles(c);
+    free_parts(st.spare64);
+    free_parts(st.spare32);
+    for (int a = 0; a < 2; a++) { dfree(st.skey[a]); dfree(st.sidx[a]); }
+    dfree(st.stmp);
     dfree(st.d_err); dfree(st.d_count); dfree(st.d_partial);
     st = CylmState{};
     c->np = 0;
