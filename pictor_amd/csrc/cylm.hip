@@ -1400,14 +1400,17 @@ static int cylm_window_shifts(pictor_ctx *c)
     if (!st.win_on) return PICTOR_OK;
     st.win_steps += 1;
     while ((double)st.win_steps * c->ph.c >= (double)(st.win_shift + 1)) {
+        // E, then B, is shifted into the J buffers, which then swap with it; J ends up
+        // holding the old B buffers and is cleared (the next step's ResetCurrent clears it too)
+        for (int f = 0; f < 2; f++) {
+            void **F = f == 0 ? c->E : c->B;
+            Ptrs6<double> P{{(double *)F[0], (double *)F[1], (double *)F[2], (double *)c->J[0], (double *)c->J[1],
+                             (double *)c->J[2]}};
+            hipLaunchKernelGGL(k_cylm_shift_fields, dim3(nblocks(c->g.ncell, 256), 3), dim3(256), 0, c->stream, c->g, P);
+            for (int i = 0; i < 3; i++) std::swap(F[i], c->J[i]);
+        }
         double *E[3], *B[3], *J[3];
         fields(c, E, B, J);
-        // J (zeroed again by the next step's ResetCurrent) holds the copy being shifted
-        for (double **F : {E, B}) {
-            launch_copy_n<double>(J, F, 3, c->g.ncell, c->stream);
-            Ptrs6<double> P{{J[0], J[1], J[2], F[0], F[1], F[2]}};
-            hipLaunchKernelGGL(k_cylm_shift_fields, dim3(nblocks(c->g.ncell, 256), 3), dim3(256), 0, c->stream, c->g, P);
-        }
         launch_fill_zero_n<double>(J, 3, c->g.ncell, c->stream);
         if (c->np > 0 && split_form(c))
             hipLaunchKernelGGL(k_cylm_shift_split, blocks_of(c->np), dim3(256), 0, c->stream, c->cq, c->cm.ps, c->np);
