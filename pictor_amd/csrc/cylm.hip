@@ -1044,17 +1044,26 @@ int ensure_room(pictor_ctx *c, long long extra)
 // cell (k-major, i fastest: the loader's order), all ones for holes; a stable radix sort
 // of (key, index) pairs, then every array gathered through the permutation
 template <class R>
-__global__ void k_cylm_sort_keys(R P, long long n, int ib, uint32_t *key, uint32_t *idx, unsigned long long *live)
+__global__ void k_cylm_sort_keys(R P, long long n, int ib, uint32_t *key, uint32_t *idx)
 {
     const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool l = false;
     if (m < n) {
-        l = P.live(m);
-        key[m] = l ? P.sort_key(m, ib) : 0xFFFFFFFFu;
+        key[m] = P.live(m) ? P.sort_key(m, ib) : 0xFFFFFFFFu;
         idx[m] = (uint32_t)m;
     }
-    const unsigned long long b = __ballot(l);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(live, (unsigned long long)__popcll(b));
+}
+
+// the live count: the holes' keys (all ones) sort last; one thread finds the first
+__global__ void k_cylm_first_hole(const uint32_t *__restrict__ key, long long n, unsigned long long *out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    long long lo = 0, hi = n;
+    while (lo < hi) {
+        const long long mid = lo + (hi - lo) / 2;
+        if (key[mid] == 0xFFFFFFFFu) hi = mid;
+        else lo = mid + 1;
+    }
+    *out = (unsigned long long)lo;
 }
 
 template <class R>
@@ -1112,13 +1121,13 @@ int sort_rec(pictor_ctx *c, P &cur)
         if (rc) return rc;
         st.spare_cap = st.cap;
     }
-    HIPTRY(hipMemsetAsync(st.d_count, 0, sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_cylm_sort_keys<R>, blocks_of(n), dim3(256), 0, c->stream, R{cur}, n, ib, st.skey[0],
-                       st.sidx[0], st.d_count);
+                       st.sidx[0]);
     size_t tb = st.stmp_bytes;
     if (hipcub::DeviceRadixSort::SortPairs(st.stmp, tb, st.skey[0], st.skey[1], st.sidx[0], st.sidx[1], (int)n, 0,
                                            end_bit, c->stream) != hipSuccess)
         return fail(PICTOR_ERR_HIP, "modal sort: radix sort failed");
+    hipLaunchKernelGGL(k_cylm_first_hole, dim3(1), dim3(64), 0, c->stream, st.skey[1], n, st.d_count);
     unsigned long long live = 0;
     HIPTRY(hipMemcpyAsync(&live, st.d_count, sizeof(live), hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipStreamSynchronize(c->stream));
