@@ -52,16 +52,22 @@ def test_yee_and_halo_bitwise():
         assert np.array_equal(G[n], o.F[n]), n
 
 
-def test_push_one_step_bitwise_fp64():
-    nx, ny = 32, 24
-    o, sim, P = _pair(nx, ny, epc=8, seed=7)
+# every tile-size instantiation of the push kernels (tile 8 and 16: 256-lane
+# workgroups; tile 32: 512 lanes FP64, 1024 split), with and without the XCD
+# tile remap (number of tiles divisible by 8)
+TILE_CASES = [(32, 24, 8), (64, 32, 8), (64, 64, 16), (48, 32, 16), (128, 64, 32), (64, 64, 32)]
+
+
+@pytest.mark.parametrize("nx,ny,tile", TILE_CASES)
+def test_push_one_step_bitwise_fp64(nx, ny, tile):
+    o, sim, P = _pair(nx, ny, epc=8, seed=7, tile=tile)
     with sim:
         sim.ResetCurrent(); sim.MoveDepositPrtl(); sim.ExchangePrtl()
         o.reset_current(); o.move_deposit(); o.exchange()
         G = sim.download_particles(with_tile=True)
         assert_same_multiset(G, o.particles())
         # particle -> tile assignment is a function of the cell index
-        assert np.array_equal(G["tile"], tiles_of(G, nx, ny, 8))
+        assert np.array_equal(G["tile"], tiles_of(G, nx, ny, tile))
         J = sim.download_fields()
         for n in ("Jx", "Jy", "Jz"):
             assert rel_err(J[n], o.F[n]) < 1e-13, n
@@ -97,9 +103,9 @@ def test_fused_step_equals_phases():
     assert np.array_equal(a["flv"], b["flv"])
 
 
-def test_steps_fp64_vs_oracle():
-    nx, ny = 32, 32
-    o, sim, P = _pair(nx, ny, epc=16, seed=4, fields=False)
+@pytest.mark.parametrize("nx,ny,tile", [(32, 32, 8), (64, 64, 32)])
+def test_steps_fp64_vs_oracle(nx, ny, tile):
+    o, sim, P = _pair(nx, ny, epc=16, seed=4, fields=False, tile=tile)
     nsteps = 40
     with sim:
         for t in range(1, nsteps + 1):
@@ -121,9 +127,9 @@ def test_steps_fp64_vs_oracle():
         assert sim.particle_count() == len(P["x"])
 
 
-def test_split_vs_fp64_oracle():
-    nx, ny = 32, 32
-    o, sim, P = _pair(nx, ny, epc=16, form="fp32_split", seed=4, fields=False)
+@pytest.mark.parametrize("nx,ny,tile", [(32, 32, 8), (64, 64, 16), (128, 64, 32), (64, 64, 32)])
+def test_split_vs_fp64_oracle(nx, ny, tile):
+    o, sim, P = _pair(nx, ny, epc=16, form="fp32_split", seed=4, fields=False, tile=tile)
     nsteps = 40
     with sim:
         for t in range(1, nsteps + 1):
@@ -169,13 +175,13 @@ def test_split_gauss_residual_stays_at_rounding(ndim):
         assert res["fp32_split"] <= 2 * res["fp64"] + 1e-11
 
 
-def test_split_deterministic():
-    nx, ny = 64, 32
+@pytest.mark.parametrize("nx,ny,tile", [(64, 32, 8), (64, 64, 16), (128, 64, 32)])
+def test_split_deterministic(nx, ny, tile):
     P = setup_weibel(nx, ny, epc=8, seed=9)
     res = []
     for rep in range(2):
-        say(f"rep={rep}")
-        with Simulation(nx, ny, epc=8, form="fp32_split", tile=8) as sim:
+        say(f"rep={rep} nx={nx} ny={ny} tile={tile}")
+        with Simulation(nx, ny, epc=8, form="fp32_split", tile=tile) as sim:
             sim.upload_particles(**P)
             sim.step(25)
             res.append((sim.download_fields(), sim.diagnostics()))
