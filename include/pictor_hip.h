@@ -54,7 +54,8 @@ extern "C" {
                                     (nx+5, 3, nz+5) with slot 0 = m 0, 1 = m 1
                                     cos, 2 = m 1 sin; particles y = theta in
                                     radians; radial cell cfg.dr axial cells;
-                                    FP64, Yee, periodic z (or the moving window) */
+                                    FP64, Yee; periodic z, or open z ends while
+                                    the moving window is on */
 
 /* Run configuration: the compile-time parameters of parameters.F90:5-87 and
  * src/vars.F90:10-18 as a runtime struct.  grid_dx = grid_dy = grid_dz = 1,
@@ -224,7 +225,10 @@ int pictor_cyl_sort_period(pictor_ctx *ctx, int32_t period);
 /* Moving window: after every fused step the fields and particles move back by
  * one axial cell whenever the window (speed c) has advanced one; the front
  * cells restart empty and receive fresh plasma (nr_plasma, ppc, seed as
- * pictor_cyl_plasma).  on = 0 stops it. */
+ * pictor_cyl_plasma).  While it is on, z is open instead of periodic: the z
+ * ghost cells hold no field, the current deposited in them is dropped, and a
+ * particle that leaves through either end becomes a hole.  on = 0 stops it
+ * and makes z periodic again. */
 int pictor_moving_window(pictor_ctx *ctx, int32_t on, int32_t nr_plasma, int32_t ppc, uint64_t seed);
 
 /* ---- measurement (no counterpart in the reference) ----

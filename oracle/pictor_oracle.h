@@ -142,13 +142,15 @@ typedef struct orc_cylm {
     double ompe, qi;
     double flvrqm[2];
     double zmin, zmax, zlen;
-    /* fdtd = 1: the Lehe B update (orc_cylm_set_fdtd) */
-    int32_t fdtd, pad;
+    /* fdtd = 1: the Lehe B update (orc_cylm_set_fdtd); zopen = 1: open z ends
+     * (the moving window, orc_cylm_set_zopen) instead of periodic z */
+    int32_t fdtd, zopen;
     double l_rr, l_rz, l_tt, l_tz, l_zz, l_zr, l_zd, l_zm;
 } orc_cylm;
 
 int orc_cylm_init(orc_cylm *m, int nr, int nz, double dr, double c, double c_ompe, double g0, double mi_me, int epc);
 int orc_cylm_set_fdtd(orc_cylm *m, int fdtd);
+void orc_cylm_set_zopen(orc_cylm *m, int zopen);
 void orc_cylm_b_half(const orc_cylm *m, orc_fields *f);
 void orc_cylm_e_phase(const orc_cylm *m, orc_fields *f, int curl, int add);
 void orc_cylm_current_halo(const orc_cylm *m, orc_fields *f);

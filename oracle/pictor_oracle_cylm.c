@@ -89,15 +89,19 @@ static inline double rad(const orc_cylm *m, double i) { return (i - 3.5) * m->dr
 /* ---------------------------------------------------------------- fields */
 
 /* z ghosts: 1..2 <- nz+1..nz+2, mz-2..mz <- 3..5 (periodic z), over x = 3..mx-3
- * as the reference's z faces (subdomains.F90:592-680, SetCommFldDomain) */
+ * as the reference's z faces (subdomains.F90:592-680, SetCommFldDomain).  Open z
+ * (the moving window): the ghosts hold no field, so nothing from the back of the
+ * box reappears at its front. */
 static void cylm_zhalo(const orc_cylm *m, double *F)
 {
     for (int s = 0; s < 3; s++)
         for (int i = 3; i <= m->mx - 3; i++) {
-            for (int k = 1; k <= 2; k++) F[MI(m, i, s, k)] = F[MI(m, i, s, k + m->nz)];
-            for (int k = m->mz - 2; k <= m->mz; k++) F[MI(m, i, s, k)] = F[MI(m, i, s, k - m->nz)];
+            for (int k = 1; k <= 2; k++) F[MI(m, i, s, k)] = m->zopen ? 0.0 : F[MI(m, i, s, k + m->nz)];
+            for (int k = m->mz - 2; k <= m->mz; k++) F[MI(m, i, s, k)] = m->zopen ? 0.0 : F[MI(m, i, s, k - m->nz)];
         }
 }
+
+void orc_cylm_set_zopen(orc_cylm *m, int zopen) { m->zopen = zopen ? 1 : 0; }
 
 /* the conducting wall: cells i >= mx - 2 hold no field */
 static void cylm_outer_zero(const orc_cylm *m, double *F)
@@ -263,7 +267,8 @@ void orc_cylm_e_phase(const orc_cylm *m, orc_fields *f, int curl, int add)
 }
 
 /* AxisCurrentBC (cyl_movdep.F90:451-465) on every mode, then the periodic z
- * fold: 3..5 += mz-2..mz, mz-4..mz-3 += 1..2 (comm_fld.F90:15-33) */
+ * fold: 3..5 += mz-2..mz, mz-4..mz-3 += 1..2 (comm_fld.F90:15-33); with open z
+ * the current deposited in the ghost rows leaves the box with its particles */
 void orc_cylm_current_halo(const orc_cylm *m, orc_fields *f)
 {
     for (int k = 1; k <= m->mz; k++)
@@ -274,6 +279,7 @@ void orc_cylm_current_halo(const orc_cylm *m, orc_fields *f)
             f->Jy[MI(m, 2, s, k)] = 0.0;
             f->Jz[MI(m, 2, s, k)] = 0.0;
         }
+    if (m->zopen) return;
     double *J[3] = {f->Jx, f->Jy, f->Jz};
     for (int c = 0; c < 3; c++)
         for (int s = 0; s < 3; s++)
@@ -414,8 +420,14 @@ int orc_cylm_move_deposit(const orc_cylm *m, orc_fields *f, orc_particles *s)
             cylm_deposit(m, f, s->q[n], x0, th0, z0, x1, th1, z1);
         }
         double zn = z1, tn = th1;
-        if (zn > m->zmax) zn = zn - m->zlen;
-        else if (zn < m->zmin) zn = m->zlen + zn;
+        if (m->zopen) {
+            /* open z: a particle that leaves through either end is dropped */
+            if (zn >= m->zmax || zn < m->zmin) s->flv[n] = 0;
+        } else if (zn > m->zmax) {
+            zn = zn - m->zlen;
+        } else if (zn < m->zmin) {
+            zn = m->zlen + zn;
+        }
         if (tn >= 2.0 * PI_D) tn = tn - 2.0 * PI_D;
         else if (tn < 0.0) tn = tn + 2.0 * PI_D;
         s->x[n] = x1; s->y[n] = tn; s->z[n] = zn;
@@ -478,7 +490,7 @@ void orc_cylm_gauss(const orc_cylm *m, const orc_fields *f, const orc_particles 
             for (int nn = 0; nn < 4; nn++) {
                 const int a = nn & 1, b = nn >> 1;
                 int kk = k + b;
-                if (kk > m->mz - 3) kk -= m->nz;   /* periodic image */
+                if (kk > m->mz - 3 && !m->zopen) kk -= m->nz;   /* periodic image */
                 rho[MI(m, i + a, sl, kk)] += qq * ph[sl] * (a ? wx : 1.0 - wx) * (b ? wz : 1.0 - wz);
             }
     }

@@ -330,7 +330,7 @@ class OrcCylm(ctypes.Structure):
                                                   "ompe", "qi")]
                 + [("flvrqm", ctypes.c_double * 2)]
                 + [(n, ctypes.c_double) for n in ("zmin", "zmax", "zlen")]
-                + [("fdtd", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                + [("fdtd", ctypes.c_int32), ("zopen", ctypes.c_int32)]
                 + [(n, ctypes.c_double) for n in ("l_rr", "l_rz", "l_tt", "l_tz", "l_zz", "l_zr", "l_zd", "l_zm")])
 
 
@@ -355,6 +355,8 @@ class CylModalOracle:
                 getattr(L, n).argtypes = [P, F]
             L.orc_cylm_e_phase.argtypes = [P, F, ctypes.c_int, ctypes.c_int]
             L.orc_cylm_set_fdtd.argtypes = [P, ctypes.c_int]
+            L.orc_cylm_set_zopen.argtypes = [P, ctypes.c_int]
+            L.orc_cylm_set_zopen.restype = None
             L.orc_cylm_move_deposit.argtypes = [P, F, S]
             L.orc_cylm_step.argtypes = [P, F, S]
             L.orc_cylm_energy.argtypes = [P, F, c_dp]
@@ -431,6 +433,11 @@ class CylModalOracle:
     def run(self, nsteps):
         for _ in range(nsteps):
             self.step()
+
+    def set_zopen(self, on=True):
+        """open z ends (the moving window): ghosts hold no field, no z fold of J,
+        particles leaving through either end are dropped"""
+        lib().orc_cylm_set_zopen(ctypes.byref(self.m), 1 if on else 0)
 
     def shift(self, nr_p, ppc, seed, shift):
         """the moving window's shift by one axial cell (shift = the shift count after it)"""

@@ -95,11 +95,18 @@ int ensure_spill(pictor_ctx *c, int need)
     if (c->spill_cap >= need) return PICTOR_OK;
     if (c->form == PICTOR_FORM_FP64) {
         free_f64(c->spill_f);
+        c->spill_cap = 0;
         HIPTRY(alloc_f64(c->spill_f, need));
     } else {
         free_split(c->spill_s);
+        c->spill_cap = 0;
         HIPTRY(alloc_split(c->spill_s, need, true));
     }
+    dfree(c->rl_grp);
+    c->rl_grp = nullptr;
+    c->rl_grp_cap = 0;
+    HIPTRY(dalloc(&c->rl_grp, need));
+    c->rl_grp_cap = need;
     c->spill_cap = need;
     return PICTOR_OK;
 }
@@ -108,13 +115,14 @@ int ensure_spill(pictor_ctx *c, int need)
 // A tile can lose or gain at most the particles within c cells of a face per
 // step; the side bins are sized for that flux with a margin, the centre bin
 // for the whole group times capacity_factor.
-void compute_layout(pictor_ctx *c, const std::vector<long long> &counts)
+void compute_layout_into(const pictor_ctx *c, const std::vector<long long> &counts, std::vector<long long> &h_base,
+                         std::vector<int> &h_cap, long long &total)
 {
     const TileP &tp = c->tp;
     const double cf = c->cfg.capacity_factor > 0 ? c->cfg.capacity_factor : 1.25;
     const double frac = std::min(1.0, c->ph.c / tp.T);
-    c->h_base.assign((size_t)tp.ntiles * tp.NB, 0);
-    c->h_cap.assign((size_t)tp.ntiles * tp.NB, 0);
+    h_base.assign((size_t)tp.ntiles * tp.NB, 0);
+    h_cap.assign((size_t)tp.ntiles * tp.NB, 0);
     long long off = 0;
     for (int t = 0; t < tp.ntiles; t++)
         for (int s = 0; s < tp.nsp; s++) {
@@ -127,12 +135,17 @@ void compute_layout(pictor_ctx *c, const std::vector<long long> &counts)
                 else cap = std::ceil(n * std::pow(frac, nz) * 1.6 * std::pow(2.0, nz - 1)) + (nz == 1 ? 32 : 16);
                 long long ci = ((long long)cap + 15) / 16 * 16;
                 const size_t b = (size_t)t * tp.NB + s * tp.ND + d;
-                c->h_base[b] = off;
-                c->h_cap[b] = (int)std::min<long long>(ci, 0x7fffffff);
+                h_base[b] = off;
+                h_cap[b] = (int)std::min<long long>(ci, 0x7fffffff);
                 off += ci;
             }
         }
-    c->layout_cap = off;
+    total = off;
+}
+
+void compute_layout(pictor_ctx *c, const std::vector<long long> &counts)
+{
+    compute_layout_into(c, counts, c->h_base, c->h_cap, c->layout_cap);
 }
 
 int upload_layout(pictor_ctx *c)
@@ -157,19 +170,22 @@ int upload_center_counts(pictor_ctx *c, int which, const std::vector<long long> 
     return PICTOR_OK;
 }
 
-std::vector<long long> group_offsets(const pictor_ctx *c, bool dense, const std::vector<long long> &counts)
+std::vector<long long> group_offsets(const pictor_ctx *c, bool dense, const std::vector<long long> &counts,
+                                     const std::vector<long long> *base = nullptr, long long total = -1)
 {
     // dense: consecutive runs (download); otherwise the centre-bin bases of the layout
+    // (the context's, or `base` with `total` slots)
     const TileP &tp = c->tp;
     const int centre = (tp.ND - 1) / 2;
     const size_t ng = (size_t)tp.ntiles * tp.nsp;
+    const std::vector<long long> &hb = base ? *base : c->h_base;
     std::vector<long long> off(ng + 1, 0);
     long long acc = 0;
     for (size_t gi = 0; gi < ng; gi++) {
         if (dense) { off[gi] = acc; acc += counts[gi]; }
-        else off[gi] = c->h_base[(gi / tp.nsp) * tp.NB + (gi % tp.nsp) * tp.ND + centre];
+        else off[gi] = hb[(gi / tp.nsp) * tp.NB + (gi % tp.nsp) * tp.ND + centre];
     }
-    off[ng] = dense ? acc : c->layout_cap;
+    off[ng] = dense ? acc : (base ? total : c->layout_cap);
     return off;
 }
 
@@ -190,76 +206,88 @@ int group_counts(pictor_ctx *c, int which, std::vector<long long> &counts)
     return PICTOR_OK;
 }
 
-// Re-layout: buffer `cur` bins + `nspill` spilled particles -> new layout in the other buffer.
+// Re-layout: buffer `cur` bins + `nspill` spilled particles -> new layout in the other
+// buffer.  The scratch lives in the context (rl_*); the host copy of the new layout
+// replaces the old one only once every allocation has succeeded, so a failure leaves
+// the context on its old layout.
 int relayout(pictor_ctx *c, int nspill)
 {
     const TileP &tp = c->tp;
     const size_t ng = (size_t)tp.ntiles * tp.nsp;
+    if (nspill > c->rl_grp_cap) return fail(PICTOR_ERR_CAPACITY, "re-layout: more spilled particles than the spill buffer");
     std::vector<long long> counts;
     int rc = group_counts(c, c->cur, counts);
     if (rc) return rc;
-    int *d_gcount = nullptr, *d_grp = nullptr, *d_fill = nullptr, *d_gtotal = nullptr;
-    long long *d_off = nullptr;
-    HIPTRY(dalloc(&d_gcount, ng));
-    HIPTRY(dalloc(&d_gtotal, ng));
-    HIPTRY(dalloc(&d_fill, ng));
-    HIPTRY(dalloc(&d_off, ng + 1));
-    HIPTRY(dalloc(&d_grp, std::max(nspill, 1)));
-    HIPTRY(hipMemsetAsync(d_gcount, 0, ng * sizeof(int), c->stream));
+    HIPTRY(hipMemsetAsync(c->rl_gcount, 0, ng * sizeof(int), c->stream));
     if (nspill > 0) {
-        if (c->form == PICTOR_FORM_FP64) launch_spill_tiles_f64(c->g, tp, c->spill_f, nspill, d_grp, d_gcount, c->stream);
-        else launch_spill_tiles_split(c->g, tp, c->spill_s, nspill, d_grp, d_gcount, c->stream);
+        if (c->form == PICTOR_FORM_FP64)
+            launch_spill_tiles_f64(c->g, tp, c->spill_f, nspill, c->rl_grp, c->rl_gcount, c->stream);
+        else
+            launch_spill_tiles_split(c->g, tp, c->spill_s, nspill, c->rl_grp, c->rl_gcount, c->stream);
     }
     std::vector<int> sc(ng);
-    HIPTRY(hipMemcpyAsync(sc.data(), d_gcount, ng * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipMemcpyAsync(sc.data(), c->rl_gcount, ng * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipStreamSynchronize(c->stream));
     std::vector<long long> total(ng);
     long long np = 0;
     for (size_t gi = 0; gi < ng; gi++) { total[gi] = counts[gi] + sc[gi]; np += total[gi]; }
-    // old layout stays on the device for the compaction source
-    std::vector<long long> old_base = c->h_base;
-    compute_layout(c, total);
+    std::vector<long long> nbase;
+    std::vector<int> ncap;
+    long long ntotal = 0;
+    compute_layout_into(c, total, nbase, ncap, ntotal);
     const int dst = 1 - c->cur;
-    rc = ensure_buffer(c, dst, c->layout_cap);
-    if (rc) return rc;
-    std::vector<long long> off = group_offsets(c, false, total);
-    HIPTRY(hipMemcpyAsync(d_off, off.data(), (ng + 1) * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    if ((rc = ensure_buffer(c, dst, ntotal))) return rc;
+    std::vector<long long> off = group_offsets(c, false, total, &nbase, ntotal);
     // fill starts after the compacted bins of each group
     std::vector<int> fill(ng), gtot(ng);
     for (size_t gi = 0; gi < ng; gi++) { fill[gi] = (int)counts[gi]; gtot[gi] = (int)total[gi]; }
-    HIPTRY(hipMemcpyAsync(d_fill, fill.data(), ng * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPTRY(hipMemcpyAsync(d_gtotal, gtot.data(), ng * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemcpyAsync(c->rl_off, off.data(), (ng + 1) * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemcpyAsync(c->rl_fill, fill.data(), ng * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemcpyAsync(c->rl_gtotal, gtot.data(), ng * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // the compaction reads the old layout on the device (d_base, d_cnt)
     if (c->form == PICTOR_FORM_FP64) {
-        launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], d_off, d_gtotal, c->pf[dst], c->stream);
-        launch_spill_scatter_f64(tp, c->spill_f, nspill, d_grp, d_off, d_fill, d_gtotal, c->pf[dst], c->stream);
+        launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], c->rl_off, c->rl_gtotal, c->pf[dst], c->stream);
+        launch_spill_scatter_f64(tp, c->spill_f, nspill, c->rl_grp, c->rl_off, c->rl_fill, c->rl_gtotal, c->pf[dst],
+                                 c->stream);
     } else {
-        launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], d_off, d_gtotal, c->ps[dst], c->stream);
-        launch_spill_scatter_split(tp, c->spill_s, nspill, d_grp, d_off, d_fill, d_gtotal, c->ps[dst], c->stream);
+        launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], c->rl_off, c->rl_gtotal, c->ps[dst],
+                             c->stream);
+        launch_spill_scatter_split(tp, c->spill_s, nspill, c->rl_grp, c->rl_off, c->rl_fill, c->rl_gtotal, c->ps[dst],
+                                   c->stream);
     }
     HIPTRY(hipGetLastError());
-    rc = upload_layout(c);
-    if (rc) return rc;
-    rc = upload_center_counts(c, dst, total);
-    if (rc) return rc;
+    // the host vectors of the copies must live until they complete
+    HIPTRY(hipStreamSynchronize(c->stream));
+    // commit the new layout
+    c->h_base.swap(nbase);
+    c->h_cap.swap(ncap);
+    c->layout_cap = ntotal;
+    if ((rc = upload_layout(c))) return rc;
+    if ((rc = upload_center_counts(c, dst, total))) return rc;
     HIPTRY(hipMemsetAsync(c->d_spill_count, 0, sizeof(int), c->stream));
     HIPTRY(hipStreamSynchronize(c->stream));
-    dfree(d_gcount); dfree(d_fill); dfree(d_off); dfree(d_grp); dfree(d_gtotal);
     c->cur = dst;
-    // the other buffer must hold the new layout too
-    rc = ensure_buffer(c, 1 - dst, c->layout_cap);
-    if (rc) return rc;
-    rc = ensure_spill(c, (int)std::min<long long>(std::max<long long>(65536, np / 64), 1 << 30));
-    if (rc) return rc;
     c->np = np;
-    return PICTOR_OK;
+    // the other buffer must hold the new layout too
+    if ((rc = ensure_buffer(c, 1 - dst, c->layout_cap))) return rc;
+    return ensure_spill(c, (int)std::min<long long>(std::max<long long>(65536, np / 64), 1 << 30));
 }
 
-// spill / error check after a push: sync, and re-layout if particles spilled
-int commit_push(pictor_ctx *c)
+// The spill count and error bits of the last push, copied to the pinned host flags
+// behind the step's kernels; `ev` marks their arrival.
+int enqueue_flags(pictor_ctx *c, hipEvent_t ev)
 {
     HIPTRY(hipMemcpyAsync(c->h_flags, c->d_spill_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipMemcpyAsync(c->h_flags + 1, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPTRY(hipStreamSynchronize(c->stream));
+    HIPTRY(hipEventRecord(ev, c->stream));
+    return PICTOR_OK;
+}
+
+// After the flags of a push have arrived: swap the buffers, report errors, and re-lay
+// out storage if particles spilled.
+int finish_push(pictor_ctx *c, hipEvent_t ev)
+{
+    HIPTRY(hipEventSynchronize(ev));
     c->cur = 1 - c->cur;
     c->pending_swap = false;
     const int nspill = c->h_flags[0], err = c->h_flags[1];
@@ -270,6 +298,13 @@ int commit_push(pictor_ctx *c)
         return fail(PICTOR_ERR_CAPACITY, "particle spill buffer exhausted");
     if (nspill > 0) return relayout(c, nspill);
     return PICTOR_OK;
+}
+
+// spill / error check after a push (pictor_exchange_particles): wait for the flags
+int commit_push(pictor_ctx *c)
+{
+    int rc = enqueue_flags(c, c->ev[22]);
+    return rc ? rc : finish_push(c, c->ev[22]);
 }
 
 int check_ctx(pictor_ctx *c)
@@ -298,14 +333,17 @@ int do_b_half(pictor_ctx *c)
     return PICTOR_OK;
 }
 
-int do_push(pictor_ctx *c, bool timed)
+// timed: the push kernel is bracketed by events e0, e1 (ev[0], ev[1] unless given)
+int do_push(pictor_ctx *c, bool timed, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
+    if (!e0) e0 = c->ev[0];
+    if (!e1) e1 = c->ev[1];
     if (c->pending_swap) return fail(PICTOR_ERR_ARG, "pictor_move_deposit called twice without pictor_exchange_particles");
     HIPTRY(hipMemsetAsync(c->d_spill_count, 0, sizeof(int), c->stream));
     const int a = c->cur, b = 1 - c->cur;
     BinsP bins{c->d_base, c->d_cap, c->d_cnt[a], c->d_cnt[b]};
     SpillP sp{c->d_spill_count, c->spill_cap, c->d_err};
-    if (timed) HIPTRY(hipEventRecord(c->ev[0], c->stream));
+    if (timed) HIPTRY(hipEventRecord(e0, c->stream));
     const bool three = (c->g.ndim == 3);
     if (c->form == PICTOR_FORM_FP64) {
         double *E[3], *B[3], *J[3];
@@ -326,7 +364,7 @@ int do_push(pictor_ctx *c, bool timed)
             launch_push2d_split(c->tp.T, c->g, c->ph, c->tp, E, B, J, c->ps[a], c->ps[b], bins, c->spill_s, sp, c->stream);
     }
     HIPTRY(hipGetLastError());
-    if (timed) HIPTRY(hipEventRecord(c->ev[1], c->stream));
+    if (timed) HIPTRY(hipEventRecord(e1, c->stream));
     c->pending_swap = true;
     return PICTOR_OK;
 }
@@ -557,29 +595,34 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
         if (hipMalloc(&c->E[i], g.ncell * rs) != hipSuccess || hipMalloc(&c->B[i], g.ncell * rs) != hipSuccess ||
             hipMalloc(&c->J[i], g.ncell * 8) != hipSuccess)
             return cleanup_fail(fail(PICTOR_ERR_HIP, "field allocation failed"));
-        hipMemsetAsync(c->E[i], 0, g.ncell * rs, c->stream);
-        hipMemsetAsync(c->B[i], 0, g.ncell * rs, c->stream);
-        hipMemsetAsync(c->J[i], 0, g.ncell * 8, c->stream);
+        if (hipMemsetAsync(c->E[i], 0, g.ncell * rs, c->stream) != hipSuccess ||
+            hipMemsetAsync(c->B[i], 0, g.ncell * rs, c->stream) != hipSuccess ||
+            hipMemsetAsync(c->J[i], 0, g.ncell * 8, c->stream) != hipSuccess)
+            return cleanup_fail(fail(PICTOR_ERR_HIP, "field clearing failed"));
         if (!f64) {
             if (hipMalloc(&c->Elo[i], g.ncell * sizeof(float)) != hipSuccess)
                 return cleanup_fail(fail(PICTOR_ERR_HIP, "field allocation failed"));
-            hipMemsetAsync(c->Elo[i], 0, g.ncell * sizeof(float), c->stream);
+            if (hipMemsetAsync(c->Elo[i], 0, g.ncell * sizeof(float), c->stream) != hipSuccess)
+                return cleanup_fail(fail(PICTOR_ERR_HIP, "field clearing failed"));
         }
         if (cfg->curr_filter > 0 && hipMalloc(&c->Jt[i], g.ncell * 8) != hipSuccess)
             return cleanup_fail(fail(PICTOR_ERR_HIP, "field allocation failed"));
     }
     c->curr_filter = cfg->curr_filter;
     const size_t nb = (size_t)tp.ntiles * tp.NB;
+    const size_t ngr = (size_t)tp.ntiles * tp.nsp;
     if (dalloc(&c->d_base, nb) || dalloc(&c->d_cap, nb) || dalloc(&c->d_cnt[0], nb) || dalloc(&c->d_cnt[1], nb) ||
-        dalloc(&c->d_spill_count, 1) || dalloc(&c->d_err, 1))
+        dalloc(&c->d_spill_count, 1) || dalloc(&c->d_err, 1) || dalloc(&c->rl_gcount, ngr) ||
+        dalloc(&c->rl_gtotal, ngr) || dalloc(&c->rl_fill, ngr) || dalloc(&c->rl_off, ngr + 1))
         return cleanup_fail(fail(PICTOR_ERR_HIP, "bin allocation failed"));
-    hipMemsetAsync(c->d_cnt[0], 0, nb * sizeof(int), c->stream);
-    hipMemsetAsync(c->d_cnt[1], 0, nb * sizeof(int), c->stream);
-    hipMemsetAsync(c->d_spill_count, 0, sizeof(int), c->stream);
-    hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream);
+    if (hipMemsetAsync(c->d_cnt[0], 0, nb * sizeof(int), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_cnt[1], 0, nb * sizeof(int), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_spill_count, 0, sizeof(int), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream) != hipSuccess)
+        return cleanup_fail(fail(PICTOR_ERR_HIP, "bin clearing failed"));
     if (hipHostMalloc((void **)&c->h_flags, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return cleanup_fail(fail(PICTOR_ERR_HIP, "pinned allocation failed"));
-    for (int i = 0; i < 8; i++)
+    for (int i = 0; i < 23; i++)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) return cleanup_fail(fail(PICTOR_ERR_HIP, "event creation failed"));
     // empty layout so that a step without particles is valid
     int rc = PICTOR_OK;
@@ -614,13 +657,14 @@ int pictor_finalize(pictor_ctx *c)
     for (int i = 0; i < 3; i++) { dfree(c->E[i]); dfree(c->B[i]); dfree(c->J[i]); dfree(c->Elo[i]); dfree(c->Jt[i]); }
     dfree(c->d_base); dfree(c->d_cap); dfree(c->d_cnt[0]); dfree(c->d_cnt[1]);
     dfree(c->d_spill_count); dfree(c->d_err);
+    dfree(c->rl_gcount); dfree(c->rl_gtotal); dfree(c->rl_fill); dfree(c->rl_off); dfree(c->rl_grp);
     for (int i = 0; i < 2; i++) { free_f64(c->pf[i]); free_split(c->ps[i]); }
     free_f64(c->spill_f); free_split(c->spill_s);
     if (c->cylm) cylm_release(c);
     cyl_release(c);
     dfree(c->d_partial); dfree(c->d_rho);
     if (c->h_flags) hipHostFree(c->h_flags);
-    for (int i = 0; i < 8; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    for (int i = 0; i < 23; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return PICTOR_OK;
@@ -938,25 +982,45 @@ int pictor_step(pictor_ctx *c, int32_t nsteps)
         }
         return PICTOR_OK;
     }
+    // MAIN.F90:42-284.  The host does not wait for a step's spill and error flags before
+    // it enqueues the next step's ResetCurrent and first B half step (which do not touch
+    // the particles): it waits for them behind those kernels, while the GPU runs them,
+    // then re-lays out storage if needed and enqueues the push.  A step's events (set
+    // ev[8 + 7 p .. 14 + 7 p], p alternating) are read once its flags have arrived.
+    auto step_times = [&](int p) {
+        hipEvent_t *E = c->ev + 8 + 7 * p;   // 0 start, 1 fields start, 2 push, 3 push end, 4 fields end, 5 end
+        float ms = 0, f1 = 0, f2 = 0;
+        if (hipEventElapsedTime(&ms, E[2], E[3]) == hipSuccess) { c->t_ms[0] += ms; c->t_n[0]++; }
+        if (hipEventElapsedTime(&f1, E[1], E[4]) == hipSuccess) { c->t_ms[1] += f1 - ms; c->t_n[1]++; }
+        if (hipEventElapsedTime(&f2, E[0], E[5]) == hipSuccess) { c->t_ms[2] += f2; c->t_n[2]++; }
+    };
+    int prev = -1;   // parity of the step whose flags are in flight
     for (int s = 0; s < nsteps; s++) {
-        HIPTRY(hipEventRecord(c->ev[4], c->stream));
+        const int p = s & 1;
+        hipEvent_t *E = c->ev + 8 + 7 * p;
+        HIPTRY(hipEventRecord(E[0], c->stream));
         if ((rc = do_reset_current(c))) return rc;
-        HIPTRY(hipEventRecord(c->ev[2], c->stream));
+        HIPTRY(hipEventRecord(E[1], c->stream));
         if ((rc = do_b_half(c))) return rc;
-        if ((rc = do_push(c, true))) return rc;
+        if (prev >= 0) {
+            rc = finish_push(c, c->ev[8 + 7 * prev + 6]);
+            step_times(prev);
+            prev = -1;
+            if (rc) return rc;
+        }
+        if ((rc = do_push(c, true, E[2], E[3]))) return rc;
         if ((rc = do_b_half(c))) return rc;
         if ((rc = do_fold(c))) return rc;
         if ((rc = do_smoothen(c))) return rc;
         if ((rc = do_e_phase(c, true, true))) return rc;
-        HIPTRY(hipEventRecord(c->ev[3], c->stream));
-        HIPTRY(hipEventRecord(c->ev[5], c->stream));
-        rc = commit_push(c);
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) { c->t_ms[0] += ms; c->t_n[0]++; }
-        float f1 = 0, f2 = 0;
-        if (hipEventElapsedTime(&f1, c->ev[2], c->ev[3]) == hipSuccess) { c->t_ms[1] += f1 - ms; c->t_n[1]++; }
-        if (hipEventElapsedTime(&f2, c->ev[4], c->ev[5]) == hipSuccess) { c->t_ms[2] += f2; c->t_n[2]++; }
-        (void)f1;
+        HIPTRY(hipEventRecord(E[4], c->stream));
+        HIPTRY(hipEventRecord(E[5], c->stream));
+        if ((rc = enqueue_flags(c, E[6]))) return rc;
+        prev = p;
+    }
+    if (prev >= 0) {
+        rc = finish_push(c, c->ev[8 + 7 * prev + 6]);
+        step_times(prev);
         if (rc) return rc;
     }
     return PICTOR_OK;
@@ -1017,6 +1081,7 @@ int pictor_moving_window(pictor_ctx *c, int32_t on, int32_t nr_plasma, int32_t p
         return fail(PICTOR_ERR_ARG, "nr_plasma must lie in 0..nx-1 and ppc >= 0");
     CylmState &st = c->cm;
     st.win_on = on != 0;
+    c->cq.zopen = on ? 1 : 0;   // the window's box has open z ends
     st.win_nrp = nr_plasma;
     st.win_ppc = ppc;
     st.win_seed = seed;

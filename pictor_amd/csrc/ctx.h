@@ -89,14 +89,22 @@ struct pictor_ctx {
     int *d_spill_count = nullptr;
     int *d_err = nullptr;
     int *h_flags = nullptr;   // pinned: {spill count, error bits}
+    // re-layout scratch, allocated with the bins: per (tile, species) group counts,
+    // totals, fill cursors and offsets; the group of each spilled particle
+    int *rl_gcount = nullptr, *rl_gtotal = nullptr, *rl_fill = nullptr, *rl_grp = nullptr;
+    long long *rl_off = nullptr;
+    int rl_grp_cap = 0;
 
     // diagnostics scratch
     double *d_partial = nullptr;
     int partial_len = 0;
     double *d_rho = nullptr;
 
-    // timers: 0 push, 1 field kernels, 2 whole step
-    hipEvent_t ev[8] = {};
+    // timers: 0 push, 1 field kernels, 2 whole step.  ev[0..7]: the phase calls and the
+    // cylindrical steps; ev[8..14] and ev[15..21]: the fused Cartesian step, two sets of 7
+    // used alternately so that one step's events are read while the next step is being
+    // enqueued; ev[22]: the spill and error flags of a phase-call push copied to the host
+    hipEvent_t ev[23] = {};
     double t_ms[3] = {0, 0, 0};
     long long t_n[3] = {0, 0, 0};
 

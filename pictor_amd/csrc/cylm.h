@@ -14,6 +14,9 @@ struct CylmP {
     double qi;
     double flvrqm[2];
     double zmin, zmax, zlen;
+    // open z ends while the moving window is on: the z ghosts hold no field, J is not
+    // folded along z and a particle that leaves through either end becomes a hole
+    int zopen;
     // the Lehe B update (fdtd = PICTOR_FDTD_LEHE): z averaging of the r and theta
     // differences (l_rr, l_rz; l_tt, l_tz) and the z difference with the wide pair (l_zz, l_zd)
     int lehe;

@@ -208,8 +208,10 @@ __global__ void k_cylm_b_axis(GridP g, CylmP q, double *__restrict__ Bx, double 
     Bz[mi(g, 2, s, k)] = p * Bz[mi(g, 4, s, k)];
 }
 
-// periodic z ghosts over x = 3..mx-3, then the wall (i >= mx-2 zero); three arrays
-__global__ void k_cylm_zhalo(GridP g, double *__restrict__ F0, double *__restrict__ F1, double *__restrict__ F2)
+// periodic z ghosts over x = 3..mx-3 (zero ghosts for open z), then the wall (i >= mx-2
+// zero); three arrays
+__global__ void k_cylm_zhalo(GridP g, int zopen, double *__restrict__ F0, double *__restrict__ F1,
+                             double *__restrict__ F2)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)g.mx * 3 * 5) return;
@@ -221,7 +223,7 @@ __global__ void k_cylm_zhalo(GridP g, double *__restrict__ F0, double *__restric
     const int src = l < 2 ? k + g.nz : k - g.nz;
     double *F[3] = {F0, F1, F2};
 #pragma unroll
-    for (int c = 0; c < 3; c++) F[c][mi(g, i, s, k)] = F[c][mi(g, i, s, src)];
+    for (int c = 0; c < 3; c++) F[c][mi(g, i, s, k)] = zopen ? 0.0 : F[c][mi(g, i, s, src)];
 }
 
 __global__ void k_cylm_wall(GridP g, double *__restrict__ F0, double *__restrict__ F1, double *__restrict__ F2)
@@ -460,8 +462,13 @@ __global__ __launch_bounds__(256) void k_cylm_push(GridP g, CylmP q, const doubl
             d.pt[2] = -(qthis * (cos(th1) - cos(th0)));
         }
         double zn = z1, tn = th1;
-        if (zn > q.zmax) zn = zn - q.zlen;
-        else if (zn < q.zmin) zn = q.zlen + zn;
+        if (q.zopen) {
+            if (zn >= q.zmax || zn < q.zmin) P.flv[m] = 0;   // open z: left the box
+        } else if (zn > q.zmax) {
+            zn = zn - q.zlen;
+        } else if (zn < q.zmin) {
+            zn = q.zlen + zn;
+        }
         if (tn >= 2.0 * kPi) tn = tn - 2.0 * kPi;
         else if (tn < 0.0) tn = tn + 2.0 * kPi;
         P.x[m] = x1; P.y[m] = tn; P.z[m] = zn;
@@ -610,10 +617,11 @@ __global__ __launch_bounds__(256) void k_cylm_push_split(GridP g, CylmP q, const
             d.pt[1] = qthis * ((double)s1 - (double)s0);
             d.pt[2] = -(qthis * ((double)c1 - (double)c0));
         }
-        // periodic z; the stored record
+        // periodic z (open z: a particle that left the box becomes a hole); the stored record
+        const bool gone = q.zopen && (k2 >= (int)q.zmax || k2 < (int)q.zmin);
         if (k2 >= (int)q.zmax) k2 -= (int)q.zlen;
         else if (k2 < (int)q.zmin) k2 += (int)q.zlen;
-        P.cell[m] = pack_cell(i2, k2, flv);
+        P.cell[m] = gone ? 0u : pack_cell(i2, k2, flv);
         P.ox[m] = ox2; P.oz[m] = oz2; P.th[m] = t1;
         P.u[m] = un; P.v[m] = vn; P.w[m] = wn;
     }
@@ -946,7 +954,8 @@ dim3 blocks_of(long long n) { return dim3(nblocks(n, 256)); }
 void halo(pictor_ctx *c, double *const F[3])
 {
     const GridP &g = c->g;
-    hipLaunchKernelGGL(k_cylm_zhalo, blocks_of((long long)g.mx * 15), dim3(256), 0, c->stream, g, F[0], F[1], F[2]);
+    hipLaunchKernelGGL(k_cylm_zhalo, blocks_of((long long)g.mx * 15), dim3(256), 0, c->stream, g, c->cq.zopen, F[0],
+                       F[1], F[2]);
     hipLaunchKernelGGL(k_cylm_wall, blocks_of(9LL * g.mz), dim3(256), 0, c->stream, g, F[0], F[1], F[2]);
 }
 
@@ -1153,6 +1162,7 @@ void cylm_params(const pictor_config &cfg, const PhysP &ph, CylmP &q)
     q.flvrqm[0] = ph.flvrqm[0];
     q.flvrqm[1] = ph.flvrqm[1];
     q.zmin = 3.0; q.zmax = 3.0 + cfg.nz; q.zlen = cfg.nz;
+    q.zopen = 0;
     // Lehe coefficients (cyl_em_update_lehe.F90:6-17): beta_rz = beta_thetaz = 1/8,
     // d_z = (1 - sin^2(pi c / 2) / c^2) / 4; no r pair, no theta pair in gradZ
     q.lehe = cfg.fdtd == PICTOR_FDTD_LEHE ? 1 : 0;
@@ -1295,8 +1305,10 @@ int cylm_current_halo(pictor_ctx *c)
     double *E[3], *B[3], *J[3];
     fields(c, E, B, J);
     hipLaunchKernelGGL(k_cylm_axis_current, blocks_of(3LL * g.mz), dim3(256), 0, c->stream, g, J[0], J[1], J[2]);
-    hipLaunchKernelGGL(k_cylm_zfold, blocks_of(9LL * g.mx), dim3(256), 0, c->stream, g, 0, J[0], J[1], J[2]);
-    hipLaunchKernelGGL(k_cylm_zfold, blocks_of(6LL * g.mx), dim3(256), 0, c->stream, g, 1, J[0], J[1], J[2]);
+    if (!c->cq.zopen) {   // open z: the ghost rows' current leaves the box with its particles
+        hipLaunchKernelGGL(k_cylm_zfold, blocks_of(9LL * g.mx), dim3(256), 0, c->stream, g, 0, J[0], J[1], J[2]);
+        hipLaunchKernelGGL(k_cylm_zfold, blocks_of(6LL * g.mx), dim3(256), 0, c->stream, g, 1, J[0], J[1], J[2]);
+    }
     HIPTRY(hipGetLastError());
     return PICTOR_OK;
 }
@@ -1347,6 +1359,8 @@ int cylm_exchange(pictor_ctx *c)
     HIPTRY(hipMemcpyAsync(&err, c->cm.d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipStreamSynchronize(c->stream));
     c->pending_swap = false;
+    // the flags report this push only: clear them for the next one
+    if (err) HIPTRY(hipMemsetAsync(c->cm.d_err, 0, sizeof(int), c->stream));
     if (err & 1) return fail(PICTOR_ERR_BOUNDS, "modal cylinder: a particle reached the outer wall");
     if (err & 2)
         return fail(PICTOR_ERR_CAPACITY, "modal cylinder, split form: a current contribution above 64 charge cells "

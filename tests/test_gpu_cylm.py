@@ -140,6 +140,7 @@ def test_moving_window_matches_oracle():
     with sim:
         sim.upload_particles(**P)
         sim.moving_window(True, nr_plasma=10, ppc=4, seed=13)
+        o.set_zopen(True)   # the window box has open z ends, as on the GPU
         nsteps, shifts = 12, 0
         sim.step(nsteps)
         for t in range(1, nsteps + 1):
@@ -456,6 +457,7 @@ def test_window_run_through_compaction_keeps_order(form):
                     mi_me=1e30) as sim:
         sim.upload_particles(**P)
         sim.moving_window(True, nr_plasma=nrp, ppc=ppc, seed=7)
+        o.set_zopen(True)   # the window box has open z ends, as on the GPU
         sim.step(nsteps)
         A = sim.download_particles()
         G = sim.download_fields()
@@ -501,6 +503,7 @@ def test_sorting_by_cell_keeps_the_physics(form):
                     mi_me=1e30) as sim:
         sim.upload_particles(**P)
         sim.moving_window(True, nr_plasma=nrp, ppc=ppc, seed=9)
+        o.set_zopen(True)   # the window box has open z ends, as on the GPU
         sim.cyl_sort_period(5)
         sim.step(nsteps)
         A = sim.download_particles()
@@ -524,3 +527,71 @@ def test_sorting_by_cell_keeps_the_physics(form):
         d = np.abs(G[k] - o.F[k]).max() / np.abs(o.F[k]).max()
         say(f"{k}: max diff / max {d:.2e}")
         assert d <= (1e-9 if form == "fp64" else 1e-3), k
+
+
+def _back_state(o, rng):
+    """fields in the back rows only (k = 3..5) and electrons at the back moving backwards,
+    plus a few slow ones in the middle of the box"""
+    mx, _, mz = o.shape
+    F = _smooth_fields(o, rng, amp=1e-2)
+    for n in FIELDS:
+        F[n][:, :, 5:] = 0.0            # python index 5 = Fortran k = 6
+        F[n][:, :, :2] = 0.0
+    n_back, n_mid = 8, 8
+    x = np.concatenate([rng.uniform(5.0, mx - 6.0, n_back + n_mid)])
+    z = np.concatenate([rng.uniform(3.05, 3.4, n_back), rng.uniform(8.0, 12.0, n_mid)])
+    w = np.concatenate([np.full(n_back, -3.0), np.zeros(n_mid)])
+    n = n_back + n_mid
+    P = {"q": np.full(n, -1.0), "x": x, "y": rng.uniform(0.0, 2 * np.pi, n), "z": z,
+         "u": np.zeros(n), "v": np.zeros(n), "w": w, "flv": np.full(n, 2, dtype=np.int32)}
+    return F, P
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32_split"])
+def test_window_back_does_not_reappear_at_front(form):
+    # open z under the moving window: fields of the back rows and electrons leaving through
+    # the back must not show up at the front of the box (periodic z would copy the back rows
+    # into the front ghosts and wrap the electrons to the front)
+    say(f"[case] window open z form={form}")
+    # nz = 32: nothing from the back rows or the electrons at rest reaches the front
+    # rows in 6 steps (the stencils spread a field by one cell per step)
+    nr, nz, c = 12, 32, 0.45
+    o = CylModalOracle(nr, nz, dr=2.0, c=c, c_ompe=10.0, epc=4, cap=1024)
+    o.set_zopen(True)
+    rng = np.random.default_rng(17)
+    F, P = _back_state(o, rng)
+    if form == "fp32_split":
+        cell = np.floor(P["x"]); P["x"] = cell + (P["x"] - cell).astype(np.float32).astype(np.float64)
+        cell = np.floor(P["z"]); P["z"] = cell + (P["z"] - cell).astype(np.float32).astype(np.float64)
+        P["y"] = np.round(P["y"] / TURN) % 2.0 ** 32 * TURN
+    o.set_fields(**F)
+    o.load_particles(P["q"], P["x"], P["y"], P["z"], P["u"], P["v"], P["w"], P["flv"], cap=1024)
+    nsteps, shifts = 6, 0
+    with Simulation(nr, 2, nz, ndim=3, epc=4, c=c, c_ompe=10.0, form=form, geometry="cyl_modal", dr=2.0,
+                    mi_me=1e30) as sim:
+        sim.upload_fields(*[np.asfortranarray(F[n]) for n in FIELDS])
+        sim.upload_particles(**P)
+        sim.moving_window(True, nr_plasma=0, ppc=0, seed=1)
+        sim.step(nsteps)
+        G = sim.download_fields()
+        A = sim.download_particles()
+    for t in range(1, nsteps + 1):
+        o.step()
+        while t * c >= shifts + 1:
+            shifts += 1
+            o.shift(0, 0, 1, shifts)
+    assert shifts == 2
+    mz = o.shape[2]
+    front = slice(mz - 8, mz)     # Fortran rows mz-7 .. mz: the front physical rows and ghosts
+    for n in FIELDS:
+        assert np.all(G[n][:, :, front] == 0.0), n
+        assert np.all(o.F[n][:, :, front] == 0.0), n
+        assert np.abs(G[n]).max() > 0.0, n
+    live = A["flv"] != 0
+    B = o.particles(live_only=True)
+    say(f"live particles GPU {int(live.sum())}, oracle {len(B['x'])}")
+    assert int(live.sum()) == len(B["x"]) == 8        # the 8 backward electrons left the box
+    assert np.all(A["z"][live] < 13.0)
+    for n in FIELDS:
+        d = np.abs(G[n] - o.F[n]).max() / np.abs(o.F[n]).max()
+        assert d <= (1e-9 if form == "fp64" else 1e-4), (n, d)

@@ -337,3 +337,38 @@ def test_modal_lehe_stays_bounded(dr, c):
     for slots in ((0,), (1, 2)):
         a, b = _vacuum_energy_halves(dr, c, slots, 8000)
         assert np.isfinite(a) and b <= 1.1 * a, (dr, c, slots, a, b)
+
+
+@pytest.mark.parametrize("zopen", [True, False])
+def test_open_z_keeps_the_back_away_from_the_front(zopen):
+    # the moving window's box has open z ends: fields in the back rows and electrons
+    # leaving through the back must not reach the front.  Periodic z (zopen False) copies
+    # the back rows into the front ghosts and wraps the electrons, which this test shows.
+    nr, nz, c = 12, 32, 0.45
+    o = CylModalOracle(nr, nz, dr=2.0, c=c, c_ompe=10.0, epc=4, cap=256)
+    o.set_zopen(zopen)
+    rng = np.random.default_rng(3)
+    mx, _, mz = o.shape
+    for n in FIELDS:
+        a = np.zeros(o.shape, order="F")
+        a[3:-3, :, 2:5] = 1e-2 * rng.normal(size=(mx - 6, 3, 3))   # Fortran rows 3..5
+        o.F[n][...] = a
+    n = 6
+    o.load_particles(np.full(n, -1.0), rng.uniform(5.0, mx - 6.0, n), rng.uniform(0, 2 * np.pi, n),
+                     rng.uniform(3.05, 3.3, n), np.zeros(n), np.zeros(n), np.full(n, -3.0),
+                     np.full(n, 2, dtype=np.int32), cap=256)
+    shifts = 0
+    for t in range(1, 7):
+        o.step()
+        while t * c >= shifts + 1:
+            shifts += 1
+            o.shift(0, 0, 1, shifts)
+    front = slice(mz - 8, mz)
+    reached = max(np.abs(o.F[k][:, :, front]).max() for k in FIELDS)
+    live = o.particles(live_only=True)
+    if zopen:
+        assert reached == 0.0
+        assert len(live["x"]) == 0        # the electrons left through the back
+    else:
+        assert reached > 1e-6
+        assert len(live["x"]) == n and np.all(live["z"] > mz // 2)   # wrapped to the front half
