@@ -12,7 +12,7 @@ value = particles * steps / wall time of the timed steps (FP32 split form).
 The FP64 form runs the same workload and is reported beside it.
 roofline: the push/deposit kernel, timed with HIP events on its own stream;
 algorithmic bytes per launch = particles * bytes per particle + cells * grid
-bytes (BYTES_PER_PARTICLE, GRID_BYTES below).  cpu_baseline: the FP64 C
+bytes (bytes_per_particle, GRID_BYTES below).  cpu_baseline: the FP64 C
 oracle (restated reference CPU path, OpenMP) on a bounded number of steps of
 the same workload.
 """
@@ -36,10 +36,19 @@ CONFIGS = {
     "weibel3d_128": (3, 128, 128, 128, 8, 0.3, 0.02),
 }
 
-# algorithmic bytes per particle update of the push kernel (read + write)
-BYTES_PER_PARTICLE = {"fp32_split": 2 * (4 + 3 * 4 + 3 * 4), "fp64": 7 * 8 + 4 + 6 * 8}
-# grid bytes per cell of the push kernel: read E,B (6 values), write J (3 FP64 values)
-GRID_BYTES = {"fp32_split": 6 * 4 + 3 * 8, "fp64": 6 * 8 + 3 * 8}
+# algorithmic bytes per particle update of the push kernel (read + write): the
+# record a particle needs; FP64 the reference SoA (q,x,y,z,u,v,w + flv read, x..w
+# written); split: cell word + in-plane FP32 offsets + FP32 momenta, species
+# implicit in the bin (2-D: cell, ox, oy, u, v, w; 3-D: also oz)
+def bytes_per_particle(form, ndim):
+    if form == "fp64":
+        return 7 * 8 + 4 + 6 * 8
+    return 2 * (4 + ndim * 4 + 3 * 4)
+
+
+# grid bytes per cell of the push kernel: read E,B (6 values), write J (3 values), at
+# the field precision s (8 FP64, 4 split)
+GRID_BYTES = {"fp32_split": 9 * 4, "fp64": 9 * 8}
 
 
 def measured_traffic(form, workload, tile, ndim):
@@ -57,7 +66,10 @@ def measured_traffic(form, workload, tile, ndim):
     return None
 
 
-def run_gpu(form, ndim, nx, ny, nz, epc, P, steps, warmup, calibrate=False, tile=0, curr_filter=0):
+def run_gpu(form, ndim, nx, ny, nz, epc, P, steps, warmup, calibrate=False, tile=0, curr_filter=0, late=None):
+    """Upload, `warmup` untimed steps, `steps` timed steps.  late = (start, n, total):
+    continue untimed to step `start`, time steps start+1 .. start+n, continue to step
+    `total`, and report the late window and the mean over all `total` steps."""
     from pictor_amd import Simulation
 
     sim = Simulation(nx, ny, nz, ndim=ndim, epc=epc, c=0.45, c_ompe=10.0, form=form, tile=tile,
@@ -67,6 +79,7 @@ def run_gpu(form, ndim, nx, ny, nz, epc, P, steps, warmup, calibrate=False, tile
     sim.synchronize()
     t_upload = time.time() - t0
     np_ = sim.particle_count()
+    t_run0 = time.perf_counter()
     sim.step(warmup)
     sim.synchronize()
     sim.timer_reset()
@@ -79,6 +92,27 @@ def run_gpu(form, ndim, nx, ny, nz, epc, P, steps, warmup, calibrate=False, tile
     step_ms, _ = sim.timer("step")
     diag = sim.diagnostics()
     mem = sim.memory_bytes()
+    late_res = None
+    if late:
+        start, n, total = late
+        done = warmup + steps
+        sim.step(start - done)
+        sim.synchronize()
+        sim.timer_reset()
+        t1 = time.perf_counter()
+        sim.step(n)
+        sim.synchronize()
+        dt_late = time.perf_counter() - t1
+        lpush, lnp = sim.timer("push")
+        sim.step(total - start - n)
+        sim.synchronize()
+        t_all = time.perf_counter() - t_run0
+        nre, nsp = sim.relayout_stats()
+        late_res = {"window": f"steps {start + 1}-{start + n}", "ms_per_step": dt_late / n * 1e3,
+                    "push_ms": lpush / max(lnp, 1), "value": np_ * n / dt_late,
+                    "mean_ms_per_step": t_all / total * 1e3, "mean_over_steps": total,
+                    "relayouts": nre, "spilled_particles": nsp,
+                    "particles_end": sim.particle_count()}
     if calibrate:
         # plain streaming kernels of known size for scaling HBM counters:
         # FP32 -> FP64 field conversion (read 4 B, write 8 B per cell) and
@@ -88,9 +122,9 @@ def run_gpu(form, ndim, nx, ny, nz, epc, P, steps, warmup, calibrate=False, tile
     sim.close()
     avg_push = push_ms / max(npush, 1) / 1e3
     ncell = nx * ny * (nz if ndim == 3 else 1)
-    bytes_launch = np_ * BYTES_PER_PARTICLE[form] + ncell * GRID_BYTES[form]
+    bytes_launch = np_ * bytes_per_particle(form, ndim) + ncell * GRID_BYTES[form]
     achieved = bytes_launch / avg_push
-    return {
+    out = {
         "value": np_ * steps / dt,
         "ms_per_step": dt / steps * 1e3,
         "particles": np_,
@@ -99,12 +133,18 @@ def run_gpu(form, ndim, nx, ny, nz, epc, P, steps, warmup, calibrate=False, tile
         "event_step_ms": step_ms / max(npush, 1),
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
-                     "bytes_per_launch": bytes_launch},
+                     "bytes_per_launch": bytes_launch,
+                     "bytes_per_particle": bytes_per_particle(form, ndim),
+                     "grid_bytes_per_cell": GRID_BYTES[form]},
         "diag": diag,
         "upload_s": t_upload,
         "device_bytes": mem,
         "tile": tile,
     }
+    if late_res:
+        late_res["frac"] = bytes_launch / (late_res["push_ms"] / 1e3) / HBM_PEAK
+        out["late"] = late_res
+    return out
 
 
 def run_cpu(ndim, nx, ny, nz, epc, P, steps, threads):
@@ -388,7 +428,7 @@ def cyl_report(args):
     push_ms /= max(npush, 1)
     value = npart * args.steps / wall
     mx, my, mz = nr + 5, nth + 5, (nz + 5 if ndim == 3 else 1)
-    algo = npart * BYTES_PER_PARTICLE["fp64"] + mx * my * mz * GRID_BYTES["fp64"]
+    algo = npart * bytes_per_particle("fp64", 3) + mx * my * mz * GRID_BYTES["fp64"]
     achieved = algo / (push_ms * 1e-3)
     cpu = None
     if args.cpu_steps != 0:
@@ -601,6 +641,46 @@ def atomics_report(iters=4096):
             "lane_ops_per_cu_clock": {k: out[8 + i] for i, k in enumerate(names)}}
 
 
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    try:
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        omp = 0
+    return {"nproc": os.cpu_count() or 1, "affinity": aff, "cpu_model": model, "omp_threads": omp}
+
+
+def c1_full_cpu(threads):
+    """The oracle on the 256^2 CPU workload in full: 16 ppc/species, 1000 FP64 steps."""
+    from pictor_amd import setup_weibel
+
+    ndim, nx, ny, nz, epc, drift, uth = CONFIGS["weibel2d_256"]
+    P = setup_weibel(nx, ny, nz, ndim=ndim, epc=epc, drift=drift, uth=uth, seed=12345)
+    from oracle.pyoracle import Oracle
+
+    o = Oracle(ndim, nx, ny, nz, epc=epc, c=0.45, c_ompe=10.0, nthreads=threads, cap=int(len(P["x"]) * 1.05) + 4096)
+    o.load_particles(**P)
+    steps = 1000
+    t0 = time.perf_counter()
+    o.run(1, steps)
+    dt = time.perf_counter() - t0
+    return {"workload": "weibel2d_256: 256x256, 16 ppc/species, 2 species, 1000 steps, FP64",
+            "particles": len(P["x"]), "steps": steps, "seconds": dt, "value": len(P["x"]) * steps / dt,
+            "unit": "particle-updates/s", "threads": threads}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -609,7 +689,12 @@ def main():
     ap.add_argument("--config", default="weibel2d_2048", choices=sorted(CONFIGS))
     ap.add_argument("--forms", default="fp32_split,fp64")
     ap.add_argument("--cpu-steps", type=int, default=-1, help="oracle steps for the CPU baseline (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads (0 = OMP_NUM_THREADS, the host cores given to one GPU, else all cores)")
+    ap.add_argument("--c1-full", type=int, default=1, help="also time the oracle on the 256^2 workload in full")
+    ap.add_argument("--late", default="1000,50,2000",
+                    help="START,N,TOTAL: for the first form also time steps START+1..START+N and the mean over "
+                         "TOTAL steps (empty = skip)")
     ap.add_argument("--calibrate", action="store_true", help="also run plain copy kernels (counter scaling)")
     ap.add_argument("--tile", type=int, default=0, help="tile edge in cells (0 = library default)")
     ap.add_argument("--curr-filter", type=int, default=0, help="passes of the 1-2-1 current filter per step")
@@ -673,25 +758,37 @@ def main():
         return
 
     forms = [f for f in args.forms.split(",") if f]
+    late = tuple(int(v) for v in args.late.split(",")) if args.late else None
     res = {}
-    for form in forms:
+    for i, form in enumerate(forms):
         res[form] = run_gpu(form, ndim, nx, ny, nz, epc, P, args.steps, args.warmup, args.calibrate, args.tile,
-                            args.curr_filter)
+                            args.curr_filter, late=late if i == 0 else None)
         r = res[form]
         r["roofline"]["traffic"] = measured_traffic(form, args.config, r["tile"], ndim)
         print(f"# {form}: {r['value']:.4e} particle-updates/s, {r['ms_per_step']:.3f} ms/step, "
               f"push {r['push_ms']:.3f} ms ({r['roofline']['frac'] * 100:.1f}% HBM)", file=sys.stderr, flush=True)
+        if "late" in r:
+            print(f"# {form} late: {json.dumps(r['late'])}", file=sys.stderr, flush=True)
 
     cpu = None
     big = len(P["x"]) > 100_000_000
     cpu_steps = args.cpu_steps if args.cpu_steps >= 0 else (2 if ndim == 3 and big else (5 if big else 20))
     if cpu_steps > 0:
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        host = host_info()
+        threads = args.cpu_threads or host["omp_threads"] or host["nproc"]
+        threads = max(1, min(threads, host["nproc"]))
         v, dt = run_cpu(ndim, nx, ny, nz, epc, P, cpu_steps, threads)
         cpu = {"value": v, "unit": "particle-updates/s", "cores": threads, "kind": "port",
                "sample": f"{cpu_steps} full steps of the same workload ({len(P['x'])} particles, FP64 oracle, "
-                         f"OpenMP) after 1 warm-up step, {dt:.1f}s"}
+                         f"OpenMP) after 1 warm-up step, {dt:.1f}s",
+               "nproc": host["nproc"], "affinity_cores": host["affinity"], "cpu_model": host["cpu_model"],
+               "threads_note": "OpenMP threads = OMP_NUM_THREADS, the host cores given to one GPU"
+                               if host["omp_threads"] and not args.cpu_threads else "--cpu-threads"}
         print(f"# cpu: {v:.4e} particle-updates/s on {threads} threads ({dt:.1f}s)", file=sys.stderr, flush=True)
+        if args.c1_full:
+            c1 = c1_full_cpu(threads)
+            cpu["c1_full"] = c1
+            print(f"# cpu C1 in full: {json.dumps(c1)}", file=sys.stderr, flush=True)
 
     head = forms[0]
     r = res[head]
@@ -722,6 +819,8 @@ def main():
                                 "dtype": "f64" if form == "fp64" else "f32"}
     if "fp64" in res and head != "fp64":
         line["split_over_fp64"] = r["value"] / res["fp64"]["value"]
+    if "late" in r:
+        line["late"] = r["late"]
     line["detail"] = {f: {k: res[f][k] for k in ("push_ms", "field_ms", "event_step_ms", "upload_s", "device_bytes",
                                                   "diag")} for f in forms}
     print(json.dumps(line), flush=True)

@@ -86,7 +86,8 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out);
 int pictor_finalize(pictor_ctx *ctx);
 const char *pictor_last_error(void);
 /* Derived scalars: {qi, qe, masse, massi, ompe, fldc, xlen, ylen, zlen, mx, my, mz,
- * tile edge, number of tiles, 0, 0}. */
+ * tile edge, number of tiles, re-layouts after a bin overflow since init,
+ * particles those re-layouts moved from the spill buffer}. */
 int pictor_get_scalars(pictor_ctx *ctx, double out[16]);
 
 /* ---- host <-> device copies in the reference layout (the CUDA Fortran array
@@ -105,6 +106,20 @@ int pictor_particle_count(pictor_ctx *ctx, int64_t *n);
 int pictor_download_particles(pictor_ctx *ctx, int64_t cap, double *q, double *x, double *y,
                               double *z, double *u, double *v, double *w, int32_t *flv,
                               int32_t *tile, int64_t *n_out);
+
+/* ---- restart files: SaveRestartData, src/io/savedata.F90:73-141 (writers :940-1210),
+ *      and the reload, src/io/reload.F90:17-338 (restart branch src/MAIN.F90:35-39).
+ * One binary file (no HDF5 here) with the reference's restart content: the run
+ * configuration and step t (param_<proc>_<t>), E and B in the form's own precision
+ * (fld_<proc>_<t>; the split form's FP32 B and FP32 (hi, lo) pair of E, so that a
+ * restarted split run continues bit for bit), and the particles in the reference SoA
+ * q, x, y, z, u, v, w, flv (prtl_<proc>_<t>), which holds the split record exactly.
+ * The modal cylinder's moving window and sort schedule are saved too.  J is not saved
+ * (each step resets it).  The file is written to path.tmp and renamed when complete.
+ * pictor_load_state needs a context of the same configuration (the tile edge may
+ * differ) and returns the saved t. */
+int pictor_save_state(pictor_ctx *ctx, const char *path, int64_t t);
+int pictor_load_state(pictor_ctx *ctx, const char *path, int64_t *t);
 
 /* ---- the step, one call per MAIN.F90 phase ---- */
 /* UpdateBfieldHalf + SendRecvFlds(B): em_update.F90:41-77, comm_fld.F90:75-91 */

@@ -36,6 +36,7 @@ EXPORTS = (
     "pictor_setup_weibel", "pictor_adjoint_forward", "pictor_adjoint_backward", "pictor_adjoint_stats",
     "pictor_adjoint_engine", "pictor_adjoint_release", "pictor_cyl_plasma", "pictor_cyl_laser",
     "pictor_moving_window", "pictor_probe_atomics", "pictor_cyl_push_scheme", "pictor_cyl_sort_period",
+    "pictor_save_state", "pictor_load_state",
 )
 
 
@@ -103,6 +104,8 @@ def lib():
     L.pictor_cyl_sort_period.argtypes = [c_vp, i32]
     L.pictor_cyl_laser.argtypes = [c_vp] + [ctypes.c_double] * 5
     L.pictor_moving_window.argtypes = [c_vp, i32, i32, i32, ctypes.c_uint64]
+    L.pictor_save_state.argtypes = [c_vp, ctypes.c_char_p, i64]
+    L.pictor_load_state.argtypes = [c_vp, ctypes.c_char_p, ctypes.POINTER(i64)]
     for name in EXPORTS:
         if name != "pictor_last_error":
             getattr(L, name).restype = ctypes.c_int

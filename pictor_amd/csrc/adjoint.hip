@@ -100,21 +100,24 @@ __global__ __launch_bounds__(256) void k_eng_push(AdjP ap, const typename V4<R>:
     unpack<R>(ap, cs, ci, cj, s, o);
     typename V4<R>::T gc[6];
     corners<R>(nE, nB, o, mx, gc);
+    // the split form (R = float) does not store z: every step starts at kZMid
+    constexpr bool kSplit = std::is_same<R, float>::value;
+    const R z0 = kSplit ? (R)kZMid : in.oz[p];
     PState<R> q;
-    push_fwd<R>(ap, s, in.ox[p], in.oy[p], in.oz[p], in.u[p], in.v[p], in.w[p], gc, q);
+    push_fwd<R>(ap, s, in.ox[p], in.oy[p], z0, in.u[p], in.v[p], in.w[p], gc, q);
     int di = q.d[0], dj = q.d[1];
     // stored offsets; a position rounded onto the upper face belongs to the next cell
     R ox = q.x1[0] - (R)di;
     if (ox >= (R)1) { ox = (R)0; di += 1; }
     R oy = q.x1[1] - (R)dj;
     if (oy >= (R)1) { oy = (R)0; dj += 1; }
-    R oz = q.x1[2] - (R)q.d[2];
+    R oz = kSplit ? (R)kZMid : q.x1[2] - (R)q.d[2];
     if (oz >= (R)1) oz = (R)0;
     double a[8], b[8];
     int ei = di, ej = dj;   // cell of the second half-segment
     if constexpr (std::is_same<R, float>::value) {
         // the split form deposits as the production kernel: FP64 to the stored position
-        pushdev::zigzag2d_split(sp2(ap.qs, s), in.ox[p], in.oy[p], in.oz[p], ox, oy, q.x1[2], di, dj, a, b);
+        pushdev::zigzag2d_split(sp2(ap.qs, s), in.ox[p], in.oy[p], kZMid, ox, oy, q.x1[2], di, dj, a, b);
     } else {
         seg_values<R>(q.F1, q.W1, a);
         seg_values<R>(q.F2, q.W2, b);
@@ -164,8 +167,9 @@ __global__ __launch_bounds__(256) void k_eng_adjoint(AdjP ap, const typename V4<
     unpack<R>(ap, st.cs[p], ci, cj, s, o);
     typename V4<R>::T gc[6];
     corners<R>(nE, nB, o, mx, gc);
+    constexpr bool kSplit = std::is_same<R, float>::value;   // no z in the split form (kZMid)
     PState<R> q;
-    push_fwd<R>(ap, s, st.ox[p], st.oy[p], st.oz[p], st.u[p], st.v[p], st.w[p], gc, q);
+    push_fwd<R>(ap, s, st.ox[p], st.oy[p], kSplit ? (R)kZMid : st.oz[p], st.u[p], st.v[p], st.w[p], gc, q);
     const R js = (R)(-ap.jscale);
     R jb1[8], jb2[8];
     jb1[0] = js * (R)Ebx[o]; jb1[1] = js * (R)Ebx[o + mx];
@@ -175,7 +179,7 @@ __global__ __launch_bounds__(256) void k_eng_adjoint(AdjP ap, const typename V4<
     jb2[0] = js * (R)Ebx[o2]; jb2[1] = js * (R)Ebx[o2 + mx];
     jb2[2] = js * (R)Eby[o2]; jb2[3] = js * (R)Eby[o2 + 1];
     jb2[4] = js * (R)Ebz[o2]; jb2[5] = js * (R)Ebz[o2 + 1]; jb2[6] = js * (R)Ebz[o2 + mx]; jb2[7] = js * (R)Ebz[o2 + mx + 1];
-    const R x1b[3] = {adj.ox[p], adj.oy[p], adj.oz[p]};
+    const R x1b[3] = {adj.ox[p], adj.oy[p], kSplit ? (R)0 : adj.oz[p]};
     const R u1b[3] = {adj.u[p], adj.v[p], adj.w[p]};
     R x0b[3], unb[3], PEb[3], PBb[3];
     reverse<R>(ap, s, q, gc, jb1, jb2, x1b, u1b, x0b, unb, PEb, PBb);
@@ -190,7 +194,7 @@ __global__ __launch_bounds__(256) void k_eng_adjoint(AdjP ap, const typename V4<
         unsafeAtomicAdd(&gBy[corner[k]], (double)(w * PBb[1]));
         unsafeAtomicAdd(&gBz[corner[k]], (double)(w * PBb[2]));
     }
-    adj.ox[p] = x0b[0]; adj.oy[p] = x0b[1]; adj.oz[p] = x0b[2];
+    adj.ox[p] = x0b[0]; adj.oy[p] = x0b[1]; adj.oz[p] = kSplit ? (R)0 : x0b[2];
     adj.u[p] = unb[0]; adj.v[p] = unb[1]; adj.w[p] = unb[2];
 }
 

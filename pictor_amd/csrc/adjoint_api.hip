@@ -70,12 +70,13 @@ hipError_t alloc_split_state(PartSplit &p, long long cap, bool with_id, bool wit
     hipError_t e;
     p = PartSplit{};
     if ((e = dalloc(&p.cell, cap)) != hipSuccess) return e;
-    float **arr[6] = {&p.ox, &p.oy, &p.oz, &p.u, &p.v, &p.w};
-    for (int i = 0; i < 6; i++)
+    // the 2-D split record: no z offset (kZMid)
+    float **arr[5] = {&p.ox, &p.oy, &p.u, &p.v, &p.w};
+    for (int i = 0; i < 5; i++)
         if ((e = dalloc(arr[i], cap)) != hipSuccess) return e;
     if (with_id && (e = dalloc(&p.id, cap)) != hipSuccess) return e;
     if (with_sp && (e = dalloc(&p.sp, cap)) != hipSuccess) return e;
-    bytes += cap * (28LL + (with_id ? 4 : 0) + (with_sp ? 4 : 0));
+    bytes += cap * (24LL + (with_id ? 4 : 0) + (with_sp ? 4 : 0));
     return hipSuccess;
 }
 

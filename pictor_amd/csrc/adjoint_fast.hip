@@ -61,7 +61,7 @@ __device__ inline void vbin_of(int v, int nsp, int &s, int &d)
 struct AdjRec {
     unsigned slot;
     int s, to, cell;
-    float ox, oy, oz, u, v, w;
+    float ox, oy, u, v, w;
 };
 
 __device__ inline void load_rec(const PartSplit &st, const int *dst, AdjRec &r)
@@ -69,7 +69,7 @@ __device__ inline void load_rec(const PartSplit &st, const int *dst, AdjRec &r)
     const unsigned sl = r.slot;
     r.to = dst[sl];
     r.cell = st.cell[sl];
-    r.ox = st.ox[sl]; r.oy = st.oy[sl]; r.oz = st.oz[sl];
+    r.ox = st.ox[sl]; r.oy = st.oy[sl];
     r.u = st.u[sl]; r.v = st.v[sl]; r.w = st.w[sl];
 }
 
@@ -161,7 +161,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1024 / BS, 1
     for (int k = (int)threadIdx.x; k < total; k += BS) {
         const AdjRec cur = nxt;
         const int t = cur.to >= 0 ? cur.to : 0;
-        const float a[6] = {ain.ox[t], ain.oy[t], ain.oz[t], ain.u[t], ain.v[t], ain.w[t]};
+        // z is not stored in the 2-D split form (kZMid): no adjoint flows through it
+        const float a[6] = {ain.ox[t], ain.oy[t], 0.0f, ain.u[t], ain.v[t], ain.w[t]};
         slot_of(k + BS < total ? k + BS : k, nxt.slot, nxt.s);
         load_rec(st, dst, nxt);
         const int ci = cur.cell & 0xFFFF, cj = cur.cell >> 16;
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1024 / BS, 1
         const int lc = (jj0 - 2) * T + (ii0 - 2);
         const float4 gc[6] = {sC[6 * lc], sC[6 * lc + 1], sC[6 * lc + 2], sC[6 * lc + 3], sC[6 * lc + 4], sC[6 * lc + 5]};
         PState<float> q;
-        push_fwd<float>(ap, cur.s, cur.ox, cur.oy, cur.oz, cur.u, cur.v, cur.w, gc, q);
+        push_fwd<float>(ap, cur.s, cur.ox, cur.oy, kZMid, cur.u, cur.v, cur.w, gc, q);
         const int e0 = jj0 * W + ii0;
         const int e2 = e0 + min(max(q.d[1], -1), 1) * W + min(max(q.d[0], -1), 1);
         const float jb1[8] = {sJb[e0], sJb[e0 + W], sJb[WW + e0], sJb[WW + e0 + 1],
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1024 / BS, 1
             atomicAdd(&sG[5 * NN + nc[m]], (GA)(w * PBb[2]));
         }
         const unsigned slot = cur.slot;
-        aout.ox[slot] = ok ? x0b[0] : 0.0f; aout.oy[slot] = ok ? x0b[1] : 0.0f; aout.oz[slot] = ok ? x0b[2] : 0.0f;
+        aout.ox[slot] = ok ? x0b[0] : 0.0f; aout.oy[slot] = ok ? x0b[1] : 0.0f; aout.oz[slot] = 0.0f;
         aout.u[slot] = ok ? unb[0] : 0.0f; aout.v[slot] = ok ? unb[1] : 0.0f; aout.w[slot] = ok ? unb[2] : 0.0f;
     }
     __syncthreads();

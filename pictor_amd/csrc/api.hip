@@ -57,13 +57,14 @@ hipError_t alloc_f64(PartF64 &p, long long n)
     return dalloc(&p.flv, n);
 }
 
-hipError_t alloc_split(PartSplit &p, long long n, bool with_sp)
+// oz only in 3-D: the 2-D split record has no z (kZMid)
+hipError_t alloc_split(PartSplit &p, long long n, bool with_sp, int ndim)
 {
     hipError_t e;
     if ((e = dalloc(&p.cell, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.ox, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.oy, n)) != hipSuccess) return e;
-    if ((e = dalloc(&p.oz, n)) != hipSuccess) return e;
+    if (ndim == 3 && (e = dalloc(&p.oz, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.u, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.v, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.w, n)) != hipSuccess) return e;
@@ -84,7 +85,7 @@ int ensure_buffer(pictor_ctx *c, int which, long long need)
         HIPTRY(alloc_f64(c->pf[which], n));
     } else {
         free_split(c->ps[which]);
-        HIPTRY(alloc_split(c->ps[which], n, false));
+        HIPTRY(alloc_split(c->ps[which], n, false, c->g.ndim));
     }
     c->buf_cap[which] = n;
     return PICTOR_OK;
@@ -100,7 +101,7 @@ int ensure_spill(pictor_ctx *c, int need)
     } else {
         free_split(c->spill_s);
         c->spill_cap = 0;
-        HIPTRY(alloc_split(c->spill_s, need, true));
+        HIPTRY(alloc_split(c->spill_s, need, true, c->g.ndim));
     }
     dfree(c->rl_grp);
     c->rl_grp = nullptr;
@@ -296,7 +297,11 @@ int finish_push(pictor_ctx *c, hipEvent_t ev)
         return fail(PICTOR_ERR_CAPACITY, "current at a node exceeded the exact fixed-point range (|J| >= 4096 q cell)");
     if ((err & kErrCapacity) || nspill > c->spill_cap)
         return fail(PICTOR_ERR_CAPACITY, "particle spill buffer exhausted");
-    if (nspill > 0) return relayout(c, nspill);
+    if (nspill > 0) {
+        c->n_relayout++;
+        c->n_spilled += nspill;
+        return relayout(c, nspill);
+    }
     return PICTOR_OK;
 }
 
@@ -677,7 +682,8 @@ int pictor_get_scalars(pictor_ctx *c, double out[16])
     out[4] = c->cfg.c / c->cfg.c_ompe; out[5] = c->ph.fldc;
     out[6] = c->ph.xlen; out[7] = c->ph.ylen; out[8] = c->ph.zlen;
     out[9] = c->g.mx; out[10] = c->g.my; out[11] = c->g.mz;
-    out[12] = c->tp.T; out[13] = c->tp.ntiles; out[14] = 0; out[15] = 0;
+    out[12] = c->tp.T; out[13] = c->tp.ntiles;
+    out[14] = (double)c->n_relayout; out[15] = (double)c->n_spilled;
     return PICTOR_OK;
 }
 
@@ -848,7 +854,7 @@ int pictor_download_particles(pictor_ctx *c, int64_t cap, double *q, double *x, 
     if (c->form == PICTOR_FORM_FP64) {
         launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], d_off, nullptr, st, c->stream);
     } else {
-        HIPTRY(alloc_split(ss, n, true));
+        HIPTRY(alloc_split(ss, n, true, c->g.ndim));
         launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], d_off, nullptr, ss, c->stream);
         launch_split_to_f64(c->g, c->ph, ss, n, st, c->stream);
     }

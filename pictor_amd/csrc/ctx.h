@@ -109,6 +109,8 @@ struct pictor_ctx {
     long long t_n[3] = {0, 0, 0};
 
     long long np = 0;
+    long long n_relayout = 0;   // re-layouts after a spill (bins overflowed) since init
+    long long n_spilled = 0;    // particles those re-layouts took from the spill buffer
 
     // cylindrical geometry: particles in the caller's order (cyl.hip)
     bool cyl = false;

@@ -111,12 +111,12 @@ __global__ void k_raw_scatter_split(GridP g, TileP tp, PartF64 raw, long long n,
     split_coord(raw.x[i], g.nx, cx, ox);
     split_coord(raw.y[i], g.ny, cy, oy);
     if (g.ndim == 3) split_coord(raw.z[i], g.nz, cz, oz);
-    else { const double f = floor(raw.z[i]); oz = (float)(raw.z[i] - f); if (oz >= 1.0f) oz = 0.0f; }
     const int grp = t;
     const int slot = atomicAdd(&fill[grp], 1);
     const long long o = cbase[grp] + perm_slot(slot, gtotal, grp);
     out.cell[o] = cell_pack(g.ndim, cx, cy, cz);
-    out.ox[o] = ox; out.oy[o] = oy; out.oz[o] = oz;
+    out.ox[o] = ox; out.oy[o] = oy;
+    if (g.ndim == 3) out.oz[o] = oz;
     out.u[o] = (float)raw.u[i]; out.v[o] = (float)raw.v[i]; out.w[o] = (float)raw.w[i];
     if (out.id) out.id[o] = (int)i;
     (void)s;
@@ -181,7 +181,8 @@ __global__ void k_compact_split(TileP tp, PartSplit src, const long long *base, 
 {
     const int s = blockIdx.x % tp.nsp;
     compact_group(tp, base, cnt, dst_off[blockIdx.x], gtotal, [&](long long a, long long b) {
-        dst.cell[b] = src.cell[a]; dst.ox[b] = src.ox[a]; dst.oy[b] = src.oy[a]; dst.oz[b] = src.oz[a];
+        dst.cell[b] = src.cell[a]; dst.ox[b] = src.ox[a]; dst.oy[b] = src.oy[a];
+        if (dst.oz) dst.oz[b] = src.oz[a];
         dst.u[b] = src.u[a]; dst.v[b] = src.v[a]; dst.w[b] = src.w[a];
         if (dst.sp) dst.sp[b] = s;
     });
@@ -205,11 +206,12 @@ __global__ void k_spill_scatter_split(PartSplit sp, int n, const int *grp_of, co
     if (i >= n) return;
     const int grp = grp_of[i];
     const long long o = dst_off[grp] + perm_slot(atomicAdd(&fill[grp], 1), gtotal, grp);
-    dst.cell[o] = sp.cell[i]; dst.ox[o] = sp.ox[i]; dst.oy[o] = sp.oy[i]; dst.oz[o] = sp.oz[i];
+    dst.cell[o] = sp.cell[i]; dst.ox[o] = sp.ox[i]; dst.oy[o] = sp.oy[i];
+    if (dst.oz) dst.oz[o] = sp.oz[i];
     dst.u[o] = sp.u[i]; dst.v[o] = sp.v[i]; dst.w[o] = sp.w[i];
 }
 
-// split -> reference FP64 record (x = 3 + cell + offset; 2-D z = 1 + offset)
+// split -> reference FP64 record (x = 3 + cell + offset; 2-D z = 1 + kZMid, z is not stored)
 __global__ void k_split_to_f64(GridP g, PhysP ph, PartSplit src, long long n, PartF64 dst)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -219,7 +221,7 @@ __global__ void k_split_to_f64(GridP g, PhysP ph, PartSplit src, long long n, Pa
     const int s = src.sp[i];
     dst.x[i] = (double)(3 + cx) + (double)src.ox[i];
     dst.y[i] = (double)(3 + cy) + (double)src.oy[i];
-    dst.z[i] = (g.ndim == 3) ? (double)(3 + cz) + (double)src.oz[i] : 1.0 + (double)src.oz[i];
+    dst.z[i] = (g.ndim == 3) ? (double)(3 + cz) + (double)src.oz[i] : 1.0 + (double)kZMid;
     dst.u[i] = src.u[i]; dst.v[i] = src.v[i]; dst.w[i] = src.w[i];
     dst.q[i] = ph.flvq[s];
     dst.flv[i] = s + 1;
@@ -321,7 +323,7 @@ __global__ __launch_bounds__(256) void k_particle_diag_split(GridP g, PhysP ph, 
                            cell_unpack(g.ndim, p.cell[o], cx, cy, cz);
                            x = (double)(3 + cx) + (double)p.ox[o];
                            y = (double)(3 + cy) + (double)p.oy[o];
-                           z = (g.ndim == 3) ? (double)(3 + cz) + (double)p.oz[o] : 1.0 + (double)p.oz[o];
+                           z = (g.ndim == 3) ? (double)(3 + cz) + (double)p.oz[o] : 1.0 + (double)kZMid;
                            u = p.u[o]; v = p.v[o]; w = p.w[o];
                            q = ph.flvq[s]; flv = s + 1;
                        });

@@ -244,7 +244,7 @@ __global__ __launch_bounds__(BS) void k_push2d_f64(GridP g, PhysP ph, TileP tp, 
 // ---------------------------------------------------------------------------
 struct SplitRec {
     int cell, s;
-    float ox, oy, oz, u, v, w;
+    float ox, oy, u, v, w;
     int slot, id;   // input slot and particle index (TRACK only)
 };
 
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
         const unsigned slot = sSrc[v] + (unsigned)(k - sStart[v]);
         const unsigned off = slot * 4u;
         r.cell = ldo(A.cell, off);
-        r.ox = ldo(A.ox, off); r.oy = ldo(A.oy, off); r.oz = ldo(A.oz, off);
+        r.ox = ldo(A.ox, off); r.oy = ldo(A.oy, off);
         r.u = ldo(A.u, off); r.v = ldo(A.v, off); r.w = ldo(A.w, off);
         r.s = sVs[v];
         if (TRACK) { r.slot = (int)slot; r.id = ldo(A.id, off); }
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
         const SplitRec cur = nxt;
         if (k + BS < total) load(k + BS, nxt);   // prefetch the next particle of this thread
         int d = -1, ncell = cur.cell;
-        float ox = cur.ox, oy = cur.oy, oz = cur.oz, un = cur.u, vn = cur.v, wn = cur.w;
+        float ox = cur.ox, oy = cur.oy, un = cur.u, vn = cur.v, wn = cur.w;
         if (active) {
             const int ci = cur.cell & 0xFFFF, cj = cur.cell >> 16;
             const int ii0 = ci - cx0, jj0 = cj - cy0;
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                 atomicOr(sp.err, (int)kErrBounds);
                 d = cur.s * 9 + 4;
             } else {
-                const float x0 = cur.ox, y0 = cur.oy, z0 = cur.oz;
+                const float x0 = cur.ox, y0 = cur.oy;
                 const float qm = sQm[cur.s], qs = sQs[cur.s];
                 const float w1 = (1.0f - x0) * (1.0f - y0), w2 = x0 * (1.0f - y0);
                 const float w3 = (1.0f - x0) * y0, w4 = x0 * y0;
@@ -401,20 +401,17 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                 gg = sqc * __builtin_amdgcn_rsqf(sqc + u0 * u0 + v0 * v0 + w0 * w0);
                 const float x1 = x0 + un * gg * idx_;
                 const float y1 = y0 + vn * gg * idy_;
-                const float z1 = z0 + wn * gg * idz_;
+                const float z1 = kZMid + wn * gg * idz_;   // z is not stored (kZMid)
                 int di = (int)floorf(x1), dj = (int)floorf(y1);
-                const int dk = (int)floorf(z1);
                 // stored offsets; a position rounded onto the upper face belongs to the next cell
                 ox = x1 - (float)di;
                 if (ox >= 1.0f) { ox = 0.0f; di += 1; }
                 oy = y1 - (float)dj;
                 if (oy >= 1.0f) { oy = 0.0f; dj += 1; }
-                oz = z1 - (float)dk;
-                if (oz >= 1.0f) oz = 0.0f;
                 // the two zigzag half-segments to the stored position: 8 node values
                 // each, as exact integer counts of 2^-36 units
                 double a[8], b[8];
-                zigzag2d_split((double)qs, x0, y0, z0, ox, oy, z1, di, dj, a, b);
+                zigzag2d_split((double)qs, x0, y0, kZMid, ox, oy, z1, di, dj, a, b);
                 // a particle that stays in its cell deposits both halves on the same
                 // 8 nodes: add them first (exact) and halve the LDS atomics
                 const bool cross = (di != 0) || (dj != 0);
@@ -456,7 +453,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
         if (active) {
             if (slot < sDstCap[d]) {
                 const unsigned off = (sDstBase[d] + (unsigned)slot) * 4u;
-                sto(Bo.cell, off, ncell); sto(Bo.ox, off, ox); sto(Bo.oy, off, oy); sto(Bo.oz, off, oz);
+                sto(Bo.cell, off, ncell); sto(Bo.ox, off, ox); sto(Bo.oy, off, oy);
                 sto(Bo.u, off, un); sto(Bo.v, off, vn); sto(Bo.w, off, wn);
                 if (TRACK) {
                     sto(Bo.id, off, cur.id);
@@ -465,7 +462,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
             } else {
                 const int si = atomicAdd(sp.count, 1);
                 if (si < sp.cap) {
-                    spill.cell[si] = ncell; spill.ox[si] = ox; spill.oy[si] = oy; spill.oz[si] = oz;
+                    spill.cell[si] = ncell; spill.ox[si] = ox; spill.oy[si] = oy;
                     spill.u[si] = un; spill.v[si] = vn; spill.w[si] = wn; spill.sp[si] = cur.s;
                 } else {
                     atomicOr(sp.err, (int)kErrCapacity);

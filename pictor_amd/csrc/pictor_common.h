@@ -71,9 +71,19 @@ __host__ __device__ inline void cell_unpack(int ndim, int c, int &i, int &j, int
     else { i = c & 0x7FF; j = (c >> 11) & 0x7FF; k = (c >> 22) & 0x1FF; }
 }
 
+// 2-D split form: z is not stored.  A 2-D run is invariant along z; the
+// reference carries z only to place the relay point of its Jz zigzag
+// (movdep.F90:185), which splits Jz between the two half-segments.  The split
+// form starts every particle's step at mid-cell in z (offset kZMid) and never
+// stores the advanced z: with |dz| <= c dt < 1/2 cell per step the relay stays at
+// the midpoint, both halves carry half the z charge flux, and the record is the
+// cell word, two offsets and three momenta (24 B).  Downloads report z = 1.5,
+// the centre of the reference's 2-D range [1, 2).
+constexpr float kZMid = 0.5f;
+
 struct PartSplit {
     int *cell;
-    float *ox, *oy, *oz, *u, *v, *w;
+    float *ox, *oy, *oz, *u, *v, *w;   // oz: 3-D only (null in 2-D)
     int *sp;   // species id, used only by the spill / raw buffers
     int *id;   // caller's index of the particle, carried only by the adjoint engine's buffers
 };

@@ -8,6 +8,7 @@ Arrays are numpy arrays in the reference layout: fields (mx, my, mz) in
 Fortran order, particles 1-D SoA.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -56,6 +57,24 @@ class Simulation:
         self.tile = int(sc[12])
         self.ntiles = int(sc[13])
         self.t = 0
+
+    def save_state(self, path, t=None):
+        """SaveRestartData (savedata.F90:73-141): the context's state to one file; t defaults to self.t."""
+        t = self.t if t is None else int(t)
+        check(lib().pictor_save_state(self._ctx, os.fsencode(path), t))
+
+    def load_state(self, path):
+        """The reload of reload.F90:17-338 into this context (same configuration); returns and sets t."""
+        t = ctypes.c_int64(0)
+        check(lib().pictor_load_state(self._ctx, os.fsencode(path), ctypes.byref(t)))
+        self.t = int(t.value)
+        return self.t
+
+    def relayout_stats(self):
+        """(re-layouts after a tile-bin overflow, particles taken from the spill buffer) since init."""
+        sc = np.zeros(16)
+        check(lib().pictor_get_scalars(self._ctx, _p(sc)))
+        return int(sc[14]), int(sc[15])
 
     def close(self):
         if self._ctx:

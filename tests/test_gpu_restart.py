@@ -1,0 +1,148 @@
+"""Restart files (SaveRestartData, savedata.F90:73-141 and :940-1210; the reload of
+reload.F90:17-338): a run of N steps saved, loaded into a fresh context and run M
+more steps ends where a straight run of N + M steps ends.
+
+Bars: the FP32 split form continues bit for bit (fields, J, diagnostics and the
+particle multiset), since its state round-trips exactly and its current does not
+depend on the particle order; the FP64 form within 1e-13 of the largest value (its
+current sums FP64 atomics in a run-dependent order).  Every geometry: 2-D and 3-D
+Cartesian, the reference's r-theta cylinder, and the modal cylinder with the moving
+window, which also saves its window position and sort schedule.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from pictor_amd import PictorError, Simulation, setup_weibel
+from tests._util import assert_same_multiset, say, sort_particles
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(A, B, exact, what):
+    for k in A:
+        a, b = np.asarray(A[k]), np.asarray(B[k])
+        if exact:
+            assert np.array_equal(a, b), (what, k)
+        else:
+            scale = max(np.abs(b).max(), 1e-300)
+            assert np.abs(a - b).max() <= 1e-13 * scale, (what, k, np.abs(a - b).max() / scale)
+
+
+def _close_multiset(A, B):
+    a, b = sort_particles(A), sort_particles(B)
+    assert len(a["x"]) == len(b["x"])
+    for k in ("x", "y", "z", "u", "v", "w"):
+        scale = max(np.abs(b[k]).max(), 1e-300)
+        assert np.abs(a[k] - b[k]).max() <= 1e-12 * scale, k
+    assert np.array_equal(a["flv"], b["flv"])
+
+
+def _roundtrip(make, load, n1, n2, exact, tmp_path, name):
+    path = str(tmp_path / f"{name}.state")
+    with make() as sim:
+        load(sim)
+        sim.step(n1 + n2)
+        F_ref, P_ref, d_ref = sim.download_fields(), sim.download_particles(), sim.diagnostics()
+    with make() as sim:
+        load(sim)
+        sim.step(n1)
+        sim.save_state(path)
+        assert os.path.exists(path) and not os.path.exists(path + ".tmp")
+    with make() as sim:
+        assert sim.load_state(path) == n1
+        sim.step(n2)
+        F, P, d = sim.download_fields(), sim.download_particles(), sim.diagnostics()
+    say(f"{name}: restarted at {n1}, {n2} more steps; B energy {d['B_energy']!r} vs {d_ref['B_energy']!r}")
+    _compare(F, F_ref, exact, name)
+    if exact:
+        assert_same_multiset(P, P_ref)
+        for k in d:
+            assert d[k] == d_ref[k], (name, k)
+    else:
+        _close_multiset(P, P_ref)
+    return path
+
+
+@pytest.mark.parametrize("form", ["fp32_split", "fp64"])
+def test_restart_weibel2d(form, tmp_path):
+    P = setup_weibel(64, 64, epc=8, drift=0.5, uth=0.01, seed=11)
+    _roundtrip(lambda: Simulation(64, 64, epc=8, form=form, tile=16), lambda s: s.upload_particles(**P), 30, 20,
+               form == "fp32_split", tmp_path, f"2d_{form}")
+
+
+@pytest.mark.parametrize("form", ["fp32_split", "fp64"])
+def test_restart_weibel3d(form, tmp_path):
+    P = setup_weibel(16, 16, 16, ndim=3, epc=4, drift=0.3, uth=0.02, seed=12)
+    _roundtrip(lambda: Simulation(16, 16, 16, ndim=3, epc=4, form=form, tile=8), lambda s: s.upload_particles(**P),
+               12, 10, form == "fp32_split", tmp_path, f"3d_{form}")
+
+
+def test_restart_lehe_filter_split(tmp_path):
+    P = setup_weibel(64, 32, epc=8, drift=0.5, uth=0.01, seed=13)
+    _roundtrip(lambda: Simulation(64, 32, epc=8, form="fp32_split", tile=8, fdtd="lehe", curr_filter=2),
+               lambda s: s.upload_particles(**P), 15, 15, True, tmp_path, "2d_lehe_filter")
+
+
+def test_restart_cylinder_rtheta(tmp_path):
+    # a random plasma in the disk r < 10 of a 16 x 8 r-theta grid (index r = x - 3.5, theta = (y - 3) dtheta)
+    nr, nth, c, n = 16, 8, 0.3, 4000
+    rng = np.random.default_rng(5)
+    r = np.sqrt(rng.uniform(0.0, 100.0, n))
+    th = rng.uniform(0.0, 2 * np.pi, n)
+    flv = np.where(np.arange(n) % 2 == 0, 1, 2).astype(np.int32)
+    P = {"q": np.where(flv == 1, 1.0, -1.0), "x": 3.5 + r, "y": 3.0 + th / (2 * np.pi / nth),
+         "z": rng.uniform(1.0, 2.0, n), "u": rng.normal(0.0, 0.1, n), "v": rng.normal(0.0, 0.1, n),
+         "w": rng.normal(0.0, 0.1, n), "flv": flv}
+    _roundtrip(lambda: Simulation(nr, nth, epc=4, c=c, geometry="cyl", form="fp64"), lambda s: s.upload_particles(**P),
+               10, 10, False, tmp_path, "cyl_rtheta")
+
+
+@pytest.mark.parametrize("form", ["fp32_split", "fp64"])
+def test_restart_modal_window(form, tmp_path):
+    nr, nz, ppc, c = 32, 128, 4, 0.8
+
+    def make():
+        return Simulation(nr, 2, nz, ndim=3, epc=ppc, c=c, c_ompe=20.0, mi_me=1e30, form=form, geometry="cyl_modal",
+                          dr=4.0, fdtd="lehe")
+
+    def load(sim):
+        sim.cyl_plasma(24, 3, nz, ppc, seed=7)
+        sim.cyl_laser(1.0, 10.0, 4.0, 6.0, nz - 30.0)
+        sim.moving_window(True, nr_plasma=24, ppc=ppc, seed=7)
+        sim.cyl_sort_period(7)
+
+    _roundtrip(make, load, 25, 30, form == "fp32_split", tmp_path, f"modal_{form}")
+
+
+def test_restart_rejects_other_configuration(tmp_path):
+    P = setup_weibel(32, 32, epc=4, seed=3)
+    path = str(tmp_path / "a.state")
+    with Simulation(32, 32, epc=4, form="fp32_split", tile=8) as sim:
+        sim.upload_particles(**P)
+        sim.step(2)
+        sim.save_state(path, t=2)
+    for kw in ({"form": "fp64"}, {"c": 0.4}, {"epc": 8}):
+        args = dict(epc=4, form="fp32_split", tile=8)
+        args.update(kw)
+        with Simulation(32, 32, **args) as sim:
+            with pytest.raises(PictorError) as e:
+                sim.load_state(path)
+            assert e.value.code == 1
+    with Simulation(64, 32, epc=4, form="fp32_split", tile=8) as sim:
+        with pytest.raises(PictorError):
+            sim.load_state(path)
+    # another tile edge is the same configuration
+    with Simulation(32, 32, epc=4, form="fp32_split", tile=16) as sim:
+        assert sim.load_state(path) == 2
+        assert sim.particle_count() == len(P["x"])
+    # a truncated file is an error, and the context keeps its state
+    data = open(path, "rb").read()
+    short = str(tmp_path / "short.state")
+    open(short, "wb").write(data[: len(data) // 2])
+    with Simulation(32, 32, epc=4, form="fp32_split", tile=8) as sim:
+        sim.upload_particles(**P)
+        with pytest.raises(PictorError):
+            sim.load_state(short)
+        assert sim.particle_count() == len(P["x"])
