@@ -105,6 +105,8 @@ int orc_params_init(orc_params *p, int ndim, int nx, int ny, int nz, double c,
     p->flvrqm[0] = p->qmi;   /* initialise.F90:84 */
     p->flvrqm[1] = p->qme;   /* initialise.F90:85 */
     p->reorder_period = 20;
+    for (int a = 0; a < 3; a++) p->periodic[a] = 1;   /* every face periodic until a BC is set */
+    for (int f = 0; f < 6; f++) p->pml_L[f] = 4.0;      /* SetBC_Fld's default PML thickness */
     return ORC_OK;
 }
 
@@ -154,10 +156,13 @@ void orc_update_e(const orc_params *p, orc_fields *f)
     const int mx = p->mx, my = p->my, mz = p->mz;
     double *Ex = f->Ex, *Ey = f->Ey, *Ez = f->Ez;
     const double *Bx = f->Bx, *By = f->By, *Bz = f->Bz;
+    /* em_update.F90:9-39 loops over 2..m-1; the Lehe build's UpdateEfield
+     * (em_update_lehe.F90:43-84) over 3..m-3 (the same cells once the halo ran) */
+    const int e0 = (p->fdtd == 1) ? 3 : 2, e1 = (p->fdtd == 1) ? 3 : 1;
     if (p->ndim == 2) {
         #pragma omp parallel for schedule(static)
-        for (int j = 2; j <= my - 1; j++)
-            for (int i = 2; i <= mx - 1; i++) {
+        for (int j = e0; j <= my - e1; j++)
+            for (int i = e0; i <= mx - e1; i++) {
                 int64_t o = IX(p, i, j, 1);
                 Ex[o] = Ex[o] + fc * p->inv_dy * (Bz[o] - Bz[IX(p, i, j - 1, 1)]);
                 Ey[o] = Ey[o] - fc * p->inv_dx * (Bz[o] - Bz[IX(p, i - 1, j, 1)]);
@@ -166,9 +171,9 @@ void orc_update_e(const orc_params *p, orc_fields *f)
             }
     } else {
         #pragma omp parallel for schedule(static)
-        for (int k = 2; k <= mz - 1; k++)
-            for (int j = 2; j <= my - 1; j++)
-                for (int i = 2; i <= mx - 1; i++) {
+        for (int k = e0; k <= mz - e1; k++)
+            for (int j = e0; j <= my - e1; j++)
+                for (int i = e0; i <= mx - e1; i++) {
                     int64_t o = IX(p, i, j, k);
                     Ex[o] = Ex[o] + fc * p->inv_dy * (Bz[o] - Bz[IX(p, i, j - 1, k)])
                                   - fc * p->inv_dz * (By[o] - By[IX(p, i, j, k - 1)]);
@@ -391,19 +396,24 @@ static void halo_add_axis(const orc_params *p, double *F, int axis)
 void orc_field_halo(const orc_params *p, double *fx, double *fy, double *fz)
 {
     double *F[3] = {fx, fy, fz};
-    for (int c = 0; c < 3; c++) halo_copy_axis(p, F[c], 1);
-    if (p->ndim == 3)
+    /* a non-periodic axis has null neighbours on one rank (PeriodicOff): no exchange */
+    if (p->periodic[1])
+        for (int c = 0; c < 3; c++) halo_copy_axis(p, F[c], 1);
+    if (p->ndim == 3 && p->periodic[2])
         for (int c = 0; c < 3; c++) halo_copy_axis(p, F[c], 2);
-    for (int c = 0; c < 3; c++) halo_copy_axis(p, F[c], 0);
+    if (p->periodic[0])
+        for (int c = 0; c < 3; c++) halo_copy_axis(p, F[c], 0);
 }
 
 /* comm_fld.F90:15-33 with indepLBaxis = 0: YZ, ZX, (XY) */
 void orc_current_halo_add(const orc_params *p, double *jx, double *jy, double *jz)
 {
     double *F[3] = {jx, jy, jz};
-    for (int c = 0; c < 3; c++) halo_add_axis(p, F[c], 0);
-    for (int c = 0; c < 3; c++) halo_add_axis(p, F[c], 1);
-    if (p->ndim == 3)
+    if (p->periodic[0])
+        for (int c = 0; c < 3; c++) halo_add_axis(p, F[c], 0);
+    if (p->periodic[1])
+        for (int c = 0; c < 3; c++) halo_add_axis(p, F[c], 1);
+    if (p->ndim == 3 && p->periodic[2])
         for (int c = 0; c < 3; c++) halo_add_axis(p, F[c], 2);
 }
 
@@ -910,6 +920,8 @@ int orc_exchange_particles(const orc_params *p, orc_particles *s)
     if (rc == ORC_OK) {
         int nfaces = three ? 6 : 4;
         for (int n = 0; n < nfaces; n++) {
+            /* a non-periodic face sends to a null neighbour: the particle is gone */
+            if (!p->periodic[n >> 1]) continue;
             obuf *b = &out[n ^ 1];
             for (int64_t r = 0; r < b->n; r++) {
                 const double *d = b->d + 8 * r;
@@ -973,27 +985,31 @@ int orc_reorder(const orc_params *p, orc_particles *s)
  * the end of a step (MAIN.F90:248); here the same reset runs at the start
  * of the next step so J stays readable after the step (nothing reads J in
  * between). */
-int orc_step(const orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, int t)
+int orc_step(orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, int t)
 {
     orc_reset_current(p, f);
     orc_update_b_half(p, f);
     orc_field_halo(p, f->Bx, f->By, f->Bz);
     orc_move_deposit(p, w, f, s);
+    orc_bc_post_movdep(p, f, s);   /* EnforceBC_PostMovDep, MAIN.F90:77 */
     int rc = orc_exchange_particles(p, s);
     if (rc != ORC_OK) return rc;
     orc_update_b_half(p, f);
+    orc_pml_b(p, f);               /* MAIN.F90:104 */
     orc_field_halo(p, f->Bx, f->By, f->Bz);
     orc_update_e(p, f);
+    orc_pml_e(p, f);               /* MAIN.F90:109 */
     orc_field_halo(p, f->Ex, f->Ey, f->Ez);
     orc_current_halo_add(p, f->Jx, f->Jy, f->Jz);
     orc_smoothen_current(p, f);
     orc_add_current(p, f);
     orc_field_halo(p, f->Ex, f->Ey, f->Ez);
+    orc_bc_final(p, f);            /* EnforceBC_Final, MAIN.F90:209 */
     if (p->reorder_period > 0 && t % p->reorder_period == 0) rc = orc_reorder(p, s);
     return rc;
 }
 
-int orc_run(const orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, int t0, int nsteps)
+int orc_run(orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, int t0, int nsteps)
 {
     for (int t = t0; t < t0 + nsteps; t++) {
         int rc = orc_step(p, w, f, s, t);
@@ -1060,6 +1076,16 @@ static inline int fold(int n, int lo, int hi, int len)
     return n;
 }
 
+/* node index folded onto the physical nodes along a periodic axis; a
+ * non-periodic axis keeps its ghost nodes */
+static inline int pfold(const orc_params *p, int axis, int n)
+{
+    if (!p->periodic[axis]) return n;
+    if (axis == 0) return fold(n, 3, p->mx - 3, p->nx);
+    if (axis == 1) return fold(n, 3, p->my - 3, p->ny);
+    return fold(n, 3, p->mz - 3, p->nz);
+}
+
 /* CIC node charge rho(i,j,k) = sum q*qi * S_x * S_y (* S_z) at the nodes of
  * the cell holding the particle, folded onto physical nodes 3..m-3.  With
  * charge-conserving deposition div E - rho is invariant in time. */
@@ -1075,7 +1101,7 @@ void orc_rho(const orc_params *p, const orc_particles *s, double *rho)
         if (p->ndim == 2) {
             for (int b = 0; b < 2; b++)
                 for (int a = 0; a < 2; a++) {
-                    int ii = fold(i + a, 3, p->mx - 3, p->nx), jj = fold(j + b, 3, p->my - 3, p->ny);
+                    int ii = pfold(p, 0, i + a), jj = pfold(p, 1, j + b);
                     rho[IX(p, ii, jj, 1)] += qt * sx[a] * sy[b];
                 }
         } else {
@@ -1085,8 +1111,8 @@ void orc_rho(const orc_params *p, const orc_particles *s, double *rho)
             for (int cc = 0; cc < 2; cc++)
                 for (int b = 0; b < 2; b++)
                     for (int a = 0; a < 2; a++) {
-                        int ii = fold(i + a, 3, p->mx - 3, p->nx), jj = fold(j + b, 3, p->my - 3, p->ny);
-                        int kk = fold(k + cc, 3, p->mz - 3, p->nz);
+                        int ii = pfold(p, 0, i + a), jj = pfold(p, 1, j + b);
+                        int kk = pfold(p, 2, k + cc);
                         rho[IX(p, ii, jj, kk)] += qt * sx[a] * sy[b] * sz[cc];
                     }
         }

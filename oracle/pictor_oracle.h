@@ -39,10 +39,32 @@ typedef struct orc_params {
     /* em_update_lehe.F90:8-25: alpha_x, alpha_y, alpha_z, beta_xy, beta_xz, beta_yx,
      * beta_yz, beta_zx, beta_zy, delta_x, delta_y, delta_z */
     double lehe[12];
+    /* Boundary faces (bc.F90): face f = 0..5 is the reference's side f + 1 (x lower,
+     * x upper, y lower, y upper, z lower, z upper).  periodic[a] = 0 once any face of
+     * axis a has a type (PeriodicOff, subdomains.F90:508-547).  Positions are global
+     * (x = local - 3 on one rank) and move by speed * c per step (updateBC_Pos).
+     * pml_L: thickness of a 'pml' face (SetPMLParameters, bc_pml.F90:39-51).
+     * cond_top_as_written = 1 applies CondBC_Fld_Top at face 3's position, as
+     * bc.F90:318 is written; 0 (default) uses face 4's own position. */
+    int32_t periodic[3];
+    int32_t bc_fld[6], bc_prtl[6];
+    double bc_pos_fld[6], bc_pos_prtl[6], bc_speed[6], pml_L[6];
+    int32_t cond_top_as_written;
 } orc_params;
+
+/* face types (bc_face%type_fld / type_prtl) */
+#define ORC_BC_PERIODIC 0   /* 'prdc' */
+#define ORC_BC_COND 1       /* field 'cond': CondBC_Fld_*, bc_reflect.F90:11-65 */
+#define ORC_BC_OPEN 2       /* field 'open': OpenBC_Fld_*, bc.F90:410-469 (x faces) */
+#define ORC_BC_PML 3        /* field 'pml': bc_pml.F90 */
+#define ORC_BC_REFL 4       /* particle 'refl': RefBC_Prtl_*, bc_reflect.F90:76-237 (x, y faces) */
+#define ORC_BC_REMV 5       /* particle 'remv': RemovePrtl_*, bc_open.F90 */
 
 typedef struct orc_fields {
     double *Ex, *Ey, *Ez, *Bx, *By, *Bz, *Jx, *Jy, *Jz;
+    /* PML auxiliary fields (bc_pml.F90:21-31), each (mx,my,mz): pml_E_psi1 x,y,z,
+     * pml_E_psi2 x,y,z, pml_B_psi1 x,y,z, pml_B_psi2 x,y,z; NULL without a 'pml' face */
+    double *pml[12];
 } orc_fields;
 
 typedef struct orc_particles {
@@ -91,9 +113,17 @@ int  orc_ws_vec_len(const orc_ws *w);                   /* 4 (2-D) or 8 (3-D) */
 int  orc_exchange_particles(const orc_params *p, orc_particles *s);
 /* Counting sort by cell, mem_prtl.F90:166-301 */
 int  orc_reorder(const orc_params *p, orc_particles *s);
+/* Boundary conditions (bc.F90:238-348, bc_reflect.F90, bc_open.F90, bc_pml.F90) */
+void orc_bc_post_movdep(const orc_params *p, orc_fields *f, orc_particles *s);  /* EnforceBC_PostMovDep */
+void orc_bc_final(orc_params *p, orc_fields *f);      /* EnforceBC_Final incl. updateBC_Pos */
+void orc_pml_b(const orc_params *p, orc_fields *f);   /* UpdateMagFld_PML */
+void orc_pml_e(const orc_params *p, orc_fields *f);   /* UpdateElcFld_PML */
+/* pml_ab / pml_c of bc_pml.F90:293-353 at global coordinate x along axis (0..2):
+ * out = {a, b, c}; exposed for the coefficient pins */
+void orc_pml_coeffs(const orc_params *p, double x, int axis, double out[3]);
 /* One MAIN.F90 time step (J reset moved to the step start). */
-int  orc_step(const orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, int t);
-int  orc_run(const orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, int t0, int nsteps);
+int  orc_step(orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, int t);
+int  orc_run(orc_params *p, orc_ws *w, orc_fields *f, orc_particles *s, int t0, int nsteps);
 
 /* Diagnostics, savedata_routines.F90:265-355 */
 void orc_energy(const orc_params *p, const orc_fields *f, const orc_particles *s, double out[4]);

@@ -31,7 +31,14 @@ class OrcParams(ctypes.Structure):
             "Bx_ext0", "By_ext0", "Bz_ext0")]
         + [("nflvr", ctypes.c_int32), ("reorder_period", ctypes.c_int32), ("flvrqm", ctypes.c_double * 8)]
         + [("fdtd", ctypes.c_int32), ("curr_filter", ctypes.c_int32), ("lehe", ctypes.c_double * 12)]
+        + [("periodic", ctypes.c_int32 * 3), ("bc_fld", ctypes.c_int32 * 6), ("bc_prtl", ctypes.c_int32 * 6)]
+        + [(n, ctypes.c_double * 6) for n in ("bc_pos_fld", "bc_pos_prtl", "bc_speed", "pml_L")]
+        + [("cond_top_as_written", ctypes.c_int32)]
     )
+
+
+# face types (pictor_oracle.h ORC_BC_*) by the reference's names
+BC_TYPES = {"prdc": 0, "cond": 1, "open": 2, "pml": 3, "refl": 4, "remv": 5}
 
 
 class OrcCyl(ctypes.Structure):
@@ -41,7 +48,7 @@ class OrcCyl(ctypes.Structure):
 
 
 class OrcFields(ctypes.Structure):
-    _fields_ = [(n, c_dp) for n in ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz")]
+    _fields_ = [(n, c_dp) for n in ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz")] + [("pml", c_dp * 12)]
 
 
 class OrcParticles(ctypes.Structure):
@@ -89,6 +96,11 @@ def lib():
         L.orc_rho.argtypes = [P, S, c_dp]
         L.orc_gauss.argtypes = [P, F, S, c_dp]
         L.orc_momentum.argtypes = [P, S, c_dp]
+        L.orc_bc_post_movdep.argtypes = [P, F, S]
+        L.orc_bc_final.argtypes = [P, F]
+        L.orc_pml_b.argtypes = [P, F]
+        L.orc_pml_e.argtypes = [P, F]
+        L.orc_pml_coeffs.argtypes = [P, ctypes.c_double, ctypes.c_int, c_dp]
         C = ctypes.POINTER(OrcCyl)
         L.orc_cyl_init.argtypes = [C, P]
         for name in ("orc_cyl_update_e", "orc_cyl_update_b_half", "orc_cyl_add_current"):
@@ -129,6 +141,7 @@ class Oracle:
         self.shape = (self.p.mx, self.p.my, self.p.mz)
         self.F = {n: np.zeros(self.shape, order="F") for n in self.FIELD_NAMES}
         self.f = OrcFields(*[_ptr(self.F[n]) for n in self.FIELD_NAMES])
+        self._pml = [None] * 12
         self._alloc_particles(cap)
         self.ws = L.orc_ws_create(ctypes.byref(self.p), nthreads)
         if not self.ws:
@@ -200,6 +213,57 @@ class Oracle:
 
     def move_deposit(self):
         lib().orc_move_deposit(ctypes.byref(self.p), self.ws, ctypes.byref(self.f), ctypes.byref(self.s))
+
+    # ---- boundary faces (bc.F90) ----
+    @staticmethod
+    def _face(axis, side):
+        a = {"x": 0, "y": 1, "z": 2}[axis] if isinstance(axis, str) else int(axis)
+        s = {"lower": 0, "upper": 1}[side] if isinstance(side, str) else int(side)
+        return a, 2 * a + s
+
+    def set_bc_fld(self, axis, position, kind, side, thickness=4.0):
+        """SetBC_Fld (bc.F90:23-50): a field face at a global position."""
+        a, f = self._face(axis, side)
+        self.p.bc_fld[f] = BC_TYPES[kind]
+        self.p.bc_pos_fld[f] = position
+        self.p.pml_L[f] = thickness
+        self.p.periodic[a] = 0
+        if kind == "pml" and self._pml[0] is None:
+            for k in range(12):
+                self._pml[k] = np.zeros(self.shape, order="F")
+                self.f.pml[k] = _ptr(self._pml[k])
+
+    def set_bc_prtl(self, axis, position, kind, side):
+        """SetBC_Prtl (bc.F90:52-63): a particle face at a global position."""
+        a, f = self._face(axis, side)
+        self.p.bc_prtl[f] = BC_TYPES[kind]
+        self.p.bc_pos_prtl[f] = position
+        self.p.periodic[a] = 0
+
+    def set_boundary_speed(self, axis, side, speed):
+        """SetBoundarySpeed (bc.F90:210-217)."""
+        self.p.bc_speed[self._face(axis, side)[1]] = speed
+
+    @property
+    def pml_psi(self):
+        return self._pml
+
+    def bc_post_movdep(self):
+        lib().orc_bc_post_movdep(ctypes.byref(self.p), ctypes.byref(self.f), ctypes.byref(self.s))
+
+    def bc_final(self):
+        lib().orc_bc_final(ctypes.byref(self.p), ctypes.byref(self.f))
+
+    def pml_b(self):
+        lib().orc_pml_b(ctypes.byref(self.p), ctypes.byref(self.f))
+
+    def pml_e(self):
+        lib().orc_pml_e(ctypes.byref(self.p), ctypes.byref(self.f))
+
+    def pml_coeffs(self, x, axis):
+        out = np.zeros(3)
+        lib().orc_pml_coeffs(ctypes.byref(self.p), float(x), int(axis), _ptr(out))
+        return out
 
     def gather_vec(self):
         L = lib()
@@ -375,6 +439,7 @@ class CylModalOracle:
         self.shape = (self.m.mx, 3, self.m.mz)
         self.F = {n: np.zeros(self.shape, order="F") for n in self.FIELD_NAMES}
         self.f = OrcFields(*[_ptr(self.F[n]) for n in self.FIELD_NAMES])
+        self._pml = [None] * 12
         self._alloc(cap)
 
     def _alloc(self, cap):
