@@ -59,7 +59,8 @@ extern "C" {
 
 /* Run configuration: the compile-time parameters of parameters.F90:5-87 and
  * src/vars.F90:10-18 as a runtime struct.  grid_dx = grid_dy = grid_dz = 1,
- * EMspeedBoost = 1, periodic faces on every side. */
+ * EMspeedBoost = 1, periodic faces on every side until pictor_set_bc_fld /
+ * pictor_set_bc_prtl set a face. */
 typedef struct pictor_config {
     int32_t ndim;          /* 2 (-DtwoD) or 3                                */
     int32_t nx, ny, nz;    /* physical cells; nz ignored in 2-D              */
@@ -121,6 +122,20 @@ int pictor_download_particles(pictor_ctx *ctx, int64_t cap, double *q, double *x
 int pictor_save_state(pictor_ctx *ctx, const char *path, int64_t t);
 int pictor_load_state(pictor_ctx *ctx, const char *path, int64_t *t);
 
+/* ---- output files: save_total_energy, src/io/savedata.F90:856-890 (energy_<t>),
+ *      and save_field_collective, savedata.F90:356-497 with CollectFld,
+ *      savedata_routines.F90:207-261 (fld_<t>).  The reference writes HDF5; this
+ * image has none, so each file is a NumPy .npz archive (uncompressed zip of .npy
+ * arrays, Fortran order) with the reference's dataset names:
+ *   energy: Benergy(1), Eenergy(1), PrtlEnergy(Nflvr), FP64 (CalcEnergy);
+ *   fields: Ex, Ey, Ez, Bx, By, Bz (+ Jx, Jy, Jz if save_current, after the
+ *           current's halo copy) as FP32 (fdatax, fdatay, fdataz): the nodes
+ *           3, 3 + res, ... 3 + res * int(n / res) of each axis over the whole box,
+ *           the staggered components averaged onto the node as CollectFld does.
+ * The field file is written for the Cartesian geometry. */
+int pictor_save_energy(pictor_ctx *ctx, const char *path);
+int pictor_save_fields(pictor_ctx *ctx, const char *path, int32_t res, int32_t save_current);
+
 /* ---- the step, one call per MAIN.F90 phase ---- */
 /* UpdateBfieldHalf + SendRecvFlds(B): em_update.F90:41-77, comm_fld.F90:75-91 */
 int pictor_update_b_half(pictor_ctx *ctx);
@@ -149,8 +164,46 @@ int pictor_field_halo(pictor_ctx *ctx, int32_t which);
 int pictor_reset_current(pictor_ctx *ctx);
 /* ReorderPrtlArr: mem_prtl.F90:166-301 -- counting sort into tiles */
 int pictor_sort_tiles(pictor_ctx *ctx);
-/* nsteps whole time steps of MAIN.F90:42-284 (J reset at the step start). */
+/* nsteps whole time steps of MAIN.F90:42-284 (J reset at the step start),
+ * with the boundary faces below where they are set. */
 int pictor_step(pictor_ctx *ctx, int32_t nsteps);
+
+/* ---- boundary faces, src/bc (Cartesian geometry, both forms, 2-D and 3-D).
+ * axis 0 = x, 1 = y, 2 = z (z only in 3-D); side 0 = lower, 1 = upper;
+ * position is global (0 .. n along the axis; local coordinate = position + 3).
+ * Setting any face of an axis makes the axis non-periodic on both sides
+ * (PeriodicOff, subdomains.F90:508-547): its halo exchanges stop, its ghost
+ * cells evolve under the Yee update (cells 2..m-1, em_update.F90), and a
+ * particle that leaves through its faces is dropped.  Not built: inflow
+ * ('iflw'), attenuating ('attn') and curved faces, the open particle face,
+ * the current filter on a non-periodic axis (curr_filter must be 0), the
+ * adjoint with faces set. */
+#define PICTOR_BC_PERIODIC 0     /* 'prdc' */
+#define PICTOR_BC_COND 1         /* field 'cond': CondBC_Fld_*, bc_reflect.F90:11-65
+                                    (x, y faces): tangential E zero beyond it */
+#define PICTOR_BC_OPEN 2         /* field 'open': OpenBC_Fld_*, bc.F90:410-469 (x faces) */
+#define PICTOR_BC_PML 3          /* field 'pml': UpdateMagFld_PML / UpdateElcFld_PML,
+                                    bc_pml.F90 (any face; thickness = layer depth) */
+#define PICTOR_BC_REFL 4         /* particle 'refl': RefBC_Prtl_*, bc_reflect.F90:76-237
+                                    (x, y faces), with the current folded back */
+#define PICTOR_BC_REMV 5         /* particle 'remv': RemovePrtl_*, bc_open.F90 (any face) */
+/* SetBC_Fld, bc.F90:23-50 (PML: SetPMLParameters, bc_pml.F90:39-51; the layer
+ * must fit inside the box, CheckAttnBCPositioning bc.F90:364-401) */
+int pictor_set_bc_fld(pictor_ctx *ctx, int32_t axis, int32_t side, int32_t type, double position,
+                      double thickness);
+/* SetBC_Prtl, bc.F90:52-63 */
+int pictor_set_bc_prtl(pictor_ctx *ctx, int32_t axis, int32_t side, int32_t type, double position);
+/* SetBoundarySpeed, bc.F90:210-217: both positions of the face move by speed * c
+ * at the end of every step (updateBC_Pos, bc.F90:342-348) */
+int pictor_set_boundary_speed(pictor_ctx *ctx, int32_t axis, int32_t side, double speed);
+/* UpdateMagFld_PML (which = 1, after the second B half step, MAIN.F90:104) or
+ * UpdateElcFld_PML (which = 0, after UpdateEfield, MAIN.F90:109); no-op
+ * without a PML face */
+int pictor_pml(pictor_ctx *ctx, int32_t which);
+/* EnforceBC_Final, bc.F90:289-340 (MAIN.F90:209): conducting and open field
+ * faces, then updateBC_Pos.  The reflecting and removing particle faces of
+ * EnforceBC_PostMovDep (MAIN.F90:77) run inside pictor_move_deposit. */
+int pictor_enforce_bc_final(pictor_ctx *ctx);
 int pictor_synchronize(pictor_ctx *ctx);
 
 /* ---- diagnostics: CalcEnergy/CalcDivE, savedata_routines.F90:265-355 (FP64).

@@ -429,8 +429,8 @@ void orc_pml_b(const orc_params *p, orc_fields *f)
     if (!pml_any(p)) return;
     const pml_geo g = pml_position(p);
     const int64_t n = (int64_t)p->mx * p->my * p->mz;
-    if (g.none) {   /* the psi arrays are deallocated, i.e. restart from zero */
-        for (int a = 6; a < 12; a++)
+    if (g.none) {   /* DeallocatePMLArr: every psi array restarts from zero */
+        for (int a = 0; a < 12; a++)
             if (f->pml[a]) memset(f->pml[a], 0, sizeof(double) * (size_t)n);
         return;
     }

@@ -24,6 +24,9 @@ GEOM_CARTESIAN = 0
 GEOM_CYL = 1
 GEOM_CYL_MODAL = 2
 
+# boundary face types (PICTOR_BC_*) by the reference's names (bc_face%type_fld / type_prtl)
+BC_TYPES = {"prdc": 0, "cond": 1, "open": 2, "pml": 3, "refl": 4, "remv": 5}
+
 # every entry point of include/pictor_hip.h
 EXPORTS = (
     "pictor_init", "pictor_finalize", "pictor_last_error", "pictor_get_scalars",
@@ -36,7 +39,9 @@ EXPORTS = (
     "pictor_setup_weibel", "pictor_adjoint_forward", "pictor_adjoint_backward", "pictor_adjoint_stats",
     "pictor_adjoint_engine", "pictor_adjoint_release", "pictor_cyl_plasma", "pictor_cyl_laser",
     "pictor_moving_window", "pictor_probe_atomics", "pictor_cyl_push_scheme", "pictor_cyl_sort_period",
-    "pictor_save_state", "pictor_load_state",
+    "pictor_save_state", "pictor_load_state", "pictor_set_bc_fld", "pictor_set_bc_prtl",
+    "pictor_set_boundary_speed", "pictor_pml", "pictor_enforce_bc_final", "pictor_save_energy",
+    "pictor_save_fields",
 )
 
 
@@ -106,6 +111,13 @@ def lib():
     L.pictor_moving_window.argtypes = [c_vp, i32, i32, i32, ctypes.c_uint64]
     L.pictor_save_state.argtypes = [c_vp, ctypes.c_char_p, i64]
     L.pictor_load_state.argtypes = [c_vp, ctypes.c_char_p, ctypes.POINTER(i64)]
+    L.pictor_set_bc_fld.argtypes = [c_vp, i32, i32, i32, ctypes.c_double, ctypes.c_double]
+    L.pictor_set_bc_prtl.argtypes = [c_vp, i32, i32, i32, ctypes.c_double]
+    L.pictor_set_boundary_speed.argtypes = [c_vp, i32, i32, ctypes.c_double]
+    L.pictor_pml.argtypes = [c_vp, i32]
+    L.pictor_enforce_bc_final.argtypes = [c_vp]
+    L.pictor_save_energy.argtypes = [c_vp, ctypes.c_char_p]
+    L.pictor_save_fields.argtypes = [c_vp, ctypes.c_char_p, i32, i32]
     for name in EXPORTS:
         if name != "pictor_last_error":
             getattr(L, name).restype = ctypes.c_int

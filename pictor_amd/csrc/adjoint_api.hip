@@ -773,6 +773,7 @@ int pictor_adjoint_forward(pictor_ctx *c, int64_t n, const double *q, const doub
     if (rc) return rc;
     if (c->g.ndim != 2 || c->cyl) return fail(PICTOR_ERR_ARG, "the adjoint is available for the 2-D Cartesian geometry");
     if (c->ph.fdtd != PICTOR_FDTD_YEE) return fail(PICTOR_ERR_ARG, "the adjoint is available for the Yee solver");
+    if (c->bc_any) return fail(PICTOR_ERR_ARG, "the adjoint is available for the periodic box (no boundary faces)");
     if (c->pending_swap) return fail(PICTOR_ERR_ARG, "adjoint inside a split step");
     if (n < 1 || nsteps < 1 || segment < 1) return fail(PICTOR_ERR_ARG, "n, nsteps and segment must be >= 1");
     if (n >= 0x7fffffffLL) return fail(PICTOR_ERR_ARG, "adjoint: more than 2^31-1 particles");

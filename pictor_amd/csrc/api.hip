@@ -319,20 +319,21 @@ int check_ctx(pictor_ctx *c)
     return PICTOR_OK;
 }
 
-int do_b_half(pictor_ctx *c)
+// halo = false leaves the ghost copy to the caller (the PML runs in between)
+int do_b_half(pictor_ctx *c, bool halo = true)
 {
     if (c->form == PICTOR_FORM_FP64) {
         double *E[3], *B[3];
         field_ptrs(c, E, B);
         if (c->ph.fdtd == PICTOR_FDTD_LEHE) launch_update_b_half_lehe<double>(c->g, c->ph, c->ph.lehe, E, B, c->stream);
         else launch_update_b_half<double>(c->g, c->ph, E, B, c->stream);
-        launch_ghost_copy<double>(c->g, B, c->stream);
+        if (halo) launch_ghost_copy<double>(c->g, B, c->stream);
     } else {
         float *E[3], *B[3];
         field_ptrs(c, E, B);
         if (c->ph.fdtd == PICTOR_FDTD_LEHE) launch_update_b_half_lehe<float>(c->g, c->ph, c->ph.lehe, E, B, c->stream);
         else launch_update_b_half<float>(c->g, c->ph, E, B, c->stream);
-        launch_ghost_copy<float>(c->g, B, c->stream);
+        if (halo) launch_ghost_copy<float>(c->g, B, c->stream);
     }
     HIPTRY(hipGetLastError());
     return PICTOR_OK;
@@ -402,7 +403,22 @@ int do_smoothen(pictor_ctx *c)
 
 double jscale(const pictor_ctx *c) { return c->ph.qi * kFixInv; }
 
-int do_e_phase(pictor_ctx *c, bool curl, bool add)
+// SendRecvFlds on E (which = 0) or B (which = 1), Cartesian geometry
+int do_halo(pictor_ctx *c, int which)
+{
+    void **F = which == 0 ? c->E : c->B;
+    if (c->form == PICTOR_FORM_FP64) {
+        double *P[3] = {(double *)F[0], (double *)F[1], (double *)F[2]};
+        launch_ghost_copy<double>(c->g, P, c->stream);
+    } else {
+        float *P[3] = {(float *)F[0], (float *)F[1], (float *)F[2]};
+        launch_ghost_copy<float>(c->g, P, c->stream, which == 0 ? c->Elo : nullptr);
+    }
+    HIPTRY(hipGetLastError());
+    return PICTOR_OK;
+}
+
+int do_e_phase(pictor_ctx *c, bool curl, bool add, bool halo = true)
 {
     if (c->form == PICTOR_FORM_FP64) {
         double *E[3], *B[3], *J[3];
@@ -411,14 +427,14 @@ int do_e_phase(pictor_ctx *c, bool curl, bool add)
         if (curl && add) launch_update_e_add_current<double, double>(c->g, c->ph, E, B, J, 1.0, c->stream);
         else if (curl) launch_update_e<double>(c->g, c->ph, E, B, c->stream);
         else launch_add_current<double, double>(c->g, c->ph, E, J, 1.0, c->stream);
-        launch_ghost_copy<double>(c->g, E, c->stream);
+        if (halo) launch_ghost_copy<double>(c->g, E, c->stream);
     } else {
         float *E[3], *B[3];
         double *J[3];
         field_ptrs(c, E, B);
         for (int i = 0; i < 3; i++) J[i] = (double *)c->J[i];
         launch_update_e_pair(c->g, c->ph, E, c->Elo, B, J, jscale(c), curl, add, c->stream);
-        launch_ghost_copy<float>(c->g, E, c->stream, c->Elo);
+        if (halo) launch_ghost_copy<float>(c->g, E, c->stream, c->Elo);
     }
     HIPTRY(hipGetLastError());
     return PICTOR_OK;
@@ -531,6 +547,7 @@ int pictor_init(const pictor_config *cfg, pictor_ctx **out)
     g.mx = g.nx + 5; g.my = g.ny + 5; g.mz = (cfg->ndim == 2) ? 1 : g.nz + 5;
     if (modal) { g.ny = 3; g.my = 3; }   // the middle index holds the three mode slots
     g.ncell = (long long)g.mx * g.my * g.mz;
+    g.per[0] = g.per[1] = g.per[2] = 1;   // periodic until a boundary face is set
 
     // InitScale, initialise.F90:44-53; vars.F90:220-223; subdomains.F90:52-67
     PhysP &ph = c->ph;
@@ -668,6 +685,7 @@ int pictor_finalize(pictor_ctx *c)
     if (c->cylm) cylm_release(c);
     cyl_release(c);
     dfree(c->d_partial); dfree(c->d_rho);
+    bc_release(c);
     if (c->h_flags) hipHostFree(c->h_flags);
     for (int i = 0; i < 23; i++) if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -889,7 +907,10 @@ int pictor_move_deposit(pictor_ctx *c)
         if (c->pending_swap) return fail(PICTOR_ERR_ARG, "pictor_move_deposit called twice without pictor_exchange_particles");
         return c->cylm ? cylm_push(c, false) : cyl_push(c, false);
     }
-    return do_push(c, false);
+    // MoveDepositPrtl, then EnforceBC_PostMovDep (MAIN.F90:65-77): the particle faces act
+    // inside the push, the current of the reflecting faces is folded back here
+    rc = do_push(c, false);
+    return rc ? rc : bc_after_push(c);
 }
 
 int pictor_exchange_particles(pictor_ctx *c)
@@ -930,16 +951,7 @@ int pictor_field_halo(pictor_ctx *c, int32_t which)
     if (rc) return rc;
     if (which != 0 && which != 1) return fail(PICTOR_ERR_ARG, "which must be 0 (E) or 1 (B)");
     if (c->cyl) return c->cylm ? cylm_field_halo(c, which) : cyl_field_halo(c, which);
-    void **F = which == 0 ? c->E : c->B;
-    if (c->form == PICTOR_FORM_FP64) {
-        double *P[3] = {(double *)F[0], (double *)F[1], (double *)F[2]};
-        launch_ghost_copy<double>(c->g, P, c->stream);
-    } else {
-        float *P[3] = {(float *)F[0], (float *)F[1], (float *)F[2]};
-        launch_ghost_copy<float>(c->g, P, c->stream, which == 0 ? c->Elo : nullptr);
-    }
-    HIPTRY(hipGetLastError());
-    return PICTOR_OK;
+    return do_halo(c, which);
 }
 
 int pictor_reset_current(pictor_ctx *c)
@@ -1015,10 +1027,25 @@ int pictor_step(pictor_ctx *c, int32_t nsteps)
             if (rc) return rc;
         }
         if ((rc = do_push(c, true, E[2], E[3]))) return rc;
-        if ((rc = do_b_half(c))) return rc;
+        if ((rc = bc_after_push(c))) return rc;   // EnforceBC_PostMovDep, MAIN.F90:77
+        if (c->pml_on) {
+            // UpdateBfieldHalf, UpdateMagFld_PML, SendRecvFlds (MAIN.F90:103-105)
+            if ((rc = do_b_half(c, false)) || (rc = bc_pml(c, true)) || (rc = do_halo(c, 1))) return rc;
+        } else if ((rc = do_b_half(c))) {
+            return rc;
+        }
         if ((rc = do_fold(c))) return rc;
         if ((rc = do_smoothen(c))) return rc;
-        if ((rc = do_e_phase(c, true, true))) return rc;
+        if (c->pml_on) {
+            // UpdateEfield and UpdateElcFld_PML (MAIN.F90:108-109) before AddCurrent; the
+            // halo between them only fills ghosts that the final halo fills again
+            if ((rc = do_e_phase(c, true, false, false)) || (rc = bc_pml(c, false)) ||
+                (rc = do_e_phase(c, false, true)))
+                return rc;
+        } else if ((rc = do_e_phase(c, true, true))) {
+            return rc;
+        }
+        if ((rc = bc_final(c))) return rc;   // EnforceBC_Final, MAIN.F90:209
         HIPTRY(hipEventRecord(E[4], c->stream));
         HIPTRY(hipEventRecord(E[5], c->stream));
         if ((rc = enqueue_flags(c, E[6]))) return rc;

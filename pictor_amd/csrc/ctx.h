@@ -9,6 +9,8 @@
 #include "cyl.h"
 #include "cylm.h"
 
+struct pictor_ctx;
+
 namespace pictor {
 
 // message of the last failed call on this thread (pictor_last_error)
@@ -36,6 +38,23 @@ hipError_t dalloc(T **p, size_t n)
     if (n == 0) n = 1;
     return hipMalloc((void **)p, n * sizeof(T));
 }
+
+// boundary faces (bc.hip), called by the step: the current folds of the
+// reflecting faces after a push, the PML after the B (b = true) or E update,
+// the field faces and face motion at the end of a step, and the release
+int bc_after_push(pictor_ctx *c);
+int bc_pml(pictor_ctx *c, bool b);
+int bc_final(pictor_ctx *c);
+void bc_release(pictor_ctx *c);
+
+// the faces as the restart files keep them (bc_face of vars.F90:148-158 and the
+// periodic axes); the PML auxiliary fields follow in the file when pml_on
+struct BcState {
+    int32_t per[3], fld[6], prtl[6], pml_on;
+    double pos_fld[6], pos_prtl[6], speed[6], pml_L[6];
+};
+void bc_get(const pictor_ctx *c, BcState &s);
+int bc_set(pictor_ctx *c, const BcState &s);   // allocates the PML fields when needed
 
 // adjoint engine owned by a context (adjoint_api.hip)
 struct AdjEngineBase {
@@ -123,6 +142,17 @@ struct pictor_ctx {
     bool cylm = false;
     pictor::CylmP cq{};
     pictor::CylmState cm{};
+
+    // boundary faces (bc.hip): face f = 0..5 is x lo, x hi, y lo, y hi, z lo, z hi;
+    // PICTOR_BC_* types, global positions, speeds (updateBC_Pos) and PML thickness
+    int bc_fld[6] = {0, 0, 0, 0, 0, 0}, bc_prtl[6] = {0, 0, 0, 0, 0, 0};
+    double bc_pos_fld[6] = {0, 0, 0, 0, 0, 0}, bc_pos_prtl[6] = {0, 0, 0, 0, 0, 0};
+    double bc_speed[6] = {0, 0, 0, 0, 0, 0}, pml_L[6] = {4, 4, 4, 4, 4, 4};
+    bool bc_any = false;        // a face has been set
+    bool pml_on = false;        // a PML face is set
+    double *pml_psi[12] = {};   // pml_E_psi1 x,y,z, pml_E_psi2, pml_B_psi1, pml_B_psi2 (ncell each)
+    double *pml_tab = nullptr;  // PML coefficient tables (bc.hip), rebuilt when a PML face moves
+    bool pml_tab_stale = true;
 
     pictor::AdjEngineBase *adj = nullptr;   // created by pictor_adjoint_forward
     int adj_engine = 0;                      // 0 tile-sorted (FP32 split form), 1 unsorted

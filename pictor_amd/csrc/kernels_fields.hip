@@ -23,16 +23,25 @@ inline dim3 phys_grid(const GridP &g)
     return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, nz);
 }
 
-// em_update.F90:41-77 (B half step), physical cells only
+// blocks over the cells yee_lo..yee_hi of every axis
+inline dim3 yee_grid(const GridP &g)
+{
+    const int ex = yee_hi(g, 0) - yee_lo(g, 0) + 1, ey = yee_hi(g, 1) - yee_lo(g, 1) + 1;
+    const int ez = (g.ndim == 3) ? yee_hi(g, 2) - yee_lo(g, 2) + 1 : 1;
+    return dim3((ex + BX - 1) / BX, (ey + BY - 1) / BY, ez);
+}
+
+// em_update.F90:41-77 (B half step): the physical cells along periodic axes,
+// 2..m-1 along non-periodic ones (yee_lo..yee_hi)
 template <typename R>
 __global__ __launch_bounds__(BX *BY) void k_update_b_half(GridP g, R hc, R idx_, R idy_, R idz_, const R *__restrict__ Ex,
                                                           const R *__restrict__ Ey, const R *__restrict__ Ez,
                                                           R *__restrict__ Bx, R *__restrict__ By, R *__restrict__ Bz)
 {
-    int i = 3 + blockIdx.x * BX + threadIdx.x;
-    int j = 3 + blockIdx.y * BY + threadIdx.y;
-    int k = (g.ndim == 3) ? 3 + (int)blockIdx.z : 1;
-    if (i > g.mx - 3 || j > g.my - 3) return;
+    int i = yee_lo(g, 0) + blockIdx.x * BX + threadIdx.x;
+    int j = yee_lo(g, 1) + blockIdx.y * BY + threadIdx.y;
+    int k = (g.ndim == 3) ? yee_lo(g, 2) + (int)blockIdx.z : 1;
+    if (i > yee_hi(g, 0) || j > yee_hi(g, 1)) return;
     long long o = fidx(g, i, j, k);
     long long oi = o + 1, oj = o + g.mx, ok = o + (long long)g.mx * g.my;
     if (g.ndim == 2) {
@@ -118,17 +127,21 @@ __device__ inline float jvalue<float, double>(const double *J, long long o, doub
 
 // em_update.F90:9-39 (E full step); with ADD, AddCurrent (em_update.F90:79-102)
 // follows in the same thread: (E + c curl B) - J, the reference's two roundings.
+// wide: the Yee range yee_lo..yee_hi; otherwise the physical cells (the Lehe
+// build's UpdateEfield, em_update_lehe.F90:43-84, and AddCurrent alone)
 template <typename R, typename JT, bool CURL, bool ADD>
 __global__ __launch_bounds__(BX *BY) void k_update_e(GridP g, R fc, R idx_, R idy_, R idz_, const R *__restrict__ Bx,
                                                      const R *__restrict__ By, const R *__restrict__ Bz,
                                                      R *__restrict__ Ex, R *__restrict__ Ey, R *__restrict__ Ez,
                                                      const JT *__restrict__ Jx, const JT *__restrict__ Jy,
-                                                     const JT *__restrict__ Jz, double jscale)
+                                                     const JT *__restrict__ Jz, double jscale, int wide)
 {
-    int i = 3 + blockIdx.x * BX + threadIdx.x;
-    int j = 3 + blockIdx.y * BY + threadIdx.y;
-    int k = (g.ndim == 3) ? 3 + (int)blockIdx.z : 1;
-    if (i > g.mx - 3 || j > g.my - 3) return;
+    const int lx = wide ? yee_lo(g, 0) : 3, ly = wide ? yee_lo(g, 1) : 3, lz = wide ? yee_lo(g, 2) : 3;
+    const int ux = wide ? yee_hi(g, 0) : g.mx - 3, uy = wide ? yee_hi(g, 1) : g.my - 3;
+    int i = lx + blockIdx.x * BX + threadIdx.x;
+    int j = ly + blockIdx.y * BY + threadIdx.y;
+    int k = (g.ndim == 3) ? lz + (int)blockIdx.z : 1;
+    if (i > ux || j > uy) return;
     long long o = fidx(g, i, j, k);
     long long oi = o - 1, oj = o - g.mx, ok = o - (long long)g.mx * g.my;
     R ex = Ex[o], ey = Ey[o], ez = Ez[o];
@@ -144,7 +157,8 @@ __global__ __launch_bounds__(BX *BY) void k_update_e(GridP g, R fc, R idx_, R id
         }
     }
     if (ADD) {
-        bool inner = (g.ndim == 2) || (k >= 3 && k <= g.mz - 3);
+        const bool inner = i >= 3 && i <= g.mx - 3 && j >= 3 && j <= g.my - 3 &&
+                           ((g.ndim == 2) || (k >= 3 && k <= g.mz - 3));
         if (inner) {
             ex = ex - jvalue<R, JT>(Jx, o, jscale) * idy_ * idz_;
             ey = ey - jvalue<R, JT>(Jy, o, jscale) * idx_ * idz_;
@@ -176,12 +190,14 @@ __global__ __launch_bounds__(BX *BY) void k_update_e_pair(GridP g, double fc, do
                                                           float *__restrict__ Lx, float *__restrict__ Ly,
                                                           float *__restrict__ Lz, const double *__restrict__ Jx,
                                                           const double *__restrict__ Jy, const double *__restrict__ Jz,
-                                                          double jscale)
+                                                          double jscale, int wide)
 {
-    int i = 3 + blockIdx.x * BX + threadIdx.x;
-    int j = 3 + blockIdx.y * BY + threadIdx.y;
-    int k = (g.ndim == 3) ? 3 + (int)blockIdx.z : 1;
-    if (i > g.mx - 3 || j > g.my - 3) return;
+    const int lx = wide ? yee_lo(g, 0) : 3, ly = wide ? yee_lo(g, 1) : 3, lz = wide ? yee_lo(g, 2) : 3;
+    const int ux = wide ? yee_hi(g, 0) : g.mx - 3, uy = wide ? yee_hi(g, 1) : g.my - 3;
+    int i = lx + blockIdx.x * BX + threadIdx.x;
+    int j = ly + blockIdx.y * BY + threadIdx.y;
+    int k = (g.ndim == 3) ? lz + (int)blockIdx.z : 1;
+    if (i > ux || j > uy) return;
     long long o = fidx(g, i, j, k);
     long long oi = o - 1, oj = o - g.mx, ok = o - (long long)g.mx * g.my;
     double ex = (double)Ex[o] + (double)Lx[o], ey = (double)Ey[o] + (double)Ly[o], ez = (double)Ez[o] + (double)Lz[o];
@@ -204,7 +220,8 @@ __global__ __launch_bounds__(BX *BY) void k_update_e_pair(GridP g, double fc, do
         ex = pair_round(ex); ey = pair_round(ey); ez = pair_round(ez);
     }
     if (ADD) {
-        bool inner = (g.ndim == 2) || (k >= 3 && k <= g.mz - 3);
+        const bool inner = i >= 3 && i <= g.mx - 3 && j >= 3 && j <= g.my - 3 &&
+                           ((g.ndim == 2) || (k >= 3 && k <= g.mz - 3));
         if (inner) {
             ex = ex - (Jx[o] * jscale) * idy_ * idz_;
             ey = ey - (Jy[o] * jscale) * idx_ * idz_;
@@ -237,7 +254,11 @@ __global__ void k_pair_to_f64(const float *__restrict__ hi, const float *__restr
 }
 
 // Ghost layers 1..2 and m-2..m take the value of their periodic image
-// (comm_fld.F90:75-130 on one rank; corners end as the doubly shifted image).
+// (comm_fld.F90:75-130 on one rank).  The exchanges run y, (z,) then x; the y
+// and z faces span x = 3..mx-3, the x faces the whole cross-section, so a
+// ghost is copied from the image reached by shifting x first (if x is
+// periodic), then y and z while the x index is physical; all periodic: corners
+// end as the doubly shifted image.  A non-periodic axis exchanges nothing.
 template <typename R>
 __global__ __launch_bounds__(256) void k_ghost_copy(GridP g, Ptrs6<R> F)
 {
@@ -251,9 +272,10 @@ __global__ __launch_bounds__(256) void k_ghost_copy(GridP g, Ptrs6<R> F)
     int j = (int)(r % g.my) + 1;
     int k = (int)(r / g.my) + 1;
     int si = i, sj = j, sk = k;
-    if (i <= 2) si = i + g.nx; else if (i >= g.mx - 2) si = i - g.nx;
-    if (j <= 2) sj = j + g.ny; else if (j >= g.my - 2) sj = j - g.ny;
-    if (g.ndim == 3) { if (k <= 2) sk = k + g.nz; else if (k >= g.mz - 2) sk = k - g.nz; }
+    if (g.per[0]) { if (i <= 2) si = i + g.nx; else if (i >= g.mx - 2) si = i - g.nx; }
+    const bool xphys = si >= 3 && si <= g.mx - 3;
+    if (g.per[1] && xphys) { if (j <= 2) sj = j + g.ny; else if (j >= g.my - 2) sj = j - g.ny; }
+    if (g.ndim == 3 && g.per[2] && xphys) { if (k <= 2) sk = k + g.nz; else if (k >= g.mz - 2) sk = k - g.nz; }
     if (si == i && sj == j && sk == k) return;
     long long s = fidx(g, si, sj, sk);
     F0[t] = F0[s];
@@ -446,7 +468,7 @@ inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / 
 template <typename R>
 void launch_update_b_half(const GridP &g, const PhysP &ph, R *const E[3], R *const B[3], hipStream_t s)
 {
-    hipLaunchKernelGGL(k_update_b_half<R>, phys_grid(g), dim3(BX, BY), 0, s, g, (R)ph.fld_halfc, (R)ph.inv_dx,
+    hipLaunchKernelGGL(k_update_b_half<R>, yee_grid(g), dim3(BX, BY), 0, s, g, (R)ph.fld_halfc, (R)ph.inv_dx,
                        (R)ph.inv_dy, (R)ph.inv_dz, E[0], E[1], E[2], B[0], B[1], B[2]);
 }
 
@@ -463,18 +485,20 @@ void launch_update_b_half_lehe(const GridP &g, const PhysP &ph, const double leh
 template <typename R>
 void launch_update_e(const GridP &g, const PhysP &ph, R *const E[3], R *const B[3], hipStream_t s)
 {
-    hipLaunchKernelGGL((k_update_e<R, double, true, false>), phys_grid(g), dim3(BX, BY), 0, s, g, (R)ph.fldc,
-                       (R)ph.inv_dx, (R)ph.inv_dy, (R)ph.inv_dz, B[0], B[1], B[2], E[0], E[1], E[2],
-                       (const double *)nullptr, (const double *)nullptr, (const double *)nullptr, 0.0);
+    const int wide = ph.fdtd == 0;
+    hipLaunchKernelGGL((k_update_e<R, double, true, false>), wide ? yee_grid(g) : phys_grid(g), dim3(BX, BY), 0, s,
+                       g, (R)ph.fldc, (R)ph.inv_dx, (R)ph.inv_dy, (R)ph.inv_dz, B[0], B[1], B[2], E[0], E[1], E[2],
+                       (const double *)nullptr, (const double *)nullptr, (const double *)nullptr, 0.0, wide);
 }
 
 template <typename R, typename JT>
 void launch_update_e_add_current(const GridP &g, const PhysP &ph, R *const E[3], R *const B[3], JT *const J[3],
                                  double jscale, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_update_e<R, JT, true, true>), phys_grid(g), dim3(BX, BY), 0, s, g, (R)ph.fldc,
-                       (R)ph.inv_dx, (R)ph.inv_dy, (R)ph.inv_dz, B[0], B[1], B[2], E[0], E[1], E[2], J[0], J[1],
-                       J[2], jscale);
+    const int wide = ph.fdtd == 0;
+    hipLaunchKernelGGL((k_update_e<R, JT, true, true>), wide ? yee_grid(g) : phys_grid(g), dim3(BX, BY), 0, s, g,
+                       (R)ph.fldc, (R)ph.inv_dx, (R)ph.inv_dy, (R)ph.inv_dz, B[0], B[1], B[2], E[0], E[1], E[2], J[0],
+                       J[1], J[2], jscale, wide);
 }
 
 template <typename R, typename JT>
@@ -483,7 +507,7 @@ void launch_add_current(const GridP &g, const PhysP &ph, R *const E[3], JT *cons
 {
     hipLaunchKernelGGL((k_update_e<R, JT, false, true>), phys_grid(g), dim3(BX, BY), 0, s, g, (R)ph.fldc,
                        (R)ph.inv_dx, (R)ph.inv_dy, (R)ph.inv_dz, (const R *)nullptr, (const R *)nullptr,
-                       (const R *)nullptr, E[0], E[1], E[2], J[0], J[1], J[2], jscale);
+                       (const R *)nullptr, E[0], E[1], E[2], J[0], J[1], J[2], jscale, 0);
 }
 
 template <typename R>
@@ -499,6 +523,7 @@ void launch_current_fold(const GridP &g, JT *const J[3], hipStream_t s, JT *cons
     Ptrs6<JT> P{{J[0], J[1], J[2], K ? K[0] : nullptr, K ? K[1] : nullptr, K ? K[2] : nullptr}};
     int naxes = (g.ndim == 3) ? 3 : 2;
     for (int a = 0; a < naxes; a++) {
+        if (!g.per[a]) continue;   // null neighbours: the ghost current stays where it is
         long long ext[3] = {g.mx, g.my, g.mz};
         if (a != 0) ext[0] = g.mx - 5;
         ext[a] = 5;
@@ -571,15 +596,20 @@ void launch_update_e_pair(const GridP &g, const PhysP &ph, float *const E[3], fl
 {
     const float *b0 = B ? B[0] : nullptr, *b1 = B ? B[1] : nullptr, *b2 = B ? B[2] : nullptr;
     const double *j0 = J ? J[0] : nullptr, *j1 = J ? J[1] : nullptr, *j2 = J ? J[2] : nullptr;
+    const int wide = curl && ph.fdtd == 0;
+    const dim3 grid = wide ? yee_grid(g) : phys_grid(g);
     if (curl && add)
-        hipLaunchKernelGGL((k_update_e_pair<true, true>), phys_grid(g), dim3(BX, BY), 0, s, g, ph.fldc, ph.inv_dx,
-                           ph.inv_dy, ph.inv_dz, b0, b1, b2, E[0], E[1], E[2], L[0], L[1], L[2], j0, j1, j2, jscale);
+        hipLaunchKernelGGL((k_update_e_pair<true, true>), grid, dim3(BX, BY), 0, s, g, ph.fldc, ph.inv_dx,
+                           ph.inv_dy, ph.inv_dz, b0, b1, b2, E[0], E[1], E[2], L[0], L[1], L[2], j0, j1, j2, jscale,
+                           wide);
     else if (curl)
-        hipLaunchKernelGGL((k_update_e_pair<true, false>), phys_grid(g), dim3(BX, BY), 0, s, g, ph.fldc, ph.inv_dx,
-                           ph.inv_dy, ph.inv_dz, b0, b1, b2, E[0], E[1], E[2], L[0], L[1], L[2], j0, j1, j2, jscale);
+        hipLaunchKernelGGL((k_update_e_pair<true, false>), grid, dim3(BX, BY), 0, s, g, ph.fldc, ph.inv_dx,
+                           ph.inv_dy, ph.inv_dz, b0, b1, b2, E[0], E[1], E[2], L[0], L[1], L[2], j0, j1, j2, jscale,
+                           wide);
     else
-        hipLaunchKernelGGL((k_update_e_pair<false, true>), phys_grid(g), dim3(BX, BY), 0, s, g, ph.fldc, ph.inv_dx,
-                           ph.inv_dy, ph.inv_dz, b0, b1, b2, E[0], E[1], E[2], L[0], L[1], L[2], j0, j1, j2, jscale);
+        hipLaunchKernelGGL((k_update_e_pair<false, true>), grid, dim3(BX, BY), 0, s, g, ph.fldc, ph.inv_dx,
+                           ph.inv_dy, ph.inv_dz, b0, b1, b2, E[0], E[1], E[2], L[0], L[1], L[2], j0, j1, j2, jscale,
+                           wide);
 }
 
 void launch_pair_from_f64(const double *src, float *hi, float *lo, long long n, hipStream_t s)

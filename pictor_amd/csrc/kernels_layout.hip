@@ -234,8 +234,11 @@ __global__ void k_tile_ids(TileP tp, const long long *dst_off, int *tile_out)
     for (long long k = a + threadIdx.x; k < b; k += blockDim.x) tile_out[k] = grp / tp.nsp;
 }
 
-__device__ inline int fold_node(int n, int m, int len)
+// a node index folded onto the physical nodes of a periodic axis (per = 1);
+// a non-periodic axis keeps its ghost nodes
+__device__ inline int fold_node(int n, int m, int len, int per)
 {
+    if (!per) return n;
     if (n < 3) return n + len;
     if (n > m - 3) return n - len;
     return n;
@@ -274,7 +277,7 @@ __device__ inline void particle_diag_tile(const GridP &g, const PhysP &ph, const
             if (g.ndim == 2) {
                 for (int bb = 0; bb < 2; bb++)
                     for (int aa = 0; aa < 2; aa++) {
-                        const int ii = fold_node(i + aa, g.mx, g.nx), jj = fold_node(j + bb, g.my, g.ny);
+                        const int ii = fold_node(i + aa, g.mx, g.nx, g.per[0]), jj = fold_node(j + bb, g.my, g.ny, g.per[1]);
                         unsafeAtomicAdd(&rho[fidx(g, ii, jj, 1)], qt * sx[aa] * sy[bb]);
                     }
             } else {
@@ -284,8 +287,8 @@ __device__ inline void particle_diag_tile(const GridP &g, const PhysP &ph, const
                 for (int cc = 0; cc < 2; cc++)
                     for (int bb = 0; bb < 2; bb++)
                         for (int aa = 0; aa < 2; aa++) {
-                            const int ii = fold_node(i + aa, g.mx, g.nx), jj = fold_node(j + bb, g.my, g.ny);
-                            const int kk = fold_node(kk0 + cc, g.mz, g.nz);
+                            const int ii = fold_node(i + aa, g.mx, g.nx, g.per[0]), jj = fold_node(j + bb, g.my, g.ny, g.per[1]);
+                            const int kk = fold_node(kk0 + cc, g.mz, g.nz, g.per[2]);
                             unsafeAtomicAdd(&rho[fidx(g, ii, jj, kk)], qt * sx[aa] * sy[bb] * sz[cc]);
                         }
             }

@@ -32,7 +32,7 @@ using namespace pushdev;
 // ---------------------------------------------------------------------------
 // FP64 form, 2-D
 // ---------------------------------------------------------------------------
-template <int T, int BS>
+template <int T, int BS, bool BC>
 __global__ __launch_bounds__(BS) void k_push2d_f64(GridP g, PhysP ph, TileP tp, const double *__restrict__ Ex,
                                                    const double *__restrict__ Ey, const double *__restrict__ Ez,
                                                    const double *__restrict__ Bx, const double *__restrict__ By,
@@ -183,34 +183,56 @@ __global__ __launch_bounds__(BS) void k_push2d_f64(GridP g, PhysP ph, TileP tp, 
                 // 2-D z wrap, movdep.F90:241-249
                 if (zn > ph.zmax) zn = zn - ph.zlen;
                 else if (zn < ph.zmin) zn = ph.zlen + zn;
-                // periodic exchange, loadprtlout.F90:31-76 + shifts subdomains.F90:720-740
+                // EnforceBC_PostMovDep (bc.F90:238-287)
+                bool keep = true;
+                if (BC) keep = particle_faces<double>(ph, g, xn, yn, zn, un, vn);
+                // periodic exchange, loadprtlout.F90:31-76 + shifts subdomains.F90:720-740; a
+                // face of a non-periodic axis sends to a null neighbour (the particle is gone)
                 bool sent = false;
-                if (xn < ph.xmin) {
-                    if ((yn <= (ph.ymax + ph.xmin - xn)) && (yn >= (ph.ymin - ph.xmin + xn))) { xn = xn + ph.xlen; sent = true; }
-                } else if (xn > ph.xmax) {
-                    if ((yn <= (ph.ymax - ph.xmax + xn)) && (yn >= (ph.ymin + ph.xmax - xn))) { xn = xn - ph.xlen; sent = true; }
-                }
-                if (!sent) {
-                    if (yn < ph.ymin) {
-                        if ((yn < (ph.ymin - ph.xmin + xn)) && (yn < (ph.ymin + ph.xmax - xn))) yn = yn + ph.ylen;
-                    } else if (yn > ph.ymax) {
-                        if ((yn > (ph.ymax + ph.xmin - xn)) && (yn > (ph.ymax - ph.xmax + xn))) yn = yn - ph.ylen;
+                if (keep) {
+                    if (xn < ph.xmin) {
+                        if ((yn <= (ph.ymax + ph.xmin - xn)) && (yn >= (ph.ymin - ph.xmin + xn))) {
+                            if (BC && !g.per[0]) keep = false;
+                            else xn = xn + ph.xlen;
+                            sent = true;
+                        }
+                    } else if (xn > ph.xmax) {
+                        if ((yn <= (ph.ymax - ph.xmax + xn)) && (yn >= (ph.ymin + ph.xmax - xn))) {
+                            if (BC && !g.per[0]) keep = false;
+                            else xn = xn - ph.xlen;
+                            sent = true;
+                        }
                     }
                 }
-                int cx = (int)xn - 3, cy = (int)yn - 3;
-                cx = cx < 0 ? 0 : (cx > g.nx - 1 ? g.nx - 1 : cx);
-                cy = cy < 0 ? 0 : (cy > g.ny - 1 ? g.ny - 1 : cy);
-                const int ddx = norm_dir(cx / T - tx, tp.ntx), ddy = norm_dir(cy / T - ty, tp.nty);
-                if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1) {
-                    atomicOr(sp.err, (int)kErrBounds);
-                    d = 4;
-                } else {
-                    d = (ddx + 1) + 3 * (ddy + 1);
+                if (keep && !sent) {
+                    if (yn < ph.ymin) {
+                        if ((yn < (ph.ymin - ph.xmin + xn)) && (yn < (ph.ymin + ph.xmax - xn))) {
+                            if (BC && !g.per[1]) keep = false;
+                            else yn = yn + ph.ylen;
+                        }
+                    } else if (yn > ph.ymax) {
+                        if ((yn > (ph.ymax + ph.xmin - xn)) && (yn > (ph.ymax - ph.xmax + xn))) {
+                            if (BC && !g.per[1]) keep = false;
+                            else yn = yn - ph.ylen;
+                        }
+                    }
+                }
+                if (keep) {
+                    int cx = (int)xn - 3, cy = (int)yn - 3;
+                    cx = cx < 0 ? 0 : (cx > g.nx - 1 ? g.nx - 1 : cx);
+                    cy = cy < 0 ? 0 : (cy > g.ny - 1 ? g.ny - 1 : cy);
+                    const int ddx = norm_dir(cx / T - tx, tp.ntx), ddy = norm_dir(cy / T - ty, tp.nty);
+                    if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1) {
+                        atomicOr(sp.err, (int)kErrBounds);
+                        d = 4;
+                    } else {
+                        d = (ddx + 1) + 3 * (ddy + 1);
+                    }
                 }
             }
         }
         const int slot = reserve_slot(d, sCnt);
-        if (active) {
+        if (active && d >= 0) {
             if (slot < sDstCap[d]) {
                 const long long o = sDstBase[d] + slot;
                 Bo.x[o] = xn; Bo.y[o] = yn; Bo.z[o] = zn;
@@ -259,7 +281,7 @@ __device__ inline void vbin(int v, int nsp, int &s, int &d)
     d = q < 4 ? q : q + 1;
 }
 
-template <int T, int BS, bool TRACK = false>
+template <int T, int BS, bool TRACK = false, bool BC = false>
 __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp, const float *__restrict__ Ex,
                                                      const float *__restrict__ Ey, const float *__restrict__ Ez,
                                                      const float *__restrict__ Bx, const float *__restrict__ By,
@@ -442,15 +464,40 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                     atomicAdd(jp + 2 * WW + W + 1, b[7]);
                 }
                 int nci = ci + di, ncj = cj + dj;
-                nci = nci < 0 ? nci + nx : (nci >= nx ? nci - nx : nci);
-                ncj = ncj < 0 ? ncj + ny : (ncj >= ny ? ncj - ny : ncj);
-                ncell = nci | (ncj << 16);
-                const int ddx = norm_dir(nci / T - tx, tp.ntx), ddy = norm_dir(ncj / T - ty, tp.nty);
-                d = cur.s * 9 + (ddx + 1) + 3 * (ddy + 1);
+                bool keep = true;
+                if (BC) {
+                    // the boundary faces on the FP64 position (exact for the cell and FP32
+                    // offset); a reflected coordinate is split again into cell and offset
+                    double X = 3.0 + (double)nci + (double)ox, Y = 3.0 + (double)ncj + (double)oy;
+                    const double X0 = X, Y0 = Y;
+                    keep = particle_faces<float>(ph, g, X, Y, 1.5, un, vn);
+                    if (keep && X != X0) {
+                        const double f = floor(X);
+                        nci = (int)f - 3;
+                        ox = (float)(X - f);
+                        if (ox >= 1.0f) { ox = 0.0f; nci += 1; }
+                    }
+                    if (keep && Y != Y0) {
+                        const double f = floor(Y);
+                        ncj = (int)f - 3;
+                        oy = (float)(Y - f);
+                        if (oy >= 1.0f) { oy = 0.0f; ncj += 1; }
+                    }
+                    // a non-periodic axis has no image: the particle leaving it is gone
+                    if ((nci < 0 || nci >= nx) && !g.per[0]) keep = false;
+                    if ((ncj < 0 || ncj >= ny) && !g.per[1]) keep = false;
+                }
+                if (keep) {
+                    nci = nci < 0 ? nci + nx : (nci >= nx ? nci - nx : nci);
+                    ncj = ncj < 0 ? ncj + ny : (ncj >= ny ? ncj - ny : ncj);
+                    ncell = nci | (ncj << 16);
+                    const int ddx = norm_dir(nci / T - tx, tp.ntx), ddy = norm_dir(ncj / T - ty, tp.nty);
+                    d = cur.s * 9 + (ddx + 1) + 3 * (ddy + 1);
+                }
             }
         }
         const int slot = reserve_slot(d, sCnt);
-        if (active) {
+        if (active && d >= 0) {
             if (slot < sDstCap[d]) {
                 const unsigned off = (sDstBase[d] + (unsigned)slot) * 4u;
                 sto(Bo.cell, off, ncell); sto(Bo.ox, off, ox); sto(Bo.oy, off, oy);
@@ -494,8 +541,14 @@ void launch_push2d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, 
 {
     dim3 grid(tp.ntiles);
 #define PUSH_F64(TT, BSZ)                                                                                          \
-    hipLaunchKernelGGL((k_push2d_f64<TT, BSZ>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], B[0], B[1], \
-                       B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp)
+    do {                                                                                                           \
+        if (ph.pbc_on)                                                                                             \
+            hipLaunchKernelGGL((k_push2d_f64<TT, BSZ, true>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2],  \
+                               B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp);                         \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_push2d_f64<TT, BSZ, false>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], \
+                               B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp);                         \
+    } while (0)
     switch (T) {
     case 8: PUSH_F64(8, 256); break;
     case 16: PUSH_F64(16, 256); break;
@@ -511,8 +564,14 @@ void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp
 {
     dim3 grid(tp.ntiles);
 #define PUSH_SPLIT(TT, BSZ)                                                                                         \
-    hipLaunchKernelGGL((k_push2d_split<TT, BSZ>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], B[0], B[1], \
-                       B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp, (int *)nullptr)
+    do {                                                                                                            \
+        if (ph.pbc_on)                                                                                              \
+            hipLaunchKernelGGL((k_push2d_split<TT, BSZ, false, true>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], \
+                               E[2], B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp, (int *)nullptr);     \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_push2d_split<TT, BSZ>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], B[0],  \
+                               B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp, (int *)nullptr);                \
+    } while (0)
     switch (T) {
     case 8: PUSH_SPLIT(8, 256); break;
     case 16: PUSH_SPLIT(16, 256); break;

@@ -17,7 +17,7 @@ using namespace pushdev;
 // FP64 form, 3-D.  Gather window: nodes [i0-1, i0+T+1] (particles may sit one
 // cell into a ghost layer after the exchange); current window [i0-2, i0+T+2].
 // ---------------------------------------------------------------------------
-template <int T, int BS>
+template <int T, int BS, bool BC>
 __global__ __launch_bounds__(BS) void k_push3d_f64(GridP g, PhysP ph, TileP tp, const double *__restrict__ Ex,
                                                    const double *__restrict__ Ey, const double *__restrict__ Ez,
                                                    const double *__restrict__ Bx, const double *__restrict__ By,
@@ -162,49 +162,73 @@ __global__ __launch_bounds__(BS) void k_push3d_f64(GridP g, PhysP ph, TileP tp, 
                     atomicAdd(&sJ[2][e + sy], (Fz * (1.0 - Wx)) * Wy);
                     atomicAdd(&sJ[2][e + 1 + sy], (Fz * Wx) * Wy);
                 }
-                // periodic exchange, loadprtlout.F90:31-96 (3-D rules)
+                // EnforceBC_PostMovDep (bc.F90:238-287)
+                bool keep = true;
+                if (BC) keep = particle_faces<double>(ph, g, xn, yn, zn, un, vn);
+                // periodic exchange, loadprtlout.F90:31-96 (3-D rules); a face of a
+                // non-periodic axis sends to a null neighbour (the particle is gone)
                 bool sent = false;
-                if (xn < ph.xmin) {
-                    if ((yn <= (ph.ymax + ph.xmin - xn)) && (yn >= (ph.ymin - ph.xmin + xn)) &&
-                        (zn <= (ph.zmax + ph.xmin - xn)) && (zn >= (ph.zmin - ph.xmin + xn))) { xn = xn + ph.xlen; sent = true; }
-                } else if (xn > ph.xmax) {
-                    if ((yn <= (ph.ymax - ph.xmax + xn)) && (yn >= (ph.ymin + ph.xmax - xn)) &&
-                        (zn <= (ph.zmax - ph.xmax + xn)) && (zn >= (ph.zmin + ph.xmax - xn))) { xn = xn - ph.xlen; sent = true; }
+                if (keep) {
+                    if (xn < ph.xmin) {
+                        if ((yn <= (ph.ymax + ph.xmin - xn)) && (yn >= (ph.ymin - ph.xmin + xn)) &&
+                            (zn <= (ph.zmax + ph.xmin - xn)) && (zn >= (ph.zmin - ph.xmin + xn))) {
+                            if (BC && !g.per[0]) keep = false; else xn = xn + ph.xlen;
+                            sent = true;
+                        }
+                    } else if (xn > ph.xmax) {
+                        if ((yn <= (ph.ymax - ph.xmax + xn)) && (yn >= (ph.ymin + ph.xmax - xn)) &&
+                            (zn <= (ph.zmax - ph.xmax + xn)) && (zn >= (ph.zmin + ph.xmax - xn))) {
+                            if (BC && !g.per[0]) keep = false; else xn = xn - ph.xlen;
+                            sent = true;
+                        }
+                    }
                 }
-                if (!sent) {
+                if (keep && !sent) {
                     if (yn < ph.ymin) {
                         if ((yn < (ph.ymin - ph.xmin + xn)) && (yn < (ph.ymin + ph.xmax - xn)) &&
-                            (zn >= (ph.zmin - ph.ymin + yn)) && (zn <= (ph.zmax + ph.ymin - yn))) { yn = yn + ph.ylen; sent = true; }
+                            (zn >= (ph.zmin - ph.ymin + yn)) && (zn <= (ph.zmax + ph.ymin - yn))) {
+                            if (BC && !g.per[1]) keep = false; else yn = yn + ph.ylen;
+                            sent = true;
+                        }
                     } else if (yn > ph.ymax) {
                         if ((yn > (ph.ymax + ph.xmin - xn)) && (yn > (ph.ymax - ph.xmax + xn)) &&
-                            (zn >= (ph.zmin + ph.ymax - yn)) && (zn <= (ph.zmax - ph.ymax + yn))) { yn = yn - ph.ylen; sent = true; }
+                            (zn >= (ph.zmin + ph.ymax - yn)) && (zn <= (ph.zmax - ph.ymax + yn))) {
+                            if (BC && !g.per[1]) keep = false; else yn = yn - ph.ylen;
+                            sent = true;
+                        }
                     }
                 }
-                if (!sent) {
+                if (keep && !sent) {
                     if (zn < ph.zmin) {
                         if ((zn < (ph.zmin - ph.xmin + xn)) && (zn < (ph.zmin + ph.xmax - xn)) &&
-                            (zn < (ph.zmin - ph.ymin + yn)) && (zn < (ph.zmin + ph.ymax - yn))) zn = zn + ph.zlen;
+                            (zn < (ph.zmin - ph.ymin + yn)) && (zn < (ph.zmin + ph.ymax - yn))) {
+                            if (BC && !g.per[2]) keep = false; else zn = zn + ph.zlen;
+                        }
                     } else if (zn > ph.zmax) {
                         if ((zn > (ph.zmax + ph.xmin - xn)) && (zn > (ph.zmax - ph.xmax + xn)) &&
-                            (zn > (ph.zmax + ph.ymin - yn)) && (zn > (ph.zmax - ph.ymax + yn))) zn = zn - ph.zlen;
+                            (zn > (ph.zmax + ph.ymin - yn)) && (zn > (ph.zmax - ph.ymax + yn))) {
+                            if (BC && !g.per[2]) keep = false; else zn = zn - ph.zlen;
+                        }
                     }
                 }
-                int cx = (int)xn - 3, cy = (int)yn - 3, cz = (int)zn - 3;
-                cx = cx < 0 ? 0 : (cx > g.nx - 1 ? g.nx - 1 : cx);
-                cy = cy < 0 ? 0 : (cy > g.ny - 1 ? g.ny - 1 : cy);
-                cz = cz < 0 ? 0 : (cz > g.nz - 1 ? g.nz - 1 : cz);
-                const int ddx = norm_dir(cx / T - tx, tp.ntx), ddy = norm_dir(cy / T - ty, tp.nty);
-                const int ddz = norm_dir(cz / T - tz, tp.ntz);
-                if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1 || ddz < -1 || ddz > 1) {
-                    atomicOr(sp.err, (int)kErrBounds);
-                    d = 13;
-                } else {
-                    d = (ddx + 1) + 3 * (ddy + 1) + 9 * (ddz + 1);
+                if (keep) {
+                    int cx = (int)xn - 3, cy = (int)yn - 3, cz = (int)zn - 3;
+                    cx = cx < 0 ? 0 : (cx > g.nx - 1 ? g.nx - 1 : cx);
+                    cy = cy < 0 ? 0 : (cy > g.ny - 1 ? g.ny - 1 : cy);
+                    cz = cz < 0 ? 0 : (cz > g.nz - 1 ? g.nz - 1 : cz);
+                    const int ddx = norm_dir(cx / T - tx, tp.ntx), ddy = norm_dir(cy / T - ty, tp.nty);
+                    const int ddz = norm_dir(cz / T - tz, tp.ntz);
+                    if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1 || ddz < -1 || ddz > 1) {
+                        atomicOr(sp.err, (int)kErrBounds);
+                        d = 13;
+                    } else {
+                        d = (ddx + 1) + 3 * (ddy + 1) + 9 * (ddz + 1);
+                    }
                 }
             }
         }
         const int slot = reserve_slot(d, sCnt);
-        if (active) {
+        if (active && d >= 0) {
             if (slot < sDstCap[d]) {
                 const long long o = sDstBase[d] + slot;
                 Bo.x[o] = xn; Bo.y[o] = yn; Bo.z[o] = zn;
@@ -242,7 +266,7 @@ struct SplitRec3 {
     float ox, oy, oz, u, v, w;
 };
 
-template <int T, int BS>
+template <int T, int BS, bool BC>
 __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp, const float *__restrict__ Ex,
                                                      const float *__restrict__ Ey, const float *__restrict__ Ez,
                                                      const float *__restrict__ Bx, const float *__restrict__ By,
@@ -425,17 +449,43 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
                 oz = z1 - (float)dk;
                 if (oz >= 1.0f) { oz = 0.0f; dk += 1; }
                 int nci = ci + di, ncj = cj + dj, nck = ck + dk;
-                nci = nci < 0 ? nci + nx : (nci >= nx ? nci - nx : nci);
-                ncj = ncj < 0 ? ncj + ny : (ncj >= ny ? ncj - ny : ncj);
-                nck = nck < 0 ? nck + nz : (nck >= nz ? nck - nz : nck);
-                ncell = cell_pack(3, nci, ncj, nck);
-                const int ddx = norm_dir(nci / T - tx, tp.ntx), ddy = norm_dir(ncj / T - ty, tp.nty);
-                const int ddz = norm_dir(nck / T - tz, tp.ntz);
-                d = cur.s * 27 + (ddx + 1) + 3 * (ddy + 1) + 9 * (ddz + 1);
+                bool keep = true;
+                if (BC) {
+                    // the boundary faces on the FP64 position; a reflected coordinate is
+                    // split again into cell and offset
+                    double X = 3.0 + (double)nci + (double)ox, Y = 3.0 + (double)ncj + (double)oy;
+                    const double Z = 3.0 + (double)nck + (double)oz;
+                    const double X0 = X, Y0 = Y;
+                    keep = particle_faces<float>(ph, g, X, Y, Z, un, vn);
+                    if (keep && X != X0) {
+                        const double f = floor(X);
+                        nci = (int)f - 3;
+                        ox = (float)(X - f);
+                        if (ox >= 1.0f) { ox = 0.0f; nci += 1; }
+                    }
+                    if (keep && Y != Y0) {
+                        const double f = floor(Y);
+                        ncj = (int)f - 3;
+                        oy = (float)(Y - f);
+                        if (oy >= 1.0f) { oy = 0.0f; ncj += 1; }
+                    }
+                    if ((nci < 0 || nci >= nx) && !g.per[0]) keep = false;
+                    if ((ncj < 0 || ncj >= ny) && !g.per[1]) keep = false;
+                    if ((nck < 0 || nck >= nz) && !g.per[2]) keep = false;
+                }
+                if (keep) {
+                    nci = nci < 0 ? nci + nx : (nci >= nx ? nci - nx : nci);
+                    ncj = ncj < 0 ? ncj + ny : (ncj >= ny ? ncj - ny : ncj);
+                    nck = nck < 0 ? nck + nz : (nck >= nz ? nck - nz : nck);
+                    ncell = cell_pack(3, nci, ncj, nck);
+                    const int ddx = norm_dir(nci / T - tx, tp.ntx), ddy = norm_dir(ncj / T - ty, tp.nty);
+                    const int ddz = norm_dir(nck / T - tz, tp.ntz);
+                    d = cur.s * 27 + (ddx + 1) + 3 * (ddy + 1) + 9 * (ddz + 1);
+                }
             }
         }
         const int slot = reserve_slot(d, sCnt);
-        if (active) {
+        if (active && d >= 0) {
             if (slot < sDstCap[d]) {
                 const unsigned off = (sDstBase[d] + (unsigned)slot) * 4u;
                 sto(Bo.cell, off, ncell); sto(Bo.ox, off, ox); sto(Bo.oy, off, oy); sto(Bo.oz, off, oz);
@@ -474,8 +524,14 @@ void launch_push3d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, 
 {
     dim3 grid(tp.ntiles);
 #define PUSH3_F64(TT, BSZ)                                                                                          \
-    hipLaunchKernelGGL((k_push3d_f64<TT, BSZ>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], B[0], B[1], \
-                       B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp)
+    do {                                                                                                           \
+        if (ph.pbc_on)                                                                                             \
+            hipLaunchKernelGGL((k_push3d_f64<TT, BSZ, true>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2],  \
+                               B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp);                         \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_push3d_f64<TT, BSZ, false>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], \
+                               B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp);                         \
+    } while (0)
     switch (T) {
     case 4: PUSH3_F64(4, 256); break;
     case 8: PUSH3_F64(8, 512); break;
@@ -490,8 +546,14 @@ void launch_push3d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp
 {
     dim3 grid(tp.ntiles);
 #define PUSH3_SPLIT(TT, BSZ)                                                                                         \
-    hipLaunchKernelGGL((k_push3d_split<TT, BSZ>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], B[0], B[1], \
-                       B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp)
+    do {                                                                                                            \
+        if (ph.pbc_on)                                                                                              \
+            hipLaunchKernelGGL((k_push3d_split<TT, BSZ, true>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], \
+                               B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp);                          \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_push3d_split<TT, BSZ, false>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1],      \
+                               E[2], B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp);                    \
+    } while (0)
     switch (T) {
     case 4: PUSH3_SPLIT(4, 256); break;
     case 8: PUSH3_SPLIT(8, 512); break;

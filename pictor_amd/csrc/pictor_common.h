@@ -29,7 +29,28 @@ enum ErrBits : int { kErrBounds = 1, kErrCapacity = 2, kErrPrecision = 4 };
 struct GridP {
     int ndim, nx, ny, nz, mx, my, mz;
     long long ncell;   // mx*my*mz
+    int per[3];        // 1: periodic axis; 0 once a boundary face is set on it
 };
+
+// range of cells the Yee update covers along `axis`.  The reference loops over
+// 2..m-1 (em_update.F90) and lets the halo overwrite what it reaches.  In the
+// periodic box the halo reaches every ghost, so the physical cells 3..m-3 are
+// enough; once an axis is non-periodic, some ghosts (those of that axis, and
+// where its ghosts meet another axis's, which the trimmed faces skip) keep the
+// updated values, so every axis is updated over 2..m-1 as in the reference.
+__host__ __device__ inline bool yee_full(const GridP &g)
+{
+    return !(g.per[0] && g.per[1] && (g.ndim == 2 || g.per[2]));
+}
+__host__ __device__ inline int yee_lo(const GridP &g, int axis)
+{
+    return (axis == 2 && g.ndim == 2) ? 1 : (yee_full(g) ? 2 : 3);
+}
+__host__ __device__ inline int yee_hi(const GridP &g, int axis)
+{
+    const int m = axis == 0 ? g.mx : (axis == 1 ? g.my : g.mz);
+    return (axis == 2 && g.ndim == 2) ? 1 : (yee_full(g) ? m - 1 : m - 3);
+}
 
 struct PhysP {
     double c, sqc, cinv, qi, fldc, fld_halfc;
@@ -42,7 +63,16 @@ struct PhysP {
     int nflvr;
     int fdtd;          // 0 Yee, 1 Lehe
     double lehe[12];   // Lehe coefficients (fdtd 1)
+    // particle faces of EnforceBC_PostMovDep: type per face (x lo, x hi, y lo,
+    // y hi, z lo, z hi; 0 none, kBcRefl, kBcRemv) and local position (global + 3);
+    // pbc_on: a face is set or an axis is not periodic (the push's face variant)
+    int pbc[6];
+    double pbc_pos[6];
+    int pbc_on;
 };
+
+// particle face types (the C ABI's PICTOR_BC_REFL / PICTOR_BC_REMV)
+constexpr int kBcRefl = 4, kBcRemv = 5;
 
 struct TileP {
     int T, ntx, nty, ntz, ntiles;

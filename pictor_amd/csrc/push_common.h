@@ -87,6 +87,41 @@ __device__ inline void zigzag2d_split(double qs, float x0, float y0, float z0, f
     }
 }
 
+// EnforceBC_PostMovDep (bc.F90:238-287) on one pushed particle, in the
+// reference's order: the reflecting faces top, bottom, right, left
+// (RefBC_Prtl_*, bc_reflect.F90:76-186: specular image 2 x_w - x with the
+// normal momentum reversed, only where the wall lies inside the array), then
+// every face again in order x lo .. z hi, reflecting once more (which changes
+// nothing then) or removing the particle beyond a 'remv' face (bc_open.F90).
+// Positions are the reference's local coordinates.  Returns false when the
+// particle is removed.  The current of a reflected particle is folded back by
+// the face kernels after the push.
+template <typename R>
+__device__ inline bool particle_faces(const PhysP &ph, const GridP &g, double &x, double &y, double z, R &u, R &v)
+{
+    const int *t = ph.pbc;
+    const double *p = ph.pbc_pos;
+    auto top = [&]() { if (p[3] < g.my - 1 && y > p[3]) { v = -v; y = 2.0 * p[3] - y; } };
+    auto bottom = [&]() { if (p[2] > 1 && y < p[2]) { v = -v; y = 2.0 * p[2] - y; } };
+    auto right = [&]() { if (p[1] < g.mx - 1 && x > p[1]) { u = -u; x = 2.0 * p[1] - x; } };
+    auto left = [&]() { if (p[0] > 1 && x < p[0]) { u = -u; x = 2.0 * p[0] - x; } };
+    if (t[3] == kBcRefl) top();
+    if (t[2] == kBcRefl) bottom();
+    if (t[1] == kBcRefl) right();
+    if (t[0] == kBcRefl) left();
+    if (t[0] == kBcRefl) left();
+    else if (t[0] == kBcRemv && x < p[0]) return false;
+    if (t[1] == kBcRefl) right();
+    else if (t[1] == kBcRemv && x > p[1]) return false;
+    if (t[2] == kBcRefl) bottom();
+    else if (t[2] == kBcRemv && y < p[2]) return false;
+    if (t[3] == kBcRefl) top();
+    else if (t[3] == kBcRemv && y > p[3]) return false;
+    if (t[4] == kBcRemv && z < p[4]) return false;
+    if (t[5] == kBcRemv && z > p[5]) return false;
+    return true;
+}
+
 // byte-offset accessors: a uniform base plus a 32-bit lane offset lets the
 // compiler use SGPR-base addressing (one VALU op per access)
 template <typename T>

@@ -14,6 +14,10 @@
 //   particles  q, x, y, z, u, v, w (FP64) and flv (int32), n values each, the
 //              reference SoA; it holds the split record exactly (cell + FP32 offset
 //              and FP32 momenta convert to FP64 and back without rounding)
+//   faces      (version 2) the boundary faces with their current positions and
+//              speeds, then the 12 PML auxiliary fields (FP64) if a PML face is set;
+//              a restart takes the face positions from the file, as the reference's
+//              SetBC_* skip the setup position when restarting (bc.F90:30, 58)
 //
 // J is not saved: every step starts by resetting it (MAIN.F90:248).  A state loads
 // into a context created with the same configuration; the tile edge may differ.
@@ -47,7 +51,7 @@ struct StateHeader {
     int32_t reserved[7];
 };
 constexpr char kMagic[8] = {'P', 'I', 'C', 'T', 'O', 'R', 'S', 'T'};
-constexpr int32_t kVersion = 1;
+constexpr int32_t kVersion = 2;   // 1: no faces section
 
 struct File {
     FILE *f = nullptr;
@@ -136,6 +140,17 @@ int pictor_save_state(pictor_ctx *c, const char *path, int64_t t)
         for (const auto *col : cols)
             if ((rc = write_all(f.f, col->data(), (size_t)np * sizeof(double), "particles"))) return rc;
         if ((rc = write_all(f.f, flv.data(), (size_t)np * sizeof(int32_t), "particle flavours"))) return rc;
+        BcState bs;
+        bc_get(c, bs);
+        if ((rc = write_all(f.f, &bs, sizeof(bs), "the boundary faces"))) return rc;
+        if (bs.pml_on) {
+            std::vector<double> pb(n);
+            for (int q = 0; q < 12; q++) {
+                HIPTRY(hipMemcpyAsync(pb.data(), c->pml_psi[q], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPTRY(hipStreamSynchronize(c->stream));
+                if ((rc = write_all(f.f, pb.data(), n * sizeof(double), "a PML field"))) return rc;
+            }
+        }
         if (fflush(f.f) != 0) return fail(PICTOR_ERR_ARG, "restart: flush failed");
     }
     // a complete file replaces an older one of the same name only once it is whole
@@ -153,7 +168,7 @@ int pictor_load_state(pictor_ctx *c, const char *path, int64_t *t)
     StateHeader h{};
     int rc = read_all(f.f, &h, sizeof(h), "the header");
     if (rc) return rc;
-    if (memcmp(h.magic, kMagic, sizeof(kMagic)) != 0 || h.version != kVersion)
+    if (memcmp(h.magic, kMagic, sizeof(kMagic)) != 0 || h.version < 1 || h.version > kVersion)
         return fail(PICTOR_ERR_ARG, "restart: not a state file of this version");
     const pictor_config &cf = c->cfg;
     std::vector<void *> arr;
@@ -178,13 +193,30 @@ int pictor_load_state(pictor_ctx *c, const char *path, int64_t *t)
     for (auto *col : cols)
         if ((rc = read_all(f.f, col->data(), (size_t)np * sizeof(double), "particles"))) return rc;
     if ((rc = read_all(f.f, flv.data(), (size_t)np * sizeof(int32_t), "particle flavours"))) return rc;
+    BcState bs;
+    bc_get(c, bs);   // version 1: the context's own faces
+    std::vector<std::vector<double>> psi;
+    if (h.version >= 2) {
+        if ((rc = read_all(f.f, &bs, sizeof(bs), "the boundary faces"))) return rc;
+        if (bs.pml_on) {
+            psi.assign(12, std::vector<double>(n));
+            for (auto &b : psi)
+                if ((rc = read_all(f.f, b.data(), n * sizeof(double), "a PML field"))) return rc;
+        }
+    }
+    if (c->cyl && (bs.per[0] == 0 || bs.per[1] == 0 || bs.per[2] == 0))
+        return fail(PICTOR_ERR_ARG, "restart: boundary faces on a cylindrical grid");
 
-    // particles first: a rejected upload leaves the fields as they were
+    // faces first (the upload's tile assignment does not depend on them), then the
+    // particles: a rejected upload leaves the fields as they were
+    if (!c->cyl && (rc = bc_set(c, bs))) return rc;
     if ((rc = pictor_upload_particles(c, np, q.data(), x.data(), y.data(), z.data(), u.data(), v.data(), w.data(),
                                       flv.data())))
         return rc;
     for (size_t i = 0; i < arr.size(); i++)
         HIPTRY(hipMemcpyAsync(arr[i], fld[i].data(), n * es, hipMemcpyHostToDevice, c->stream));
+    for (size_t q = 0; q < psi.size(); q++)
+        HIPTRY(hipMemcpyAsync(c->pml_psi[q], psi[q].data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPTRY(hipStreamSynchronize(c->stream));
     if ((rc = pictor_reset_current(c))) return rc;
     if (c->cylm) {

@@ -2,10 +2,11 @@
 !
 ! Each phase subroutine below keeps the name and position of the call it
 ! replaces in the reference loop and forwards to the C ABI.  Phases the
-! periodic one-GPU run does not need (BCs, output, load balancing) are left
-! out.  Usage:  pictor_main [name=value ...]  with names from parameters.F90
-! (nx, ny, epc, tfinish, form, fused, fdtd, curr_filter, seed, c, c_ompe,
-! energy_save_period, prtl_reorder_period).
+! one-GPU run does not need (load balancing, particle exchange between ranks)
+! are left out.  Usage:  pictor_main [name=value ...]  with names from
+! parameters.F90 (nx, ny, epc, tfinish, form, fused, fdtd, curr_filter, seed, c,
+! c_ompe, energy_save_period, energy_file, fld_save_period, fsave_ratio,
+! restart_save_period, data_folder, restart_file, prtl_reorder_period).
 program MAIN
    use iso_c_binding
    use parameters
@@ -16,11 +17,19 @@ program MAIN
    integer(c_int64_t) :: c0, c1, crate
    real(c_double) :: secs
 
+   integer :: tstart
+   integer(c_int64_t) :: t_restart
+
    call ReadArgs
    call Initialisations
-   call SaveEnergy(0)
+   tstart = 1
+   if (len_trim(restart_file) > 0) then   ! the restart branch, MAIN.F90:35-39
+      call check(pictor_load_state(gpu, trim(restart_file)//c_null_char, t_restart), 'pictor_load_state')
+      tstart = int(t_restart) + 1
+   end if
+   call SaveEnergy(tstart - 1)
    call system_clock(c0, crate)
-   do t = 1, tfinish
+   do t = tstart, tfinish
       if (fused == 1) then
          call check(pictor_step(gpu, 1_c_int32_t), 'pictor_step')
       else
@@ -28,16 +37,20 @@ program MAIN
          call MoveDepositPrtl           ! MAIN.F90:65/68
          call ExchangePrtl              ! MAIN.F90:88/92 (+ AppendParticles :147)
          call UpdateBfieldHalf          ! MAIN.F90:103/114
+         call UpdateMagFld_PML          ! MAIN.F90:104
          call UpdateEfield              ! MAIN.F90:108/115
+         call UpdateElcFld_PML          ! MAIN.F90:109
          call UpdateCurrentAllFaces     ! MAIN.F90:124/129
          call smoothen_current          ! MAIN.F90:158/162
          call AddCurrent                ! MAIN.F90:171/174, SendRecvFlds(E) :184/190
+         call EnforceBC_Final           ! MAIN.F90:209
          call ResetCurrent              ! MAIN.F90:248/251
          call ReorderPrtl               ! MAIN.F90:261/264
       end if
       if (energy_save_period > 0) then
          if (mod(t, energy_save_period) == 0) call SaveEnergy(t)
       end if
+      call Savedata(t)                  ! MAIN.F90:229
    end do
    call check(pictor_synchronize(gpu), 'pictor_synchronize')
    call system_clock(c1)
@@ -78,6 +91,12 @@ contains
          case ('c_ompe'); read (val, *, iostat=ios) c_ompe
          case ('energy_save_period'); read (val, *, iostat=ios) energy_save_period
          case ('prtl_reorder_period'); read (val, *, iostat=ios) prtl_reorder_period
+         case ('energy_file'); read (val, *, iostat=ios) energy_file
+         case ('fld_save_period'); read (val, *, iostat=ios) fld_save_period
+         case ('fsave_ratio'); read (val, *, iostat=ios) fsave_ratio
+         case ('restart_save_period'); read (val, *, iostat=ios) restart_save_period
+         case ('data_folder'); data_folder = val
+         case ('restart_file'); restart_file = val
          case default
             write (*, '(a,a)') 'unknown parameter ', trim(key)
             error stop 2
@@ -138,6 +157,35 @@ contains
       call check(pictor_reset_current(gpu), 'ResetCurrent')
    end subroutine ResetCurrent
 
+   subroutine UpdateMagFld_PML
+      call check(pictor_pml(gpu, 1_c_int32_t), 'UpdateMagFld_PML')
+   end subroutine UpdateMagFld_PML
+
+   subroutine UpdateElcFld_PML
+      call check(pictor_pml(gpu, 0_c_int32_t), 'UpdateElcFld_PML')
+   end subroutine UpdateElcFld_PML
+
+   subroutine EnforceBC_Final
+      call check(pictor_enforce_bc_final(gpu), 'EnforceBC_Final')
+   end subroutine EnforceBC_Final
+
+   ! Savedata (savedata.F90:11-60): field files and restart files at their periods
+   subroutine Savedata(step)
+      integer, intent(in) :: step
+      character(len=32) :: ts
+      write (ts, '(i0)') step
+      if (fld_save_period > 0) then
+         if (mod(step, fld_save_period) == 0) &
+            call check(pictor_save_fields(gpu, trim(data_folder)//'/fld_'//trim(ts)//c_null_char, &
+                                          int(fsave_ratio, c_int32_t), 1_c_int32_t), 'save_field_collective')
+      end if
+      if (restart_save_period > 0) then
+         if (mod(step, restart_save_period) == 0) &
+            call check(pictor_save_state(gpu, trim(data_folder)//'/restart_'//trim(ts)//c_null_char, &
+                                         int(step, c_int64_t)), 'SaveRestartData')
+      end if
+   end subroutine Savedata
+
    subroutine ReorderPrtl
       if (prtl_reorder_period > 0) then
          if (modulo(t, prtl_reorder_period) == 0) call check(pictor_sort_tiles(gpu), 'ReorderPrtl')
@@ -148,8 +196,14 @@ contains
    subroutine SaveEnergy(step)
       integer, intent(in) :: step
       real(c_double) :: d(9)
+      character(len=32) :: ts
       call check(pictor_diagnostics(gpu, d), 'pictor_diagnostics')
       write (*, '(a,i8,5es26.17)') 'energy ', step, d(1), d(2), d(3), d(4), d(8)
+      if (energy_file == 1) then   ! save_total_energy, savedata.F90:856-890
+         write (ts, '(i0)') step
+         call check(pictor_save_energy(gpu, trim(data_folder)//'/energy_'//trim(ts)//c_null_char), &
+                    'save_total_energy')
+      end if
    end subroutine SaveEnergy
 
 end program MAIN

@@ -20,6 +20,12 @@ module parameters
    integer :: curr_filter = 0                        ! passes of the 1-2-1 current filter per step
    integer :: prtl_reorder_period = 20               ! counting sort into tiles every 20 steps
    integer :: energy_save_period = 10                ! print CalcEnergy every this many steps
+   integer :: energy_file = 0                        ! 1: also write data_folder/energy_<t> (save_total_energy)
+   integer :: fld_save_period = 0                    ! write data_folder/fld_<t> every this many steps (0: never)
+   integer :: fsave_ratio = 1                        ! sampling of the field files (CollectFld)
+   integer :: restart_save_period = 0                ! write data_folder/restart_<t> every this many steps
+   character(len=200) :: data_folder = 'data'        ! output folder (must exist)
+   character(len=200) :: restart_file = ''           ! start from this restart file (empty: from the setup)
    ! run-time choices of the GPU path
    integer :: form = 1                               ! 0 = FP64 reference SoA, 1 = FP32 split
    integer :: fused = 0                              ! 1 = pictor_step per step, 0 = one call per phase

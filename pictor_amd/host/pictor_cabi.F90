@@ -17,6 +17,13 @@ module pictor_cabi
    integer(c_int32_t), parameter :: PICTOR_GEOM_CARTESIAN = 0
    integer(c_int32_t), parameter :: PICTOR_GEOM_CYL = 1
    integer(c_int32_t), parameter :: PICTOR_GEOM_CYL_MODAL = 2
+   ! boundary face types (bc_face%type_fld / type_prtl: 'prdc', 'cond', 'open', 'pml', 'refl', 'remv')
+   integer(c_int32_t), parameter :: PICTOR_BC_PERIODIC = 0
+   integer(c_int32_t), parameter :: PICTOR_BC_COND = 1
+   integer(c_int32_t), parameter :: PICTOR_BC_OPEN = 2
+   integer(c_int32_t), parameter :: PICTOR_BC_PML = 3
+   integer(c_int32_t), parameter :: PICTOR_BC_REFL = 4
+   integer(c_int32_t), parameter :: PICTOR_BC_REMV = 5
 
    type, bind(C) :: pictor_config
       integer(c_int32_t) :: ndim, nx, ny, nz, epc, form, tile, fdtd, geometry, curr_filter
@@ -168,6 +175,63 @@ module pictor_cabi
          real(c_double) :: q(*), x(*), y(*), z(*), u(*), v(*), w(*)
          integer(c_int32_t) :: flv(*)
          integer(c_int64_t), intent(out) :: n_out
+      end function
+      ! SetBC_Fld / SetBC_Prtl / SetBoundarySpeed (bc.F90:23-63, 210-217); axis 0..2, side 0 lower, 1 upper
+      integer(c_int) function pictor_set_bc_fld(ctx, axis, side, type, position, thickness) &
+            bind(C, name="pictor_set_bc_fld")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: axis, side, type
+         real(c_double), value :: position, thickness
+      end function
+      integer(c_int) function pictor_set_bc_prtl(ctx, axis, side, type, position) bind(C, name="pictor_set_bc_prtl")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: axis, side, type
+         real(c_double), value :: position
+      end function
+      integer(c_int) function pictor_set_boundary_speed(ctx, axis, side, speed) &
+            bind(C, name="pictor_set_boundary_speed")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: axis, side
+         real(c_double), value :: speed
+      end function
+      ! UpdateMagFld_PML (which = 1) / UpdateElcFld_PML (which = 0), bc_pml.F90
+      integer(c_int) function pictor_pml(ctx, which) bind(C, name="pictor_pml")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: which
+      end function
+      ! EnforceBC_Final, bc.F90:289-340
+      integer(c_int) function pictor_enforce_bc_final(ctx) bind(C, name="pictor_enforce_bc_final")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      ! save_total_energy / save_field_collective (savedata.F90:856-890, 356-497), .npz files
+      integer(c_int) function pictor_save_energy(ctx, path) bind(C, name="pictor_save_energy")
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: ctx
+         character(kind=c_char) :: path(*)
+      end function
+      integer(c_int) function pictor_save_fields(ctx, path, res, save_current) bind(C, name="pictor_save_fields")
+         import :: c_int, c_ptr, c_char, c_int32_t
+         type(c_ptr), value :: ctx
+         character(kind=c_char) :: path(*)
+         integer(c_int32_t), value :: res, save_current
+      end function
+      ! SaveRestartData / the restart reload (savedata.F90:73-141, reload.F90:17-338)
+      integer(c_int) function pictor_save_state(ctx, path, t) bind(C, name="pictor_save_state")
+         import :: c_int, c_ptr, c_char, c_int64_t
+         type(c_ptr), value :: ctx
+         character(kind=c_char) :: path(*)
+         integer(c_int64_t), value :: t
+      end function
+      integer(c_int) function pictor_load_state(ctx, path, t) bind(C, name="pictor_load_state")
+         import :: c_int, c_ptr, c_char, c_int64_t
+         type(c_ptr), value :: ctx
+         character(kind=c_char) :: path(*)
+         integer(c_int64_t), intent(out) :: t
       end function
    end interface
 

@@ -13,8 +13,8 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import (FDTD_LEHE, FDTD_YEE, FORM_FP32_SPLIT, FORM_FP64, GEOM_CARTESIAN, GEOM_CYL, GEOM_CYL_MODAL, PictorConfig, check,
-                   lib)
+from ._lib import (BC_TYPES, FDTD_LEHE, FDTD_YEE, FORM_FP32_SPLIT, FORM_FP64, GEOM_CARTESIAN, GEOM_CYL, GEOM_CYL_MODAL,
+                   PictorConfig, check, lib)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -69,6 +69,14 @@ class Simulation:
         check(lib().pictor_load_state(self._ctx, os.fsencode(path), ctypes.byref(t)))
         self.t = int(t.value)
         return self.t
+
+    def save_total_energy(self, path):
+        """save_total_energy (savedata.F90:856-890): Benergy, Eenergy, PrtlEnergy as a .npz file."""
+        check(lib().pictor_save_energy(self._ctx, os.fsencode(path)))
+
+    def save_field_collective(self, path, res=1, save_current=False):
+        """save_field_collective (savedata.F90:356-497): CollectFld node samples as a .npz file."""
+        check(lib().pictor_save_fields(self._ctx, os.fsencode(path), int(res), 1 if save_current else 0))
 
     def relayout_stats(self):
         """(re-layouts after a tile-bin overflow, particles taken from the spill buffer) since init."""
@@ -179,6 +187,32 @@ class Simulation:
     def ResetCurrent(self):              # current.F90:13-17
         check(lib().pictor_reset_current(self._ctx))
 
+    # ---- boundary faces (src/bc), the reference's setup calls ----
+    @staticmethod
+    def _face(axis, side):
+        return {"x": 0, "y": 1, "z": 2}[axis], {"lower": 0, "upper": 1}[side]
+
+    def SetBC_Fld(self, Axis, Position, Type, Side, AttnThickness=4.0):   # bc.F90:23-50
+        a, s = self._face(Axis, Side)
+        check(lib().pictor_set_bc_fld(self._ctx, a, s, BC_TYPES[Type], float(Position), float(AttnThickness)))
+
+    def SetBC_Prtl(self, Axis, Position, Type, Side):                     # bc.F90:52-63
+        a, s = self._face(Axis, Side)
+        check(lib().pictor_set_bc_prtl(self._ctx, a, s, BC_TYPES[Type], float(Position)))
+
+    def SetBoundarySpeed(self, Axis, Side, Speed):                         # bc.F90:210-217
+        a, s = self._face(Axis, Side)
+        check(lib().pictor_set_boundary_speed(self._ctx, a, s, float(Speed)))
+
+    def UpdateMagFld_PML(self):          # bc_pml.F90:185-290 + SendRecvFlds(B)
+        check(lib().pictor_pml(self._ctx, 1))
+
+    def UpdateElcFld_PML(self):          # bc_pml.F90:85-183 + SendRecvFlds(E)
+        check(lib().pictor_pml(self._ctx, 0))
+
+    def EnforceBC_Final(self):           # bc.F90:289-340
+        check(lib().pictor_enforce_bc_final(self._ctx))
+
     def ReorderPrtl(self):               # mem_prtl.F90:162-301
         check(lib().pictor_sort_tiles(self._ctx))
 
@@ -189,10 +223,13 @@ class Simulation:
         self.MoveDepositPrtl()
         self.ExchangePrtl()
         self.UpdateBfieldHalf()
+        self.UpdateMagFld_PML()
         self.UpdateEfield()
+        self.UpdateElcFld_PML()
         self.UpdateCurrentAllFaces()
         self.smoothen_current()
         self.AddCurrent()
+        self.EnforceBC_Final()
         self.t += 1
 
     def step(self, n=1):

@@ -102,10 +102,19 @@ def _weibel_big(ndim, nx, ny, nz, epc, drift, uth, steps=20):
             G = sim.download_fields()
         out[name] = (d, G)
         say(f"{name}: {d}")
+    # the split step is deterministic: fields, current and field energies repeat bit for
+    # bit; the particle sums and the diagnostic's charge density add the particles in
+    # their slot order within a bin (which varies from run to run) and with FP64
+    # atomics, so they repeat to rounding
     for k in FIELDS + ("Jx", "Jy", "Jz"):
         assert np.array_equal(out["split_a"][1][k], out["split_b"][1][k]), k
-    for k in ("B_energy", "E_energy", "prtl_energy_1", "prtl_energy_2", "gauss_residual"):
+    for k in ("B_energy", "E_energy"):
         assert out["split_a"][0][k] == out["split_b"][0][k], k
+    for k in ("prtl_energy_1", "prtl_energy_2", "rho_max"):
+        a, b = out["split_a"][0][k], out["split_b"][0][k]
+        assert abs(a - b) <= 1e-11 * abs(b), (k, a, b)
+    ga, gb = out["split_a"][0]["gauss_residual"], out["split_b"][0]["gauss_residual"]
+    assert abs(ga - gb) <= 1e-12 * out["split_b"][0]["rho_max"], (ga, gb)
     ds, d64 = out["split_a"][0], out["fp64"][0]
     err = abs(_field_energy(ds) - _field_energy(d64)) / _field_energy(d64)
     say(f"split vs FP64 field energy after {steps} steps: {err:.3e}")
