@@ -40,6 +40,10 @@ extern "C" {
 #define PICTOR_FDTD_LEHE 1       /* src/maxwell/em_update_lehe.F90:8-155:
                                     extended B-update stencils, Yee E update */
 
+/* Size of the static device array behind pictor_energy_noalloc, in doubles
+ * (2 MiB per GPU): 2056 + 5 per particle tile. */
+#define PICTOR_NOALLOC_SCRATCH_DOUBLES 262144
+
 #define PICTOR_GEOM_CARTESIAN 0  /* periodic x, y (, z)                      */
 #define PICTOR_GEOM_CYL 1        /* the -Dcyl build with the axis (inc_axis):
                                     x = r (index i at r = i - 3.5), y = theta
@@ -298,6 +302,18 @@ int pictor_cyl_sort_period(pictor_ctx *ctx, int32_t period);
  * particle that leaves through either end becomes a hole.  on = 0 stops it
  * and makes z periodic again. */
 int pictor_moving_window(pictor_ctx *ctx, int32_t on, int32_t nr_plasma, int32_t ppc, uint64_t seed);
+
+/* ---- allocation-free diagnostics (no counterpart in the reference) ----
+ * For callers that cannot allocate device memory (code under stream capture,
+ * serving loops, harnesses that forbid hipMalloc): out[0..6] of
+ * pictor_diagnostics (field energies, kinetic energy per flavour, momentum sums),
+ * bitwise equal to it, computed with the same kernels into a static device array
+ * of PICTOR_NOALLOC_SCRATCH_DOUBLES doubles per GPU instead of the context's
+ * buffer; it calls no hipMalloc or hipFree.  Cartesian geometry; a grid whose
+ * tiles need more than the array holds (2056 + 5 per tile) is PICTOR_ERR_ARG.
+ * Every call on one GPU shares that GPU's array, so calls that overlap on one
+ * GPU (two host threads, or two contexts on different streams) race. */
+int pictor_energy_noalloc(pictor_ctx *ctx, double out[7]);
 
 /* ---- measurement (no counterpart in the reference) ----
  * Atomic throughput on the current device: every lane of 4 workgroups per CU

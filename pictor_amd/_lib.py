@@ -41,7 +41,7 @@ EXPORTS = (
     "pictor_moving_window", "pictor_probe_atomics", "pictor_cyl_push_scheme", "pictor_cyl_sort_period",
     "pictor_save_state", "pictor_load_state", "pictor_set_bc_fld", "pictor_set_bc_prtl",
     "pictor_set_boundary_speed", "pictor_pml", "pictor_enforce_bc_final", "pictor_save_energy",
-    "pictor_save_fields",
+    "pictor_save_fields", "pictor_energy_noalloc",
 )
 
 
@@ -118,6 +118,7 @@ def lib():
     L.pictor_enforce_bc_final.argtypes = [c_vp]
     L.pictor_save_energy.argtypes = [c_vp, ctypes.c_char_p]
     L.pictor_save_fields.argtypes = [c_vp, ctypes.c_char_p, i32, i32]
+    L.pictor_energy_noalloc.argtypes = [c_vp, c_dp]
     for name in EXPORTS:
         if name != "pictor_last_error":
             getattr(L, name).restype = ctypes.c_int

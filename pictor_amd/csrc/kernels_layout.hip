@@ -245,7 +245,8 @@ __device__ inline int fold_node(int n, int m, int len, int per)
 }
 
 // Kinetic energy (CalcPrtlEnergy, savedata_routines.F90:283-308) and momentum
-// partial sums per tile, plus CIC node charge for the Gauss-law residual.
+// partial sums per tile, plus CIC node charge for the Gauss-law residual
+// (skipped when rho is null).
 template <typename Rec>
 __device__ inline void particle_diag_tile(const GridP &g, const PhysP &ph, const TileP &tp, const long long *base,
                                           const int *cnt, double *partial, double *rho, Rec rec)
@@ -269,6 +270,7 @@ __device__ inline void particle_diag_tile(const GridP &g, const PhysP &ph, const
             acc[flv - 1] += e;
             const double m = ph.mass[flv - 1];
             acc[2] += m * u; acc[3] += m * v; acc[4] += m * w;
+            if (!rho) continue;
             // CIC charge at the nodes of the particle's cell
             const double qt = q * ph.qi;
             const int i = (int)floor(x), j = (int)floor(y);
