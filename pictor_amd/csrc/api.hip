@@ -245,15 +245,19 @@ int relayout(pictor_ctx *c, int nspill)
     HIPTRY(hipMemcpyAsync(c->rl_off, off.data(), (ng + 1) * sizeof(long long), hipMemcpyHostToDevice, c->stream));
     HIPTRY(hipMemcpyAsync(c->rl_fill, fill.data(), ng * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPTRY(hipMemcpyAsync(c->rl_gtotal, gtot.data(), ng * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    // the compaction reads the old layout on the device (d_base, d_cnt)
+    // the compaction reads the old layout on the device (d_base, d_cnt).  It copies each
+    // group's bins in order (coalesced): unlike a fresh upload in the loader's cell order,
+    // the particles of a bin are already mixed by the pushes that filled it, so the slot
+    // permutation of the upload is not needed here (and its scattered writes made a
+    // re-layout of C2 take 60 ms)
     if (c->form == PICTOR_FORM_FP64) {
-        launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], c->rl_off, c->rl_gtotal, c->pf[dst], c->stream);
-        launch_spill_scatter_f64(tp, c->spill_f, nspill, c->rl_grp, c->rl_off, c->rl_fill, c->rl_gtotal, c->pf[dst],
+        launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], c->rl_off, nullptr, c->pf[dst], c->stream);
+        launch_spill_scatter_f64(tp, c->spill_f, nspill, c->rl_grp, c->rl_off, c->rl_fill, nullptr, c->pf[dst],
                                  c->stream);
     } else {
-        launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], c->rl_off, c->rl_gtotal, c->ps[dst],
+        launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], c->rl_off, nullptr, c->ps[dst],
                              c->stream);
-        launch_spill_scatter_split(tp, c->spill_s, nspill, c->rl_grp, c->rl_off, c->rl_fill, c->rl_gtotal, c->ps[dst],
+        launch_spill_scatter_split(tp, c->spill_s, nspill, c->rl_grp, c->rl_off, c->rl_fill, nullptr, c->ps[dst],
                                    c->stream);
     }
     HIPTRY(hipGetLastError());
@@ -442,10 +446,13 @@ int do_e_phase(pictor_ctx *c, bool curl, bool add, bool halo = true)
 
 int choose_tile(const pictor_config *cfg)
 {
-    // 3-D windows are (T+3)^3 / (T+5)^3 nodes: 8 is the largest tile whose LDS fits
-    const int cands2[3] = {32, 16, 8}, cands3[2] = {8, 4};
+    // 2-D: tile 16 (512-lane workgroups, 35 KB of LDS: four per CU) is the fastest push
+    // in both forms once the grid has 1024 tiles of it; smaller grids take tile 8 for
+    // more workgroups.  3-D windows are (T+3)^3 / (T+5)^3 nodes: 8 is the largest tile
+    // whose LDS fits
+    const int cands2[2] = {16, 8}, cands3[2] = {8, 4};
     const int *cands = cfg->ndim == 3 ? cands3 : cands2;
-    const int ncand = cfg->ndim == 3 ? 2 : 3;
+    const int ncand = 2;
     auto fits = [&](int T) {
         if (cfg->nx % T || cfg->ny % T) return false;
         if (cfg->ndim == 3 && cfg->nz % T) return false;

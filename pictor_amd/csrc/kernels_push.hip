@@ -551,8 +551,8 @@ void launch_push2d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, 
     } while (0)
     switch (T) {
     case 8: PUSH_F64(8, 256); break;
-    case 16: PUSH_F64(16, 256); break;
-    case 32: PUSH_F64(32, 512); break;
+    case 16: PUSH_F64(16, 512); break;
+    case 32: PUSH_F64(32, 1024); break;
     default: break;
     }
 #undef PUSH_F64
@@ -574,7 +574,7 @@ void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp
     } while (0)
     switch (T) {
     case 8: PUSH_SPLIT(8, 256); break;
-    case 16: PUSH_SPLIT(16, 256); break;
+    case 16: PUSH_SPLIT(16, 512); break;
     case 32: PUSH_SPLIT(32, 1024); break;
     default: break;
     }
@@ -591,7 +591,7 @@ void launch_push2d_split_track(int T, const GridP &g, const PhysP &ph, const Til
                        B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp, dst)
     switch (T) {
     case 8: PUSH_SPLIT(8, 256); break;
-    case 16: PUSH_SPLIT(16, 256); break;
+    case 16: PUSH_SPLIT(16, 512); break;
     case 32: PUSH_SPLIT(32, 1024); break;
     default: break;
     }

@@ -52,9 +52,9 @@ def test_yee_and_halo_bitwise():
         assert np.array_equal(G[n], o.F[n]), n
 
 
-# every tile-size instantiation of the push kernels (tile 8 and 16: 256-lane
-# workgroups; tile 32: 512 lanes FP64, 1024 split), with and without the XCD
-# tile remap (number of tiles divisible by 8)
+# every tile-size instantiation of the push kernels (tile 8: 256-lane
+# workgroups; 16: 512 lanes; 32: 1024 lanes), with and without the XCD tile
+# remap (number of tiles divisible by 8)
 TILE_CASES = [(32, 24, 8), (64, 32, 8), (64, 64, 16), (48, 32, 16), (128, 64, 32), (64, 64, 32)]
 
 
@@ -211,6 +211,50 @@ def test_spill_relayout_keeps_particles():
             assert sim.particle_count() == n
         G = sim.download_particles()
         assert np.allclose(np.sort(G["x"]), np.sort(o.particles()["x"]), rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32_split"])
+def test_sort_tiles_against_oracle_reorder(form):
+    # pictor_sort_tiles (ReorderPrtlArr) after a push that moved particles
+    # across tiles (the push writes each into an arrival bin of its new tile),
+    # against orc_reorder on the same particles: the same multiset (FP64; the
+    # split form's offsets differ from the oracle's positions at FP32 rounding),
+    # every particle in its own tile's bin, the bins in tile order and, per tile,
+    # the particles that the oracle's cell-ordered array holds in that tile
+    nx, ny, T = 64, 32, 8
+    say(f"sort_tiles form={form} nx={nx} ny={ny} tile={T}")
+    rng = np.random.default_rng(21)
+    P = setup_weibel(nx, ny, epc=4, drift=0.9, uth=0.3, seed=21)
+    perm = rng.permutation(len(P["x"]))
+    P = {k: v[perm] for k, v in P.items()}
+    o = Oracle(2, nx, ny, epc=4, c=0.45, cap=2 * len(P["x"]) + 4096)
+    F = smooth_fields(o, seed=21)
+    o.load_particles(**P)
+    with Simulation(nx, ny, epc=4, c=0.45, form=form, tile=T) as sim:
+        sim.upload_fields(*[F[n] for n in FIELDS])
+        sim.upload_particles(**P)
+        sim.ResetCurrent(); sim.MoveDepositPrtl(); sim.ExchangePrtl()
+        o.reset_current(); o.move_deposit(); o.exchange()
+        moved = np.sum(tiles_of(P, nx, ny, T) != tiles_of(o.particles(), nx, ny, T)) if form == "fp64" else 1
+        sim.ReorderPrtl()
+        o.reorder()
+        G = sim.download_particles(with_tile=True)
+    R = o.particles()
+    assert moved > 0
+    cells = (np.floor(R["y"]).astype(np.int64) - 3) * nx + np.floor(R["x"]).astype(np.int64) - 3
+    assert np.all(np.diff(cells) >= 0)                    # the oracle: ordered by cell
+    assert len(G["x"]) == len(R["x"]) == len(P["x"])
+    assert np.array_equal(G["tile"], tiles_of(G, nx, ny, T))
+    assert np.all(np.diff(G["tile"]) >= 0)
+    rt = tiles_of(R, nx, ny, T)
+    for t in np.unique(rt):
+        a = {k: v[G["tile"] == t] for k, v in G.items() if k != "tile"}
+        b = {k: v[rt == t] for k, v in R.items()}
+        if form == "fp64":
+            assert_same_multiset(a, b)
+        else:
+            assert len(a["x"]) == len(b["x"]), t
+            assert np.max(np.abs(np.sort(a["x"]) - np.sort(b["x"]))) < 1e-5, t
 
 
 def test_upload_rejects_out_of_domain():
