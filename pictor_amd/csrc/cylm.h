@@ -49,6 +49,10 @@ struct CylmState {
     PartCylS spare32;
     long long spare_cap = 0;
     long long cap = 0;                   // allocated particle slots (c->np slots in use, holes included)
+    // the split push's node-centred FP32 copy of E and B (4 float4 + 1 float2 per
+    // (i, k) node, mx * mz nodes), rebuilt before every split push
+    float4 *nf4 = nullptr;
+    float2 *nf2 = nullptr;
     int *d_err = nullptr;
     unsigned long long *d_count = nullptr;
     double *d_partial = nullptr;

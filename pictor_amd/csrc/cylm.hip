@@ -506,24 +506,92 @@ __global__ __launch_bounds__(256) void k_cylm_push(GridP g, CylmP q, const doubl
     }
 }
 
-// the FP32 gather: FP64 field values, FP32 sums, at (cell ip, kp; offsets dx, dz) and
-// the phases (1, cos theta, sin theta)
-__device__ inline void gather_f32(const GridP &g, const double *const *E, const double *const *B, int ip, int kp,
-                                  float dx, float dz, float c1, float s1, float pe[3], float pb[3])
+// The node-centred FP32 copy of E and B that the split push gathers from, rebuilt before
+// every split push: node (i, k) holds its 18 values v[6 s + c] (slot s = 0..2; c = 0..5:
+// Ex, Ey, Ez, Bx, By, Bz, computed in FP64 as node_vals does and rounded once to FP32),
+// v[0..15] in 4 float4 and v[16..17] in 1 float2, so the gather makes 4 x 5 vector loads
+// per particle instead of 156 scalar FP64 loads.  Node index (k - 1) * mx + (i - 1);
+// nodes with i = 1 or k = 1 (no lower neighbour) hold zero.
+__global__ __launch_bounds__(256) void k_cylm_nodes_f32(GridP g, const double *__restrict__ Ex,
+                                                        const double *__restrict__ Ey, const double *__restrict__ Ez,
+                                                        const double *__restrict__ Bx, const double *__restrict__ By,
+                                                        const double *__restrict__ Bz, float4 *__restrict__ nf4,
+                                                        float2 *__restrict__ nf2)
 {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.mx * g.mz) return;
+    const int i = 1 + (int)(t % g.mx), k = 1 + (int)(t / g.mx);
+    float v[18];
+#pragma unroll
+    for (int e = 0; e < 18; e++) v[e] = 0.0f;
+    if (i >= 2 && k >= 2) {
+        const double *E[3] = {Ex, Ey, Ez}, *B[3] = {Bx, By, Bz};
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+            double e[3], b[3];
+            node_vals(g, E, B, i, s, k, e, b);
+#pragma unroll
+            for (int c = 0; c < 3; c++) { v[6 * s + c] = (float)e[c]; v[6 * s + 3 + c] = (float)b[c]; }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) nf4[4 * t + j] = make_float4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+    nf2[t] = make_float2(v[16], v[17]);
+}
+
+// the FP32 gather of the split push from the node copy: per (slot, component) the
+// bilinear sum over the cell's nodes nn = 0..3 (weights (1 - dx or dx) (1 - dz or dz)),
+// then the slots combined with the phases (1, cos theta, sin theta) in order 0..2
+__device__ inline void gather_nodes(const GridP &g, const float4 *__restrict__ nf4, const float2 *__restrict__ nf2,
+                                    int ip, int kp, float dx, float dz, float c1, float s1, float pe[3], float pb[3])
+{
+    float se[18];
+#pragma unroll
+    for (int e = 0; e < 18; e++) se[e] = 0.0f;
+    const long long n0 = (long long)(kp - 1) * g.mx + (ip - 1);
+#pragma unroll
+    for (int nn = 0; nn < 4; nn++) {
+        const int a = nn & 1, b = nn >> 1;
+        const float wt = (a ? dx : 1.0f - dx) * (b ? dz : 1.0f - dz);
+        const long long n = n0 + a + (long long)b * g.mx;
+        float v[18];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float4 f = nf4[4 * n + j];
+            v[4 * j] = f.x; v[4 * j + 1] = f.y; v[4 * j + 2] = f.z; v[4 * j + 3] = f.w;
+        }
+        const float2 f2 = nf2[n];
+        v[16] = f2.x; v[17] = f2.y;
+#pragma unroll
+        for (int e = 0; e < 18; e++) se[e] += wt * v[e];
+    }
     const float ph[3] = {1.0f, c1, s1};
     for (int c = 0; c < 3; c++) { pe[c] = 0.0f; pb[c] = 0.0f; }
-    for (int s = 0; s < 3; s++) {
-        float se[3] = {0.0f, 0.0f, 0.0f}, sb[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int nn = 0; nn < 4; nn++) {
-            const int a = nn & 1, b = nn >> 1;
-            const float wt = (a ? dx : 1.0f - dx) * (b ? dz : 1.0f - dz);
-            double e[3], bb[3];
-            node_vals(g, E, B, ip + a, s, kp + b, e, bb);
-            for (int c = 0; c < 3; c++) { se[c] += wt * (float)e[c]; sb[c] += wt * (float)bb[c]; }
-        }
-        for (int c = 0; c < 3; c++) { pe[c] += ph[s] * se[c]; pb[c] += ph[s] * sb[c]; }
+    for (int s = 0; s < 3; s++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) { pe[c] += ph[s] * se[6 * s + c]; pb[c] += ph[s] * se[6 * s + 3 + c]; }
+}
+
+// modal_deposit for a particle that stays in its cell (i1 = i2, k1 = k2): the two
+// r-z half-segments land on the same nodes, so their fixed-point counts are added
+// first (integers below 2^53: exact) and each node takes one add instead of two
+template <class Count, class Add>
+__device__ inline void modal_deposit_same_cell(const DepMove &d, Count count, Add add)
+{
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        add(0, s, d.i1, d.k1, count(d.pa[s] * d.Fx1 * (1.0 - d.Wz1)) + count(d.pa[s] * d.Fx2 * (1.0 - d.Wz2)));
+        add(0, s, d.i1, d.k1 + 1, count(d.pa[s] * d.Fx1 * d.Wz1) + count(d.pa[s] * d.Fx2 * d.Wz2));
+        add(2, s, d.i1, d.k1, count(d.pa[s] * d.Fz1 * (1.0 - d.Wx1)) + count(d.pa[s] * d.Fz2 * (1.0 - d.Wx2)));
+        add(2, s, d.i1 + 1, d.k1, count(d.pa[s] * d.Fz1 * d.Wx1) + count(d.pa[s] * d.Fz2 * d.Wx2));
+    }
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        add(1, s, d.i2, d.k2, count(d.pt[s] * (1.0 - d.wx) * (1.0 - d.wz)));
+        add(1, s, d.i2 + 1, d.k2, count(d.pt[s] * d.wx * (1.0 - d.wz)));
+        add(1, s, d.i2, d.k2 + 1, count(d.pt[s] * (1.0 - d.wx) * d.wz));
+        add(1, s, d.i2 + 1, d.k2 + 1, count(d.pt[s] * d.wx * d.wz));
     }
 }
 
@@ -533,10 +601,8 @@ __device__ inline void gather_f32(const GridP &g, const double *const *E, const 
 // count: every sum is exact, so J does not depend on the order of the atomics.
 // Flags: 1 = a particle reached the wall, 2 = a contribution above 64 charge cells (the
 // fixed-point range).
-__global__ __launch_bounds__(256) void k_cylm_push_split(GridP g, CylmP q, const double *__restrict__ Ex,
-                                                         const double *__restrict__ Ey, const double *__restrict__ Ez,
-                                                         const double *__restrict__ Bx, const double *__restrict__ By,
-                                                         const double *__restrict__ Bz, double *__restrict__ Jx,
+__global__ __launch_bounds__(256) void k_cylm_push_split(GridP g, CylmP q, const float4 *__restrict__ nf4,
+                                                         const float2 *__restrict__ nf2, double *__restrict__ Jx,
                                                          double *__restrict__ Jy, double *__restrict__ Jz, PartCylS P,
                                                          long long n, int *__restrict__ err)
 {
@@ -547,14 +613,13 @@ __global__ __launch_bounds__(256) void k_cylm_push_split(GridP g, CylmP q, const
     bool dep = false;
     DepMove d;
     if (cw != 0u) {
-        const double *E[3] = {Ex, Ey, Ez}, *B[3] = {Bx, By, Bz};
         const int i0 = cell_i(cw), k0 = cell_k(cw), flv = cell_flv(cw);
         const float ox = P.ox[m], oz = P.oz[m];
         const uint32_t t0 = P.th[m];
         float s0, c0;
         sincosf((float)((double)t0 * kTurnRad), &s0, &c0);
         float pe[3], pb[3];
-        gather_f32(g, E, B, i0, k0, ox, oz, c0, s0, pe, pb);
+        gather_nodes(g, nf4, nf2, i0, k0, ox, oz, c0, s0, pe, pb);
         const float qm = (float)(q.flvrqm[flv - 1] * 0.5);
         const float c = (float)q.c, sqc = (float)q.sqc, cinv = (float)q.cinv;
         const float pEx = pe[0] * qm, pEy = pe[1] * qm, pEz = pe[2] * qm;
@@ -648,10 +713,15 @@ __global__ __launch_bounds__(256) void k_cylm_push_split(GridP g, CylmP q, const
     const int rl = lane & (R - 1);
     for (int e = threadIdx.x; e < 9 * cells * R; e += 256) sJ[e] = 0.0;
     __syncthreads();
-    if (inside)
-        modal_deposit(d, [&](int cc, int s, int i, int k, double v) {
-            atomicAdd(&sJ[((cc * 3 + s) * cells + (k - bkl) * wi + (i - bil)) * R + rl], count(v));
-        });
+    if (inside) {
+        auto add = [&](int cc, int s, int i, int k, double cnt) {
+            atomicAdd(&sJ[((cc * 3 + s) * cells + (k - bkl) * wi + (i - bil)) * R + rl], cnt);
+        };
+        if (d.i1 == d.i2 && d.k1 == d.k2)
+            modal_deposit_same_cell(d, count, add);
+        else
+            modal_deposit(d, [&](int cc, int s, int i, int k, double v) { add(cc, s, i, k, count(v)); });
+    }
     if (big) atomicOr(err, 2);
     __syncthreads();
     for (int e = threadIdx.x; e < 9 * cells; e += 256) {
@@ -1199,6 +1269,7 @@ void cylm_release(pictor_ctx *c)
     for (int a = 0; a < 2; a++) { dfree(st.skey[a]); dfree(st.sidx[a]); }
     dfree(st.stmp);
     dfree(st.d_err); dfree(st.d_count); dfree(st.d_partial);
+    dfree(st.nf4); dfree(st.nf2);
     st = CylmState{};
     c->np = 0;
 }
@@ -1341,10 +1412,18 @@ int cylm_push(pictor_ctx *c, bool timed)
     double *E[3], *B[3], *J[3];
     fields(c, E, B, J);
     if (timed) HIPTRY(hipEventRecord(c->ev[0], c->stream));
-    if (c->np > 0 && split_form(c))
-        hipLaunchKernelGGL(k_cylm_push_split, blocks_of(c->np), dim3(256), 0, c->stream, c->g, c->cq, E[0], E[1], E[2],
-                           B[0], B[1], B[2], J[0], J[1], J[2], c->cm.ps, c->np, c->cm.d_err);
-    else if (c->np > 0)
+    if (c->np > 0 && split_form(c)) {
+        CylmState &st = c->cm;
+        const long long nodes = (long long)c->g.mx * c->g.mz;
+        if (!st.nf4) {
+            HIPTRY(dalloc(&st.nf4, 4 * nodes));
+            HIPTRY(dalloc(&st.nf2, nodes));
+        }
+        hipLaunchKernelGGL(k_cylm_nodes_f32, blocks_of(nodes), dim3(256), 0, c->stream, c->g, E[0], E[1], E[2], B[0],
+                           B[1], B[2], st.nf4, st.nf2);
+        hipLaunchKernelGGL(k_cylm_push_split, blocks_of(c->np), dim3(256), 0, c->stream, c->g, c->cq, st.nf4, st.nf2,
+                           J[0], J[1], J[2], st.ps, c->np, st.d_err);
+    } else if (c->np > 0)
         hipLaunchKernelGGL(k_cylm_push, blocks_of(c->np), dim3(256), 0, c->stream, c->g, c->cq, E[0], E[1], E[2], B[0],
                            B[1], B[2], J[0], J[1], J[2], c->pcyl, c->np, c->cm.d_err);
     if (timed) HIPTRY(hipEventRecord(c->ev[1], c->stream));
