@@ -298,21 +298,27 @@ def adjoint_report(args):
     th = [torch.from_numpy(P0[k]) for k in ("u", "v", "w")]
     out = {"workload": f"adjoint: 2-D Weibel {nx}x{ny}, {epc} ppc/species, {len(P0['x'])} particles, "
                        f"{nsteps} steps, checkpoints every {seg}", "forms": {}}
+    # the autograd graph on GPU tensors: the momenta, particles and gradients stay in
+    # device memory (pictor_adjoint_*_device)
+    dev = "cuda"
+    sgn_d, th_d = sgn.to(dev), [t.to(dev) for t in th]
+    P0d = {k: torch.from_numpy(a).to(dev) for k, a in P0.items()}
+    out["tensors"] = dev
     grads = {}
     for form in args.adjoint_forms.split(","):
-        d = torch.tensor(drift, dtype=torch.float64, requires_grad=True)
-        tl = [t.clone().requires_grad_(True) for t in th]
-        u, v, w = drift_boost(tl[0], tl[1], tl[2], sgn, d)
+        d = torch.tensor(drift, dtype=torch.float64, device=dev, requires_grad=True)
+        tl = [t.clone().requires_grad_(True) for t in th_d]
+        u, v, w = drift_boost(tl[0], tl[1], tl[2], sgn_d, d)
         for t in (u, v, w):
             t.retain_grad()
         with Simulation(nx, ny, epc=epc, form=form) as sim:
             sim.adjoint_engine(args.adjoint_engine)
-            L = ParticleStepObjective.apply(u, v, w, sim, P0, nsteps, seg)
+            L = ParticleStepObjective.apply(u, v, w, sim, P0d, nsteps, seg)
             L.backward()
             st = sim.adjoint_stats()
             # plain forward of the same steps with the production kernels
             P = dict(P0)
-            P["u"], P["v"], P["w"] = u.detach().numpy(), v.detach().numpy(), w.detach().numpy()
+            P["u"], P["v"], P["w"] = [t.detach().cpu().numpy() for t in (u, v, w)]
             sim.upload_particles(**P)
             sim.synchronize()
             t0 = time.perf_counter()
@@ -320,7 +326,7 @@ def adjoint_report(args):
             sim.synchronize()
             t_prod = (time.perf_counter() - t0) * 1e3
             L_prod = sim.diagnostics()["B_energy"]
-        grads[form] = (float(d.grad), [t.grad.numpy().copy() for t in (u, v, w)], float(L))
+        grads[form] = (float(d.grad), [t.grad.cpu().numpy().copy() for t in (u, v, w)], float(L))
         out["forms"][form] = {
             "objective": float(L), "objective_production_forward": L_prod, "d_drift": float(d.grad),
             "forward_ms": st["forward_ms"], "recompute_ms": st["recompute_ms"], "reverse_ms": st["reverse_ms"],

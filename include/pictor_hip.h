@@ -254,6 +254,16 @@ int pictor_adjoint_forward(pictor_ctx *ctx, int64_t n, const double *q, const do
                            const double *w, const int32_t *flv, int32_t nsteps, int32_t segment,
                            double *objective);
 int pictor_adjoint_backward(pictor_ctx *ctx, double scale, double *du, double *dv, double *dw);
+/* The same two calls with the particle arrays (forward) and the gradient
+ * arrays (backward) in device memory of the context's GPU (for example torch
+ * tensors on that device): nothing crosses PCIe but L.  The call waits for
+ * its own stream only, so the caller finishes writing the inputs (and reads
+ * the outputs) on its own streams around it. */
+int pictor_adjoint_forward_device(pictor_ctx *ctx, int64_t n, const double *q, const double *x,
+                                  const double *y, const double *z, const double *u, const double *v,
+                                  const double *w, const int32_t *flv, int32_t nsteps, int32_t segment,
+                                  double *objective);
+int pictor_adjoint_backward_device(pictor_ctx *ctx, double scale, double *du, double *dv, double *dw);
 int pictor_adjoint_stats(pictor_ctx *ctx, double out[10]);
 /* Engine of the FP32 split form: 0 = tile-sorted (default; the particles in
  * the tile bins of pictor_move_deposit, every forward step the production

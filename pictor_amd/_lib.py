@@ -41,7 +41,8 @@ EXPORTS = (
     "pictor_moving_window", "pictor_probe_atomics", "pictor_cyl_push_scheme", "pictor_cyl_sort_period",
     "pictor_save_state", "pictor_load_state", "pictor_set_bc_fld", "pictor_set_bc_prtl",
     "pictor_set_boundary_speed", "pictor_pml", "pictor_enforce_bc_final", "pictor_save_energy",
-    "pictor_save_fields", "pictor_energy_noalloc",
+    "pictor_save_fields", "pictor_energy_noalloc", "pictor_adjoint_forward_device",
+    "pictor_adjoint_backward_device",
 )
 
 
@@ -100,6 +101,9 @@ def lib():
                                       i64] + [c_dp] * 7 + [c_ip, ctypes.POINTER(i64)]
     L.pictor_adjoint_forward.argtypes = [c_vp, i64] + [c_dp] * 7 + [c_ip, i32, i32, c_dp]
     L.pictor_adjoint_backward.argtypes = [c_vp, ctypes.c_double, c_dp, c_dp, c_dp]
+    # device-memory variants: raw device addresses (torch.Tensor.data_ptr())
+    L.pictor_adjoint_forward_device.argtypes = [c_vp, i64] + [c_vp] * 8 + [i32, i32, c_dp]
+    L.pictor_adjoint_backward_device.argtypes = [c_vp, ctypes.c_double, c_vp, c_vp, c_vp]
     L.pictor_adjoint_stats.argtypes = [c_vp, c_dp]
     L.pictor_adjoint_release.argtypes = [c_vp]
     L.pictor_adjoint_engine.argtypes = [c_vp, i32]

@@ -47,16 +47,35 @@ class ParticleStepObjective(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, u, v, w, sim, P, nsteps, segment):
-        arr = [t.detach().to("cpu", torch.float64).numpy() for t in (u, v, w)]
-        L = sim.adjoint_forward(P["q"], P["x"], P["y"], P["z"], arr[0], arr[1], arr[2], P["flv"], nsteps, segment)
         ctx.sim = sim
         ctx.dev = u.device
+        ctx.on_device = u.device.type == "cuda"
+        if ctx.on_device:
+            # momenta and particles stay on the GPU: the library reads them there
+            dev = [t.detach().to(torch.float64).contiguous() for t in (u, v, w)]
+            pd = {k: torch.as_tensor(P[k], device=u.device).to(torch.float64 if k != "flv" else torch.int32)
+                  .contiguous() for k in ("q", "x", "y", "z", "flv")}
+            torch.cuda.synchronize(u.device)
+            L = sim.adjoint_forward_device(dev[0].numel(), pd["q"].data_ptr(), pd["x"].data_ptr(),
+                                           pd["y"].data_ptr(), pd["z"].data_ptr(), dev[0].data_ptr(),
+                                           dev[1].data_ptr(), dev[2].data_ptr(), pd["flv"].data_ptr(), nsteps,
+                                           segment)
+            ctx.n = dev[0].numel()
+        else:
+            arr = [t.detach().to("cpu", torch.float64).numpy() for t in (u, v, w)]
+            L = sim.adjoint_forward(P["q"], P["x"], P["y"], P["z"], arr[0], arr[1], arr[2], P["flv"], nsteps,
+                                    segment)
         return torch.tensor(L, dtype=torch.float64, device=u.device)
 
     @staticmethod
     def backward(ctx, gL):
-        du, dv, dw = ctx.sim.adjoint_backward(float(gL))
-        out = [torch.from_numpy(a).to(ctx.dev) for a in (du, dv, dw)]
+        if ctx.on_device:
+            out = [torch.empty(ctx.n, dtype=torch.float64, device=ctx.dev) for _ in range(3)]
+            torch.cuda.synchronize(ctx.dev)
+            ctx.sim.adjoint_backward_device(out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(), float(gL))
+        else:
+            du, dv, dw = ctx.sim.adjoint_backward(float(gL))
+            out = [torch.from_numpy(a).to(ctx.dev) for a in (du, dv, dw)]
         return out[0], out[1], out[2], None, None, None, None
 
 
@@ -70,8 +89,10 @@ def weibel_gradients(nx, ny, epc, drift, uth, seed, nsteps, segment, form="fp32_
     u, v, w = drift_boost(th[0], th[1], th[2], sgn, d)
     for t in (u, v, w):
         t.retain_grad()
+    # on a GPU device the particle arrays go there once, as the momenta do
+    Pd = P if torch.device(device).type == "cpu" else {k: torch.from_numpy(a).to(device) for k, a in P.items()}
     with Simulation(nx, ny, epc=epc, form=form) as sim:
-        L = ParticleStepObjective.apply(u, v, w, sim, P, nsteps, segment)
+        L = ParticleStepObjective.apply(u, v, w, sim, Pd, nsteps, segment)
         L.backward()
         stats = sim.adjoint_stats()
     return {"objective": float(L), "d_drift": float(d.grad), "du": u.grad, "dv": v.grad, "dw": w.grad,

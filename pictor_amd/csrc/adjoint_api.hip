@@ -117,9 +117,24 @@ void engine_layout(const TileP &tp, double c, const std::vector<long long> &coun
     total = off;
 }
 
+// flavour and charge of every particle of caller arrays in device memory: bit 0 a
+// flavour other than 1 or 2, bit 1 a charge other than the flavour charge
+__global__ void k_adj_check(const double *__restrict__ q, const int32_t *__restrict__ flv, long long n, double q1,
+                            double q2, int *__restrict__ err)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int f = flv[i];
+    if (f != 1 && f != 2) atomicOr(err, 1);
+    else if (q[i] != (f == 1 ? q1 : q2)) atomicOr(err, 2);
+}
+
 template <typename R>
 struct AdjEngine : AdjEngineBase {
     pictor_ctx *c = nullptr;
+    // where the caller's particle arrays (forward) and gradients (backward) live:
+    // host memory, or device memory of this context's GPU
+    hipMemcpyKind kin = hipMemcpyHostToDevice, kout = hipMemcpyDeviceToHost;
     AdjP ap{};
     double ctx_jscale = 1.0;   // scale of J in the E update (split form: 2^-36 qi)
     long long n = 0;
@@ -326,10 +341,22 @@ struct AdjEngine : AdjEngineBase {
     int load(const double *q, const double *x, const double *y, const double *z, const double *u, const double *v,
              const double *w, const int32_t *flv)
     {
-        for (long long i = 0; i < n; i++) {
-            if (flv[i] != 1 && flv[i] != 2) return fail(PICTOR_ERR_ARG, "adjoint: flavour must be 1 or 2");
-            if (q[i] != c->ph.flvq[flv[i] - 1])
-                return fail(PICTOR_ERR_ARG, "adjoint: q must be the flavour charge (+1 / -1)");
+        if (kin == hipMemcpyHostToDevice) {
+            for (long long i = 0; i < n; i++) {
+                if (flv[i] != 1 && flv[i] != 2) return fail(PICTOR_ERR_ARG, "adjoint: flavour must be 1 or 2");
+                if (q[i] != c->ph.flvq[flv[i] - 1])
+                    return fail(PICTOR_ERR_ARG, "adjoint: q must be the flavour charge (+1 / -1)");
+            }
+        } else {
+            int err = 0;
+            HIPTRY(hipMemsetAsync(d_err, 0, sizeof(int), c->stream));
+            hipLaunchKernelGGL(k_adj_check, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, q, flv, n,
+                               c->ph.flvq[0], c->ph.flvq[1], d_err);
+            HIPTRY(hipGetLastError());
+            HIPTRY(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPTRY(hipStreamSynchronize(c->stream));
+            if (err & 1) return fail(PICTOR_ERR_ARG, "adjoint: flavour must be 1 or 2");
+            if (err & 2) return fail(PICTOR_ERR_ARG, "adjoint: q must be the flavour charge (+1 / -1)");
         }
         if (fast) return load_bins(q, x, y, z, u, v, w, flv);
         double *tmp[6] = {};
@@ -337,10 +364,10 @@ struct AdjEngine : AdjEngineBase {
         const double *src[6] = {x, y, z, u, v, w};
         for (int i = 0; i < 6; i++) {
             HIPTRY(dalloc(&tmp[i], n));
-            HIPTRY(hipMemcpyAsync(tmp[i], src[i], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            HIPTRY(hipMemcpyAsync(tmp[i], src[i], n * sizeof(double), kin, c->stream));
         }
         HIPTRY(dalloc(&tf, n));
-        HIPTRY(hipMemcpyAsync(tf, flv, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPTRY(hipMemcpyAsync(tf, flv, n * sizeof(int), kin, c->stream));
         HIPTRY(hipMemsetAsync(d_err, 0, sizeof(int), c->stream));
         const double *const raw[6] = {tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5]};
         launch_eng_load<R>(ap, raw, tf, ck[0], n, d_err, c->stream);
@@ -376,9 +403,9 @@ struct AdjEngine : AdjEngineBase {
         do {
             bool ok = true;
             for (int i = 0; i < 7 && ok; i++) ok = dalloc(ds[i], n) == hipSuccess &&
-                hipMemcpyAsync(*ds[i], hs[i], n * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess;
+                hipMemcpyAsync(*ds[i], hs[i], n * sizeof(double), kin, s) == hipSuccess;
             ok = ok && dalloc(&raw.flv, n) == hipSuccess &&
-                 hipMemcpyAsync(raw.flv, flv, n * sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess;
+                 hipMemcpyAsync(raw.flv, flv, n * sizeof(int), kin, s) == hipSuccess;
             ok = ok && dalloc(&d_tile, n) == hipSuccess && dalloc(&d_count, ng) == hipSuccess &&
                  dalloc(&d_fill, ng) == hipSuccess && dalloc(&d_off, ng + 1) == hipSuccess;
             ok = ok && hipMemsetAsync(d_count, 0, ng * sizeof(int), s) == hipSuccess &&
@@ -703,7 +730,7 @@ struct AdjEngine : AdjEngineBase {
             HIPTRY(hipGetLastError());
             double *h[3] = {du, dv, dw};
             for (int i = 0; i < 3; i++)
-                if (h[i]) HIPTRY(hipMemcpyAsync(h[i], d[i], n * sizeof(double), hipMemcpyDeviceToHost, s));
+                if (h[i]) HIPTRY(hipMemcpyAsync(h[i], d[i], n * sizeof(double), kout, s));
             HIPTRY(hipStreamSynchronize(s));
             for (int i = 0; i < 3; i++) dfree(d[i]);
         }
@@ -728,7 +755,7 @@ int check_ctx2(pictor_ctx *c)
 template <typename R>
 int adj_forward(pictor_ctx *c, int64_t n, const double *q, const double *x, const double *y, const double *z,
                 const double *u, const double *v, const double *w, const int32_t *flv, int32_t nsteps,
-                int32_t segment, int engine, double *objective)
+                int32_t segment, int engine, bool device, double *objective)
 {
     AdjEngine<R> *e = engine_of<R>(c);
     if (!e) {
@@ -737,6 +764,7 @@ int adj_forward(pictor_ctx *c, int64_t n, const double *q, const double *x, cons
         e = new AdjEngine<R>();
         c->adj = e;
     }
+    e->kin = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     // tile bins that overflow during the forward pass: the pass is repeated
     // with the centre bins twice as large (at most three times)
     double cf = 1.5;
@@ -761,13 +789,10 @@ int adj_forward(pictor_ctx *c, int64_t n, const double *q, const double *x, cons
     return PICTOR_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int pictor_adjoint_forward(pictor_ctx *c, int64_t n, const double *q, const double *x, const double *y,
-                           const double *z, const double *u, const double *v, const double *w, const int32_t *flv,
-                           int32_t nsteps, int32_t segment, double *objective)
+// the argument checks and engine dispatch of the two forward entry points
+int adjoint_forward_entry(pictor_ctx *c, int64_t n, const double *q, const double *x, const double *y,
+                          const double *z, const double *u, const double *v, const double *w, const int32_t *flv,
+                          int32_t nsteps, int32_t segment, bool device, double *objective)
 {
     int rc = check_ctx2(c);
     if (rc) return rc;
@@ -780,19 +805,54 @@ int pictor_adjoint_forward(pictor_ctx *c, int64_t n, const double *q, const doub
     if (!q || !x || !y || !z || !u || !v || !w || !flv) return fail(PICTOR_ERR_ARG, "null particle array");
     if (segment > nsteps) segment = nsteps;
     const int engine = c->adj_engine;
-    if (c->form == PICTOR_FORM_FP64) return adj_forward<double>(c, n, q, x, y, z, u, v, w, flv, nsteps, segment, engine, objective);
+    if (c->form == PICTOR_FORM_FP64)
+        return adj_forward<double>(c, n, q, x, y, z, u, v, w, flv, nsteps, segment, engine, device, objective);
     if (engine == 0 && c->tp.T != 8 && c->tp.T != 16 && c->tp.T != 32)
         return fail(PICTOR_ERR_ARG, "adjoint: the tile-sorted engine needs a tile edge of 8, 16 or 32");
-    return adj_forward<float>(c, n, q, x, y, z, u, v, w, flv, nsteps, segment, engine, objective);
+    return adj_forward<float>(c, n, q, x, y, z, u, v, w, flv, nsteps, segment, engine, device, objective);
+}
+
+template <typename R>
+int adjoint_backward_entry(AdjEngine<R> *e, double scale, double *du, double *dv, double *dw, bool device)
+{
+    e->kout = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    return e->backward(scale, du, dv, dw);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pictor_adjoint_forward(pictor_ctx *c, int64_t n, const double *q, const double *x, const double *y,
+                           const double *z, const double *u, const double *v, const double *w, const int32_t *flv,
+                           int32_t nsteps, int32_t segment, double *objective)
+{
+    return adjoint_forward_entry(c, n, q, x, y, z, u, v, w, flv, nsteps, segment, false, objective);
+}
+
+int pictor_adjoint_forward_device(pictor_ctx *c, int64_t n, const double *q, const double *x, const double *y,
+                                  const double *z, const double *u, const double *v, const double *w,
+                                  const int32_t *flv, int32_t nsteps, int32_t segment, double *objective)
+{
+    return adjoint_forward_entry(c, n, q, x, y, z, u, v, w, flv, nsteps, segment, true, objective);
 }
 
 int pictor_adjoint_backward(pictor_ctx *c, double scale, double *du, double *dv, double *dw)
 {
     int rc = check_ctx2(c);
     if (rc) return rc;
-    if (auto *e = engine_of<double>(c)) return e->backward(scale, du, dv, dw);
-    if (auto *e = engine_of<float>(c)) return e->backward(scale, du, dv, dw);
+    if (auto *e = engine_of<double>(c)) return adjoint_backward_entry(e, scale, du, dv, dw, false);
+    if (auto *e = engine_of<float>(c)) return adjoint_backward_entry(e, scale, du, dv, dw, false);
     return fail(PICTOR_ERR_ARG, "pictor_adjoint_backward without pictor_adjoint_forward");
+}
+
+int pictor_adjoint_backward_device(pictor_ctx *c, double scale, double *du, double *dv, double *dw)
+{
+    int rc = check_ctx2(c);
+    if (rc) return rc;
+    if (auto *e = engine_of<double>(c)) return adjoint_backward_entry(e, scale, du, dv, dw, true);
+    if (auto *e = engine_of<float>(c)) return adjoint_backward_entry(e, scale, du, dv, dw, true);
+    return fail(PICTOR_ERR_ARG, "pictor_adjoint_backward_device without pictor_adjoint_forward");
 }
 
 int pictor_adjoint_stats(pictor_ctx *c, double out[10])

@@ -279,6 +279,22 @@ class Simulation:
         check(lib().pictor_adjoint_backward(self._ctx, scale, _p(du), _p(dv), _p(dw)))
         return du, dv, dw
 
+    def adjoint_forward_device(self, n, q, x, y, z, u, v, w, flv, nsteps, segment):
+        """adjoint_forward from device addresses (contiguous FP64 arrays q .. w and
+        an int32 flv of n particles on this context's GPU); the caller's writes
+        to them must be complete (e.g. torch.cuda.synchronize())."""
+        L = ctypes.c_double()
+        ptrs = [ctypes.c_void_p(int(a)) for a in (q, x, y, z, u, v, w, flv)]
+        check(lib().pictor_adjoint_forward_device(self._ctx, int(n), *ptrs, nsteps, segment, ctypes.byref(L)))
+        self._adj_n = int(n)
+        return L.value
+
+    def adjoint_backward_device(self, du, dv, dw, scale=1.0):
+        """scale * dL/d(u, v, w) written to the device addresses du, dv, dw
+        (contiguous FP64 arrays of n elements); complete when the call returns."""
+        ptrs = [ctypes.c_void_p(int(a)) for a in (du, dv, dw)]
+        check(lib().pictor_adjoint_backward_device(self._ctx, scale, *ptrs))
+
     def adjoint_stats(self):
         out = np.zeros(10)
         check(lib().pictor_adjoint_stats(self._ctx, _p(out)))

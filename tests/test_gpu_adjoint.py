@@ -210,3 +210,41 @@ def test_fp64_adjoint_matches_torch_autograd(nsteps):
             f"max |g| {np.abs(G_ref[k]).max():.3e}")
     for k in range(3):
         assert np.max(np.abs(G[k] - G_ref[k])) <= 1e-9 * np.max(np.abs(G_ref[k])), k
+
+
+@pytest.mark.parametrize("form", ["fp32_split", "fp64"])
+def test_device_tensors_match_host_arrays(form):
+    # ParticleStepObjective on CUDA tensors (the _device entry points: momenta,
+    # particles and gradients stay on the GPU) against the same call on CPU
+    # tensors (host arrays through PCIe)
+    res = {}
+    for dev in ("cpu", "cuda"):
+        P0, sgn = _load(0.5)
+        d = torch.tensor(0.5, dtype=torch.float64, device=dev, requires_grad=True)
+        th = [torch.from_numpy(P0[k]).to(dev).requires_grad_(True) for k in ("u", "v", "w")]
+        u, v, w = drift_boost(th[0], th[1], th[2], sgn.to(dev), d)
+        for t in (u, v, w):
+            t.retain_grad()
+        P = P0 if dev == "cpu" else {k: torch.from_numpy(a).to(dev) for k, a in P0.items()}
+        say(f"adjoint inputs on {dev}, form {form}")
+        with Simulation(NX, NY, epc=EPC, form=form, tile=8) as sim:
+            L = ParticleStepObjective.apply(u, v, w, sim, P, 20, 5)
+            L.backward()
+        res[dev] = (float(L), float(d.grad), [t.grad.cpu().numpy() for t in (u, v, w)], u.grad.device.type)
+    a, b = res["cpu"], res["cuda"]
+    assert b[3] == "cuda"
+    assert abs(a[0] - b[0]) <= 1e-12 * a[0]
+    assert abs(a[1] - b[1]) <= 1e-8 * abs(a[1])
+    for k in range(3):
+        assert np.linalg.norm(a[2][k] - b[2][k]) <= 1e-8 * np.linalg.norm(a[2][k]), k
+
+
+def test_device_entry_refuses_bad_flavour():
+    from pictor_amd import PictorError
+    P0, _ = _load(0.5)
+    P0["flv"][7] = 3
+    T = {k: torch.from_numpy(a).to("cuda").contiguous() for k, a in P0.items()}
+    with Simulation(NX, NY, epc=EPC, form="fp32_split", tile=8) as sim:
+        with pytest.raises(PictorError):
+            sim.adjoint_forward_device(len(P0["x"]), *[T[k].data_ptr() for k in ("q", "x", "y", "z", "u", "v", "w",
+                                                                                "flv")], 4, 2)
