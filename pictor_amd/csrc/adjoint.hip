@@ -4,7 +4,7 @@
 // kept in a fixed order (the caller's), so that a particle's adjoint state is
 // found at the same index at every step.  Its forward is the split-form step
 // of the production kernels in the same arithmetic (FP32: hardware rsq/rcp,
-// currents as exact 2^-36 counts; FP64: IEEE sqrt/div, physical currents),
+// currents as exact 2^-30 counts; FP64: IEEE sqrt/div, physical currents),
 // with the field gather read from node arrays and the deposit made with
 // global FP64 atomics.
 //

@@ -29,7 +29,7 @@ using namespace pictor;
 
 namespace {
 
-constexpr double kTwo36 = 68719476736.0;
+constexpr double kFixQ = kFixScale;   // the split form's current counts per charge cell
 constexpr int kRetryLayout = -100;   // internal: tile bins overflowed, redo with larger bins
 
 template <typename R>
@@ -136,7 +136,7 @@ struct AdjEngine : AdjEngineBase {
     // host memory, or device memory of this context's GPU
     hipMemcpyKind kin = hipMemcpyHostToDevice, kout = hipMemcpyDeviceToHost;
     AdjP ap{};
-    double ctx_jscale = 1.0;   // scale of J in the E update (split form: 2^-36 qi)
+    double ctx_jscale = 1.0;   // scale of J in the E update (split form: 2^-30 qi)
     long long n = 0;
     int nsteps = 0, seg = 1, nseg = 0;
     long long ncell = 0, bytes = 0;
@@ -249,10 +249,10 @@ struct AdjEngine : AdjEngineBase {
         const bool f32 = std::is_same<R, float>::value;
         for (int s = 0; s < 2; s++) {
             ap.qm[s] = ph.flvrqm[s] * 0.5;
-            ap.qs[s] = f32 ? ph.flvq[s] * kTwo36 : ph.flvq[s] * ph.qi;
+            ap.qs[s] = f32 ? ph.flvq[s] * kFixQ : ph.flvq[s] * ph.qi;
         }
         // the field update scales J by ctx_jscale; the adjoint current is -ctx_jscale * Eb
-        ctx_jscale = f32 ? ph.qi / kTwo36 : 1.0;
+        ctx_jscale = f32 ? ph.qi / kFixQ : 1.0;
         ap.jscale = ctx_jscale;
         for (int i = 0; i < 3; i++) {
             HIPTRY(dalloc(&E[i], ncell)); HIPTRY(dalloc(&B[i], ncell)); HIPTRY(dalloc(&J[i], ncell));

@@ -29,7 +29,7 @@ template <> __device__ inline double rsq_<double>(double v) { return 1.0 / sqrt(
 template <typename R> __device__ inline R rcp_(R v);
 template <> __device__ inline float rcp_<float>(float v) { return __builtin_amdgcn_rcpf(v); }
 template <> __device__ inline double rcp_<double>(double v) { return 1.0 / v; }
-// current contribution: FP32 -> exact count of 2^-36 units, FP64 -> as is
+// current contribution: FP32 -> exact count of 2^-30 units, FP64 -> as is
 template <typename R> __device__ inline double qnt(R v);
 template <> __device__ inline double qnt<float>(float v) { return (double)__builtin_rintf(v); }
 template <> __device__ inline double qnt<double>(double v) { return v; }

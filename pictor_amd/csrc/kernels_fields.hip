@@ -118,7 +118,7 @@ __device__ inline double jvalue<double, double>(const double *J, long long o, do
 {
     return J[o];
 }
-// split form: J holds exact sums of contributions quantized to 2^-36 (charge x cell)
+// split form: J holds exact sums of contributions quantized to 2^-30 (charge x cell)
 template <>
 __device__ inline float jvalue<float, double>(const double *J, long long o, double scale)
 {

@@ -17,9 +17,10 @@
 // follows the reference expression order (bitwise with -ffp-contract=off) and
 // the current window accumulates with LDS FP64 atomics.
 // Split form: int32 cell + FP32 offsets, FP32 push; every current contribution
-// is quantised to a multiple of 2^-36 (charge x cell) and accumulated in FP64,
-// where all such sums are exact, so the current does not depend on the order
-// of the atomics: the split form is deterministic.  Both species of a tile go
+// is quantised to a multiple of 2^-30 (charge x cell) and accumulated as a
+// 64-bit integer (LDS) and in FP64 (J), where all such sums are exact, so the
+// current does not depend on the order of the atomics: the split form is
+// deterministic.  Both species of a tile go
 // through one pass, and each thread prefetches its next particle while it
 // pushes the current one.
 #include "pictor_common.h"
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
     // cell-corner field layout: cell lc holds 6 float4 = (component c at corners 1..4),
     // the reference's VecE/VecB(nn, cell) (movdep_routines.F90:6-223) for the tile's cells
     __shared__ float4 sC[6 * T * T];
-    __shared__ double sJ[3][WW];
+    __shared__ unsigned long long sJ[3][WW];
     __shared__ int sStart[MB + 1], sVs[MB];
     __shared__ unsigned sSrc[MB], sDstBase[MB];
     __shared__ int sDstCap[MB], sDstBin[MB], sCnt[MB];
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
             acc += bins.cnt_in[tile * tp.NB + s * 9 + d];
         }
         sStart[NV] = acc;
-        // q/m / 2 and the charge pre-scaled by 2^36 (exact) for the fixed-point current
+        // q/m / 2 and the charge pre-scaled by 2^30 (exact) for the fixed-point current
         for (int s = 0; s < nsp; s++) { sQm[s] = (float)(ph.flvrqm[s] * 0.5); sQs[s] = (float)(ph.flvq[s] * kFixScale); }
     }
     const float bx0 = (float)ph.bext[0], by0 = (float)ph.bext[1], bz0 = (float)ph.bext[2];
@@ -357,7 +358,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
 #pragma unroll
         for (int cc = 0; cc < 6; cc++) sC[6 * lc + cc] = make_float4(v[cc][0], v[cc][1], v[cc][2], v[cc][3]);
     }
-    for (int e = tid; e < WW; e += BS) { sJ[0][e] = 0.0; sJ[1][e] = 0.0; sJ[2][e] = 0.0; }
+    for (int e = tid; e < WW; e += BS) { sJ[0][e] = 0ull; sJ[1][e] = 0ull; sJ[2][e] = 0ull; }
     __syncthreads();
 
     const float c = (float)ph.c, sqc = (float)ph.sqc, cinv = (float)ph.cinv;
@@ -431,37 +432,45 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                 oy = y1 - (float)dj;
                 if (oy >= 1.0f) { oy = 0.0f; dj += 1; }
                 // the two zigzag half-segments to the stored position: 8 node values
-                // each, as exact integer counts of 2^-36 units
+                // each, as exact integer counts of 2^-30 units
                 double a[8], b[8];
                 zigzag2d_split((double)qs, x0, y0, kZMid, ox, oy, z1, di, dj, a, b);
                 // a particle that stays in its cell deposits both halves on the same
                 // 8 nodes: add them first (exact) and halve the LDS atomics
                 const bool cross = (di != 0) || (dj != 0);
+                // the counts as 64-bit integers (|count| < 2^31 at the 2^-30 scale: one
+                // conversion each), added with integer LDS atomics
+                unsigned long long ia[8], ib[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    ia[q] = (unsigned long long)(long long)(int)a[q];
+                    ib[q] = (unsigned long long)(long long)(int)b[q];
+                }
                 if (!cross) {
 #pragma unroll
-                    for (int q = 0; q < 8; q++) a[q] += b[q];
+                    for (int q = 0; q < 8; q++) ia[q] += ib[q];
                 }
                 {
-                    double *jp = &sJ[0][e0];
-                    atomicAdd(jp, a[0]);
-                    atomicAdd(jp + W, a[1]);
-                    atomicAdd(jp + WW, a[2]);
-                    atomicAdd(jp + WW + 1, a[3]);
-                    atomicAdd(jp + 2 * WW, a[4]);
-                    atomicAdd(jp + 2 * WW + 1, a[5]);
-                    atomicAdd(jp + 2 * WW + W, a[6]);
-                    atomicAdd(jp + 2 * WW + W + 1, a[7]);
+                    unsigned long long *jp = &sJ[0][e0];
+                    atomicAdd(jp, ia[0]);
+                    atomicAdd(jp + W, ia[1]);
+                    atomicAdd(jp + WW, ia[2]);
+                    atomicAdd(jp + WW + 1, ia[3]);
+                    atomicAdd(jp + 2 * WW, ia[4]);
+                    atomicAdd(jp + 2 * WW + 1, ia[5]);
+                    atomicAdd(jp + 2 * WW + W, ia[6]);
+                    atomicAdd(jp + 2 * WW + W + 1, ia[7]);
                 }
                 if (cross) {
-                    double *jp = &sJ[0][e0 + dj * W + di];
-                    atomicAdd(jp, b[0]);
-                    atomicAdd(jp + W, b[1]);
-                    atomicAdd(jp + WW, b[2]);
-                    atomicAdd(jp + WW + 1, b[3]);
-                    atomicAdd(jp + 2 * WW, b[4]);
-                    atomicAdd(jp + 2 * WW + 1, b[5]);
-                    atomicAdd(jp + 2 * WW + W, b[6]);
-                    atomicAdd(jp + 2 * WW + W + 1, b[7]);
+                    unsigned long long *jp = &sJ[0][e0 + dj * W + di];
+                    atomicAdd(jp, ib[0]);
+                    atomicAdd(jp + W, ib[1]);
+                    atomicAdd(jp + WW, ib[2]);
+                    atomicAdd(jp + WW + 1, ib[3]);
+                    atomicAdd(jp + 2 * WW, ib[4]);
+                    atomicAdd(jp + 2 * WW + 1, ib[5]);
+                    atomicAdd(jp + 2 * WW + W, ib[6]);
+                    atomicAdd(jp + 2 * WW + W + 1, ib[7]);
                 }
                 int nci = ci + di, ncj = cj + dj;
                 bool keep = true;
@@ -524,7 +533,8 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
     bool big = false;
     for (int e = tid; e < WW; e += BS) {
         const long long o = (long long)(iw + e % W - 1) + (long long)mx * (jw + e / W - 1);
-        const double a = sJ[0][e], b = sJ[1][e], cz = sJ[2][e];
+        const double a = (double)(long long)sJ[0][e], b = (double)(long long)sJ[1][e];
+        const double cz = (double)(long long)sJ[2][e];
         big = big || fabs(a) >= lim || fabs(b) >= lim || fabs(cz) >= lim;
         if (a != 0.0) unsafeAtomicAdd(&Jx[o], a);
         if (b != 0.0) unsafeAtomicAdd(&Jy[o], b);

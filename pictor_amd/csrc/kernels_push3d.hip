@@ -279,7 +279,7 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
     constexpr int WJ = T + 3, WJ3 = WJ * WJ * WJ;
     constexpr int MB = 54;   // species (<= 2) x 27 directions
     __shared__ float4 sN[2 * WN3];   // node e: [2e] = (Ex,Ey,Ez,0), [2e+1] = (Bx,By,Bz,0)
-    __shared__ double sJ[3][WJ3];
+    __shared__ unsigned long long sJ[3][WJ3];
     __shared__ int sStart[MB + 1], sVs[MB];
     __shared__ unsigned sSrc[MB], sDstBase[MB];
     __shared__ int sDstCap[MB], sDstBin[MB], sCnt[MB];
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
                                     (By[oi - mxy] + By[oi] + By[ok] + By[o]) * 0.25f + by0,
                                     (Bz[oi - mx] + Bz[oi] + Bz[oj] + Bz[o]) * 0.25f + bz0, 0.f);
     }
-    for (int e = tid; e < WJ3; e += BS) { sJ[0][e] = 0.0; sJ[1][e] = 0.0; sJ[2][e] = 0.0; }
+    for (int e = tid; e < WJ3; e += BS) { sJ[0][e] = 0ull; sJ[1][e] = 0ull; sJ[2][e] = 0ull; }
     __syncthreads();
 
     const float c = (float)ph.c, sqc = (float)ph.sqc, cinv = (float)ph.cinv;
@@ -403,7 +403,7 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
                 const float xr = fminf((float)(min(0, di) + 1), fmaxf((float)max(0, di), 0.5f * (x0 + x1)));
                 const float yr = fminf((float)(min(0, dj) + 1), fmaxf((float)max(0, dj), 0.5f * (y0 + y1)));
                 const float zr = fminf((float)(min(0, dk) + 1), fmaxf((float)max(0, dk), 0.5f * (z0 + z1)));
-                double a[12], bq[12];
+                long long a[12], bq[12];
 #pragma unroll
                 for (int seg = 0; seg < 2; seg++) {
                     const float Fx = seg ? qs * (x1 - xr) : qs * (xr - x0);
@@ -412,13 +412,13 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
                     const float Wx = seg ? 0.5f * (x1 + xr) - (float)di : 0.5f * (x0 + xr);
                     const float Wy = seg ? 0.5f * (y1 + yr) - (float)dj : 0.5f * (y0 + yr);
                     const float Wz = seg ? 0.5f * (z1 + zr) - (float)dk : 0.5f * (z0 + zr);
-                    double *t = seg ? bq : a;
-                    t[0] = quant((Fx * (1.0f - Wy)) * (1.0f - Wz)); t[1] = quant((Fx * Wy) * (1.0f - Wz));
-                    t[2] = quant((Fx * (1.0f - Wy)) * Wz);         t[3] = quant((Fx * Wy) * Wz);
-                    t[4] = quant((Fy * (1.0f - Wx)) * (1.0f - Wz)); t[5] = quant((Fy * Wx) * (1.0f - Wz));
-                    t[6] = quant((Fy * (1.0f - Wx)) * Wz);         t[7] = quant((Fy * Wx) * Wz);
-                    t[8] = quant((Fz * (1.0f - Wx)) * (1.0f - Wy)); t[9] = quant((Fz * Wx) * (1.0f - Wy));
-                    t[10] = quant((Fz * (1.0f - Wx)) * Wy);        t[11] = quant((Fz * Wx) * Wy);
+                    long long *t = seg ? bq : a;
+                    t[0] = quanti((Fx * (1.0f - Wy)) * (1.0f - Wz)); t[1] = quanti((Fx * Wy) * (1.0f - Wz));
+                    t[2] = quanti((Fx * (1.0f - Wy)) * Wz);         t[3] = quanti((Fx * Wy) * Wz);
+                    t[4] = quanti((Fy * (1.0f - Wx)) * (1.0f - Wz)); t[5] = quanti((Fy * Wx) * (1.0f - Wz));
+                    t[6] = quanti((Fy * (1.0f - Wx)) * Wz);         t[7] = quanti((Fy * Wx) * Wz);
+                    t[8] = quanti((Fz * (1.0f - Wx)) * (1.0f - Wy)); t[9] = quanti((Fz * Wx) * (1.0f - Wy));
+                    t[10] = quanti((Fz * (1.0f - Wx)) * Wy);        t[11] = quanti((Fz * Wx) * Wy);
                 }
                 const bool cross = (di != 0) || (dj != 0) || (dk != 0);
                 if (!cross) {
@@ -427,20 +427,20 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
                 }
                 constexpr int sy = WJ, sz = WJ * WJ;
                 {
-                    double *jp = &sJ[0][((gk + 1) * WJ + (gj + 1)) * WJ + (gi + 1)];
-                    atomicAdd(jp, a[0]); atomicAdd(jp + sy, a[1]); atomicAdd(jp + sz, a[2]); atomicAdd(jp + sy + sz, a[3]);
+                    unsigned long long *jp = &sJ[0][((gk + 1) * WJ + (gj + 1)) * WJ + (gi + 1)];
+                    atomicAdd(jp, (unsigned long long)a[0]); atomicAdd(jp + sy, (unsigned long long)a[1]); atomicAdd(jp + sz, (unsigned long long)a[2]); atomicAdd(jp + sy + sz, (unsigned long long)a[3]);
                     jp += WJ3;
-                    atomicAdd(jp, a[4]); atomicAdd(jp + 1, a[5]); atomicAdd(jp + sz, a[6]); atomicAdd(jp + 1 + sz, a[7]);
+                    atomicAdd(jp, (unsigned long long)a[4]); atomicAdd(jp + 1, (unsigned long long)a[5]); atomicAdd(jp + sz, (unsigned long long)a[6]); atomicAdd(jp + 1 + sz, (unsigned long long)a[7]);
                     jp += WJ3;
-                    atomicAdd(jp, a[8]); atomicAdd(jp + 1, a[9]); atomicAdd(jp + sy, a[10]); atomicAdd(jp + 1 + sy, a[11]);
+                    atomicAdd(jp, (unsigned long long)a[8]); atomicAdd(jp + 1, (unsigned long long)a[9]); atomicAdd(jp + sy, (unsigned long long)a[10]); atomicAdd(jp + 1 + sy, (unsigned long long)a[11]);
                 }
                 if (cross) {
-                    double *jp = &sJ[0][((gk + 1 + dk) * WJ + (gj + 1 + dj)) * WJ + (gi + 1 + di)];
-                    atomicAdd(jp, bq[0]); atomicAdd(jp + sy, bq[1]); atomicAdd(jp + sz, bq[2]); atomicAdd(jp + sy + sz, bq[3]);
+                    unsigned long long *jp = &sJ[0][((gk + 1 + dk) * WJ + (gj + 1 + dj)) * WJ + (gi + 1 + di)];
+                    atomicAdd(jp, (unsigned long long)bq[0]); atomicAdd(jp + sy, (unsigned long long)bq[1]); atomicAdd(jp + sz, (unsigned long long)bq[2]); atomicAdd(jp + sy + sz, (unsigned long long)bq[3]);
                     jp += WJ3;
-                    atomicAdd(jp, bq[4]); atomicAdd(jp + 1, bq[5]); atomicAdd(jp + sz, bq[6]); atomicAdd(jp + 1 + sz, bq[7]);
+                    atomicAdd(jp, (unsigned long long)bq[4]); atomicAdd(jp + 1, (unsigned long long)bq[5]); atomicAdd(jp + sz, (unsigned long long)bq[6]); atomicAdd(jp + 1 + sz, (unsigned long long)bq[7]);
                     jp += WJ3;
-                    atomicAdd(jp, bq[8]); atomicAdd(jp + 1, bq[9]); atomicAdd(jp + sy, bq[10]); atomicAdd(jp + 1 + sy, bq[11]);
+                    atomicAdd(jp, (unsigned long long)bq[8]); atomicAdd(jp + 1, (unsigned long long)bq[9]); atomicAdd(jp + sy, (unsigned long long)bq[10]); atomicAdd(jp + 1 + sy, (unsigned long long)bq[11]);
                 }
                 ox = x1 - (float)di;
                 if (ox >= 1.0f) { ox = 0.0f; di += 1; }
@@ -507,7 +507,8 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
     bool big = false;
     for (int e = tid; e < WJ3; e += BS) {
         const long long o = fidx(g, i0 - 1 + e % WJ, j0 - 1 + (e / WJ) % WJ, k0 - 1 + e / (WJ * WJ));
-        const double a = sJ[0][e], b = sJ[1][e], cz = sJ[2][e];
+        const double a = (double)(long long)sJ[0][e], b = (double)(long long)sJ[1][e];
+        const double cz = (double)(long long)sJ[2][e];
         big = big || fabs(a) >= lim || fabs(b) >= lim || fabs(cz) >= lim;
         if (a != 0.0) unsafeAtomicAdd(&Jx[o], a);
         if (b != 0.0) unsafeAtomicAdd(&Jy[o], b);

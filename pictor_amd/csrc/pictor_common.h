@@ -13,15 +13,17 @@
 
 namespace pictor {
 
-// Split-form current: every contribution is rounded to a multiple of 2^-36
-// (charge x cell) and summed in FP64 as an integer count of 2^-36 units.  All
-// partial sums stay integers below 2^53, so every addition is exact: the sum
-// does not depend on the order of the atomics and is deterministic.  A node
-// gets contributions from at most 4 tile windows and their ghost images; each
-// tile's partial sum is checked against kFixLimit at the flush, which keeps the
-// total below 8 * 2^14 = 2^17 = 2^53 * 2^-36.
-constexpr double kFixScale = 68719476736.0;          // 2^36
-constexpr double kFixInv = 1.0 / 68719476736.0;
+// Split-form current: every contribution is rounded to a multiple of 2^-30
+// (charge x cell), below 2^31 counts for a unit charge, so one conversion makes
+// it an integer; a tile sums its window with 64-bit integer LDS atomics and
+// adds it to J, which holds the counts in FP64.  All partial sums stay integers
+// below 2^53, so every addition is exact: the sum does not depend on the order
+// of the atomics and is deterministic.  A node gets contributions from at most
+// 4 (2-D) or 8 (3-D) tile windows and their ghost images; each tile's partial
+// sum is checked against kFixLimit at the flush, which keeps the total below
+// 16 * 2^14 = 2^18 charge cells, far inside 2^53 * 2^-30 = 2^23.
+constexpr double kFixScale = 1073741824.0;           // 2^30
+constexpr double kFixInv = 1.0 / 1073741824.0;
 constexpr double kFixLimit = 16384.0;                // per-tile |node sum| bound (charge x cell)
 
 enum ErrBits : int { kErrBounds = 1, kErrCapacity = 2, kErrPrecision = 4 };

@@ -46,16 +46,17 @@ __device__ inline int norm_dir(int dd, int n)
     return dd;
 }
 
-// FP32 -> exact integer count of 2^-36 units (the argument is already scaled by 2^36)
-__device__ inline double quant(float v) { return (double)__builtin_rintf(v); }
+// FP32 -> exact integer count of 2^-30 units (the argument is already scaled by 2^30;
+// |v| < 2^31, so one conversion)
+__device__ inline long long quanti(float v) { return (long long)(int)__builtin_rintf(v); }
 
 // The two zigzag half-segments of the 2-D split form (movdep.F90:177-237)
 // from the old offsets (x0, y0, z0) to the new position as stored: offsets
 // (ox, oy) in the cell displaced by (di, dj), z advanced to z1.  Evaluated in
 // FP64, every relay point, flux F and weight W is exact for FP32 inputs and
-// each node value is rounded once before its rounding to a 2^-36 count, so
+// each node value is rounded once before its rounding to a 2^-30 count, so
 // the deposited Jx, Jy carry exactly the charge moved between the stored
-// positions up to that last rounding.  qs: charge times 2^36.  a: the first
+// positions up to that last rounding.  qs: charge times 2^30.  a: the first
 // half-segment (nodes of the old cell), b: the second (nodes of the new cell),
 // each Jx (i,j), (i,j+1); Jy (i,j), (i+1,j); Jz (i,j), (i+1,j), (i,j+1), (i+1,j+1).
 __device__ inline void zigzag2d_split(double qs, float x0, float y0, float z0, float ox, float oy, float z1, int di,

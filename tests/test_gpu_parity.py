@@ -151,8 +151,8 @@ def test_split_vs_fp64_oracle(nx, ny, tile):
 def test_split_gauss_residual_stays_at_rounding(ndim):
     # the split form deposits the exact charge flux of its stored positions (2-D) and
     # accumulates E in FP64 as an FP32 (hi, lo) pair, so in 2-D div E - rho only drifts by
-    # the rounding of each deposit to the 2^-36 q fixed-point grid of the current arrays
-    # (a random walk of about 1e-11 q per deposit: 2.5e-10 of max |rho| after 300 steps
+    # the rounding of each deposit to the 2^-30 q fixed-point grid of the current arrays
+    # (a random walk of about 1e-9 q per deposit: about 2e-8 of max |rho| after 300 steps
     # here, where an FP32 E accumulates 1e-6 over 2000 steps at 2048^2); in 3-D the
     # reference zigzag itself drops the q dx dy dz / 12 term, so there the bar is the FP64
     # form's own residual
@@ -170,7 +170,7 @@ def test_split_gauss_residual_stays_at_rounding(ndim):
             res[form] = d["gauss_residual"] / d["rho_max"]
     say(f"{ndim}-D Gauss residual / max rho after {nsteps} steps: {res}")
     if ndim == 2:
-        assert res["fp32_split"] <= 1e-8
+        assert res["fp32_split"] <= 1e-7
     else:
         assert res["fp32_split"] <= 2 * res["fp64"] + 1e-11
 
