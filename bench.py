@@ -240,6 +240,18 @@ def accuracy_report(args, ndim, nx, ny, nz, epc, P):
     t0 = time.time()
     series["fp64_f32_momenta"] = _series_gpu("fp64", ndim, nx, ny, nz, epc, Pr, steps, every)
     print(f"# fp64 from FP32-rounded momenta: {steps} steps in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    # an FP64 ensemble: the same run from momenta perturbed by 2^-24 relative Gaussian noise
+    # (FP32's relative resolution), one seed per member; its spread around the reference is
+    # what the chaotic late phase does to a perturbation of the size the split record rounds to
+    for k in range(1, args.ensemble + 1):
+        rng = np.random.default_rng(k)
+        Pk = dict(P)
+        for c in ("u", "v", "w"):
+            a = np.asarray(P[c])
+            Pk[c] = a * (1.0 + 2.0 ** -24 * rng.standard_normal(len(a)))
+        t0 = time.time()
+        series[f"fp64_ensemble_{k}"] = _series_gpu("fp64", ndim, nx, ny, nz, epc, Pk, steps, every)
+        print(f"# fp64 ensemble member {k}: {steps} steps in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     if args.cpu_steps != 0:
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
         t0 = time.time()
@@ -647,6 +659,23 @@ def atomics_report(iters=4096):
             "lane_ops_per_cu_clock": {k: out[8 + i] for i, k in enumerate(names)}}
 
 
+def stream_report(n=268435456):
+    """The stream ceiling of the 2-D split record's access pattern: six FP32
+    arrays of n elements read and six written (48 B per element), with one,
+    two (lane-contiguous) or four elements per lane (pictor_probe_stream)."""
+    import ctypes
+
+    from pictor_amd._lib import check, lib
+
+    out = (ctypes.c_double * 8)()
+    check(lib().pictor_probe_stream(n, out))
+    names = ("one_4B_element_per_lane", "two_4B_elements_per_lane", "float2_per_lane", "float4_per_lane")
+    return {"workload": f"6 float arrays of {n} elements read, 6 written (48 B per element)",
+            "bytes_per_s": {k: out[i] for i, k in enumerate(names)},
+            "ms": {k: out[4 + i] for i, k in enumerate(names)},
+            "frac_of_8TBps": {k: out[i] / 8e12 for i, k in enumerate(names)}}
+
+
 def host_info():
     model = ""
     try:
@@ -708,6 +737,8 @@ def main():
                     help="instead of timing: run --steps steps of both forms (and the oracle unless "
                          "--cpu-steps 0) and report field energy, growth rate and Gauss residual errors")
     ap.add_argument("--every", type=int, default=10, help="diagnostic interval of --accuracy")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="--accuracy: FP64 members from momenta perturbed by 2^-24 relative noise")
     ap.add_argument("--adjoint", action="store_true",
                     help="instead of timing the step: the adjoint workload (gradients, FD check, cost ratios) "
                          "over --steps steps with checkpoints every --segment")
@@ -728,6 +759,8 @@ def main():
                     help="split vs FP64 on lwfa_1024x128 (no window): energies every --every steps, Gauss change")
     ap.add_argument("--atomics", action="store_true",
                     help="instead of the Weibel step: LDS versus global atomic throughput (pictor_probe_atomics)")
+    ap.add_argument("--stream", action="store_true",
+                    help="instead of the Weibel step: the stream ceiling of the particle arrays (pictor_probe_stream)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -743,6 +776,9 @@ def main():
         return
     if args.atomics:
         print(json.dumps(atomics_report()), flush=True)
+        return
+    if args.stream:
+        print(json.dumps(stream_report()), flush=True)
         return
     if args.lwfa_accuracy:
         print(json.dumps(lwfa_accuracy_report(args)), flush=True)

@@ -79,7 +79,7 @@ typedef struct pictor_config {
     double c_ompe;         /* parameters.F90:19 */
     double g0;             /* parameters.F90:16 */
     double mi_me;          /* parameters.F90:18 */
-    double capacity_factor;/* particle storage per tile / particles at layout time (0 = 1.25) */
+    double capacity_factor;/* particle storage per tile / particles at layout time (0 = 1.75) */
     double dr;             /* PICTOR_GEOM_CYL_MODAL: radial cell in axial cells (0 = 1) */
 } pictor_config;
 
@@ -111,6 +111,19 @@ int pictor_particle_count(pictor_ctx *ctx, int64_t *n);
 int pictor_download_particles(pictor_ctx *ctx, int64_t cap, double *q, double *x, double *y,
                               double *z, double *u, double *v, double *w, int32_t *flv,
                               int32_t *tile, int64_t *n_out);
+/* The same two calls with the reference's per-particle tag (tagp, src/vars.F90:43 and the
+ * particle type :54-79; set by GetTag, init_prtl.F90:160): an int32 carried unchanged with
+ * every particle through pushes, re-layouts, spills and restart files, so that a caller can
+ * re-associate the particles after a download in tile order (Cartesian geometry, both forms).
+ * A tagged upload makes the context carry tags until an untagged upload; downloading tags
+ * from an untagged context is an error.  The tag costs 4 B read and 4 B written per
+ * particle-step. */
+int pictor_upload_particles_tagged(pictor_ctx *ctx, int64_t n, const double *q, const double *x,
+                                   const double *y, const double *z, const double *u, const double *v,
+                                   const double *w, const int32_t *flv, const int32_t *tag);
+int pictor_download_particles_tagged(pictor_ctx *ctx, int64_t cap, double *q, double *x, double *y,
+                                     double *z, double *u, double *v, double *w, int32_t *flv,
+                                     int32_t *tile, int32_t *tag, int64_t *n_out);
 
 /* ---- restart files: SaveRestartData, src/io/savedata.F90:73-141 (writers :940-1210),
  *      and the reload, src/io/reload.F90:17-338 (restart branch src/MAIN.F90:35-39).
@@ -332,6 +345,13 @@ int pictor_energy_noalloc(pictor_ctx *ctx, double out[7]);
  * global FP64, FP32 atomics (4 Mi-element array); out[8..13] the same per CU
  * per clock; out[6] CUs, out[7] clock (Hz). */
 int pictor_probe_atomics(int32_t iters, double out[16]);
+/* The stream ceiling of the particle record's access pattern on the current
+ * device: six FP32 arrays of n elements read and six written (48 B per
+ * element; n a multiple of 512, at most 2^28), with one 4-byte element per
+ * lane (out[0]), two lane-contiguous 4-byte elements per lane (out[1]), a
+ * float2 (out[2]) and a float4 (out[3]) per lane: bytes per second; out[4..7]
+ * the milliseconds per pass. */
+int pictor_probe_stream(int64_t n, double out[8]);
 
 #ifdef __cplusplus
 }

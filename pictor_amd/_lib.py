@@ -42,7 +42,8 @@ EXPORTS = (
     "pictor_save_state", "pictor_load_state", "pictor_set_bc_fld", "pictor_set_bc_prtl",
     "pictor_set_boundary_speed", "pictor_pml", "pictor_enforce_bc_final", "pictor_save_energy",
     "pictor_save_fields", "pictor_energy_noalloc", "pictor_adjoint_forward_device",
-    "pictor_adjoint_backward_device",
+    "pictor_adjoint_backward_device", "pictor_probe_stream", "pictor_upload_particles_tagged",
+    "pictor_download_particles_tagged",
 )
 
 
@@ -88,6 +89,8 @@ def lib():
     L.pictor_upload_particles.argtypes = [c_vp, i64] + [c_dp] * 7 + [c_ip]
     L.pictor_particle_count.argtypes = [c_vp, ctypes.POINTER(i64)]
     L.pictor_download_particles.argtypes = [c_vp, i64] + [c_dp] * 7 + [c_ip, c_ip, ctypes.POINTER(i64)]
+    L.pictor_upload_particles_tagged.argtypes = [c_vp, i64] + [c_dp] * 7 + [c_ip, c_ip]
+    L.pictor_download_particles_tagged.argtypes = [c_vp, i64] + [c_dp] * 7 + [c_ip, c_ip, c_ip, ctypes.POINTER(i64)]
     for name in ("pictor_update_b_half", "pictor_move_deposit", "pictor_exchange_particles", "pictor_update_e",
                  "pictor_current_halo_add", "pictor_smoothen_current", "pictor_add_current", "pictor_reset_current", "pictor_sort_tiles",
                  "pictor_synchronize", "pictor_timer_reset"):
@@ -108,6 +111,7 @@ def lib():
     L.pictor_adjoint_release.argtypes = [c_vp]
     L.pictor_adjoint_engine.argtypes = [c_vp, i32]
     L.pictor_probe_atomics.argtypes = [i32, c_dp]
+    L.pictor_probe_stream.argtypes = [i64, c_dp]
     L.pictor_cyl_plasma.argtypes = [c_vp, i32, i32, i32, i32, ctypes.c_uint64]
     L.pictor_cyl_push_scheme.argtypes = [c_vp, i32]
     L.pictor_cyl_sort_period.argtypes = [c_vp, i32]

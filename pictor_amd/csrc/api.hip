@@ -33,20 +33,21 @@ void field_ptrs(pictor_ctx *c, R *E[3], R *B[3])
 void free_f64(PartF64 &p)
 {
     dfree(p.q); dfree(p.x); dfree(p.y); dfree(p.z);
-    dfree(p.u); dfree(p.v); dfree(p.w); dfree(p.flv);
+    dfree(p.u); dfree(p.v); dfree(p.w); dfree(p.flv); dfree(p.tag);
     p = PartF64{};
 }
 
 void free_split(PartSplit &p)
 {
     dfree(p.cell); dfree(p.ox); dfree(p.oy); dfree(p.oz);
-    dfree(p.u); dfree(p.v); dfree(p.w); dfree(p.sp);
+    dfree(p.u); dfree(p.v); dfree(p.w); dfree(p.sp); dfree(p.id);
     p = PartSplit{};
 }
 
-hipError_t alloc_f64(PartF64 &p, long long n)
+hipError_t alloc_f64(PartF64 &p, long long n, bool tagged = false)
 {
     hipError_t e;
+    if (tagged && (e = dalloc(&p.tag, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.q, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.x, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.y, n)) != hipSuccess) return e;
@@ -58,9 +59,10 @@ hipError_t alloc_f64(PartF64 &p, long long n)
 }
 
 // oz only in 3-D: the 2-D split record has no z (kZMid)
-hipError_t alloc_split(PartSplit &p, long long n, bool with_sp, int ndim)
+hipError_t alloc_split(PartSplit &p, long long n, bool with_sp, int ndim, bool tagged = false)
 {
     hipError_t e;
+    if (tagged && (e = dalloc(&p.id, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.cell, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.ox, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.oy, n)) != hipSuccess) return e;
@@ -82,10 +84,10 @@ int ensure_buffer(pictor_ctx *c, int which, long long need)
     if (c->form != PICTOR_FORM_FP64) n = std::min(n, (1LL << 30) - 1);
     if (c->form == PICTOR_FORM_FP64) {
         free_f64(c->pf[which]);
-        HIPTRY(alloc_f64(c->pf[which], n));
+        HIPTRY(alloc_f64(c->pf[which], n, c->tagged));
     } else {
         free_split(c->ps[which]);
-        HIPTRY(alloc_split(c->ps[which], n, false, c->g.ndim));
+        HIPTRY(alloc_split(c->ps[which], n, false, c->g.ndim, c->tagged));
     }
     c->buf_cap[which] = n;
     return PICTOR_OK;
@@ -97,11 +99,11 @@ int ensure_spill(pictor_ctx *c, int need)
     if (c->form == PICTOR_FORM_FP64) {
         free_f64(c->spill_f);
         c->spill_cap = 0;
-        HIPTRY(alloc_f64(c->spill_f, need));
+        HIPTRY(alloc_f64(c->spill_f, need, c->tagged));
     } else {
         free_split(c->spill_s);
         c->spill_cap = 0;
-        HIPTRY(alloc_split(c->spill_s, need, true, c->g.ndim));
+        HIPTRY(alloc_split(c->spill_s, need, true, c->g.ndim, c->tagged));
     }
     dfree(c->rl_grp);
     c->rl_grp = nullptr;
@@ -115,12 +117,22 @@ int ensure_spill(pictor_ctx *c, int need)
 // Bin capacities from the particle count of every (tile, species) group.
 // A tile can lose or gain at most the particles within c cells of a face per
 // step; the side bins are sized for that flux with a margin, the centre bin
-// for the whole group times capacity_factor.
+// for the whole group times capacity_factor.  A re-layout also passes the bin
+// counts of the last push (`seen`, with the capacities they had): a bin that
+// was full gets twice the room (the centre bin: twice its group's count, a
+// side bin: twice its old capacity), and a side bin at least 1.5 times the
+// arrivals it saw.  In the nonlinear Weibel phase the filaments keep
+// gathering particles into the same tiles, whose centre bins would otherwise
+// overflow again a few dozen steps after every re-layout.
 void compute_layout_into(const pictor_ctx *c, const std::vector<long long> &counts, std::vector<long long> &h_base,
-                         std::vector<int> &h_cap, long long &total)
+                         std::vector<int> &h_cap, long long &total, const std::vector<int> *seen = nullptr,
+                         const std::vector<int> *seen_cap = nullptr)
 {
     const TileP &tp = c->tp;
-    const double cf = c->cfg.capacity_factor > 0 ? c->cfg.capacity_factor : 1.25;
+    // 1.75: the filaments of the nonlinear Weibel phase grow some tiles by half their
+    // population within a few hundred steps (C2: 63 re-layouts over 2000 steps at 1.25,
+    // 16 at 1.75, for 8 GB more of the 288 GB)
+    const double cf = c->cfg.capacity_factor > 0 ? c->cfg.capacity_factor : 1.75;
     const double frac = std::min(1.0, c->ph.c / tp.T);
     h_base.assign((size_t)tp.ntiles * tp.NB, 0);
     h_cap.assign((size_t)tp.ntiles * tp.NB, 0);
@@ -132,10 +144,19 @@ void compute_layout_into(const pictor_ctx *c, const std::vector<long long> &coun
                 int nz = 0, r = d;
                 for (int a = 0; a < c->g.ndim; a++) { nz += (r % 3 != 1); r /= 3; }
                 double cap;
+                const size_t b = (size_t)t * tp.NB + s * tp.ND + d;
                 if (nz == 0) cap = std::ceil(n * cf) + 64;
                 else cap = std::ceil(n * std::pow(frac, nz) * 1.6 * std::pow(2.0, nz - 1)) + (nz == 1 ? 32 : 16);
+                if (seen && seen_cap && b < seen->size() && b < seen_cap->size()) {
+                    const int was = (*seen)[b], had = (*seen_cap)[b];
+                    if (nz == 0) {
+                        if (was >= had) cap = std::max(cap, 2.0 * n + 64);
+                    } else {
+                        cap = std::max(cap, std::ceil(1.5 * was) + 16);
+                        if (was >= had) cap = std::max(cap, 2.0 * had);
+                    }
+                }
                 long long ci = ((long long)cap + 15) / 16 * 16;
-                const size_t b = (size_t)t * tp.NB + s * tp.ND + d;
                 h_base[b] = off;
                 h_cap[b] = (int)std::min<long long>(ci, 0x7fffffff);
                 off += ci;
@@ -191,10 +212,12 @@ std::vector<long long> group_offsets(const pictor_ctx *c, bool dense, const std:
 }
 
 // per-group particle counts of buffer `which` (host copy of the bin counts)
-int group_counts(pictor_ctx *c, int which, std::vector<long long> &counts)
+int group_counts(pictor_ctx *c, int which, std::vector<long long> &counts, std::vector<int> *bins = nullptr)
 {
     const TileP &tp = c->tp;
-    std::vector<int> cnt((size_t)tp.ntiles * tp.NB);
+    std::vector<int> local;
+    std::vector<int> &cnt = bins ? *bins : local;
+    cnt.resize((size_t)tp.ntiles * tp.NB);
     HIPTRY(hipMemcpyAsync(cnt.data(), c->d_cnt[which], cnt.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipStreamSynchronize(c->stream));
     counts.assign((size_t)tp.ntiles * tp.nsp, 0);
@@ -217,7 +240,8 @@ int relayout(pictor_ctx *c, int nspill)
     const size_t ng = (size_t)tp.ntiles * tp.nsp;
     if (nspill > c->rl_grp_cap) return fail(PICTOR_ERR_CAPACITY, "re-layout: more spilled particles than the spill buffer");
     std::vector<long long> counts;
-    int rc = group_counts(c, c->cur, counts);
+    std::vector<int> seen;
+    int rc = group_counts(c, c->cur, counts, &seen);
     if (rc) return rc;
     HIPTRY(hipMemsetAsync(c->rl_gcount, 0, ng * sizeof(int), c->stream));
     if (nspill > 0) {
@@ -235,7 +259,7 @@ int relayout(pictor_ctx *c, int nspill)
     std::vector<long long> nbase;
     std::vector<int> ncap;
     long long ntotal = 0;
-    compute_layout_into(c, total, nbase, ncap, ntotal);
+    compute_layout_into(c, total, nbase, ncap, ntotal, &seen, &c->h_cap);
     const int dst = 1 - c->cur;
     if ((rc = ensure_buffer(c, dst, ntotal))) return rc;
     std::vector<long long> off = group_offsets(c, false, total, &nbase, ntotal);
@@ -766,8 +790,10 @@ int pictor_download_fields(pictor_ctx *c, double *ex, double *ey, double *ez, do
     return PICTOR_OK;
 }
 
-int pictor_upload_particles(pictor_ctx *c, int64_t n, const double *q, const double *x, const double *y,
-                            const double *z, const double *u, const double *v, const double *w, const int32_t *flv)
+// the upload of both entry points; tag (may be null): the caller's per-particle tags
+static int upload_particles_impl(pictor_ctx *c, int64_t n, const double *q, const double *x, const double *y,
+                                 const double *z, const double *u, const double *v, const double *w,
+                                 const int32_t *flv, const int32_t *tag)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -775,13 +801,25 @@ int pictor_upload_particles(pictor_ctx *c, int64_t n, const double *q, const dou
     if (n > 0 && (!q || !x || !y || !z || !u || !v || !w || !flv)) return fail(PICTOR_ERR_ARG, "null particle pointer");
     if (n >= 0x7fffffffLL) return fail(PICTOR_ERR_ARG, "more than 2^31-1 particles per device");
     c->pending_swap = false;
-    if (c->cyl) return c->cylm ? cylm_upload_particles(c, n, q, x, y, z, u, v, w, flv) : cyl_upload_particles(c, n, q, x, y, z, u, v, w, flv);
+    if (c->cyl) {
+        if (tag) return fail(PICTOR_ERR_ARG, "tagged particles: Cartesian geometry only");
+        return c->cylm ? cylm_upload_particles(c, n, q, x, y, z, u, v, w, flv) : cyl_upload_particles(c, n, q, x, y, z, u, v, w, flv);
+    }
+    const bool tagged = tag != nullptr;
+    if (tagged != c->tagged) {
+        // the particle buffers change their record: the next allocation brings (or drops) the tags
+        for (int i = 0; i < 2; i++) { free_f64(c->pf[i]); free_split(c->ps[i]); c->buf_cap[i] = 0; }
+        free_f64(c->spill_f); free_split(c->spill_s);
+        c->spill_cap = 0;
+        c->tagged = tagged;
+    }
     const TileP &tp = c->tp;
     const size_t ng = (size_t)tp.ntiles * tp.nsp;
     PartF64 raw{};
     int *d_tile = nullptr, *d_count = nullptr, *d_fill = nullptr;
     long long *d_off = nullptr;
-    HIPTRY(alloc_f64(raw, n));
+    HIPTRY(alloc_f64(raw, n, tagged));
+    if (tagged && n > 0) HIPTRY(hipMemcpyAsync(raw.tag, tag, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     HIPTRY(dalloc(&d_tile, n));
     HIPTRY(dalloc(&d_count, ng));
     HIPTRY(dalloc(&d_fill, ng));
@@ -832,6 +870,21 @@ int pictor_upload_particles(pictor_ctx *c, int64_t n, const double *q, const dou
     return PICTOR_OK;
 }
 
+int pictor_upload_particles(pictor_ctx *c, int64_t n, const double *q, const double *x, const double *y,
+                            const double *z, const double *u, const double *v, const double *w, const int32_t *flv)
+{
+    return upload_particles_impl(c, n, q, x, y, z, u, v, w, flv, nullptr);
+}
+
+int pictor_upload_particles_tagged(pictor_ctx *c, int64_t n, const double *q, const double *x, const double *y,
+                                   const double *z, const double *u, const double *v, const double *w,
+                                   const int32_t *flv, const int32_t *tag)
+{
+    if (n > 0 && !tag) return fail(PICTOR_ERR_ARG, "null tag pointer");
+    static const int32_t none = 0;
+    return upload_particles_impl(c, n, q, x, y, z, u, v, w, flv, tag ? tag : &none);
+}
+
 int pictor_particle_count(pictor_ctx *c, int64_t *n)
 {
     int rc = check_ctx(c);
@@ -846,12 +899,14 @@ int pictor_particle_count(pictor_ctx *c, int64_t *n)
     return PICTOR_OK;
 }
 
-int pictor_download_particles(pictor_ctx *c, int64_t cap, double *q, double *x, double *y, double *z, double *u,
-                              double *v, double *w, int32_t *flv, int32_t *tile, int64_t *n_out)
+static int download_particles_impl(pictor_ctx *c, int64_t cap, double *q, double *x, double *y, double *z,
+                                   double *u, double *v, double *w, int32_t *flv, int32_t *tile, int32_t *tag,
+                                   int64_t *n_out)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
     if (c->pending_swap) return fail(PICTOR_ERR_ARG, "download between move_deposit and exchange_particles");
+    if (tag && !c->tagged) return fail(PICTOR_ERR_ARG, "the particles carry no tags (upload them with pictor_upload_particles_tagged)");
     if (c->cyl) {
         long long no = 0;
         rc = c->cylm ? cylm_download_particles(c, cap, q, x, y, z, u, v, w, flv, tile, &no)
@@ -874,12 +929,12 @@ int pictor_download_particles(pictor_ctx *c, int64_t cap, double *q, double *x, 
     PartF64 st{};
     PartSplit ss{};
     HIPTRY(dalloc(&d_off, ng + 1));
-    HIPTRY(alloc_f64(st, n));
+    HIPTRY(alloc_f64(st, n, c->tagged));
     HIPTRY(hipMemcpyAsync(d_off, off.data(), (ng + 1) * sizeof(long long), hipMemcpyHostToDevice, c->stream));
     if (c->form == PICTOR_FORM_FP64) {
         launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], d_off, nullptr, st, c->stream);
     } else {
-        HIPTRY(alloc_split(ss, n, true, c->g.ndim));
+        HIPTRY(alloc_split(ss, n, true, c->g.ndim, c->tagged));
         launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], d_off, nullptr, ss, c->stream);
         launch_split_to_f64(c->g, c->ph, ss, n, st, c->stream);
     }
@@ -894,10 +949,24 @@ int pictor_download_particles(pictor_ctx *c, int64_t cap, double *q, double *x, 
         if (hd[i]) HIPTRY(hipMemcpyAsync(hd[i], dd[i], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (flv) HIPTRY(hipMemcpyAsync(flv, st.flv, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     if (tile) HIPTRY(hipMemcpyAsync(tile, d_tile, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (tag) HIPTRY(hipMemcpyAsync(tag, st.tag, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipStreamSynchronize(c->stream));
     dfree(d_off); dfree(d_tile);
     free_f64(st); free_split(ss);
     return PICTOR_OK;
+}
+
+int pictor_download_particles(pictor_ctx *c, int64_t cap, double *q, double *x, double *y, double *z, double *u,
+                              double *v, double *w, int32_t *flv, int32_t *tile, int64_t *n_out)
+{
+    return download_particles_impl(c, cap, q, x, y, z, u, v, w, flv, tile, nullptr, n_out);
+}
+
+int pictor_download_particles_tagged(pictor_ctx *c, int64_t cap, double *q, double *x, double *y, double *z,
+                                     double *u, double *v, double *w, int32_t *flv, int32_t *tile, int32_t *tag,
+                                     int64_t *n_out)
+{
+    return download_particles_impl(c, cap, q, x, y, z, u, v, w, flv, tile, tag, n_out);
 }
 
 int pictor_update_b_half(pictor_ctx *c)

@@ -128,6 +128,9 @@ struct pictor_ctx {
     long long t_n[3] = {0, 0, 0};
 
     long long np = 0;
+    // the particles carry the caller's tags (uploaded with pictor_upload_particles_tagged):
+    // a tag array beside every particle buffer (PartF64::tag, PartSplit::id)
+    bool tagged = false;
     long long n_relayout = 0;   // re-layouts after a spill (bins overflowed) since init
     long long n_spilled = 0;    // particles those re-layouts took from the spill buffer
 

@@ -96,6 +96,7 @@ __global__ void k_raw_scatter_f64(TileP tp, PartF64 raw, long long n, const int 
     const long long o = cbase[t] + perm_slot(slot, gtotal, t);
     out.q[o] = raw.q[i]; out.x[o] = raw.x[i]; out.y[o] = raw.y[i]; out.z[o] = raw.z[i];
     out.u[o] = raw.u[i]; out.v[o] = raw.v[i]; out.w[o] = raw.w[i]; out.flv[o] = raw.flv[i];
+    if (out.tag && raw.tag) out.tag[o] = raw.tag[i];
 }
 
 __global__ void k_raw_scatter_split(GridP g, TileP tp, PartF64 raw, long long n, const int *tile_of,
@@ -118,7 +119,8 @@ __global__ void k_raw_scatter_split(GridP g, TileP tp, PartF64 raw, long long n,
     out.ox[o] = ox; out.oy[o] = oy;
     if (g.ndim == 3) out.oz[o] = oz;
     out.u[o] = (float)raw.u[i]; out.v[o] = (float)raw.v[i]; out.w[o] = (float)raw.w[i];
-    if (out.id) out.id[o] = (int)i;
+    // the carried value: the caller's tag, or (adjoint engine) the caller's index
+    if (out.id) out.id[o] = raw.tag ? raw.tag[i] : (int)i;
     (void)s;
 }
 
@@ -173,6 +175,7 @@ __global__ void k_compact_f64(TileP tp, PartF64 src, const long long *base, cons
     compact_group(tp, base, cnt, dst_off[blockIdx.x], gtotal, [&](long long a, long long b) {
         dst.q[b] = src.q[a]; dst.x[b] = src.x[a]; dst.y[b] = src.y[a]; dst.z[b] = src.z[a];
         dst.u[b] = src.u[a]; dst.v[b] = src.v[a]; dst.w[b] = src.w[a]; dst.flv[b] = src.flv[a];
+        if (dst.tag && src.tag) dst.tag[b] = src.tag[a];
     });
 }
 
@@ -185,6 +188,7 @@ __global__ void k_compact_split(TileP tp, PartSplit src, const long long *base, 
         if (dst.oz) dst.oz[b] = src.oz[a];
         dst.u[b] = src.u[a]; dst.v[b] = src.v[a]; dst.w[b] = src.w[a];
         if (dst.sp) dst.sp[b] = s;
+        if (dst.id && src.id) dst.id[b] = src.id[a];
     });
 }
 
@@ -197,6 +201,7 @@ __global__ void k_spill_scatter_f64(PartF64 sp, int n, const int *grp_of, const 
     const long long o = dst_off[grp] + perm_slot(atomicAdd(&fill[grp], 1), gtotal, grp);
     dst.q[o] = sp.q[i]; dst.x[o] = sp.x[i]; dst.y[o] = sp.y[i]; dst.z[o] = sp.z[i];
     dst.u[o] = sp.u[i]; dst.v[o] = sp.v[i]; dst.w[o] = sp.w[i]; dst.flv[o] = sp.flv[i];
+    if (dst.tag && sp.tag) dst.tag[o] = sp.tag[i];
 }
 
 __global__ void k_spill_scatter_split(PartSplit sp, int n, const int *grp_of, const long long *dst_off, int *fill,
@@ -209,6 +214,7 @@ __global__ void k_spill_scatter_split(PartSplit sp, int n, const int *grp_of, co
     dst.cell[o] = sp.cell[i]; dst.ox[o] = sp.ox[i]; dst.oy[o] = sp.oy[i];
     if (dst.oz) dst.oz[o] = sp.oz[i];
     dst.u[o] = sp.u[i]; dst.v[o] = sp.v[i]; dst.w[o] = sp.w[i];
+    if (dst.id && sp.id) dst.id[o] = sp.id[i];
 }
 
 // split -> reference FP64 record (x = 3 + cell + offset; 2-D z = 1 + kZMid, z is not stored)
@@ -225,6 +231,7 @@ __global__ void k_split_to_f64(GridP g, PhysP ph, PartSplit src, long long n, Pa
     dst.u[i] = src.u[i]; dst.v[i] = src.v[i]; dst.w[i] = src.w[i];
     dst.q[i] = ph.flvq[s];
     dst.flv[i] = s + 1;
+    if (dst.tag && src.id) dst.tag[i] = src.id[i];
 }
 
 __global__ void k_tile_ids(TileP tp, const long long *dst_off, int *tile_out)

@@ -33,7 +33,7 @@ using namespace pushdev;
 // ---------------------------------------------------------------------------
 // FP64 form, 2-D
 // ---------------------------------------------------------------------------
-template <int T, int BS, bool BC>
+template <int T, int BS, bool BC, bool TAG = false>
 __global__ __launch_bounds__(BS) void k_push2d_f64(GridP g, PhysP ph, TileP tp, const double *__restrict__ Ex,
                                                    const double *__restrict__ Ey, const double *__restrict__ Ez,
                                                    const double *__restrict__ Bx, const double *__restrict__ By,
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(BS) void k_push2d_f64(GridP g, PhysP ph, TileP tp, 
         const bool active = k < total;
         double x0 = 0, y0 = 0, z0 = 0, un = 0, vn = 0, wn = 0, q = 0;
         double xn = 0, yn = 0, zn = 0;
-        int flv = 0, d = -1;
+        int flv = 0, d = -1, tg = 0;
         if (active) {
             int b = 0;
             while (k >= sStart[b + 1]) b++;
@@ -106,6 +106,7 @@ __global__ __launch_bounds__(BS) void k_push2d_f64(GridP g, PhysP ph, TileP tp, 
             x0 = A.x[src]; y0 = A.y[src]; z0 = A.z[src];
             un = A.u[src]; vn = A.v[src]; wn = A.w[src];
             q = A.q[src]; flv = A.flv[src];
+            if (TAG) tg = A.tag[src];
             const int ip0 = (int)x0, jp0 = (int)y0, kp0 = (int)z0;
             const int ii0 = ip0 - iw, jj0 = jp0 - jw;
             xn = x0; yn = y0; zn = z0;
@@ -239,12 +240,14 @@ __global__ __launch_bounds__(BS) void k_push2d_f64(GridP g, PhysP ph, TileP tp, 
                 Bo.x[o] = xn; Bo.y[o] = yn; Bo.z[o] = zn;
                 Bo.u[o] = un; Bo.v[o] = vn; Bo.w[o] = wn;
                 Bo.q[o] = q; Bo.flv[o] = flv;
+                if (TAG) Bo.tag[o] = tg;
             } else {
                 const int s = atomicAdd(sp.count, 1);
                 if (s < sp.cap) {
                     spill.x[s] = xn; spill.y[s] = yn; spill.z[s] = zn;
                     spill.u[s] = un; spill.v[s] = vn; spill.w[s] = wn;
                     spill.q[s] = q; spill.flv[s] = flv;
+                    if (TAG) spill.tag[s] = tg;
                 } else {
                     atomicOr(sp.err, (int)kErrCapacity);
                 }
@@ -268,7 +271,7 @@ __global__ __launch_bounds__(BS) void k_push2d_f64(GridP g, PhysP ph, TileP tp, 
 struct SplitRec {
     int cell, s;
     float ox, oy, u, v, w;
-    int slot, id;   // input slot and particle index (TRACK only)
+    int slot, id;   // input slot (TRACK) and the carried value: particle index (TRACK) or tag (TAG)
 };
 
 // virtual bin v of a tile: the centre bins of all species first (most
@@ -282,7 +285,7 @@ __device__ inline void vbin(int v, int nsp, int &s, int &d)
     d = q < 4 ? q : q + 1;
 }
 
-template <int T, int BS, bool TRACK = false, bool BC = false>
+template <int T, int BS, bool TRACK = false, bool BC = false, bool TAG = false>
 __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp, const float *__restrict__ Ex,
                                                      const float *__restrict__ Ey, const float *__restrict__ Ez,
                                                      const float *__restrict__ Bx, const float *__restrict__ By,
@@ -374,7 +377,8 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
         r.ox = ldo(A.ox, off); r.oy = ldo(A.oy, off);
         r.u = ldo(A.u, off); r.v = ldo(A.v, off); r.w = ldo(A.w, off);
         r.s = sVs[v];
-        if (TRACK) { r.slot = (int)slot; r.id = ldo(A.id, off); }
+        if (TRACK) r.slot = (int)slot;
+        if (TRACK || TAG) r.id = ldo(A.id, off);
     };
 
     SplitRec nxt{};
@@ -511,15 +515,14 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                 const unsigned off = (sDstBase[d] + (unsigned)slot) * 4u;
                 sto(Bo.cell, off, ncell); sto(Bo.ox, off, ox); sto(Bo.oy, off, oy);
                 sto(Bo.u, off, un); sto(Bo.v, off, vn); sto(Bo.w, off, wn);
-                if (TRACK) {
-                    sto(Bo.id, off, cur.id);
-                    if (dst) dst[cur.slot] = (int)(sDstBase[d] + (unsigned)slot);
-                }
+                if (TRACK || TAG) sto(Bo.id, off, cur.id);
+                if (TRACK && dst) dst[cur.slot] = (int)(sDstBase[d] + (unsigned)slot);
             } else {
                 const int si = atomicAdd(sp.count, 1);
                 if (si < sp.cap) {
                     spill.cell[si] = ncell; spill.ox[si] = ox; spill.oy[si] = oy;
                     spill.u[si] = un; spill.v[si] = vn; spill.w[si] = wn; spill.sp[si] = cur.s;
+                    if (TAG) spill.id[si] = cur.id;
                 } else {
                     atomicOr(sp.err, (int)kErrCapacity);
                 }
@@ -545,27 +548,29 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
 
 }  // namespace
 
+// the kernel variant for the boundary faces (BC) and carried tags (TAG) of this context
+#define PICK_BC_TAG(K, bc, tag, ...)                                                                                \
+    do {                                                                                                           \
+        if (bc) { if (tag) K(true, true, __VA_ARGS__); else K(true, false, __VA_ARGS__); }                           \
+        else { if (tag) K(false, true, __VA_ARGS__); else K(false, false, __VA_ARGS__); }                            \
+    } while (0)
+
 void launch_push2d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, double *const E[3],
                        double *const B[3], double *const J[3], PartF64 A, PartF64 Bo, BinsP bins, PartF64 spill,
                        SpillP sp, hipStream_t s)
 {
     dim3 grid(tp.ntiles);
-#define PUSH_F64(TT, BSZ)                                                                                          \
-    do {                                                                                                           \
-        if (ph.pbc_on)                                                                                             \
-            hipLaunchKernelGGL((k_push2d_f64<TT, BSZ, true>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2],  \
-                               B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp);                         \
-        else                                                                                                       \
-            hipLaunchKernelGGL((k_push2d_f64<TT, BSZ, false>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], \
-                               B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp);                         \
-    } while (0)
+#define K_F64(BCV, TAGV, TT, BSZ)                                                                                  \
+    hipLaunchKernelGGL((k_push2d_f64<TT, BSZ, BCV, TAGV>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], B[0], \
+                       B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp)
+    const bool bc = ph.pbc_on != 0, tag = A.tag != nullptr;
     switch (T) {
-    case 8: PUSH_F64(8, 256); break;
-    case 16: PUSH_F64(16, 512); break;
-    case 32: PUSH_F64(32, 1024); break;
+    case 8: PICK_BC_TAG(K_F64, bc, tag, 8, 256); break;
+    case 16: PICK_BC_TAG(K_F64, bc, tag, 16, 512); break;
+    case 32: PICK_BC_TAG(K_F64, bc, tag, 32, 1024); break;
     default: break;
     }
-#undef PUSH_F64
+#undef K_F64
 }
 
 void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp, float *const E[3],
@@ -573,22 +578,17 @@ void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp
                          PartSplit spill, SpillP sp, hipStream_t s)
 {
     dim3 grid(tp.ntiles);
-#define PUSH_SPLIT(TT, BSZ)                                                                                         \
-    do {                                                                                                            \
-        if (ph.pbc_on)                                                                                              \
-            hipLaunchKernelGGL((k_push2d_split<TT, BSZ, false, true>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], \
-                               E[2], B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp, (int *)nullptr);     \
-        else                                                                                                        \
-            hipLaunchKernelGGL((k_push2d_split<TT, BSZ>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], B[0],  \
-                               B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp, (int *)nullptr);                \
-    } while (0)
+#define K_SPLIT(BCV, TAGV, TT, BSZ)                                                                                \
+    hipLaunchKernelGGL((k_push2d_split<TT, BSZ, false, BCV, TAGV>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1],   \
+                       E[2], B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp, (int *)nullptr)
+    const bool bc = ph.pbc_on != 0, tag = A.id != nullptr;
     switch (T) {
-    case 8: PUSH_SPLIT(8, 256); break;
-    case 16: PUSH_SPLIT(16, 512); break;
-    case 32: PUSH_SPLIT(32, 1024); break;
+    case 8: PICK_BC_TAG(K_SPLIT, bc, tag, 8, 256); break;
+    case 16: PICK_BC_TAG(K_SPLIT, bc, tag, 16, 512); break;
+    case 32: PICK_BC_TAG(K_SPLIT, bc, tag, 32, 1024); break;
     default: break;
     }
-#undef PUSH_SPLIT
+#undef K_SPLIT
 }
 
 void launch_push2d_split_track(int T, const GridP &g, const PhysP &ph, const TileP &tp, float *const E[3],
@@ -607,5 +607,6 @@ void launch_push2d_split_track(int T, const GridP &g, const PhysP &ph, const Til
     }
 #undef PUSH_SPLIT
 }
+#undef PICK_BC_TAG
 
 }  // namespace pictor

@@ -17,7 +17,7 @@ using namespace pushdev;
 // FP64 form, 3-D.  Gather window: nodes [i0-1, i0+T+1] (particles may sit one
 // cell into a ghost layer after the exchange); current window [i0-2, i0+T+2].
 // ---------------------------------------------------------------------------
-template <int T, int BS, bool BC>
+template <int T, int BS, bool BC, bool TAG = false>
 __global__ __launch_bounds__(BS) void k_push3d_f64(GridP g, PhysP ph, TileP tp, const double *__restrict__ Ex,
                                                    const double *__restrict__ Ey, const double *__restrict__ Ez,
                                                    const double *__restrict__ Bx, const double *__restrict__ By,
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(BS) void k_push3d_f64(GridP g, PhysP ph, TileP tp, 
         const int k = base + tid;
         const bool active = k < total;
         double x0 = 0, y0 = 0, z0 = 0, un = 0, vn = 0, wn = 0, q = 0, xn = 0, yn = 0, zn = 0;
-        int flv = 0, d = -1;
+        int flv = 0, d = -1, tg = 0;
         if (active) {
             int b = 0;
             while (k >= sStart[b + 1]) b++;
@@ -86,6 +86,7 @@ __global__ __launch_bounds__(BS) void k_push3d_f64(GridP g, PhysP ph, TileP tp, 
             x0 = A.x[src]; y0 = A.y[src]; z0 = A.z[src];
             un = A.u[src]; vn = A.v[src]; wn = A.w[src];
             q = A.q[src]; flv = A.flv[src];
+            if (TAG) tg = A.tag[src];
             xn = x0; yn = y0; zn = z0;
             const int ip0 = (int)x0, jp0 = (int)y0, kp0 = (int)z0;
             // node-window coordinates of the cell (origin i0-1)
@@ -234,12 +235,14 @@ __global__ __launch_bounds__(BS) void k_push3d_f64(GridP g, PhysP ph, TileP tp, 
                 Bo.x[o] = xn; Bo.y[o] = yn; Bo.z[o] = zn;
                 Bo.u[o] = un; Bo.v[o] = vn; Bo.w[o] = wn;
                 Bo.q[o] = q; Bo.flv[o] = flv;
+                if (TAG) Bo.tag[o] = tg;
             } else {
                 const int s = atomicAdd(sp.count, 1);
                 if (s < sp.cap) {
                     spill.x[s] = xn; spill.y[s] = yn; spill.z[s] = zn;
                     spill.u[s] = un; spill.v[s] = vn; spill.w[s] = wn;
                     spill.q[s] = q; spill.flv[s] = flv;
+                    if (TAG) spill.tag[s] = tg;
                 } else {
                     atomicOr(sp.err, (int)kErrCapacity);
                 }
@@ -264,9 +267,10 @@ __global__ __launch_bounds__(BS) void k_push3d_f64(GridP g, PhysP ph, TileP tp, 
 struct SplitRec3 {
     int cell, s;
     float ox, oy, oz, u, v, w;
+    int id;   // the caller's tag (TAG only)
 };
 
-template <int T, int BS, bool BC>
+template <int T, int BS, bool BC, bool TAG = false>
 __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp, const float *__restrict__ Ex,
                                                      const float *__restrict__ Ey, const float *__restrict__ Ez,
                                                      const float *__restrict__ Bx, const float *__restrict__ By,
@@ -347,6 +351,7 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
         r.ox = ldo(A.ox, off); r.oy = ldo(A.oy, off); r.oz = ldo(A.oz, off);
         r.u = ldo(A.u, off); r.v = ldo(A.v, off); r.w = ldo(A.w, off);
         r.s = sVs[v];
+        if (TAG) r.id = ldo(A.id, off);
     };
 
     SplitRec3 nxt{};
@@ -490,11 +495,13 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
                 const unsigned off = (sDstBase[d] + (unsigned)slot) * 4u;
                 sto(Bo.cell, off, ncell); sto(Bo.ox, off, ox); sto(Bo.oy, off, oy); sto(Bo.oz, off, oz);
                 sto(Bo.u, off, un); sto(Bo.v, off, vn); sto(Bo.w, off, wn);
+                if (TAG) sto(Bo.id, off, cur.id);
             } else {
                 const int si = atomicAdd(sp.count, 1);
                 if (si < sp.cap) {
                     spill.cell[si] = ncell; spill.ox[si] = ox; spill.oy[si] = oy; spill.oz[si] = oz;
                     spill.u[si] = un; spill.v[si] = vn; spill.w[si] = wn; spill.sp[si] = cur.s;
+                    if (TAG) spill.id[si] = cur.id;
                 } else {
                     atomicOr(sp.err, (int)kErrCapacity);
                 }
@@ -519,26 +526,28 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
 
 }  // namespace
 
+// the kernel variant for the boundary faces (BC) and carried tags (TAG) of this context
+#define PICK_BC_TAG(K, bc, tag, ...)                                                                                \
+    do {                                                                                                           \
+        if (bc) { if (tag) K(true, true, __VA_ARGS__); else K(true, false, __VA_ARGS__); }                           \
+        else { if (tag) K(false, true, __VA_ARGS__); else K(false, false, __VA_ARGS__); }                            \
+    } while (0)
+
 void launch_push3d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, double *const E[3],
                        double *const B[3], double *const J[3], PartF64 A, PartF64 Bo, BinsP bins, PartF64 spill,
                        SpillP sp, hipStream_t s)
 {
     dim3 grid(tp.ntiles);
-#define PUSH3_F64(TT, BSZ)                                                                                          \
-    do {                                                                                                           \
-        if (ph.pbc_on)                                                                                             \
-            hipLaunchKernelGGL((k_push3d_f64<TT, BSZ, true>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2],  \
-                               B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp);                         \
-        else                                                                                                       \
-            hipLaunchKernelGGL((k_push3d_f64<TT, BSZ, false>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], \
-                               B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp);                         \
-    } while (0)
+#define K3_F64(BCV, TAGV, TT, BSZ)                                                                                 \
+    hipLaunchKernelGGL((k_push3d_f64<TT, BSZ, BCV, TAGV>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], B[0], \
+                       B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp)
+    const bool bc = ph.pbc_on != 0, tag = A.tag != nullptr;
     switch (T) {
-    case 4: PUSH3_F64(4, 256); break;
-    case 8: PUSH3_F64(8, 512); break;
+    case 4: PICK_BC_TAG(K3_F64, bc, tag, 4, 256); break;
+    case 8: PICK_BC_TAG(K3_F64, bc, tag, 8, 1024); break;
     default: break;
     }
-#undef PUSH3_F64
+#undef K3_F64
 }
 
 void launch_push3d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp, float *const E[3],
@@ -546,21 +555,17 @@ void launch_push3d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp
                          PartSplit spill, SpillP sp, hipStream_t s)
 {
     dim3 grid(tp.ntiles);
-#define PUSH3_SPLIT(TT, BSZ)                                                                                         \
-    do {                                                                                                            \
-        if (ph.pbc_on)                                                                                              \
-            hipLaunchKernelGGL((k_push3d_split<TT, BSZ, true>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2], \
-                               B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp);                          \
-        else                                                                                                        \
-            hipLaunchKernelGGL((k_push3d_split<TT, BSZ, false>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1],      \
-                               E[2], B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp);                    \
-    } while (0)
+#define K3_SPLIT(BCV, TAGV, TT, BSZ)                                                                               \
+    hipLaunchKernelGGL((k_push3d_split<TT, BSZ, BCV, TAGV>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2],     \
+                       B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp)
+    const bool bc = ph.pbc_on != 0, tag = A.id != nullptr;
     switch (T) {
-    case 4: PUSH3_SPLIT(4, 256); break;
-    case 8: PUSH3_SPLIT(8, 512); break;
+    case 4: PICK_BC_TAG(K3_SPLIT, bc, tag, 4, 256); break;
+    case 8: PICK_BC_TAG(K3_SPLIT, bc, tag, 8, 512); break;
     default: break;
     }
-#undef PUSH3_SPLIT
+#undef K3_SPLIT
 }
+#undef PICK_BC_TAG
 
 }  // namespace pictor

@@ -84,10 +84,12 @@ struct TileP {
     int xcd_remap;  // 1 when ntiles % 8 == 0: give each XCD a contiguous block of tiles
 };
 
-// FP64 form: the reference SoA (src/vars.F90:43-45)
+// FP64 form: the reference SoA (src/vars.F90:43-45); tag: the caller's per-particle
+// tag (tagp), carried only when the particles were uploaded with tags
 struct PartF64 {
-    double *q, *x, *y, *z, *u, *v, *w;
-    int *flv;
+    double *q = nullptr, *x = nullptr, *y = nullptr, *z = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
+    int *flv = nullptr;
+    int *tag = nullptr;
 };
 
 // FP32 split form: packed physical cell coordinates + FP32 in-cell offsets.
@@ -114,10 +116,13 @@ __host__ __device__ inline void cell_unpack(int ndim, int c, int &i, int &j, int
 constexpr float kZMid = 0.5f;
 
 struct PartSplit {
-    int *cell;
-    float *ox, *oy, *oz, *u, *v, *w;   // oz: 3-D only (null in 2-D)
-    int *sp;   // species id, used only by the spill / raw buffers
-    int *id;   // caller's index of the particle, carried only by the adjoint engine's buffers
+    int *cell = nullptr;
+    float *ox = nullptr, *oy = nullptr, *oz = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;   // oz: 3-D only
+    int *sp = nullptr;   // species id, used only by the spill / raw buffers
+    // a 32-bit value carried with the particle: the caller's tag (tagp) in a context
+    // whose particles were uploaded with tags, the caller's index in the adjoint
+    // engine's buffers; null otherwise
+    int *id = nullptr;
 };
 
 __host__ __device__ inline long long fidx(const GridP &g, int i, int j, int k)

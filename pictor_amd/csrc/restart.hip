@@ -48,10 +48,11 @@ struct StateHeader {
     uint64_t win_seed;
     // the reference r-theta cylinder: push scheme (0 Boris, 1 Strang)
     int32_t cyl_scheme;
-    int32_t reserved[7];
+    int32_t has_tags;      // version 3: the particles' tags follow their flavours
+    int32_t reserved[6];
 };
 constexpr char kMagic[8] = {'P', 'I', 'C', 'T', 'O', 'R', 'S', 'T'};
-constexpr int32_t kVersion = 2;   // 1: no faces section
+constexpr int32_t kVersion = 3;   // 1: no faces section; 2: no tags
 
 struct File {
     FILE *f = nullptr;
@@ -95,10 +96,10 @@ int pictor_save_state(pictor_ctx *c, const char *path, int64_t t)
     int rc = pictor_particle_count(c, &np);
     if (rc) return rc;
     std::vector<double> q(np), x(np), y(np), z(np), u(np), v(np), w(np);
-    std::vector<int32_t> flv(np);
+    std::vector<int32_t> flv(np), tag(c->tagged ? np : 0);
     int64_t got = 0;
-    if ((rc = pictor_download_particles(c, np, q.data(), x.data(), y.data(), z.data(), u.data(), v.data(), w.data(),
-                                        flv.data(), nullptr, &got)))
+    if ((rc = pictor_download_particles_tagged(c, np, q.data(), x.data(), y.data(), z.data(), u.data(), v.data(),
+                                               w.data(), flv.data(), nullptr, c->tagged ? tag.data() : nullptr, &got)))
         return rc;
     if (got != np) return fail(PICTOR_ERR_ARG, "restart: particle count changed during the download");
 
@@ -115,6 +116,7 @@ int pictor_save_state(pictor_ctx *c, const char *path, int64_t t)
     h.form = cf.form; h.tile = cf.tile; h.fdtd = cf.fdtd; h.curr_filter = cf.curr_filter;
     h.c = cf.c; h.c_ompe = cf.c_ompe; h.g0 = cf.g0; h.mi_me = cf.mi_me; h.dr = cf.dr;
     h.t = t; h.np = np; h.ncell = c->g.ncell;
+    h.has_tags = c->tagged ? 1 : 0;
     if (c->cylm) {
         const CylmState &st = c->cm;
         h.win_on = st.win_on; h.win_nrp = st.win_nrp; h.win_ppc = st.win_ppc; h.sort_period = st.sort_period;
@@ -140,6 +142,7 @@ int pictor_save_state(pictor_ctx *c, const char *path, int64_t t)
         for (const auto *col : cols)
             if ((rc = write_all(f.f, col->data(), (size_t)np * sizeof(double), "particles"))) return rc;
         if ((rc = write_all(f.f, flv.data(), (size_t)np * sizeof(int32_t), "particle flavours"))) return rc;
+        if (h.has_tags && (rc = write_all(f.f, tag.data(), (size_t)np * sizeof(int32_t), "particle tags"))) return rc;
         BcState bs;
         bc_get(c, bs);
         if ((rc = write_all(f.f, &bs, sizeof(bs), "the boundary faces"))) return rc;
@@ -193,6 +196,9 @@ int pictor_load_state(pictor_ctx *c, const char *path, int64_t *t)
     for (auto *col : cols)
         if ((rc = read_all(f.f, col->data(), (size_t)np * sizeof(double), "particles"))) return rc;
     if ((rc = read_all(f.f, flv.data(), (size_t)np * sizeof(int32_t), "particle flavours"))) return rc;
+    const bool has_tags = h.version >= 3 && h.has_tags;
+    std::vector<int32_t> tag(has_tags ? np : 0);
+    if (has_tags && (rc = read_all(f.f, tag.data(), (size_t)np * sizeof(int32_t), "particle tags"))) return rc;
     BcState bs;
     bc_get(c, bs);   // version 1: the context's own faces
     std::vector<std::vector<double>> psi;
@@ -210,9 +216,14 @@ int pictor_load_state(pictor_ctx *c, const char *path, int64_t *t)
     // faces first (the upload's tile assignment does not depend on them), then the
     // particles: a rejected upload leaves the fields as they were
     if (!c->cyl && (rc = bc_set(c, bs))) return rc;
-    if ((rc = pictor_upload_particles(c, np, q.data(), x.data(), y.data(), z.data(), u.data(), v.data(), w.data(),
-                                      flv.data())))
+    if (has_tags) {
+        if ((rc = pictor_upload_particles_tagged(c, np, q.data(), x.data(), y.data(), z.data(), u.data(), v.data(),
+                                                 w.data(), flv.data(), tag.data())))
+            return rc;
+    } else if ((rc = pictor_upload_particles(c, np, q.data(), x.data(), y.data(), z.data(), u.data(), v.data(),
+                                             w.data(), flv.data()))) {
         return rc;
+    }
     for (size_t i = 0; i < arr.size(); i++)
         HIPTRY(hipMemcpyAsync(arr[i], fld[i].data(), n * es, hipMemcpyHostToDevice, c->stream));
     for (size_t q = 0; q < psi.size(); q++)

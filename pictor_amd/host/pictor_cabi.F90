@@ -82,6 +82,25 @@ module pictor_cabi
          type(c_ptr), value :: tile
          integer(c_int64_t), intent(out) :: n_out
       end function
+      ! the reference's tagp carried with every particle (Cartesian geometry)
+      integer(c_int) function pictor_upload_particles_tagged(ctx, n, q, x, y, z, u, v, w, flv, tag) &
+            bind(C, name="pictor_upload_particles_tagged")
+         import :: c_int, c_ptr, c_double, c_int64_t, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: n
+         real(c_double), intent(in) :: q(*), x(*), y(*), z(*), u(*), v(*), w(*)
+         integer(c_int32_t), intent(in) :: flv(*), tag(*)
+      end function
+      integer(c_int) function pictor_download_particles_tagged(ctx, cap, q, x, y, z, u, v, w, flv, tile, tag, &
+            n_out) bind(C, name="pictor_download_particles_tagged")
+         import :: c_int, c_ptr, c_double, c_int64_t, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: cap
+         real(c_double) :: q(*), x(*), y(*), z(*), u(*), v(*), w(*)
+         integer(c_int32_t) :: flv(*), tag(*)
+         type(c_ptr), value :: tile
+         integer(c_int64_t), intent(out) :: n_out
+      end function
       integer(c_int) function pictor_update_b_half(ctx) bind(C, name="pictor_update_b_half")
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx

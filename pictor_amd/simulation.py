@@ -115,25 +115,40 @@ class Simulation:
                                                                               "Jx", "Jy", "Jz")]))
         return out
 
-    def upload_particles(self, q, x, y, z, u, v, w, flv):
+    def upload_particles(self, q, x, y, z, u, v, w, flv, tag=None):
+        """Particles in the reference SoA; tag (optional, int32 per particle): the
+        reference's tagp, carried with each particle (download_particles(with_tag=True))."""
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (q, x, y, z, u, v, w)]
         f = np.ascontiguousarray(flv, dtype=np.int32)
         n = len(f)
-        check(lib().pictor_upload_particles(self._ctx, n, *[_p(a) for a in arrs], _p(f, _ip)))
+        if tag is None:
+            check(lib().pictor_upload_particles(self._ctx, n, *[_p(a) for a in arrs], _p(f, _ip)))
+        else:
+            t = np.ascontiguousarray(tag, dtype=np.int32)
+            if len(t) != n:
+                raise ValueError("one tag per particle")
+            check(lib().pictor_upload_particles_tagged(self._ctx, n, *[_p(a) for a in arrs], _p(f, _ip), _p(t, _ip)))
 
     def particle_count(self):
         n = ctypes.c_int64()
         check(lib().pictor_particle_count(self._ctx, ctypes.byref(n)))
         return n.value
 
-    def download_particles(self, with_tile=False):
+    def download_particles(self, with_tile=False, with_tag=False):
         n = self.particle_count()
         out = {k: np.zeros(n) for k in ("q", "x", "y", "z", "u", "v", "w")}
         out["flv"] = np.zeros(n, dtype=np.int32)
         tile = np.zeros(n, dtype=np.int32) if with_tile else None
         nout = ctypes.c_int64()
-        check(lib().pictor_download_particles(self._ctx, n, *[_p(out[k]) for k in ("q", "x", "y", "z", "u", "v", "w")],
-                                              _p(out["flv"], _ip), _p(tile, _ip), ctypes.byref(nout)))
+        cols = [_p(out[k]) for k in ("q", "x", "y", "z", "u", "v", "w")]
+        if with_tag:
+            tag = np.zeros(n, dtype=np.int32)
+            check(lib().pictor_download_particles_tagged(self._ctx, n, *cols, _p(out["flv"], _ip), _p(tile, _ip),
+                                                         _p(tag, _ip), ctypes.byref(nout)))
+            out["tag"] = tag
+        else:
+            check(lib().pictor_download_particles(self._ctx, n, *cols, _p(out["flv"], _ip), _p(tile, _ip),
+                                                  ctypes.byref(nout)))
         if with_tile:
             out["tile"] = tile
         return out
