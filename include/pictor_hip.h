@@ -102,7 +102,10 @@ int pictor_upload_fields(pictor_ctx *ctx, const double *ex, const double *ey, co
 int pictor_download_fields(pictor_ctx *ctx, double *ex, double *ey, double *ez,
                            double *bx, double *by, double *bz,
                            double *jx, double *jy, double *jz);
-/* Holes (flv = 0) are dropped.  The FP32 split form needs one charge per flavour. */
+/* Holes (flv = 0) are dropped.  The FP32 split form keeps the charge implicit when every
+ * particle carries its flavour's charge; otherwise (weighted particles, q = wtp * FlvrCharge,
+ * init_prtl.F90:161) each particle record gains an FP32 charge weight q / FlvrCharge (4 B
+ * read and written per particle-step more); a charge of the wrong sign is refused. */
 int pictor_upload_particles(pictor_ctx *ctx, int64_t n, const double *q, const double *x,
                             const double *y, const double *z, const double *u, const double *v,
                             const double *w, const int32_t *flv);
@@ -118,6 +121,11 @@ int pictor_download_particles(pictor_ctx *ctx, int64_t cap, double *q, double *x
  * A tagged upload makes the context carry tags until an untagged upload; downloading tags
  * from an untagged context is an error.  The tag costs 4 B read and 4 B written per
  * particle-step. */
+/* SetFilteredEfldGPU, src/maxwell/fields_gpu.F90:70-90 (the E field the reference's test
+ * particles feel): E through `passes` rounds (nMoverEMfilter) of the 1-2-1 filter of
+ * smoothen_current, then the periodic halo copy, written to fx, fy, fz (mx*my*mz each,
+ * the reference layout).  Cartesian geometry without boundary faces. */
+int pictor_filtered_efield(pictor_ctx *ctx, int32_t passes, double *fx, double *fy, double *fz);
 int pictor_upload_particles_tagged(pictor_ctx *ctx, int64_t n, const double *q, const double *x,
                                    const double *y, const double *z, const double *u, const double *v,
                                    const double *w, const int32_t *flv, const int32_t *tag);

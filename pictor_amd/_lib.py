@@ -43,7 +43,7 @@ EXPORTS = (
     "pictor_set_boundary_speed", "pictor_pml", "pictor_enforce_bc_final", "pictor_save_energy",
     "pictor_save_fields", "pictor_energy_noalloc", "pictor_adjoint_forward_device",
     "pictor_adjoint_backward_device", "pictor_probe_stream", "pictor_upload_particles_tagged",
-    "pictor_download_particles_tagged",
+    "pictor_download_particles_tagged", "pictor_filtered_efield",
 )
 
 
@@ -89,6 +89,7 @@ def lib():
     L.pictor_upload_particles.argtypes = [c_vp, i64] + [c_dp] * 7 + [c_ip]
     L.pictor_particle_count.argtypes = [c_vp, ctypes.POINTER(i64)]
     L.pictor_download_particles.argtypes = [c_vp, i64] + [c_dp] * 7 + [c_ip, c_ip, ctypes.POINTER(i64)]
+    L.pictor_filtered_efield.argtypes = [c_vp, i32, c_dp, c_dp, c_dp]
     L.pictor_upload_particles_tagged.argtypes = [c_vp, i64] + [c_dp] * 7 + [c_ip, c_ip]
     L.pictor_download_particles_tagged.argtypes = [c_vp, i64] + [c_dp] * 7 + [c_ip, c_ip, c_ip, ctypes.POINTER(i64)]
     for name in ("pictor_update_b_half", "pictor_move_deposit", "pictor_exchange_particles", "pictor_update_e",

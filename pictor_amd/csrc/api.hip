@@ -40,7 +40,7 @@ void free_f64(PartF64 &p)
 void free_split(PartSplit &p)
 {
     dfree(p.cell); dfree(p.ox); dfree(p.oy); dfree(p.oz);
-    dfree(p.u); dfree(p.v); dfree(p.w); dfree(p.sp); dfree(p.id);
+    dfree(p.u); dfree(p.v); dfree(p.w); dfree(p.sp); dfree(p.id); dfree(p.wq);
     p = PartSplit{};
 }
 
@@ -59,10 +59,11 @@ hipError_t alloc_f64(PartF64 &p, long long n, bool tagged = false)
 }
 
 // oz only in 3-D: the 2-D split record has no z (kZMid)
-hipError_t alloc_split(PartSplit &p, long long n, bool with_sp, int ndim, bool tagged = false)
+hipError_t alloc_split(PartSplit &p, long long n, bool with_sp, int ndim, bool tagged = false, bool weighted = false)
 {
     hipError_t e;
     if (tagged && (e = dalloc(&p.id, n)) != hipSuccess) return e;
+    if (weighted && (e = dalloc(&p.wq, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.cell, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.ox, n)) != hipSuccess) return e;
     if ((e = dalloc(&p.oy, n)) != hipSuccess) return e;
@@ -87,7 +88,7 @@ int ensure_buffer(pictor_ctx *c, int which, long long need)
         HIPTRY(alloc_f64(c->pf[which], n, c->tagged));
     } else {
         free_split(c->ps[which]);
-        HIPTRY(alloc_split(c->ps[which], n, false, c->g.ndim, c->tagged));
+        HIPTRY(alloc_split(c->ps[which], n, false, c->g.ndim, c->tagged, c->weighted));
     }
     c->buf_cap[which] = n;
     return PICTOR_OK;
@@ -103,7 +104,7 @@ int ensure_spill(pictor_ctx *c, int need)
     } else {
         free_split(c->spill_s);
         c->spill_cap = 0;
-        HIPTRY(alloc_split(c->spill_s, need, true, c->g.ndim, c->tagged));
+        HIPTRY(alloc_split(c->spill_s, need, true, c->g.ndim, c->tagged, c->weighted));
     }
     dfree(c->rl_grp);
     c->rl_grp = nullptr;
@@ -790,6 +791,43 @@ int pictor_download_fields(pictor_ctx *c, double *ex, double *ey, double *ez, do
     return PICTOR_OK;
 }
 
+// SetFilteredEfldGPU (fields_gpu.F90:70-90): E (the FP64 value, hi + lo in the split
+// form) through `passes` rounds of the 1-2-1 filter of smoothen_current (the
+// reference's nMoverEMfilter passes of MovingAverageFilterGPU over x, y (, z)), then
+// the periodic halo copy (the edge exchanges); the result goes to the host arrays
+int pictor_filtered_efield(pictor_ctx *c, int32_t passes, double *fx, double *fy, double *fz)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->cyl) return fail(PICTOR_ERR_ARG, "filtered E: Cartesian geometry only");
+    if (c->bc_any) return fail(PICTOR_ERR_ARG, "filtered E: the filter reads periodic images (no boundary faces)");
+    if (passes < 0) return fail(PICTOR_ERR_ARG, "filtered E: negative number of passes");
+    if (!fx || !fy || !fz) return fail(PICTOR_ERR_ARG, "null output array");
+    const size_t n = (size_t)c->g.ncell;
+    double *F[3] = {}, *T[3] = {};
+    auto release = [&]() { for (int i = 0; i < 3; i++) { dfree(F[i]); dfree(T[i]); } };
+    for (int i = 0; i < 3; i++)
+        if (dalloc(&F[i], n) != hipSuccess || dalloc(&T[i], n) != hipSuccess) {
+            release();
+            return fail(PICTOR_ERR_HIP, "filtered E: device allocation failed");
+        }
+    for (int i = 0; i < 3; i++) {
+        if (c->form == PICTOR_FORM_FP64)
+            HIPTRY(hipMemcpyAsync(F[i], c->E[i], n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        else
+            launch_pair_to_f64((const float *)c->E[i], c->Elo[i], F[i], (long long)n, c->stream);
+    }
+    launch_smoothen_current(c->g, passes, F, T, c->stream);
+    launch_ghost_copy<double>(c->g, F, c->stream);
+    double *dst[3] = {fx, fy, fz};
+    for (int i = 0; i < 3; i++)
+        HIPTRY(hipMemcpyAsync(dst[i], F[i], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    release();
+    if (e != hipSuccess) return fail(PICTOR_ERR_HIP, std::string("filtered E: ") + hipGetErrorString(e));
+    return PICTOR_OK;
+}
+
 // the upload of both entry points; tag (may be null): the caller's per-particle tags
 static int upload_particles_impl(pictor_ctx *c, int64_t n, const double *q, const double *x, const double *y,
                                  const double *z, const double *u, const double *v, const double *w,
@@ -806,12 +844,20 @@ static int upload_particles_impl(pictor_ctx *c, int64_t n, const double *q, cons
         return c->cylm ? cylm_upload_particles(c, n, q, x, y, z, u, v, w, flv) : cyl_upload_particles(c, n, q, x, y, z, u, v, w, flv);
     }
     const bool tagged = tag != nullptr;
-    if (tagged != c->tagged) {
-        // the particle buffers change their record: the next allocation brings (or drops) the tags
+    // split form: a charge weight per particle when some particle's charge is not its
+    // flavour's (the FP64 record keeps q itself)
+    bool weighted = false;
+    if (c->form == PICTOR_FORM_FP32_SPLIT)
+        for (int64_t i = 0; i < n && !weighted; i++)
+            weighted = flv[i] >= 1 && flv[i] <= c->ph.nflvr && q[i] != c->ph.flvq[flv[i] - 1];
+    if (tagged != c->tagged || weighted != c->weighted) {
+        // the particle buffers change their record: the next allocation brings (or drops)
+        // the tags and weights
         for (int i = 0; i < 2; i++) { free_f64(c->pf[i]); free_split(c->ps[i]); c->buf_cap[i] = 0; }
         free_f64(c->spill_f); free_split(c->spill_s);
         c->spill_cap = 0;
         c->tagged = tagged;
+        c->weighted = weighted;
     }
     const TileP &tp = c->tp;
     const size_t ng = (size_t)tp.ntiles * tp.nsp;
@@ -831,8 +877,8 @@ static int upload_particles_impl(pictor_ctx *c, int64_t n, const double *q, cons
     if (n > 0) HIPTRY(hipMemcpyAsync(raw.flv, flv, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPTRY(hipMemsetAsync(d_count, 0, ng * sizeof(int), c->stream));
     HIPTRY(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
-    launch_raw_tiles_f64(c->g, c->ph, tp, raw, n, c->form == PICTOR_FORM_FP32_SPLIT ? 1 : 0, d_tile, d_count,
-                         c->d_err, c->stream);
+    launch_raw_tiles_f64(c->g, c->ph, tp, raw, n, c->form == PICTOR_FORM_FP32_SPLIT ? (weighted ? 2 : 1) : 0, d_tile,
+                         d_count, c->d_err, c->stream);
     HIPTRY(hipGetLastError());
     std::vector<int> cnt(ng);
     HIPTRY(hipMemcpyAsync(cnt.data(), d_count, ng * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -841,7 +887,7 @@ static int upload_particles_impl(pictor_ctx *c, int64_t n, const double *q, cons
     if (c->h_flags[1]) {
         hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream);
         free_f64(raw); dfree(d_tile); dfree(d_count); dfree(d_fill); dfree(d_off);
-        return fail(PICTOR_ERR_ARG, "particle outside the domain (x,y in [2, m-1)), invalid flavour, or (split form) a charge differing from its flavour's");
+        return fail(PICTOR_ERR_ARG, "particle outside the domain (x,y in [2, m-1)), invalid flavour, or (split form) a charge of the wrong sign for its flavour");
     }
     std::vector<long long> counts(ng);
     long long np = 0;
@@ -934,7 +980,7 @@ static int download_particles_impl(pictor_ctx *c, int64_t cap, double *q, double
     if (c->form == PICTOR_FORM_FP64) {
         launch_compact_f64(tp, c->pf[c->cur], c->d_base, c->d_cnt[c->cur], d_off, nullptr, st, c->stream);
     } else {
-        HIPTRY(alloc_split(ss, n, true, c->g.ndim, c->tagged));
+        HIPTRY(alloc_split(ss, n, true, c->g.ndim, c->tagged, c->weighted));
         launch_compact_split(tp, c->ps[c->cur], c->d_base, c->d_cnt[c->cur], d_off, nullptr, ss, c->stream);
         launch_split_to_f64(c->g, c->ph, ss, n, st, c->stream);
     }

@@ -131,6 +131,9 @@ struct pictor_ctx {
     // the particles carry the caller's tags (uploaded with pictor_upload_particles_tagged):
     // a tag array beside every particle buffer (PartF64::tag, PartSplit::id)
     bool tagged = false;
+    // split form: the particles do not all carry their flavour's charge, so every particle
+    // record has a charge weight (PartSplit::wq); set by the upload
+    bool weighted = false;
     long long n_relayout = 0;   // re-layouts after a spill (bins overflowed) since init
     long long n_spilled = 0;    // particles those re-layouts took from the spill buffer
 

@@ -67,7 +67,10 @@ __global__ void k_raw_tiles(GridP g, PhysP ph, TileP tp, PartF64 raw, long long 
     ok = ok && x >= 2.0 && x < (double)(g.mx - 1) && y >= 2.0 && y < (double)(g.my - 1);
     if (g.ndim == 3) ok = ok && z >= 2.0 && z < (double)(g.mz - 1);
     else ok = ok && z >= 0.0 && z < 3.0;
-    if (split && ok) ok = (raw.q[i] == ph.flvq[flv - 1]);
+    // split = 1: every charge is its flavour's; split = 2 (charge weights): a nonzero
+    // multiple of it with the same sign
+    if (split == 1 && ok) ok = (raw.q[i] == ph.flvq[flv - 1]);
+    if (split == 2 && ok) ok = (raw.q[i] / ph.flvq[flv - 1] > 0.0);
     if (!ok) { atomicOr(err, 1); tile_out[i] = -1; return; }
     int grp;
     if (split) {
@@ -99,7 +102,7 @@ __global__ void k_raw_scatter_f64(TileP tp, PartF64 raw, long long n, const int 
     if (out.tag && raw.tag) out.tag[o] = raw.tag[i];
 }
 
-__global__ void k_raw_scatter_split(GridP g, TileP tp, PartF64 raw, long long n, const int *tile_of,
+__global__ void k_raw_scatter_split(GridP g, PhysP ph, TileP tp, PartF64 raw, long long n, const int *tile_of,
                                     const long long *cbase, int *fill, const int *gtotal, PartSplit out)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -121,6 +124,7 @@ __global__ void k_raw_scatter_split(GridP g, TileP tp, PartF64 raw, long long n,
     out.u[o] = (float)raw.u[i]; out.v[o] = (float)raw.v[i]; out.w[o] = (float)raw.w[i];
     // the carried value: the caller's tag, or (adjoint engine) the caller's index
     if (out.id) out.id[o] = raw.tag ? raw.tag[i] : (int)i;
+    if (out.wq) out.wq[o] = (float)(raw.q[i] / ph.flvq[raw.flv[i] - 1]);
     (void)s;
 }
 
@@ -189,6 +193,7 @@ __global__ void k_compact_split(TileP tp, PartSplit src, const long long *base, 
         dst.u[b] = src.u[a]; dst.v[b] = src.v[a]; dst.w[b] = src.w[a];
         if (dst.sp) dst.sp[b] = s;
         if (dst.id && src.id) dst.id[b] = src.id[a];
+        if (dst.wq && src.wq) dst.wq[b] = src.wq[a];
     });
 }
 
@@ -215,6 +220,7 @@ __global__ void k_spill_scatter_split(PartSplit sp, int n, const int *grp_of, co
     if (dst.oz) dst.oz[o] = sp.oz[i];
     dst.u[o] = sp.u[i]; dst.v[o] = sp.v[i]; dst.w[o] = sp.w[i];
     if (dst.id && sp.id) dst.id[o] = sp.id[i];
+    if (dst.wq && sp.wq) dst.wq[o] = sp.wq[i];
 }
 
 // split -> reference FP64 record (x = 3 + cell + offset; 2-D z = 1 + kZMid, z is not stored)
@@ -229,7 +235,7 @@ __global__ void k_split_to_f64(GridP g, PhysP ph, PartSplit src, long long n, Pa
     dst.y[i] = (double)(3 + cy) + (double)src.oy[i];
     dst.z[i] = (g.ndim == 3) ? (double)(3 + cz) + (double)src.oz[i] : 1.0 + (double)kZMid;
     dst.u[i] = src.u[i]; dst.v[i] = src.v[i]; dst.w[i] = src.w[i];
-    dst.q[i] = ph.flvq[s];
+    dst.q[i] = src.wq ? ph.flvq[s] * (double)src.wq[i] : ph.flvq[s];
     dst.flv[i] = s + 1;
     if (dst.tag && src.id) dst.tag[i] = src.id[i];
 }
@@ -337,7 +343,8 @@ __global__ __launch_bounds__(256) void k_particle_diag_split(GridP g, PhysP ph, 
                            y = (double)(3 + cy) + (double)p.oy[o];
                            z = (g.ndim == 3) ? (double)(3 + cz) + (double)p.oz[o] : 1.0 + (double)kZMid;
                            u = p.u[o]; v = p.v[o]; w = p.w[o];
-                           q = ph.flvq[s]; flv = s + 1;
+                           q = p.wq ? ph.flvq[s] * (double)p.wq[o] : ph.flvq[s];
+                           flv = s + 1;
                        });
 }
 
@@ -361,13 +368,13 @@ void launch_raw_scatter_f64(const GridP &, const TileP &tp, PartF64 raw, long lo
                        out);
 }
 
-void launch_raw_scatter_split(const GridP &g, const PhysP &, const TileP &tp, PartF64 raw, long long n,
+void launch_raw_scatter_split(const GridP &g, const PhysP &ph, const TileP &tp, PartF64 raw, long long n,
                               const int *tile_of, const long long *base, int *fill, const int *gtotal, PartSplit out,
                               hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_raw_scatter_split, dim3(nblk(n, 256)), dim3(256), 0, s, g, tp, raw, n, tile_of, base, fill,
-                       gtotal, out);
+    hipLaunchKernelGGL(k_raw_scatter_split, dim3(nblk(n, 256)), dim3(256), 0, s, g, ph, tp, raw, n, tile_of, base,
+                       fill, gtotal, out);
 }
 
 void launch_spill_tiles_f64(const GridP &g, const TileP &tp, PartF64 spill, int n, int *tile_out, int *counts,

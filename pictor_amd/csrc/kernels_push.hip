@@ -272,6 +272,7 @@ struct SplitRec {
     int cell, s;
     float ox, oy, u, v, w;
     int slot, id;   // input slot (TRACK) and the carried value: particle index (TRACK) or tag (TAG)
+    float wq;       // charge weight (WQ)
 };
 
 // virtual bin v of a tile: the centre bins of all species first (most
@@ -285,7 +286,7 @@ __device__ inline void vbin(int v, int nsp, int &s, int &d)
     d = q < 4 ? q : q + 1;
 }
 
-template <int T, int BS, bool TRACK = false, bool BC = false, bool TAG = false>
+template <int T, int BS, bool TRACK = false, bool BC = false, bool TAG = false, bool WQ = false>
 __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp, const float *__restrict__ Ex,
                                                      const float *__restrict__ Ey, const float *__restrict__ Ez,
                                                      const float *__restrict__ Bx, const float *__restrict__ By,
@@ -379,6 +380,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
         r.s = sVs[v];
         if (TRACK) r.slot = (int)slot;
         if (TRACK || TAG) r.id = ldo(A.id, off);
+        if (WQ) r.wq = ldo(A.wq, off);
     };
 
     SplitRec nxt{};
@@ -398,7 +400,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                 d = cur.s * 9 + 4;
             } else {
                 const float x0 = cur.ox, y0 = cur.oy;
-                const float qm = sQm[cur.s], qs = sQs[cur.s];
+                const float qm = sQm[cur.s], qs = WQ ? sQs[cur.s] * cur.wq : sQs[cur.s];
                 const float w1 = (1.0f - x0) * (1.0f - y0), w2 = x0 * (1.0f - y0);
                 const float w3 = (1.0f - x0) * y0, w4 = x0 * y0;
                 const int e0 = jj0 * W + ii0;
@@ -516,6 +518,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                 sto(Bo.cell, off, ncell); sto(Bo.ox, off, ox); sto(Bo.oy, off, oy);
                 sto(Bo.u, off, un); sto(Bo.v, off, vn); sto(Bo.w, off, wn);
                 if (TRACK || TAG) sto(Bo.id, off, cur.id);
+                if (WQ) sto(Bo.wq, off, cur.wq);
                 if (TRACK && dst) dst[cur.slot] = (int)(sDstBase[d] + (unsigned)slot);
             } else {
                 const int si = atomicAdd(sp.count, 1);
@@ -523,6 +526,7 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
                     spill.cell[si] = ncell; spill.ox[si] = ox; spill.oy[si] = oy;
                     spill.u[si] = un; spill.v[si] = vn; spill.w[si] = wn; spill.sp[si] = cur.s;
                     if (TAG) spill.id[si] = cur.id;
+                    if (WQ) spill.wq[si] = cur.wq;
                 } else {
                     atomicOr(sp.err, (int)kErrCapacity);
                 }
@@ -555,6 +559,18 @@ __global__ __launch_bounds__(BS) void k_push2d_split(GridP g, PhysP ph, TileP tp
         else { if (tag) K(false, true, __VA_ARGS__); else K(false, false, __VA_ARGS__); }                            \
     } while (0)
 
+// the split form's variant for the faces (BC), carried tags (TAG) and charge weights (WQ)
+#define PICK_BC_TAG_WQ(K, bc, tag, wq, ...)                                                                         \
+    do {                                                                                                           \
+        if (wq) {                                                                                                  \
+            if (bc) { if (tag) K(true, true, true, __VA_ARGS__); else K(true, false, true, __VA_ARGS__); }           \
+            else { if (tag) K(false, true, true, __VA_ARGS__); else K(false, false, true, __VA_ARGS__); }            \
+        } else {                                                                                                   \
+            if (bc) { if (tag) K(true, true, false, __VA_ARGS__); else K(true, false, false, __VA_ARGS__); }         \
+            else { if (tag) K(false, true, false, __VA_ARGS__); else K(false, false, false, __VA_ARGS__); }          \
+        }                                                                                                          \
+    } while (0)
+
 void launch_push2d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, double *const E[3],
                        double *const B[3], double *const J[3], PartF64 A, PartF64 Bo, BinsP bins, PartF64 spill,
                        SpillP sp, hipStream_t s)
@@ -578,14 +594,14 @@ void launch_push2d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp
                          PartSplit spill, SpillP sp, hipStream_t s)
 {
     dim3 grid(tp.ntiles);
-#define K_SPLIT(BCV, TAGV, TT, BSZ)                                                                                \
-    hipLaunchKernelGGL((k_push2d_split<TT, BSZ, false, BCV, TAGV>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1],   \
-                       E[2], B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp, (int *)nullptr)
-    const bool bc = ph.pbc_on != 0, tag = A.id != nullptr;
+#define K_SPLIT(BCV, TAGV, WQV, TT, BSZ)                                                                           \
+    hipLaunchKernelGGL((k_push2d_split<TT, BSZ, false, BCV, TAGV, WQV>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0],    \
+                       E[1], E[2], B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp, (int *)nullptr)
+    const bool bc = ph.pbc_on != 0, tag = A.id != nullptr, wq = A.wq != nullptr;
     switch (T) {
-    case 8: PICK_BC_TAG(K_SPLIT, bc, tag, 8, 256); break;
-    case 16: PICK_BC_TAG(K_SPLIT, bc, tag, 16, 512); break;
-    case 32: PICK_BC_TAG(K_SPLIT, bc, tag, 32, 1024); break;
+    case 8: PICK_BC_TAG_WQ(K_SPLIT, bc, tag, wq, 8, 256); break;
+    case 16: PICK_BC_TAG_WQ(K_SPLIT, bc, tag, wq, 16, 512); break;
+    case 32: PICK_BC_TAG_WQ(K_SPLIT, bc, tag, wq, 32, 1024); break;
     default: break;
     }
 #undef K_SPLIT
@@ -608,5 +624,6 @@ void launch_push2d_split_track(int T, const GridP &g, const PhysP &ph, const Til
 #undef PUSH_SPLIT
 }
 #undef PICK_BC_TAG
+#undef PICK_BC_TAG_WQ
 
 }  // namespace pictor

@@ -267,10 +267,11 @@ __global__ __launch_bounds__(BS) void k_push3d_f64(GridP g, PhysP ph, TileP tp, 
 struct SplitRec3 {
     int cell, s;
     float ox, oy, oz, u, v, w;
-    int id;   // the caller's tag (TAG only)
+    int id;     // the caller's tag (TAG only)
+    float wq;   // charge weight (WQ only)
 };
 
-template <int T, int BS, bool BC, bool TAG = false>
+template <int T, int BS, bool BC, bool TAG = false, bool WQ = false>
 __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp, const float *__restrict__ Ex,
                                                      const float *__restrict__ Ey, const float *__restrict__ Ez,
                                                      const float *__restrict__ Bx, const float *__restrict__ By,
@@ -352,6 +353,7 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
         r.u = ldo(A.u, off); r.v = ldo(A.v, off); r.w = ldo(A.w, off);
         r.s = sVs[v];
         if (TAG) r.id = ldo(A.id, off);
+        if (WQ) r.wq = ldo(A.wq, off);
     };
 
     SplitRec3 nxt{};
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
                 d = cur.s * 27 + 13;
             } else {
                 const float x0 = cur.ox, y0 = cur.oy, z0 = cur.oz;
-                const float qm = sQm[cur.s], qs = sQs[cur.s];
+                const float qm = sQm[cur.s], qs = WQ ? sQs[cur.s] * cur.wq : sQs[cur.s];
                 float wt[8];
 #pragma unroll
                 for (int cn = 0; cn < 8; cn++)
@@ -496,12 +498,14 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
                 sto(Bo.cell, off, ncell); sto(Bo.ox, off, ox); sto(Bo.oy, off, oy); sto(Bo.oz, off, oz);
                 sto(Bo.u, off, un); sto(Bo.v, off, vn); sto(Bo.w, off, wn);
                 if (TAG) sto(Bo.id, off, cur.id);
+                if (WQ) sto(Bo.wq, off, cur.wq);
             } else {
                 const int si = atomicAdd(sp.count, 1);
                 if (si < sp.cap) {
                     spill.cell[si] = ncell; spill.ox[si] = ox; spill.oy[si] = oy; spill.oz[si] = oz;
                     spill.u[si] = un; spill.v[si] = vn; spill.w[si] = wn; spill.sp[si] = cur.s;
                     if (TAG) spill.id[si] = cur.id;
+                    if (WQ) spill.wq[si] = cur.wq;
                 } else {
                     atomicOr(sp.err, (int)kErrCapacity);
                 }
@@ -533,6 +537,18 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
         else { if (tag) K(false, true, __VA_ARGS__); else K(false, false, __VA_ARGS__); }                            \
     } while (0)
 
+// the split form's variant for the faces (BC), carried tags (TAG) and charge weights (WQ)
+#define PICK_BC_TAG_WQ(K, bc, tag, wq, ...)                                                                         \
+    do {                                                                                                           \
+        if (wq) {                                                                                                  \
+            if (bc) { if (tag) K(true, true, true, __VA_ARGS__); else K(true, false, true, __VA_ARGS__); }           \
+            else { if (tag) K(false, true, true, __VA_ARGS__); else K(false, false, true, __VA_ARGS__); }            \
+        } else {                                                                                                   \
+            if (bc) { if (tag) K(true, true, false, __VA_ARGS__); else K(true, false, false, __VA_ARGS__); }         \
+            else { if (tag) K(false, true, false, __VA_ARGS__); else K(false, false, false, __VA_ARGS__); }          \
+        }                                                                                                          \
+    } while (0)
+
 void launch_push3d_f64(int T, const GridP &g, const PhysP &ph, const TileP &tp, double *const E[3],
                        double *const B[3], double *const J[3], PartF64 A, PartF64 Bo, BinsP bins, PartF64 spill,
                        SpillP sp, hipStream_t s)
@@ -555,17 +571,18 @@ void launch_push3d_split(int T, const GridP &g, const PhysP &ph, const TileP &tp
                          PartSplit spill, SpillP sp, hipStream_t s)
 {
     dim3 grid(tp.ntiles);
-#define K3_SPLIT(BCV, TAGV, TT, BSZ)                                                                               \
-    hipLaunchKernelGGL((k_push3d_split<TT, BSZ, BCV, TAGV>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1], E[2],     \
-                       B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp)
-    const bool bc = ph.pbc_on != 0, tag = A.id != nullptr;
+#define K3_SPLIT(BCV, TAGV, WQV, TT, BSZ)                                                                          \
+    hipLaunchKernelGGL((k_push3d_split<TT, BSZ, BCV, TAGV, WQV>), grid, dim3(BSZ), 0, s, g, ph, tp, E[0], E[1],      \
+                       E[2], B[0], B[1], B[2], J[0], J[1], J[2], A, Bo, bins, spill, sp)
+    const bool bc = ph.pbc_on != 0, tag = A.id != nullptr, wq = A.wq != nullptr;
     switch (T) {
-    case 4: PICK_BC_TAG(K3_SPLIT, bc, tag, 4, 256); break;
-    case 8: PICK_BC_TAG(K3_SPLIT, bc, tag, 8, 512); break;
+    case 4: PICK_BC_TAG_WQ(K3_SPLIT, bc, tag, wq, 4, 256); break;
+    case 8: PICK_BC_TAG_WQ(K3_SPLIT, bc, tag, wq, 8, 512); break;
     default: break;
     }
 #undef K3_SPLIT
 }
 #undef PICK_BC_TAG
+#undef PICK_BC_TAG_WQ
 
 }  // namespace pictor

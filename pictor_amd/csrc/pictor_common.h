@@ -123,6 +123,10 @@ struct PartSplit {
     // whose particles were uploaded with tags, the caller's index in the adjoint
     // engine's buffers; null otherwise
     int *id = nullptr;
+    // the particle's charge in units of its flavour's charge (the reference's weight,
+    // q = wtp * FlvrCharge, init_prtl.F90:161), in a context whose particles do not all
+    // carry their flavour's charge; null otherwise (every weight 1)
+    float *wq = nullptr;
 };
 
 __host__ __device__ inline long long fidx(const GridP &g, int i, int j, int k)

@@ -134,6 +134,13 @@ class Simulation:
         check(lib().pictor_particle_count(self._ctx, ctypes.byref(n)))
         return n.value
 
+    def filtered_efield(self, passes):
+        """SetFilteredEfldGPU (fields_gpu.F90:70-90): E after `passes` rounds of the 1-2-1
+        filter and the halo copy (the field the reference's test particles feel)."""
+        out = {n: np.zeros(self.shape, order="F") for n in ("Ex", "Ey", "Ez")}
+        check(lib().pictor_filtered_efield(self._ctx, passes, *[_p(out[n]) for n in ("Ex", "Ey", "Ez")]))
+        return out
+
     def download_particles(self, with_tile=False, with_tag=False):
         n = self.particle_count()
         out = {k: np.zeros(n) for k in ("q", "x", "y", "z", "u", "v", "w")}

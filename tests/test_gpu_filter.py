@@ -149,3 +149,27 @@ def test_split_engine_objective_with_filter_equals_production():
         sim.upload_particles(**P0)
         sim.step(nsteps)
         assert L == sim.diagnostics()["B_energy"]
+
+
+@pytest.mark.parametrize("ndim,passes,form", [(2, 1, "fp64"), (2, 3, "fp64"), (3, 2, "fp64"), (2, 2, "fp32_split")])
+def test_filtered_efield_matches_oracle(ndim, passes, form):
+    # SetFilteredEfldGPU (fields_gpu.F90:70-90): E through `passes` rounds of the same
+    # 1-2-1 filter, then the halo copy; the oracle filters the same values as its J
+    n3 = (32, 24, 1) if ndim == 2 else (12, 12, 12)
+    say(f"filtered E ndim={ndim} passes={passes} form={form}")
+    o = Oracle(ndim, *n3, epc=1, cap=16, curr_filter=passes)
+    rng = np.random.default_rng(passes + 10 * ndim)
+    F = {n: np.asfortranarray(rng.standard_normal(o.shape)) for n in FIELDS}
+    o.set_fields(**F)
+    o.field_halo("E"); o.field_halo("B")
+    with Simulation(*n3, ndim=ndim, epc=1, form=form, tile=8 if ndim == 2 else 4) as sim:
+        sim.upload_fields(*[o.F[n] for n in FIELDS])
+        G = sim.filtered_efield(passes)
+        E0 = sim.download_fields()
+    for a, b in (("Ex", "Jx"), ("Ey", "Jy"), ("Ez", "Jz")):
+        o.F[b][...] = E0[a]                 # the context's E as the GPU holds it (FP32 pair in split)
+    o.smoothen_current()
+    o.field_halo("J")
+    for a, b in (("Ex", "Jx"), ("Ey", "Jy"), ("Ez", "Jz")):
+        assert rel_err(_phys(G[a], ndim), _phys(o.F[b], ndim)) <= 1e-15, a
+        assert not np.array_equal(_phys(G[a], ndim), _phys(E0[a], ndim))   # the filter did act
