@@ -347,6 +347,36 @@ def adjoint_report(args):
             "gradient_over_production_forward": (st["forward_ms"] + st["recompute_ms"] + st["reverse_ms"]) / t_prod,
         }
         print(f"# adjoint {form}: {out['forms'][form]}", file=sys.stderr, flush=True)
+    # the FP64 adjoint from momenta perturbed at FP32's relative resolution: how far the
+    # gradient itself moves when the trajectory changes by that much (the floor for the
+    # split form, whose forward steps round to FP32)
+    if "fp64" in grads and args.adjoint_ensemble > 0:
+        out["fp64_ensemble"] = []
+        g64 = grads["fp64"]
+        for k in range(1, args.adjoint_ensemble + 1):
+            gen = torch.Generator().manual_seed(k)
+            d = torch.tensor(drift, dtype=torch.float64, device=dev, requires_grad=True)
+            tl = [(t * (1.0 + 2.0 ** -24 * torch.randn(t.shape, generator=gen, dtype=torch.float64))).to(dev)
+                  .requires_grad_(True) for t in th]
+            u, v, w = drift_boost(tl[0], tl[1], tl[2], sgn_d, d)
+            for t in (u, v, w):
+                t.retain_grad()
+            with Simulation(nx, ny, epc=epc, form="fp64") as sim:
+                L = ParticleStepObjective.apply(u, v, w, sim, P0d, nsteps, seg)
+                L.backward()
+            gm = [t.grad.cpu().numpy() for t in (u, v, w)]
+            num = np.sqrt(sum(np.sum((gm[c] - g64[1][c]) ** 2) for c in range(3)))
+            den = np.sqrt(sum(np.sum(g64[1][c] ** 2) for c in range(3)))
+            rec = {"member": k, "objective": float(L), "d_drift": float(d.grad),
+                   "rel_diff_drift": abs(float(d.grad) - g64[0]) / abs(g64[0]), "rel_l2_momenta": float(num / den)}
+            out["fp64_ensemble"].append(rec)
+            print(f"# fp64 ensemble member {k}: {rec}", file=sys.stderr, flush=True)
+    if "fp64" in grads and "fp32_split" in grads:
+        a64, a32 = grads["fp64"], grads["fp32_split"]
+        num = np.sqrt(sum(np.sum((a32[1][c] - a64[1][c]) ** 2) for c in range(3)))
+        den = np.sqrt(sum(np.sum(a64[1][c] ** 2) for c in range(3)))
+        out["rel_err_split_vs_fp64_drift"] = abs(a32[0] - a64[0]) / abs(a64[0])
+        out["rel_l2_split_vs_fp64_momenta"] = float(num / den)
     if "fp64" not in grads or "fp32_split" not in grads or not args.adjoint_fd:
         return out
     # central finite differences of the FP64 objective in the drift speed; the zigzag
@@ -749,6 +779,8 @@ def main():
     ap.add_argument("--adjoint-engine", default="sorted", choices=("sorted", "unsorted"))
     ap.add_argument("--adjoint-forms", default="fp32_split,fp64")
     ap.add_argument("--adjoint-fd", type=int, default=1, help="finite-difference check of the drift gradient")
+    ap.add_argument("--adjoint-ensemble", type=int, default=0,
+                    help="--adjoint: FP64 gradients from momenta perturbed by 2^-24 relative noise")
     ap.add_argument("--cyl", default=None, choices=sorted(CYL_CONFIGS),
                     help="instead of the Weibel step: time the cylindrical geometry on this grid")
     ap.add_argument("--lwfa", default=None, choices=sorted(LWFA_CONFIGS),
