@@ -283,7 +283,10 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
     constexpr int WN = T + 1, WN3 = WN * WN * WN;
     constexpr int WJ = T + 3, WJ3 = WJ * WJ * WJ;
     constexpr int MB = 54;   // species (<= 2) x 27 directions
-    __shared__ float4 sN[2 * WN3];   // node e: [2e] = (Ex,Ey,Ez,0), [2e+1] = (Bx,By,Bz,0)
+    // node e: sNa[e] = (Ex, Ey, Ez, Bx), sNb[e] = (By, Bz); 24 B per node keeps the LDS of a
+    // tile-8 workgroup under a third of the CU's
+    __shared__ float4 sNa[WN3];
+    __shared__ float2 sNb[WN3];
     __shared__ unsigned long long sJ[3][WJ3];
     __shared__ int sStart[MB + 1], sVs[MB];
     __shared__ unsigned sSrc[MB], sDstBase[MB];
@@ -332,10 +335,10 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
         const int ii = e % WN, jj = (e / WN) % WN, kk = e / (WN * WN);
         const long long o = fidx(g, i0 + ii, j0 + jj, k0 + kk);
         const long long oi = o - 1, oj = o - mx, ok = o - mxy;
-        sN[2 * e] = make_float4((Ex[oi] + Ex[o]) * 0.5f, (Ey[oj] + Ey[o]) * 0.5f, (Ez[ok] + Ez[o]) * 0.5f, 0.f);
-        sN[2 * e + 1] = make_float4((Bx[oj - mxy] + Bx[ok] + Bx[oj] + Bx[o]) * 0.25f + bx0,
-                                    (By[oi - mxy] + By[oi] + By[ok] + By[o]) * 0.25f + by0,
-                                    (Bz[oi - mx] + Bz[oi] + Bz[oj] + Bz[o]) * 0.25f + bz0, 0.f);
+        sNa[e] = make_float4((Ex[oi] + Ex[o]) * 0.5f, (Ey[oj] + Ey[o]) * 0.5f, (Ez[ok] + Ez[o]) * 0.5f,
+                             (Bx[oj - mxy] + Bx[ok] + Bx[oj] + Bx[o]) * 0.25f + bx0);
+        sNb[e] = make_float2((By[oi - mxy] + By[oi] + By[ok] + By[o]) * 0.25f + by0,
+                             (Bz[oi - mx] + Bz[oi] + Bz[oj] + Bz[o]) * 0.25f + bz0);
     }
     for (int e = tid; e < WJ3; e += BS) { sJ[0][e] = 0ull; sJ[1][e] = 0ull; sJ[2][e] = 0ull; }
     __syncthreads();
@@ -384,9 +387,10 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
 #pragma unroll
                 for (int cn = 0; cn < 8; cn++) {
                     const int e = e0 + (cn & 1) + ((cn >> 1) & 1) * WN + (cn >> 2) * WN * WN;
-                    const float4 a = sN[2 * e], b = sN[2 * e + 1];
+                    const float4 a = sNa[e];
+                    const float2 b = sNb[e];
                     pE[0] += wt[cn] * a.x; pE[1] += wt[cn] * a.y; pE[2] += wt[cn] * a.z;
-                    hB[0] += wt[cn] * b.x; hB[1] += wt[cn] * b.y; hB[2] += wt[cn] * b.z;
+                    hB[0] += wt[cn] * a.w; hB[1] += wt[cn] * b.x; hB[2] += wt[cn] * b.y;
                 }
                 const float pEx = pE[0] * qm, pEy = pE[1] * qm, pEz = pE[2] * qm;
                 float u0 = c * un + pEx;
