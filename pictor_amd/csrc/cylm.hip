@@ -597,8 +597,8 @@ __device__ inline void modal_deposit_same_cell(const DepMove &d, Count count, Ad
 
 // MoveDepositPrtl of the split form: the push in FP32 on (cell, offset) positions, theta
 // in 2^-32 turns; the deposit in FP64 from the stored old and new positions (exact in
-// FP64), each contribution rounded to a multiple of 2^-36 and summed as an FP64 integer
-// count: every sum is exact, so J does not depend on the order of the atomics.
+// FP64), each contribution rounded to a multiple of 2^-36 and summed as a 64-bit integer
+// count in the LDS window: every sum is exact, so J does not depend on the order of the atomics.
 // Flags: 1 = a particle reached the wall, 2 = a contribution above 64 charge cells (the
 // fixed-point range).
 __global__ __launch_bounds__(256) void k_cylm_push_split(GridP g, CylmP q, const float4 *__restrict__ nf4,
@@ -606,7 +606,7 @@ __global__ __launch_bounds__(256) void k_cylm_push_split(GridP g, CylmP q, const
                                                          double *__restrict__ Jy, double *__restrict__ Jz, PartCylS P,
                                                          long long n, int *__restrict__ err)
 {
-    __shared__ double sJ[9 * kDepCells];
+    __shared__ unsigned long long sJ[9 * kDepCells];
     __shared__ int sred[4][4], sbox[4];
     const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t cw = m < n ? P.cell[m] : 0u;
@@ -697,25 +697,27 @@ __global__ __launch_bounds__(256) void k_cylm_push_split(GridP g, CylmP q, const
     if (!any) return;
     double *J[3] = {Jx, Jy, Jz};
     bool big = false;
-    auto count = [&](double v) {
+    // the count rint(v 2^36) as a 64-bit integer: |v| <= 64 keeps v 2^36 below 2^51, where
+    // adding 1.5 2^52 rounds it to an integer in the low mantissa bits (ties to even, as rint)
+    auto count = [&](double v) -> long long {
         big = big || !(fabs(v) <= 64.0);
-        return rint(v * kJq);
+        return __double_as_longlong(v * kJq + 6755399441055744.0) - 0x4338000000000000LL;
     };
     const bool inside = dep && min(d.i1, d.i2) >= bil && max(d.i1, d.i2) + 1 < bil + wi && min(d.k1, d.k2) >= bkl &&
                         max(d.k1, d.k2) + 1 < bkl + wk;
     if (dep && !inside)
         modal_deposit(d, [&](int cc, int s, int i, int k, double v) {
-            unsafeAtomicAdd(&J[cc][mi(g, i, s, k)], count(v) * kJqInv);
+            unsafeAtomicAdd(&J[cc][mi(g, i, s, k)], (double)count(v) * kJqInv);
         });
     const int cells = wi * wk;
     int R = kDepCopies;
     while (R > 1 && cells * R > kDepCells) R >>= 1;
     const int rl = lane & (R - 1);
-    for (int e = threadIdx.x; e < 9 * cells * R; e += 256) sJ[e] = 0.0;
+    for (int e = threadIdx.x; e < 9 * cells * R; e += 256) sJ[e] = 0ull;
     __syncthreads();
     if (inside) {
-        auto add = [&](int cc, int s, int i, int k, double cnt) {
-            atomicAdd(&sJ[((cc * 3 + s) * cells + (k - bkl) * wi + (i - bil)) * R + rl], cnt);
+        auto add = [&](int cc, int s, int i, int k, long long cnt) {
+            atomicAdd(&sJ[((cc * 3 + s) * cells + (k - bkl) * wi + (i - bil)) * R + rl], (unsigned long long)cnt);
         };
         if (d.i1 == d.i2 && d.k1 == d.k2)
             modal_deposit_same_cell(d, count, add);
@@ -725,12 +727,12 @@ __global__ __launch_bounds__(256) void k_cylm_push_split(GridP g, CylmP q, const
     if (big) atomicOr(err, 2);
     __syncthreads();
     for (int e = threadIdx.x; e < 9 * cells; e += 256) {
-        double v = 0.0;
-        for (int r = 0; r < R; r++) v += sJ[e * R + r];
-        if (v == 0.0) continue;
+        long long v = 0;
+        for (int r = 0; r < R; r++) v += (long long)sJ[e * R + r];
+        if (v == 0) continue;
         const int cs = e / cells, cell = e - cs * cells;
         const int i = bil + cell % wi, k = bkl + cell / wi;
-        unsafeAtomicAdd(&J[cs / 3][mi(g, i, cs % 3, k)], v * kJqInv);
+        unsafeAtomicAdd(&J[cs / 3][mi(g, i, cs % 3, k)], (double)v * kJqInv);
     }
 }
 
