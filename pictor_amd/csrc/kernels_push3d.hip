@@ -80,8 +80,7 @@ __global__ __launch_bounds__(BS) void k_push3d_f64(GridP g, PhysP ph, TileP tp, 
         double x0 = 0, y0 = 0, z0 = 0, un = 0, vn = 0, wn = 0, q = 0, xn = 0, yn = 0, zn = 0;
         int flv = 0, d = -1, tg = 0;
         if (active) {
-            int b = 0;
-            while (k >= sStart[b + 1]) b++;
+            const int b = bin_of<27>(k, sStart, 27);
             const long long src = sSrc[b] + (k - sStart[b]);
             x0 = A.x[src]; y0 = A.y[src]; z0 = A.z[src];
             un = A.u[src]; vn = A.v[src]; wn = A.w[src];
@@ -348,8 +347,7 @@ __global__ __launch_bounds__(BS) void k_push3d_split(GridP g, PhysP ph, TileP tp
     const int total = sStart[NV];
 
     auto load = [&](int k, SplitRec3 &r) {
-        int v = 0;
-        while (k >= sStart[v + 1]) v++;
+        const int v = bin_of<MB>(k, sStart, NV);
         const unsigned off = (sSrc[v] + (unsigned)(k - sStart[v])) * 4u;
         r.cell = ldo(A.cell, off);
         r.ox = ldo(A.ox, off); r.oy = ldo(A.oy, off); r.oz = ldo(A.oz, off);

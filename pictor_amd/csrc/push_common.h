@@ -14,6 +14,25 @@ __device__ inline unsigned long long lanemask_lt()
     return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
 }
 
+// Bin of position k (0 <= k < sStart[n]) among n bins laid end to end: the last b < n with
+// sStart[b] <= k (an empty bin shares its start with the next).  A search of fixed depth (5 or 6
+// LDS reads) for the 27-direction bins of the 3-D layout, where a linear walk from the first bin
+// takes up to 53 steps; the 2-D kernels, whose centre bins come first, keep the walk.
+template <int NMAX>
+__device__ inline int bin_of(int k, const int *sStart, int n)
+{
+    constexpr int top = NMAX > 32 ? 32 : NMAX > 16 ? 16 : NMAX > 8 ? 8 : NMAX > 4 ? 4 : NMAX > 2 ? 2 : 1;
+    static_assert(NMAX <= 64, "bin search depth");
+    int b = 0;
+#pragma unroll
+    for (int step = top; step >= 1; step >>= 1) {
+        const int c = b + step;
+        const int sv = sStart[c < n ? c : n - 1];
+        b = ((c < n) & (sv <= k)) ? c : b;
+    }
+    return b;
+}
+
 // Per-wave reservation of consecutive slots in the LDS counter of each
 // destination; every lane of the wave must call it (d = -1 for idle lanes).
 __device__ inline int reserve_slot(int d, int *sCnt)
