@@ -156,37 +156,66 @@ __device__ __forceinline__ double dth(const GridP &g, const double *F, long long
     return s == 0 ? 0.0 : (s == 1 ? F[o + g.mx] : -F[o - g.mx]);
 }
 
-__global__ void k_cylm_update_b_lehe(GridP g, CylmP q, double *__restrict__ Bx, double *__restrict__ By,
-                                     double *__restrict__ Bz, const double *__restrict__ Ex,
-                                     const double *__restrict__ Ey, const double *__restrict__ Ez)
+// The Lehe B half step on i = 4..mx-1, k = 2..mz-2 (gradZ reads k - 1 and k + 2; the z halo rewrites
+// k = 1 and mz - 1):
+//   Bx -= h gradT(Ez) / r - h/dz gradZ(Ey),  By -= h/dz gradZ(Ex) - h/dr gradX(Ez),
+//   Bz -= h (1/dr gradX1(Ey) - gradT(Ex)) / (r + dr/2),
+// with the differences above in the reference's term order.  Each thread walks KC consecutive k of
+// one (i, s): the E rows k - 1 .. k + 2 it needs stay in registers and each further k reads one new
+// row (one thread per (i, s, k) read every row three or four times through the caches).
+template <int KC>
+__global__ __launch_bounds__(256) void k_cylm_update_b_lehe_col(GridP g, CylmP q, double *__restrict__ Bx,
+                                                                double *__restrict__ By, double *__restrict__ Bz,
+                                                                const double *__restrict__ Ex,
+                                                                const double *__restrict__ Ey,
+                                                                const double *__restrict__ Ez)
 {
-    // i = 4..mx-1, k = 2..mz-2: gradZ reads k - 1 and k + 2; the z halo rewrites k = 1, mz - 1
-    const int ni = g.mx - 4, nk = g.mz - 3;
+    const int ni = g.mx - 4, nk = g.mz - 3, nkc = (nk + KC - 1) / KC;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)ni * 3 * nk) return;
+    if (t >= (long long)ni * 3 * nkc) return;
     const int i = 4 + (int)(t % ni);
     t /= ni;
     const int s = (int)(t % 3);
-    const int k = 2 + (int)(t / 3);
-    const long long o = mi(g, i, s, k), sk = 3LL * g.mx;
+    const int kb = 2 + KC * (int)(t / 3), ke = min(kb + KC, 2 + nk);   // k = kb .. ke - 1, ke <= mz - 1
+    const long long sk = 3LL * g.mx;
     const double h = q.fld_halfc, idr = q.inv_dr, idz = q.inv_dz;
     const double r = rad(q, i), rph = r + 0.5 * q.dr, rp = r + q.dr;
-    auto gr = [&](const double *F) {
-        return q.l_rr * (F[o + 1] - F[o]) + q.l_rz * (F[o + 1 + sk] - F[o + sk]) + q.l_rz * (F[o + 1 - sk] - F[o - sk]);
+    // row m of (i, s): Ex, Ey at i (rows k - 1 .. k + 2), Ey, Ez at i + 1 and Ez at i, and the
+    // theta differences of Ex, Ez (rows k - 1 .. k + 1)
+    double xa[4], ya[4], yb[3], za[3], zb[3], xt[3], zt[3];
+    auto row3 = [&](int m, int j) {
+        const long long o = mi(g, i, s, m);
+        yb[j] = Ey[o + 1]; za[j] = Ez[o]; zb[j] = Ez[o + 1];
+        xt[j] = dth(g, Ex, o, s); zt[j] = dth(g, Ez, o, s);
     };
-    auto gr1 = [&](const double *F) {
-        return q.l_rr * (rp * F[o + 1] - r * F[o]) + q.l_rz * (rp * F[o + 1 + sk] - r * F[o + sk])
-               + q.l_rz * (rp * F[o + 1 - sk] - r * F[o - sk]);
+    auto row4 = [&](int m, int j) {
+        const long long o = mi(g, i, s, m);
+        xa[j] = Ex[o]; ya[j] = Ey[o];
     };
-    auto gt = [&](const double *F) {
-        return q.l_tt * dth(g, F, o, s) + q.l_tz * dth(g, F, o + sk, s) + q.l_tz * dth(g, F, o - sk, s);
-    };
-    auto gz = [&](const double *F) {
-        return q.l_zz * (F[o + sk] - F[o]) + q.l_zd * (F[o + 2 * sk] - F[o - sk]);
-    };
-    Bx[o] = Bx[o] - h * gt(Ez) / r + h * idz * gz(Ey);
-    By[o] = By[o] - h * idz * gz(Ex) + h * idr * gr(Ez);
-    Bz[o] = Bz[o] - h * (idr * gr1(Ey) - gt(Ex)) / rph;
+    for (int j = 0; j < 3; j++) row3(kb - 1 + j, j);
+    for (int j = 0; j < 4; j++) row4(kb - 1 + j, j);
+    for (int k = kb; k < ke; k++) {
+        const long long o = mi(g, i, s, k);
+        // index 0: k - 1, 1: k, 2: k + 1, 3: k + 2
+        const double gtz = q.l_tt * zt[1] + q.l_tz * zt[2] + q.l_tz * zt[0];
+        const double gzy = q.l_zz * (ya[2] - ya[1]) + q.l_zd * (ya[3] - ya[0]);
+        const double gzx = q.l_zz * (xa[2] - xa[1]) + q.l_zd * (xa[3] - xa[0]);
+        const double grz = q.l_rr * (zb[1] - za[1]) + q.l_rz * (zb[2] - za[2]) + q.l_rz * (zb[0] - za[0]);
+        const double gr1y = q.l_rr * (rp * yb[1] - r * ya[1]) + q.l_rz * (rp * yb[2] - r * ya[2])
+                            + q.l_rz * (rp * yb[0] - r * ya[0]);
+        const double gtx = q.l_tt * xt[1] + q.l_tz * xt[2] + q.l_tz * xt[0];
+        Bx[o] = Bx[o] - h * gtz / r + h * idz * gzy;
+        By[o] = By[o] - h * idz * gzx + h * idr * grz;
+        Bz[o] = Bz[o] - h * (idr * gr1y - gtx) / rph;
+        if (k + 1 < ke) {
+            for (int j = 0; j < 2; j++) {
+                yb[j] = yb[j + 1]; za[j] = za[j + 1]; zb[j] = zb[j + 1]; xt[j] = xt[j + 1]; zt[j] = zt[j + 1];
+            }
+            for (int j = 0; j < 3; j++) { xa[j] = xa[j + 1]; ya[j] = ya[j + 1]; }
+            row3(k + 2, 2);
+            row4(k + 3, 3);
+        }
+    }
 }
 
 // Bz on the axis (cyl_em_update.F90:231-250, the m = 0 circulation of E_theta at
@@ -1340,9 +1369,10 @@ int cylm_b_half(pictor_ctx *c)
     double *E[3], *B[3], *J[3];
     fields(c, E, B, J);
     if (c->cq.lehe) {
-        const long long nb = (long long)(g.mx - 4) * 3 * (g.mz - 3);
-        hipLaunchKernelGGL(k_cylm_update_b_lehe, blocks_of(nb), dim3(256), 0, c->stream, g, c->cq, B[0], B[1], B[2],
-                           E[0], E[1], E[2]);
+        constexpr int KC = 8;
+        const long long nb = (long long)(g.mx - 4) * 3 * ((g.mz - 3 + KC - 1) / KC);
+        hipLaunchKernelGGL((k_cylm_update_b_lehe_col<KC>), blocks_of(nb), dim3(256), 0, c->stream, g, c->cq, B[0],
+                           B[1], B[2], E[0], E[1], E[2]);
     } else {
         const long long nb = (long long)(g.mx - 4) * 3 * (g.mz - 1);
         hipLaunchKernelGGL(k_cylm_update_b, blocks_of(nb), dim3(256), 0, c->stream, g, c->cq, B[0], B[1], B[2], E[0],
